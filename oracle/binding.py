@@ -1,0 +1,180 @@
+"""ctypes binding of the C oracle (oracle/_build/liborion_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, never by the orion_amd package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "liborion_oracle.so")
+REF_HARNESS = os.path.join(HERE, "_ref", "ref_harness")
+
+FIELDS = {"prim": (0, 6), "metrics": (1, 9), "vol": (2, 1), "q": (3, 5), "dq": (4, 5), "dt": (5, 1),
+          "c": (6, 1), "mu": (7, 1), "spec": (8, 3), "spec_visc": (9, 3)}
+
+
+class _Inputs(C.Structure):
+    _fields_ = [("moo", C.c_double), ("reynolds", C.c_double), ("tref", C.c_double),
+                ("twall", C.c_double), ("cfl", C.c_double), ("attack", C.c_double),
+                ("sideslip", C.c_double), ("csrv", C.c_int), ("nvis", C.c_int), ("method", C.c_int),
+                ("ntmst", C.c_int), ("gcl", C.c_int)]
+
+
+class _Params(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("gama", "prl", "moo", "reynolds", "visc", "tref", "twall",
+                                           "cfl", "roo", "uoo", "voo", "woo", "poo", "too")] + \
+               [(n, C.c_int) for n in ("csrv", "nvis", "method", "ntmst", "gcl")] + \
+               [("phydtime", C.c_double)]
+
+
+class _Residual(C.Structure):
+    _fields_ = [("rms", C.c_double), ("max", C.c_double), ("max_loc", C.c_int * 5)]
+
+
+def build(quiet: bool = True) -> str:
+    subprocess.run(["make", "-C", HERE, "oracle"], check=True,
+                   stdout=subprocess.DEVNULL if quiet else None)
+    return LIB_PATH
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        L.ob_create.restype = C.c_void_p
+        L.ob_create.argtypes = [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double), C.c_int,
+                                C.POINTER(C.c_int), C.POINTER(_Inputs)]
+        L.ob_destroy.argtypes = [C.c_void_p]
+        L.ob_last_error.restype = C.c_char_p
+        L.ob_field.restype = C.POINTER(C.c_double)
+        L.ob_field.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.ob_get_params.restype = C.POINTER(_Params)
+        L.ob_get_params.argtypes = [C.c_void_p]
+        for n in ("ob_update_temperature", "ob_update_flow_states", "ob_calculate_time_step",
+                  "ob_compute_viscous_rhs", "ob_compute_inviscid_rhs", "ob_update_solution"):
+            getattr(L, n).argtypes = [C.c_void_p]
+            getattr(L, n).restype = None
+        for n in ("ob_exchange_bc", "ob_apply_physical_bc", "ob_average_interface_residuals"):
+            getattr(L, n).argtypes = [C.c_void_p]
+            getattr(L, n).restype = C.c_int
+        L.ob_residual_compute.restype = _Residual
+        L.ob_residual_compute.argtypes = [C.c_void_p]
+        L.ob_step.argtypes = [C.c_void_p, C.POINTER(_Residual)]
+        L.ob_run.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double)]
+        L.ob_set_threads.argtypes = [C.c_int]
+        L.ob_lusgs_sweeps.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double)]
+        _lib = L
+    return _lib
+
+
+class OracleCase:
+    """One case held by the C oracle; fields are exposed as numpy views in
+    OrionArray layout: (nvar, nz, ny, nx), i fastest."""
+
+    def __init__(self, case, threads: int = 1, apply_init: bool = True):
+        L = lib()
+        L.ob_set_threads(int(threads))
+        self.case = case
+        dims = np.ascontiguousarray(case.dims, dtype=np.int32).reshape(-1)
+        xyz = np.ascontiguousarray(case.xyz_flat())
+        regs = np.ascontiguousarray(case.region_table(), dtype=np.int32)
+        inp = _Inputs(**{k: case.inputs[k] for k in (n for n, _ in _Inputs._fields_)})
+        self._h = L.ob_create(case.nblocks, dims.ctypes.data_as(C.POINTER(C.c_int)),
+                              xyz.ctypes.data_as(C.POINTER(C.c_double)), regs.shape[0],
+                              regs.ctypes.data_as(C.POINTER(C.c_int)), C.byref(inp))
+        if not self._h:
+            raise RuntimeError("oracle: " + L.ob_last_error().decode())
+        if apply_init and case.init_rho is not None:
+            for b in range(case.nblocks):
+                self.field(b, "prim")[...] = case.initial_prim(b)
+            L.ob_update_temperature(self._h)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().ob_destroy(self._h)
+            self._h = None
+
+    def field(self, b: int, name: str) -> np.ndarray:
+        fid, nv = FIELDS[name]
+        nx, ny, nz = self.case.padded(b)
+        ptr = lib().ob_field(self._h, b, fid)
+        arr = np.ctypeslib.as_array(ptr, shape=(nv * nz * ny * nx,))
+        return arr.reshape(nv, nz, ny, nx)
+
+    def params(self):
+        p = lib().ob_get_params(self._h).contents
+        return {n: getattr(p, n) for n, _ in _Params._fields_}
+
+    def op(self, name: str):
+        fn = getattr(lib(), "ob_" + name)
+        rc = fn(self._h)
+        if isinstance(rc, int) and rc != 0:
+            raise RuntimeError("oracle %s: %s" % (name, lib().ob_last_error().decode()))
+
+    def residual(self):
+        r = lib().ob_residual_compute(self._h)
+        return r.rms, r.max, tuple(r.max_loc)
+
+    def step(self):
+        r = _Residual()
+        if lib().ob_step(self._h, C.byref(r)) != 0:
+            raise RuntimeError("oracle step: " + lib().ob_last_error().decode())
+        return r.rms, r.max, tuple(r.max_loc)
+
+    def run(self, n: int) -> np.ndarray:
+        hist = np.zeros(2 * n)
+        if lib().ob_run(self._h, n, hist.ctypes.data_as(C.POINTER(C.c_double))) != 0:
+            raise RuntimeError("oracle run: " + lib().ob_last_error().decode())
+        return hist.reshape(n, 2)
+
+    def lusgs_sweeps(self, b: int, rhs: np.ndarray) -> None:
+        rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+        lib().ob_lusgs_sweeps(self._h, b, rhs.ctypes.data_as(C.POINTER(C.c_double)))
+
+
+# --------------------------------------------------------------------------
+# reference harness (oracle/_ref/ref_harness, this container only)
+# --------------------------------------------------------------------------
+
+def have_reference() -> bool:
+    return os.path.exists(REF_HARNESS)
+
+
+def run_reference(case, casedir: str, outdir: str, nsteps: int, dump_steps: int = 0):
+    """Write ``case`` in the reference's formats and run the reference
+    solver's operators for ``nsteps``.  Returns (hist[n,8], reader) where
+    reader(stage_file) -> list of per-block dicts of fields."""
+    case.write(casedir)
+    os.makedirs(outdir, exist_ok=True)
+    env = dict(os.environ)
+    env.pop("LD_LIBRARY_PATH", None)
+    subprocess.run([REF_HARNESS, casedir, str(nsteps), outdir, str(dump_steps)], check=True,
+                   stdout=subprocess.DEVNULL, env=env)
+    hist = np.fromfile(os.path.join(outdir, "hist.bin")).reshape(nsteps, 8)
+
+    def read(name: str):
+        raw = np.fromfile(os.path.join(outdir, name))
+        out, pos = [], 0
+        for b in range(case.nblocks):
+            nx, ny, nz = case.padded(b)
+            P = nx * ny * nz
+            blk = {}
+            for key, nv in (("prim", 6), ("dq", 5), ("c", 1), ("mu", 1), ("dt", 1)):
+                blk[key] = raw[pos:pos + nv * P].reshape(nv, nz, ny, nx)
+                pos += nv * P
+            out.append(blk)
+        assert pos == raw.size
+        return out
+
+    return hist, read
