@@ -1,0 +1,141 @@
+/*
+ * orion_hip.h -- C ABI of the MI355X (gfx950) implementation of ORION's
+ * per-pseudo-time-step hot path.
+ *
+ * The entry points replace the reference's operator surface one for one;
+ * each comment names the reference interface it stands in for.  All state
+ * lives on the device in one HBM arena per context, laid out exactly like
+ * the reference's OrionArray fields (ghost-padded box (idim+6, jdim+6,
+ * kdim+6), i fastest, variable slowest; src/core/OrionArray.hpp:241-261), so
+ * host buffers in the reference's layout copy in and out unchanged.
+ *
+ * Conventions: every function returns 0 on success and a negative value on
+ * failure, with the message available from orion_hip_last_error() (the
+ * reference throws std::runtime_error, src/core/OrionArray.hpp:25-27).
+ * Operators are asynchronous on the context's stream and run in the order
+ * they are called; only functions that return host values synchronise.
+ * One context must not be used from two threads at once.
+ */
+#ifndef ORION_HIP_H
+#define ORION_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orion_hip_ctx orion_hip_ctx;
+
+/* One BC region in file order (bc::BCRegion, src/bc/BCData.hpp:8-50):
+ * 1-based windows as read by BCReader (src/bc/BCReader.cpp:107-175); block
+ * is the 0-based owner; nbt/ibcwin are the 1-based target block and window
+ * of an interface (bctype < 0). */
+typedef struct {
+  int block;
+  int bctype;
+  int s_st[3], s_ed[3];
+  int t_st[3], t_ed[3];
+  int nbt;
+  int ibcwin;
+} orion_region_desc;
+
+/* The namelist inputs the hot path depends on (core::Params,
+ * src/core/Params.hpp:11-106).  The freestream constants are derived as in
+ * preprocess::init_inflow (src/preprocess/Inflow.cpp:300-459) for the
+ * custom-atmosphere path with tref given. */
+typedef struct {
+  double moo, reynolds, tref, twall, cfl;
+  double attack, sideslip; /* degrees */
+  int csrv;                /* integer, as the reference parses it */
+  int nvis, method, ntmst, gcl;
+} orion_inputs;
+
+/* solver::ResidualStats (src/solver/ResidualMonitor.hpp:8-12) plus the
+ * step's params.step.phydtime. */
+typedef struct {
+  double rms_residual;
+  double max_residual;
+  int max_loc[5]; /* 0-based global block, 1-based i, j, k, var */
+  double phydtime;
+} orion_residual_stats;
+
+/* Field ids (preprocess::BlockField, src/preprocess/FlowField.hpp:19-71). */
+enum {
+  ORION_FIELD_PRIM = 0,    /* 6 vars: rho,u,v,w,p,T */
+  ORION_FIELD_METRICS = 1, /* 9 vars */
+  ORION_FIELD_VOL = 2,
+  ORION_FIELD_Q = 3,       /* 5 vars; only when created with keep_q */
+  ORION_FIELD_DQ = 4,      /* 5 vars */
+  ORION_FIELD_DT = 5,      /* dt/vol */
+  ORION_FIELD_C = 6,
+  ORION_FIELD_MU = 7
+};
+
+/* Backend init hook; replaces backend_cpu_init (src/backend/cpu/BackendCPU.cpp:3).
+ * Selects the device for contexts created afterwards on this thread. */
+int orion_hip_backend_init(int device);
+const char* orion_hip_last_error(void);
+
+/* Setup: replaces the per-case part of OrionSolver::preprocess
+ * (src/core/OrionSolver.cpp:61-93): bc::set_bc_index / prepare_bc_topology
+ * (src/bc/BCPreprocess.cpp:194-270), allocate_other_variable
+ * (src/preprocess/FlowField.cpp:25-107), mesh::compute_grid_metrics and
+ * check_grid_metrics (src/mesh/MetricComputer.cpp:103-500, on the device),
+ * init_inflow, initialize_flow and update_temperature.  dims holds
+ * 3*nblocks interior sizes (each > 6); xyz holds per block x, y then z
+ * (i fastest).  keep_q: also maintain the write-only conservative field. */
+int orion_hip_ctx_create(int nblocks, const int* dims, const double* xyz, int nregions,
+                         const orion_region_desc* regions, const orion_inputs* in, int keep_q,
+                         orion_hip_ctx** out);
+void orion_hip_ctx_destroy(orion_hip_ctx* ctx);
+
+/* Device pointer and element count of one block field (OrionArray::devicePtr,
+ * src/core/OrionArray.hpp:139-141). */
+int orion_hip_field(orion_hip_ctx* ctx, int block, int field, double** dev_ptr, long long* count);
+/* Host <-> device copies of a whole block field (OrionArray::toDevice /
+ * fromDevice, src/core/OrionArray.hpp:146-155); synchronous. */
+int orion_hip_upload(orion_hip_ctx* ctx, int block, int field, const double* host, long long count);
+int orion_hip_download(orion_hip_ctx* ctx, int block, int field, double* host, long long count);
+
+/* preprocess::update_temperature (src/preprocess/InitialCondition.cpp:91-147). */
+int orion_hip_update_temperature(orion_hip_ctx* ctx);
+
+/* solver::HaloExchanger::exchange_bc (src/solver/HaloExchanger.hpp:20). */
+int orion_hip_exchange_bc(orion_hip_ctx* ctx);
+/* bc::apply_physical_bc (src/bc/PhysicalBC.hpp:17). */
+int orion_hip_apply_physical_bc(orion_hip_ctx* ctx);
+/* solver::StateUpdater::update_flow_states (src/solver/StateUpdater.hpp:14). */
+int orion_hip_update_flow_states(orion_hip_ctx* ctx);
+/* solver::TimeIntegrator::calculate_time_step (src/solver/TimeIntegrator.hpp:10);
+ * phydtime may be NULL (then nothing synchronises). */
+int orion_hip_calculate_time_step(orion_hip_ctx* ctx, double* phydtime);
+/* solver::FluxComputer::compute_viscous_rhs (src/solver/FluxComputer.hpp:15). */
+int orion_hip_compute_viscous_rhs(orion_hip_ctx* ctx);
+/* solver::HaloExchanger::average_interface_residuals (src/solver/HaloExchanger.hpp:25). */
+int orion_hip_average_interface_residuals(orion_hip_ctx* ctx);
+/* solver::InviscidFluxComputer::compute_inviscid_rhs (src/solver/InviscidFluxComputer.hpp:17). */
+int orion_hip_compute_inviscid_rhs(orion_hip_ctx* ctx);
+/* solver::StateUpdater::update_solution (src/solver/StateUpdater.hpp:21). */
+int orion_hip_update_solution(orion_hip_ctx* ctx);
+/* solver::ResidualMonitor::compute (src/solver/ResidualMonitor.hpp:20); synchronises. */
+int orion_hip_residual(orion_hip_ctx* ctx, orion_residual_stats* out);
+
+/* nsteps pseudo-time steps in the order of OrionSolver::solve
+ * (src/core/OrionSolver.cpp:139-209, without the debug block at :156-201).
+ * hist (nsteps entries, may be NULL) receives each step's residual; it is
+ * read back once, after the last step. */
+int orion_hip_run(orion_hip_ctx* ctx, int nsteps, orion_residual_stats* hist);
+
+int orion_hip_synchronize(orion_hip_ctx* ctx);
+/* Use a caller-owned hipStream_t (NULL restores the context's own stream). */
+int orion_hip_set_stream(orion_hip_ctx* ctx, void* stream);
+/* Device bytes held by the context (arena plus maps). */
+size_t orion_hip_device_bytes(const orion_hip_ctx* ctx);
+/* Interior node count (the unit of cell-steps). */
+long long orion_hip_interior_nodes(const orion_hip_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ORION_HIP_H */

@@ -1,0 +1,871 @@
+// context.hip -- host side of the C ABI (include/orion_hip.h).
+//
+// A context owns one HBM arena with every block's fields, the multi-block
+// iteration tables, and the boundary topology compiled once into device
+// maps: a halo gather list, per-rank physical-BC task lists, interface-
+// averaging staging maps and the frozen-node masks.  Operators enqueue
+// kernels on the context's stream; nothing synchronises inside a step.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/orion_hip.h"
+#include "orion_dev.h"
+#include "orion_kernels.h"
+
+using namespace orion_hip;
+
+static thread_local std::string g_err;
+static thread_local int g_device = 0;
+
+static int fail(const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return -1;
+}
+
+#define HIPCHK(x)                                                                              \
+  do {                                                                                         \
+    hipError_t e_ = (x);                                                                       \
+    if (e_ != hipSuccess) return fail("%s failed: %s (%s:%d)", #x, hipGetErrorString(e_), __FILE__, __LINE__); \
+  } while (0)
+
+struct Space {
+  std::vector<long long> h;  // prefix offsets (nb+1)
+  long long* d = nullptr;
+  long long total() const { return h.back(); }
+};
+
+struct orion_hip_ctx {
+  int device = 0;
+  hipStream_t own_stream = nullptr;
+  hipStream_t stream = nullptr;
+  int nb = 0;
+  std::vector<BlockDev> hb;
+  BlockDev* db = nullptr;
+  Params p{};
+  char* arena = nullptr;
+  size_t arena_bytes = 0;
+  size_t map_bytes = 0;
+  Space pad, intr, face[3], line[3], plane;
+  int max_planes = 0;
+  long long* d_halo = nullptr;
+  long long n_halo = 0;
+  BCTask* d_bc = nullptr;
+  std::vector<int> bc_first, bc_count;  // per rank: task offset and count
+  std::vector<long long> bc_total;      // per rank: face nodes
+  AvgSrc* d_avg_src = nullptr;
+  AvgDst* d_avg_dst = nullptr;
+  long long n_avg_src = 0, n_avg_dst = 0;
+  double* d_stage = nullptr;
+  double* d_part = nullptr;
+  long long* d_pidx = nullptr;
+  int nparts = 1;
+  double* d_res = nullptr;  // 4 doubles per residual slot
+  int res_cap = 0;
+  unsigned long long* d_dtmin = nullptr;
+  int dtmin_cap = 0;
+  unsigned long long* d_count = nullptr;
+  std::vector<void*> allocs;
+  int keep_q = 0;
+};
+
+static int dalloc(orion_hip_ctx* c, void** p, size_t bytes) {
+  if (bytes == 0) bytes = 8;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) return fail("hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+  c->allocs.push_back(*p);
+  c->map_bytes += bytes;
+  return 0;
+}
+
+template <class T>
+static int upload_vec(orion_hip_ctx* c, T** dst, const std::vector<T>& v) {
+  if (dalloc(c, (void**)dst, sizeof(T) * std::max<size_t>(v.size(), 1))) return -1;
+  if (!v.empty()) HIPCHK(hipMemcpy(*dst, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+  return 0;
+}
+
+static int make_space(orion_hip_ctx* c, Space& s, const std::vector<long long>& counts) {
+  s.h.assign(counts.size() + 1, 0);
+  for (size_t b = 0; b < counts.size(); ++b) s.h[b + 1] = s.h[b] + counts[b];
+  return upload_vec(c, &s.d, s.h);
+}
+
+static Launch LA(orion_hip_ctx* c, const Space& s) { return Launch{c->stream, c->db, s.d, c->nb}; }
+
+// ---------------------------------------------------------------------------
+// region preprocessing (BCReader.cpp:125-143, BCPreprocess.cpp:16-192)
+// ---------------------------------------------------------------------------
+struct Reg {
+  orion_region_desc d;
+  int s_nd = 0, s_lr = 0, t_nd = 0;
+  int map_dims[3] = {0, 0, 0};
+  std::vector<int> image, jmage, kmage;
+};
+
+static int analyze_connect(Reg& r) {
+  const int* s_st = r.d.s_st;
+  const int* s_ed = r.d.s_ed;
+  const int* t_st = r.d.t_st;
+  const int* t_ed = r.d.t_ed;
+  const int s_nd = r.s_nd;
+  r.t_nd = 0;
+  for (int m = 0; m < 3; ++m)
+    if (t_st[m] == t_ed[m]) r.t_nd = m + 1;
+  const int t_nd = r.t_nd;
+  if (t_nd == 0 || s_nd == 0) return fail("interface region without a constant axis");
+  int dirm[3][3] = {{0}};
+  bool done = false;
+  for (int m = 0; m < 3 && !done; ++m)
+    for (int n = 0; n < 3 && !done; ++n)
+      if ((m + 1) != s_nd && (n + 1) != t_nd) {
+        int js1 = s_st[m];
+        if (std::abs(js1) < std::abs(s_ed[m])) js1 = s_ed[m];
+        int js2 = t_st[n];
+        if (std::abs(js2) < std::abs(t_ed[n])) js2 = t_ed[n];
+        if (js1 * js2 > 0) {
+          dirm[n][m] = 1;
+          dirm[t_nd - 1][s_nd - 1] = 1;
+          dirm[6 - (n + 1) - t_nd - 1][6 - (m + 1) - s_nd - 1] = 1;
+          done = true;
+        }
+      }
+  int s_sign[3], t_sign[3], st_sign[3];
+  for (int m = 0; m < 3; ++m) {
+    s_sign[m] = ((m + 1) != s_nd && std::abs(s_st[m]) > std::abs(s_ed[m])) ? -1 : 1;
+    t_sign[m] = ((m + 1) != t_nd && std::abs(t_st[m]) > std::abs(t_ed[m])) ? -1 : 1;
+  }
+  for (int m = 0; m < 3; ++m) {
+    int co = 0;
+    for (int n = 0; n < 3; ++n)
+      if (dirm[n][m] == 1) co = n;
+    st_sign[m] = t_sign[co] * s_sign[m];
+  }
+  int lo[3];
+  for (int m = 0; m < 3; ++m) {
+    const int a = std::abs(s_st[m]), b = std::abs(s_ed[m]);
+    lo[m] = std::min(a, b);
+    r.map_dims[m] = std::max(a, b) - lo[m] + 1;
+  }
+  const long long tot = (long long)r.map_dims[0] * r.map_dims[1] * r.map_dims[2];
+  r.image.assign(tot, 0);
+  r.jmage.assign(tot, 0);
+  r.kmage.assign(tot, 0);
+  for (int k = std::abs(s_st[2]); k != std::abs(s_ed[2]) + s_sign[2]; k += s_sign[2])
+    for (int j = std::abs(s_st[1]); j != std::abs(s_ed[1]) + s_sign[1]; j += s_sign[1])
+      for (int i = std::abs(s_st[0]); i != std::abs(s_ed[0]) + s_sign[0]; i += s_sign[0]) {
+        const int di = (i - std::abs(s_st[0])) * st_sign[0];
+        const int dj = (j - std::abs(s_st[1])) * st_sign[1];
+        const int dk = (k - std::abs(s_st[2])) * st_sign[2];
+        const long long f = ((long long)(k - lo[2]) * r.map_dims[1] + (j - lo[1])) * r.map_dims[0] + (i - lo[0]);
+        r.image[f] = std::abs(t_st[0]) + dirm[0][0] * di + dirm[0][1] * dj + dirm[0][2] * dk;
+        r.jmage[f] = std::abs(t_st[1]) + dirm[1][0] * di + dirm[1][1] * dj + dirm[1][2] * dk;
+        r.kmage[f] = std::abs(t_st[2]) + dirm[2][0] * di + dirm[2][1] * dj + dirm[2][2] * dk;
+      }
+  return 0;
+}
+
+// get_memory_window (HaloExchanger.cpp:17-64): padded inclusive ranges
+static void memory_window(const Reg& r, const BlockDev& b, int mode, int* rb, int* re) {
+  const int dphy[3] = {b.idim, b.jdim, b.kdim};
+  int cc[3];
+  for (int m = 0; m < 3; ++m) cc[m] = std::abs(r.d.s_st[m]) - 1 + NG;
+  const int dir = r.s_nd - 1;
+  if (mode == 0) {
+    if (r.s_lr == -1) { rb[dir] = cc[dir]; re[dir] = cc[dir] + NG - 1; }
+    else { rb[dir] = cc[dir] - NG + 1; re[dir] = cc[dir]; }
+  } else {
+    if (r.s_lr == -1) { rb[dir] = cc[dir] - NG; re[dir] = cc[dir] - 1; }
+    else { rb[dir] = cc[dir] + 1; re[dir] = cc[dir] + NG; }
+  }
+  for (int m = 0; m < 3; ++m) {
+    if (m == dir) continue;
+    const int a = std::abs(r.d.s_st[m]), bb = std::abs(r.d.s_ed[m]);
+    int ps = std::min(a, bb), pe = std::max(a, bb);
+    if (ps > 1) ps -= 1;
+    if (pe < dphy[m]) pe += 1;
+    rb[m] = ps - 1 + NG;
+    re[m] = pe - 1 + NG;
+  }
+}
+
+static void init_params(Params& p, const orion_inputs* in) {
+  const double PAI = 3.14159265358979323846;
+  std::memset(&p, 0, sizeof p);
+  p.gama = 1.40;
+  p.prl = 0.72;
+  const double attack = in->attack * PAI / 180.0;
+  const double sideslip = in->sideslip * PAI / 180.0;
+  p.uoo = std::cos(attack) * std::cos(sideslip);
+  p.voo = std::sin(attack) * std::cos(sideslip);
+  p.woo = std::sin(sideslip);
+  p.roo = 1.0;
+  p.too = 1.0;
+  p.reynolds = (in->reynolds / 1.0) * 1.0;
+  p.moo = in->moo;
+  p.poo = 1.0 / (p.gama * p.moo * p.moo);
+  p.tref = in->tref;
+  p.twall = in->twall;
+  p.visc = (in->tref != 0.0) ? 110.4 / in->tref : 0.0;
+  p.cfl = in->cfl;
+  p.csrv = in->csrv;
+  p.nvis = in->nvis;
+  p.method = in->method;
+  p.ntmst = in->ntmst;
+  p.gcl = in->gcl;
+}
+
+static long long arena_fields(const BlockDev& b, int keep_q, long long lines_total) {
+  const long long P = b.P;
+  long long n = 0;
+  auto al = [](long long x) { return (x + 31) / 32 * 32; };  // 256-byte alignment
+  n += al(6 * P) + al(9 * P) + al(P) + al(5 * P) + al(5 * P) + al(P) + al(P) + al(P);
+  if (keep_q) n += al(5 * P);
+  n += al(12 * P) + al(15 * P) + al(30 * lines_total) + al((P + 7) / 8);
+  return n;
+}
+
+static int build_topology(orion_hip_ctx* c, const std::vector<std::vector<Reg>>& R) {
+  const int nb = c->nb;
+  // ---------------- halo gather list ----------------
+  struct Msg {
+    int tag, block, reg;
+    bool used;
+  };
+  std::vector<Msg> msgs;
+  for (int b = 0; b < nb; ++b)
+    for (int r = 0; r < (int)R[b].size(); ++r)
+      if (R[b][r].d.nbt > 0) msgs.push_back({R[b][r].d.nbt * 1000 + R[b][r].d.ibcwin, b, r, false});
+  std::map<long long, std::pair<long long, long long>> last;  // dst -> (order, src)
+  long long order = 0;
+  auto arena_off = [&](int b, int m, long long id) {
+    return (long long)((c->hb[b].prim - (double*)c->arena) + (long long)m * c->hb[b].P + id);
+  };
+  for (int b = 0; b < nb; ++b)
+    for (int r = 0; r < (int)R[b].size(); ++r) {
+      const Reg& rg = R[b][r];
+      if (rg.d.nbt <= 0) continue;
+      const int tag = (b + 1) * 1000 + (r + 1);
+      Msg* M = nullptr;
+      for (auto& m : msgs)
+        if (!m.used && m.tag == tag) { M = &m; break; }
+      if (!M) return fail("halo: no partner message for block %d window %d", b + 1, r + 1);
+      M->used = true;
+      const BlockDev& A = c->hb[M->block];
+      int sb[3], se[3], rb[3], re[3];
+      memory_window(R[M->block][M->reg], A, 0, sb, se);
+      memory_window(rg, c->hb[b], 1, rb, re);
+      const int sn[3] = {se[0] - sb[0] + 1, se[1] - sb[1] + 1, se[2] - sb[2] + 1};
+      const long long npts = (long long)sn[0] * sn[1] * sn[2];
+      const long long count = 5 * npts;
+      long long ptr = 0;
+      for (int m = 0; m < 5; ++m)
+        for (int k = rb[2]; k <= re[2]; ++k)
+          for (int j = rb[1]; j <= re[1]; ++j)
+            for (int i = rb[0]; i <= re[0]; ++i) {
+              if (ptr >= count) continue;
+              const long long sm = ptr / npts, sl = ptr % npts;
+              const int si = sb[0] + (int)(sl % sn[0]);
+              const int sj = sb[1] + (int)((sl / sn[0]) % sn[1]);
+              const int sk = sb[2] + (int)(sl / ((long long)sn[0] * sn[1]));
+              ++ptr;
+              const long long dst = arena_off(b, m, pidx(c->hb[b], i, j, k));
+              const long long src = arena_off(M->block, (int)sm, pidx(A, si, sj, sk));
+              last[dst] = {order++, src};
+            }
+    }
+  std::vector<long long> pairs;
+  pairs.reserve(2 * last.size());
+  for (auto& kv : last) {
+    pairs.push_back(kv.first);
+    pairs.push_back(kv.second.second);
+  }
+  c->n_halo = (long long)last.size();
+  if (upload_vec(c, &c->d_halo, pairs)) return -1;
+
+  // ---------------- physical BC tasks per rank ----------------
+  int maxrank = 0;
+  std::vector<std::vector<const Reg*>> phys(nb);
+  for (int b = 0; b < nb; ++b) {
+    for (auto& rg : R[b])
+      if (rg.d.bctype > 0) phys[b].push_back(&rg);
+    maxrank = std::max(maxrank, (int)phys[b].size());
+  }
+  std::vector<BCTask> tasks;
+  for (int rk = 0; rk < maxrank; ++rk) {
+    c->bc_first.push_back((int)tasks.size());
+    long long tot = 0;
+    for (int b = 0; b < nb; ++b) {
+      if (rk >= (int)phys[b].size()) continue;
+      const Reg& rg = *phys[b][rk];
+      BCTask t{};
+      t.block = b;
+      t.bctype = rg.d.bctype;
+      for (int m = 0; m < 3; ++m) {
+        t.lo[m] = rg.d.s_st[m] - 1 + NG;
+        t.hi[m] = rg.d.s_ed[m] - 1 + NG;
+        t.mid[m] = (t.lo[m] + t.hi[m]) / 2;
+      }
+      t.s_nd = rg.s_nd;
+      t.s_lr = rg.s_lr;
+      if (t.s_nd == 0) return fail("block %d: physical BC region is not a face", b + 1);
+      long long cnt = 1;
+      for (int m = 0; m < 3; ++m) cnt *= std::max(0, t.hi[m] - t.lo[m] + 1);
+      if (cnt == 0) continue;
+      t.start = tot;
+      tot += cnt;
+      tasks.push_back(t);
+    }
+    c->bc_count.push_back((int)tasks.size() - c->bc_first.back());
+    c->bc_total.push_back(tot);
+  }
+  if (upload_vec(c, &c->d_bc, tasks)) return -1;
+
+  // ---------------- interface averaging maps ----------------
+  struct Ent {
+    int sb;
+    long long sidx;
+  };
+  std::vector<std::vector<std::pair<int, long long>>> filler(nb);  // per (b, r): (A, X)
+  std::map<std::pair<int, int>, std::pair<int, int>> fill_of;
+  for (int a = 0; a < nb; ++a)
+    for (int x = 0; x < (int)R[a].size(); ++x) {
+      const Reg& X = R[a][x];
+      if (X.d.bctype >= 0) continue;
+      fill_of[{X.d.nbt - 1, X.d.ibcwin - 1}] = {a, x};  // last packer wins
+    }
+  std::map<std::pair<int, long long>, std::vector<Ent>> dst_entries;
+  std::vector<std::pair<int, long long>> dst_order;
+  for (int b = 0; b < nb; ++b)
+    for (int w = 0; w < (int)R[b].size(); ++w) {
+      const Reg& W = R[b][w];
+      if (W.d.bctype >= 0) continue;
+      auto it = fill_of.find({b, w});
+      if (it == fill_of.end()) return fail("interface buffer never filled for block %d window %d", b + 1, w + 1);
+      const int a = it->second.first;
+      const Reg& X = R[a][it->second.second];
+      int xlo[3], xn[3], lo[3], n[3], tlo[3], tn[3];
+      for (int m = 0; m < 3; ++m) {
+        int u = std::abs(X.d.s_st[m]), v = std::abs(X.d.s_ed[m]);
+        xlo[m] = std::min(u, v);
+        xn[m] = std::max(u, v) - xlo[m] + 1;
+        u = std::abs(W.d.s_st[m]);
+        v = std::abs(W.d.s_ed[m]);
+        lo[m] = std::min(u, v);
+        n[m] = std::max(u, v) - lo[m] + 1;
+        u = std::abs(W.d.t_st[m]);
+        v = std::abs(W.d.t_ed[m]);
+        tlo[m] = std::min(u, v);
+        tn[m] = std::max(u, v) - tlo[m] + 1;
+      }
+      const long long xcount = (long long)xn[0] * xn[1] * xn[2];
+      for (int k = 0; k < n[2]; ++k)
+        for (int j = 0; j < n[1]; ++j)
+          for (int i = 0; i < n[0]; ++i) {
+            const long long f = (long long)k * (n[0] * n[1]) + (long long)j * n[0] + i;
+            const int ir = W.image[f] - tlo[0], jr = W.jmage[f] - tlo[1], kr = W.kmage[f] - tlo[2];
+            const long long rem = (long long)kr * (tn[0] * tn[1]) + (long long)jr * tn[0] + ir;
+            if (rem < 0 || rem >= xcount)
+              return fail("interface map of block %d window %d leaves its partner window", b + 1, w + 1);
+            const int xi = (int)(rem % xn[0]), xj = (int)((rem / xn[0]) % xn[1]), xk = (int)(rem / ((long long)xn[0] * xn[1]));
+            const long long sidx = pidx(c->hb[a], xlo[0] - 1 + NG + xi, xlo[1] - 1 + NG + xj, xlo[2] - 1 + NG + xk);
+            const long long didx = pidx(c->hb[b], lo[0] - 1 + NG + i, lo[1] - 1 + NG + j, lo[2] - 1 + NG + k);
+            auto key = std::make_pair(b, didx);
+            auto& lst = dst_entries[key];
+            if (lst.empty()) dst_order.push_back(key);
+            lst.push_back({a, sidx});
+          }
+    }
+  std::vector<AvgSrc> src;
+  std::vector<AvgDst> dst;
+  std::sort(dst_order.begin(), dst_order.end());
+  for (auto& key : dst_order) {
+    auto& lst = dst_entries[key];
+    AvgDst D{};
+    D.block = key.first;
+    D.idx = key.second;
+    D.start = (long long)src.size();
+    D.count = (int)lst.size();
+    for (auto& e : lst) src.push_back(AvgSrc{e.sb, e.sidx});
+    dst.push_back(D);
+  }
+  c->n_avg_src = (long long)src.size();
+  c->n_avg_dst = (long long)dst.size();
+  if (upload_vec(c, &c->d_avg_src, src)) return -1;
+  if (upload_vec(c, &c->d_avg_dst, dst)) return -1;
+  if (dalloc(c, (void**)&c->d_stage, sizeof(double) * 5 * std::max<long long>(1, c->n_avg_src))) return -1;
+
+  // ---------------- frozen masks (set_boundary_dq_zero windows) ----------------
+  for (int b = 0; b < nb; ++b) {
+    const BlockDev& B = c->hb[b];
+    std::vector<unsigned char> mask(B.P, 0);
+    for (auto& rg : R[b]) {
+      if (rg.d.bctype <= 0) continue;
+      for (int k = rg.d.s_st[2] - 1 + NG; k <= rg.d.s_ed[2] - 1 + NG; ++k)
+        for (int j = rg.d.s_st[1] - 1 + NG; j <= rg.d.s_ed[1] - 1 + NG; ++j)
+          for (int i = rg.d.s_st[0] - 1 + NG; i <= rg.d.s_ed[0] - 1 + NG; ++i)
+            if (i >= 0 && j >= 0 && k >= 0 && i < B.nx && j < B.ny && k < B.nz) mask[pidx(B, i, j, k)] = 1;
+    }
+    HIPCHK(hipMemcpy(B.frozen, mask.data(), B.P, hipMemcpyHostToDevice));
+  }
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char* orion_hip_last_error(void) { return g_err.c_str(); }
+
+int orion_hip_backend_init(int device) {
+  int n = 0;
+  HIPCHK(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n) return fail("device %d out of range (%d devices)", device, n);
+  HIPCHK(hipSetDevice(device));
+  g_device = device;
+  return 0;
+}
+
+void orion_hip_ctx_destroy(orion_hip_ctx* c) {
+  if (!c) return;
+  hipSetDevice(c->device);
+  if (c->stream) hipStreamSynchronize(c->stream);
+  for (void* p : c->allocs) hipFree(p);
+  if (c->arena) hipFree(c->arena);
+  if (c->own_stream) hipStreamDestroy(c->own_stream);
+  delete c;
+}
+
+int orion_hip_ctx_create(int nblocks, const int* dims, const double* xyz, int nregions,
+                         const orion_region_desc* regions, const orion_inputs* in, int keep_q,
+                         orion_hip_ctx** out) {
+  *out = nullptr;
+  if (nblocks <= 0) return fail("nblocks must be positive");
+  orion_hip_ctx* c = new orion_hip_ctx();
+  auto bail = [&](int rc) {
+    orion_hip_ctx_destroy(c);
+    return rc;
+  };
+  c->device = g_device;
+  if (hipSetDevice(c->device) != hipSuccess) return bail(fail("hipSetDevice(%d) failed", c->device));
+  // default-priority stream
+  if (hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess)
+    return bail(fail("hipStreamCreate failed"));
+  c->stream = c->own_stream;
+  c->nb = nblocks;
+  c->keep_q = keep_q;
+  init_params(c->p, in);
+  // blocks
+  c->hb.resize(nblocks);
+  std::vector<long long> lines_tot(nblocks);
+  for (int b = 0; b < nblocks; ++b) {
+    BlockDev& B = c->hb[b];
+    B = BlockDev{};
+    B.idim = dims[3 * b];
+    B.jdim = dims[3 * b + 1];
+    B.kdim = dims[3 * b + 2];
+    if (B.idim <= 2 * NG || B.jdim <= 2 * NG || B.kdim <= 2 * NG)
+      return bail(fail("block %d too small (%d,%d,%d): every dimension must exceed %d (MetricComputer.cpp:154)", b,
+                       B.idim, B.jdim, B.kdim, 2 * NG));
+    B.nx = B.idim + 2 * NG;
+    B.ny = B.jdim + 2 * NG;
+    B.nz = B.kdim + 2 * NG;
+    B.P = (long long)B.nx * B.ny * B.nz;
+    lines_tot[b] = nlines(B, 0) + nlines(B, 1) + nlines(B, 2);
+  }
+  // arena
+  long long total = 0;
+  for (int b = 0; b < nblocks; ++b) total += arena_fields(c->hb[b], keep_q, lines_tot[b]);
+  c->arena_bytes = (size_t)total * sizeof(double);
+  if (hipMalloc((void**)&c->arena, c->arena_bytes) != hipSuccess)
+    return bail(fail("arena hipMalloc(%zu bytes) failed", c->arena_bytes));
+  if (hipMemsetAsync(c->arena, 0, c->arena_bytes, c->stream) != hipSuccess) return bail(fail("arena memset failed"));
+  {
+    double* a = (double*)c->arena;
+    auto al = [](long long x) { return (x + 31) / 32 * 32; };
+    for (int b = 0; b < nblocks; ++b) {
+      BlockDev& B = c->hb[b];
+      const long long P = B.P;
+      B.prim = a; a += al(6 * P);
+      B.met = a; a += al(9 * P);
+      B.vol = a; a += al(P);
+      B.dq = a; a += al(5 * P);
+      B.rhs = a; a += al(5 * P);
+      B.dt = a; a += al(P);
+      B.c = a; a += al(P);
+      B.mu = a; a += al(P);
+      B.q = nullptr;
+      if (keep_q) { B.q = a; a += al(5 * P); }
+      B.duvwt = a; a += al(12 * P);
+      B.fx = a; a += al(15 * P);
+      B.fcov = a; a += al(30 * lines_tot[b]);
+      B.fcov_off[0] = 0;
+      B.fcov_off[1] = 30 * nlines(B, 0);
+      B.fcov_off[2] = 30 * (nlines(B, 0) + nlines(B, 1));
+      B.frozen = (unsigned char*)a; a += al((P + 7) / 8);
+    }
+  }
+  // regions
+  std::vector<std::vector<Reg>> R(nblocks);
+  for (int r = 0; r < nregions; ++r) {
+    const orion_region_desc& d = regions[r];
+    if (d.block < 0 || d.block >= nblocks) return bail(fail("region %d: bad block %d", r, d.block));
+    Reg rg;
+    rg.d = d;
+    if (d.bctype >= 0) {
+      rg.d.nbt = -1;
+      rg.d.ibcwin = 0;
+    }
+    const int bt = d.bctype;
+    if (!(bt == -1 || bt == 2 || bt == 3 || bt == 4 || bt == 5 || bt == 6))
+      return bail(fail("block %d: bctype %d not among {-1,4,5,6,2,3} (BCPreprocess.cpp:196)", d.block + 1, bt));
+    const BlockDev& B = c->hb[d.block];
+    const int dm[3] = {B.idim, B.jdim, B.kdim};
+    // windows must lie inside their blocks (the reference does not check;
+    // here an out-of-range window would address outside the arena)
+    for (int m = 0; m < 3; ++m) {
+      const int a = std::abs(d.s_st[m]), e = std::abs(d.s_ed[m]);
+      if (a < 1 || e < 1 || a > dm[m] || e > dm[m])
+        return bail(fail("block %d region %d: window outside the block", d.block + 1, r + 1));
+      if (bt < 0) {
+        if (d.nbt < 1 || d.nbt > nblocks) return bail(fail("block %d region %d: bad target block", d.block + 1, r + 1));
+        const BlockDev& T = c->hb[d.nbt - 1];
+        const int tm[3] = {T.idim, T.jdim, T.kdim};
+        const int ta = std::abs(d.t_st[m]), te = std::abs(d.t_ed[m]);
+        if (ta < 1 || te < 1 || ta > tm[m] || te > tm[m])
+          return bail(fail("block %d region %d: target window outside block %d", d.block + 1, r + 1, d.nbt));
+      }
+    }
+    for (int m = 0; m < 3; ++m)
+      if (d.s_st[m] == d.s_ed[m]) {
+        rg.s_nd = m + 1;
+        rg.s_lr = (d.s_st[m] == 1) ? -1 : (d.s_st[m] == dm[m] ? 1 : 0);
+        break;
+      }
+    if (bt < 0) {
+      if (rg.s_nd == 0 || d.ibcwin <= 0 || d.nbt < 1 || d.nbt > nblocks)
+        return bail(fail("block %d: invalid interface region", d.block + 1));
+      if (analyze_connect(rg)) return bail(-1);
+    }
+    R[d.block].push_back(std::move(rg));
+  }
+  for (int b = 0; b < nblocks; ++b)
+    for (auto& rg : R[b])
+      if (rg.d.bctype < 0) {
+        const int tb = rg.d.nbt - 1, tw = rg.d.ibcwin - 1;
+        if (tw >= (int)R[tb].size()) return bail(fail("block %d: target window out of range", b + 1));
+      }
+  // tables
+  std::vector<long long> cp(nblocks), ci(nblocks), cf[3], cl[3], cpl(nblocks);
+  for (int d = 0; d < 3; ++d) {
+    cf[d].resize(nblocks);
+    cl[d].resize(nblocks);
+  }
+  for (int b = 0; b < nblocks; ++b) {
+    const BlockDev& B = c->hb[b];
+    cp[b] = B.P;
+    ci[b] = (long long)B.idim * B.jdim * B.kdim;
+    for (int d = 0; d < 3; ++d) {
+      cl[d][b] = nlines(B, d);
+      cf[d][b] = nlines(B, d) * (dcount(B, d) + 1);
+    }
+    cpl[b] = (long long)B.jdim * B.kdim;
+    c->max_planes = std::max(c->max_planes, B.idim + B.jdim + B.kdim - 2);
+  }
+  if (make_space(c, c->pad, cp) || make_space(c, c->intr, ci) || make_space(c, c->plane, cpl)) return bail(-1);
+  for (int d = 0; d < 3; ++d)
+    if (make_space(c, c->face[d], cf[d]) || make_space(c, c->line[d], cl[d])) return bail(-1);
+  if (upload_vec(c, &c->db, c->hb)) return bail(-1);
+  if (build_topology(c, R)) return bail(-1);
+  c->nparts = residual_parts(c->intr.total());
+  if (dalloc(c, (void**)&c->d_part, sizeof(double) * 2 * 1024) || dalloc(c, (void**)&c->d_pidx, sizeof(long long) * 1024) ||
+      dalloc(c, (void**)&c->d_count, sizeof(unsigned long long) * 2))
+    return bail(-1);
+  // initial state (InitialCondition.cpp:49-80): freestream prim everywhere
+  const Params& p = c->p;
+  const double prim0[6] = {p.roo, p.uoo, p.voo, p.woo, p.poo, p.too};
+  const double v2 = p.uoo * p.uoo + p.voo * p.voo + p.woo * p.woo;
+  const double q0[5] = {p.roo, p.roo * p.uoo, p.roo * p.voo, p.roo * p.woo, p.poo / (p.gama - 1.0) + 0.5 * p.roo * v2};
+  for (int b = 0; b < nblocks; ++b) {
+    const BlockDev& B = c->hb[b];
+    for (int m = 0; m < 6; ++m) launch_fill(B.prim + m * B.P, B.P, prim0[m], c->stream);
+    if (B.q)
+      for (int m = 0; m < 5; ++m) launch_fill(B.q + m * B.P, B.P, q0[m], c->stream);
+  }
+  // grid -> metrics on the device
+  {
+    long long off = 0;
+    for (int b = 0; b < nblocks; ++b) {
+      const BlockDev& B = c->hb[b];
+      const long long n = (long long)B.idim * B.jdim * B.kdim;
+      std::vector<double> pad(3 * B.P, 0.0);
+      for (int m = 0; m < 3; ++m)
+        for (int k = 0; k < B.kdim; ++k)
+          for (int j = 0; j < B.jdim; ++j)
+            for (int i = 0; i < B.idim; ++i)
+              pad[m * B.P + pidx(B, i + NG, j + NG, k + NG)] = xyz[off + m * n + ((long long)k * B.jdim + j) * B.idim + i];
+      off += 3 * n;
+      if (hipMemcpyAsync(B.fx, pad.data(), sizeof(double) * 3 * B.P, hipMemcpyHostToDevice, c->stream) != hipSuccess)
+        return bail(fail("grid upload failed"));
+      if (hipStreamSynchronize(c->stream) != hipSuccess) return bail(fail("grid upload sync failed"));
+    }
+    launch_metrics(LA(c, c->intr), c->intr.total(), p.gcl, nullptr, c->stream);
+    if (hipMemsetAsync(c->d_count, 0, sizeof(unsigned long long), c->stream) != hipSuccess) return bail(fail("memset"));
+    launch_count_small_vol(LA(c, c->intr), c->intr.total(), c->d_count);
+    unsigned long long bad = 0;
+    if (hipMemcpyAsync(&bad, c->d_count, sizeof bad, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess)
+      return bail(fail("metric check failed: %s", hipGetErrorString(hipGetLastError())));
+    if (bad) return bail(fail("ERROR(Jacobi<0): %llu points with volume below 1e-15 (MetricComputer.cpp:488-491)", bad));
+  }
+  if (p.nvis != 0) launch_temperature(LA(c, c->pad), c->pad.total(), p.gama, p.moo);
+  if (hipStreamSynchronize(c->stream) != hipSuccess) return bail(fail("setup failed: %s", hipGetErrorString(hipGetLastError())));
+  *out = c;
+  return 0;
+}
+
+static int field_info(orion_hip_ctx* c, int b, int f, double** ptr, long long* n) {
+  if (!c || b < 0 || b >= c->nb) return fail("bad context or block");
+  const BlockDev& B = c->hb[b];
+  switch (f) {
+    case ORION_FIELD_PRIM: *ptr = B.prim; *n = 6 * B.P; break;
+    case ORION_FIELD_METRICS: *ptr = B.met; *n = 9 * B.P; break;
+    case ORION_FIELD_VOL: *ptr = B.vol; *n = B.P; break;
+    case ORION_FIELD_Q:
+      if (!B.q) return fail("context created without keep_q");
+      *ptr = B.q; *n = 5 * B.P; break;
+    case ORION_FIELD_DQ: *ptr = B.dq; *n = 5 * B.P; break;
+    case ORION_FIELD_DT: *ptr = B.dt; *n = B.P; break;
+    case ORION_FIELD_C: *ptr = B.c; *n = B.P; break;
+    case ORION_FIELD_MU: *ptr = B.mu; *n = B.P; break;
+    default: return fail("unknown field %d", f);
+  }
+  return 0;
+}
+
+int orion_hip_field(orion_hip_ctx* c, int b, int f, double** ptr, long long* n) { return field_info(c, b, f, ptr, n); }
+
+int orion_hip_upload(orion_hip_ctx* c, int b, int f, const double* host, long long count) {
+  double* p;
+  long long n;
+  if (field_info(c, b, f, &p, &n)) return -1;
+  if (count != n) return fail("upload: field %d of block %d holds %lld values, got %lld", f, b, n, count);
+  HIPCHK(hipMemcpyAsync(p, host, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int orion_hip_download(orion_hip_ctx* c, int b, int f, double* host, long long count) {
+  double* p;
+  long long n;
+  if (field_info(c, b, f, &p, &n)) return -1;
+  if (count != n) return fail("download: field %d of block %d holds %lld values, got %lld", f, b, n, count);
+  HIPCHK(hipMemcpyAsync(host, p, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+static int check_launch(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail("%s: kernel launch failed: %s", what, hipGetErrorString(e));
+  return 0;
+}
+
+int orion_hip_update_temperature(orion_hip_ctx* c) {
+  if (c->p.nvis == 0) return 0;
+  launch_temperature(LA(c, c->pad), c->pad.total(), c->p.gama, c->p.moo);
+  return check_launch("update_temperature");
+}
+
+static void op_halo(orion_hip_ctx* c) { launch_halo(c->d_halo, c->n_halo, (double*)c->arena, c->stream); }
+
+static void op_bc(orion_hip_ctx* c) {
+  for (size_t rk = 0; rk < c->bc_first.size(); ++rk)
+    launch_bc(c->db, c->d_bc + c->bc_first[rk], c->bc_count[rk], c->bc_total[rk], c->p, c->stream);
+}
+
+static void op_visc(orion_hip_ctx* c) {
+  if (c->p.nvis != 1) return;
+  const double cp = 1.0 / ((c->p.gama - 1.0) * c->p.moo * c->p.moo);
+  const double cp_prl = cp / c->p.prl;
+  const double re = c->p.reynolds;
+  const double re_inv = (std::fabs(re) > 1.0e-30) ? 1.0 / re : 0.0;
+  launch_visc_deriv(LA(c, c->intr), c->intr.total());
+  for (int d = 0; d < 3; ++d) launch_visc_face(LA(c, c->face[d]), d, c->face[d].total(), cp_prl);
+  launch_visc_diff(LA(c, c->intr), c->intr.total(), re_inv);
+}
+
+static void op_avg(orion_hip_ctx* c) {
+  launch_avg_stage(c->db, c->d_avg_src, c->n_avg_src, c->d_stage, c->stream);
+  launch_avg_apply(c->db, c->d_avg_dst, c->n_avg_dst, c->d_stage, c->stream);
+}
+
+static void op_inv(orion_hip_ctx* c) {
+  for (int d = 0; d < 3; ++d) {
+    launch_inv_face(LA(c, c->face[d]), d, c->face[d].total(), c->p.gama);
+    launch_inv_closure(LA(c, c->line[d]), d, c->line[d].total(), c->p.gama);
+  }
+  launch_inv_diff(LA(c, c->intr), c->intr.total());
+}
+
+static void op_update(orion_hip_ctx* c) {
+  launch_lu_prep(LA(c, c->intr), c->intr.total());
+  const Launch lp = LA(c, c->plane);
+  const long long np = c->plane.total();
+  for (int s = 0; s < c->max_planes; ++s) launch_lusgs_plane(lp, np, s, 0, c->p);
+  for (int s = c->max_planes - 1; s >= 0; --s) launch_lusgs_plane(lp, np, s, 1, c->p);
+  for (int s = 0; s < c->max_planes; ++s) launch_lusgs_plane(lp, np, s, 2, c->p);
+  for (int s = c->max_planes - 1; s >= 0; --s) launch_lusgs_plane(lp, np, s, 3, c->p);
+  launch_update(LA(c, c->intr), c->intr.total(), c->p.gama);
+}
+
+static int ensure_res(orion_hip_ctx* c, int n) {
+  if (n <= c->res_cap) return 0;
+  // grow: old buffers stay owned by the context until destroy
+  if (dalloc(c, (void**)&c->d_res, sizeof(double) * 4 * n)) return -1;
+  if (dalloc(c, (void**)&c->d_dtmin, sizeof(unsigned long long) * n)) return -1;
+  c->res_cap = n;
+  return 0;
+}
+
+static void decode_stats(const orion_hip_ctx* c, const double* r, unsigned long long dtbits, orion_residual_stats* o) {
+  o->rms_residual = r[0];
+  o->max_residual = r[1];
+  const long long key = (long long)r[2];
+  const long long g = key / 5;
+  const int m = (int)(key % 5);
+  int b = 0;
+  while (b + 1 < c->nb && c->intr.h[b + 1] <= g) ++b;
+  const BlockDev& B = c->hb[b];
+  const long long n = g - c->intr.h[b];
+  o->max_loc[0] = b;
+  o->max_loc[1] = (int)(n % B.idim) + 1;
+  o->max_loc[2] = (int)((n / B.idim) % B.jdim) + 1;
+  o->max_loc[3] = (int)(n / ((long long)B.idim * B.jdim)) + 1;
+  o->max_loc[4] = m + 1;
+  double d;
+  std::memcpy(&d, &dtbits, sizeof d);
+  o->phydtime = d;
+}
+
+int orion_hip_exchange_bc(orion_hip_ctx* c) {
+  op_halo(c);
+  return check_launch("exchange_bc");
+}
+int orion_hip_apply_physical_bc(orion_hip_ctx* c) {
+  op_bc(c);
+  return check_launch("apply_physical_bc");
+}
+int orion_hip_update_flow_states(orion_hip_ctx* c) {
+  launch_state(LA(c, c->pad), c->pad.total(), c->p, 1, nullptr);
+  return check_launch("update_flow_states");
+}
+int orion_hip_calculate_time_step(orion_hip_ctx* c, double* phydtime) {
+  if (ensure_res(c, 1)) return -1;
+  const double init = 1000.0;
+  unsigned long long bits;
+  std::memcpy(&bits, &init, sizeof bits);
+  HIPCHK(hipMemcpyAsync(c->d_dtmin, &bits, sizeof bits, hipMemcpyHostToDevice, c->stream));
+  launch_state(LA(c, c->pad), c->pad.total(), c->p, 2, c->d_dtmin);
+  if (check_launch("calculate_time_step")) return -1;
+  if (phydtime) {
+    HIPCHK(hipMemcpyAsync(&bits, c->d_dtmin, sizeof bits, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    std::memcpy(phydtime, &bits, sizeof bits);
+  }
+  return 0;
+}
+int orion_hip_compute_viscous_rhs(orion_hip_ctx* c) {
+  op_visc(c);
+  return check_launch("compute_viscous_rhs");
+}
+int orion_hip_average_interface_residuals(orion_hip_ctx* c) {
+  op_avg(c);
+  return check_launch("average_interface_residuals");
+}
+int orion_hip_compute_inviscid_rhs(orion_hip_ctx* c) {
+  op_inv(c);
+  return check_launch("compute_inviscid_rhs");
+}
+int orion_hip_update_solution(orion_hip_ctx* c) {
+  op_update(c);
+  return check_launch("update_solution");
+}
+int orion_hip_residual(orion_hip_ctx* c, orion_residual_stats* out) {
+  if (ensure_res(c, 1)) return -1;
+  launch_residual(LA(c, c->intr), c->intr.total(), 5 * c->intr.total(), c->d_part, c->d_pidx, c->nparts, c->d_res);
+  if (check_launch("residual")) return -1;
+  double r[4];
+  unsigned long long bits = 0;
+  HIPCHK(hipMemcpyAsync(r, c->d_res, sizeof r, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(&bits, c->d_dtmin, sizeof bits, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (out) decode_stats(c, r, bits, out);
+  return 0;
+}
+
+int orion_hip_run(orion_hip_ctx* c, int nsteps, orion_residual_stats* hist) {
+  if (nsteps <= 0) return 0;
+  if (ensure_res(c, nsteps)) return -1;
+  {
+    const double init = 1000.0;
+    unsigned long long bits;
+    std::memcpy(&bits, &init, sizeof bits);
+    std::vector<unsigned long long> v(nsteps, bits);
+    HIPCHK(hipMemcpyAsync(c->d_dtmin, v.data(), sizeof(unsigned long long) * nsteps, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
+  for (int s = 0; s < nsteps; ++s) {
+    op_halo(c);
+    op_bc(c);
+    launch_state(LA(c, c->pad), c->pad.total(), c->p, 3, c->d_dtmin + s);
+    if (c->p.nvis == 1) {
+      op_visc(c);
+      op_avg(c);
+    }
+    op_inv(c);
+    op_avg(c);
+    op_update(c);
+    launch_residual(LA(c, c->intr), c->intr.total(), 5 * c->intr.total(), c->d_part, c->d_pidx, c->nparts,
+                    c->d_res + 4 * s);
+    if (check_launch("run")) return -1;
+  }
+  if (hist) {
+    std::vector<double> r(4 * (size_t)nsteps);
+    std::vector<unsigned long long> bits(nsteps);
+    HIPCHK(hipMemcpyAsync(r.data(), c->d_res, sizeof(double) * 4 * nsteps, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(bits.data(), c->d_dtmin, sizeof(unsigned long long) * nsteps, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int s = 0; s < nsteps; ++s) decode_stats(c, &r[4 * s], bits[s], &hist[s]);
+  }
+  return 0;
+}
+
+int orion_hip_synchronize(orion_hip_ctx* c) {
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int orion_hip_set_stream(orion_hip_ctx* c, void* s) {
+  c->stream = s ? (hipStream_t)s : c->own_stream;
+  return 0;
+}
+
+size_t orion_hip_device_bytes(const orion_hip_ctx* c) { return c->arena_bytes + c->map_bytes; }
+long long orion_hip_interior_nodes(const orion_hip_ctx* c) { return c->intr.total(); }
+
+}  // extern "C"

@@ -1,0 +1,610 @@
+// k_flux.hip -- right-hand-side kernels for gfx950 (FP64).
+//
+// Inviscid RHS (src/solver/InviscidFluxComputer.cpp:14-647):
+//   inv_face    one thread per face along direction d: WCNS-E-5 left/right
+//               states (fixed boundary stencils where the third ghost rho is
+//               below 1e-20), low-order fallback, Roe flux with Harten fix.
+//               Face f of a line is stored at the padded index of its left
+//               node (position f+2 along d).
+//   inv_closure one thread per line whose end carries the boundary signal:
+//               replays the reference's boundary recalculation (:377-521) on
+//               a private window and writes the three end-node differences.
+//   inv_diff    one thread per interior node: dq += fc_xi, fc_eta, fc_zeta in
+//               that order (6th-order difference, 3rd/4th-order ends, or the
+//               closure value).
+// Viscous RHS (src/solver/FluxComputer.cpp:13-732):
+//   visc_deriv  one thread per interior node: 4th-order node derivatives of
+//               (u,v,w,T) along xi, eta, zeta (one-sided next to rho<0 ghosts).
+//   visc_face   one thread per face: face interpolation of (u,v,w,T) (flagged),
+//               of mu, vol, metrics and cross derivatives (always central,
+//               so the zero ghost metrics and volumes enter the end faces),
+//               face derivative, chain rule, stress and heat flux.
+//   visc_diff   one thread per interior node: dq -= Re^-1 dFv per direction.
+//
+// Expression forms follow the cited reference lines operand for operand.
+#include "orion_dev.h"
+#include "orion_kernels.h"
+
+namespace orion_hip {
+
+// ---------------------------------------------------------------------------
+// Roe flux, InviscidFluxComputer.cpp:14-131
+// ---------------------------------------------------------------------------
+__device__ inline void phys_flux(const double* q, double g, double nx, double ny, double nz, double nt, double* f) {
+  const double rho = q[0], u = q[1], v = q[2], w = q[3], p = q[4];
+  const double V = u * nx + v * ny + w * nz + nt;
+  const double E = p / (g - 1.0) + 0.5 * rho * (u * u + v * v + w * w);
+  const double H = (E + p) / rho;
+  f[0] = rho * V;
+  f[1] = rho * u * V + p * nx;
+  f[2] = rho * v * V + p * ny;
+  f[3] = rho * w * V + p * nz;
+  f[4] = rho * H * V + p * nt;
+}
+
+__device__ inline void roe_flux(const double* ql, const double* qr, double nx, double ny, double nz, double nt,
+                                double g, double* f) {
+  const double efix = 0.1;
+  double fl[5], fr[5];
+  phys_flux(ql, g, nx, ny, nz, nt, fl);
+  phys_flux(qr, g, nx, ny, nz, nt, fr);
+  const double drho = qr[0] - ql[0], du = qr[1] - ql[1], dv = qr[2] - ql[2], dw = qr[3] - ql[3];
+  const double dp = qr[4] - ql[4];
+  const double g1 = g - 1.0;
+  const double g2 = g / g1;
+  const double hl = g2 * ql[4] / ql[0] + 0.5 * (ql[1] * ql[1] + ql[2] * ql[2] + ql[3] * ql[3]);
+  const double hr = g2 * qr[4] / qr[0] + 0.5 * (qr[1] * qr[1] + qr[2] * qr[2] + qr[3] * qr[3]);
+  const double sl = sqrt(ql[0]), sr = sqrt(qr[0]);
+  const double iden = 1.0 / (sl + sr);
+  const double rroe = sl * sr;
+  const double uroe = (sl * ql[1] + sr * qr[1]) * iden;
+  const double vroe = (sl * ql[2] + sr * qr[2]) * iden;
+  const double wroe = (sl * ql[3] + sr * qr[3]) * iden;
+  const double hroe = (sl * hl + sr * hr) * iden;
+  const double v2 = uroe * uroe + vroe * vroe + wroe * wroe;
+  double c2 = (hroe - 0.5 * v2) * g1;
+  if (c2 <= 0.0) c2 = 1.0e-6;
+  const double c = sqrt(c2);
+  const double U = uroe * nx + vroe * ny + wroe * nz + nt;
+  double gm = sqrt(nx * nx + ny * ny + nz * nz);
+  gm = (gm < 1.0e-30) ? 1.0e-30 : gm;
+  const double im = 1.0 / gm;
+  const double nxn = nx * im, nyn = ny * im, nzn = nz * im;
+  const double cm = c * gm;
+  double l1 = fabs(U - cm), l4 = fabs(U), l5 = fabs(U + cm);
+  const double de = efix * cm;
+  const double de2 = de * de;
+  if (l1 < de) l1 = sqrt(l1 * l1 + de2);
+  if (l4 < de) l4 = sqrt(l4 * l4 + de2);
+  if (l5 < de) l5 = sqrt(l5 * l5 + de2);
+  const double rodcta = rroe * (du * nxn + dv * nyn + dw * nzn);
+  const double dpc2 = dp / c2;
+  const double rodctac = rodcta / c;
+  const double a1 = l4 * (drho - dpc2);
+  const double a2 = l5 * (dpc2 + rodctac) * 0.5;
+  const double a3 = l1 * (dpc2 - rodctac) * 0.5;
+  const double a4 = a1 + a2 + a3;
+  const double a5 = c * (a2 - a3);
+  const double a6 = l4 * (rroe * du - nxn * rodcta);
+  const double a7 = l4 * (rroe * dv - nyn * rodcta);
+  const double a8 = l4 * (rroe * dw - nzn * rodcta);
+  double df[5];
+  df[0] = a4;
+  df[1] = uroe * a4 + nxn * a5 + a6;
+  df[2] = vroe * a4 + nyn * a5 + a7;
+  df[3] = wroe * a4 + nzn * a5 + a8;
+  df[4] = hroe * a4 + U * im * a5 + uroe * a6 + vroe * a7 + wroe * a8 - c2 * a1 / g1;
+#pragma unroll
+  for (int m = 0; m < 5; ++m) f[m] = 0.5 * (fl[m] + fr[m] - df[m]);
+}
+
+// WCNS-E-5 left/right states of one variable at the face between line
+// positions p+2 and p+3, from q[p..p+5] (InviscidFluxComputer.cpp:268-314).
+__device__ inline void weno5(double qm2, double qm1, double q0, double qp1, double qp2, double qp3, double* ql,
+                             double* qr) {
+  const double CL1 = 1.0 / 16.0, CL2 = 10.0 / 16.0, CL3 = 5.0 / 16.0, EPS = 1.0e-6;
+  const double s1 = qm2 - 2.0 * qm1 + q0;
+  const double s2 = qm1 - 2.0 * q0 + qp1;
+  const double s3 = q0 - 2.0 * qp1 + qp2;
+  const double g1 = 0.5 * (qm2 - 4.0 * qm1 + 3.0 * q0);
+  const double g2 = 0.5 * (qp1 - qm1);
+  const double g3 = 0.5 * (-3.0 * q0 + 4.0 * qp1 - qp2);
+  const double is1 = g1 * g1 + s1 * s1, is2 = g2 * g2 + s2 * s2, is3 = g3 * g3 + s3 * s3;
+  const double w1 = CL1 / ((EPS + is1) * (EPS + is1));
+  const double w2 = CL2 / ((EPS + is2) * (EPS + is2));
+  const double w3 = CL3 / ((EPS + is3) * (EPS + is3));
+  const double sw = w1 + w2 + w3;
+  *ql = q0 + 0.125 * ((w1 / sw) * (s1 + 4.0 * g1) + (w2 / sw) * (s2 + 4.0 * g2) + (w3 / sw) * (s3 + 4.0 * g3));
+  const double r1s = s2, r2s = s3, r3s = qp1 - 2.0 * qp2 + qp3;
+  const double h1 = 0.5 * (qm1 - 4.0 * q0 + 3.0 * qp1);
+  const double h2 = 0.5 * (qp2 - q0);
+  const double h3 = 0.5 * (-3.0 * qp1 + 4.0 * qp2 - qp3);
+  const double js1 = h1 * h1 + r1s * r1s, js2 = h2 * h2 + r2s * r2s, js3 = h3 * h3 + r3s * r3s;
+  const double v1 = CL3 / ((EPS + js1) * (EPS + js1));
+  const double v2 = CL2 / ((EPS + js2) * (EPS + js2));
+  const double v3 = CL1 / ((EPS + js3) * (EPS + js3));
+  const double sv = v1 + v2 + v3;
+  *qr = qp1 + 0.125 * ((v1 / sv) * (r1s - 4.0 * h1) + (v2 / sv) * (r2s - 4.0 * h2) + (v3 / sv) * (r3s - 4.0 * h3));
+}
+
+struct LineRef {
+  const double* prim;
+  long long P;
+  long long base;
+  long long s;
+  __device__ inline double at(int m, int p) const { return prim[(long long)m * P + base + (long long)p * s]; }
+};
+
+// Initial face states of face f (after the WCNS pass and the fixed boundary
+// stencils, InviscidFluxComputer.cpp:262-337).
+__device__ inline void face_states(const LineRef& L, int n, int ist, int ied, int f, double* qwl, double* qwr) {
+  if (f >= ist && f <= ied) {
+#pragma unroll
+    for (int m = 0; m < 5; ++m)
+      weno5(L.at(m, f), L.at(m, f + 1), L.at(m, f + 2), L.at(m, f + 3), L.at(m, f + 4), L.at(m, f + 5), &qwl[m],
+            &qwr[m]);
+  }
+  if (ist > 1 && f <= 2) {
+#pragma unroll
+    for (int m = 0; m < 5; ++m) {
+      const double q1 = L.at(m, 3), q2 = L.at(m, 4), q3 = L.at(m, 5), q4 = L.at(m, 6);
+      if (f == 0) qwl[m] = (35. * q1 - 35. * q2 + 21. * q3 - 5. * q4) / 16.;
+      else if (f == 1) qwl[m] = (5. * q1 + 15. * q2 - 5. * q3 + q4) / 16.;
+      else qwl[m] = (-q1 + 9. * q2 + 9. * q3 - q4) / 16.;
+      if (f <= 1) qwr[m] = qwl[m];
+    }
+  }
+  if (ied < n && f >= n - 2) {
+#pragma unroll
+    for (int m = 0; m < 5; ++m) {
+      const double qn = L.at(m, n + 2), qn1 = L.at(m, n + 1), qn2 = L.at(m, n), qn3 = L.at(m, n - 1);
+      if (f == n) qwr[m] = (35. * qn - 35. * qn1 + 21. * qn2 - 5. * qn3) / 16.;
+      else if (f == n - 1) qwr[m] = (5. * qn + 15. * qn1 - 5. * qn2 + qn3) / 16.;
+      else qwr[m] = (-qn + 9. * qn1 + 9. * qn2 - qn3) / 16.;
+      if (f >= n - 1) qwl[m] = qwr[m];
+    }
+  }
+}
+
+// Half-node metric (kx,ky,kz) at face f, interp_metrics_half (:136-192).
+__device__ inline void metric_face(const BlockDev& b, int d, long long base, long long s, int n, int f, double* k3) {
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const double* M = b.met + (long long)(3 * d + c) * b.P + base;
+    double v;
+    if (f == 0) v = (35.0 * M[3 * s] + -35.0 * M[4 * s] + 21.0 * M[5 * s] + -5.0 * M[6 * s]) / 16.0;
+    else if (f == 1) v = (5.0 * M[3 * s] + 15.0 * M[4 * s] + -5.0 * M[5 * s] + 1.0 * M[6 * s]) / 16.0;
+    else if (f == n)
+      v = (35.0 * M[(n + 2) * s] + -35.0 * M[(n + 1) * s] + 21.0 * M[n * s] + -5.0 * M[(n - 1) * s]) / 16.0;
+    else if (f == n - 1)
+      v = (5.0 * M[(n + 2) * s] + 15.0 * M[(n + 1) * s] + -5.0 * M[n * s] + 1.0 * M[(n - 1) * s]) / 16.0;
+    else
+      v = (9.0 * (M[(f + 2) * s] + M[(f + 3) * s]) + -1.0 * (M[(f + 4) * s] + M[(f + 1) * s])) / 16.0;
+    k3[c] = v;
+  }
+}
+
+// Roe flux from face states with the low-order fallback (:342-364).
+__device__ inline void face_flux(const LineRef& L, int n, int f, const double* qwl, const double* qwr,
+                                 const double* k3, double g, double* out) {
+  double a[5], c[5];
+#pragma unroll
+  for (int m = 0; m < 5; ++m) {
+    a[m] = qwl[m];
+    c[m] = qwr[m];
+  }
+  if (a[0] <= 1e-6 || a[4] <= 1e-6) {
+    const int ix = (f - 1 > 0 ? f - 1 : 0) + 2;
+#pragma unroll
+    for (int m = 0; m < 5; ++m) a[m] = L.at(m, ix);
+  }
+  if (c[0] <= 1e-6 || c[4] <= 1e-6) {
+    const int ix = (n - 1 < f ? n - 1 : f) + 2 + 1;
+#pragma unroll
+    for (int m = 0; m < 5; ++m) c[m] = L.at(m, ix);
+  }
+  roe_flux(a, c, k3[0], k3[1], k3[2], 0.0, g, out);
+}
+
+__global__ void __launch_bounds__(256) inv_face_kernel(const BlockDev* __restrict__ blocks,
+                                                       const long long* __restrict__ off, int nb, int d,
+                                                       double gamma) {
+  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= off[nb]) return;
+  const int bi = find_block(off, nb, g);
+  const BlockDev& b = blocks[bi];
+  long long base;
+  int f;
+  decode_face(b, d, g - off[bi], &base, &f);
+  const int n = dcount(b, d);
+  const long long s = dstride(b, d);
+  const LineRef L{b.prim, b.P, base, s};
+  const int ist = (L.at(0, 0) < 1.0e-20) ? 2 : 0;
+  const int ied = (L.at(0, n + 5) < 1.0e-20) ? n - 2 : n;
+  double qwl[5], qwr[5], k3[3], fo[5];
+  face_states(L, n, ist, ied, f, qwl, qwr);
+  metric_face(b, d, base, s, n, f, k3);
+  face_flux(L, n, f, qwl, qwr, k3, gamma, fo);
+  double* F = b.fx + (long long)d * 5 * b.P + base + (long long)(f + 2) * s;
+#pragma unroll
+  for (int m = 0; m < 5; ++m) F[(long long)m * b.P] = fo[m];
+}
+
+// Boundary recalculation (InviscidFluxComputer.cpp:377-521) for one line.
+__global__ void __launch_bounds__(64) inv_closure_kernel(const BlockDev* __restrict__ blocks,
+                                                         const long long* __restrict__ off, int nb, int d,
+                                                         double gamma) {
+  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= off[nb]) return;
+  const int bi = find_block(off, nb, g);
+  const BlockDev& b = blocks[bi];
+  const long long ln = g - off[bi];
+  const long long base = decode_line(b, d, ln);
+  const int n = dcount(b, d);
+  const long long s = dstride(b, d);
+  const LineRef L{b.prim, b.P, base, s};
+  const int ist = (L.at(0, 0) < 1.0e-20) ? 2 : 0;
+  const int ied = (L.at(0, n + 5) < 1.0e-20) ? n - 2 : n;
+  if (ist <= 1 && ied >= n) return;
+  // private window over faces [0..5] U [n-5..n]; slot(f) is unique per face
+  constexpr int W = 12;
+  double QL[W][5], QR[W][5], FF[W][5];
+  auto slot = [n](int f) { return (n <= 11) ? f : (f <= 5 ? f : f - (n - 11)); };
+  double k3[W][3];
+  for (int f = 0; f <= n; ++f) {
+    const bool need = ((ist > 1) && f <= 5) || ((ied < n) && f >= n - 5);
+    if (!need) continue;
+    const int sl = slot(f);
+    face_states(L, n, ist, ied, f, QL[sl], QR[sl]);
+    metric_face(b, d, base, s, n, f, k3[sl]);
+  }
+  auto flux = [&](int st, int en) {
+    for (int f = st; f <= en; ++f) face_flux(L, n, f, QL[slot(f)], QR[slot(f)], k3[slot(f)], gamma, FF[slot(f)]);
+  };
+  auto F = [&](int f, int m) { return FF[slot(f)][m]; };
+  double* out = b.fcov + b.fcov_off[d] + ln * 30;
+  if (ist > 1) {
+    double uro[5][5];
+    for (int i = 1; i <= 4; ++i)
+      for (int m = 0; m < 5; ++m) uro[i][m] = QR[slot(i)][m];
+    for (int m = 0; m < 5; ++m) {
+      const double a = L.at(m, 3), bb = L.at(m, 4), cc = L.at(m, 5), dd = L.at(m, 6);
+      QL[slot(0)][m] = (3. * a - 1. * bb) / 2.0;
+      QR[slot(0)][m] = QL[slot(0)][m];
+      QL[slot(1)][m] = (19. * a + 25. * bb - 2. * cc) / 42.0;
+      QR[slot(1)][m] = QL[slot(1)][m];
+      QL[slot(2)][m] = (4. * a + 3. * bb + 9. * cc + 2. * dd) / 18.0;
+      QR[slot(2)][m] = QL[slot(2)][m];
+      QL[slot(3)][m] = (-2. * a + 3. * bb + 3. * cc + 2. * dd) / 6.0;
+      QR[slot(3)][m] = QL[slot(3)][m];
+    }
+    flux(0, 3);
+    for (int m = 0; m < 5; ++m) out[0 * 5 + m] = (-23. * F(0, m) + 21. * F(1, m) + 3. * F(2, m) - F(3, m)) / 24.0;
+    for (int m = 0; m < 5; ++m) {
+      const double a = L.at(m, 3), bb = L.at(m, 4), cc = L.at(m, 5), dd = L.at(m, 6);
+      QL[slot(1)][m] = (3. * a + 6. * bb - cc) / 8.0;
+      QR[slot(1)][m] = QL[slot(1)][m];
+      QL[slot(2)][m] = (-30. * a + 145. * bb + 29. * cc - 8. * dd) / 136.0;
+      QR[slot(2)][m] = (2. * a + 49. * bb + 125. * cc - 40. * dd) / 136.0;
+      QL[slot(3)][m] = (-bb + 6. * cc + 3. * dd) / 8.0;
+      QR[slot(3)][m] = QL[slot(3)][m];
+      QL[slot(4)][m] = (bb + 2. * cc + 5. * dd) / 8.0;
+      QR[slot(4)][m] = QL[slot(4)][m];
+      QL[slot(5)][m] = (bb + cc + 6. * dd) / 8.0;
+      QR[slot(5)][m] = QL[slot(5)][m];
+    }
+    flux(1, 5);
+    for (int m = 0; m < 5; ++m)
+      out[1 * 5 + m] = (-22. * F(1, m) + 17. * F(2, m) + 9. * F(3, m) - 5. * F(4, m) + F(5, m)) / 24.0;
+    for (int m = 0; m < 5; ++m) {
+      const double a = L.at(m, 3), bb = L.at(m, 4), cc = L.at(m, 5), dd = L.at(m, 6);
+      QR[slot(1)][m] = uro[1][m];
+      QL[slot(2)][m] = (-29. * a + 170. * bb + 63. * cc + 12. * dd) / 216.0;
+      QR[slot(2)][m] = uro[2][m];
+      QR[slot(3)][m] = uro[3][m];
+      QR[slot(4)][m] = uro[4][m];
+    }
+    flux(1, 4);
+    for (int m = 0; m < 5; ++m) out[2 * 5 + m] = (F(1, m) - 27. * F(2, m) + 27. * F(3, m) - F(4, m)) / 24.0;
+  }
+  if (ied < n) {
+    const int N = n;
+    double ulo[9][5];
+    for (int i = 1; i <= 4; ++i)
+      for (int m = 0; m < 5; ++m) ulo[9 - i][m] = QL[slot(N - i)][m];
+    for (int m = 0; m < 5; ++m) {
+      const double a = L.at(m, N + 2), bb = L.at(m, N + 1), cc = L.at(m, N), dd = L.at(m, N - 1);
+      QR[slot(N)][m] = (3. * a - 1. * bb) / 2.0;
+      QL[slot(N)][m] = QR[slot(N)][m];
+      QR[slot(N - 1)][m] = (19. * a + 25. * bb - 2. * cc) / 42.0;
+      QL[slot(N - 1)][m] = QR[slot(N - 1)][m];
+      QR[slot(N - 2)][m] = (4. * a + 3. * bb + 9. * cc + 2. * dd) / 18.0;
+      QL[slot(N - 2)][m] = QR[slot(N - 2)][m];
+      QR[slot(N - 3)][m] = (-2. * a + 3. * bb + 3. * cc + 2. * dd) / 6.0;
+      QL[slot(N - 3)][m] = QR[slot(N - 3)][m];
+    }
+    flux(N - 3, N);
+    for (int m = 0; m < 5; ++m)
+      out[5 * 5 + m] = -(-23. * F(N, m) + 21. * F(N - 1, m) + 3. * F(N - 2, m) - F(N - 3, m)) / 24.0;
+    for (int m = 0; m < 5; ++m) {
+      const double a = L.at(m, N + 2), bb = L.at(m, N + 1), cc = L.at(m, N), dd = L.at(m, N - 1);
+      QR[slot(N - 1)][m] = (3. * a + 6. * bb - cc) / 8.0;
+      QL[slot(N - 1)][m] = QR[slot(N - 1)][m];
+      QR[slot(N - 2)][m] = (-30. * a + 145. * bb + 29. * cc - 8. * dd) / 136.0;
+      QL[slot(N - 2)][m] = (2. * a + 49. * bb + 125. * cc - 40. * dd) / 136.0;
+      QR[slot(N - 3)][m] = (-bb + 6. * cc + 3. * dd) / 8.0;
+      QL[slot(N - 3)][m] = QR[slot(N - 3)][m];
+      QR[slot(N - 4)][m] = (bb + 2. * cc + 5. * dd) / 8.0;
+      QL[slot(N - 4)][m] = QR[slot(N - 4)][m];
+      QR[slot(N - 5)][m] = (bb + cc + 6. * dd) / 8.0;
+      QL[slot(N - 5)][m] = QR[slot(N - 5)][m];
+    }
+    flux(N - 5, N - 1);
+    for (int m = 0; m < 5; ++m)
+      out[4 * 5 + m] =
+          -(-22. * F(N - 1, m) + 17. * F(N - 2, m) + 9. * F(N - 3, m) - 5. * F(N - 4, m) + F(N - 5, m)) / 24.0;
+    for (int m = 0; m < 5; ++m) {
+      const double a = L.at(m, N + 2), bb = L.at(m, N + 1), cc = L.at(m, N), dd = L.at(m, N - 1);
+      QL[slot(N - 1)][m] = ulo[8][m];
+      QR[slot(N - 2)][m] = (-29. * a + 170. * bb + 63. * cc + 12. * dd) / 216.0;
+      QL[slot(N - 2)][m] = ulo[7][m];
+      QL[slot(N - 3)][m] = ulo[6][m];
+      QL[slot(N - 4)][m] = ulo[5][m];
+    }
+    flux(N - 4, N - 1);
+    for (int m = 0; m < 5; ++m)
+      out[3 * 5 + m] = -(F(N - 1, m) - 27. * F(N - 2, m) + 27. * F(N - 3, m) - F(N - 4, m)) / 24.0;
+  }
+}
+
+// 6th-order difference with 3rd/4th-order ends
+// (compute_flux_diff_high_order :197-233 == compute_flux_diff_1d :257-296)
+// for interior node t (0-based along the line); F(f) is face f.
+__device__ inline double diff_node(const double* F0, long long s, int n, int t) {
+  auto F = [&](int f) { return F0[(long long)(f + 2) * s]; };
+  if (t >= 2 && t < n - 2)
+    return (2250.0 * (F(t + 1) - F(t)) + -125.0 * (F(t + 2) - F(t - 1)) + 9.0 * (F(t + 3) - F(t - 2))) / 1920.0;
+  if (t == 1) return (F(0) - 27.0 * F(1) + 27.0 * F(2) - F(3)) / 24.0;
+  if (t == n - 2) return -(F(n) - 27.0 * F(n - 1) + 27.0 * F(n - 2) - F(n - 3)) / 24.0;
+  if (t == 0) return (-23.0 * F(0) + 21.0 * F(1) + 3.0 * F(2) - F(3)) / 24.0;
+  return -(-23.0 * F(n) + 21.0 * F(n - 1) + 3.0 * F(n - 2) - F(n - 3)) / 24.0;
+}
+
+__global__ void __launch_bounds__(256) inv_diff_kernel(const BlockDev* __restrict__ blocks,
+                                                       const long long* __restrict__ off, int nb) {
+  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= off[nb]) return;
+  const int bi = find_block(off, nb, g);
+  const BlockDev& b = blocks[bi];
+  int i, j, k;
+  decode_interior(b, g - off[bi], &i, &j, &k);
+  const long long id = pidx(b, i, j, k);
+  double acc[5];
+#pragma unroll
+  for (int m = 0; m < 5; ++m) acc[m] = b.dq[(long long)m * b.P + id];
+  const int pos[3] = {i, j, k};
+  for (int d = 0; d < 3; ++d) {
+    const int n = dcount(b, d);
+    const long long s = dstride(b, d);
+    const int t = pos[d] - NG;
+    const long long lbase = id - (long long)pos[d] * s;
+    const bool lflag = b.prim[lbase] < 1.0e-20;
+    const bool rflag = b.prim[lbase + (long long)(n + 5) * s] < 1.0e-20;
+    const double* cov = nullptr;
+    if (lflag && t <= 2) cov = b.fcov + b.fcov_off[d] + line_id(b, d, i, j, k) * 30 + t * 5;
+    else if (rflag && t >= n - 3) cov = b.fcov + b.fcov_off[d] + line_id(b, d, i, j, k) * 30 + (3 + t - (n - 3)) * 5;
+#pragma unroll
+    for (int m = 0; m < 5; ++m) {
+      double fc;
+      if (cov) fc = cov[m];
+      else fc = diff_node(b.fx + ((long long)d * 5 + m) * b.P + lbase, s, n, t);
+      acc[m] += fc;
+    }
+  }
+#pragma unroll
+  for (int m = 0; m < 5; ++m) b.dq[(long long)m * b.P + id] = acc[m];
+}
+
+// ---------------------------------------------------------------------------
+// Viscous RHS
+// ---------------------------------------------------------------------------
+__device__ inline double uvwt_at(const BlockDev& b, long long id, int v) {
+  // (u,v,w,T) = prim 1,2,3,5
+  return b.prim[(long long)(v < 3 ? v + 1 : 5) * b.P + id];
+}
+
+// compute_node_deriv_line (FluxComputer.cpp:13-67) at 1-based position i1.
+__device__ inline double node_deriv(const double* q, long long s, int n, int i1, bool fl, bool fr) {
+  // q points at line position 0; Q(p) = value at position p
+  auto Q = [&](int p) { return q[(long long)p * s]; };
+  if (fl && i1 == 1) return (-11.0 * Q(3) + 18.0 * Q(4) - 9.0 * Q(5) + 2.0 * Q(6)) / 6.0;
+  if (fl && i1 == 2) return (-3.0 * Q(3) + -10.0 * Q(4) + 18.0 * Q(5) + -6.0 * Q(6) + 1.0 * Q(7)) / 12.0;
+  if (fr && i1 == n) return -(-11.0 * Q(n + 2) + 18.0 * Q(n + 1) - 9.0 * Q(n) + 2.0 * Q(n - 1)) / 6.0;
+  if (fr && i1 == n - 1)
+    return -(-3.0 * Q(n + 2) + -10.0 * Q(n + 1) + 18.0 * Q(n) + -6.0 * Q(n - 1) + 1.0 * Q(n - 2)) / 12.0;
+  return (8.0 * (Q(i1 + 3) - Q(i1 + 1)) + -1.0 * (Q(i1 + 4) - Q(i1))) / 12.0;
+}
+
+__global__ void __launch_bounds__(256) visc_deriv_kernel(const BlockDev* __restrict__ blocks,
+                                                         const long long* __restrict__ off, int nb) {
+  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= off[nb]) return;
+  const int bi = find_block(off, nb, g);
+  const BlockDev& b = blocks[bi];
+  int i, j, k;
+  decode_interior(b, g - off[bi], &i, &j, &k);
+  const long long id = pidx(b, i, j, k);
+  const int pos[3] = {i, j, k};
+  for (int d = 0; d < 3; ++d) {
+    const int n = dcount(b, d);
+    const long long s = dstride(b, d);
+    const long long lbase = id - (long long)pos[d] * s;
+    const bool fl = b.prim[lbase] < 0.0;
+    const bool fr = b.prim[lbase + (long long)(n + 5) * s] < 0.0;
+    const int i1 = pos[d] - NG + 1;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const double* q = b.prim + (long long)(v < 3 ? v + 1 : 5) * b.P + lbase;
+      b.duvwt[(long long)(4 * d + v) * b.P + id] = node_deriv(q, s, n, i1, fl, fr);
+    }
+  }
+}
+
+// interp_line_to_half central form at face f (positions f+1..f+4)
+__device__ inline double half_central(const double* q, long long s, int f) {
+  return (9.0 * (q[(long long)(f + 2) * s] + q[(long long)(f + 3) * s]) +
+          -1.0 * (q[(long long)(f + 4) * s] + q[(long long)(f + 1) * s])) /
+         16.0;
+}
+// interp_line_to_half with flags (FluxComputer.cpp:129-178)
+__device__ inline double half_flagged(const double* q, long long s, int n, int f, bool fl, bool fr) {
+  auto Q = [&](int p) { return q[(long long)p * s]; };
+  if (fl && f == 0) return (35.0 * Q(3) + -35.0 * Q(4) + 21.0 * Q(5) + -5.0 * Q(6)) / 16.0;
+  if (fl && f == 1) return (5.0 * Q(3) + 15.0 * Q(4) + -5.0 * Q(5) + 1.0 * Q(6)) / 16.0;
+  if (fr && f == n) return (35.0 * Q(n + 2) + -35.0 * Q(n + 1) + 21.0 * Q(n) + -5.0 * Q(n - 1)) / 16.0;
+  if (fr && f == n - 1) return (5.0 * Q(n + 2) + 15.0 * Q(n + 1) + -5.0 * Q(n) + 1.0 * Q(n - 1)) / 16.0;
+  return half_central(q, s, f);
+}
+// compute_half_deriv_line (FluxComputer.cpp:72-124)
+__device__ inline double half_deriv(const double* q, long long s, int n, int f, bool fl, bool fr) {
+  auto Q = [&](int p) { return q[(long long)p * s]; };
+  if (fl && f == 0) return (-71.0 * Q(3) + 141.0 * Q(4) + -93.0 * Q(5) + 23.0 * Q(6)) / 24.0;
+  if (fl && f == 1) return (-22.0 * Q(3) + 17.0 * Q(4) + 9.0 * Q(5) + -5.0 * Q(6) + 1.0 * Q(7)) / 24.0;
+  if (fr && f == n) return -(-71.0 * Q(n + 2) + 141.0 * Q(n + 1) + -93.0 * Q(n) + 23.0 * Q(n - 1)) / 24.0;
+  if (fr && f == n - 1)
+    return -(-22.0 * Q(n + 2) + 17.0 * Q(n + 1) + 9.0 * Q(n) + -5.0 * Q(n - 1) + 1.0 * Q(n - 2)) / 24.0;
+  return (27.0 * (Q(f + 3) - Q(f + 2)) + -1.0 * (Q(f + 4) - Q(f + 1))) / 24.0;
+}
+
+__global__ void __launch_bounds__(256) visc_face_kernel(const BlockDev* __restrict__ blocks,
+                                                        const long long* __restrict__ off, int nb, int d,
+                                                        double cp_prl) {
+  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= off[nb]) return;
+  const int bi = find_block(off, nb, g);
+  const BlockDev& b = blocks[bi];
+  long long base;
+  int f;
+  decode_face(b, d, g - off[bi], &base, &f);
+  const int n = dcount(b, d);
+  const long long s = dstride(b, d);
+  const long long P = b.P;
+  const bool fl = b.prim[base] < 0.0;
+  const bool fr = b.prim[base + (long long)(n + 5) * s] < 0.0;
+  double uvwt[4], dmid[4];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const double* q = b.prim + (long long)(v < 3 ? v + 1 : 5) * P + base;
+    uvwt[v] = half_flagged(q, s, n, f, fl, fr);
+    dmid[v] = half_deriv(q, s, n, f, fl, fr);
+  }
+  const double vs1 = half_central(b.mu + base, s, f);
+  const double vs2 = vs1 * cp_prl;
+  const double volh = half_central(b.vol + base, s, f);
+  double kx[9];
+#pragma unroll
+  for (int m = 0; m < 9; ++m) kx[m] = half_central(b.met + (long long)m * P + base, s, f);
+  // cross derivatives at the face from node derivatives clamped to the
+  // interior (positions f+1..f+4 -> clamp to [NG, n+NG-1])
+  const int o0 = (d == 0) ? 1 : 0, o1 = (d == 2) ? 1 : 2;
+  double duv[12];
+  auto clampp = [n](int p) { return p < NG ? NG : (p > n + NG - 1 ? n + NG - 1 : p); };
+  const int p1 = clampp(f + 1), p2 = clampp(f + 2), p3 = clampp(f + 3), p4 = clampp(f + 4);
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    duv[4 * d + v] = dmid[v];
+    const double* A = b.duvwt + (long long)(4 * o0 + v) * P + base;
+    duv[4 * o0 + v] = (9.0 * (A[p2 * s] + A[p3 * s]) + -1.0 * (A[p4 * s] + A[p1 * s])) / 16.0;
+    const double* B = b.duvwt + (long long)(4 * o1 + v) * P + base;
+    duv[4 * o1 + v] = (9.0 * (B[p2 * s] + B[p3 * s]) + -1.0 * (B[p4 * s] + B[p1 * s])) / 16.0;
+  }
+  // transform_derivs_to_phys (:183-205)
+  const double vinv = 1.0 / volh;
+  double gx[12];
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    const double k1 = kx[m], k2 = kx[m + 3], k3 = kx[m + 6];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) gx[v * 3 + m] = (k1 * duv[v] + k2 * duv[v + 4] + k3 * duv[v + 8]) * vinv;
+  }
+  // compute_viscous_flux_1d (:210-252)
+  const double vscc = vs1 * (2.0 / 3.0);
+  const double dudx = gx[0], dudy = gx[1], dudz = gx[2];
+  const double dvdx = gx[3], dvdy = gx[4], dvdz = gx[5];
+  const double dwdx = gx[6], dwdy = gx[7], dwdz = gx[8];
+  const double txx = vscc * (2.0 * dudx - dvdy - dwdz);
+  const double tyy = vscc * (2.0 * dvdy - dwdz - dudx);
+  const double tzz = vscc * (2.0 * dwdz - dudx - dvdy);
+  const double txy = vs1 * (dudy + dvdx);
+  const double txz = vs1 * (dudz + dwdx);
+  const double tyz = vs1 * (dvdz + dwdy);
+  const double nx = kx[3 * d], ny = kx[3 * d + 1], nz = kx[3 * d + 2];
+  const double f1 = txx * nx + txy * ny + txz * nz;
+  const double f2 = txy * nx + tyy * ny + tyz * nz;
+  const double f3 = txz * nx + tyz * ny + tzz * nz;
+  const double work = uvwt[0] * f1 + uvwt[1] * f2 + uvwt[2] * f3;
+  const double heat = vs2 * (gx[9] * nx + gx[10] * ny + gx[11] * nz);
+  double* F = b.fx + (long long)d * 5 * P + base + (long long)(f + 2) * s;
+  F[0] = 0.0;
+  F[P] = f1;
+  F[2 * P] = f2;
+  F[3 * P] = f3;
+  F[4 * P] = work + heat;
+}
+
+__global__ void __launch_bounds__(256) visc_diff_kernel(const BlockDev* __restrict__ blocks,
+                                                        const long long* __restrict__ off, int nb, double re_inv) {
+  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= off[nb]) return;
+  const int bi = find_block(off, nb, g);
+  const BlockDev& b = blocks[bi];
+  int i, j, k;
+  decode_interior(b, g - off[bi], &i, &j, &k);
+  const long long id = pidx(b, i, j, k);
+  double acc[5];
+#pragma unroll
+  for (int m = 0; m < 5; ++m) acc[m] = b.dq[(long long)m * b.P + id];
+  const int pos[3] = {i, j, k};
+  for (int d = 0; d < 3; ++d) {
+    const int n = dcount(b, d);
+    const long long s = dstride(b, d);
+    const long long lbase = id - (long long)pos[d] * s;
+    const int t = pos[d] - NG;
+#pragma unroll
+    for (int m = 0; m < 5; ++m) acc[m] -= re_inv * diff_node(b.fx + ((long long)d * 5 + m) * b.P + lbase, s, n, t);
+  }
+#pragma unroll
+  for (int m = 0; m < 5; ++m) b.dq[(long long)m * b.P + id] = acc[m];
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+static inline unsigned grid_of(long long n, int tb) { return (unsigned)((n + tb - 1) / tb); }
+
+void launch_inv_face(const Launch& L, int d, long long n, double gamma) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(inv_face_kernel, dim3(grid_of(n, 256)), dim3(256), 0, L.stream, L.blocks, L.off, L.nb, d, gamma);
+}
+void launch_inv_closure(const Launch& L, int d, long long n, double gamma) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(inv_closure_kernel, dim3(grid_of(n, 64)), dim3(64), 0, L.stream, L.blocks, L.off, L.nb, d, gamma);
+}
+void launch_inv_diff(const Launch& L, long long n) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(inv_diff_kernel, dim3(grid_of(n, 256)), dim3(256), 0, L.stream, L.blocks, L.off, L.nb);
+}
+void launch_visc_deriv(const Launch& L, long long n) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(visc_deriv_kernel, dim3(grid_of(n, 256)), dim3(256), 0, L.stream, L.blocks, L.off, L.nb);
+}
+void launch_visc_face(const Launch& L, int d, long long n, double cp_prl) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(visc_face_kernel, dim3(grid_of(n, 256)), dim3(256), 0, L.stream, L.blocks, L.off, L.nb, d, cp_prl);
+}
+void launch_visc_diff(const Launch& L, long long n, double re_inv) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(visc_diff_kernel, dim3(grid_of(n, 256)), dim3(256), 0, L.stream, L.blocks, L.off, L.nb, re_inv);
+}
+
+}  // namespace orion_hip

@@ -1,0 +1,331 @@
+// k_point.hip -- pointwise kernels for gfx950 (FP64).
+//
+//   state_kernel  update_flow_states (src/solver/StateUpdater.cpp:15-168) and
+//                 calculate_time_step (src/solver/TimeIntegrator.cpp:20-178)
+//                 fused per point: c and T on the interior plus the ghost
+//                 "cross" layers (|rho| there), Sutherland mu on every point,
+//                 dq=0, then the spectral radii and dt/vol on interior points
+//                 (the only points whose radii the solver reads) and the
+//                 minimum physical time step.
+//   lu_prep       rhs = dq with physical-BC windows zeroed
+//                 (StateUpdater.cpp:447-468, 557-570).
+//   update        zero the BC windows again and apply update_limited
+//                 (StateUpdater.cpp:470-542, 577-578).
+//   residual      ResidualMonitor::compute (src/solver/ResidualMonitor.cpp:11-138):
+//                 deterministic two-pass reduction of sum r^2 and of the
+//                 first maximum in (block,k,j,i,var) order.
+#include "orion_dev.h"
+#include "orion_kernels.h"
+
+namespace orion_hip {
+
+__global__ void __launch_bounds__(256) state_kernel(const BlockDev* __restrict__ blocks,
+                                                    const long long* __restrict__ off, int nb, Params p, int mode,
+                                                    unsigned long long* dtmin_bits) {
+  __shared__ double smin[256];
+  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  double mydt = 1000.0;
+  if (g < off[nb]) {
+    const int bi = find_block(off, nb, g);
+    const BlockDev& b = blocks[bi];
+    const long long n = g - off[bi];
+    const int i = (int)(n % b.nx);
+    const long long r = n / b.nx;
+    const int j = (int)(r % b.ny), k = (int)(r / b.ny);
+    const int oi = (i < NG || i >= NG + b.idim), oj = (j < NG || j >= NG + b.jdim), ok = (k < NG || k >= NG + b.kdim);
+    const int nout = oi + oj + ok;
+    const long long P = b.P;
+    const long long id = n;
+    double c = 0.0, mu = 0.0;
+    if (mode & 1) {
+      if (nout <= 1) {
+        double rho = b.prim[id];
+        const double pp = b.prim[4 * P + id];
+        if (nout == 1) rho = fabs(rho);
+        const double a2 = p.gama * pp / rho;
+        c = sqrt(fabs(a2));
+        b.c[id] = c;
+        if (p.nvis == 1) b.prim[5 * P + id] = (p.moo * p.moo) * a2;
+      }
+      if (p.nvis == 1) {
+        double tm = b.prim[5 * P + id];
+        if (tm < 1e-6) tm = 1e-6;
+        mu = tm * sqrt(tm) * (1.0 + p.visc) / (tm + p.visc);
+        b.mu[id] = mu;
+      }
+      if (b.q && nout <= 1) {
+        const double rho = b.prim[id], u = b.prim[P + id], v = b.prim[2 * P + id], w = b.prim[3 * P + id];
+        const double pp = b.prim[4 * P + id];
+        const double rc = (nout == 1 || rho < 0.0) ? fabs(rho) : rho;
+        b.q[id] = rc;
+        b.q[P + id] = rc * u;
+        b.q[2 * P + id] = rc * v;
+        b.q[3 * P + id] = rc * w;
+        const double ke = 0.5 * rc * (u * u + v * v + w * w);
+        b.q[4 * P + id] = pp * (1.0 / (p.gama - 1.0)) + ke;
+      }
+      if (nout == 0) {
+#pragma unroll
+        for (int m = 0; m < 5; ++m) b.dq[m * P + id] = 0.0;
+      }
+    }
+    if ((mode & 2) && nout == 0) {
+      if (!(mode & 1)) {
+        c = b.c[id];
+        mu = b.mu[id];
+      }
+      const double u = b.prim[P + id], v = b.prim[2 * P + id], w = b.prim[3 * P + id];
+      const double rho = b.prim[id], vol = b.vol[id];
+      const double csrv = (double)p.csrv;
+      double sum = 0.0;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const double mx = b.met[(3 * d) * P + id], my = b.met[(3 * d + 1) * P + id], mz = b.met[(3 * d + 2) * P + id];
+        const double U = fabs(u * mx + v * my + w * mz);
+        const double gl = sqrt(mx * mx + my * my + mz * mz);
+        const double spec = U + c * gl;
+        double sv = 0.0;
+        if (p.nvis == 1) {
+          const double g2 = mx * mx + my * my + mz * mz;
+          const double denom = p.reynolds * rho * vol + 1.0e-30;
+          sv = (2.0 * mu / denom) * g2 * csrv;
+        }
+        sum += spec + sv;
+      }
+      const double dtp = p.cfl / (sum + 1.0e-30);
+      const double dtv = dtp / vol;
+      b.dt[id] = dtv;
+      const double back = dtv * vol;
+      mydt = back < mydt ? back : mydt;
+    }
+  }
+  if (!dtmin_bits) return;
+  smin[threadIdx.x] = mydt;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      const double o = smin[threadIdx.x + s];
+      if (o < smin[threadIdx.x]) smin[threadIdx.x] = o;
+    }
+    __syncthreads();
+  }
+  // positive doubles order like their bit patterns
+  if (threadIdx.x == 0 && smin[0] < 1000.0) atomicMin(dtmin_bits, (unsigned long long)__double_as_longlong(smin[0]));
+}
+
+__global__ void __launch_bounds__(256) lu_prep_kernel(const BlockDev* __restrict__ blocks,
+                                                      const long long* __restrict__ off, int nb) {
+  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= off[nb]) return;
+  const int bi = find_block(off, nb, g);
+  const BlockDev& b = blocks[bi];
+  int i, j, k;
+  decode_interior(b, g - off[bi], &i, &j, &k);
+  const long long id = pidx(b, i, j, k);
+  const bool fz = b.frozen[id] != 0;
+#pragma unroll
+  for (int m = 0; m < 5; ++m) b.rhs[m * b.P + id] = fz ? 0.0 : b.dq[m * b.P + id];
+}
+
+__global__ void __launch_bounds__(256) update_kernel(const BlockDev* __restrict__ blocks,
+                                                     const long long* __restrict__ off, int nb, double gama) {
+  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= off[nb]) return;
+  const int bi = find_block(off, nb, g);
+  const BlockDev& b = blocks[bi];
+  int i, j, k;
+  decode_interior(b, g - off[bi], &i, &j, &k);
+  const long long id = pidx(b, i, j, k);
+  const long long P = b.P;
+  double d[5];
+  if (b.frozen[id]) {
+#pragma unroll
+    for (int m = 0; m < 5; ++m) {
+      d[m] = 0.0;
+      b.dq[m * P + id] = 0.0;
+    }
+  } else {
+#pragma unroll
+    for (int m = 0; m < 5; ++m) d[m] = b.dq[m * P + id];
+  }
+  const double gm1 = gama - 1.0;
+  const double r = b.prim[id], u = b.prim[P + id], v = b.prim[2 * P + id], w = b.prim[3 * P + id];
+  const double pp = b.prim[4 * P + id];
+  const double E = pp / gm1 + 0.5 * r * (u * u + v * v + w * w);
+  const double q0 = r + d[0], q1 = r * u + d[1], q2 = r * v + d[2], q3 = r * w + d[3], q4 = E + d[4];
+  double rn = q0;
+  double ir = 1.0 / rn;
+  const double a = q1 * ir, bb = q2 * ir, cc = q3 * ir;
+  const double pn = gm1 * (q4 - 0.5 * rn * (a * a + bb * bb + cc * cc));
+  const double dpp = fabs((pn - pp) / pp);
+  const double x = -0.3 + dpp;
+  const double term = (0.0 < x) ? x : 0.0;  // std::max(0.0, x)
+  const double Fa = 1.0 / (1.0 + 2.0 * term);
+  const double f0 = r + d[0] * Fa, f1 = r * u + d[1] * Fa, f2 = r * v + d[2] * Fa, f3 = r * w + d[3] * Fa;
+  const double f4 = E + d[4] * Fa;
+  rn = f0;
+  ir = 1.0 / rn;
+  const double un = f1 * ir, vn = f2 * ir, wn = f3 * ir;
+  const double pf = gm1 * (f4 - 0.5 * rn * (un * un + vn * vn + wn * wn));
+  if (!(rn <= 1e-6 || pf <= 1e-6)) {
+    b.prim[id] = rn;
+    b.prim[P + id] = un;
+    b.prim[2 * P + id] = vn;
+    b.prim[3 * P + id] = wn;
+    b.prim[4 * P + id] = pf;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// residual
+// ---------------------------------------------------------------------------
+struct MaxLoc {
+  double v;
+  long long key;
+};
+__device__ inline bool better(const MaxLoc& a, const MaxLoc& b) {
+  return a.v > b.v || (a.v == b.v && a.key < b.key);
+}
+
+constexpr int RES_TB = 256;
+constexpr int RES_MAXPARTS = 1024;
+
+int residual_parts(long long n) {
+  long long w = (n + RES_TB - 1) / RES_TB;
+  if (w < 1) w = 1;
+  return (int)(w < RES_MAXPARTS ? w : RES_MAXPARTS);
+}
+
+__global__ void __launch_bounds__(RES_TB) residual_kernel(const BlockDev* __restrict__ blocks,
+                                                          const long long* __restrict__ off, int nb, double* partial,
+                                                          double* pmax, long long* pidx_) {
+  __shared__ double ssum[RES_TB];
+  __shared__ double smax[RES_TB];
+  __shared__ long long skey[RES_TB];
+  const long long total = off[nb];
+  const long long stride = (long long)gridDim.x * RES_TB;
+  double sum = 0.0;
+  MaxLoc best{-1.0, 0x7fffffffffffffffLL};
+  for (long long g = (long long)blockIdx.x * RES_TB + threadIdx.x; g < total; g += stride) {
+    const int bi = find_block(off, nb, g);
+    const BlockDev& b = blocks[bi];
+    int i, j, k;
+    decode_interior(b, g - off[bi], &i, &j, &k);
+    const long long id = pidx(b, i, j, k);
+    double vol = b.vol[id], dt = b.dt[id];
+    if (vol < 1e-30) vol = 1e-30;
+    if (dt < 1e-30) dt = 1e-30;
+    const double inv = 1.0 / (dt * vol);
+#pragma unroll
+    for (int m = 0; m < 5; ++m) {
+      const double r = fabs(b.dq[m * b.P + id]) * inv;
+      sum += r * r;
+      MaxLoc c{r, g * 5 + m};
+      if (better(c, best)) best = c;
+    }
+  }
+  ssum[threadIdx.x] = sum;
+  smax[threadIdx.x] = best.v;
+  skey[threadIdx.x] = best.key;
+  __syncthreads();
+  for (int s = RES_TB / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      ssum[threadIdx.x] = ssum[threadIdx.x] + ssum[threadIdx.x + s];
+      MaxLoc a{smax[threadIdx.x], skey[threadIdx.x]}, c{smax[threadIdx.x + s], skey[threadIdx.x + s]};
+      if (better(c, a)) {
+        smax[threadIdx.x] = c.v;
+        skey[threadIdx.x] = c.key;
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    partial[blockIdx.x] = ssum[0];
+    pmax[blockIdx.x] = smax[0];
+    pidx_[blockIdx.x] = skey[0];
+  }
+}
+
+__global__ void __launch_bounds__(RES_MAXPARTS) residual_final_kernel(const double* partial, const double* pmax,
+                                                                      const long long* pidx_, int nparts,
+                                                                      long long total5, double* out) {
+  __shared__ double ssum[RES_MAXPARTS];
+  __shared__ double smax[RES_MAXPARTS];
+  __shared__ long long skey[RES_MAXPARTS];
+  const int t = threadIdx.x;
+  ssum[t] = t < nparts ? partial[t] : 0.0;
+  smax[t] = t < nparts ? pmax[t] : -1.0;
+  skey[t] = t < nparts ? pidx_[t] : 0x7fffffffffffffffLL;
+  __syncthreads();
+  for (int s = RES_MAXPARTS / 2; s > 0; s >>= 1) {
+    if (t < s) {
+      ssum[t] = ssum[t] + ssum[t + s];
+      MaxLoc a{smax[t], skey[t]}, c{smax[t + s], skey[t + s]};
+      if (better(c, a)) {
+        smax[t] = c.v;
+        skey[t] = c.key;
+      }
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    out[0] = total5 > 0 ? sqrt(ssum[0] / (double)total5) : 0.0;
+    out[1] = smax[0];
+    out[2] = (double)skey[0];
+  }
+}
+
+__global__ void fill_kernel(double* p, long long n, double v) {
+  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g < n) p[g] = v;
+}
+
+// update_temperature, src/preprocess/InitialCondition.cpp:91-147
+__global__ void __launch_bounds__(256) temperature_kernel(const BlockDev* __restrict__ blocks,
+                                                          const long long* __restrict__ off, int nb, double gama,
+                                                          double moo) {
+  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= off[nb]) return;
+  const int bi = find_block(off, nb, g);
+  const BlockDev& b = blocks[bi];
+  const long long id = g - off[bi];
+  double rho = b.prim[id];
+  const double p = b.prim[4 * b.P + id];
+  if (rho < 1.0e-30) rho = 1.0e-30;
+  const double a2 = gama * p / rho;
+  b.prim[5 * b.P + id] = (moo * moo) * a2;
+}
+
+static inline unsigned grid_of(long long n, int tb) { return (unsigned)((n + tb - 1) / tb); }
+
+void launch_state(const Launch& L, long long n, const Params& p, int mode, unsigned long long* dtmin_bits) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(state_kernel, dim3(grid_of(n, 256)), dim3(256), 0, L.stream, L.blocks, L.off, L.nb, p, mode,
+                     dtmin_bits);
+}
+void launch_lu_prep(const Launch& L, long long n) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(lu_prep_kernel, dim3(grid_of(n, 256)), dim3(256), 0, L.stream, L.blocks, L.off, L.nb);
+}
+void launch_update(const Launch& L, long long n, double gamma) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(update_kernel, dim3(grid_of(n, 256)), dim3(256), 0, L.stream, L.blocks, L.off, L.nb, gamma);
+}
+void launch_residual(const Launch& L, long long n, long long total5, double* partial, long long* pidx_, int nparts,
+                     double* out) {
+  hipLaunchKernelGGL(residual_kernel, dim3(nparts), dim3(RES_TB), 0, L.stream, L.blocks, L.off, L.nb, partial,
+                     partial + RES_MAXPARTS, pidx_);
+  hipLaunchKernelGGL(residual_final_kernel, dim3(1), dim3(RES_MAXPARTS), 0, L.stream, partial, partial + RES_MAXPARTS,
+                     pidx_, nparts, total5, out);
+}
+void launch_fill(double* p, long long n, double v, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(fill_kernel, dim3(grid_of(n, 256)), dim3(256), 0, s, p, n, v);
+}
+void launch_temperature(const Launch& L, long long n, double gama, double moo) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(temperature_kernel, dim3(grid_of(n, 256)), dim3(256), 0, L.stream, L.blocks, L.off, L.nb, gama,
+                     moo);
+}
+
+}  // namespace orion_hip

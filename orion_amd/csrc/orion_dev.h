@@ -1,0 +1,130 @@
+// orion_dev.h -- device data model shared by the gfx950 kernels.
+//
+// Every block lives in one HBM arena as ghost-padded SoA fields with the
+// reference's layout (src/core/OrionArray.hpp:241-261): box
+// (idim+6, jdim+6, kdim+6), i fastest, variable slowest.  Kernels cover all
+// blocks in one launch: a thread's flat index is mapped to (block, local
+// index) through a prefix-offset table of the iteration space in use.
+//
+// All arithmetic is FP64 and every translation unit is compiled with
+// -ffp-contract=off, so a*b+c stays two rounded operations exactly as in
+// the reference's x86-64 build; f64 division and sqrt on gfx950 are the
+// correctly rounded sequences.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define ORION_NG 3
+
+namespace orion_hip {
+
+constexpr int NG = ORION_NG;
+
+// Fields of one block (device pointers into the arena).
+struct BlockDev {
+  int idim, jdim, kdim;  // interior sizes
+  int nx, ny, nz;        // padded sizes
+  long long P;           // nx*ny*nz
+  double* prim;          // [6][P] rho,u,v,w,p,T
+  double* met;           // [9][P]
+  double* vol;           // [P]
+  double* dq;            // [5][P]
+  double* rhs;           // [5][P] LU-SGS right-hand side copy
+  double* dt;            // [P]
+  double* c;             // [P]
+  double* mu;            // [P]
+  double* q;             // [5][P] conservative (write-only; may be null)
+  double* duvwt;         // [12][P] viscous cell derivatives (scratch)
+  double* fx;            // [3][5][P] face fluxes per direction (scratch)
+  double* fcov;          // closure overrides, [3][lines][6][5] (scratch)
+  unsigned char* frozen; // [P] 1 inside a physical-BC window
+  long long fcov_off[3]; // offset (doubles) of each direction's overrides
+};
+
+struct Params {
+  double gama, prl, moo, reynolds, visc, tref, twall, cfl;
+  double roo, uoo, voo, woo, poo, too;
+  int csrv, nvis, method, ntmst, gcl;
+};
+
+__host__ __device__ inline long long pidx(const BlockDev& b, int i, int j, int k) {
+  return (long long)i + (long long)b.nx * ((long long)j + (long long)b.ny * (long long)k);
+}
+
+// block lookup: largest b with off[b] <= g (off has nb+1 entries)
+__device__ inline int find_block(const long long* __restrict__ off, int nb, long long g) {
+  int lo = 0, hi = nb - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (off[mid] <= g) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Per-direction geometry helpers: stride of padded index along d and the
+// interior count along d.
+__host__ __device__ inline long long dstride(const BlockDev& b, int d) {
+  return d == 0 ? 1LL : (d == 1 ? (long long)b.nx : (long long)b.nx * b.ny);
+}
+__host__ __device__ inline int dcount(const BlockDev& b, int d) {
+  return d == 0 ? b.idim : (d == 1 ? b.jdim : b.kdim);
+}
+
+// Decode an index of the "faces along d" space (faces 0..n_d on every
+// interior line, i fastest in memory order) into the padded index of the
+// face's line start (position 0 along d) and the face number f.
+__device__ inline void decode_face(const BlockDev& b, int d, long long n, long long* base, int* f) {
+  if (d == 0) {
+    int nf = b.idim + 1;
+    *f = (int)(n % nf);
+    long long r = n / nf;
+    int j = (int)(r % b.jdim) + NG, k = (int)(r / b.jdim) + NG;
+    *base = pidx(b, 0, j, k);
+  } else if (d == 1) {
+    int i = (int)(n % b.idim) + NG;
+    long long r = n / b.idim;
+    int nf = b.jdim + 1;
+    *f = (int)(r % nf);
+    int k = (int)(r / nf) + NG;
+    *base = pidx(b, i, 0, k);
+  } else {
+    int i = (int)(n % b.idim) + NG;
+    long long r = n / b.idim;
+    int j = (int)(r % b.jdim) + NG;
+    *f = (int)(r / b.jdim);
+    *base = pidx(b, i, j, 0);
+  }
+}
+
+// Decode an index of the "lines along d" space (interior lines, i fastest).
+__device__ inline long long decode_line(const BlockDev& b, int d, long long n) {
+  if (d == 0) {
+    int j = (int)(n % b.jdim) + NG, k = (int)(n / b.jdim) + NG;
+    return pidx(b, 0, j, k);
+  } else if (d == 1) {
+    int i = (int)(n % b.idim) + NG, k = (int)(n / b.idim) + NG;
+    return pidx(b, i, 0, k);
+  }
+  int i = (int)(n % b.idim) + NG, j = (int)(n / b.idim) + NG;
+  return pidx(b, i, j, 0);
+}
+__host__ __device__ inline long long nlines(const BlockDev& b, int d) {
+  return d == 0 ? (long long)b.jdim * b.kdim : (d == 1 ? (long long)b.idim * b.kdim : (long long)b.idim * b.jdim);
+}
+// line number (within direction d) of an interior point
+__device__ inline long long line_id(const BlockDev& b, int d, int i, int j, int k) {
+  if (d == 0) return (long long)(j - NG) + (long long)b.jdim * (k - NG);
+  if (d == 1) return (long long)(i - NG) + (long long)b.idim * (k - NG);
+  return (long long)(i - NG) + (long long)b.idim * (j - NG);
+}
+
+// decode an interior point index into padded (i,j,k)
+__device__ inline void decode_interior(const BlockDev& b, long long n, int* i, int* j, int* k) {
+  *i = (int)(n % b.idim) + NG;
+  long long r = n / b.idim;
+  *j = (int)(r % b.jdim) + NG;
+  *k = (int)(r / b.jdim) + NG;
+}
+
+}  // namespace orion_hip

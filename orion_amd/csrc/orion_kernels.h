@@ -1,0 +1,69 @@
+// orion_kernels.h -- host-side launch wrappers for the gfx950 kernels.
+#pragma once
+#include "orion_dev.h"
+
+namespace orion_hip {
+
+// One multi-block launch: every kernel maps its flat thread index onto
+// blocks through `off` (device prefix table of the iteration space, nb+1).
+struct Launch {
+  hipStream_t stream;
+  const BlockDev* blocks;  // device
+  const long long* off;    // device
+  int nb;
+};
+
+// physical-BC work item: one region of one block (memory coordinates)
+struct BCTask {
+  int block, bctype;
+  int lo[3], hi[3];  // face window, padded 0-based inclusive
+  int s_nd, s_lr;    // 1-based normal axis, -1/0/+1 face side
+  int mid[3];        // window mid point (symmetry normal)
+  long long start;   // first flat index of this task in the launch
+};
+
+// interface averaging maps
+struct AvgSrc {
+  int block;
+  long long idx;
+};
+struct AvgDst {
+  int block;
+  int count;
+  long long idx;
+  long long start;  // first staging entry (entries are in region order)
+};
+
+// k_flux.hip
+void launch_inv_face(const Launch& L, int d, long long n, double gamma);
+void launch_inv_closure(const Launch& L, int d, long long n, double gamma);
+void launch_inv_diff(const Launch& L, long long n);
+void launch_visc_deriv(const Launch& L, long long n);
+void launch_visc_face(const Launch& L, int d, long long n, double cp_prl);
+void launch_visc_diff(const Launch& L, long long n, double re_inv);
+
+// k_point.hip
+void launch_state(const Launch& L, long long n, const Params& p, int mode, unsigned long long* dtmin_bits);
+void launch_lu_prep(const Launch& L, long long n);
+void launch_update(const Launch& L, long long n, double gamma);
+void launch_residual(const Launch& L, long long n, long long total5, double* partial, long long* pidx,
+                     int nparts, double* out);
+int residual_parts(long long n);
+void launch_fill(double* p, long long n, double v, hipStream_t s);
+void launch_temperature(const Launch& L, long long n, double gama, double moo);
+
+// k_bc.hip
+void launch_halo(const long long* pairs, long long n, double* arena, hipStream_t s);
+void launch_bc(const BlockDev* blocks, const BCTask* tasks, int ntasks, long long n, const Params& p,
+               hipStream_t s);
+void launch_avg_stage(const BlockDev* blocks, const AvgSrc* src, long long n, double* stage, hipStream_t s);
+void launch_avg_apply(const BlockDev* blocks, const AvgDst* dst, long long n, const double* stage, hipStream_t s);
+
+// k_lusgs.hip
+void launch_lusgs_plane(const Launch& L, long long n, int plane, int type, const Params& p);
+
+// k_metrics.hip
+void launch_metrics(const Launch& L, long long n, int gcl, double* tmp, hipStream_t s);
+void launch_count_small_vol(const Launch& L, long long n, unsigned long long* count);
+
+}  // namespace orion_hip
