@@ -127,6 +127,15 @@ int orion_hip_residual(orion_hip_ctx* ctx, orion_residual_stats* out);
  * read back once, after the last step. */
 int orion_hip_run(orion_hip_ctx* ctx, int nsteps, orion_residual_stats* hist);
 
+/* Measurement helper with no counterpart in the reference: runs nsteps
+ * like orion_hip_run with HIP events between the kernels of each phase and
+ * returns, per phase, the summed device time (ms) and the number of kernel
+ * launches.  Phases, in order: halo, bc, state+dt, visc_deriv, visc_face,
+ * visc_diff, avg1, inv_face, inv_closure, inv_diff, avg2, lu_prep, lu_fwd,
+ * lu_bwd, gs_fwd, gs_bwd, update, residual. */
+#define ORION_PHASES 18
+int orion_hip_profile_run(orion_hip_ctx* ctx, int nsteps, double* phase_ms, long long* phase_launches);
+
 int orion_hip_synchronize(orion_hip_ctx* ctx);
 /* Use a caller-owned hipStream_t (NULL restores the context's own stream). */
 int orion_hip_set_stream(orion_hip_ctx* ctx, void* stream);

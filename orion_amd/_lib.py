@@ -37,6 +37,10 @@ class ResidualStats(C.Structure):
         return (self.rms_residual, self.max_residual, tuple(self.max_loc), self.phydtime)
 
 
+PHASES = ["halo", "bc", "state_dt", "visc_deriv", "visc_face", "visc_diff", "avg1", "inv_face", "inv_closure",
+          "inv_diff", "avg2", "lu_prep", "lu_fwd", "lu_bwd", "gs_fwd", "gs_bwd", "update", "residual"]
+
+
 class OrionHipError(RuntimeError):
     pass
 
@@ -79,6 +83,7 @@ def load():
         "orion_hip_update_solution": ([P], C.c_int),
         "orion_hip_residual": ([P, C.POINTER(ResidualStats)], C.c_int),
         "orion_hip_run": ([P, C.c_int, C.POINTER(ResidualStats)], C.c_int),
+        "orion_hip_profile_run": ([P, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_longlong)], C.c_int),
         "orion_hip_synchronize": ([P], C.c_int),
         "orion_hip_set_stream": ([P, C.c_void_p], C.c_int),
         "orion_hip_device_bytes": ([P], C.c_size_t),

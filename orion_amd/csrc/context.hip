@@ -61,6 +61,8 @@ struct orion_hip_ctx {
   size_t map_bytes = 0;
   Space pad, intr, face[3], line[3], plane;
   int max_planes = 0;
+  LuPacked lu{};
+  std::vector<long long> plane_start;  // max_planes+1
   long long* d_halo = nullptr;
   long long n_halo = 0;
   BCTask* d_bc = nullptr;
@@ -234,7 +236,7 @@ static long long arena_fields(const BlockDev& b, int keep_q, long long lines_tot
   auto al = [](long long x) { return (x + 31) / 32 * 32; };  // 256-byte alignment
   n += al(6 * P) + al(9 * P) + al(P) + al(5 * P) + al(5 * P) + al(P) + al(P) + al(P);
   if (keep_q) n += al(5 * P);
-  n += al(12 * P) + al(15 * P) + al(30 * lines_total) + al((P + 7) / 8);
+  n += al(12 * P) + al(15 * P) + al(30 * lines_total) + al((P + 1) / 2) + al((P + 7) / 8);
   return n;
 }
 
@@ -249,7 +251,10 @@ static int build_topology(orion_hip_ctx* c, const std::vector<std::vector<Reg>>&
   for (int b = 0; b < nb; ++b)
     for (int r = 0; r < (int)R[b].size(); ++r)
       if (R[b][r].d.nbt > 0) msgs.push_back({R[b][r].d.nbt * 1000 + R[b][r].d.ibcwin, b, r, false});
-  std::map<long long, std::pair<long long, long long>> last;  // dst -> (order, src)
+  struct HP {
+    long long dst, order, src;
+  };
+  std::vector<HP> hp;
   long long order = 0;
   auto arena_off = [&](int b, int m, long long id) {
     return (long long)((c->hb[b].prim - (double*)c->arena) + (long long)m * c->hb[b].P + id);
@@ -282,18 +287,22 @@ static int build_topology(orion_hip_ctx* c, const std::vector<std::vector<Reg>>&
               const int sj = sb[1] + (int)((sl / sn[0]) % sn[1]);
               const int sk = sb[2] + (int)(sl / ((long long)sn[0] * sn[1]));
               ++ptr;
-              const long long dst = arena_off(b, m, pidx(c->hb[b], i, j, k));
-              const long long src = arena_off(M->block, (int)sm, pidx(A, si, sj, sk));
-              last[dst] = {order++, src};
+              hp.push_back({arena_off(b, m, pidx(c->hb[b], i, j, k)), order++,
+                            arena_off(M->block, (int)sm, pidx(A, si, sj, sk))});
             }
     }
+  // last writer wins: keep the highest order per destination
+  std::sort(hp.begin(), hp.end(), [](const HP& x, const HP& y) {
+    return x.dst < y.dst || (x.dst == y.dst && x.order < y.order);
+  });
   std::vector<long long> pairs;
-  pairs.reserve(2 * last.size());
-  for (auto& kv : last) {
-    pairs.push_back(kv.first);
-    pairs.push_back(kv.second.second);
+  pairs.reserve(2 * hp.size());
+  for (size_t q = 0; q < hp.size(); ++q) {
+    if (q + 1 < hp.size() && hp[q + 1].dst == hp[q].dst) continue;
+    pairs.push_back(hp[q].dst);
+    pairs.push_back(hp[q].src);
   }
-  c->n_halo = (long long)last.size();
+  c->n_halo = (long long)(pairs.size() / 2);
   if (upload_vec(c, &c->d_halo, pairs)) return -1;
 
   // ---------------- physical BC tasks per rank ----------------
@@ -424,6 +433,60 @@ static int build_topology(orion_hip_ctx* c, const std::vector<std::vector<Reg>>&
   return 0;
 }
 
+// Plane-major numbering of interior cells for the LU-SGS sweeps: plane s of
+// every block in block order, within a block k then j (i = s - j - k).
+static int build_planes(orion_hip_ctx* c) {
+  const int nb = c->nb;
+  long long ncell = c->intr.total();
+  std::vector<std::vector<int>> pos(nb);
+  for (int b = 0; b < nb; ++b) pos[b].assign(c->hb[b].P, -1);
+  c->plane_start.assign(c->max_planes + 1, 0);
+  long long q = 0;
+  for (int s = 0; s < c->max_planes; ++s) {
+    c->plane_start[s] = q;
+    for (int b = 0; b < nb; ++b) {
+      const BlockDev& B = c->hb[b];
+      const int ni = B.idim, nj = B.jdim, nk = B.kdim;
+      const int k0 = std::max(0, s - (ni - 1) - (nj - 1)), k1 = std::min(nk - 1, s);
+      for (int k = k0; k <= k1; ++k) {
+        const int j0 = std::max(0, s - k - (ni - 1)), j1 = std::min(nj - 1, s - k);
+        for (int j = j0; j <= j1; ++j) {
+          const int i = s - j - k;
+          pos[b][pidx(B, i + NG, j + NG, k + NG)] = (int)q++;
+        }
+      }
+    }
+  }
+  c->plane_start[c->max_planes] = q;
+  if (q != ncell) return fail("plane numbering covers %lld of %lld cells", q, ncell);
+  if (ncell >= (1LL << 31)) return fail("too many cells for 32-bit plane indices");
+  std::vector<int> nbr(6 * ncell, -1);
+  for (int b = 0; b < nb; ++b) {
+    const BlockDev& B = c->hb[b];
+    const long long st[3] = {1, B.nx, (long long)B.nx * B.ny};
+    for (int k = NG; k < NG + B.kdim; ++k)
+      for (int j = NG; j < NG + B.jdim; ++j)
+        for (int i = NG; i < NG + B.idim; ++i) {
+          const long long id = pidx(B, i, j, k);
+          const int me = pos[b][id];
+          const int cc[3] = {i, j, k}, lim[3] = {B.idim, B.jdim, B.kdim};
+          for (int ax = 0; ax < 3; ++ax) {
+            if (cc[ax] - NG > 0) nbr[(long long)ax * ncell + me] = pos[b][id - st[ax]];
+            if (cc[ax] - NG + 1 < lim[ax]) nbr[(long long)(3 + ax) * ncell + me] = pos[b][id + st[ax]];
+          }
+        }
+    HIPCHK(hipMemcpy(B.lpos, pos[b].data(), sizeof(int) * B.P, hipMemcpyHostToDevice));
+  }
+  c->lu.ncell = ncell;
+  if (dalloc(c, (void**)&c->lu.coef, sizeof(double) * 40 * ncell) ||
+      dalloc(c, (void**)&c->lu.rhs, sizeof(double) * 5 * ncell) ||
+      dalloc(c, (void**)&c->lu.src, sizeof(double) * 6 * ncell) ||
+      dalloc(c, (void**)&c->lu.dq, sizeof(double) * 5 * ncell))
+    return -1;
+  if (upload_vec(c, &c->lu.nbr, nbr)) return -1;
+  return 0;
+}
+
 // ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
@@ -516,6 +579,7 @@ int orion_hip_ctx_create(int nblocks, const int* dims, const double* xyz, int nr
       B.fcov_off[0] = 0;
       B.fcov_off[1] = 30 * nlines(B, 0);
       B.fcov_off[2] = 30 * (nlines(B, 0) + nlines(B, 1));
+      B.lpos = (int*)a; a += al((P + 1) / 2);
       B.frozen = (unsigned char*)a; a += al((P + 7) / 8);
     }
   }
@@ -591,6 +655,7 @@ int orion_hip_ctx_create(int nblocks, const int* dims, const double* xyz, int nr
     if (make_space(c, c->face[d], cf[d]) || make_space(c, c->line[d], cl[d])) return bail(-1);
   if (upload_vec(c, &c->db, c->hb)) return bail(-1);
   if (build_topology(c, R)) return bail(-1);
+  if (build_planes(c)) return bail(-1);
   c->nparts = residual_parts(c->intr.total());
   if (dalloc(c, (void**)&c->d_part, sizeof(double) * 2 * 1024) || dalloc(c, (void**)&c->d_pidx, sizeof(long long) * 1024) ||
       dalloc(c, (void**)&c->d_count, sizeof(unsigned long long) * 2))
@@ -722,15 +787,19 @@ static void op_inv(orion_hip_ctx* c) {
   launch_inv_diff(LA(c, c->intr), c->intr.total());
 }
 
+static void op_sweep(orion_hip_ctx* c, int type) {
+  const std::vector<long long>& S = c->plane_start;
+  if (type == 0 || type == 2)
+    for (int s = 0; s < c->max_planes; ++s) launch_lusgs_sweep(c->lu, S[s], S[s + 1] - S[s], type, c->p, c->stream);
+  else
+    for (int s = c->max_planes - 1; s >= 0; --s)
+      launch_lusgs_sweep(c->lu, S[s], S[s + 1] - S[s], type, c->p, c->stream);
+}
+
 static void op_update(orion_hip_ctx* c) {
-  launch_lu_prep(LA(c, c->intr), c->intr.total());
-  const Launch lp = LA(c, c->plane);
-  const long long np = c->plane.total();
-  for (int s = 0; s < c->max_planes; ++s) launch_lusgs_plane(lp, np, s, 0, c->p);
-  for (int s = c->max_planes - 1; s >= 0; --s) launch_lusgs_plane(lp, np, s, 1, c->p);
-  for (int s = 0; s < c->max_planes; ++s) launch_lusgs_plane(lp, np, s, 2, c->p);
-  for (int s = c->max_planes - 1; s >= 0; --s) launch_lusgs_plane(lp, np, s, 3, c->p);
-  launch_update(LA(c, c->intr), c->intr.total(), c->p.gama);
+  launch_lu_coef(LA(c, c->intr), c->intr.total(), c->lu, c->p);
+  for (int t = 0; t < 4; ++t) op_sweep(c, t);
+  launch_update(LA(c, c->intr), c->intr.total(), c->lu, c->p.gama);
 }
 
 static int ensure_res(orion_hip_ctx* c, int n) {
@@ -851,6 +920,65 @@ int orion_hip_run(orion_hip_ctx* c, int nsteps, orion_residual_stats* hist) {
     HIPCHK(hipMemcpyAsync(bits.data(), c->d_dtmin, sizeof(unsigned long long) * nsteps, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     for (int s = 0; s < nsteps; ++s) decode_stats(c, &r[4 * s], bits[s], &hist[s]);
+  }
+  return 0;
+}
+
+int orion_hip_profile_run(orion_hip_ctx* c, int nsteps, double* phase_ms, long long* phase_launches) {
+  if (nsteps <= 0) return 0;
+  if (ensure_res(c, nsteps)) return -1;
+  const int NP = ORION_PHASES;
+  hipEvent_t ev[ORION_PHASES + 1];
+  for (int i = 0; i <= NP; ++i) HIPCHK(hipEventCreate(&ev[i]));
+  std::vector<double> acc(NP, 0.0);
+  std::vector<long long> nl(NP, 0);
+  const long long planes = c->max_planes;
+  for (int s = 0; s < nsteps; ++s) {
+    int k = 0;
+    auto mark = [&](long long launches) {
+      hipEventRecord(ev[k + 1], c->stream);
+      nl[k] += launches;
+      ++k;
+    };
+    hipEventRecord(ev[0], c->stream);
+    op_halo(c); mark(1);
+    op_bc(c); mark((long long)c->bc_first.size());
+    launch_state(LA(c, c->pad), c->pad.total(), c->p, 3, c->d_dtmin + s); mark(1);
+    if (c->p.nvis == 1) {
+      const double cp = 1.0 / ((c->p.gama - 1.0) * c->p.moo * c->p.moo);
+      const double re = c->p.reynolds;
+      launch_visc_deriv(LA(c, c->intr), c->intr.total()); mark(1);
+      for (int d = 0; d < 3; ++d) launch_visc_face(LA(c, c->face[d]), d, c->face[d].total(), cp / c->p.prl);
+      mark(3);
+      launch_visc_diff(LA(c, c->intr), c->intr.total(), (std::fabs(re) > 1.0e-30) ? 1.0 / re : 0.0); mark(1);
+      op_avg(c); mark(2);
+    } else {
+      mark(0); mark(0); mark(0); mark(0);
+    }
+    for (int d = 0; d < 3; ++d) launch_inv_face(LA(c, c->face[d]), d, c->face[d].total(), c->p.gama);
+    mark(3);
+    for (int d = 0; d < 3; ++d) launch_inv_closure(LA(c, c->line[d]), d, c->line[d].total(), c->p.gama);
+    mark(3);
+    launch_inv_diff(LA(c, c->intr), c->intr.total()); mark(1);
+    op_avg(c); mark(2);
+    launch_lu_coef(LA(c, c->intr), c->intr.total(), c->lu, c->p); mark(1);
+    for (int t = 0; t < 4; ++t) { op_sweep(c, t); mark(planes); }
+    launch_update(LA(c, c->intr), c->intr.total(), c->lu, c->p.gama); mark(1);
+    launch_residual(LA(c, c->intr), c->intr.total(), 5 * c->intr.total(), c->d_part, c->d_pidx, c->nparts,
+                    c->d_res + 4 * s);
+    mark(2);
+    if (check_launch("profile_run")) return -1;
+    HIPCHK(hipEventSynchronize(ev[k]));
+    for (int i = 0; i < k; ++i) {
+      float ms = 0.f;
+      HIPCHK(hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
+      acc[i] += ms;
+    }
+  }
+  for (int i = 0; i <= NP; ++i) hipEventDestroy(ev[i]);
+  for (int i = 0; i < NP; ++i) {
+    if (phase_ms) phase_ms[i] = acc[i];
+    if (phase_launches) phase_launches[i] = nl[i];
   }
   return 0;
 }

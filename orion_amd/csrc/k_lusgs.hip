@@ -1,6 +1,6 @@
-// k_lusgs.hip -- LU-SGS sweeps for gfx950 (FP64).
+// k_lusgs.hip -- LU-SGS update for gfx950 (FP64).
 //
-// StateUpdater::update_solution (src/solver/StateUpdater.cpp:297-445) sweeps
+// StateUpdater::update_solution (src/solver/StateUpdater.cpp:297-580) sweeps
 // each block lexicographically (k, then j, then i; reversed for the backward
 // sweeps) and every cell reads only its six face neighbours.  On the
 // hyperplane s = i+j+k, lower neighbours lie on s-1 and upper ones on s+1,
@@ -8,10 +8,23 @@
 // plane in parallel reads exactly the same neighbour values as the
 // lexicographic sweep: the results are identical, not an approximation.
 //
-// Sweep types: 0 LU forward, 1 LU backward, 2 GS forward, 3 GS backward.
-// Per-cell arithmetic follows calc_spectral_radius (:175-191),
-// calc_visc_spectral_radius (:194-207), compute_flux_splitting (:209-265),
-// get_diag_coeff (:267-295) and the sweep bodies (:318-365, :392-441).
+// Layout: the sweeps work on a plane-major copy of their data.  Cells are
+// numbered plane by plane over all blocks (plane s of block 0, then of block
+// 1, ...; within a block by k then j), so the cells of one plane are
+// contiguous and a plane's neighbours on s-1 and s+1 are contiguous too.
+//
+// Split form: everything in a cell's update that does not depend on the
+// evolving dq -- the diagonal 1/(1/dt + sum of radii), the normalised own
+// metrics, and for each of the six neighbour slots the split eigenvalue
+// coefficients l1, x1, x2, ct_n and the viscous 0.5*r_v, plus each cell's
+// own c^2, H and a_f used when it is a neighbour -- is computed once per
+// update by lu_coef (fully parallel, natural order) with the reference's
+// expressions (calc_spectral_radius :175-191, calc_visc_spectral_radius
+// :194-207, compute_flux_splitting :209-254, get_diag_coeff :267-295).
+// The sweeps evaluate only the dq-linear remainder of
+// compute_flux_splitting (:255-264) and the sweep bodies (:318-365,
+// :392-441) with the same expressions, so results are bit-identical to
+// recomputing everything in the sweep.
 #include "orion_dev.h"
 #include "orion_kernels.h"
 
@@ -37,152 +50,233 @@ __device__ inline double visc_rad(double rho, double mu, double vol, double re, 
   return coef * g2;
 }
 
-__device__ inline void flux_split(const double* pr, const double* dq, double nx, double ny, double nz, double sr,
-                                  int sign, double gamma, double* res) {
-  double rm = pr[0], pm = pr[4];
-  const double um = pr[1], vm = pr[2], wm = pr[3];
-  const double drm = dq[0], dum = dq[1], dvm = dq[2], dwm = dq[3], dEm = dq[4];
-  const double nt = 0.0;
-  const double v2 = um * um + vm * vm + wm * wm;
-  if (rm < 1e-30) rm = 1e-30;
-  if (pm < 1e-30) pm = 1e-30;
-  const double c2 = gamma * pm / rm;
-  const double cm = sqrt(c2);
-  const double H = (gamma / (gamma - 1.0)) * (pm / rm) + 0.5 * v2;
-  const double ct = nx * um + ny * vm + nz * wm + nt;
-  double gm = sqrt(nx * nx + ny * ny + nz * nz);
-  if (gm < 1.0e-30) gm = 1.0e-30;
-  const double ssr = (double)sign * sr;
-  const double l1 = 0.5 * (ct + ssr);
-  const double l4 = 0.5 * ((ct + cm * gm) + ssr);
-  const double l5 = 0.5 * ((ct - cm * gm) + ssr);
-  const double x1 = (2.0 * l1 - l4 - l5) / (2.0 * c2);
-  const double x2 = (l4 - l5) / (2.0 * cm);
-  const double im = 1.0 / gm;
-  const double nxn = nx * im, nyn = ny * im, nzn = nz * im, ctn = (ct - nt) * im;
-  const double gm1 = gamma - 1.0;
-  const double af = 0.5 * gm1 * v2;
-  const double dc = ctn * drm - (nxn * dum + nyn * dvm + nzn * dwm);
-  const double dh = af * drm - gm1 * (um * dum + vm * dvm + wm * dwm - dEm);
-  const double c2dc = c2 * dc;
-  res[0] = l1 * drm - dh * x1 - dc * x2;
-  res[1] = l1 * dum + (nxn * c2dc - um * dh) * x1 + (nxn * dh - um * dc) * x2;
-  res[2] = l1 * dvm + (nyn * c2dc - vm * dh) * x1 + (nyn * dh - vm * dc) * x2;
-  res[3] = l1 * dwm + (nzn * c2dc - wm * dh) * x1 + (nzn * dh - wm * dc) * x2;
-  res[4] = l1 * dEm + (ctn * c2dc - H * dh) * x1 + (ctn * dh - H * dc) * x2;
-}
+// packed coefficient fields ([field][ncell])
+//   0 coed | 1..9 nx_n,ny_n,nz_n per axis | 10+5*slot+{0..4} l1,x1,x2,ct_n,hv
+// source fields: 0..2 u,v,w | 3 c2 | 4 H | 5 af
+// slots: 0 i-1, 1 j-1, 2 k-1, 3 i+1, 4 j+1, 5 k+1
+constexpr int LC_COED = 0, LC_NRM = 1, LC_SLOT = 10;
 
-__global__ void __launch_bounds__(256) lusgs_plane_kernel(const BlockDev* __restrict__ blocks,
-                                                          const long long* __restrict__ off, int nb, int plane,
-                                                          int type, Params p) {
+__global__ void __launch_bounds__(256) lu_coef_kernel(const BlockDev* __restrict__ blocks,
+                                                      const long long* __restrict__ off, int nb, LuPacked L,
+                                                      Params p) {
   const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= off[nb]) return;
   const int bi = find_block(off, nb, g);
   const BlockDev& b = blocks[bi];
-  const long long n = g - off[bi];
-  const int jj = (int)(n % b.jdim), kk = (int)(n / b.jdim);
-  const int ii = plane - jj - kk;
-  if (ii < 0 || ii >= b.idim) return;
-  const int i = ii + NG, j = jj + NG, k = kk + NG;
+  int i, j, k;
+  decode_interior(b, g - off[bi], &i, &j, &k);
   const long long P = b.P;
   const long long id = pidx(b, i, j, k);
+  const long long N = L.ncell;
+  const long long q = b.lpos[id];
   const double gamma = p.gama, re = p.reynolds;
-  const int nvis = p.nvis;
-  // get_diag_coeff
+  // rhs copy with the physical-BC windows zeroed (StateUpdater.cpp:557-570)
+  const bool fz = b.frozen[id] != 0;
+#pragma unroll
+  for (int m = 0; m < 5; ++m) L.rhs[m * N + q] = fz ? 0.0 : b.dq[m * P + id];
   double pr[5];
 #pragma unroll
   for (int m = 0; m < 5; ++m) pr[m] = b.prim[m * P + id];
   double met[9];
 #pragma unroll
   for (int m = 0; m < 9; ++m) met[m] = b.met[m * P + id];
-  const double ra = spec_rad(pr, gamma, met[0], met[1], met[2]);
-  const double rb = spec_rad(pr, gamma, met[3], met[4], met[5]);
-  const double rc = spec_rad(pr, gamma, met[6], met[7], met[8]);
-  double rva = 0, rvb = 0, rvc = 0;
-  if (nvis == 1) {
-    const double mu = b.mu[id];
-    double vol = b.vol[id];
-    if (vol < 1e-30) vol = 1e-30;
-    rva = visc_rad(pr[0], mu, vol, re, met[0], met[1], met[2]);
-    rvb = visc_rad(pr[0], mu, vol, re, met[3], met[4], met[5]);
-    rvc = visc_rad(pr[0], mu, vol, re, met[6], met[7], met[8]);
-  }
-  const double rs = ra + rb + rc + rva + rvb + rvc;
-  const double term = (1.0 / b.dt[id]) + 1.0 * 1.0 * rs;
-  const double coed = 1.0 / term;
-
-  double r0[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-  const bool gs = type >= 2;
-  const int dir = (type == 0 || type == 2) ? 1 : -1;
-  // neighbour list: LU uses the three on the sweep's trailing side with
-  // sign=dir; GS uses lower (sign +1) then upper (sign -1).
-  const int nnb = gs ? 6 : 3;
-  for (int t = 0; t < nnb; ++t) {
-    int ax, sgn, step;
-    if (gs) {
-      ax = t % 3;
-      sgn = t < 3 ? 1 : -1;
-      step = t < 3 ? -1 : 1;
-    } else {
-      ax = t;
-      sgn = dir;
-      step = -dir;
+  {  // get_diag_coeff
+    const double ra = spec_rad(pr, gamma, met[0], met[1], met[2]);
+    const double rb = spec_rad(pr, gamma, met[3], met[4], met[5]);
+    const double rc = spec_rad(pr, gamma, met[6], met[7], met[8]);
+    double rva = 0, rvb = 0, rvc = 0;
+    if (p.nvis == 1) {
+      const double mu = b.mu[id];
+      double vol = b.vol[id];
+      if (vol < 1e-30) vol = 1e-30;
+      rva = visc_rad(pr[0], mu, vol, re, met[0], met[1], met[2]);
+      rvb = visc_rad(pr[0], mu, vol, re, met[3], met[4], met[5]);
+      rvc = visc_rad(pr[0], mu, vol, re, met[6], met[7], met[8]);
     }
+    const double rs = ra + rb + rc + rva + rvb + rvc;
+    const double term = (1.0 / b.dt[id]) + 1.0 * 1.0 * rs;
+    L.coef[LC_COED * N + q] = 1.0 / term;
+  }
+  {  // this cell as a neighbour
+    double rm = pr[0], pm = pr[4];
+    const double v2 = pr[1] * pr[1] + pr[2] * pr[2] + pr[3] * pr[3];
+    if (rm < 1e-30) rm = 1e-30;
+    if (pm < 1e-30) pm = 1e-30;
+    L.src[0 * N + q] = pr[1];
+    L.src[1 * N + q] = pr[2];
+    L.src[2 * N + q] = pr[3];
+    L.src[3 * N + q] = gamma * pm / rm;
+    L.src[4 * N + q] = (gamma / (gamma - 1.0)) * (pm / rm) + 0.5 * v2;
+    L.src[5 * N + q] = 0.5 * (gamma - 1.0) * v2;
+  }
+  double gmv[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const double nx = met[3 * a], ny = met[3 * a + 1], nz = met[3 * a + 2];
+    double gm = sqrt(nx * nx + ny * ny + nz * nz);
+    if (gm < 1.0e-30) gm = 1.0e-30;
+    gmv[a] = gm;
+    const double im = 1.0 / gm;
+    L.coef[(LC_NRM + 3 * a) * N + q] = nx * im;
+    L.coef[(LC_NRM + 3 * a + 1) * N + q] = ny * im;
+    L.coef[(LC_NRM + 3 * a + 2) * N + q] = nz * im;
+  }
+#pragma unroll
+  for (int sl = 0; sl < 6; ++sl) {
+    const int ax = sl % 3;
+    const int step = sl < 3 ? -1 : 1;
+    const int sgn = sl < 3 ? 1 : -1;
     int in = i, jn = j, kn = k;
     if (ax == 0) in += step;
     else if (ax == 1) jn += step;
     else kn += step;
-    if (in < NG || in >= b.nx - NG || jn < NG || jn >= b.ny - NG || kn < NG || kn >= b.nz - NG) continue;
-    const long long idn = pidx(b, in, jn, kn);
-    double pn[5], dn[5], fs[5];
+    double* S = L.coef + (long long)(LC_SLOT + 5 * sl) * N + q;
+    if (in < NG || in >= b.nx - NG || jn < NG || jn >= b.ny - NG || kn < NG || kn >= b.nz - NG) {
 #pragma unroll
-    for (int m = 0; m < 5; ++m) {
-      pn[m] = b.prim[m * P + idn];
-      dn[m] = b.dq[m * P + idn];
+      for (int t = 0; t < 5; ++t) S[t * N] = 0.0;
+      continue;
     }
+    const long long idn = pidx(b, in, jn, kn);
+    double pn[5];
+#pragma unroll
+    for (int m = 0; m < 5; ++m) pn[m] = b.prim[m * P + idn];
     const double nx = met[3 * ax], ny = met[3 * ax + 1], nz = met[3 * ax + 2];
     const double r = spec_rad(pn, gamma, nx, ny, nz);
-    flux_split(pn, dn, nx, ny, nz, r, sgn, gamma, fs);
-    if (nvis == 1) {
+    double rm = pn[0], pm = pn[4];
+    const double um = pn[1], vm = pn[2], wm = pn[3];
+    if (rm < 1e-30) rm = 1e-30;
+    if (pm < 1e-30) pm = 1e-30;
+    const double c2 = gamma * pm / rm;
+    const double cm = sqrt(c2);
+    const double nt = 0.0;
+    const double ct = nx * um + ny * vm + nz * wm + nt;
+    const double gm = gmv[ax];
+    const double ssr = (double)sgn * r;
+    const double l1 = 0.5 * (ct + ssr);
+    const double l4 = 0.5 * ((ct + cm * gm) + ssr);
+    const double l5 = 0.5 * ((ct - cm * gm) + ssr);
+    S[0] = l1;
+    S[N] = (2.0 * l1 - l4 - l5) / (2.0 * c2);
+    S[2 * N] = (l4 - l5) / (2.0 * cm);
+    S[3 * N] = (ct - nt) * (1.0 / gm);
+    double hv = 0.0;
+    if (p.nvis == 1) {
       const double mun = b.mu[idn];
       double voln = b.vol[idn];
       if (voln < 1e-30) voln = 1e-30;
-      const double rv = visc_rad(pn[0], mun, voln, re, nx, ny, nz);
-      if (gs) {
-#pragma unroll
-        for (int m = 0; m < 5; ++m) fs[m] += 0.5 * rv * dn[m] * (double)sgn;
-      } else {
-#pragma unroll
-        for (int m = 0; m < 5; ++m) fs[m] += 0.5 * rv * dn[m];
-      }
+      hv = 0.5 * visc_rad(pn[0], mun, voln, re, nx, ny, nz);
     }
-    if (gs && sgn == -1) {
+    S[4 * N] = hv;
+  }
+}
+
+// dq-linear part of one neighbour's contribution (compute_flux_splitting
+// :255-264 plus the viscous off-diagonal :341-343 / :419-421).
+template <bool GS, int SL>
+__device__ __forceinline__ void slot_contrib(const LuPacked& L, long long q, long long qn, double gm1, int nvis,
+                                             double* r0) {
+  const long long N = L.ncell;
+  constexpr int ax = SL % 3;
+  constexpr int sgn = SL < 3 ? 1 : -1;
+  const double* C = L.coef;
+  const double l1 = C[(LC_SLOT + 5 * SL) * N + q], x1 = C[(LC_SLOT + 5 * SL + 1) * N + q];
+  const double x2 = C[(LC_SLOT + 5 * SL + 2) * N + q], ctn = C[(LC_SLOT + 5 * SL + 3) * N + q];
+  const double nxn = C[(LC_NRM + 3 * ax) * N + q], nyn = C[(LC_NRM + 3 * ax + 1) * N + q];
+  const double nzn = C[(LC_NRM + 3 * ax + 2) * N + q];
+  const double um = L.src[qn], vm = L.src[N + qn], wm = L.src[2 * N + qn];
+  const double c2 = L.src[3 * N + qn], H = L.src[4 * N + qn], af = L.src[5 * N + qn];
+  const double drm = L.dq[qn], dum = L.dq[N + qn], dvm = L.dq[2 * N + qn], dwm = L.dq[3 * N + qn];
+  const double dEm = L.dq[4 * N + qn];
+  const double dc = ctn * drm - (nxn * dum + nyn * dvm + nzn * dwm);
+  const double dh = af * drm - gm1 * (um * dum + vm * dvm + wm * dwm - dEm);
+  const double c2dc = c2 * dc;
+  double res[5];
+  res[0] = l1 * drm - dh * x1 - dc * x2;
+  res[1] = l1 * dum + (nxn * c2dc - um * dh) * x1 + (nxn * dh - um * dc) * x2;
+  res[2] = l1 * dvm + (nyn * c2dc - vm * dh) * x1 + (nyn * dh - vm * dc) * x2;
+  res[3] = l1 * dwm + (nzn * c2dc - wm * dh) * x1 + (nzn * dh - wm * dc) * x2;
+  res[4] = l1 * dEm + (ctn * c2dc - H * dh) * x1 + (ctn * dh - H * dc) * x2;
+  if (nvis == 1) {
+    const double hv = C[(LC_SLOT + 5 * SL + 4) * N + q];
+    const double dn[5] = {drm, dum, dvm, dwm, dEm};
+    if (GS) {
 #pragma unroll
-      for (int m = 0; m < 5; ++m) r0[m] -= fs[m];
+      for (int m = 0; m < 5; ++m) res[m] += hv * dn[m] * (double)sgn;
     } else {
 #pragma unroll
-      for (int m = 0; m < 5; ++m) r0[m] += fs[m];
+      for (int m = 0; m < 5; ++m) res[m] += hv * dn[m];
     }
   }
-  const double wmig = 1.0;
-  if (type == 0) {
+  if (GS && sgn == -1) {
 #pragma unroll
-    for (int m = 0; m < 5; ++m) b.dq[m * P + id] = (-b.rhs[m * P + id] + wmig * r0[m]) * coed;
-  } else if (type == 1) {
-#pragma unroll
-    for (int m = 0; m < 5; ++m) b.dq[m * P + id] = b.dq[m * P + id] - (wmig * r0[m]) * coed;
+    for (int m = 0; m < 5; ++m) r0[m] -= res[m];
   } else {
 #pragma unroll
-    for (int m = 0; m < 5; ++m) b.dq[m * P + id] = (wmig * r0[m] + -b.rhs[m * P + id]) * coed;
+    for (int m = 0; m < 5; ++m) r0[m] += res[m];
+  }
+}
+
+// Sweep types: 0 LU forward, 1 LU backward, 2 GS forward, 3 GS backward.
+// One launch per plane: the plane's cells are L[start, start+count).
+template <int TYPE>
+__global__ void __launch_bounds__(256) lusgs_sweep_kernel(LuPacked L, long long start, long long count, double gm1,
+                                                          int nvis) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= count) return;
+  const long long q = start + t;
+  const long long N = L.ncell;
+  double r0[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  constexpr bool GS = TYPE >= 2;
+  const int* nb6 = L.nbr;
+  if (TYPE == 1) {
+    const int a = nb6[3 * N + q], c = nb6[4 * N + q], d = nb6[5 * N + q];
+    if (a >= 0) slot_contrib<false, 3>(L, q, a, gm1, nvis, r0);
+    if (c >= 0) slot_contrib<false, 4>(L, q, c, gm1, nvis, r0);
+    if (d >= 0) slot_contrib<false, 5>(L, q, d, gm1, nvis, r0);
+  } else {
+    const int a = nb6[q], c = nb6[N + q], d = nb6[2 * N + q];
+    if (a >= 0) slot_contrib<GS, 0>(L, q, a, gm1, nvis, r0);
+    if (c >= 0) slot_contrib<GS, 1>(L, q, c, gm1, nvis, r0);
+    if (d >= 0) slot_contrib<GS, 2>(L, q, d, gm1, nvis, r0);
+    if (GS) {
+      const int e = nb6[3 * N + q], f = nb6[4 * N + q], h = nb6[5 * N + q];
+      if (e >= 0) slot_contrib<true, 3>(L, q, e, gm1, nvis, r0);
+      if (f >= 0) slot_contrib<true, 4>(L, q, f, gm1, nvis, r0);
+      if (h >= 0) slot_contrib<true, 5>(L, q, h, gm1, nvis, r0);
+    }
+  }
+  const double coed = L.coef[LC_COED * N + q];
+  const double wmig = 1.0;
+  if (TYPE == 0) {
+#pragma unroll
+    for (int m = 0; m < 5; ++m) L.dq[m * N + q] = (-L.rhs[m * N + q] + wmig * r0[m]) * coed;
+  } else if (TYPE == 1) {
+#pragma unroll
+    for (int m = 0; m < 5; ++m) L.dq[m * N + q] = L.dq[m * N + q] - (wmig * r0[m]) * coed;
+  } else {
+#pragma unroll
+    for (int m = 0; m < 5; ++m) L.dq[m * N + q] = (wmig * r0[m] + -L.rhs[m * N + q]) * coed;
   }
 }
 
 static inline unsigned grid_of(long long n, int tb) { return (unsigned)((n + tb - 1) / tb); }
 
-void launch_lusgs_plane(const Launch& L, long long n, int plane, int type, const Params& p) {
+void launch_lu_coef(const Launch& La, long long n, const LuPacked& L, const Params& p) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(lusgs_plane_kernel, dim3(grid_of(n, 256)), dim3(256), 0, L.stream, L.blocks, L.off, L.nb, plane,
-                     type, p);
+  hipLaunchKernelGGL(lu_coef_kernel, dim3(grid_of(n, 256)), dim3(256), 0, La.stream, La.blocks, La.off, La.nb, L, p);
+}
+
+void launch_lusgs_sweep(const LuPacked& L, long long start, long long count, int type, const Params& p,
+                        hipStream_t s) {
+  if (count <= 0) return;
+  const double gm1 = p.gama - 1.0;
+  const dim3 gr(grid_of(count, 256)), tb(256);
+  switch (type) {
+    case 0: hipLaunchKernelGGL(lusgs_sweep_kernel<0>, gr, tb, 0, s, L, start, count, gm1, p.nvis); break;
+    case 1: hipLaunchKernelGGL(lusgs_sweep_kernel<1>, gr, tb, 0, s, L, start, count, gm1, p.nvis); break;
+    case 2: hipLaunchKernelGGL(lusgs_sweep_kernel<2>, gr, tb, 0, s, L, start, count, gm1, p.nvis); break;
+    default: hipLaunchKernelGGL(lusgs_sweep_kernel<3>, gr, tb, 0, s, L, start, count, gm1, p.nvis); break;
+  }
 }
 
 }  // namespace orion_hip

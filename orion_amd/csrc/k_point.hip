@@ -7,10 +7,9 @@
 //                 dq=0, then the spectral radii and dt/vol on interior points
 //                 (the only points whose radii the solver reads) and the
 //                 minimum physical time step.
-//   lu_prep       rhs = dq with physical-BC windows zeroed
-//                 (StateUpdater.cpp:447-468, 557-570).
-//   update        zero the BC windows again and apply update_limited
-//                 (StateUpdater.cpp:470-542, 577-578).
+//   update        copy the swept increments back from the plane-major
+//                 working set, zero the BC windows again and apply
+//                 update_limited (StateUpdater.cpp:470-542, 577-578).
 //   residual      ResidualMonitor::compute (src/solver/ResidualMonitor.cpp:11-138):
 //                 deterministic two-pass reduction of sum r^2 and of the
 //                 first maximum in (block,k,j,i,var) order.
@@ -113,22 +112,9 @@ __global__ void __launch_bounds__(256) state_kernel(const BlockDev* __restrict__
   if (threadIdx.x == 0 && smin[0] < 1000.0) atomicMin(dtmin_bits, (unsigned long long)__double_as_longlong(smin[0]));
 }
 
-__global__ void __launch_bounds__(256) lu_prep_kernel(const BlockDev* __restrict__ blocks,
-                                                      const long long* __restrict__ off, int nb) {
-  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= off[nb]) return;
-  const int bi = find_block(off, nb, g);
-  const BlockDev& b = blocks[bi];
-  int i, j, k;
-  decode_interior(b, g - off[bi], &i, &j, &k);
-  const long long id = pidx(b, i, j, k);
-  const bool fz = b.frozen[id] != 0;
-#pragma unroll
-  for (int m = 0; m < 5; ++m) b.rhs[m * b.P + id] = fz ? 0.0 : b.dq[m * b.P + id];
-}
-
 __global__ void __launch_bounds__(256) update_kernel(const BlockDev* __restrict__ blocks,
-                                                     const long long* __restrict__ off, int nb, double gama) {
+                                                     const long long* __restrict__ off, int nb, LuPacked lp,
+                                                     double gama) {
   const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= off[nb]) return;
   const int bi = find_block(off, nb, g);
@@ -145,8 +131,12 @@ __global__ void __launch_bounds__(256) update_kernel(const BlockDev* __restrict_
       b.dq[m * P + id] = 0.0;
     }
   } else {
+    const long long q = b.lpos[id];
 #pragma unroll
-    for (int m = 0; m < 5; ++m) d[m] = b.dq[m * P + id];
+    for (int m = 0; m < 5; ++m) {
+      d[m] = lp.dq[m * lp.ncell + q];
+      b.dq[m * P + id] = d[m];
+    }
   }
   const double gm1 = gama - 1.0;
   const double r = b.prim[id], u = b.prim[P + id], v = b.prim[2 * P + id], w = b.prim[3 * P + id];
@@ -303,13 +293,9 @@ void launch_state(const Launch& L, long long n, const Params& p, int mode, unsig
   hipLaunchKernelGGL(state_kernel, dim3(grid_of(n, 256)), dim3(256), 0, L.stream, L.blocks, L.off, L.nb, p, mode,
                      dtmin_bits);
 }
-void launch_lu_prep(const Launch& L, long long n) {
+void launch_update(const Launch& L, long long n, const LuPacked& lp, double gamma) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(lu_prep_kernel, dim3(grid_of(n, 256)), dim3(256), 0, L.stream, L.blocks, L.off, L.nb);
-}
-void launch_update(const Launch& L, long long n, double gamma) {
-  if (n <= 0) return;
-  hipLaunchKernelGGL(update_kernel, dim3(grid_of(n, 256)), dim3(256), 0, L.stream, L.blocks, L.off, L.nb, gamma);
+  hipLaunchKernelGGL(update_kernel, dim3(grid_of(n, 256)), dim3(256), 0, L.stream, L.blocks, L.off, L.nb, lp, gamma);
 }
 void launch_residual(const Launch& L, long long n, long long total5, double* partial, long long* pidx_, int nparts,
                      double* out) {

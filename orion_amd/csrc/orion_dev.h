@@ -37,8 +37,19 @@ struct BlockDev {
   double* duvwt;         // [12][P] viscous cell derivatives (scratch)
   double* fx;            // [3][5][P] face fluxes per direction (scratch)
   double* fcov;          // closure overrides, [3][lines][6][5] (scratch)
+  int* lpos;             // [P] plane-major LU-SGS index of each interior point
   unsigned char* frozen; // [P] 1 inside a physical-BC window
   long long fcov_off[3]; // offset (doubles) of each direction's overrides
+};
+
+// Plane-major working set of the LU-SGS sweeps (all blocks, ncell cells).
+struct LuPacked {
+  double* coef;  // [40][ncell] dq-independent coefficients
+  double* rhs;   // [5][ncell]
+  double* src;   // [6][ncell] u,v,w,c2,H,af of each cell
+  double* dq;    // [5][ncell]
+  int* nbr;      // [6][ncell] neighbour cell or -1 (i-1,j-1,k-1,i+1,j+1,k+1)
+  long long ncell;
 };
 
 struct Params {

@@ -44,8 +44,7 @@ void launch_visc_diff(const Launch& L, long long n, double re_inv);
 
 // k_point.hip
 void launch_state(const Launch& L, long long n, const Params& p, int mode, unsigned long long* dtmin_bits);
-void launch_lu_prep(const Launch& L, long long n);
-void launch_update(const Launch& L, long long n, double gamma);
+void launch_update(const Launch& L, long long n, const LuPacked& lp, double gamma);
 void launch_residual(const Launch& L, long long n, long long total5, double* partial, long long* pidx,
                      int nparts, double* out);
 int residual_parts(long long n);
@@ -60,7 +59,9 @@ void launch_avg_stage(const BlockDev* blocks, const AvgSrc* src, long long n, do
 void launch_avg_apply(const BlockDev* blocks, const AvgDst* dst, long long n, const double* stage, hipStream_t s);
 
 // k_lusgs.hip
-void launch_lusgs_plane(const Launch& L, long long n, int plane, int type, const Params& p);
+void launch_lu_coef(const Launch& La, long long n, const LuPacked& L, const Params& p);
+void launch_lusgs_sweep(const LuPacked& L, long long start, long long count, int type, const Params& p,
+                        hipStream_t s);
 
 // k_metrics.hip
 void launch_metrics(const Launch& L, long long n, int gcl, double* tmp, hipStream_t s);
