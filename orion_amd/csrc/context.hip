@@ -82,6 +82,11 @@ struct orion_hip_ctx {
   unsigned long long* d_count = nullptr;
   std::vector<void*> allocs;
   int keep_q = 0;
+  int* d_step = nullptr;             // step index for per-step outputs
+  hipGraph_t graph = nullptr;        // one captured pseudo-time step
+  hipGraphExec_t gexec = nullptr;
+  hipStream_t graph_stream = nullptr;
+  int use_graph = 1;
 };
 
 static int dalloc(orion_hip_ctx* c, void** p, size_t bytes) {
@@ -509,6 +514,8 @@ void orion_hip_ctx_destroy(orion_hip_ctx* c) {
   if (c->stream) hipStreamSynchronize(c->stream);
   for (void* p : c->allocs) hipFree(p);
   if (c->arena) hipFree(c->arena);
+  if (c->gexec) hipGraphExecDestroy(c->gexec);
+  if (c->graph) hipGraphDestroy(c->graph);
   if (c->own_stream) hipStreamDestroy(c->own_stream);
   delete c;
 }
@@ -658,8 +665,9 @@ int orion_hip_ctx_create(int nblocks, const int* dims, const double* xyz, int nr
   if (build_planes(c)) return bail(-1);
   c->nparts = residual_parts(c->intr.total());
   if (dalloc(c, (void**)&c->d_part, sizeof(double) * 2 * 1024) || dalloc(c, (void**)&c->d_pidx, sizeof(long long) * 1024) ||
-      dalloc(c, (void**)&c->d_count, sizeof(unsigned long long) * 2))
+      dalloc(c, (void**)&c->d_count, sizeof(unsigned long long) * 2) || dalloc(c, (void**)&c->d_step, sizeof(int)))
     return bail(-1);
+  if (const char* e = std::getenv("ORION_HIP_GRAPH")) c->use_graph = std::atoi(e) != 0;
   // initial state (InitialCondition.cpp:49-80): freestream prim everywhere
   const Params& p = c->p;
   const double prim0[6] = {p.roo, p.uoo, p.voo, p.woo, p.poo, p.too};
@@ -808,6 +816,10 @@ static int ensure_res(orion_hip_ctx* c, int n) {
   if (dalloc(c, (void**)&c->d_res, sizeof(double) * 4 * n)) return -1;
   if (dalloc(c, (void**)&c->d_dtmin, sizeof(unsigned long long) * n)) return -1;
   c->res_cap = n;
+  if (c->gexec) {  // the captured step points at the old buffers
+    hipGraphExecDestroy(c->gexec);
+    c->gexec = nullptr;
+  }
   return 0;
 }
 
@@ -840,7 +852,7 @@ int orion_hip_apply_physical_bc(orion_hip_ctx* c) {
   return check_launch("apply_physical_bc");
 }
 int orion_hip_update_flow_states(orion_hip_ctx* c) {
-  launch_state(LA(c, c->pad), c->pad.total(), c->p, 1, nullptr);
+  launch_state(LA(c, c->pad), c->pad.total(), c->p, 1, nullptr, nullptr);
   return check_launch("update_flow_states");
 }
 int orion_hip_calculate_time_step(orion_hip_ctx* c, double* phydtime) {
@@ -849,7 +861,7 @@ int orion_hip_calculate_time_step(orion_hip_ctx* c, double* phydtime) {
   unsigned long long bits;
   std::memcpy(&bits, &init, sizeof bits);
   HIPCHK(hipMemcpyAsync(c->d_dtmin, &bits, sizeof bits, hipMemcpyHostToDevice, c->stream));
-  launch_state(LA(c, c->pad), c->pad.total(), c->p, 2, c->d_dtmin);
+  launch_state(LA(c, c->pad), c->pad.total(), c->p, 2, c->d_dtmin, nullptr);
   if (check_launch("calculate_time_step")) return -1;
   if (phydtime) {
     HIPCHK(hipMemcpyAsync(&bits, c->d_dtmin, sizeof bits, hipMemcpyDeviceToHost, c->stream));
@@ -876,7 +888,8 @@ int orion_hip_update_solution(orion_hip_ctx* c) {
 }
 int orion_hip_residual(orion_hip_ctx* c, orion_residual_stats* out) {
   if (ensure_res(c, 1)) return -1;
-  launch_residual(LA(c, c->intr), c->intr.total(), 5 * c->intr.total(), c->d_part, c->d_pidx, c->nparts, c->d_res);
+  launch_residual(LA(c, c->intr), c->intr.total(), 5 * c->intr.total(), c->d_part, c->d_pidx, c->nparts, c->d_res,
+                  nullptr);
   if (check_launch("residual")) return -1;
   double r[4];
   unsigned long long bits = 0;
@@ -884,6 +897,45 @@ int orion_hip_residual(orion_hip_ctx* c, orion_residual_stats* out) {
   HIPCHK(hipMemcpyAsync(&bits, c->d_dtmin, sizeof bits, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   if (out) decode_stats(c, r, bits, out);
+  return 0;
+}
+
+// One pseudo-time step; per-step outputs go to slot *d_step, which the last
+// kernel advances, so the same sequence can be captured once and replayed.
+static void step_body(orion_hip_ctx* c) {
+  op_halo(c);
+  op_bc(c);
+  launch_state(LA(c, c->pad), c->pad.total(), c->p, 3, c->d_dtmin, c->d_step);
+  if (c->p.nvis == 1) {
+    op_visc(c);
+    op_avg(c);
+  }
+  op_inv(c);
+  op_avg(c);
+  op_update(c);
+  launch_residual(LA(c, c->intr), c->intr.total(), 5 * c->intr.total(), c->d_part, c->d_pidx, c->nparts, c->d_res,
+                  c->d_step);
+  launch_step_advance(c->d_step, c->stream);
+}
+
+static int ensure_graph(orion_hip_ctx* c) {
+  if (c->gexec && c->graph_stream == c->stream) return 0;
+  if (c->gexec) {
+    hipGraphExecDestroy(c->gexec);
+    c->gexec = nullptr;
+  }
+  if (c->graph) {
+    hipGraphDestroy(c->graph);
+    c->graph = nullptr;
+  }
+  HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  step_body(c);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(c->stream, &g);
+  if (e != hipSuccess) return fail("step capture failed: %s", hipGetErrorString(e));
+  c->graph = g;
+  HIPCHK(hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
+  c->graph_stream = c->stream;
   return 0;
 }
 
@@ -895,24 +947,18 @@ int orion_hip_run(orion_hip_ctx* c, int nsteps, orion_residual_stats* hist) {
     unsigned long long bits;
     std::memcpy(&bits, &init, sizeof bits);
     std::vector<unsigned long long> v(nsteps, bits);
+    const int zero = 0;
     HIPCHK(hipMemcpyAsync(c->d_dtmin, v.data(), sizeof(unsigned long long) * nsteps, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->d_step, &zero, sizeof zero, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
   }
-  for (int s = 0; s < nsteps; ++s) {
-    op_halo(c);
-    op_bc(c);
-    launch_state(LA(c, c->pad), c->pad.total(), c->p, 3, c->d_dtmin + s);
-    if (c->p.nvis == 1) {
-      op_visc(c);
-      op_avg(c);
-    }
-    op_inv(c);
-    op_avg(c);
-    op_update(c);
-    launch_residual(LA(c, c->intr), c->intr.total(), 5 * c->intr.total(), c->d_part, c->d_pidx, c->nparts,
-                    c->d_res + 4 * s);
-    if (check_launch("run")) return -1;
+  if (c->use_graph) {
+    if (ensure_graph(c)) return -1;
+    for (int s = 0; s < nsteps; ++s) HIPCHK(hipGraphLaunch(c->gexec, c->stream));
+  } else {
+    for (int s = 0; s < nsteps; ++s) step_body(c);
   }
+  if (check_launch("run")) return -1;
   if (hist) {
     std::vector<double> r(4 * (size_t)nsteps);
     std::vector<unsigned long long> bits(nsteps);
@@ -943,7 +989,7 @@ int orion_hip_profile_run(orion_hip_ctx* c, int nsteps, double* phase_ms, long l
     hipEventRecord(ev[0], c->stream);
     op_halo(c); mark(1);
     op_bc(c); mark((long long)c->bc_first.size());
-    launch_state(LA(c, c->pad), c->pad.total(), c->p, 3, c->d_dtmin + s); mark(1);
+    launch_state(LA(c, c->pad), c->pad.total(), c->p, 3, c->d_dtmin + s, nullptr); mark(1);
     if (c->p.nvis == 1) {
       const double cp = 1.0 / ((c->p.gama - 1.0) * c->p.moo * c->p.moo);
       const double re = c->p.reynolds;
@@ -965,7 +1011,7 @@ int orion_hip_profile_run(orion_hip_ctx* c, int nsteps, double* phase_ms, long l
     for (int t = 0; t < 4; ++t) { op_sweep(c, t); mark(planes); }
     launch_update(LA(c, c->intr), c->intr.total(), c->lu, c->p.gama); mark(1);
     launch_residual(LA(c, c->intr), c->intr.total(), 5 * c->intr.total(), c->d_part, c->d_pidx, c->nparts,
-                    c->d_res + 4 * s);
+                    c->d_res + 4 * s, nullptr);
     mark(2);
     if (check_launch("profile_run")) return -1;
     HIPCHK(hipEventSynchronize(ev[k]));

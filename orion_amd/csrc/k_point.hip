@@ -20,7 +20,7 @@ namespace orion_hip {
 
 __global__ void __launch_bounds__(256) state_kernel(const BlockDev* __restrict__ blocks,
                                                     const long long* __restrict__ off, int nb, Params p, int mode,
-                                                    unsigned long long* dtmin_bits) {
+                                                    unsigned long long* dtmin_base, const int* step) {
   __shared__ double smin[256];
   const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   double mydt = 1000.0;
@@ -98,7 +98,8 @@ __global__ void __launch_bounds__(256) state_kernel(const BlockDev* __restrict__
       mydt = back < mydt ? back : mydt;
     }
   }
-  if (!dtmin_bits) return;
+  if (!dtmin_base) return;
+  unsigned long long* dtmin_bits = dtmin_base + (step ? *step : 0);
   smin[threadIdx.x] = mydt;
   __syncthreads();
   for (int s = 128; s > 0; s >>= 1) {
@@ -238,7 +239,8 @@ __global__ void __launch_bounds__(RES_TB) residual_kernel(const BlockDev* __rest
 
 __global__ void __launch_bounds__(RES_MAXPARTS) residual_final_kernel(const double* partial, const double* pmax,
                                                                       const long long* pidx_, int nparts,
-                                                                      long long total5, double* out) {
+                                                                      long long total5, double* out_base,
+                                                                      const int* step) {
   __shared__ double ssum[RES_MAXPARTS];
   __shared__ double smax[RES_MAXPARTS];
   __shared__ long long skey[RES_MAXPARTS];
@@ -259,10 +261,15 @@ __global__ void __launch_bounds__(RES_MAXPARTS) residual_final_kernel(const doub
     __syncthreads();
   }
   if (t == 0) {
+    double* out = out_base + 4 * (step ? *step : 0);
     out[0] = total5 > 0 ? sqrt(ssum[0] / (double)total5) : 0.0;
     out[1] = smax[0];
     out[2] = (double)skey[0];
   }
+}
+
+__global__ void step_advance_kernel(int* step) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *step += 1;
 }
 
 __global__ void fill_kernel(double* p, long long n, double v) {
@@ -288,21 +295,25 @@ __global__ void __launch_bounds__(256) temperature_kernel(const BlockDev* __rest
 
 static inline unsigned grid_of(long long n, int tb) { return (unsigned)((n + tb - 1) / tb); }
 
-void launch_state(const Launch& L, long long n, const Params& p, int mode, unsigned long long* dtmin_bits) {
+void launch_state(const Launch& L, long long n, const Params& p, int mode, unsigned long long* dtmin_base,
+                  const int* step) {
   if (n <= 0) return;
   hipLaunchKernelGGL(state_kernel, dim3(grid_of(n, 256)), dim3(256), 0, L.stream, L.blocks, L.off, L.nb, p, mode,
-                     dtmin_bits);
+                     dtmin_base, step);
+}
+void launch_step_advance(int* step, hipStream_t s) {
+  hipLaunchKernelGGL(step_advance_kernel, dim3(1), dim3(64), 0, s, step);
 }
 void launch_update(const Launch& L, long long n, const LuPacked& lp, double gamma) {
   if (n <= 0) return;
   hipLaunchKernelGGL(update_kernel, dim3(grid_of(n, 256)), dim3(256), 0, L.stream, L.blocks, L.off, L.nb, lp, gamma);
 }
 void launch_residual(const Launch& L, long long n, long long total5, double* partial, long long* pidx_, int nparts,
-                     double* out) {
+                     double* out, const int* step) {
   hipLaunchKernelGGL(residual_kernel, dim3(nparts), dim3(RES_TB), 0, L.stream, L.blocks, L.off, L.nb, partial,
                      partial + RES_MAXPARTS, pidx_);
   hipLaunchKernelGGL(residual_final_kernel, dim3(1), dim3(RES_MAXPARTS), 0, L.stream, partial, partial + RES_MAXPARTS,
-                     pidx_, nparts, total5, out);
+                     pidx_, nparts, total5, out, step);
 }
 void launch_fill(double* p, long long n, double v, hipStream_t s) {
   if (n <= 0) return;

@@ -43,10 +43,12 @@ void launch_visc_face(const Launch& L, int d, long long n, double cp_prl);
 void launch_visc_diff(const Launch& L, long long n, double re_inv);
 
 // k_point.hip
-void launch_state(const Launch& L, long long n, const Params& p, int mode, unsigned long long* dtmin_bits);
+void launch_state(const Launch& L, long long n, const Params& p, int mode, unsigned long long* dtmin_base,
+                  const int* step);
+void launch_step_advance(int* step, hipStream_t s);
 void launch_update(const Launch& L, long long n, const LuPacked& lp, double gamma);
 void launch_residual(const Launch& L, long long n, long long total5, double* partial, long long* pidx,
-                     int nparts, double* out);
+                     int nparts, double* out, const int* step);
 int residual_parts(long long n);
 void launch_fill(double* p, long long n, double v, hipStream_t s);
 void launch_temperature(const Launch& L, long long n, double gama, double moo);
