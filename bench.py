@@ -28,9 +28,21 @@ sys.path.insert(0, ROOT)
 
 METRIC = "FP64 cell-steps/s at 8M cells vs CPU; served solves/s + p50/p99 solve latency"
 HBM_PEAK_GBS = 8000.0
-# compulsory FP64 words per interior cell for each stage: prim, geometry
-# (9 metrics + vol), right-hand side and increments read or written once
-WORDS = {"lu_fwd": 25, "lu_bwd": 25, "gs_fwd": 30, "gs_bwd": 30, "rhs": 20, "update": 25}
+# Compulsory FP64 words per interior cell for each stage of the step (prim,
+# geometry = 9 metrics + vol, right-hand side and increments read or written
+# once; derived per-cell scalars recomputed; O(surface) work excluded).  The
+# stages sum to 155 words = 1240 B per cell-step.  Each stage lists the
+# kernels that implement it and the profiled phases they run in.
+STAGES = {
+    "rhs": (20, ["visc_deriv", "visc_face", "visc_diff", "inv_face", "inv_closure", "inv_diff"],
+            "visc_deriv_kernel+visc_face_kernel+visc_diff_kernel+inv_face_kernel+inv_closure_kernel+inv_diff_kernel"),
+    "lu_fwd": (25, ["lu_prep", "lu_fwd"], "lu_coef_kernel+lusgs_sweep_kernel<0>"),
+    "lu_bwd": (25, ["lu_bwd"], "lusgs_sweep_kernel<1>"),
+    "gs_fwd": (30, ["gs_fwd"], "lusgs_sweep_kernel<2>"),
+    "gs_bwd": (30, ["gs_bwd"], "lusgs_sweep_kernel<3>"),
+    "update": (25, ["update", "residual"], "update_kernel+residual_kernel+residual_final_kernel"),
+}
+STEP_WORDS = sum(v[0] for v in STAGES.values())
 
 
 def make_case(name, nsteps):
@@ -44,7 +56,9 @@ def make_case(name, nsteps):
     raise SystemExit("unknown workload " + name)
 
 
-def roofline(fs, case, steps):
+def profile_phases(fs, steps):
+    """Per-phase ms/step and launches/step, timed with HIP events recorded on
+    the solver's own stream around each phase of ``steps`` eager steps."""
     import ctypes as C
     from orion_amd import _lib
     L = _lib.load()
@@ -52,37 +66,139 @@ def roofline(fs, case, steps):
     ms = (C.c_double * n)()
     nl = (C.c_longlong * n)()
     _lib.check(L.orion_hip_profile_run(fs.ctx, steps, ms, nl), "profile_run")
-    phase = {_lib.PHASES[i]: (ms[i] / steps, nl[i] / steps) for i in range(n)}
-    N = case.interior_nodes()
-    # dominant kernel family with a defined algorithmic byte count
-    cand = {k: v for k, v in phase.items() if k in WORDS}
-    dom = max(cand, key=lambda k: cand[k][0])
-    t_ms, launches = cand[dom]
-    bytes_step = WORDS[dom] * 8.0 * N
-    achieved = bytes_step / (t_ms * 1e-3) / 1e9
-    return {
+    return {_lib.PHASES[i]: (ms[i] / steps, nl[i] / steps) for i in range(n)}
+
+
+def roofline(phase, N, ms_per_step):
+    """Stage rooflines: algorithmic bytes (stage words x 8 B x interior
+    cells) over the stage's measured time.  The dominant stage is reported
+    as ``roofline``; every stage and the whole step are reported beside it."""
+    stages = {}
+    for name, (words, phases, kernels) in STAGES.items():
+        t_ms = sum(phase[p][0] for p in phases)
+        launches = sum(phase[p][1] for p in phases)
+        nbytes = words * 8.0 * N
+        gbs = nbytes / (t_ms * 1e-3) / 1e9
+        stages[name] = {"kernels": kernels, "ms": round(t_ms, 4), "launches": launches,
+                        "algorithmic_bytes": nbytes, "achieved_GBs": round(gbs, 1),
+                        "frac": round(gbs / HBM_PEAK_GBS, 5)}
+    dom = max(stages, key=lambda k: stages[k]["ms"])
+    d = stages[dom]
+    roof = {
         "bound": "hbm",
-        "kernel": "lusgs_plane_kernel (%s)" % dom if dom.startswith(("lu", "gs")) else dom,
-        "achieved": round(achieved, 2),
+        "kernel": "%s (%s stage)" % (d["kernels"], dom),
+        "achieved": d["achieved_GBs"],
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBS, 5),
+        "frac": d["frac"],
         "traffic": None,
-        "launch_avg_us": round(1e3 * t_ms / max(launches, 1), 3),
-        "launches_per_step": launches,
-        "algorithmic_bytes_per_launch": bytes_step / max(launches, 1),
-    }, {k: round(v[0], 4) for k, v in phase.items()}
+        "algorithmic_bytes_per_launch": d["algorithmic_bytes"] / max(d["launches"], 1),
+        "launch_avg_us": round(1e3 * d["ms"] / max(d["launches"], 1), 3),
+        "launches_per_step": d["launches"],
+    }
+    step_gbs = STEP_WORDS * 8.0 * N / (ms_per_step * 1e-3) / 1e9
+    step = {"bytes_per_cell_step": STEP_WORDS * 8, "achieved_GBs": round(step_gbs, 1),
+            "frac": round(step_gbs / HBM_PEAK_GBS, 5)}
+    return roof, stages, step
 
 
-def cpu_baseline(case, threads):
+def copy_calibration(mib):
+    """A plain device copy of ``mib`` MiB (read mib, write mib) repeated 5
+    times: the counter scale for FETCH_SIZE / WRITE_SIZE."""
+    import torch
+    n = mib * (1 << 20) // 8
+    src = torch.ones(n, dtype=torch.float64, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": n * 8, "launches": 5}), flush=True)
+    return 0
+
+
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "hbm_traffic_{workload}.txt")
+
+
+def _kernel_key(name):
+    """Short kernel name of a rocprofv3 Kernel_Name ("void ns::k<2>(args)" -> "k<2>")."""
+    n = name.split("(")[0].replace("void ", "").strip()
+    return n.split("::")[-1]
+
+
+def summarize_counters(fetch_csv, write_csv, cfetch_csv, cwrite_csv, steps, cmd):
+    """Per-kernel HBM bytes per step from two rocprofv3 --pmc runs
+    (FETCH_SIZE, WRITE_SIZE, KiB per dispatch), each scaled by what the same
+    counter reads for a plain device copy of known size."""
+    import csv
+    from collections import defaultdict
+
+    def load(path):
+        agg = defaultdict(lambda: [0, 0.0])
+        for r in csv.DictReader(open(path)):
+            a = agg[_kernel_key(r["Kernel_Name"])]
+            a[0] += 1
+            a[1] += float(r["Counter_Value"])
+        return agg
+
+    def copy_kib(path):
+        rows = [r for r in csv.DictReader(open(path)) if "copyBuffer" in r["Kernel_Name"]]
+        return sum(float(r["Counter_Value"]) for r in rows) / len(rows)
+
+    copy_bytes = 1024 * (1 << 20)
+    sf = copy_bytes / (copy_kib(cfetch_csv) * 1024.0)
+    sw = copy_bytes / (copy_kib(cwrite_csv) * 1024.0)
+    f, w = load(fetch_csv), load(write_csv)
+    lines = ["# HBM traffic per kernel (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE)",
+             "# command: " + cmd,
+             "# calibration: a %d-byte device copy reads FETCH_SIZE x %.4f and WRITE_SIZE x %.4f bytes" % (
+                 copy_bytes, sf, sw),
+             "# columns: kernel dispatches_per_step fetch_bytes_per_step write_bytes_per_step",
+             "scale_fetch %.6f" % sf, "scale_write %.6f" % sw]
+    for k in sorted(f, key=lambda k: -(f[k][1] * sf + w[k][1] * sw)):
+        lines.append("%s %g %.0f %.0f" % (k, f[k][0] / steps, f[k][1] * 1024 * sf / steps,
+                                          w.get(k, [0, 0.0])[1] * 1024 * sw / steps))
+    return "\n".join(lines) + "\n"
+
+
+def stage_traffic(workload, stages):
+    """HBM bytes per launch of each stage from the committed counter summary
+    (None when there is none for this workload)."""
+    path = TRAFFIC_SUMMARY.format(workload=workload)
+    if not os.path.exists(path):
+        return {}
+    per = {}
+    for line in open(path):
+        parts = line.split()
+        if line.startswith("#") or len(parts) != 4:
+            continue
+        per[parts[0]] = (float(parts[1]), float(parts[2]) + float(parts[3]))
+    out = {}
+    for name, (_, _, kernels) in stages.items():
+        ks = kernels.split("+")
+        if not all(k in per for k in ks):
+            continue
+        nbytes = sum(per[k][1] for k in ks)
+        launches = sum(per[k][0] for k in ks)
+        out[name] = {"bytes_per_step": nbytes, "bytes_per_launch": nbytes / max(launches, 1)}
+    return out
+
+
+def cpu_baseline(case, threads, seconds):
+    """The C oracle (a restatement of the reference path, OpenMP over blocks)
+    on the same case: whole pseudo-steps until ``seconds`` have passed."""
     from oracle.binding import OracleCase
     o = OracleCase(case, threads=threads)
-    t0 = time.perf_counter()
-    o.step()
-    dt = time.perf_counter() - t0
-    return {"value": case.interior_nodes() / dt, "unit": "cell-steps/s", "cores": threads, "kind": "port",
-            "sample": "1 full pseudo-step of the same %d-block, %d-node case on %d OpenMP threads (%.1f s)" % (
-                case.nblocks, case.interior_nodes(), threads, dt)}
+    n, t0 = 0, time.perf_counter()
+    while True:
+        o.step()
+        n += 1
+        dt = time.perf_counter() - t0
+        if dt >= seconds:
+            break
+    print("[bench] cpu baseline: %d steps in %.1f s on %d threads" % (n, dt, threads), file=sys.stderr, flush=True)
+    return {"value": case.interior_nodes() * n / dt, "unit": "cell-steps/s", "cores": threads, "kind": "port",
+            "sample": "%d full pseudo-steps of the same %d-block, %d-node case on %d OpenMP threads (%.1f s)" % (
+                n, case.nblocks, case.interior_nodes(), threads, dt)}
 
 
 def main():
@@ -94,11 +210,23 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="only run a plain device copy of this many MiB (HBM counter calibration)")
+    ap.add_argument("--summarize-counters", nargs=4, metavar=("FETCH", "WRITE", "COPY_FETCH", "COPY_WRITE"),
+                    help="write the per-kernel HBM traffic summary from rocprofv3 counter CSVs and exit")
+    ap.add_argument("--counter-steps", type=int, default=2, help="steps covered by the counter run")
+    ap.add_argument("--counter-command", default="", help="command recorded in the summary")
     a = ap.parse_args()
+    if a.summarize_counters:
+        sys.stdout.write(summarize_counters(*a.summarize_counters, a.counter_steps, a.counter_command))
+        return 0
     if a.gpus != 1:
         print("bench.py: this build measures one GPU only (the path shards by block across GPUs, which is "
               "designed but not built here); run with --gpus 1", file=sys.stderr)
         return 2
+    if a.copy_calibration_mib:
+        return copy_calibration(a.copy_calibration_mib)
     from orion_amd.solver import OrionSolver
     case = make_case(a.workload, a.steps)
     N = case.interior_nodes()
@@ -117,8 +245,16 @@ def main():
     value = N * a.steps / t
     print("[bench] %d steps in %.3f s, rms %.6e -> %.6e" % (a.steps, t, hist[0].rms_residual, hist[-1].rms_residual),
           file=sys.stderr, flush=True)
-    roof, phases = roofline(s.fs, case, a.profile_steps)
+    phase = profile_phases(s.fs, a.profile_steps)
+    phases = {k: round(v[0], 4) for k, v in phase.items()}
     print("[bench] phase ms/step: " + json.dumps(phases), file=sys.stderr, flush=True)
+    roof, stages, step_roof = roofline(phase, N, 1e3 * t / a.steps)
+    traffic = stage_traffic(a.workload, STAGES)
+    for name, tr in traffic.items():
+        stages[name]["traffic_bytes_per_launch"] = tr["bytes_per_launch"]
+    dom = roof["kernel"].rsplit("(", 1)[1].split()[0]
+    if dom in traffic:
+        roof["traffic"] = traffic[dom]["bytes_per_launch"]
     out = {
         "metric": METRIC,
         "value": value,
@@ -136,13 +272,15 @@ def main():
                    "scheme": "WCNS-E-5+Roe, 6th-order viscous, LU-SGS+2 SGS sweeps", "precision": "fp64",
                    "moo": case.inputs["moo"], "reynolds": case.inputs["reynolds"], "cfl": case.inputs["cfl"]},
         "roofline": roof,
+        "stage_rooflines": stages,
+        "step_roofline": step_roof,
         "phase_ms_per_step": phases,
         "residual_first_last": [hist[0].rms_residual, hist[-1].rms_residual],
     }
     s.fs.close()
     if not a.no_cpu:
         threads = a.cpu_threads or min(16, os.cpu_count() or 1)
-        out["cpu_baseline"] = cpu_baseline(case, threads)
+        out["cpu_baseline"] = cpu_baseline(case, threads, a.cpu_seconds)
         out["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]
     print(json.dumps(out), flush=True)
     return 0

@@ -21,7 +21,7 @@ namespace orion_hip {
 
 __global__ void __launch_bounds__(256) halo_kernel(const long long* __restrict__ pairs, long long n,
                                                    double* __restrict__ arena) {
-  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long g = xcd_thread();
   if (g >= n) return;
   arena[pairs[2 * g]] = arena[pairs[2 * g + 1]];
 }
@@ -62,7 +62,7 @@ __device__ inline void dif_average(const BlockDev& b, const int* o, int dir, int
 
 __global__ void __launch_bounds__(256) bc_kernel(const BlockDev* __restrict__ blocks, const BCTask* __restrict__ tasks,
                                                  int ntasks, long long total, Params p) {
-  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long g = xcd_thread();
   if (g >= total) return;
   int t = 0;
   {
@@ -292,7 +292,7 @@ __global__ void __launch_bounds__(256) bc_kernel(const BlockDev* __restrict__ bl
 __global__ void __launch_bounds__(256) avg_stage_kernel(const BlockDev* __restrict__ blocks,
                                                         const AvgSrc* __restrict__ src, long long n,
                                                         double* __restrict__ stage) {
-  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long g = xcd_thread();
   if (g >= n) return;
   const AvgSrc s = src[g];
   const BlockDev& b = blocks[s.block];
@@ -304,7 +304,7 @@ __global__ void __launch_bounds__(256) avg_stage_kernel(const BlockDev* __restri
 __global__ void __launch_bounds__(256) avg_apply_kernel(const BlockDev* __restrict__ blocks,
                                                         const AvgDst* __restrict__ dst, long long n,
                                                         const double* __restrict__ stage) {
-  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long g = xcd_thread();
   if (g >= n) return;
   const AvgDst D = dst[g];
   const BlockDev& b = blocks[D.block];

@@ -209,7 +209,7 @@ __device__ inline void face_flux(const LineRef& L, int n, int f, const double* q
 __global__ void __launch_bounds__(256) inv_face_kernel(const BlockDev* __restrict__ blocks,
                                                        const long long* __restrict__ off, int nb, int d,
                                                        double gamma) {
-  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long g = xcd_thread();
   if (g >= off[nb]) return;
   const int bi = find_block(off, nb, g);
   const BlockDev& b = blocks[bi];
@@ -234,7 +234,7 @@ __global__ void __launch_bounds__(256) inv_face_kernel(const BlockDev* __restric
 __global__ void __launch_bounds__(64) inv_closure_kernel(const BlockDev* __restrict__ blocks,
                                                          const long long* __restrict__ off, int nb, int d,
                                                          double gamma) {
-  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long g = xcd_thread();
   if (g >= off[nb]) return;
   const int bi = find_block(off, nb, g);
   const BlockDev& b = blocks[bi];
@@ -372,7 +372,7 @@ __device__ inline double diff_node(const double* F0, long long s, int n, int t) 
 
 __global__ void __launch_bounds__(256) inv_diff_kernel(const BlockDev* __restrict__ blocks,
                                                        const long long* __restrict__ off, int nb) {
-  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long g = xcd_thread();
   if (g >= off[nb]) return;
   const int bi = find_block(off, nb, g);
   const BlockDev& b = blocks[bi];
@@ -427,7 +427,7 @@ __device__ inline double node_deriv(const double* q, long long s, int n, int i1,
 
 __global__ void __launch_bounds__(256) visc_deriv_kernel(const BlockDev* __restrict__ blocks,
                                                          const long long* __restrict__ off, int nb) {
-  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long g = xcd_thread();
   if (g >= off[nb]) return;
   const int bi = find_block(off, nb, g);
   const BlockDev& b = blocks[bi];
@@ -479,7 +479,7 @@ __device__ inline double half_deriv(const double* q, long long s, int n, int f, 
 __global__ void __launch_bounds__(256) visc_face_kernel(const BlockDev* __restrict__ blocks,
                                                         const long long* __restrict__ off, int nb, int d,
                                                         double cp_prl) {
-  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long g = xcd_thread();
   if (g >= off[nb]) return;
   const int bi = find_block(off, nb, g);
   const BlockDev& b = blocks[bi];
@@ -554,7 +554,7 @@ __global__ void __launch_bounds__(256) visc_face_kernel(const BlockDev* __restri
 
 __global__ void __launch_bounds__(256) visc_diff_kernel(const BlockDev* __restrict__ blocks,
                                                         const long long* __restrict__ off, int nb, double re_inv) {
-  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long g = xcd_thread();
   if (g >= off[nb]) return;
   const int bi = find_block(off, nb, g);
   const BlockDev& b = blocks[bi];

@@ -62,7 +62,7 @@ constexpr int LC_COED = 0, LC_NRM = 1, LC_SLOT = 10;
 __global__ void __launch_bounds__(256) lu_coef_kernel(const BlockDev* __restrict__ blocks,
                                                       const long long* __restrict__ off, int nb, LuPacked L,
                                                       Params p) {
-  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long g = xcd_thread();
   if (g >= off[nb]) return;
   const int bi = find_block(off, nb, g);
   const BlockDev& b = blocks[bi];
@@ -241,7 +241,7 @@ __device__ __forceinline__ void slot_apply(const SlotIn& s, bool valid, double g
 template <int TYPE>
 __global__ void __launch_bounds__(256) lusgs_sweep_kernel(LuPacked L, long long start, long long count, double gm1,
                                                           int nvis) {
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long t = xcd_thread();
   if (t >= count) return;
   const long long q = start + t;
   const long long N = L.ncell;

@@ -52,7 +52,7 @@ __device__ inline double dnode(const BlockDev& b, const double* fld, long long i
 
 __global__ void __launch_bounds__(256) metric_a_kernel(const BlockDev* __restrict__ blocks,
                                                        const long long* __restrict__ off, int nb, int gcl) {
-  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long g = xcd_thread();
   if (g >= off[nb]) return;
   const int bi = find_block(off, nb, g);
   const BlockDev& b = blocks[bi];
@@ -96,7 +96,7 @@ __global__ void __launch_bounds__(256) metric_a_kernel(const BlockDev* __restric
 
 __global__ void __launch_bounds__(256) metric_gcl_kernel(const BlockDev* __restrict__ blocks,
                                                          const long long* __restrict__ off, int nb) {
-  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long g = xcd_thread();
   if (g >= off[nb]) return;
   const int bi = find_block(off, nb, g);
   const BlockDev& b = blocks[bi];
@@ -128,7 +128,7 @@ __global__ void __launch_bounds__(256) metric_gcl_kernel(const BlockDev* __restr
 __global__ void __launch_bounds__(256) small_vol_kernel(const BlockDev* __restrict__ blocks,
                                                         const long long* __restrict__ off, int nb,
                                                         unsigned long long* count) {
-  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long g = xcd_thread();
   if (g >= off[nb]) return;
   const int bi = find_block(off, nb, g);
   const BlockDev& b = blocks[bi];

@@ -22,7 +22,7 @@ __global__ void __launch_bounds__(256) state_kernel(const BlockDev* __restrict__
                                                     const long long* __restrict__ off, int nb, Params p, int mode,
                                                     unsigned long long* dtmin_base, const int* step) {
   __shared__ double smin[256];
-  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long g = xcd_thread();
   double mydt = 1000.0;
   if (g < off[nb]) {
     const int bi = find_block(off, nb, g);
@@ -116,7 +116,7 @@ __global__ void __launch_bounds__(256) state_kernel(const BlockDev* __restrict__
 __global__ void __launch_bounds__(256) update_kernel(const BlockDev* __restrict__ blocks,
                                                      const long long* __restrict__ off, int nb, LuPacked lp,
                                                      double gama) {
-  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long g = xcd_thread();
   if (g >= off[nb]) return;
   const int bi = find_block(off, nb, g);
   const BlockDev& b = blocks[bi];
@@ -273,7 +273,7 @@ __global__ void step_advance_kernel(int* step) {
 }
 
 __global__ void fill_kernel(double* p, long long n, double v) {
-  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long g = xcd_thread();
   if (g < n) p[g] = v;
 }
 
@@ -281,7 +281,7 @@ __global__ void fill_kernel(double* p, long long n, double v) {
 __global__ void __launch_bounds__(256) temperature_kernel(const BlockDev* __restrict__ blocks,
                                                           const long long* __restrict__ off, int nb, double gama,
                                                           double moo) {
-  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long g = xcd_thread();
   if (g >= off[nb]) return;
   const int bi = find_block(off, nb, g);
   const BlockDev& b = blocks[bi];

@@ -62,6 +62,19 @@ __host__ __device__ inline long long pidx(const BlockDev& b, int i, int j, int k
   return (long long)i + (long long)b.nx * ((long long)j + (long long)b.ny * (long long)k);
 }
 
+// Workgroups of a launch are dispatched to the device's 8 XCDs round-robin
+// (workgroup w runs on XCD w % 8) and every XCD has its own L2.  xcd_block()
+// renumbers them so that each XCD walks one contiguous range of the
+// iteration space: neighbouring tiles, which share stencil data, then meet in
+// the same L2.  It is a bijection of [0, gridDim.x).
+__host__ __device__ inline unsigned xcd_remap(unsigned b, unsigned n) {
+  const unsigned per = n >> 3, rem = n & 7u, x = b & 7u, q = b >> 3;
+  return x < rem ? x * (per + 1) + q : rem * (per + 1) + (x - rem) * per + q;
+}
+__device__ inline long long xcd_thread() {
+  return (long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+}
+
 // block lookup: largest b with off[b] <= g (off has nb+1 entries)
 __device__ inline int find_block(const long long* __restrict__ off, int nb, long long g) {
   int lo = 0, hi = nb - 1;
