@@ -36,10 +36,10 @@ HBM_PEAK_GBS = 8000.0
 STAGES = {
     "rhs": (20, ["visc_deriv", "visc_face", "visc_diff", "inv_face", "inv_closure", "inv_diff"],
             "visc_deriv_kernel+visc_face_kernel+visc_diff_kernel+inv_face_kernel+inv_closure_kernel+inv_diff_kernel"),
-    "lu_fwd": (25, ["lu_prep", "lu_fwd"], "lu_coef_kernel+lusgs_sweep_kernel<0>"),
-    "lu_bwd": (25, ["lu_bwd"], "lusgs_sweep_kernel<1>"),
-    "gs_fwd": (30, ["gs_fwd"], "lusgs_sweep_kernel<2>"),
-    "gs_bwd": (30, ["gs_bwd"], "lusgs_sweep_kernel<3>"),
+    "lu_fwd": (25, ["lu_pack", "lu_fwd"], "lu_pack_kernel+lusgs_tile_kernel<0>"),
+    "lu_bwd": (25, ["lu_bwd"], "lusgs_tile_kernel<1>"),
+    "gs_fwd": (30, ["gs_fwd"], "lusgs_tile_kernel<2>"),
+    "gs_bwd": (30, ["gs_bwd"], "lusgs_tile_kernel<3>"),
     "update": (25, ["update", "residual"], "update_kernel+residual_kernel+residual_final_kernel"),
 }
 STEP_WORDS = sum(v[0] for v in STAGES.values())

@@ -59,10 +59,16 @@ struct orion_hip_ctx {
   char* arena = nullptr;
   size_t arena_bytes = 0;
   size_t map_bytes = 0;
-  Space pad, intr, face[3], line[3], plane;
-  int max_planes = 0;
-  LuPacked lu{};
-  std::vector<long long> plane_start;  // max_planes+1
+  Space pad, intr, face[3], line[3];
+  SweepArrays sw{};                    // LU-SGS working set in sweep layout
+  std::vector<SweepTile> tiles;
+  SweepTile* d_tiles = nullptr;
+  int* d_order[2] = {nullptr, nullptr};  // tile launch order: forward, backward sweeps
+  unsigned long long* d_flags = nullptr; // per-tile progress
+  unsigned long long* d_epoch = nullptr; // sweep counter
+  int* d_err = nullptr;                  // set by a sweep whose wait ran out
+  int sweep_pub = 1;                     // steps between progress publications
+  int sweep_sync = 2;                    // cross-tile visibility scheme (k_lusgs.hip)
   long long* d_halo = nullptr;
   long long n_halo = 0;
   BCTask* d_bc = nullptr;
@@ -438,57 +444,77 @@ static int build_topology(orion_hip_ctx* c, const std::vector<std::vector<Reg>>&
   return 0;
 }
 
-// Plane-major numbering of interior cells for the LU-SGS sweeps: plane s of
-// every block in block order, within a block k then j (i = s - j - k).
-static int build_planes(orion_hip_ctx* c) {
-  const int nb = c->nb;
-  long long ncell = c->intr.total();
-  std::vector<std::vector<int>> pos(nb);
-  for (int b = 0; b < nb; ++b) pos[b].assign(c->hb[b].P, -1);
-  c->plane_start.assign(c->max_planes + 1, 0);
-  long long q = 0;
-  for (int s = 0; s < c->max_planes; ++s) {
-    c->plane_start[s] = q;
-    for (int b = 0; b < nb; ++b) {
-      const BlockDev& B = c->hb[b];
-      const int ni = B.idim, nj = B.jdim, nk = B.kdim;
-      const int k0 = std::max(0, s - (ni - 1) - (nj - 1)), k1 = std::min(nk - 1, s);
-      for (int k = k0; k <= k1; ++k) {
-        const int j0 = std::max(0, s - k - (ni - 1)), j1 = std::min(nj - 1, s - k);
-        for (int j = j0; j <= j1; ++j) {
-          const int i = s - j - k;
-          pos[b][pidx(B, i + NG, j + NG, k + NG)] = (int)q++;
-        }
-      }
+// LU-SGS sweep layout (k_lusgs.hip): per block the march axis is the longest
+// one and col is i whenever i is not the march axis; the (col, row)
+// cross-section is cut into 16 x 16 tiles.  Tiles are launched so that every
+// tile's upstream neighbours come first: by increasing (row tile + col tile)
+// for the forward sweeps, by decreasing for the backward ones.
+static int build_sweep(orion_hip_ctx* c) {
+  const int TWD = 16;
+  long long n = 0;
+  c->tiles.clear();
+  for (int b = 0; b < c->nb; ++b) {
+    BlockDev& B = c->hb[b];
+    const int d[3] = {B.idim, B.jdim, B.kdim};
+    int am = 0;
+    for (int a = 1; a < 3; ++a)
+      if (d[a] > d[am]) am = a;
+    int ac, ar;
+    if (am != 0) {
+      ac = 0;
+      ar = 3 - am;
+    } else {
+      ac = d[1] >= d[2] ? 1 : 2;
+      ar = 3 - ac;
     }
+    B.sax[0] = am; B.sax[1] = ac; B.sax[2] = ar;
+    B.sn[0] = d[am]; B.sn[1] = d[ac]; B.sn[2] = d[ar];
+    B.sncp = (d[ac] + TWD - 1) / TWD * TWD;
+    B.sbase = n;
+    const long long planes = (long long)d[0] + d[1] + d[2] - 2;
+    if (planes >= 60000) return fail("block %d: %lld hyperplanes exceed the sweep limit", b + 1, planes);
+    n += planes * B.sn[2] * B.sncp;
+    const int tc = (B.sn[1] + TWD - 1) / TWD, tr = (B.sn[2] + TWD - 1) / TWD;
+    const int first = (int)c->tiles.size();
+    for (int R = 0; R < tr; ++R)
+      for (int C = 0; C < tc; ++C) {
+        SweepTile t{};
+        t.block = b;
+        t.r0 = R * TWD;
+        t.c0 = C * TWD;
+        const int rhi = std::min(t.r0 + TWD - 1, B.sn[2] - 1), chi = std::min(t.c0 + TWD - 1, B.sn[1] - 1);
+        t.s_first = t.r0 + t.c0;
+        t.s_last = rhi + chi + B.sn[0] - 1;
+        t.nbr[0] = C > 0 ? first + R * tc + C - 1 : -1;
+        t.nbr[1] = R > 0 ? first + (R - 1) * tc + C : -1;
+        t.nbr[2] = C + 1 < tc ? first + R * tc + C + 1 : -1;
+        t.nbr[3] = R + 1 < tr ? first + (R + 1) * tc + C : -1;
+        c->tiles.push_back(t);
+      }
   }
-  c->plane_start[c->max_planes] = q;
-  if (q != ncell) return fail("plane numbering covers %lld of %lld cells", q, ncell);
-  if (ncell >= (1LL << 31)) return fail("too many cells for 32-bit plane indices");
-  std::vector<int> nbr(6 * ncell, -1);
-  for (int b = 0; b < nb; ++b) {
-    const BlockDev& B = c->hb[b];
-    const long long st[3] = {1, B.nx, (long long)B.nx * B.ny};
-    for (int k = NG; k < NG + B.kdim; ++k)
-      for (int j = NG; j < NG + B.jdim; ++j)
-        for (int i = NG; i < NG + B.idim; ++i) {
-          const long long id = pidx(B, i, j, k);
-          const int me = pos[b][id];
-          const int cc[3] = {i, j, k}, lim[3] = {B.idim, B.jdim, B.kdim};
-          for (int ax = 0; ax < 3; ++ax) {
-            if (cc[ax] - NG > 0) nbr[(long long)ax * ncell + me] = pos[b][id - st[ax]];
-            if (cc[ax] - NG + 1 < lim[ax]) nbr[(long long)(3 + ax) * ncell + me] = pos[b][id + st[ax]];
-          }
-        }
-    HIPCHK(hipMemcpy(B.lpos, pos[b].data(), sizeof(int) * B.P, hipMemcpyHostToDevice));
-  }
-  c->lu.ncell = ncell;
-  if (dalloc(c, (void**)&c->lu.coef, sizeof(double) * 40 * ncell) ||
-      dalloc(c, (void**)&c->lu.rhs, sizeof(double) * 5 * ncell) ||
-      dalloc(c, (void**)&c->lu.src, sizeof(double) * 6 * ncell) ||
-      dalloc(c, (void**)&c->lu.dq, sizeof(double) * 5 * ncell))
+  n = (n + 31) / 32 * 32;
+  const int nt = (int)c->tiles.size();
+  std::vector<int> ord(nt);
+  for (int i = 0; i < nt; ++i) ord[i] = i;
+  auto diag = [&](int i) { return c->tiles[i].r0 / TWD + c->tiles[i].c0 / TWD; };
+  std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return diag(a) < diag(b); });
+  if (upload_vec(c, &c->d_order[0], ord)) return -1;
+  std::reverse(ord.begin(), ord.end());
+  if (upload_vec(c, &c->d_order[1], ord)) return -1;
+  if (upload_vec(c, &c->d_tiles, c->tiles)) return -1;
+  c->sw.n = n;
+  if (dalloc(c, (void**)&c->sw.met, sizeof(double) * 9 * n) || dalloc(c, (void**)&c->sw.D, sizeof(double) * 8 * n) ||
+      dalloc(c, (void**)&c->sw.coed, sizeof(double) * n) || dalloc(c, (void**)&c->sw.rhs, sizeof(double) * 5 * n) ||
+      dalloc(c, (void**)&c->sw.dq, sizeof(double) * 5 * n) ||
+      dalloc(c, (void**)&c->d_flags, sizeof(unsigned long long) * std::max(nt, 1)) ||
+      dalloc(c, (void**)&c->d_epoch, sizeof(unsigned long long)) || dalloc(c, (void**)&c->d_err, sizeof(int)))
     return -1;
-  if (upload_vec(c, &c->lu.nbr, nbr)) return -1;
+  HIPCHK(hipMemset(c->sw.dq, 0, sizeof(double) * 5 * n));
+  HIPCHK(hipMemset(c->d_flags, 0, sizeof(unsigned long long) * std::max(nt, 1)));
+  HIPCHK(hipMemset(c->d_epoch, 0, sizeof(unsigned long long)));
+  HIPCHK(hipMemset(c->d_err, 0, sizeof(int)));
+  if (const char* e = std::getenv("ORION_LU_PUBLISH")) c->sweep_pub = std::max(1, std::atoi(e));
+  if (const char* e = std::getenv("ORION_LU_SYNC")) c->sweep_sync = std::min(2, std::max(0, std::atoi(e)));
   return 0;
 }
 
@@ -586,7 +612,6 @@ int orion_hip_ctx_create(int nblocks, const int* dims, const double* xyz, int nr
       B.fcov_off[0] = 0;
       B.fcov_off[1] = 30 * nlines(B, 0);
       B.fcov_off[2] = 30 * (nlines(B, 0) + nlines(B, 1));
-      B.lpos = (int*)a; a += al((P + 1) / 2);
       B.frozen = (unsigned char*)a; a += al((P + 7) / 8);
     }
   }
@@ -641,7 +666,7 @@ int orion_hip_ctx_create(int nblocks, const int* dims, const double* xyz, int nr
         if (tw >= (int)R[tb].size()) return bail(fail("block %d: target window out of range", b + 1));
       }
   // tables
-  std::vector<long long> cp(nblocks), ci(nblocks), cf[3], cl[3], cpl(nblocks);
+  std::vector<long long> cp(nblocks), ci(nblocks), cf[3], cl[3];
   for (int d = 0; d < 3; ++d) {
     cf[d].resize(nblocks);
     cl[d].resize(nblocks);
@@ -654,15 +679,13 @@ int orion_hip_ctx_create(int nblocks, const int* dims, const double* xyz, int nr
       cl[d][b] = nlines(B, d);
       cf[d][b] = nlines(B, d) * (dcount(B, d) + 1);
     }
-    cpl[b] = (long long)B.jdim * B.kdim;
-    c->max_planes = std::max(c->max_planes, B.idim + B.jdim + B.kdim - 2);
   }
-  if (make_space(c, c->pad, cp) || make_space(c, c->intr, ci) || make_space(c, c->plane, cpl)) return bail(-1);
+  if (make_space(c, c->pad, cp) || make_space(c, c->intr, ci)) return bail(-1);
   for (int d = 0; d < 3; ++d)
     if (make_space(c, c->face[d], cf[d]) || make_space(c, c->line[d], cl[d])) return bail(-1);
+  if (build_sweep(c)) return bail(-1);
   if (upload_vec(c, &c->db, c->hb)) return bail(-1);
   if (build_topology(c, R)) return bail(-1);
-  if (build_planes(c)) return bail(-1);
   c->nparts = residual_parts(c->intr.total());
   if (dalloc(c, (void**)&c->d_part, sizeof(double) * 2 * 1024) || dalloc(c, (void**)&c->d_pidx, sizeof(long long) * 1024) ||
       dalloc(c, (void**)&c->d_count, sizeof(unsigned long long) * 2) || dalloc(c, (void**)&c->d_step, sizeof(int)))
@@ -705,6 +728,7 @@ int orion_hip_ctx_create(int nblocks, const int* dims, const double* xyz, int nr
       return bail(fail("metric check failed: %s", hipGetErrorString(hipGetLastError())));
     if (bad) return bail(fail("ERROR(Jacobi<0): %llu points with volume below 1e-15 (MetricComputer.cpp:488-491)", bad));
   }
+  launch_lu_geom(LA(c, c->intr), c->intr.total(), c->sw);
   if (p.nvis != 0) launch_temperature(LA(c, c->pad), c->pad.total(), p.gama, p.moo);
   if (hipStreamSynchronize(c->stream) != hipSuccess) return bail(fail("setup failed: %s", hipGetErrorString(hipGetLastError())));
   *out = c;
@@ -796,18 +820,24 @@ static void op_inv(orion_hip_ctx* c) {
 }
 
 static void op_sweep(orion_hip_ctx* c, int type) {
-  const std::vector<long long>& S = c->plane_start;
-  if (type == 0 || type == 2)
-    for (int s = 0; s < c->max_planes; ++s) launch_lusgs_sweep(c->lu, S[s], S[s + 1] - S[s], type, c->p, c->stream);
-  else
-    for (int s = c->max_planes - 1; s >= 0; --s)
-      launch_lusgs_sweep(c->lu, S[s], S[s + 1] - S[s], type, c->p, c->stream);
+  const int bwd = (type == 1 || type == 3) ? 1 : 0;
+  launch_lusgs_tiles(c->sw, c->db, c->d_tiles, c->d_order[bwd], (int)c->tiles.size(), c->d_flags, c->d_epoch,
+                     c->d_err, c->sweep_pub, c->sweep_sync, type, c->p, c->stream);
 }
 
 static void op_update(orion_hip_ctx* c) {
-  launch_lu_coef(LA(c, c->intr), c->intr.total(), c->lu, c->p);
+  launch_lu_pack(LA(c, c->intr), c->intr.total(), c->sw, c->p);
   for (int t = 0; t < 4; ++t) op_sweep(c, t);
-  launch_update(LA(c, c->intr), c->intr.total(), c->lu, c->p.gama);
+  launch_update(LA(c, c->intr), c->intr.total(), c->sw, c->p.gama);
+}
+
+// a sweep whose wait for an upstream tile ran out has left wrong values
+static int check_sweeps(orion_hip_ctx* c) {
+  int e = 0;
+  HIPCHK(hipMemcpyAsync(&e, c->d_err, sizeof e, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (e) return fail("LU-SGS sweep: a tile waited past its bound for an upstream tile; results are invalid");
+  return 0;
 }
 
 static int ensure_res(orion_hip_ctx* c, int n) {
@@ -896,6 +926,7 @@ int orion_hip_residual(orion_hip_ctx* c, orion_residual_stats* out) {
   HIPCHK(hipMemcpyAsync(r, c->d_res, sizeof r, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipMemcpyAsync(&bits, c->d_dtmin, sizeof bits, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
+  if (check_sweeps(c)) return -1;
   if (out) decode_stats(c, r, bits, out);
   return 0;
 }
@@ -958,7 +989,7 @@ int orion_hip_run(orion_hip_ctx* c, int nsteps, orion_residual_stats* hist) {
   } else {
     for (int s = 0; s < nsteps; ++s) step_body(c);
   }
-  if (check_launch("run")) return -1;
+  if (check_launch("run") || check_sweeps(c)) return -1;
   if (hist) {
     std::vector<double> r(4 * (size_t)nsteps);
     std::vector<unsigned long long> bits(nsteps);
@@ -978,7 +1009,6 @@ int orion_hip_profile_run(orion_hip_ctx* c, int nsteps, double* phase_ms, long l
   for (int i = 0; i <= NP; ++i) HIPCHK(hipEventCreate(&ev[i]));
   std::vector<double> acc(NP, 0.0);
   std::vector<long long> nl(NP, 0);
-  const long long planes = c->max_planes;
   for (int s = 0; s < nsteps; ++s) {
     int k = 0;
     auto mark = [&](long long launches) {
@@ -1007,9 +1037,9 @@ int orion_hip_profile_run(orion_hip_ctx* c, int nsteps, double* phase_ms, long l
     mark(3);
     launch_inv_diff(LA(c, c->intr), c->intr.total()); mark(1);
     op_avg(c); mark(2);
-    launch_lu_coef(LA(c, c->intr), c->intr.total(), c->lu, c->p); mark(1);
-    for (int t = 0; t < 4; ++t) { op_sweep(c, t); mark(planes); }
-    launch_update(LA(c, c->intr), c->intr.total(), c->lu, c->p.gama); mark(1);
+    launch_lu_pack(LA(c, c->intr), c->intr.total(), c->sw, c->p); mark(1);
+    for (int t = 0; t < 4; ++t) { op_sweep(c, t); mark(2); }
+    launch_update(LA(c, c->intr), c->intr.total(), c->sw, c->p.gama); mark(1);
     launch_residual(LA(c, c->intr), c->intr.total(), 5 * c->intr.total(), c->d_part, c->d_pidx, c->nparts,
                     c->d_res + 4 * s, nullptr);
     mark(2);
@@ -1022,6 +1052,7 @@ int orion_hip_profile_run(orion_hip_ctx* c, int nsteps, double* phase_ms, long l
     }
   }
   for (int i = 0; i <= NP; ++i) hipEventDestroy(ev[i]);
+  if (check_sweeps(c)) return -1;
   for (int i = 0; i < NP; ++i) {
     if (phase_ms) phase_ms[i] = acc[i];
     if (phase_launches) phase_launches[i] = nl[i];
@@ -1031,7 +1062,7 @@ int orion_hip_profile_run(orion_hip_ctx* c, int nsteps, double* phase_ms, long l
 
 int orion_hip_synchronize(orion_hip_ctx* c) {
   HIPCHK(hipStreamSynchronize(c->stream));
-  return 0;
+  return check_sweeps(c);
 }
 
 int orion_hip_set_stream(orion_hip_ctx* c, void* s) {

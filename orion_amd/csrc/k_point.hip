@@ -7,9 +7,6 @@
 //                 dq=0, then the spectral radii and dt/vol on interior points
 //                 (the only points whose radii the solver reads) and the
 //                 minimum physical time step.
-//   update        copy the swept increments back from the plane-major
-//                 working set, zero the BC windows again and apply
-//                 update_limited (StateUpdater.cpp:470-542, 577-578).
 //   residual      ResidualMonitor::compute (src/solver/ResidualMonitor.cpp:11-138):
 //                 deterministic two-pass reduction of sum r^2 and of the
 //                 first maximum in (block,k,j,i,var) order.
@@ -111,60 +108,6 @@ __global__ void __launch_bounds__(256) state_kernel(const BlockDev* __restrict__
   }
   // positive doubles order like their bit patterns
   if (threadIdx.x == 0 && smin[0] < 1000.0) atomicMin(dtmin_bits, (unsigned long long)__double_as_longlong(smin[0]));
-}
-
-__global__ void __launch_bounds__(256) update_kernel(const BlockDev* __restrict__ blocks,
-                                                     const long long* __restrict__ off, int nb, LuPacked lp,
-                                                     double gama) {
-  const long long g = xcd_thread();
-  if (g >= off[nb]) return;
-  const int bi = find_block(off, nb, g);
-  const BlockDev& b = blocks[bi];
-  int i, j, k;
-  decode_interior(b, g - off[bi], &i, &j, &k);
-  const long long id = pidx(b, i, j, k);
-  const long long P = b.P;
-  double d[5];
-  if (b.frozen[id]) {
-#pragma unroll
-    for (int m = 0; m < 5; ++m) {
-      d[m] = 0.0;
-      b.dq[m * P + id] = 0.0;
-    }
-  } else {
-    const long long q = b.lpos[id];
-#pragma unroll
-    for (int m = 0; m < 5; ++m) {
-      d[m] = lp.dq[m * lp.ncell + q];
-      b.dq[m * P + id] = d[m];
-    }
-  }
-  const double gm1 = gama - 1.0;
-  const double r = b.prim[id], u = b.prim[P + id], v = b.prim[2 * P + id], w = b.prim[3 * P + id];
-  const double pp = b.prim[4 * P + id];
-  const double E = pp / gm1 + 0.5 * r * (u * u + v * v + w * w);
-  const double q0 = r + d[0], q1 = r * u + d[1], q2 = r * v + d[2], q3 = r * w + d[3], q4 = E + d[4];
-  double rn = q0;
-  double ir = 1.0 / rn;
-  const double a = q1 * ir, bb = q2 * ir, cc = q3 * ir;
-  const double pn = gm1 * (q4 - 0.5 * rn * (a * a + bb * bb + cc * cc));
-  const double dpp = fabs((pn - pp) / pp);
-  const double x = -0.3 + dpp;
-  const double term = (0.0 < x) ? x : 0.0;  // std::max(0.0, x)
-  const double Fa = 1.0 / (1.0 + 2.0 * term);
-  const double f0 = r + d[0] * Fa, f1 = r * u + d[1] * Fa, f2 = r * v + d[2] * Fa, f3 = r * w + d[3] * Fa;
-  const double f4 = E + d[4] * Fa;
-  rn = f0;
-  ir = 1.0 / rn;
-  const double un = f1 * ir, vn = f2 * ir, wn = f3 * ir;
-  const double pf = gm1 * (f4 - 0.5 * rn * (un * un + vn * vn + wn * wn));
-  if (!(rn <= 1e-6 || pf <= 1e-6)) {
-    b.prim[id] = rn;
-    b.prim[P + id] = un;
-    b.prim[2 * P + id] = vn;
-    b.prim[3 * P + id] = wn;
-    b.prim[4 * P + id] = pf;
-  }
 }
 
 // ---------------------------------------------------------------------------
@@ -303,10 +246,6 @@ void launch_state(const Launch& L, long long n, const Params& p, int mode, unsig
 }
 void launch_step_advance(int* step, hipStream_t s) {
   hipLaunchKernelGGL(step_advance_kernel, dim3(1), dim3(64), 0, s, step);
-}
-void launch_update(const Launch& L, long long n, const LuPacked& lp, double gamma) {
-  if (n <= 0) return;
-  hipLaunchKernelGGL(update_kernel, dim3(grid_of(n, 256)), dim3(256), 0, L.stream, L.blocks, L.off, L.nb, lp, gamma);
 }
 void launch_residual(const Launch& L, long long n, long long total5, double* partial, long long* pidx_, int nparts,
                      double* out, const int* step) {

@@ -37,20 +37,41 @@ struct BlockDev {
   double* duvwt;         // [12][P] viscous cell derivatives (scratch)
   double* fx;            // [3][5][P] face fluxes per direction (scratch)
   double* fcov;          // closure overrides, [3][lines][6][5] (scratch)
-  int* lpos;             // [P] plane-major LU-SGS index of each interior point
   unsigned char* frozen; // [P] 1 inside a physical-BC window
   long long fcov_off[3]; // offset (doubles) of each direction's overrides
+  // LU-SGS sweep layout of this block: axes (march, col, row) as real axes
+  // 0=i,1=j,2=k, their interior sizes, the padded column count and the
+  // block's first cell in the sweep arrays.
+  int sax[3];
+  int sn[3];
+  int sncp;
+  long long sbase;
 };
 
-// Plane-major working set of the LU-SGS sweeps (all blocks, ncell cells).
-struct LuPacked {
-  double* coef;  // [40][ncell] dq-independent coefficients
-  double* rhs;   // [5][ncell]
-  double* src;   // [6][ncell] u,v,w,c2,H,af of each cell
-  double* dq;    // [5][ncell]
-  int* nbr;      // [6][ncell] neighbour cell or -1 (i-1,j-1,k-1,i+1,j+1,k+1)
-  long long ncell;
+// LU-SGS working set in sweep layout (all blocks, n cells incl. padding).
+// A block's cell (m, r, c) along (march, row, col) lies on hyperplane
+// s = m + r + c and is stored at sbase + (s * nrow + r) * ncol_padded + c:
+// the cells of one hyperplane row are contiguous.
+struct SweepArrays {
+  double* met;   // [9][n] metrics
+  double* D;     // [8][n] u, v, w, c2, H, af, c, 2mu/(|Re| rho vol + 1e-30)
+  double* coed;  // [n] diagonal 1/(1/dt + sum of radii)
+  double* rhs;   // [5][n] right-hand side (zero on physical-BC windows)
+  double* dq;    // [5][n] increment, updated in place by the sweeps
+  long long n;
 };
+
+// One tile of the LU-SGS pipeline: 16 x 16 (row, col) pencils of a block.
+struct SweepTile {
+  int block;
+  int r0, c0;           // first row and column
+  int s_first, s_last;  // hyperplanes holding cells of the tile
+  int nbr[4];           // tiles across the col-1, row-1, col+1, row+1 edges (-1: none)
+};
+
+__host__ __device__ inline long long sweep_index(const BlockDev& b, int m, int r, int c) {
+  return b.sbase + ((long long)(m + r + c) * b.sn[2] + r) * b.sncp + c;
+}
 
 struct Params {
   double gama, prl, moo, reynolds, visc, tref, twall, cfl;

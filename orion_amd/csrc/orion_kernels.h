@@ -46,7 +46,6 @@ void launch_visc_diff(const Launch& L, long long n, double re_inv);
 void launch_state(const Launch& L, long long n, const Params& p, int mode, unsigned long long* dtmin_base,
                   const int* step);
 void launch_step_advance(int* step, hipStream_t s);
-void launch_update(const Launch& L, long long n, const LuPacked& lp, double gamma);
 void launch_residual(const Launch& L, long long n, long long total5, double* partial, long long* pidx,
                      int nparts, double* out, const int* step);
 int residual_parts(long long n);
@@ -61,9 +60,12 @@ void launch_avg_stage(const BlockDev* blocks, const AvgSrc* src, long long n, do
 void launch_avg_apply(const BlockDev* blocks, const AvgDst* dst, long long n, const double* stage, hipStream_t s);
 
 // k_lusgs.hip
-void launch_lu_coef(const Launch& La, long long n, const LuPacked& L, const Params& p);
-void launch_lusgs_sweep(const LuPacked& L, long long start, long long count, int type, const Params& p,
-                        hipStream_t s);
+void launch_lu_geom(const Launch& La, long long n, const SweepArrays& S);
+void launch_lu_pack(const Launch& La, long long n, const SweepArrays& S, const Params& p);
+void launch_lusgs_tiles(const SweepArrays& S, const BlockDev* blocks, const SweepTile* tiles, const int* order,
+                        int ntiles, unsigned long long* flags, unsigned long long* epoch, int* err, int pub, int sync,
+                        int type, const Params& p, hipStream_t s);
+void launch_update(const Launch& La, long long n, const SweepArrays& S, double gamma);
 
 // k_metrics.hip
 void launch_metrics(const Launch& L, long long n, int gcl, double* tmp, hipStream_t s);
