@@ -16,9 +16,13 @@
  *     unit-lattice volume, BC layer-3 sign flips, state closed forms,
  *     dt closed form, viscous u=y^2 exactness, halo and interface-average
  *     fingerprints (tests/test_oracle_kat.py);
- *   - mathematics: WCNS-E-5 exactness on quadratics, 6th-order difference
- *     exactness on quintics, Roe consistency F(q,q)=F(q), freestream
- *     preservation, lexicographic LU-SGS against brute force on tiny blocks.
+ *   - mathematics (tests/test_oracle_kat.py): the inviscid RHS is exact
+ *     when every flux component is quadratic (WCNS-E-5 exact on quadratics,
+ *     Roe dissipation zero for equal states, exact 6th-order difference),
+ *     metrics exact under quadratic stretching, freestream preservation
+ *     with the GCL metrics, LU-SGS gives dq = 0 for a zero right-hand side.
+ *     The LU-SGS sweep order and the stage order are pinned only by the
+ *     stage-by-stage comparison with the reference build.
  */
 #include "orion_oracle.h"
 

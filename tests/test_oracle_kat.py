@@ -240,3 +240,34 @@ def test_lusgs_zero_rhs():
     P = o.field(0, "dq").size // 5
     o.lusgs_sweeps(0, np.zeros(5 * P))
     assert np.all(o.field(0, "dq")[:, NG:-NG, NG:-NG, NG:-NG] == 0.0)
+
+
+def test_inviscid_quadratic_exact():
+    """WCNS-E-5 + Roe + 6th-order difference (InviscidFluxComputer.cpp:197-
+    315) on the unit lattice with rho quadratic along i and u, p constant:
+    every flux component is then quadratic in i, each candidate stencil of
+    the reconstruction is exact (so any weights summing to one give the
+    exact face state), left and right states agree so the Roe dissipation
+    vanishes, and the difference is exact -- the interior RHS must equal
+    dF/dxi.  A wrong weight normalisation, interpolation or difference
+    coefficient, or a dissipation term that does not vanish for equal
+    states, breaks it."""
+    ni, nj, nk = 24, 9, 9
+    o = OracleCase(unit_lattice_case(ni, nj, nk), apply_init=False)
+    prim = o.field(0, "prim")
+    nx = prim.shape[3]
+    i = np.arange(nx, dtype=float) - NG
+    a, b, u, p = 0.02, 0.003, 0.3, 0.7
+    rho = 1.0 + a * i + b * i * i
+    prim[0] = rho[None, None, :]
+    prim[1], prim[2], prim[3], prim[4], prim[5] = u, 0.0, 0.0, p, 1.0
+    o.op("update_flow_states")
+    o.op("compute_inviscid_rhs")
+    dq = o.field(0, "dq")
+    drho = a + 2.0 * b * i
+    want = [u * drho, u * u * drho, 0 * drho, 0 * drho, 0.5 * u ** 3 * drho]
+    sl = (slice(NG + 4, -NG - 4), slice(NG + 4, -NG - 4), slice(NG + 5, -NG - 5))
+    for m in range(5):
+        got = dq[m][sl]
+        exp = np.broadcast_to(want[m][sl[2]][None, None, :], got.shape)
+        assert np.abs(got - exp).max() < 1e-12, (m, np.abs(got - exp).max())
