@@ -21,6 +21,10 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(BUILD, "liborion_hip.so")
 SOURCES = ["context.hip", "k_flux.hip", "k_point.hip", "k_bc.hip", "k_lusgs.hip", "k_metrics.hip"]
+HOST = os.path.join(HERE, "host")
+HOST_SOURCES = ["orion_host.cpp", "orion_run.cpp"]
+RUNNER = os.path.join(BUILD, "orion_run")
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 ARCH = os.environ.get("ORION_OFFLOAD_ARCH", "gfx950")
 
 
@@ -60,7 +64,26 @@ def build(verbose: bool = False, jobs: int = 6) -> str:
     if r.returncode != 0:
         raise RuntimeError("link failed:\n%s%s" % (r.stdout, r.stderr))
     os.replace(tmp, LIB)
+    build_host(verbose)
     return LIB
+
+
+def build_host(verbose: bool = False) -> str:
+    """The C++ host mirror (orion_amd/host) and the orion_run executable,
+    compiled by the host C++ compiler against the C ABI only."""
+    cxx = os.environ.get("CXX", "g++")
+    tmp = RUNNER + ".tmp"
+    cmd = [cxx, "-O2", "-std=c++17", "-Wall", "-Wextra", "-I" + os.path.join(os.path.dirname(HERE), "include")]
+    cmd += [os.path.join(HOST, s) for s in HOST_SOURCES]
+    cmd += ["-o", tmp, "-L" + BUILD, "-lorion_hip", "-Wl,-rpath,$ORIGIN", "-L" + os.path.join(ROCM, "lib"),
+            "-Wl,-rpath-link," + os.path.join(ROCM, "lib")]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("host build failed:\n%s%s" % (r.stdout, r.stderr[-20000:]))
+    os.replace(tmp, RUNNER)
+    return RUNNER
 
 
 if __name__ == "__main__":

@@ -247,7 +247,7 @@ static long long arena_fields(const BlockDev& b, int keep_q, long long lines_tot
   auto al = [](long long x) { return (x + 31) / 32 * 32; };  // 256-byte alignment
   n += al(6 * P) + al(9 * P) + al(P) + al(5 * P) + al(5 * P) + al(P) + al(P) + al(P);
   if (keep_q) n += al(5 * P);
-  n += al(12 * P) + al(15 * P) + al(30 * lines_total) + al((P + 1) / 2) + al((P + 7) / 8);
+  n += al(12 * P) + al(15 * P) + al(30 * lines_total) + al((P + 7) / 8);
   return n;
 }
 
