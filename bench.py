@@ -269,6 +269,7 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": case.name, "blocks": case.nblocks, "interior_nodes": N,
+                   "device_bytes": s.fs.device_bytes(),
                    "scheme": "WCNS-E-5+Roe, 6th-order viscous, LU-SGS+2 SGS sweeps", "precision": "fp64",
                    "moo": case.inputs["moo"], "reynolds": case.inputs["reynolds"], "cfl": case.inputs["cfl"]},
         "roofline": roof,

@@ -45,6 +45,9 @@ static int fail(const char* fmt, ...) {
 struct Space {
   std::vector<long long> h;  // prefix offsets (nb+1)
   long long* d = nullptr;
+  int* wgb = nullptr;        // block of each workgroup of the padded space
+  int* wg0 = nullptr;        // first workgroup of each block
+  long long nwg = 0;
   long long total() const { return h.back(); }
 };
 
@@ -114,10 +117,20 @@ static int upload_vec(orion_hip_ctx* c, T** dst, const std::vector<T>& v) {
 static int make_space(orion_hip_ctx* c, Space& s, const std::vector<long long>& counts) {
   s.h.assign(counts.size() + 1, 0);
   for (size_t b = 0; b < counts.size(); ++b) s.h[b + 1] = s.h[b] + counts[b];
-  return upload_vec(c, &s.d, s.h);
+  std::vector<int> wgb, wg0(counts.size());
+  for (size_t b = 0; b < counts.size(); ++b) {
+    wg0[b] = (int)wgb.size();
+    const long long n = (counts[b] + SPAN_TB - 1) / SPAN_TB;
+    for (long long w = 0; w < n; ++w) wgb.push_back((int)b);
+  }
+  if (wgb.size() >= (size_t)INT32_MAX) return fail("iteration space too large");
+  s.nwg = (long long)wgb.size();
+  return upload_vec(c, &s.d, s.h) || upload_vec(c, &s.wgb, wgb) || upload_vec(c, &s.wg0, wg0);
 }
 
-static Launch LA(orion_hip_ctx* c, const Space& s) { return Launch{c->stream, c->db, s.d, c->nb}; }
+static Launch LA(orion_hip_ctx* c, const Space& s) {
+  return Launch{c->stream, c->db, s.d, c->nb, Span{s.d, s.wgb, s.wg0, c->nb}, s.nwg};
+}
 
 // ---------------------------------------------------------------------------
 // region preprocessing (BCReader.cpp:125-143, BCPreprocess.cpp:16-192)

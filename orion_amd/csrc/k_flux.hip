@@ -207,15 +207,15 @@ __device__ inline void face_flux(const LineRef& L, int n, int f, const double* q
 }
 
 __global__ void __launch_bounds__(256) inv_face_kernel(const BlockDev* __restrict__ blocks,
-                                                       const long long* __restrict__ off, int nb, int d,
+                                                       Span SP, int d,
                                                        double gamma) {
-  const long long g = xcd_thread();
-  if (g >= off[nb]) return;
-  const int bi = find_block(off, nb, g);
+  int bi;
+  long long gi;
+  if (!span_item(SP, &bi, &gi)) return;
   const BlockDev& b = blocks[bi];
   long long base;
   int f;
-  decode_face(b, d, g - off[bi], &base, &f);
+  decode_face(b, d, gi, &base, &f);
   const int n = dcount(b, d);
   const long long s = dstride(b, d);
   const LineRef L{b.prim, b.P, base, s};
@@ -231,14 +231,14 @@ __global__ void __launch_bounds__(256) inv_face_kernel(const BlockDev* __restric
 }
 
 // Boundary recalculation (InviscidFluxComputer.cpp:377-521) for one line.
-__global__ void __launch_bounds__(64) inv_closure_kernel(const BlockDev* __restrict__ blocks,
-                                                         const long long* __restrict__ off, int nb, int d,
+__global__ void __launch_bounds__(256) inv_closure_kernel(const BlockDev* __restrict__ blocks,
+                                                         Span SP, int d,
                                                          double gamma) {
-  const long long g = xcd_thread();
-  if (g >= off[nb]) return;
-  const int bi = find_block(off, nb, g);
+  int bi;
+  long long gi;
+  if (!span_item(SP, &bi, &gi)) return;
   const BlockDev& b = blocks[bi];
-  const long long ln = g - off[bi];
+  const long long ln = gi;
   const long long base = decode_line(b, d, ln);
   const int n = dcount(b, d);
   const long long s = dstride(b, d);
@@ -371,13 +371,13 @@ __device__ inline double diff_node(const double* F0, long long s, int n, int t) 
 }
 
 __global__ void __launch_bounds__(256) inv_diff_kernel(const BlockDev* __restrict__ blocks,
-                                                       const long long* __restrict__ off, int nb) {
-  const long long g = xcd_thread();
-  if (g >= off[nb]) return;
-  const int bi = find_block(off, nb, g);
+                                                       Span SP) {
+  int bi;
+  long long gi;
+  if (!span_item(SP, &bi, &gi)) return;
   const BlockDev& b = blocks[bi];
   int i, j, k;
-  decode_interior(b, g - off[bi], &i, &j, &k);
+  decode_interior(b, gi, &i, &j, &k);
   const long long id = pidx(b, i, j, k);
   double acc[5];
 #pragma unroll
@@ -426,13 +426,13 @@ __device__ inline double node_deriv(const double* q, long long s, int n, int i1,
 }
 
 __global__ void __launch_bounds__(256) visc_deriv_kernel(const BlockDev* __restrict__ blocks,
-                                                         const long long* __restrict__ off, int nb) {
-  const long long g = xcd_thread();
-  if (g >= off[nb]) return;
-  const int bi = find_block(off, nb, g);
+                                                         Span SP) {
+  int bi;
+  long long gi;
+  if (!span_item(SP, &bi, &gi)) return;
   const BlockDev& b = blocks[bi];
   int i, j, k;
-  decode_interior(b, g - off[bi], &i, &j, &k);
+  decode_interior(b, gi, &i, &j, &k);
   const long long id = pidx(b, i, j, k);
   const int pos[3] = {i, j, k};
   for (int d = 0; d < 3; ++d) {
@@ -477,15 +477,15 @@ __device__ inline double half_deriv(const double* q, long long s, int n, int f, 
 }
 
 __global__ void __launch_bounds__(256) visc_face_kernel(const BlockDev* __restrict__ blocks,
-                                                        const long long* __restrict__ off, int nb, int d,
+                                                        Span SP, int d,
                                                         double cp_prl) {
-  const long long g = xcd_thread();
-  if (g >= off[nb]) return;
-  const int bi = find_block(off, nb, g);
+  int bi;
+  long long gi;
+  if (!span_item(SP, &bi, &gi)) return;
   const BlockDev& b = blocks[bi];
   long long base;
   int f;
-  decode_face(b, d, g - off[bi], &base, &f);
+  decode_face(b, d, gi, &base, &f);
   const int n = dcount(b, d);
   const long long s = dstride(b, d);
   const long long P = b.P;
@@ -553,13 +553,13 @@ __global__ void __launch_bounds__(256) visc_face_kernel(const BlockDev* __restri
 }
 
 __global__ void __launch_bounds__(256) visc_diff_kernel(const BlockDev* __restrict__ blocks,
-                                                        const long long* __restrict__ off, int nb, double re_inv) {
-  const long long g = xcd_thread();
-  if (g >= off[nb]) return;
-  const int bi = find_block(off, nb, g);
+                                                        Span SP, double re_inv) {
+  int bi;
+  long long gi;
+  if (!span_item(SP, &bi, &gi)) return;
   const BlockDev& b = blocks[bi];
   int i, j, k;
-  decode_interior(b, g - off[bi], &i, &j, &k);
+  decode_interior(b, gi, &i, &j, &k);
   const long long id = pidx(b, i, j, k);
   double acc[5];
 #pragma unroll
@@ -584,27 +584,27 @@ static inline unsigned grid_of(long long n, int tb) { return (unsigned)((n + tb 
 
 void launch_inv_face(const Launch& L, int d, long long n, double gamma) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(inv_face_kernel, dim3(grid_of(n, 256)), dim3(256), 0, L.stream, L.blocks, L.off, L.nb, d, gamma);
+  hipLaunchKernelGGL(inv_face_kernel, dim3((unsigned)L.nwg), dim3(SPAN_TB), 0, L.stream, L.blocks, L.span, d, gamma);
 }
 void launch_inv_closure(const Launch& L, int d, long long n, double gamma) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(inv_closure_kernel, dim3(grid_of(n, 64)), dim3(64), 0, L.stream, L.blocks, L.off, L.nb, d, gamma);
+  hipLaunchKernelGGL(inv_closure_kernel, dim3((unsigned)L.nwg), dim3(SPAN_TB), 0, L.stream, L.blocks, L.span, d, gamma);
 }
 void launch_inv_diff(const Launch& L, long long n) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(inv_diff_kernel, dim3(grid_of(n, 256)), dim3(256), 0, L.stream, L.blocks, L.off, L.nb);
+  hipLaunchKernelGGL(inv_diff_kernel, dim3((unsigned)L.nwg), dim3(SPAN_TB), 0, L.stream, L.blocks, L.span);
 }
 void launch_visc_deriv(const Launch& L, long long n) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(visc_deriv_kernel, dim3(grid_of(n, 256)), dim3(256), 0, L.stream, L.blocks, L.off, L.nb);
+  hipLaunchKernelGGL(visc_deriv_kernel, dim3((unsigned)L.nwg), dim3(SPAN_TB), 0, L.stream, L.blocks, L.span);
 }
 void launch_visc_face(const Launch& L, int d, long long n, double cp_prl) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(visc_face_kernel, dim3(grid_of(n, 256)), dim3(256), 0, L.stream, L.blocks, L.off, L.nb, d, cp_prl);
+  hipLaunchKernelGGL(visc_face_kernel, dim3((unsigned)L.nwg), dim3(SPAN_TB), 0, L.stream, L.blocks, L.span, d, cp_prl);
 }
 void launch_visc_diff(const Launch& L, long long n, double re_inv) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(visc_diff_kernel, dim3(grid_of(n, 256)), dim3(256), 0, L.stream, L.blocks, L.off, L.nb, re_inv);
+  hipLaunchKernelGGL(visc_diff_kernel, dim3((unsigned)L.nwg), dim3(SPAN_TB), 0, L.stream, L.blocks, L.span, re_inv);
 }
 
 }  // namespace orion_hip

@@ -76,13 +76,13 @@ __device__ inline void natural_to_sweep(const BlockDev& b, int i, int j, int k, 
 // packing (one thread per interior point, natural order)
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) lu_geom_kernel(const BlockDev* __restrict__ blocks,
-                                                      const long long* __restrict__ off, int nb, SweepArrays S) {
-  const long long g = xcd_thread();
-  if (g >= off[nb]) return;
-  const int bi = find_block(off, nb, g);
+                                                      Span SP, SweepArrays S) {
+  int bi;
+  long long gi;
+  if (!span_item(SP, &bi, &gi)) return;
   const BlockDev& b = blocks[bi];
   int i, j, k, m, r, c;
-  decode_interior(b, g - off[bi], &i, &j, &k);
+  decode_interior(b, gi, &i, &j, &k);
   natural_to_sweep(b, i, j, k, &m, &r, &c);
   const long long q = sweep_index(b, m, r, c);
   const long long id = pidx(b, i, j, k);
@@ -91,14 +91,14 @@ __global__ void __launch_bounds__(256) lu_geom_kernel(const BlockDev* __restrict
 }
 
 __global__ void __launch_bounds__(256) lu_pack_kernel(const BlockDev* __restrict__ blocks,
-                                                      const long long* __restrict__ off, int nb, SweepArrays S,
+                                                      Span SP, SweepArrays S,
                                                       Params p) {
-  const long long g = xcd_thread();
-  if (g >= off[nb]) return;
-  const int bi = find_block(off, nb, g);
+  int bi;
+  long long gi;
+  if (!span_item(SP, &bi, &gi)) return;
   const BlockDev& b = blocks[bi];
   int i, j, k, m, r, c;
-  decode_interior(b, g - off[bi], &i, &j, &k);
+  decode_interior(b, gi, &i, &j, &k);
   natural_to_sweep(b, i, j, k, &m, &r, &c);
   const long long q = sweep_index(b, m, r, c);
   const long long N = S.n;
@@ -153,14 +153,14 @@ __global__ void __launch_bounds__(256) lu_pack_kernel(const BlockDev* __restrict
 // dq back to the block arrays (zero on physical-BC windows,
 // StateUpdater.cpp:577) and update_limited (:470-542).
 __global__ void __launch_bounds__(256) update_kernel(const BlockDev* __restrict__ blocks,
-                                                     const long long* __restrict__ off, int nb, SweepArrays S,
+                                                     Span SP, SweepArrays S,
                                                      double gama) {
-  const long long g = xcd_thread();
-  if (g >= off[nb]) return;
-  const int bi = find_block(off, nb, g);
+  int bi;
+  long long gi;
+  if (!span_item(SP, &bi, &gi)) return;
   const BlockDev& b = blocks[bi];
   int i, j, k, m, r, c;
-  decode_interior(b, g - off[bi], &i, &j, &k);
+  decode_interior(b, gi, &i, &j, &k);
   const long long id = pidx(b, i, j, k);
   const long long P = b.P;
   double d[5];
@@ -594,17 +594,17 @@ static inline unsigned grid_of(long long n, int tb) { return (unsigned)((n + tb 
 
 void launch_lu_geom(const Launch& La, long long n, const SweepArrays& S) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(lu_geom_kernel, dim3(grid_of(n, 256)), dim3(256), 0, La.stream, La.blocks, La.off, La.nb, S);
+  hipLaunchKernelGGL(lu_geom_kernel, dim3((unsigned)La.nwg), dim3(SPAN_TB), 0, La.stream, La.blocks, La.span, S);
 }
 
 void launch_lu_pack(const Launch& La, long long n, const SweepArrays& S, const Params& p) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(lu_pack_kernel, dim3(grid_of(n, 256)), dim3(256), 0, La.stream, La.blocks, La.off, La.nb, S, p);
+  hipLaunchKernelGGL(lu_pack_kernel, dim3((unsigned)La.nwg), dim3(SPAN_TB), 0, La.stream, La.blocks, La.span, S, p);
 }
 
 void launch_update(const Launch& La, long long n, const SweepArrays& S, double gamma) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(update_kernel, dim3(grid_of(n, 256)), dim3(256), 0, La.stream, La.blocks, La.off, La.nb, S,
+  hipLaunchKernelGGL(update_kernel, dim3((unsigned)La.nwg), dim3(SPAN_TB), 0, La.stream, La.blocks, La.span, S,
                      gamma);
 }
 

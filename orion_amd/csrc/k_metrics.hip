@@ -51,13 +51,13 @@ __device__ inline double dnode(const BlockDev& b, const double* fld, long long i
 }
 
 __global__ void __launch_bounds__(256) metric_a_kernel(const BlockDev* __restrict__ blocks,
-                                                       const long long* __restrict__ off, int nb, int gcl) {
-  const long long g = xcd_thread();
-  if (g >= off[nb]) return;
-  const int bi = find_block(off, nb, g);
+                                                       Span SP, int gcl) {
+  int bi;
+  long long gi;
+  if (!span_item(SP, &bi, &gi)) return;
   const BlockDev& b = blocks[bi];
   int i, j, k;
-  decode_interior(b, g - off[bi], &i, &j, &k);
+  decode_interior(b, gi, &i, &j, &k);
   const long long id = pidx(b, i, j, k);
   const int pos[3] = {i, j, k};
   const long long P = b.P;
@@ -95,13 +95,13 @@ __global__ void __launch_bounds__(256) metric_a_kernel(const BlockDev* __restric
 }
 
 __global__ void __launch_bounds__(256) metric_gcl_kernel(const BlockDev* __restrict__ blocks,
-                                                         const long long* __restrict__ off, int nb) {
-  const long long g = xcd_thread();
-  if (g >= off[nb]) return;
-  const int bi = find_block(off, nb, g);
+                                                         Span SP) {
+  int bi;
+  long long gi;
+  if (!span_item(SP, &bi, &gi)) return;
   const BlockDev& b = blocks[bi];
   int i, j, k;
-  decode_interior(b, g - off[bi], &i, &j, &k);
+  decode_interior(b, gi, &i, &j, &k);
   const long long id = pidx(b, i, j, k);
   const int pos[3] = {i, j, k};
   const long long P = b.P;
@@ -126,14 +126,14 @@ __global__ void __launch_bounds__(256) metric_gcl_kernel(const BlockDev* __restr
 }
 
 __global__ void __launch_bounds__(256) small_vol_kernel(const BlockDev* __restrict__ blocks,
-                                                        const long long* __restrict__ off, int nb,
+                                                        Span SP,
                                                         unsigned long long* count) {
-  const long long g = xcd_thread();
-  if (g >= off[nb]) return;
-  const int bi = find_block(off, nb, g);
+  int bi;
+  long long gi;
+  if (!span_item(SP, &bi, &gi)) return;
   const BlockDev& b = blocks[bi];
   int i, j, k;
-  decode_interior(b, g - off[bi], &i, &j, &k);
+  decode_interior(b, gi, &i, &j, &k);
   // check_grid_metrics (MetricComputer.cpp:433-470): vol < 1e-15 is fatal
   if (!(b.vol[pidx(b, i, j, k)] >= 1.0e-15)) atomicAdd(count, 1ULL);
 }
@@ -142,13 +142,13 @@ static inline unsigned grid_of(long long n, int tb) { return (unsigned)((n + tb 
 
 void launch_metrics(const Launch& L, long long n, int gcl, double*, hipStream_t) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(metric_a_kernel, dim3(grid_of(n, 256)), dim3(256), 0, L.stream, L.blocks, L.off, L.nb, gcl);
+  hipLaunchKernelGGL(metric_a_kernel, dim3((unsigned)L.nwg), dim3(SPAN_TB), 0, L.stream, L.blocks, L.span, gcl);
   if (gcl == 1)
-    hipLaunchKernelGGL(metric_gcl_kernel, dim3(grid_of(n, 256)), dim3(256), 0, L.stream, L.blocks, L.off, L.nb);
+    hipLaunchKernelGGL(metric_gcl_kernel, dim3((unsigned)L.nwg), dim3(SPAN_TB), 0, L.stream, L.blocks, L.span);
 }
 void launch_count_small_vol(const Launch& L, long long n, unsigned long long* count) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(small_vol_kernel, dim3(grid_of(n, 256)), dim3(256), 0, L.stream, L.blocks, L.off, L.nb, count);
+  hipLaunchKernelGGL(small_vol_kernel, dim3((unsigned)L.nwg), dim3(SPAN_TB), 0, L.stream, L.blocks, L.span, count);
 }
 
 }  // namespace orion_hip

@@ -16,15 +16,14 @@
 namespace orion_hip {
 
 __global__ void __launch_bounds__(256) state_kernel(const BlockDev* __restrict__ blocks,
-                                                    const long long* __restrict__ off, int nb, Params p, int mode,
+                                                    Span SP, Params p, int mode,
                                                     unsigned long long* dtmin_base, const int* step) {
   __shared__ double smin[256];
-  const long long g = xcd_thread();
   double mydt = 1000.0;
-  if (g < off[nb]) {
-    const int bi = find_block(off, nb, g);
+  int bi;
+  long long n;
+  if (span_item(SP, &bi, &n)) {
     const BlockDev& b = blocks[bi];
-    const long long n = g - off[bi];
     const int i = (int)(n % b.nx);
     const long long r = n / b.nx;
     const int j = (int)(r % b.ny), k = (int)(r / b.ny);
@@ -222,13 +221,13 @@ __global__ void fill_kernel(double* p, long long n, double v) {
 
 // update_temperature, src/preprocess/InitialCondition.cpp:91-147
 __global__ void __launch_bounds__(256) temperature_kernel(const BlockDev* __restrict__ blocks,
-                                                          const long long* __restrict__ off, int nb, double gama,
+                                                          Span SP, double gama,
                                                           double moo) {
-  const long long g = xcd_thread();
-  if (g >= off[nb]) return;
-  const int bi = find_block(off, nb, g);
+  int bi;
+  long long gi;
+  if (!span_item(SP, &bi, &gi)) return;
   const BlockDev& b = blocks[bi];
-  const long long id = g - off[bi];
+  const long long id = gi;
   double rho = b.prim[id];
   const double p = b.prim[4 * b.P + id];
   if (rho < 1.0e-30) rho = 1.0e-30;
@@ -241,7 +240,7 @@ static inline unsigned grid_of(long long n, int tb) { return (unsigned)((n + tb 
 void launch_state(const Launch& L, long long n, const Params& p, int mode, unsigned long long* dtmin_base,
                   const int* step) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(state_kernel, dim3(grid_of(n, 256)), dim3(256), 0, L.stream, L.blocks, L.off, L.nb, p, mode,
+  hipLaunchKernelGGL(state_kernel, dim3((unsigned)L.nwg), dim3(SPAN_TB), 0, L.stream, L.blocks, L.span, p, mode,
                      dtmin_base, step);
 }
 void launch_step_advance(int* step, hipStream_t s) {
@@ -260,7 +259,7 @@ void launch_fill(double* p, long long n, double v, hipStream_t s) {
 }
 void launch_temperature(const Launch& L, long long n, double gama, double moo) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(temperature_kernel, dim3(grid_of(n, 256)), dim3(256), 0, L.stream, L.blocks, L.off, L.nb, gama,
+  hipLaunchKernelGGL(temperature_kernel, dim3((unsigned)L.nwg), dim3(SPAN_TB), 0, L.stream, L.blocks, L.span, gama,
                      moo);
 }
 

@@ -96,6 +96,30 @@ __device__ inline long long xcd_thread() {
   return (long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
 }
 
+// A multi-block iteration space as launched: each block's items are padded
+// to whole 256-thread workgroups, so a workgroup never spans two blocks and
+// its block comes from one uniform table load (and the block's fields from
+// scalar loads), instead of a per-lane search.
+constexpr int SPAN_TB = 256;
+struct Span {
+  const long long* off;  // [nb+1] prefix of item counts
+  const int* wgb;        // [nwg] block of each workgroup
+  const int* wg0;        // [nb] first workgroup of each block
+  int nb;
+};
+
+// This thread's (block, item) of span S, false for the padding threads.
+// Workgroups are renumbered XCD-contiguously first (xcd_remap).
+__device__ inline bool span_item(const Span& S, int* bi, long long* item) {
+  const unsigned wg = xcd_remap(blockIdx.x, gridDim.x);
+  const int b = S.wgb[wg];
+  const long long n = (long long)(wg - (unsigned)S.wg0[b]) * SPAN_TB + threadIdx.x;
+  if (n >= S.off[b + 1] - S.off[b]) return false;
+  *bi = b;
+  *item = n;
+  return true;
+}
+
 // block lookup: largest b with off[b] <= g (off has nb+1 entries)
 __device__ inline int find_block(const long long* __restrict__ off, int nb, long long g) {
   int lo = 0, hi = nb - 1;

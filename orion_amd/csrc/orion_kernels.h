@@ -9,8 +9,10 @@ namespace orion_hip {
 struct Launch {
   hipStream_t stream;
   const BlockDev* blocks;  // device
-  const long long* off;    // device
+  const long long* off;    // device prefix of item counts (nb+1)
   int nb;
+  Span span;               // the same space padded per block to whole workgroups
+  long long nwg;           // workgroups of the padded space
 };
 
 // physical-BC work item: one region of one block (memory coordinates)
