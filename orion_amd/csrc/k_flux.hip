@@ -136,14 +136,16 @@ struct LineRef {
 };
 
 // Initial face states of face f (after the WCNS pass and the fixed boundary
-// stencils, InviscidFluxComputer.cpp:262-337).
+// stencils, InviscidFluxComputer.cpp:262-337).  The WCNS stencil of every
+// face lies inside the padded line, so it is evaluated for all faces -- its
+// loads then need not wait for the boundary flags -- and the faces outside
+// [ist, ied] (f < 2 or f > n - 2) have both states replaced below, exactly
+// as the reference leaves them to the fixed stencils.
 __device__ inline void face_states(const LineRef& L, int n, int ist, int ied, int f, double* qwl, double* qwr) {
-  if (f >= ist && f <= ied) {
 #pragma unroll
-    for (int m = 0; m < 5; ++m)
-      weno5(L.at(m, f), L.at(m, f + 1), L.at(m, f + 2), L.at(m, f + 3), L.at(m, f + 4), L.at(m, f + 5), &qwl[m],
-            &qwr[m]);
-  }
+  for (int m = 0; m < 5; ++m)
+    weno5(L.at(m, f), L.at(m, f + 1), L.at(m, f + 2), L.at(m, f + 3), L.at(m, f + 4), L.at(m, f + 5), &qwl[m],
+          &qwr[m]);
   if (ist > 1 && f <= 2) {
 #pragma unroll
     for (int m = 0; m < 5; ++m) {
@@ -393,11 +395,12 @@ __global__ void __launch_bounds__(256) inv_diff_kernel(const BlockDev* __restric
     const double* cov = nullptr;
     if (lflag && t <= 2) cov = b.fcov + b.fcov_off[d] + line_id(b, d, i, j, k) * 30 + t * 5;
     else if (rflag && t >= n - 3) cov = b.fcov + b.fcov_off[d] + line_id(b, d, i, j, k) * 30 + (3 + t - (n - 3)) * 5;
+    // the difference is formed for every node (its loads do not wait for
+    // the flags) and replaced by the closure value where the line has one
 #pragma unroll
     for (int m = 0; m < 5; ++m) {
-      double fc;
+      double fc = diff_node(b.fx + ((long long)d * 5 + m) * b.P + lbase, s, n, t);
       if (cov) fc = cov[m];
-      else fc = diff_node(b.fx + ((long long)d * 5 + m) * b.P + lbase, s, n, t);
       acc[m] += fc;
     }
   }
@@ -456,24 +459,27 @@ __device__ inline double half_central(const double* q, long long s, int f) {
           -1.0 * (q[(long long)(f + 4) * s] + q[(long long)(f + 1) * s])) /
          16.0;
 }
-// interp_line_to_half with flags (FluxComputer.cpp:129-178)
+// interp_line_to_half with flags (FluxComputer.cpp:129-178); the central
+// stencil is loaded first so its loads do not wait for the flags
 __device__ inline double half_flagged(const double* q, long long s, int n, int f, bool fl, bool fr) {
   auto Q = [&](int p) { return q[(long long)p * s]; };
+  const double c1 = Q(f + 1), c2 = Q(f + 2), c3 = Q(f + 3), c4 = Q(f + 4);
   if (fl && f == 0) return (35.0 * Q(3) + -35.0 * Q(4) + 21.0 * Q(5) + -5.0 * Q(6)) / 16.0;
   if (fl && f == 1) return (5.0 * Q(3) + 15.0 * Q(4) + -5.0 * Q(5) + 1.0 * Q(6)) / 16.0;
   if (fr && f == n) return (35.0 * Q(n + 2) + -35.0 * Q(n + 1) + 21.0 * Q(n) + -5.0 * Q(n - 1)) / 16.0;
   if (fr && f == n - 1) return (5.0 * Q(n + 2) + 15.0 * Q(n + 1) + -5.0 * Q(n) + 1.0 * Q(n - 1)) / 16.0;
-  return half_central(q, s, f);
+  return (9.0 * (c2 + c3) + -1.0 * (c4 + c1)) / 16.0;
 }
 // compute_half_deriv_line (FluxComputer.cpp:72-124)
 __device__ inline double half_deriv(const double* q, long long s, int n, int f, bool fl, bool fr) {
   auto Q = [&](int p) { return q[(long long)p * s]; };
+  const double c1 = Q(f + 1), c2 = Q(f + 2), c3 = Q(f + 3), c4 = Q(f + 4);
   if (fl && f == 0) return (-71.0 * Q(3) + 141.0 * Q(4) + -93.0 * Q(5) + 23.0 * Q(6)) / 24.0;
   if (fl && f == 1) return (-22.0 * Q(3) + 17.0 * Q(4) + 9.0 * Q(5) + -5.0 * Q(6) + 1.0 * Q(7)) / 24.0;
   if (fr && f == n) return -(-71.0 * Q(n + 2) + 141.0 * Q(n + 1) + -93.0 * Q(n) + 23.0 * Q(n - 1)) / 24.0;
   if (fr && f == n - 1)
     return -(-22.0 * Q(n + 2) + 17.0 * Q(n + 1) + 9.0 * Q(n) + -5.0 * Q(n - 1) + 1.0 * Q(n - 2)) / 24.0;
-  return (27.0 * (Q(f + 3) - Q(f + 2)) + -1.0 * (Q(f + 4) - Q(f + 1))) / 24.0;
+  return (27.0 * (c3 - c2) + -1.0 * (c4 - c1)) / 24.0;
 }
 
 __global__ void __launch_bounds__(256) visc_face_kernel(const BlockDev* __restrict__ blocks,

@@ -320,8 +320,10 @@ __device__ __forceinline__ void load_nbr(const SweepArrays& S, long long q, bool
 }
 
 // Sweep types: 0 LU forward, 1 LU backward, 2 GS forward, 3 GS backward.
-template <int TYPE>
-__global__ void __launch_bounds__(TL, 2) lusgs_tile_kernel(SweepArrays S, const BlockDev* __restrict__ blocks,
+// WAVES: the minimum resident waves per SIMD the register allocation targets
+// (2: two tiles per CU; 1: one tile per CU and no spills).
+template <int TYPE, int WAVES>
+__global__ void __launch_bounds__(TL, WAVES) lusgs_tile_kernel(SweepArrays S, const BlockDev* __restrict__ blocks,
                                                        const SweepTile* __restrict__ tiles,
                                                        const int* __restrict__ order, unsigned long long* flags,
                                                        const unsigned long long* __restrict__ epoch, int* err,
@@ -610,16 +612,20 @@ void launch_update(const Launch& La, long long n, const SweepArrays& S, double g
 
 void launch_lusgs_tiles(const SweepArrays& S, const BlockDev* blocks, const SweepTile* tiles, const int* order,
                         int ntiles, unsigned long long* flags, unsigned long long* epoch, int* err, int pub, int sync,
-                        int type, const Params& p, hipStream_t s) {
+                        int waves, int type, const Params& p, hipStream_t s) {
   if (ntiles <= 0) return;
   hipLaunchKernelGGL(epoch_kernel, dim3(1), dim3(64), 0, s, epoch);
   const double gm1 = p.gama - 1.0;
   const dim3 gr(ntiles), tb(TL);
+#define ORION_TILE_LAUNCH(T, W) \
+  hipLaunchKernelGGL((lusgs_tile_kernel<T, W>), gr, tb, 0, s, S, blocks, tiles, order, flags, epoch, err, pub, sync, gm1, p.nvis)
+  const bool one = waves == 1;
   switch (type) {
-    case 0: hipLaunchKernelGGL(lusgs_tile_kernel<0>, gr, tb, 0, s, S, blocks, tiles, order, flags, epoch, err, pub, sync, gm1, p.nvis); break;
-    case 1: hipLaunchKernelGGL(lusgs_tile_kernel<1>, gr, tb, 0, s, S, blocks, tiles, order, flags, epoch, err, pub, sync, gm1, p.nvis); break;
-    case 2: hipLaunchKernelGGL(lusgs_tile_kernel<2>, gr, tb, 0, s, S, blocks, tiles, order, flags, epoch, err, pub, sync, gm1, p.nvis); break;
-    default: hipLaunchKernelGGL(lusgs_tile_kernel<3>, gr, tb, 0, s, S, blocks, tiles, order, flags, epoch, err, pub, sync, gm1, p.nvis); break;
+    case 0: if (one) ORION_TILE_LAUNCH(0, 1); else ORION_TILE_LAUNCH(0, 2); break;
+    case 1: if (one) ORION_TILE_LAUNCH(1, 1); else ORION_TILE_LAUNCH(1, 2); break;
+    case 2: if (one) ORION_TILE_LAUNCH(2, 1); else ORION_TILE_LAUNCH(2, 2); break;
+    default: if (one) ORION_TILE_LAUNCH(3, 1); else ORION_TILE_LAUNCH(3, 2); break;
+#undef ORION_TILE_LAUNCH
   }
 }
 

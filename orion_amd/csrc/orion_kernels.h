@@ -66,7 +66,7 @@ void launch_lu_geom(const Launch& La, long long n, const SweepArrays& S);
 void launch_lu_pack(const Launch& La, long long n, const SweepArrays& S, const Params& p);
 void launch_lusgs_tiles(const SweepArrays& S, const BlockDev* blocks, const SweepTile* tiles, const int* order,
                         int ntiles, unsigned long long* flags, unsigned long long* epoch, int* err, int pub, int sync,
-                        int type, const Params& p, hipStream_t s);
+                        int waves, int type, const Params& p, hipStream_t s);
 void launch_update(const Launch& La, long long n, const SweepArrays& S, double gamma);
 
 // k_metrics.hip
