@@ -71,8 +71,6 @@ struct orion_hip_ctx {
   unsigned long long* d_epoch = nullptr; // sweep counter
   int* d_err = nullptr;                  // set by a sweep whose wait ran out
   int sweep_pub = 1;                     // steps between progress publications
-  int sweep_sync = 2;                    // cross-tile visibility scheme (k_lusgs.hip)
-  int sweep_waves[4] = {2, 2, 1, 1};     // register target per sweep type (waves per SIMD)
   long long* d_halo = nullptr;
   long long n_halo = 0;
   BCTask* d_bc = nullptr;
@@ -517,21 +515,19 @@ static int build_sweep(orion_hip_ctx* c) {
   if (upload_vec(c, &c->d_order[1], ord)) return -1;
   if (upload_vec(c, &c->d_tiles, c->tiles)) return -1;
   c->sw.n = n;
-  if (dalloc(c, (void**)&c->sw.met, sizeof(double) * 9 * n) || dalloc(c, (void**)&c->sw.D, sizeof(double) * 8 * n) ||
-      dalloc(c, (void**)&c->sw.coed, sizeof(double) * n) || dalloc(c, (void**)&c->sw.rhs, sizeof(double) * 5 * n) ||
-      dalloc(c, (void**)&c->sw.dq, sizeof(double) * 5 * n) ||
+  if (dalloc(c, (void**)&c->sw.own, sizeof(double2) * SW_OWN * n) ||
+      dalloc(c, (void**)&c->sw.D, sizeof(double2) * SW_D * n) ||
+      dalloc(c, (void**)&c->sw.dq, sizeof(double2) * SW_DQ * n) ||
       dalloc(c, (void**)&c->d_flags, sizeof(unsigned long long) * std::max(nt, 1)) ||
       dalloc(c, (void**)&c->d_epoch, sizeof(unsigned long long)) || dalloc(c, (void**)&c->d_err, sizeof(int)))
     return -1;
-  HIPCHK(hipMemset(c->sw.dq, 0, sizeof(double) * 5 * n));
+  HIPCHK(hipMemset(c->sw.dq, 0, sizeof(double2) * SW_DQ * n));
+  HIPCHK(hipMemset(c->sw.own, 0, sizeof(double2) * SW_OWN * n));
+  HIPCHK(hipMemset(c->sw.D, 0, sizeof(double2) * SW_D * n));
   HIPCHK(hipMemset(c->d_flags, 0, sizeof(unsigned long long) * std::max(nt, 1)));
   HIPCHK(hipMemset(c->d_epoch, 0, sizeof(unsigned long long)));
   HIPCHK(hipMemset(c->d_err, 0, sizeof(int)));
   if (const char* e = std::getenv("ORION_LU_PUBLISH")) c->sweep_pub = std::max(1, std::atoi(e));
-  if (const char* e = std::getenv("ORION_LU_SYNC")) c->sweep_sync = std::min(2, std::max(0, std::atoi(e)));
-  if (const char* e = std::getenv("ORION_LU_WAVES")) {  // e.g. "2211": one digit per sweep type
-    for (int t = 0; t < 4 && e[t]; ++t) c->sweep_waves[t] = (e[t] == '1') ? 1 : 2;
-  }
   return 0;
 }
 
@@ -839,7 +835,7 @@ static void op_inv(orion_hip_ctx* c) {
 static void op_sweep(orion_hip_ctx* c, int type) {
   const int bwd = (type == 1 || type == 3) ? 1 : 0;
   launch_lusgs_tiles(c->sw, c->db, c->d_tiles, c->d_order[bwd], (int)c->tiles.size(), c->d_flags, c->d_epoch,
-                     c->d_err, c->sweep_pub, c->sweep_sync, c->sweep_waves[type], type, c->p, c->stream);
+                     c->d_err, c->sweep_pub, type, c->p, c->stream);
 }
 
 static void op_update(orion_hip_ctx* c) {

@@ -19,12 +19,11 @@
 //    the march axis they are the lane's own previous / next cell (registers).
 //  * Across tiles, a tile publishes how far it has come in a progress flag
 //    and its downstream tiles wait on it (bounded spin) before reading edge
-//    values.  Visibility (argument `sync`): 2 (default) stores dq with
-//    agent-scope atomic stores, waits for their completion before the flag
-//    store, and reads edge dq with agent-scope atomic loads, so no cache
-//    maintenance is needed; 1 uses agent-scope release/acquire fences and
-//    0 sequentially consistent fences around plain stores (both flush and
-//    invalidate L2, several times slower).
+//    values.  dq is stored with agent-scope atomic stores and each wave
+//    waits for their completion before the barrier that precedes the flag
+//    store; edge dq is read with agent-scope atomic loads.  No cache
+//    maintenance is needed (fence-based release/acquire around plain stores
+//    flushes and invalidates L2 and measured 2-8x slower).
 //    Tiles are launched in dependency order and a wait only ever targets a
 //    tile launched earlier, so the persistent grid always drains; a spin
 //    that exceeds its bound records an error instead of hanging.
@@ -86,8 +85,9 @@ __global__ void __launch_bounds__(256) lu_geom_kernel(const BlockDev* __restrict
   natural_to_sweep(b, i, j, k, &m, &r, &c);
   const long long q = sweep_index(b, m, r, c);
   const long long id = pidx(b, i, j, k);
+  double* own = reinterpret_cast<double*>(S.own);
 #pragma unroll
-  for (int f = 0; f < 9; ++f) S.met[f * S.n + q] = b.met[f * b.P + id];
+  for (int f = 0; f < 9; ++f) own[sweep_field(q, c, b.sncp, SW_OWN, f)] = b.met[f * b.P + id];
 }
 
 __global__ void __launch_bounds__(256) lu_pack_kernel(const BlockDev* __restrict__ blocks,
@@ -101,14 +101,16 @@ __global__ void __launch_bounds__(256) lu_pack_kernel(const BlockDev* __restrict
   decode_interior(b, gi, &i, &j, &k);
   natural_to_sweep(b, i, j, k, &m, &r, &c);
   const long long q = sweep_index(b, m, r, c);
-  const long long N = S.n;
+  const int ncp = b.sncp;
+  double* own = reinterpret_cast<double*>(S.own);
+  double* Dp = reinterpret_cast<double*>(S.D);
   const long long P = b.P;
   const long long id = pidx(b, i, j, k);
   const double gamma = p.gama, re = p.reynolds;
   // rhs copy with the physical-BC windows zeroed (StateUpdater.cpp:557-570)
   const bool fz = b.frozen[id] != 0;
 #pragma unroll
-  for (int f = 0; f < 5; ++f) S.rhs[f * N + q] = fz ? 0.0 : b.dq[f * P + id];
+  for (int f = 0; f < 5; ++f) own[sweep_field(q, c, ncp, SW_OWN, 10 + f)] = fz ? 0.0 : b.dq[f * P + id];
   double pr[5];
 #pragma unroll
   for (int f = 0; f < 5; ++f) pr[f] = b.prim[f * P + id];
@@ -130,7 +132,7 @@ __global__ void __launch_bounds__(256) lu_pack_kernel(const BlockDev* __restrict
     }
     const double rs = ra + rb + rc + rva + rvb + rvc;
     const double term = (1.0 / b.dt[id]) + 1.0 * 1.0 * rs;
-    S.coed[q] = 1.0 / term;
+    own[sweep_field(q, c, ncp, SW_OWN, 9)] = 1.0 / term;
   }
   // this cell seen as a neighbour: compute_flux_splitting's per-cell terms,
   // the sound speed of calc_spectral_radius and the viscous coefficient of
@@ -140,14 +142,14 @@ __global__ void __launch_bounds__(256) lu_pack_kernel(const BlockDev* __restrict
   if (rm < 1e-30) rm = 1e-30;
   if (pm < 1e-30) pm = 1e-30;
   const double c2 = gamma * pm / rm;
-  S.D[0 * N + q] = pr[1];
-  S.D[1 * N + q] = pr[2];
-  S.D[2 * N + q] = pr[3];
-  S.D[3 * N + q] = c2;
-  S.D[4 * N + q] = (gamma / (gamma - 1.0)) * (pm / rm) + 0.5 * v2;
-  S.D[5 * N + q] = 0.5 * (gamma - 1.0) * v2;
-  S.D[6 * N + q] = sqrt(c2);
-  S.D[7 * N + q] = 2.0 * mu / (fabs(re) * (rm * vol) + 1.0e-30);
+  Dp[sweep_field(q, c, ncp, SW_D, 0)] = pr[1];
+  Dp[sweep_field(q, c, ncp, SW_D, 1)] = pr[2];
+  Dp[sweep_field(q, c, ncp, SW_D, 2)] = pr[3];
+  Dp[sweep_field(q, c, ncp, SW_D, 3)] = c2;
+  Dp[sweep_field(q, c, ncp, SW_D, 4)] = (gamma / (gamma - 1.0)) * (pm / rm) + 0.5 * v2;
+  Dp[sweep_field(q, c, ncp, SW_D, 5)] = 0.5 * (gamma - 1.0) * v2;
+  Dp[sweep_field(q, c, ncp, SW_D, 6)] = sqrt(c2);
+  Dp[sweep_field(q, c, ncp, SW_D, 7)] = 2.0 * mu / (fabs(re) * (rm * vol) + 1.0e-30);
 }
 
 // dq back to the block arrays (zero on physical-BC windows,
@@ -175,7 +177,7 @@ __global__ void __launch_bounds__(256) update_kernel(const BlockDev* __restrict_
     const long long q = sweep_index(b, m, r, c);
 #pragma unroll
     for (int f = 0; f < 5; ++f) {
-      d[f] = S.dq[f * S.n + q];
+      d[f] = reinterpret_cast<const double*>(S.dq)[sweep_field(q, c, b.sncp, SW_DQ, f)];
       b.dq[f * P + id] = d[f];
     }
   }
@@ -297,41 +299,86 @@ struct Own {  // a cell's own inputs to its update
   double r5[5];  // rhs, or (backward LU) the forward sweep's dq
 };
 
-template <int TYPE>
-__device__ __forceinline__ void load_own(const SweepArrays& S, long long q, bool ok, Own& o) {
-  const long long N = S.n;
-  if (!ok) q = 0;
-#pragma unroll
-  for (int f = 0; f < 9; ++f) o.met[f] = ok ? S.met[f * N + q] : 0.0;
-  o.coed = ok ? S.coed[q] : 0.0;
-#pragma unroll
-  for (int f = 0; f < 5; ++f) o.r5[f] = ok ? (TYPE == 1 ? S.dq[f * N + q] : S.rhs[f * N + q]) : 0.0;
+// Loads are unconditional: a lane whose cell is absent reads the block's
+// first cell instead (always a valid address) and never uses the values.
+// (rb, c): the cell's row base q - c and column; pair k of a K-pair group
+// is one 16-byte load
+__device__ __forceinline__ double2 ld_pair(const double2* g, long long rb, int c, int ncp, int K, int k) {
+  return g[rb * K + (long long)k * ncp + c];
 }
 
+template <int TYPE>
+__device__ __forceinline__ void load_own(const SweepArrays& S, long long rb, int c, int ncp, Own& o) {
+  double2 v[SW_OWN];
+#pragma unroll
+  for (int k = 0; k < (TYPE == 1 ? 5 : SW_OWN); ++k) v[k] = ld_pair(S.own, rb, c, ncp, SW_OWN, k);
+  if (TYPE == 1) {
+#pragma unroll
+    for (int k = 0; k < SW_DQ; ++k) v[5 + k] = ld_pair(S.dq, rb, c, ncp, SW_DQ, k);
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    o.met[2 * k] = v[k].x;
+    o.met[2 * k + 1] = v[k].y;
+  }
+  o.met[8] = v[4].x;
+  o.coed = v[4].y;
+  o.r5[0] = v[5].x;
+  o.r5[1] = v[5].y;
+  o.r5[2] = v[6].x;
+  o.r5[3] = v[6].y;
+  o.r5[4] = v[7].x;
+}
+
+// D (8) and, for the Gauss-Seidel sweeps, the cell's old dq (5); x holds
+// 13 words for the Gauss-Seidel sweeps and 8 for the LU ones
 template <bool GS>
-__device__ __forceinline__ void load_nbr(const SweepArrays& S, long long q, bool ok, double* x) {
-  // D (8) and, for the Gauss-Seidel sweeps, the cell's old dq (5)
-  const long long N = S.n;
-  if (!ok) q = 0;
+__device__ __forceinline__ void load_nbr(const SweepArrays& S, long long rb, int c, int ncp, double* x) {
 #pragma unroll
-  for (int f = 0; f < 8; ++f) x[f] = ok ? S.D[f * N + q] : 0.0;
-#pragma unroll
-  for (int f = 0; f < 5; ++f) x[8 + f] = (GS && ok) ? S.dq[f * N + q] : 0.0;
+  for (int k = 0; k < SW_D; ++k) {
+    const double2 v = ld_pair(S.D, rb, c, ncp, SW_D, k);
+    x[2 * k] = v.x;
+    x[2 * k + 1] = v.y;
+  }
+  if constexpr (GS) {
+    const double2 a = ld_pair(S.dq, rb, c, ncp, SW_DQ, 0), b2 = ld_pair(S.dq, rb, c, ncp, SW_DQ, 1);
+    const double2 e = ld_pair(S.dq, rb, c, ncp, SW_DQ, 2);
+    x[8] = a.x;
+    x[9] = a.y;
+    x[10] = b2.x;
+    x[11] = b2.y;
+    x[12] = e.x;
+  }
 }
 
 // Sweep types: 0 LU forward, 1 LU backward, 2 GS forward, 3 GS backward.
-// WAVES: the minimum resident waves per SIMD the register allocation targets
-// (2: two tiles per CU; 1: one tile per CU and no spills).
-template <int TYPE, int WAVES>
-__global__ void __launch_bounds__(TL, WAVES) lusgs_tile_kernel(SweepArrays S, const BlockDev* __restrict__ blocks,
-                                                       const SweepTile* __restrict__ tiles,
-                                                       const int* __restrict__ order, unsigned long long* flags,
-                                                       const unsigned long long* __restrict__ epoch, int* err,
-                                                       int pub, int sync, double gm1, int nvis) {
+//
+// Pipeline of one tile.  Step t works on hyperplane s(t) = s0 + dir*t:
+//   phase 1  edge values (loaded two steps ago) and, for Gauss-Seidel, the
+//            lanes' next cells go to LDS; barrier;
+//   phase 2  the update of each lane's cell from LDS and registers; dq is
+//            stored with agent-scope stores; when publishing, each wave waits
+//            for its stores; barrier;
+//   phase 3  thread 0 publishes plane s(t); the lanes' (D, new dq) go to
+//            LDS; thread 0 makes sure the upstream tiles have finished plane
+//            s(t+1), from flag values it read one step earlier (it spins
+//            only when they have not); then the loads for later steps are
+//            issued: own inputs of t+2, neighbour views of t+3, edge values
+//            of t+2.
+// Issuing every prefetch after the store wait keeps that wait from draining
+// them, and reading the flags a step early keeps the flag round trip off
+// the critical path, at the price of one step of lag per tile edge.
+template <int TYPE>
+__global__ void __launch_bounds__(TL, 1) lusgs_tile_kernel(SweepArrays S, const BlockDev* __restrict__ blocks,
+                                                          const SweepTile* __restrict__ tiles,
+                                                          const int* __restrict__ order, unsigned long long* flags,
+                                                          const unsigned long long* __restrict__ epoch, int* err,
+                                                          int pub, double gm1, int nvis) {
   constexpr bool FWD = (TYPE == 0 || TYPE == 2);
   constexpr bool GS = TYPE >= 2;
   constexpr int dir = FWD ? 1 : -1;
   constexpr int NROLE = GS ? 4 : 2;
+  constexpr int NB = GS ? NW : 8;              // words of a neighbour view
   __shared__ double shA[NW][TL];               // (D, new dq) of each lane's cell of the last step
   __shared__ double shB[GS ? NW : 1][TL];      // (D, old dq) of each lane's cell of the next step
   __shared__ double shX[NROLE][NW][TW];        // edge values from neighbouring tiles
@@ -341,75 +388,82 @@ __global__ void __launch_bounds__(TL, WAVES) lusgs_tile_kernel(SweepArrays S, co
   const BlockDev& b = blocks[T.block];
   const int nm = b.sn[0], nc = b.sn[1], nr = b.sn[2];
   const int am = b.sax[0], ac = b.sax[1];
-  const long long N = S.n;
   const int lane = threadIdx.x, lc = lane % TW, lr = lane / TW;
   const int c = T.c0 + lc, r = T.r0 + lr;
   const bool lane_ok = c < nc && r < nr;
-  // upstream tiles (their cells precede ours in sweep order) and downstream
   const int up_c = FWD ? T.nbr[0] : T.nbr[2], up_r = FWD ? T.nbr[1] : T.nbr[3];
   const unsigned long long ep = (*epoch) << KEY_BITS;
-  // plane ranges of the upstream tiles (for the clamped wait targets)
-  SweepTile ut[2];
+  SweepTile ut[2];  // plane ranges of the upstream tiles (clamped wait targets)
   ut[0] = up_c >= 0 ? tiles[up_c] : T;
   ut[1] = up_r >= 0 ? tiles[up_r] : T;
   const int nsteps = T.s_last - T.s_first + 1;
   const int s0 = FWD ? T.s_first : T.s_last;
+  const long long plane_stride = (long long)nr * b.sncp;  // one hyperplane in the sweep layout
+  const long long qstep = dir * plane_stride;
+
+  // this lane's cell at step t: valid flag and sweep index (block's first
+  // cell when absent)
+  const int ncp = b.sncp;
+  const int m0 = s0 - r - c;
+  const long long rb0 = b.sbase + ((long long)s0 * nr + r) * ncp;  // row base q - c at step 0
+  auto cell_ok = [&](int t) {
+    const int m = m0 + dir * t;
+    return lane_ok && t < nsteps && m >= 0 && m < nm;
+  };
+  // row base and column of the lane's cell at step t (the block's first
+  // cell when absent)
+  auto cell_rb = [&](int t) { return cell_ok(t) ? rb0 + (long long)t * qstep : b.sbase; };
+  auto cell_c = [&](int t) { return cell_ok(t) ? c : 0; };
 
   // edge loaders: lanes 0..16*NROLE-1 of the first wave, one (role, index)
   // each.  Roles: 0 col-upstream, 1 row-upstream, 2 col-downstream, 3
-  // row-downstream; the upstream ones read new values, the others old ones.
+  // row-downstream; the upstream ones read new values (agent-scope loads),
+  // the others old ones.  A neighbour of a cell on plane s lies on s -/+ dir.
   const int xrole = lane / TW, xi = lane % TW;
   const bool xloader = lane < NROLE * TW;
-  auto xload = [&](int sig, double* xv) {
-    bool ok = false;
-    long long q = 0;
-    if (xloader) {
-      const bool upr = xrole < 2;
-      const int sp = upr ? sig - dir : sig + dir;       // plane of the neighbour cells
-      const int off = upr ? -dir : dir;
-      int rr, cc;
-      if ((xrole & 1) == 0) {  // across the col edge
-        cc = (upr == FWD ? T.c0 : T.c0 + TW - 1) + off;
-        rr = T.r0 + xi;
-      } else {  // across the row edge
-        rr = (upr == FWD ? T.r0 : T.r0 + TW - 1) + off;
-        cc = T.c0 + xi;
-      }
-      const int mm = sp - rr - cc;
-      ok = rr >= 0 && rr < nr && cc >= 0 && cc < nc && mm >= 0 && mm < nm;
-      if (ok) q = sweep_index(b, mm, rr, cc);
-      if (ok) {
-#pragma unroll
-        for (int f = 0; f < 8; ++f) xv[f] = S.D[f * N + q];
-        if (upr) {
-#pragma unroll
-          for (int f = 0; f < 5; ++f)
-            xv[8 + f] = __hip_atomic_load(&S.dq[f * N + q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        } else {
-#pragma unroll
-          for (int f = 0; f < 5; ++f) xv[8 + f] = S.dq[f * N + q];
-        }
-      }
+  const bool xup = xrole < 2;
+  int xr, xc;
+  {
+    const int off = xup ? -dir : dir;
+    if ((xrole & 1) == 0) {  // across the col edge
+      xc = (xup == FWD ? T.c0 : T.c0 + TW - 1) + off;
+      xr = T.r0 + xi;
+    } else {  // across the row edge
+      xr = (xup == FWD ? T.r0 : T.r0 + TW - 1) + off;
+      xc = T.c0 + xi;
     }
-    if (!ok) {
+  }
+  const bool xin = xloader && xr >= 0 && xr < nr && xc >= 0 && xc < nc;
+  auto xload = [&](int t, double* xv) {  // edge values for step t
+    const int sp = s0 + dir * t + (xup ? -dir : dir);
+    const int mm = sp - xr - xc;
+    const bool ok = xin && t < nsteps && mm >= 0 && mm < nm;
+    const long long rb = ok ? b.sbase + ((long long)sp * nr + xr) * ncp : b.sbase;
+    const int cc = ok ? xc : 0;
+    if (xup) {
+      load_nbr<false>(S, rb, cc, ncp, xv);
+      const double* dqp = reinterpret_cast<const double*>(S.dq);
 #pragma unroll
-      for (int f = 0; f < NW; ++f) xv[f] = 0.0;
+      for (int f = 0; f < 5; ++f)
+        xv[8 + f] = __hip_atomic_load(&dqp[sweep_field(rb + cc, cc, ncp, SW_DQ, f)], __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      load_nbr<GS>(S, rb, cc, ncp, xv);
     }
   };
-  // wait (thread 0) until the upstream tiles have published plane s
-  unsigned long long seen_c = 0, seen_r = 0;
-  auto wait_up = [&](int s) {
+
+  // thread 0: the upstream tiles must have finished plane s; pre[] holds the
+  // flag values read earlier, re-read (bounded spin) only when short
+  unsigned long long pre[2] = {0, 0};
+  auto ensure_up = [&](int s) {
     if (lane != 0) return;
-    bool got = false;
-    for (int q = 0; q < 2; ++q) {
-      const int u = q ? up_r : up_c;
+    for (int k = 0; k < 2; ++k) {
+      const int u = k ? up_r : up_c;
       if (u < 0) continue;
-      const unsigned long long k = need_key(FWD, ut[q], s);
-      if (k == 0) continue;
-      const unsigned long long want = ep + k;
-      unsigned long long& seen = q ? seen_r : seen_c;
-      if (seen >= want) continue;
-      unsigned long long v = __hip_atomic_load(&flags[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const unsigned long long key = need_key(FWD, ut[k], s);
+      if (key == 0) continue;
+      const unsigned long long want = ep + key;
+      unsigned long long v = pre[k];
       long long spins = 0;
       while (v < want) {
         // far beyond any legal wait (a whole sweep takes milliseconds), or
@@ -420,81 +474,64 @@ __global__ void __launch_bounds__(TL, WAVES) lusgs_tile_kernel(SweepArrays S, co
           v = want;
           break;
         }
-        __builtin_amdgcn_s_sleep(2);
+        if (spins > 1) __builtin_amdgcn_s_sleep(2);
         v = __hip_atomic_load(&flags[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
-      seen = v;
-      got = true;
+      pre[k] = v;
     }
-    if (got) {
-      if (sync == 0) __threadfence();
-      else if (sync == 1) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    }
+  };
+  auto read_up = [&]() {  // thread 0: early flag reads
+    if (lane != 0) return;
+    if (up_c >= 0) pre[0] = __hip_atomic_load(&flags[up_c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (up_r >= 0) pre[1] = __hip_atomic_load(&flags[up_r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   };
 
-  // own inputs of step t (cur), the neighbour view (D, old dq) of this
-  // lane's cell of step t+1 (nxt); each step loads the own inputs of t+1 and
-  // the neighbour view of t+2
-  auto cell_of = [&](int t, int* m) {
-    const int s = s0 + dir * t;
-    *m = s - r - c;
-    return lane_ok && t >= 0 && t < nsteps && *m >= 0 && *m < nm;
-  };
-  Own cur;
-  double nxt[NW];
-  double Dcur[8];
+  Own own0, own1;
+  double Dcur[8], nb1[NB], nb2[NB], xv0[NW], xv1[NW];
   {
-    int m0, m1;
-    const bool ok0 = cell_of(0, &m0), ok1 = cell_of(1, &m1);
-    load_own<TYPE>(S, ok0 ? sweep_index(b, m0, r, c) : 0, ok0, cur);
-    double t0[NW];
-    load_nbr<false>(S, ok0 ? sweep_index(b, m0, r, c) : 0, ok0, t0);
+    double t0[NB];
+    load_own<TYPE>(S, cell_rb(0), cell_c(0), ncp, own0);
+    load_own<TYPE>(S, cell_rb(1), cell_c(1), ncp, own1);
+    load_nbr<GS>(S, cell_rb(0), cell_c(0), ncp, t0);
 #pragma unroll
     for (int f = 0; f < 8; ++f) Dcur[f] = t0[f];
-    load_nbr<GS>(S, ok1 ? sweep_index(b, m1, r, c) : 0, ok1, nxt);
+    load_nbr<GS>(S, cell_rb(1), cell_c(1), ncp, nb1);
+    load_nbr<GS>(S, cell_rb(2), cell_c(2), ncp, nb2);
   }
 #pragma unroll
   for (int f = 0; f < NW; ++f) shA[f][lane] = 0.0;
-  double xv[NW];
-  wait_up(s0 - dir);
-  __syncthreads();
-  xload(s0, xv);
+  ensure_up(s0);  // covers the edge values of steps 0 and 1
+  if (xloader) {
+    xload(0, xv0);
+    xload(1, xv1);
+  }
+  read_up();
 
   for (int t = 0; t < nsteps; ++t) {
     const int s = s0 + dir * t;
-    int m;
-    const bool act = cell_of(t, &m);
-    // phase 1: publish edge values and (GS) next-cell values, wait upstream
+    const int m = m0 + dir * t;
+    const bool act = cell_ok(t);
+    // phase 1
     if (xloader) {
 #pragma unroll
-      for (int f = 0; f < NW; ++f) shX[xrole][f][xi] = xv[f];
+      for (int f = 0; f < NW; ++f) shX[xrole][f][xi] = xv0[f];
     }
     if (GS) {
 #pragma unroll
-      for (int f = 0; f < NW; ++f) shB[f][lane] = nxt[f];
+      for (int f = 0; f < NW; ++f) shB[f][lane] = nb1[f];
     }
-    wait_up(s);
     __syncthreads();
-    // phase 2: loads for later steps, then this cell
-    xload(s + dir, xv);
-    Own own_n;
-    double nxt2[NW];
-    {
-      int m1, m2;
-      const bool ok1 = cell_of(t + 1, &m1), ok2 = cell_of(t + 2, &m2);
-      load_own<TYPE>(S, ok1 ? sweep_index(b, m1, r, c) : 0, ok1, own_n);
-      load_nbr<GS>(S, ok2 ? sweep_index(b, m2, r, c) : 0, ok2, nxt2);
-    }
+    // phase 2
     double dqn[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     if (act) {
       double r0[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
       AxisGeo ag[3];
-      axis_geo(&cur.met[0], ag[0]);
-      axis_geo(&cur.met[3], ag[1]);
-      axis_geo(&cur.met[6], ag[2]);
-      // the lane's previous cell is its own entry of shA; for the
-      // Gauss-Seidel sweeps its next cell is its own entry of shB
-      // neighbour of real slot sl: returns its (D, dq) and validity
+      axis_geo(&own0.met[0], ag[0]);
+      axis_geo(&own0.met[3], ag[1]);
+      axis_geo(&own0.met[6], ag[2]);
+      // neighbour of real slot sl: its (D, dq) and validity.  The lane's
+      // previous cell is its own entry of shA; for the Gauss-Seidel sweeps
+      // its next cell is its own entry of shB.
       auto nb = [&](int sl, double* v) -> bool {
         const int ax = sl % 3, d = sl < 3 ? -1 : 1;
         const bool before = (d == -dir);  // precedes this cell in sweep order
@@ -532,59 +569,69 @@ __global__ void __launch_bounds__(TL, WAVES) lusgs_tile_kernel(SweepArrays S, co
       double v[NW];
       if (TYPE == 1) {
         bool ok;
-        ok = nb(3, v); slot<false, 3>(&cur.met[0], ag[0], v, v + 8, ok, gm1, nvis, r0);
-        ok = nb(4, v); slot<false, 4>(&cur.met[3], ag[1], v, v + 8, ok, gm1, nvis, r0);
-        ok = nb(5, v); slot<false, 5>(&cur.met[6], ag[2], v, v + 8, ok, gm1, nvis, r0);
+        ok = nb(3, v); slot<false, 3>(&own0.met[0], ag[0], v, v + 8, ok, gm1, nvis, r0);
+        ok = nb(4, v); slot<false, 4>(&own0.met[3], ag[1], v, v + 8, ok, gm1, nvis, r0);
+        ok = nb(5, v); slot<false, 5>(&own0.met[6], ag[2], v, v + 8, ok, gm1, nvis, r0);
       } else {
         bool ok;
-        ok = nb(0, v); slot<GS, 0>(&cur.met[0], ag[0], v, v + 8, ok, gm1, nvis, r0);
-        ok = nb(1, v); slot<GS, 1>(&cur.met[3], ag[1], v, v + 8, ok, gm1, nvis, r0);
-        ok = nb(2, v); slot<GS, 2>(&cur.met[6], ag[2], v, v + 8, ok, gm1, nvis, r0);
+        ok = nb(0, v); slot<GS, 0>(&own0.met[0], ag[0], v, v + 8, ok, gm1, nvis, r0);
+        ok = nb(1, v); slot<GS, 1>(&own0.met[3], ag[1], v, v + 8, ok, gm1, nvis, r0);
+        ok = nb(2, v); slot<GS, 2>(&own0.met[6], ag[2], v, v + 8, ok, gm1, nvis, r0);
         if (GS) {
-          ok = nb(3, v); slot<true, 3>(&cur.met[0], ag[0], v, v + 8, ok, gm1, nvis, r0);
-          ok = nb(4, v); slot<true, 4>(&cur.met[3], ag[1], v, v + 8, ok, gm1, nvis, r0);
-          ok = nb(5, v); slot<true, 5>(&cur.met[6], ag[2], v, v + 8, ok, gm1, nvis, r0);
+          ok = nb(3, v); slot<true, 3>(&own0.met[0], ag[0], v, v + 8, ok, gm1, nvis, r0);
+          ok = nb(4, v); slot<true, 4>(&own0.met[3], ag[1], v, v + 8, ok, gm1, nvis, r0);
+          ok = nb(5, v); slot<true, 5>(&own0.met[6], ag[2], v, v + 8, ok, gm1, nvis, r0);
         }
       }
       const double wmig = 1.0;
       if (TYPE == 0) {
 #pragma unroll
-        for (int f = 0; f < 5; ++f) dqn[f] = (-cur.r5[f] + wmig * r0[f]) * cur.coed;
+        for (int f = 0; f < 5; ++f) dqn[f] = (-own0.r5[f] + wmig * r0[f]) * own0.coed;
       } else if (TYPE == 1) {
 #pragma unroll
-        for (int f = 0; f < 5; ++f) dqn[f] = cur.r5[f] - (wmig * r0[f]) * cur.coed;
+        for (int f = 0; f < 5; ++f) dqn[f] = own0.r5[f] - (wmig * r0[f]) * own0.coed;
       } else {
 #pragma unroll
-        for (int f = 0; f < 5; ++f) dqn[f] = (wmig * r0[f] + -cur.r5[f]) * cur.coed;
+        for (int f = 0; f < 5; ++f) dqn[f] = (wmig * r0[f] + -own0.r5[f]) * own0.coed;
       }
-      const long long q = sweep_index(b, m, r, c);
-      if (sync == 2) {
+      const long long q = rb0 + (long long)t * qstep + c;
+      double* dqp = reinterpret_cast<double*>(S.dq);
 #pragma unroll
-        for (int f = 0; f < 5; ++f) __hip_atomic_store(&S.dq[f * N + q], dqn[f], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      } else {
-#pragma unroll
-        for (int f = 0; f < 5; ++f) S.dq[f * N + q] = dqn[f];
-      }
+      for (int f = 0; f < 5; ++f)
+        __hip_atomic_store(&dqp[sweep_field(q, c, ncp, SW_DQ, f)], dqn[f], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     const bool publish = (t % pub == pub - 1) || t == nsteps - 1;
-    if (publish) {
-      if (sync == 0) __threadfence();
-      else if (sync == 1) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-      else __builtin_amdgcn_s_waitcnt(0);  // this wave's agent-coherent stores have completed
-    }
+    if (publish) __builtin_amdgcn_s_waitcnt(0);  // this wave's agent-coherent stores have completed
     __syncthreads();
-    // phase 3: progress flag, this step's values for the next step
+    // phase 3
     if (publish && lane == 0)
       __hip_atomic_store(&flags[tix], ep + plane_key(FWD, s), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 #pragma unroll
     for (int f = 0; f < 8; ++f) shA[f][lane] = Dcur[f];
 #pragma unroll
     for (int f = 0; f < 5; ++f) shA[8 + f][lane] = dqn[f];
+    if (t + 2 < nsteps) ensure_up(s + dir);  // edge values of step t+2 lie on plane s(t+1)
+    double xv2[NW];
+    if (xloader) xload(t + 2, xv2);
+    Own own2;
+    double nb3[NB];
+    load_own<TYPE>(S, cell_rb(t + 2), cell_c(t + 2), ncp, own2);
+    load_nbr<GS>(S, cell_rb(t + 3), cell_c(t + 3), ncp, nb3);
+    read_up();
 #pragma unroll
-    for (int f = 0; f < 8; ++f) Dcur[f] = nxt[f];
-    cur = own_n;
+    for (int f = 0; f < 8; ++f) Dcur[f] = nb1[f];
+    own0 = own1;
+    own1 = own2;
 #pragma unroll
-    for (int f = 0; f < NW; ++f) nxt[f] = nxt2[f];
+    for (int f = 0; f < NB; ++f) {
+      nb1[f] = nb2[f];
+      nb2[f] = nb3[f];
+    }
+#pragma unroll
+    for (int f = 0; f < NW; ++f) {
+      xv0[f] = xv1[f];
+      xv1[f] = xloader ? xv2[f] : 0.0;
+    }
   }
 }
 
@@ -611,21 +658,17 @@ void launch_update(const Launch& La, long long n, const SweepArrays& S, double g
 }
 
 void launch_lusgs_tiles(const SweepArrays& S, const BlockDev* blocks, const SweepTile* tiles, const int* order,
-                        int ntiles, unsigned long long* flags, unsigned long long* epoch, int* err, int pub, int sync,
-                        int waves, int type, const Params& p, hipStream_t s) {
+                        int ntiles, unsigned long long* flags, unsigned long long* epoch, int* err, int pub, int type,
+                        const Params& p, hipStream_t s) {
   if (ntiles <= 0) return;
   hipLaunchKernelGGL(epoch_kernel, dim3(1), dim3(64), 0, s, epoch);
   const double gm1 = p.gama - 1.0;
   const dim3 gr(ntiles), tb(TL);
-#define ORION_TILE_LAUNCH(T, W) \
-  hipLaunchKernelGGL((lusgs_tile_kernel<T, W>), gr, tb, 0, s, S, blocks, tiles, order, flags, epoch, err, pub, sync, gm1, p.nvis)
-  const bool one = waves == 1;
   switch (type) {
-    case 0: if (one) ORION_TILE_LAUNCH(0, 1); else ORION_TILE_LAUNCH(0, 2); break;
-    case 1: if (one) ORION_TILE_LAUNCH(1, 1); else ORION_TILE_LAUNCH(1, 2); break;
-    case 2: if (one) ORION_TILE_LAUNCH(2, 1); else ORION_TILE_LAUNCH(2, 2); break;
-    default: if (one) ORION_TILE_LAUNCH(3, 1); else ORION_TILE_LAUNCH(3, 2); break;
-#undef ORION_TILE_LAUNCH
+    case 0: hipLaunchKernelGGL(lusgs_tile_kernel<0>, gr, tb, 0, s, S, blocks, tiles, order, flags, epoch, err, pub, gm1, p.nvis); break;
+    case 1: hipLaunchKernelGGL(lusgs_tile_kernel<1>, gr, tb, 0, s, S, blocks, tiles, order, flags, epoch, err, pub, gm1, p.nvis); break;
+    case 2: hipLaunchKernelGGL(lusgs_tile_kernel<2>, gr, tb, 0, s, S, blocks, tiles, order, flags, epoch, err, pub, gm1, p.nvis); break;
+    default: hipLaunchKernelGGL(lusgs_tile_kernel<3>, gr, tb, 0, s, S, blocks, tiles, order, flags, epoch, err, pub, gm1, p.nvis); break;
   }
 }
 

@@ -50,16 +50,25 @@ struct BlockDev {
 
 // LU-SGS working set in sweep layout (all blocks, n cells incl. padding).
 // A block's cell (m, r, c) along (march, row, col) lies on hyperplane
-// s = m + r + c and is stored at sbase + (s * nrow + r) * ncol_padded + c:
-// the cells of one hyperplane row are contiguous.
+// s = m + r + c; its cell index q = sbase + (s * nrow + r) * ncol_padded + c
+// puts one hyperplane row in contiguous cells.  Fields are stored in pairs,
+// so that one 16-byte load brings two fields of a cell: pair k of a group
+// of K pairs lies at (q - c) * K + k * ncol_padded + c, i.e. every row holds
+// K planes of pairs and consecutive columns are consecutive pairs.
+constexpr int SW_OWN = 8;  // met0..met8, coed, rhs0..rhs4 (+1 unused)
+constexpr int SW_D = 4;    // u, v, w, c2, H, af, c, 2mu/(|Re| rho vol + 1e-30)
+constexpr int SW_DQ = 3;   // dq0..dq4 (+1 unused)
 struct SweepArrays {
-  double* met;   // [9][n] metrics
-  double* D;     // [8][n] u, v, w, c2, H, af, c, 2mu/(|Re| rho vol + 1e-30)
-  double* coed;  // [n] diagonal 1/(1/dt + sum of radii)
-  double* rhs;   // [5][n] right-hand side (zero on physical-BC windows)
-  double* dq;    // [5][n] increment, updated in place by the sweeps
+  double2* own;
+  double2* D;
+  double2* dq;
   long long n;
 };
+
+// field f of the cell (q, c) in a pair group of K pairs
+__host__ __device__ inline long long sweep_field(long long q, int c, int ncp, int K, int f) {
+  return 2 * ((q - c) * K + (long long)(f >> 1) * ncp + c) + (f & 1);
+}
 
 // One tile of the LU-SGS pipeline: 16 x 16 (row, col) pencils of a block.
 struct SweepTile {
