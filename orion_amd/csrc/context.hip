@@ -70,7 +70,6 @@ struct orion_hip_ctx {
   unsigned long long* d_flags = nullptr; // per-tile progress
   unsigned long long* d_epoch = nullptr; // sweep counter
   int* d_err = nullptr;                  // set by a sweep whose wait ran out
-  int sweep_pub = 1;                     // steps between progress publications
   long long* d_halo = nullptr;
   long long n_halo = 0;
   BCTask* d_bc = nullptr;
@@ -518,16 +517,19 @@ static int build_sweep(orion_hip_ctx* c) {
   if (dalloc(c, (void**)&c->sw.own, sizeof(double2) * SW_OWN * n) ||
       dalloc(c, (void**)&c->sw.D, sizeof(double2) * SW_D * n) ||
       dalloc(c, (void**)&c->sw.dq, sizeof(double2) * SW_DQ * n) ||
+      dalloc(c, (void**)&c->sw.ga, sizeof(double2) * SW_GA * n) ||
+      dalloc(c, (void**)&c->sw.pre, sizeof(double2) * SW_PRE * n) ||
       dalloc(c, (void**)&c->d_flags, sizeof(unsigned long long) * std::max(nt, 1)) ||
       dalloc(c, (void**)&c->d_epoch, sizeof(unsigned long long)) || dalloc(c, (void**)&c->d_err, sizeof(int)))
     return -1;
   HIPCHK(hipMemset(c->sw.dq, 0, sizeof(double2) * SW_DQ * n));
   HIPCHK(hipMemset(c->sw.own, 0, sizeof(double2) * SW_OWN * n));
   HIPCHK(hipMemset(c->sw.D, 0, sizeof(double2) * SW_D * n));
+  HIPCHK(hipMemset(c->sw.ga, 0, sizeof(double2) * SW_GA * n));
+  HIPCHK(hipMemset(c->sw.pre, 0, sizeof(double2) * SW_PRE * n));
   HIPCHK(hipMemset(c->d_flags, 0, sizeof(unsigned long long) * std::max(nt, 1)));
   HIPCHK(hipMemset(c->d_epoch, 0, sizeof(unsigned long long)));
   HIPCHK(hipMemset(c->d_err, 0, sizeof(int)));
-  if (const char* e = std::getenv("ORION_LU_PUBLISH")) c->sweep_pub = std::max(1, std::atoi(e));
   return 0;
 }
 
@@ -835,7 +837,7 @@ static void op_inv(orion_hip_ctx* c) {
 static void op_sweep(orion_hip_ctx* c, int type) {
   const int bwd = (type == 1 || type == 3) ? 1 : 0;
   launch_lusgs_tiles(c->sw, c->db, c->d_tiles, c->d_order[bwd], (int)c->tiles.size(), c->d_flags, c->d_epoch,
-                     c->d_err, c->sweep_pub, type, c->p, c->stream);
+                     c->d_err, type, c->p, c->stream);
 }
 
 static void op_update(orion_hip_ctx* c) {

@@ -210,10 +210,9 @@ __global__ void __launch_bounds__(256) update_kernel(const BlockDev* __restrict_
 }
 
 // ---------------------------------------------------------------------------
-// one neighbour slot: coefficients from this cell's metrics and the
-// neighbour's packed values (compute_flux_splitting :209-254 with
-// calc_spectral_radius and calc_visc_spectral_radius), then the dq-linear
-// remainder (:255-264) and the viscous off-diagonal (:341-343 / :419-421)
+// one neighbour slot: the inviscid part of compute_flux_splitting (:209-264,
+// with calc_spectral_radius) from this cell's metrics and the neighbour's
+// packed values and dq
 // ---------------------------------------------------------------------------
 struct AxisGeo {
   double g2, sq, gm, im;  // |n|^2, sqrt(|n|^2), clamped, 1/clamped
@@ -228,13 +227,12 @@ __device__ __forceinline__ void axis_geo(const double* n, AxisGeo& a) {
 
 // n: this cell's metrics of the slot's axis; Dn, d: the neighbour's packed
 // values and dq
-template <bool GS, int SL>
-__device__ __forceinline__ void slot(const double* n, const AxisGeo& a, const double* Dn, const double* d, bool valid,
-                                     double gm1, int nvis, double* r0) {
+template <int SL>
+__device__ __forceinline__ void slot_inv(const double* n, const AxisGeo& a, const double* Dn, const double* d,
+                                         double gm1, double* res) {
   constexpr int sgn = SL < 3 ? 1 : -1;
   const double nx = n[0], ny = n[1], nz = n[2];
   const double um = Dn[0], vm = Dn[1], wm = Dn[2], c2 = Dn[3], H = Dn[4], af = Dn[5], cm = Dn[6];
-  // calc_spectral_radius with this cell's metrics and the neighbour's state
   const double U = um * nx + vm * ny + wm * nz + 0.0;
   const double r = fabs(U) + cm * a.sq;
   const double ct = nx * um + ny * vm + nz * wm + 0.0;
@@ -250,27 +248,11 @@ __device__ __forceinline__ void slot(const double* n, const AxisGeo& a, const do
   const double dc = ctn * drm - (nxn * dum + nyn * dvm + nzn * dwm);
   const double dh = af * drm - gm1 * (um * dum + vm * dvm + wm * dwm - dEm);
   const double c2dc = c2 * dc;
-  double res[5];
   res[0] = l1 * drm - dh * x1 - dc * x2;
   res[1] = l1 * dum + (nxn * c2dc - um * dh) * x1 + (nxn * dh - um * dc) * x2;
   res[2] = l1 * dvm + (nyn * c2dc - vm * dh) * x1 + (nyn * dh - vm * dc) * x2;
   res[3] = l1 * dwm + (nzn * c2dc - wm * dh) * x1 + (nzn * dh - wm * dc) * x2;
   res[4] = l1 * dEm + (ctn * c2dc - H * dh) * x1 + (ctn * dh - H * dc) * x2;
-  if (nvis == 1) {
-    const double hv = 0.5 * (Dn[7] * a.g2);
-    if (GS) {
-#pragma unroll
-      for (int f = 0; f < 5; ++f) res[f] += hv * d[f] * (double)sgn;
-    } else {
-#pragma unroll
-      for (int f = 0; f < 5; ++f) res[f] += hv * d[f];
-    }
-  }
-#pragma unroll
-  for (int f = 0; f < 5; ++f) {
-    const double acc = (GS && sgn == -1) ? r0[f] - res[f] : r0[f] + res[f];
-    r0[f] = valid ? acc : r0[f];
-  }
 }
 
 // ---------------------------------------------------------------------------
@@ -293,95 +275,144 @@ __device__ inline unsigned long long need_key(bool fwd, const SweepTile& u, int 
   return plane_key(false, s > u.s_first ? s : u.s_first);
 }
 
-struct Own {  // a cell's own inputs to its update
-  double met[9];
-  double coed;
-  double r5[5];  // rhs, or (backward LU) the forward sweep's dq
+// s_waitcnt immediate that waits until at most n vector memory operations
+// of the wave are outstanding (GFX9 encoding; expcnt and lgkmcnt at their
+// maxima, i.e. not waited on)
+constexpr unsigned waitcnt_vm(unsigned n) { return (n & 15u) | ((n >> 4) << 14) | (7u << 4) | (15u << 8); }
+
+// What a lane loads for one step of its pencil besides the neighbours'
+// values, which come through LDS:
+//   own   met0..8, coed, r5 (rhs; for the LU backward sweep the cell's dq
+//         from the forward sweep); Gauss-Seidel backward appends the
+//         partial sum its forward sweep left (5)
+//   ga    Gauss-Seidel forward: the three after-neighbour terms (15)
+//   D     the cell's packed values, handed on to its downstream neighbours
+//   xd,xq the lane's share of the upstream edge values (first wave only)
+template <int TYPE>
+struct StepIn {
+  double own[TYPE == 3 ? 20 : 15];
+  double ga[TYPE == 2 ? 15 : 1];
+  double D[8];
+  double2 xd[2];
+  double xq[3];
 };
 
-// Loads are unconditional: a lane whose cell is absent reads the block's
-// first cell instead (always a valid address) and never uses the values.
-// (rb, c): the cell's row base q - c and column; pair k of a K-pair group
-// is one 16-byte load
+// Sweep layout pair loads of the own inputs: 16-byte loads, unconditional
+// (a lane whose cell is absent reads the block's first cell, always a valid
+// address, and never uses the values).  (rb, c): the cell's row base q - c
+// and column.
 __device__ __forceinline__ double2 ld_pair(const double2* g, long long rb, int c, int ncp, int K, int k) {
   return g[rb * K + (long long)k * ncp + c];
 }
 
-template <int TYPE>
-__device__ __forceinline__ void load_own(const SweepArrays& S, long long rb, int c, int ncp, Own& o) {
-  double2 v[SW_OWN];
-#pragma unroll
-  for (int k = 0; k < (TYPE == 1 ? 5 : SW_OWN); ++k) v[k] = ld_pair(S.own, rb, c, ncp, SW_OWN, k);
-  if (TYPE == 1) {
-#pragma unroll
-    for (int k = 0; k < SW_DQ; ++k) v[5 + k] = ld_pair(S.dq, rb, c, ncp, SW_DQ, k);
-  }
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    o.met[2 * k] = v[k].x;
-    o.met[2 * k + 1] = v[k].y;
-  }
-  o.met[8] = v[4].x;
-  o.coed = v[4].y;
-  o.r5[0] = v[5].x;
-  o.r5[1] = v[5].y;
-  o.r5[2] = v[6].x;
-  o.r5[3] = v[6].y;
-  o.r5[4] = v[7].x;
+// a pair whose second half is padding is loaded as one double (the
+// register of a loaded but unused half could be reused while the load is in
+// flight, which costs a wait)
+__device__ __forceinline__ double ld_half(const double2* g, long long rb, int c, int ncp, int K, int k) {
+  return reinterpret_cast<const double*>(g)[2 * (rb * K + (long long)k * ncp + c)];
 }
 
-// D (8) and, for the Gauss-Seidel sweeps, the cell's old dq (5); x holds
-// 13 words for the Gauss-Seidel sweeps and 8 for the LU ones
-template <bool GS>
-__device__ __forceinline__ void load_nbr(const SweepArrays& S, long long rb, int c, int ncp, double* x) {
+template <int TYPE>
+__device__ __forceinline__ void load_own(const SweepArrays& S, long long rb, int c, int ncp, double* o) {
+  double2 v[10];
+  double last, lastp = 0.0;
+  if constexpr (TYPE == 1) {
+#pragma unroll
+    for (int k = 0; k < 5; ++k) v[k] = ld_pair(S.own, rb, c, ncp, SW_OWN, k);
+#pragma unroll
+    for (int k = 0; k < 2; ++k) v[5 + k] = ld_pair(S.dq, rb, c, ncp, SW_DQ, k);
+    last = ld_half(S.dq, rb, c, ncp, SW_DQ, 2);
+  } else {
+#pragma unroll
+    for (int k = 0; k < 7; ++k) v[k] = ld_pair(S.own, rb, c, ncp, SW_OWN, k);
+    last = ld_half(S.own, rb, c, ncp, SW_OWN, 7);
+  }
+  if constexpr (TYPE == 3) {
+#pragma unroll
+    for (int k = 0; k < 2; ++k) v[7 + k] = ld_pair(S.pre, rb, c, ncp, SW_PRE, k);
+    lastp = ld_half(S.pre, rb, c, ncp, SW_PRE, 2);
+  }
+#pragma unroll
+  for (int i = 0; i < 14; ++i) o[i] = (i & 1) ? v[i >> 1].y : v[i >> 1].x;
+  o[14] = last;
+  if constexpr (TYPE == 3) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) o[15 + i] = (i & 1) ? v[7 + (i >> 1)].y : v[7 + (i >> 1)].x;
+    o[19] = lastp;
+  }
+}
+
+template <int TYPE>
+__device__ __forceinline__ void load_mid(const SweepArrays& S, long long rb, int c, int ncp, StepIn<TYPE>& P) {
 #pragma unroll
   for (int k = 0; k < SW_D; ++k) {
     const double2 v = ld_pair(S.D, rb, c, ncp, SW_D, k);
-    x[2 * k] = v.x;
-    x[2 * k + 1] = v.y;
+    P.D[2 * k] = v.x;
+    P.D[2 * k + 1] = v.y;
   }
-  if constexpr (GS) {
-    const double2 a = ld_pair(S.dq, rb, c, ncp, SW_DQ, 0), b2 = ld_pair(S.dq, rb, c, ncp, SW_DQ, 1);
-    const double2 e = ld_pair(S.dq, rb, c, ncp, SW_DQ, 2);
-    x[8] = a.x;
-    x[9] = a.y;
-    x[10] = b2.x;
-    x[11] = b2.y;
-    x[12] = e.x;
+  if constexpr (TYPE == 2) {
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+      const double2 v = ld_pair(S.ga, rb, c, ncp, SW_GA, k);
+      P.ga[2 * k] = v.x;
+      P.ga[2 * k + 1] = v.y;
+    }
+    P.ga[14] = ld_half(S.ga, rb, c, ncp, SW_GA, 7);
   }
 }
 
-// Sweep types: 0 LU forward, 1 LU backward, 2 GS forward, 3 GS backward.
+// Vector memory loads every wave issues per step (own inputs, D, ga); the
+// first wave issues 5 more (edge values) after them.
+template <int TYPE>
+constexpr unsigned step_loads() {
+  return (TYPE == 1 ? 5 + SW_DQ : SW_OWN) + (TYPE == 3 ? SW_PRE : 0) + SW_D + (TYPE == 2 ? SW_GA : 0);
+}
+
+// Sweep types: 0 LU forward, 1 LU backward, 2 Gauss-Seidel forward,
+// 3 Gauss-Seidel backward.
+//
+// In every sweep a cell's update sums six neighbour terms in the order of
+// slots 0..5 (the -1 neighbours along i, j, k, then the +1 ones).  The
+// terms of the neighbours before the cell in sweep order use their new dq
+// and are computed here; for the Gauss-Seidel sweeps the other three use
+// values that an earlier sweep already finished, and they come in ready:
+//  * Gauss-Seidel forward: its after-neighbours (slots 3..5) carry the LU
+//    backward sweep's final dq, and the LU backward sweep evaluates exactly
+//    those terms for the same cell pairs (its own before-neighbours).  It
+//    stores them (with the Gauss-Seidel sign of the viscous part) in ga,
+//    zero for a missing neighbour, so that r - 0 leaves r unchanged.
+//  * Gauss-Seidel backward: its sum starts with slots 0..2, whose
+//    neighbours carry the forward sweep's final dq, which is what the
+//    forward sweep used for them: the forward sweep's partial sum after its
+//    first three slots is stored in pre and is the backward sweep's start.
 //
 // Pipeline of one tile.  Step t works on hyperplane s(t) = s0 + dir*t:
-//   phase 1  edge values (loaded two steps ago) and, for Gauss-Seidel, the
-//            lanes' next cells go to LDS; barrier;
-//   phase 2  the update of each lane's cell from LDS and registers; dq is
-//            stored with agent-scope stores; when publishing, each wave waits
-//            for its stores; barrier;
-//   phase 3  thread 0 publishes plane s(t); the lanes' (D, new dq) go to
-//            LDS; thread 0 makes sure the upstream tiles have finished plane
-//            s(t+1), from flag values it read one step earlier (it spins
-//            only when they have not); then the loads for later steps are
-//            issued: own inputs of t+2, neighbour views of t+3, edge values
-//            of t+2.
-// Issuing every prefetch after the store wait keeps that wait from draining
-// them, and reading the flags a step early keeps the flag round trip off
-// the critical path, at the price of one step of lag per tile edge.
+//   A  the first wave puts this step's upstream edge values into LDS;
+//   B  each wave waits for its own stores of step t-1 (by count: only the
+//      loads issued after them may stay in flight); barrier;
+//   C  thread 0 publishes plane s(t-1);
+//   D  the update of each lane's cell from LDS and registers; dq is stored
+//      with agent-scope stores (ga / pre with plain ones); the lane's
+//      (D, new dq) go to LDS;
+//   E  thread 0 makes sure the upstream tiles have finished plane s(t+1),
+//      from flag values it read one step earlier (it spins only when they
+//      have not); then the loads: own inputs and edge values of step t+2,
+//      D and ga of step t+1.
+// LDS buffers alternate by step parity, so one barrier per step suffices.
+// The three StepIn buffers rotate by unrolling (no register copies, which
+// would wait for the loads still in flight).
 template <int TYPE>
 __global__ void __launch_bounds__(TL, 1) lusgs_tile_kernel(SweepArrays S, const BlockDev* __restrict__ blocks,
                                                           const SweepTile* __restrict__ tiles,
                                                           const int* __restrict__ order, unsigned long long* flags,
                                                           const unsigned long long* __restrict__ epoch, int* err,
-                                                          int pub, double gm1, int nvis) {
+                                                          double gm1, int nvis) {
   constexpr bool FWD = (TYPE == 0 || TYPE == 2);
-  constexpr bool GS = TYPE >= 2;
   constexpr int dir = FWD ? 1 : -1;
-  constexpr int NROLE = GS ? 4 : 2;
-  constexpr int NB = GS ? NW : 8;              // words of a neighbour view
-  __shared__ double shA[NW][TL];               // (D, new dq) of each lane's cell of the last step
-  __shared__ double shB[GS ? NW : 1][TL];      // (D, old dq) of each lane's cell of the next step
-  __shared__ double shX[NROLE][NW][TW];        // edge values from neighbouring tiles
+  constexpr int SL0 = FWD ? 0 : 3;  // slots of the neighbours before a cell in sweep order
+  constexpr unsigned NLD = step_loads<TYPE>();
+  __shared__ double shA[2][NW][TL];     // (D, new dq) of each lane's cell, by step parity
+  __shared__ double shX[2][2][NW][TW];  // upstream edge values, by step parity
 
   const int tix = order[blockIdx.x];
   const SweepTile T = tiles[tix];
@@ -398,57 +429,75 @@ __global__ void __launch_bounds__(TL, 1) lusgs_tile_kernel(SweepArrays S, const 
   ut[1] = up_r >= 0 ? tiles[up_r] : T;
   const int nsteps = T.s_last - T.s_first + 1;
   const int s0 = FWD ? T.s_first : T.s_last;
-  const long long plane_stride = (long long)nr * b.sncp;  // one hyperplane in the sweep layout
-  const long long qstep = dir * plane_stride;
-
-  // this lane's cell at step t: valid flag and sweep index (block's first
-  // cell when absent)
   const int ncp = b.sncp;
+  const long long qstep = dir * (long long)nr * ncp;  // one hyperplane in the sweep layout
+
+  // this lane's cell at step t: valid flag, row base q - c and column (the
+  // block's first cell when absent)
   const int m0 = s0 - r - c;
-  const long long rb0 = b.sbase + ((long long)s0 * nr + r) * ncp;  // row base q - c at step 0
+  const long long rb0 = b.sbase + ((long long)s0 * nr + r) * ncp;
   auto cell_ok = [&](int t) {
     const int m = m0 + dir * t;
     return lane_ok && t < nsteps && m >= 0 && m < nm;
   };
-  // row base and column of the lane's cell at step t (the block's first
-  // cell when absent)
   auto cell_rb = [&](int t) { return cell_ok(t) ? rb0 + (long long)t * qstep : b.sbase; };
   auto cell_c = [&](int t) { return cell_ok(t) ? c : 0; };
 
-  // edge loaders: lanes 0..16*NROLE-1 of the first wave, one (role, index)
-  // each.  Roles: 0 col-upstream, 1 row-upstream, 2 col-downstream, 3
-  // row-downstream; the upstream ones read new values (agent-scope loads),
-  // the others old ones.  A neighbour of a cell on plane s lies on s -/+ dir.
-  const int xrole = lane / TW, xi = lane % TW;
-  const bool xloader = lane < NROLE * TW;
-  const bool xup = xrole < 2;
-  int xr, xc;
-  {
-    const int off = xup ? -dir : dir;
-    if ((xrole & 1) == 0) {  // across the col edge
-      xc = (xup == FWD ? T.c0 : T.c0 + TW - 1) + off;
-      xr = T.r0 + xi;
-    } else {  // across the row edge
-      xr = (xup == FWD ? T.r0 : T.r0 + TW - 1) + off;
-      xc = T.c0 + xi;
-    }
-  }
-  const bool xin = xloader && xr >= 0 && xr < nr && xc >= 0 && xc < nc;
-  auto xload = [&](int t, double* xv) {  // edge values for step t
-    const int sp = s0 + dir * t + (xup ? -dir : dir);
-    const int mm = sp - xr - xc;
-    const bool ok = xin && t < nsteps && mm >= 0 && mm < nm;
-    const long long rb = ok ? b.sbase + ((long long)sp * nr + xr) * ncp : b.sbase;
-    const int cc = ok ? xc : 0;
-    if (xup) {
-      load_nbr<false>(S, rb, cc, ncp, xv);
-      const double* dqp = reinterpret_cast<const double*>(S.dq);
+  // Upstream edge values, loaded by the first wave (whose lane 0 checks the
+  // upstream progress): per step 2 x 16 cells of 4 D pairs and 5 dq words.
+  // Lane l loads D items l and l + 64 and dq items l, l + 64, l + 128; item
+  // e is (role, index, part), role 0 the cells across the upstream col
+  // edge, role 1 those across the upstream row edge.  A neighbour of a cell
+  // on plane s lies on s - dir.
+  const bool w0 = lane < 64;
+  int er[5], ec[5], xo[5];
+  bool ein[5];
 #pragma unroll
-      for (int f = 0; f < 5; ++f)
-        xv[8 + f] = __hip_atomic_load(&dqp[sweep_field(rb + cc, cc, ncp, SW_DQ, f)], __ATOMIC_RELAXED,
-                                      __HIP_MEMORY_SCOPE_AGENT);
+  for (int j = 0; j < 5; ++j) {
+    int role, idx, word;
+    if (j < 2) {
+      const int e = lane + 64 * j;
+      role = e >> 6;
+      idx = (e >> 2) & 15;
+      word = 2 * (e & 3);
     } else {
-      load_nbr<GS>(S, rb, cc, ncp, xv);
+      const int e = lane + 64 * (j - 2);
+      role = e < 80 ? 0 : 1;
+      idx = (e % 80) / 5;
+      word = 8 + e % 5;
+    }
+    if (role == 0) {
+      ec[j] = FWD ? T.c0 - 1 : T.c0 + TW;
+      er[j] = T.r0 + idx;
+    } else {
+      er[j] = FWD ? T.r0 - 1 : T.r0 + TW;
+      ec[j] = T.c0 + idx;
+    }
+    ein[j] = w0 && er[j] >= 0 && er[j] < nr && ec[j] >= 0 && ec[j] < nc && (j < 4 || lane < 32);
+    xo[j] = (role * NW + word) * TW + idx;  // offset in one parity of shX
+  }
+  auto edge_rb = [&](int j, int t, int* cc) -> long long {
+    const int sp = s0 + dir * (t - 1);
+    const int mm = sp - er[j] - ec[j];
+    const bool ok = ein[j] && t < nsteps && mm >= 0 && mm < nm;
+    *cc = ok ? ec[j] : 0;
+    return ok ? b.sbase + ((long long)sp * nr + er[j]) * ncp : b.sbase;
+  };
+  auto load_edges = [&](int t, StepIn<TYPE>& P) {
+    if (!w0) return;
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      int cc;
+      const long long rb = edge_rb(j, t, &cc);
+      P.xd[j] = ld_pair(S.D, rb, cc, ncp, SW_D, lane & 3);
+    }
+    const double* dqp = reinterpret_cast<const double*>(S.dq);
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      int cc;
+      const long long rb = edge_rb(2 + j, t, &cc);
+      P.xq[j] = __hip_atomic_load(&dqp[sweep_field(rb + cc, cc, ncp, SW_DQ, (lane + 64 * j) % 5)],
+                                  __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
   };
 
@@ -486,160 +535,182 @@ __global__ void __launch_bounds__(TL, 1) lusgs_tile_kernel(SweepArrays S, const 
     if (up_r >= 0) pre[1] = __hip_atomic_load(&flags[up_r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   };
 
-  Own own0, own1;
-  double Dcur[8], nb1[NB], nb2[NB], xv0[NW], xv1[NW];
-  {
-    double t0[NB];
-    load_own<TYPE>(S, cell_rb(0), cell_c(0), ncp, own0);
-    load_own<TYPE>(S, cell_rb(1), cell_c(1), ncp, own1);
-    load_nbr<GS>(S, cell_rb(0), cell_c(0), ncp, t0);
+  // before-neighbour of this lane's cell along real axis ax at step parity
+  // p: its (D, dq) and whether it exists.  Along the march axis it is the
+  // lane's own cell of the previous step.
+  auto nbr = [&](int ax, int m, int p, double* v) -> bool {
+    constexpr int d = -dir;
+    if (ax == am) {
 #pragma unroll
-    for (int f = 0; f < 8; ++f) Dcur[f] = t0[f];
-    load_nbr<GS>(S, cell_rb(1), cell_c(1), ncp, nb1);
-    load_nbr<GS>(S, cell_rb(2), cell_c(2), ncp, nb2);
-  }
+      for (int f = 0; f < NW; ++f) v[f] = shA[p ^ 1][f][lane];
+      return m + d >= 0 && m + d < nm;
+    }
+    const bool col = (ax == ac);
+    const int pos = col ? lc : lr, lim = col ? nc : nr, coord = col ? c : r;
+    if (pos + d >= 0 && pos + d < TW) {
+      const int ln = lane + (col ? d : d * TW);
 #pragma unroll
-  for (int f = 0; f < NW; ++f) shA[f][lane] = 0.0;
-  ensure_up(s0);  // covers the edge values of steps 0 and 1
-  if (xloader) {
-    xload(0, xv0);
-    xload(1, xv1);
-  }
-  read_up();
+      for (int f = 0; f < NW; ++f) v[f] = shA[p ^ 1][f][ln];
+    } else {
+      const int idx = col ? lr : lc;
+#pragma unroll
+      for (int f = 0; f < NW; ++f) v[f] = shX[p][col ? 0 : 1][f][idx];
+    }
+    return coord + d >= 0 && coord + d < lim;
+  };
 
-  for (int t = 0; t < nsteps; ++t) {
+  auto step = [&](int t, StepIn<TYPE>& cur, StepIn<TYPE>& mid, StepIn<TYPE>& nxt) __attribute__((always_inline)) {
+    const int p = t & 1;
     const int s = s0 + dir * t;
     const int m = m0 + dir * t;
-    const bool act = cell_ok(t);
-    // phase 1
-    if (xloader) {
-#pragma unroll
-      for (int f = 0; f < NW; ++f) shX[xrole][f][xi] = xv0[f];
+    // A
+    if (w0) {
+      double* x = &shX[p][0][0][0];
+      x[xo[0]] = cur.xd[0].x;
+      x[xo[0] + TW] = cur.xd[0].y;
+      x[xo[1]] = cur.xd[1].x;
+      x[xo[1] + TW] = cur.xd[1].y;
+      x[xo[2]] = cur.xq[0];
+      x[xo[3]] = cur.xq[1];
+      if (lane < 32) x[xo[4]] = cur.xq[2];
     }
-    if (GS) {
-#pragma unroll
-      for (int f = 0; f < NW; ++f) shB[f][lane] = nb1[f];
-    }
+    // B: at most the loads issued after this wave's stores of step t-1 stay
+    // outstanding (in-order counter), so those stores have completed
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_sched_barrier(0);
+    if (w0)
+      __builtin_amdgcn_s_waitcnt(waitcnt_vm(NLD + 5));
+    else
+      __builtin_amdgcn_s_waitcnt(waitcnt_vm(NLD));
+    __builtin_amdgcn_sched_barrier(0);
     __syncthreads();
-    // phase 2
+    // C
+    if (t > 0 && t <= nsteps && lane == 0)
+      __hip_atomic_store(&flags[tix], ep + plane_key(FWD, s - dir), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // D
     double dqn[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    if (act) {
-      double r0[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-      AxisGeo ag[3];
-      axis_geo(&own0.met[0], ag[0]);
-      axis_geo(&own0.met[3], ag[1]);
-      axis_geo(&own0.met[6], ag[2]);
-      // neighbour of real slot sl: its (D, dq) and validity.  The lane's
-      // previous cell is its own entry of shA; for the Gauss-Seidel sweeps
-      // its next cell is its own entry of shB.
-      auto nb = [&](int sl, double* v) -> bool {
-        const int ax = sl % 3, d = sl < 3 ? -1 : 1;
-        const bool before = (d == -dir);  // precedes this cell in sweep order
-        if (ax == am) {
-          const bool ok = m + d >= 0 && m + d < nm;
-          if (before) {
+    if (cell_ok(t)) {
+      const double* o = cur.own;
+      double r0[5];
 #pragma unroll
-            for (int f = 0; f < NW; ++f) v[f] = shA[f][lane];
-          } else {
+      for (int f = 0; f < 5; ++f) r0[f] = TYPE == 3 ? o[15 + f] : 0.0;
+      double gout[15];
 #pragma unroll
-            for (int f = 0; f < NW; ++f) v[f] = GS ? shB[GS ? f : 0][lane] : 0.0;
+      for (int k = 0; k < 3; ++k) {
+        AxisGeo ag;
+        axis_geo(o + 3 * k, ag);
+        double v[NW];
+        const bool ok = nbr(k, m, p, v);
+        double inv[5], tv[5];
+        slot_inv<SL0>(o + 3 * k, ag, v, v + 8, gm1, inv);
+        if (nvis == 1) {
+          const double hv = 0.5 * (v[7] * ag.g2);
+#pragma unroll
+          for (int f = 0; f < 5; ++f) tv[f] = hv * v[8 + f];
+        }
+#pragma unroll
+        for (int f = 0; f < 5; ++f) {
+          if (TYPE == 3) {  // Gauss-Seidel term of a +1 neighbour: r - (inviscid - viscous)
+            const double res = nvis == 1 ? inv[f] + tv[f] * -1.0 : inv[f];
+            const double acc = r0[f] - res;
+            r0[f] = ok ? acc : r0[f];
+          } else {  // LU terms, and Gauss-Seidel terms of -1 neighbours: r + (inviscid + viscous)
+            const double res = nvis == 1 ? inv[f] + tv[f] : inv[f];
+            const double acc = r0[f] + res;
+            r0[f] = ok ? acc : r0[f];
           }
-          return ok;
-        }
-        const bool col = (ax == ac);
-        const int pos = col ? lc : lr, lim = col ? nc : nr, coord = col ? c : r;
-        const bool ok = coord + d >= 0 && coord + d < lim;
-        if (pos + d >= 0 && pos + d < TW) {
-          const int ln = lane + (col ? d : d * TW);
-          if (before) {
-#pragma unroll
-            for (int f = 0; f < NW; ++f) v[f] = shA[f][ln];
-          } else {
-#pragma unroll
-            for (int f = 0; f < NW; ++f) v[f] = GS ? shB[GS ? f : 0][ln] : 0.0;
+          if (TYPE == 1) {  // the Gauss-Seidel forward sweep's term for the same pair
+            const double g = nvis == 1 ? inv[f] + tv[f] * -1.0 : inv[f];
+            gout[5 * k + f] = ok ? g : 0.0;
           }
-        } else {
-          const int role = (before ? 0 : 2) + (col ? 0 : 1);
-          const int idx = col ? lr : lc;
+        }
+      }
+      const long long rb = rb0 + (long long)t * qstep;
+      if (TYPE == 2) {
+        double2* pp = S.pre + rb * SW_PRE + c;
+        pp[0] = make_double2(r0[0], r0[1]);
+        pp[ncp] = make_double2(r0[2], r0[3]);
+        pp[2 * ncp] = make_double2(r0[4], 0.0);
 #pragma unroll
-          for (int f = 0; f < NW; ++f) v[f] = shX[role < NROLE ? role : 0][f][idx];
-        }
-        return ok;
-      };
-      double v[NW];
-      if (TYPE == 1) {
-        bool ok;
-        ok = nb(3, v); slot<false, 3>(&own0.met[0], ag[0], v, v + 8, ok, gm1, nvis, r0);
-        ok = nb(4, v); slot<false, 4>(&own0.met[3], ag[1], v, v + 8, ok, gm1, nvis, r0);
-        ok = nb(5, v); slot<false, 5>(&own0.met[6], ag[2], v, v + 8, ok, gm1, nvis, r0);
-      } else {
-        bool ok;
-        ok = nb(0, v); slot<GS, 0>(&own0.met[0], ag[0], v, v + 8, ok, gm1, nvis, r0);
-        ok = nb(1, v); slot<GS, 1>(&own0.met[3], ag[1], v, v + 8, ok, gm1, nvis, r0);
-        ok = nb(2, v); slot<GS, 2>(&own0.met[6], ag[2], v, v + 8, ok, gm1, nvis, r0);
-        if (GS) {
-          ok = nb(3, v); slot<true, 3>(&own0.met[0], ag[0], v, v + 8, ok, gm1, nvis, r0);
-          ok = nb(4, v); slot<true, 4>(&own0.met[3], ag[1], v, v + 8, ok, gm1, nvis, r0);
-          ok = nb(5, v); slot<true, 5>(&own0.met[6], ag[2], v, v + 8, ok, gm1, nvis, r0);
-        }
+        for (int k = 0; k < 3; ++k)
+#pragma unroll
+          for (int f = 0; f < 5; ++f) r0[f] = r0[f] - cur.ga[5 * k + f];
       }
       const double wmig = 1.0;
+      const double* r5 = o + 10;
+      const double coed = o[9];
       if (TYPE == 0) {
 #pragma unroll
-        for (int f = 0; f < 5; ++f) dqn[f] = (-own0.r5[f] + wmig * r0[f]) * own0.coed;
+        for (int f = 0; f < 5; ++f) dqn[f] = (-r5[f] + wmig * r0[f]) * coed;
       } else if (TYPE == 1) {
 #pragma unroll
-        for (int f = 0; f < 5; ++f) dqn[f] = own0.r5[f] - (wmig * r0[f]) * own0.coed;
+        for (int f = 0; f < 5; ++f) dqn[f] = r5[f] - (wmig * r0[f]) * coed;
       } else {
 #pragma unroll
-        for (int f = 0; f < 5; ++f) dqn[f] = (wmig * r0[f] + -own0.r5[f]) * own0.coed;
+        for (int f = 0; f < 5; ++f) dqn[f] = (wmig * r0[f] + -r5[f]) * coed;
       }
-      const long long q = rb0 + (long long)t * qstep + c;
       double* dqp = reinterpret_cast<double*>(S.dq);
 #pragma unroll
       for (int f = 0; f < 5; ++f)
-        __hip_atomic_store(&dqp[sweep_field(q, c, ncp, SW_DQ, f)], dqn[f], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&dqp[sweep_field(rb + c, c, ncp, SW_DQ, f)], dqn[f], __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+      if (TYPE == 1) {
+        double2* gp = S.ga + rb * SW_GA + c;
+#pragma unroll
+        for (int k = 0; k < SW_GA; ++k)
+          gp[(long long)k * ncp] = make_double2(gout[2 * k], 2 * k + 1 < 15 ? gout[2 * k + 1] : 0.0);
+      }
     }
-    const bool publish = (t % pub == pub - 1) || t == nsteps - 1;
-    if (publish) __builtin_amdgcn_s_waitcnt(0);  // this wave's agent-coherent stores have completed
-    __syncthreads();
-    // phase 3
-    if (publish && lane == 0)
-      __hip_atomic_store(&flags[tix], ep + plane_key(FWD, s), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 #pragma unroll
-    for (int f = 0; f < 8; ++f) shA[f][lane] = Dcur[f];
+    for (int f = 0; f < 8; ++f) shA[p][f][lane] = cur.D[f];
 #pragma unroll
-    for (int f = 0; f < 5; ++f) shA[8 + f][lane] = dqn[f];
+    for (int f = 0; f < 5; ++f) shA[p][8 + f][lane] = dqn[f];
+    // E: nothing below may move above the stores (the count in B relies on it)
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_sched_barrier(0);
     if (t + 2 < nsteps) ensure_up(s + dir);  // edge values of step t+2 lie on plane s(t+1)
-    double xv2[NW];
-    if (xloader) xload(t + 2, xv2);
-    Own own2;
-    double nb3[NB];
-    load_own<TYPE>(S, cell_rb(t + 2), cell_c(t + 2), ncp, own2);
-    load_nbr<GS>(S, cell_rb(t + 3), cell_c(t + 3), ncp, nb3);
+    // Waiting for a load also waits for every older one, so the loads are
+    // issued in the order they are consumed: flags (next step's check), D
+    // and ga (end of next step), own inputs and edges (step t+2).
     read_up();
+    load_mid<TYPE>(S, cell_rb(t + 1), cell_c(t + 1), ncp, mid);
+    load_own<TYPE>(S, cell_rb(t + 2), cell_c(t + 2), ncp, nxt.own);
+    load_edges(t + 2, nxt);
+  };
+
+  StepIn<TYPE> PA, PB, PC;
 #pragma unroll
-    for (int f = 0; f < 8; ++f) Dcur[f] = nb1[f];
-    own0 = own1;
-    own1 = own2;
-#pragma unroll
-    for (int f = 0; f < NB; ++f) {
-      nb1[f] = nb2[f];
-      nb2[f] = nb3[f];
-    }
-#pragma unroll
-    for (int f = 0; f < NW; ++f) {
-      xv0[f] = xv1[f];
-      xv1[f] = xloader ? xv2[f] : 0.0;
-    }
+  for (int f = 0; f < NW; ++f) {
+    shA[0][f][lane] = 0.0;
+    shA[1][f][lane] = 0.0;
   }
+  // same issue order as in the loop, so that the waits the loop needs are
+  // no stricter on entry
+  ensure_up(s0);  // covers the edge values of steps 0 and 1
+  read_up();
+  load_edges(0, PA);
+  load_own<TYPE>(S, cell_rb(0), cell_c(0), ncp, PA.own);
+  load_mid<TYPE>(S, cell_rb(0), cell_c(0), ncp, PA);
+  load_own<TYPE>(S, cell_rb(1), cell_c(1), ncp, PB.own);
+  load_edges(1, PB);
+  // whole triples of steps: the steps past the end are idle (no cell, no
+  // store, no publication) but keep the loop free of side exits, whose
+  // paths would make the compiler's load waits stricter
+  for (int t = 0; t < nsteps; t += 3) {
+    step(t, PA, PB, PC);
+    step(t + 1, PB, PC, PA);
+    step(t + 2, PC, PA, PB);
+  }
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  if (lane == 0)
+    __hip_atomic_store(&flags[tix], ep + plane_key(FWD, s0 + dir * (nsteps - 1)), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __global__ void epoch_kernel(unsigned long long* epoch) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *epoch += 1;
 }
-
-static inline unsigned grid_of(long long n, int tb) { return (unsigned)((n + tb - 1) / tb); }
 
 void launch_lu_geom(const Launch& La, long long n, const SweepArrays& S) {
   if (n <= 0) return;
@@ -658,17 +729,17 @@ void launch_update(const Launch& La, long long n, const SweepArrays& S, double g
 }
 
 void launch_lusgs_tiles(const SweepArrays& S, const BlockDev* blocks, const SweepTile* tiles, const int* order,
-                        int ntiles, unsigned long long* flags, unsigned long long* epoch, int* err, int pub, int type,
+                        int ntiles, unsigned long long* flags, unsigned long long* epoch, int* err, int type,
                         const Params& p, hipStream_t s) {
   if (ntiles <= 0) return;
   hipLaunchKernelGGL(epoch_kernel, dim3(1), dim3(64), 0, s, epoch);
   const double gm1 = p.gama - 1.0;
   const dim3 gr(ntiles), tb(TL);
   switch (type) {
-    case 0: hipLaunchKernelGGL(lusgs_tile_kernel<0>, gr, tb, 0, s, S, blocks, tiles, order, flags, epoch, err, pub, gm1, p.nvis); break;
-    case 1: hipLaunchKernelGGL(lusgs_tile_kernel<1>, gr, tb, 0, s, S, blocks, tiles, order, flags, epoch, err, pub, gm1, p.nvis); break;
-    case 2: hipLaunchKernelGGL(lusgs_tile_kernel<2>, gr, tb, 0, s, S, blocks, tiles, order, flags, epoch, err, pub, gm1, p.nvis); break;
-    default: hipLaunchKernelGGL(lusgs_tile_kernel<3>, gr, tb, 0, s, S, blocks, tiles, order, flags, epoch, err, pub, gm1, p.nvis); break;
+    case 0: hipLaunchKernelGGL(lusgs_tile_kernel<0>, gr, tb, 0, s, S, blocks, tiles, order, flags, epoch, err, gm1, p.nvis); break;
+    case 1: hipLaunchKernelGGL(lusgs_tile_kernel<1>, gr, tb, 0, s, S, blocks, tiles, order, flags, epoch, err, gm1, p.nvis); break;
+    case 2: hipLaunchKernelGGL(lusgs_tile_kernel<2>, gr, tb, 0, s, S, blocks, tiles, order, flags, epoch, err, gm1, p.nvis); break;
+    default: hipLaunchKernelGGL(lusgs_tile_kernel<3>, gr, tb, 0, s, S, blocks, tiles, order, flags, epoch, err, gm1, p.nvis); break;
   }
 }
 

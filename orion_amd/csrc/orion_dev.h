@@ -58,10 +58,14 @@ struct BlockDev {
 constexpr int SW_OWN = 8;  // met0..met8, coed, rhs0..rhs4 (+1 unused)
 constexpr int SW_D = 4;    // u, v, w, c2, H, af, c, 2mu/(|Re| rho vol + 1e-30)
 constexpr int SW_DQ = 3;   // dq0..dq4 (+1 unused)
+constexpr int SW_GA = 8;   // Gauss-Seidel forward: terms of the +1 neighbours along i, j, k (3 x 5, +1 unused)
+constexpr int SW_PRE = 3;  // Gauss-Seidel backward: sum of the -1 neighbours' terms (5, +1 unused)
 struct SweepArrays {
   double2* own;
   double2* D;
   double2* dq;
+  double2* ga;
+  double2* pre;
   long long n;
 };
 

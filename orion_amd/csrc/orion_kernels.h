@@ -65,7 +65,7 @@ void launch_avg_apply(const BlockDev* blocks, const AvgDst* dst, long long n, co
 void launch_lu_geom(const Launch& La, long long n, const SweepArrays& S);
 void launch_lu_pack(const Launch& La, long long n, const SweepArrays& S, const Params& p);
 void launch_lusgs_tiles(const SweepArrays& S, const BlockDev* blocks, const SweepTile* tiles, const int* order,
-                        int ntiles, unsigned long long* flags, unsigned long long* epoch, int* err, int pub, int type,
+                        int ntiles, unsigned long long* flags, unsigned long long* epoch, int* err, int type,
                         const Params& p, hipStream_t s);
 void launch_update(const Launch& La, long long n, const SweepArrays& S, double gamma);
 
