@@ -519,7 +519,7 @@ static int build_sweep(orion_hip_ctx* c) {
       dalloc(c, (void**)&c->sw.dq, sizeof(double2) * SW_DQ * n) ||
       dalloc(c, (void**)&c->sw.ga, sizeof(double2) * SW_GA * n) ||
       dalloc(c, (void**)&c->sw.pre, sizeof(double2) * SW_PRE * n) ||
-      dalloc(c, (void**)&c->d_flags, sizeof(unsigned long long) * std::max(nt, 1)) ||
+      dalloc(c, (void**)&c->d_flags, sizeof(unsigned long long) * FLAG_STRIDE * std::max(nt, 1)) ||
       dalloc(c, (void**)&c->d_epoch, sizeof(unsigned long long)) || dalloc(c, (void**)&c->d_err, sizeof(int)))
     return -1;
   HIPCHK(hipMemset(c->sw.dq, 0, sizeof(double2) * SW_DQ * n));
@@ -527,7 +527,7 @@ static int build_sweep(orion_hip_ctx* c) {
   HIPCHK(hipMemset(c->sw.D, 0, sizeof(double2) * SW_D * n));
   HIPCHK(hipMemset(c->sw.ga, 0, sizeof(double2) * SW_GA * n));
   HIPCHK(hipMemset(c->sw.pre, 0, sizeof(double2) * SW_PRE * n));
-  HIPCHK(hipMemset(c->d_flags, 0, sizeof(unsigned long long) * std::max(nt, 1)));
+  HIPCHK(hipMemset(c->d_flags, 0, sizeof(unsigned long long) * FLAG_STRIDE * std::max(nt, 1)));
   HIPCHK(hipMemset(c->d_epoch, 0, sizeof(unsigned long long)));
   HIPCHK(hipMemset(c->d_err, 0, sizeof(int)));
   return 0;

@@ -295,6 +295,7 @@ struct StepIn {
   double D[8];
   double2 xd[2];
   double xq[3];
+  unsigned long long pf[2];  // upstream flags, read a step before they are checked (thread 0)
 };
 
 // Sweep layout pair loads of the own inputs: 16-byte loads, unconditional
@@ -361,11 +362,11 @@ __device__ __forceinline__ void load_mid(const SweepArrays& S, long long rb, int
   }
 }
 
-// Vector memory loads every wave issues per step (own inputs, D, ga); the
-// first wave issues 5 more (edge values) after them.
+// Vector memory loads every wave issues per step: own inputs, D, ga and
+// the 5 edge loads.
 template <int TYPE>
 constexpr unsigned step_loads() {
-  return (TYPE == 1 ? 5 + SW_DQ : SW_OWN) + (TYPE == 3 ? SW_PRE : 0) + SW_D + (TYPE == 2 ? SW_GA : 0);
+  return (TYPE == 1 ? 5 + SW_DQ : SW_OWN) + (TYPE == 3 ? SW_PRE : 0) + SW_D + (TYPE == 2 ? SW_GA : 0) + 5;
 }
 
 // Sweep types: 0 LU forward, 1 LU backward, 2 Gauss-Seidel forward,
@@ -401,16 +402,17 @@ constexpr unsigned step_loads() {
 // LDS buffers alternate by step parity, so one barrier per step suffices.
 // The three StepIn buffers rotate by unrolling (no register copies, which
 // would wait for the loads still in flight).
-template <int TYPE>
+template <int TYPE, bool VIS>
 __global__ void __launch_bounds__(TL, 1) lusgs_tile_kernel(SweepArrays S, const BlockDev* __restrict__ blocks,
                                                           const SweepTile* __restrict__ tiles,
                                                           const int* __restrict__ order, unsigned long long* flags,
                                                           const unsigned long long* __restrict__ epoch, int* err,
-                                                          double gm1, int nvis) {
+                                                          double gm1) {
   constexpr bool FWD = (TYPE == 0 || TYPE == 2);
   constexpr int dir = FWD ? 1 : -1;
   constexpr int SL0 = FWD ? 0 : 3;  // slots of the neighbours before a cell in sweep order
   constexpr unsigned NLD = step_loads<TYPE>();
+  constexpr unsigned NEDGE = 5;  // edge loads, the last of a step's loads
   __shared__ double shA[2][NW][TL];     // (D, new dq) of each lane's cell, by step parity
   __shared__ double shX[2][2][NW][TW];  // upstream edge values, by step parity
 
@@ -443,7 +445,7 @@ __global__ void __launch_bounds__(TL, 1) lusgs_tile_kernel(SweepArrays S, const 
   auto cell_rb = [&](int t) { return cell_ok(t) ? rb0 + (long long)t * qstep : b.sbase; };
   auto cell_c = [&](int t) { return cell_ok(t) ? c : 0; };
 
-  // Upstream edge values, loaded by the first wave (whose lane 0 checks the
+  // Upstream edge values, taken by the first wave (whose lane 0 checks the
   // upstream progress): per step 2 x 16 cells of 4 D pairs and 5 dq words.
   // Lane l loads D items l and l + 64 and dq items l, l + 64, l + 128; item
   // e is (role, index, part), role 0 the cells across the upstream col
@@ -483,8 +485,10 @@ __global__ void __launch_bounds__(TL, 1) lusgs_tile_kernel(SweepArrays S, const 
     *cc = ok ? ec[j] : 0;
     return ok ? b.sbase + ((long long)sp * nr + er[j]) * ncp : b.sbase;
   };
+  // Every wave issues these loads (the other waves read the block's first
+  // cell and never use the values): with the same loads on every path the
+  // compiler's waits for them stay as loose as the pipeline allows.
   auto load_edges = [&](int t, StepIn<TYPE>& P) {
-    if (!w0) return;
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
       int cc;
@@ -501,10 +505,9 @@ __global__ void __launch_bounds__(TL, 1) lusgs_tile_kernel(SweepArrays S, const 
     }
   };
 
-  // thread 0: the upstream tiles must have finished plane s; pre[] holds the
-  // flag values read earlier, re-read (bounded spin) only when short
-  unsigned long long pre[2] = {0, 0};
-  auto ensure_up = [&](int s) {
+  // thread 0: the upstream tiles must have finished plane s; pf holds the
+  // flag values read a step earlier, re-read (bounded spin) only when short
+  auto ensure_up = [&](int s, const unsigned long long* pf) {
     if (lane != 0) return;
     for (int k = 0; k < 2; ++k) {
       const int u = k ? up_r : up_c;
@@ -512,7 +515,7 @@ __global__ void __launch_bounds__(TL, 1) lusgs_tile_kernel(SweepArrays S, const 
       const unsigned long long key = need_key(FWD, ut[k], s);
       if (key == 0) continue;
       const unsigned long long want = ep + key;
-      unsigned long long v = pre[k];
+      unsigned long long v = pf[k];
       long long spins = 0;
       while (v < want) {
         // far beyond any legal wait (a whole sweep takes milliseconds), or
@@ -520,49 +523,42 @@ __global__ void __launch_bounds__(TL, 1) lusgs_tile_kernel(SweepArrays S, const 
         if (++spins > (1LL << 20) ||
             ((spins & 255) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)) {
           atomicOr(err, 1);
-          v = want;
           break;
         }
         if (spins > 1) __builtin_amdgcn_s_sleep(2);
-        v = __hip_atomic_load(&flags[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        v = __hip_atomic_load(&flags[(long long)u * FLAG_STRIDE], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
-      pre[k] = v;
     }
   };
-  auto read_up = [&]() {  // thread 0: early flag reads
+  auto read_up = [&](unsigned long long* pf) {  // thread 0: flag reads for the next check
     if (lane != 0) return;
-    if (up_c >= 0) pre[0] = __hip_atomic_load(&flags[up_c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (up_r >= 0) pre[1] = __hip_atomic_load(&flags[up_r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (up_c >= 0) pf[0] = __hip_atomic_load(&flags[(long long)up_c * FLAG_STRIDE], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (up_r >= 0) pf[1] = __hip_atomic_load(&flags[(long long)up_r * FLAG_STRIDE], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   };
 
   // before-neighbour of this lane's cell along real axis ax at step parity
   // p: its (D, dq) and whether it exists.  Along the march axis it is the
-  // lane's own cell of the previous step.
+  // lane's own cell of the previous step.  Branch free: the source (a lane's
+  // entry of shA, or the edge values in shX) is selected by address.
   auto nbr = [&](int ax, int m, int p, double* v) -> bool {
     constexpr int d = -dir;
-    if (ax == am) {
-#pragma unroll
-      for (int f = 0; f < NW; ++f) v[f] = shA[p ^ 1][f][lane];
-      return m + d >= 0 && m + d < nm;
-    }
+    const bool march = (ax == am);
     const bool col = (ax == ac);
     const int pos = col ? lc : lr, lim = col ? nc : nr, coord = col ? c : r;
-    if (pos + d >= 0 && pos + d < TW) {
-      const int ln = lane + (col ? d : d * TW);
+    const bool inside = march || (pos + d >= 0 && pos + d < TW);
+    const int ln = march ? lane : lane + (col ? d : d * TW);
+    const double* base = inside ? &shA[p ^ 1][0][ln] : &shX[p][col ? 0 : 1][0][col ? lr : lc];
+    const int stride = inside ? TL : TW;
 #pragma unroll
-      for (int f = 0; f < NW; ++f) v[f] = shA[p ^ 1][f][ln];
-    } else {
-      const int idx = col ? lr : lc;
-#pragma unroll
-      for (int f = 0; f < NW; ++f) v[f] = shX[p][col ? 0 : 1][f][idx];
-    }
-    return coord + d >= 0 && coord + d < lim;
+    for (int f = 0; f < NW; ++f) v[f] = base[f * stride];
+    return march ? (m + d >= 0 && m + d < nm) : (coord + d >= 0 && coord + d < lim);
   };
 
   auto step = [&](int t, StepIn<TYPE>& cur, StepIn<TYPE>& mid, StepIn<TYPE>& nxt) __attribute__((always_inline)) {
     const int p = t & 1;
     const int s = s0 + dir * t;
     const int m = m0 + dir * t;
+    const bool act = cell_ok(t);
     // A
     if (w0) {
       double* x = &shX[p][0][0][0];
@@ -574,81 +570,90 @@ __global__ void __launch_bounds__(TL, 1) lusgs_tile_kernel(SweepArrays S, const 
       x[xo[3]] = cur.xq[1];
       if (lane < 32) x[xo[4]] = cur.xq[2];
     }
-    // B: at most the loads issued after this wave's stores of step t-1 stay
-    // outstanding (in-order counter), so those stores have completed
+    // B: only the edge loads, issued after this wave's stores of step t-1,
+    // may stay outstanding (in-order counter), so those stores have completed
     asm volatile("" ::: "memory");
     __builtin_amdgcn_sched_barrier(0);
-    if (w0)
-      __builtin_amdgcn_s_waitcnt(waitcnt_vm(NLD + 5));
-    else
-      __builtin_amdgcn_s_waitcnt(waitcnt_vm(NLD));
+    __builtin_amdgcn_s_waitcnt(waitcnt_vm(NEDGE));
     __builtin_amdgcn_sched_barrier(0);
     __syncthreads();
     // C
     if (t > 0 && t <= nsteps && lane == 0)
-      __hip_atomic_store(&flags[tix], ep + plane_key(FWD, s - dir), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    // D
-    double dqn[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    if (cell_ok(t)) {
-      const double* o = cur.own;
-      double r0[5];
+      __hip_atomic_store(&flags[(long long)tix * FLAG_STRIDE], ep + plane_key(FWD, s - dir), __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+    // the upstream flags for the check of the next step, read here so that
+    // they return long before any wait of this wave needs them
+    read_up(mid.pf);
+    // D: the update of every lane's cell, branch free (a lane without a cell
+    // computes on the block's first cell and stores nothing), with the loads
+    // of later steps interleaved: D and ga of step t+1, own inputs of t+2
+    load_mid<TYPE>(S, cell_rb(t + 1), cell_c(t + 1), ncp, mid);
+    load_own<TYPE>(S, cell_rb(t + 2), cell_c(t + 2), ncp, nxt.own);
+    const double* o = cur.own;
+    double r0[5];
 #pragma unroll
-      for (int f = 0; f < 5; ++f) r0[f] = TYPE == 3 ? o[15 + f] : 0.0;
-      double gout[15];
+    for (int f = 0; f < 5; ++f) r0[f] = TYPE == 3 ? o[15 + f] : 0.0;
+    double gout[15];
 #pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        AxisGeo ag;
-        axis_geo(o + 3 * k, ag);
-        double v[NW];
-        const bool ok = nbr(k, m, p, v);
-        double inv[5], tv[5];
-        slot_inv<SL0>(o + 3 * k, ag, v, v + 8, gm1, inv);
-        if (nvis == 1) {
-          const double hv = 0.5 * (v[7] * ag.g2);
+    for (int k = 0; k < 3; ++k) {
+      AxisGeo ag;
+      axis_geo(o + 3 * k, ag);
+      double v[NW];
+      const bool ok = nbr(k, m, p, v);
+      double inv[5], tv[5];
+      slot_inv<SL0>(o + 3 * k, ag, v, v + 8, gm1, inv);
+      if (VIS) {
+        const double hv = 0.5 * (v[7] * ag.g2);
 #pragma unroll
-          for (int f = 0; f < 5; ++f) tv[f] = hv * v[8 + f];
+        for (int f = 0; f < 5; ++f) tv[f] = hv * v[8 + f];
+      }
+#pragma unroll
+      for (int f = 0; f < 5; ++f) {
+        if (TYPE == 3) {  // Gauss-Seidel term of a +1 neighbour: r - (inviscid - viscous)
+          const double res = VIS ? inv[f] + tv[f] * -1.0 : inv[f];
+          const double acc = r0[f] - res;
+          r0[f] = ok ? acc : r0[f];
+        } else {  // LU terms, and Gauss-Seidel terms of -1 neighbours: r + (inviscid + viscous)
+          const double res = VIS ? inv[f] + tv[f] : inv[f];
+          const double acc = r0[f] + res;
+          r0[f] = ok ? acc : r0[f];
         }
-#pragma unroll
-        for (int f = 0; f < 5; ++f) {
-          if (TYPE == 3) {  // Gauss-Seidel term of a +1 neighbour: r - (inviscid - viscous)
-            const double res = nvis == 1 ? inv[f] + tv[f] * -1.0 : inv[f];
-            const double acc = r0[f] - res;
-            r0[f] = ok ? acc : r0[f];
-          } else {  // LU terms, and Gauss-Seidel terms of -1 neighbours: r + (inviscid + viscous)
-            const double res = nvis == 1 ? inv[f] + tv[f] : inv[f];
-            const double acc = r0[f] + res;
-            r0[f] = ok ? acc : r0[f];
-          }
-          if (TYPE == 1) {  // the Gauss-Seidel forward sweep's term for the same pair
-            const double g = nvis == 1 ? inv[f] + tv[f] * -1.0 : inv[f];
-            gout[5 * k + f] = ok ? g : 0.0;
-          }
+        if (TYPE == 1) {  // the Gauss-Seidel forward sweep's term for the same pair
+          const double g = VIS ? inv[f] + tv[f] * -1.0 : inv[f];
+          gout[5 * k + f] = ok ? g : 0.0;
         }
       }
+    }
+    double pre5[5];
+    if (TYPE == 2) {
+#pragma unroll
+      for (int f = 0; f < 5; ++f) pre5[f] = r0[f];
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int f = 0; f < 5; ++f) r0[f] = r0[f] - cur.ga[5 * k + f];
+    }
+    const double wmig = 1.0;
+    const double* r5 = o + 10;
+    const double coed = o[9];
+    double dqn[5];
+#pragma unroll
+    for (int f = 0; f < 5; ++f) {
+      double x;
+      if (TYPE == 0) x = (-r5[f] + wmig * r0[f]) * coed;
+      else if (TYPE == 1) x = r5[f] - (wmig * r0[f]) * coed;
+      else x = (wmig * r0[f] + -r5[f]) * coed;
+      dqn[f] = act ? x : 0.0;
+    }
+    // one prefetch load between runs of arithmetic
+#pragma unroll
+    for (int i = 0; i < (int)(NLD - NEDGE); ++i) {
+      __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
+      __builtin_amdgcn_sched_group_barrier(0x002, 24, 0);
+    }
+    // F
+    if (act) {
       const long long rb = rb0 + (long long)t * qstep;
-      if (TYPE == 2) {
-        double2* pp = S.pre + rb * SW_PRE + c;
-        pp[0] = make_double2(r0[0], r0[1]);
-        pp[ncp] = make_double2(r0[2], r0[3]);
-        pp[2 * ncp] = make_double2(r0[4], 0.0);
-#pragma unroll
-        for (int k = 0; k < 3; ++k)
-#pragma unroll
-          for (int f = 0; f < 5; ++f) r0[f] = r0[f] - cur.ga[5 * k + f];
-      }
-      const double wmig = 1.0;
-      const double* r5 = o + 10;
-      const double coed = o[9];
-      if (TYPE == 0) {
-#pragma unroll
-        for (int f = 0; f < 5; ++f) dqn[f] = (-r5[f] + wmig * r0[f]) * coed;
-      } else if (TYPE == 1) {
-#pragma unroll
-        for (int f = 0; f < 5; ++f) dqn[f] = r5[f] - (wmig * r0[f]) * coed;
-      } else {
-#pragma unroll
-        for (int f = 0; f < 5; ++f) dqn[f] = (wmig * r0[f] + -r5[f]) * coed;
-      }
       double* dqp = reinterpret_cast<double*>(S.dq);
 #pragma unroll
       for (int f = 0; f < 5; ++f)
@@ -660,7 +665,14 @@ __global__ void __launch_bounds__(TL, 1) lusgs_tile_kernel(SweepArrays S, const 
         for (int k = 0; k < SW_GA; ++k)
           gp[(long long)k * ncp] = make_double2(gout[2 * k], 2 * k + 1 < 15 ? gout[2 * k + 1] : 0.0);
       }
+      if (TYPE == 2) {
+        double2* pp = S.pre + rb * SW_PRE + c;
+        pp[0] = make_double2(pre5[0], pre5[1]);
+        pp[ncp] = make_double2(pre5[2], pre5[3]);
+        pp[2 * ncp] = make_double2(pre5[4], 0.0);
+      }
     }
+    // G
 #pragma unroll
     for (int f = 0; f < 8; ++f) shA[p][f][lane] = cur.D[f];
 #pragma unroll
@@ -668,13 +680,7 @@ __global__ void __launch_bounds__(TL, 1) lusgs_tile_kernel(SweepArrays S, const 
     // E: nothing below may move above the stores (the count in B relies on it)
     asm volatile("" ::: "memory");
     __builtin_amdgcn_sched_barrier(0);
-    if (t + 2 < nsteps) ensure_up(s + dir);  // edge values of step t+2 lie on plane s(t+1)
-    // Waiting for a load also waits for every older one, so the loads are
-    // issued in the order they are consumed: flags (next step's check), D
-    // and ga (end of next step), own inputs and edges (step t+2).
-    read_up();
-    load_mid<TYPE>(S, cell_rb(t + 1), cell_c(t + 1), ncp, mid);
-    load_own<TYPE>(S, cell_rb(t + 2), cell_c(t + 2), ncp, nxt.own);
+    if (t + 2 < nsteps) ensure_up(s + dir, cur.pf);  // edge values of step t+2 lie on plane s(t+1)
     load_edges(t + 2, nxt);
   };
 
@@ -686,8 +692,11 @@ __global__ void __launch_bounds__(TL, 1) lusgs_tile_kernel(SweepArrays S, const 
   }
   // same issue order as in the loop, so that the waits the loop needs are
   // no stricter on entry
-  ensure_up(s0);  // covers the edge values of steps 0 and 1
-  read_up();
+  {
+    const unsigned long long none[2] = {0, 0};
+    ensure_up(s0, none);  // covers the edge values of steps 0 and 1
+  }
+  read_up(PA.pf);
   load_edges(0, PA);
   load_own<TYPE>(S, cell_rb(0), cell_c(0), ncp, PA.own);
   load_mid<TYPE>(S, cell_rb(0), cell_c(0), ncp, PA);
@@ -704,7 +713,7 @@ __global__ void __launch_bounds__(TL, 1) lusgs_tile_kernel(SweepArrays S, const 
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   if (lane == 0)
-    __hip_atomic_store(&flags[tix], ep + plane_key(FWD, s0 + dir * (nsteps - 1)), __ATOMIC_RELAXED,
+    __hip_atomic_store(&flags[(long long)tix * FLAG_STRIDE], ep + plane_key(FWD, s0 + dir * (nsteps - 1)), __ATOMIC_RELAXED,
                        __HIP_MEMORY_SCOPE_AGENT);
 }
 
@@ -735,11 +744,16 @@ void launch_lusgs_tiles(const SweepArrays& S, const BlockDev* blocks, const Swee
   hipLaunchKernelGGL(epoch_kernel, dim3(1), dim3(64), 0, s, epoch);
   const double gm1 = p.gama - 1.0;
   const dim3 gr(ntiles), tb(TL);
-  switch (type) {
-    case 0: hipLaunchKernelGGL(lusgs_tile_kernel<0>, gr, tb, 0, s, S, blocks, tiles, order, flags, epoch, err, gm1, p.nvis); break;
-    case 1: hipLaunchKernelGGL(lusgs_tile_kernel<1>, gr, tb, 0, s, S, blocks, tiles, order, flags, epoch, err, gm1, p.nvis); break;
-    case 2: hipLaunchKernelGGL(lusgs_tile_kernel<2>, gr, tb, 0, s, S, blocks, tiles, order, flags, epoch, err, gm1, p.nvis); break;
-    default: hipLaunchKernelGGL(lusgs_tile_kernel<3>, gr, tb, 0, s, S, blocks, tiles, order, flags, epoch, err, gm1, p.nvis); break;
+  const bool v = p.nvis == 1;
+  switch (type * 2 + (v ? 1 : 0)) {
+    case 0: hipLaunchKernelGGL((lusgs_tile_kernel<0, false>), gr, tb, 0, s, S, blocks, tiles, order, flags, epoch, err, gm1); break;
+    case 1: hipLaunchKernelGGL((lusgs_tile_kernel<0, true>), gr, tb, 0, s, S, blocks, tiles, order, flags, epoch, err, gm1); break;
+    case 2: hipLaunchKernelGGL((lusgs_tile_kernel<1, false>), gr, tb, 0, s, S, blocks, tiles, order, flags, epoch, err, gm1); break;
+    case 3: hipLaunchKernelGGL((lusgs_tile_kernel<1, true>), gr, tb, 0, s, S, blocks, tiles, order, flags, epoch, err, gm1); break;
+    case 4: hipLaunchKernelGGL((lusgs_tile_kernel<2, false>), gr, tb, 0, s, S, blocks, tiles, order, flags, epoch, err, gm1); break;
+    case 5: hipLaunchKernelGGL((lusgs_tile_kernel<2, true>), gr, tb, 0, s, S, blocks, tiles, order, flags, epoch, err, gm1); break;
+    case 6: hipLaunchKernelGGL((lusgs_tile_kernel<3, false>), gr, tb, 0, s, S, blocks, tiles, order, flags, epoch, err, gm1); break;
+    default: hipLaunchKernelGGL((lusgs_tile_kernel<3, true>), gr, tb, 0, s, S, blocks, tiles, order, flags, epoch, err, gm1); break;
   }
 }
 

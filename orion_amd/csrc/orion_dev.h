@@ -74,6 +74,11 @@ __host__ __device__ inline long long sweep_field(long long q, int c, int ncp, in
   return 2 * ((q - c) * K + (long long)(f >> 1) * ncp + c) + (f & 1);
 }
 
+// Progress flags of the LU-SGS tiles, one per 256 bytes: a flag is polled by
+// the tile's downstream neighbours while the tile stores it, and flags of
+// unrelated tiles in one line would make those accesses contend.
+constexpr int FLAG_STRIDE = 32;
+
 // One tile of the LU-SGS pipeline: 16 x 16 (row, col) pencils of a block.
 struct SweepTile {
   int block;
