@@ -63,6 +63,7 @@ struct orion_hip_ctx {
   size_t arena_bytes = 0;
   size_t map_bytes = 0;
   Space pad, intr, face[3], line[3];
+  Space ptile;                         // LU-SGS pack tiles: 16 x 16 (march, col) cells of one row
   SweepArrays sw{};                    // LU-SGS working set in sweep layout
   std::vector<SweepTile> tiles;
   SweepTile* d_tiles = nullptr;
@@ -699,6 +700,14 @@ int orion_hip_ctx_create(int nblocks, const int* dims, const double* xyz, int nr
   for (int d = 0; d < 3; ++d)
     if (make_space(c, c->face[d], cf[d]) || make_space(c, c->line[d], cl[d])) return bail(-1);
   if (build_sweep(c)) return bail(-1);
+  {
+    std::vector<long long> ct(nblocks);
+    for (int b = 0; b < nblocks; ++b) {
+      const BlockDev& B = c->hb[b];
+      ct[b] = (long long)((B.sn[0] + 15) / 16) * ((B.sn[1] + 15) / 16) * B.sn[2] * 256;
+    }
+    if (make_space(c, c->ptile, ct)) return bail(-1);
+  }
   if (upload_vec(c, &c->db, c->hb)) return bail(-1);
   if (build_topology(c, R)) return bail(-1);
   c->nparts = residual_parts(c->intr.total());
@@ -743,7 +752,6 @@ int orion_hip_ctx_create(int nblocks, const int* dims, const double* xyz, int nr
       return bail(fail("metric check failed: %s", hipGetErrorString(hipGetLastError())));
     if (bad) return bail(fail("ERROR(Jacobi<0): %llu points with volume below 1e-15 (MetricComputer.cpp:488-491)", bad));
   }
-  launch_lu_geom(LA(c, c->intr), c->intr.total(), c->sw);
   if (p.nvis != 0) launch_temperature(LA(c, c->pad), c->pad.total(), p.gama, p.moo);
   if (hipStreamSynchronize(c->stream) != hipSuccess) return bail(fail("setup failed: %s", hipGetErrorString(hipGetLastError())));
   *out = c;
@@ -841,9 +849,9 @@ static void op_sweep(orion_hip_ctx* c, int type) {
 }
 
 static void op_update(orion_hip_ctx* c) {
-  launch_lu_pack(LA(c, c->intr), c->intr.total(), c->sw, c->p);
+  launch_lu_pack(LA(c, c->ptile), c->ptile.total(), c->sw, c->p);
   for (int t = 0; t < 4; ++t) op_sweep(c, t);
-  launch_update(LA(c, c->intr), c->intr.total(), c->sw, c->p.gama);
+  launch_update(LA(c, c->ptile), c->ptile.total(), c->sw, c->p.gama);
 }
 
 // a sweep whose wait for an upstream tile ran out has left wrong values
@@ -1052,9 +1060,9 @@ int orion_hip_profile_run(orion_hip_ctx* c, int nsteps, double* phase_ms, long l
     mark(3);
     launch_inv_diff(LA(c, c->intr), c->intr.total()); mark(1);
     op_avg(c); mark(2);
-    launch_lu_pack(LA(c, c->intr), c->intr.total(), c->sw, c->p); mark(1);
+    launch_lu_pack(LA(c, c->ptile), c->ptile.total(), c->sw, c->p); mark(1);
     for (int t = 0; t < 4; ++t) { op_sweep(c, t); mark(2); }
-    launch_update(LA(c, c->intr), c->intr.total(), c->sw, c->p.gama); mark(1);
+    launch_update(LA(c, c->ptile), c->ptile.total(), c->sw, c->p.gama); mark(1);
     launch_residual(LA(c, c->intr), c->intr.total(), 5 * c->intr.total(), c->d_part, c->d_pidx, c->nparts,
                     c->d_res + 4 * s, nullptr);
     mark(2);

@@ -72,84 +72,131 @@ __device__ inline void natural_to_sweep(const BlockDev& b, int i, int j, int k, 
 }
 
 // ---------------------------------------------------------------------------
-// packing (one thread per interior point, natural order)
+// packing and unpacking between the block arrays and the sweep layout
+//
+// A hyperplane row of the sweep layout holds cells of consecutive columns
+// and decreasing march positions, so neither natural-order nor sweep-order
+// threads reach both sides contiguously.  Each workgroup takes a tile of
+// 16 x 16 (march, col) cells of one row and stages it through LDS: the
+// block arrays are accessed with lane = (march, col) (contiguous along the
+// col axis whenever col = i), the sweep arrays with the tile's cells in
+// sweep order (contiguous runs of up to 16 columns per hyperplane row).
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) lu_geom_kernel(const BlockDev* __restrict__ blocks,
-                                                      Span SP, SweepArrays S) {
-  int bi;
-  long long gi;
-  if (!span_item(SP, &bi, &gi)) return;
-  const BlockDev& b = blocks[bi];
-  int i, j, k, m, r, c;
-  decode_interior(b, gi, &i, &j, &k);
-  natural_to_sweep(b, i, j, k, &m, &r, &c);
-  const long long q = sweep_index(b, m, r, c);
-  const long long id = pidx(b, i, j, k);
-  double* own = reinterpret_cast<double*>(S.own);
-#pragma unroll
-  for (int f = 0; f < 9; ++f) own[sweep_field(q, c, b.sncp, SW_OWN, f)] = b.met[f * b.P + id];
+constexpr int PT = 16;
+
+// the tile of launch item `item` (256 items per tile)
+__device__ inline void pack_tile(const BlockDev& b, long long item, int* m0, int* c0, int* r) {
+  const int tc = (b.sn[1] + PT - 1) / PT, tm = (b.sn[0] + PT - 1) / PT;
+  long long t = item / (PT * PT);
+  *c0 = (int)(t % tc) * PT;
+  t /= tc;
+  *m0 = (int)(t % tm) * PT;
+  *r = (int)(t / tm);
 }
 
+// tile position (mi * PT + ci) of the tile's u-th cell in sweep order:
+// by diagonal mi + ci (the hyperplane), then by column
+__device__ inline int sweep_slot(int u) {
+  int sd = 0, cnt = 0;
+  for (;;) {
+    const int len = sd < PT ? sd + 1 : 2 * PT - 1 - sd;
+    if (u < cnt + len) break;
+    cnt += len;
+    ++sd;
+  }
+  const int ci = (sd < PT ? 0 : sd - PT + 1) + (u - cnt);
+  return (sd - ci) * PT + ci;
+}
+
+// padded block index of sweep cell (m, r, c)
+__device__ inline long long natural_id(const BlockDev& b, int m, int r, int c) {
+  return pidx(b, NG, NG, NG) + (long long)m * dstride(b, b.sax[0]) + (long long)c * dstride(b, b.sax[1]) +
+         (long long)r * dstride(b, b.sax[2]);
+}
+
+constexpr int NPK = 23;  // packed words per cell: met 9, coed, rhs 5, D 8
+
+// Per update: each cell's own inputs (metrics, the diagonal coed of
+// get_diag_coeff, the rhs) and, for its neighbours, u, v, w, c^2, H, a_f,
+// c and the viscous coefficient.
 __global__ void __launch_bounds__(256) lu_pack_kernel(const BlockDev* __restrict__ blocks,
                                                       Span SP, SweepArrays S,
                                                       Params p) {
+  __shared__ double sh[NPK][PT * PT];
   int bi;
   long long gi;
-  if (!span_item(SP, &bi, &gi)) return;
+  if (!span_item(SP, &bi, &gi)) return;  // whole tiles: every lane of a workgroup has an item
   const BlockDev& b = blocks[bi];
-  int i, j, k, m, r, c;
-  decode_interior(b, gi, &i, &j, &k);
-  natural_to_sweep(b, i, j, k, &m, &r, &c);
-  const long long q = sweep_index(b, m, r, c);
-  const int ncp = b.sncp;
-  double* own = reinterpret_cast<double*>(S.own);
-  double* Dp = reinterpret_cast<double*>(S.D);
-  const long long P = b.P;
-  const long long id = pidx(b, i, j, k);
-  const double gamma = p.gama, re = p.reynolds;
-  // rhs copy with the physical-BC windows zeroed (StateUpdater.cpp:557-570)
-  const bool fz = b.frozen[id] != 0;
+  int m0, c0, r;
+  pack_tile(b, gi, &m0, &c0, &r);
+  const int lane = threadIdx.x;
+  {
+    const int m = m0 + lane / PT, c = c0 + lane % PT;
+    if (m < b.sn[0] && c < b.sn[1]) {
+      const long long P = b.P;
+      const long long id = natural_id(b, m, r, c);
+      const double gamma = p.gama, re = p.reynolds;
+      // rhs copy with the physical-BC windows zeroed (StateUpdater.cpp:557-570)
+      const bool fz = b.frozen[id] != 0;
 #pragma unroll
-  for (int f = 0; f < 5; ++f) own[sweep_field(q, c, ncp, SW_OWN, 10 + f)] = fz ? 0.0 : b.dq[f * P + id];
-  double pr[5];
+      for (int f = 0; f < 5; ++f) sh[10 + f][lane] = fz ? 0.0 : b.dq[f * P + id];
+      double pr[5];
 #pragma unroll
-  for (int f = 0; f < 5; ++f) pr[f] = b.prim[f * P + id];
-  double met[9];
+      for (int f = 0; f < 5; ++f) pr[f] = b.prim[f * P + id];
+      double met[9];
 #pragma unroll
-  for (int f = 0; f < 9; ++f) met[f] = b.met[f * P + id];
-  const double mu = b.mu[id];
-  double vol = b.vol[id];
-  if (vol < 1e-30) vol = 1e-30;
-  {  // get_diag_coeff
-    const double ra = spec_rad(pr, gamma, met[0], met[1], met[2]);
-    const double rb = spec_rad(pr, gamma, met[3], met[4], met[5]);
-    const double rc = spec_rad(pr, gamma, met[6], met[7], met[8]);
-    double rva = 0, rvb = 0, rvc = 0;
-    if (p.nvis == 1) {
-      rva = visc_rad(pr[0], mu, vol, re, met[0], met[1], met[2]);
-      rvb = visc_rad(pr[0], mu, vol, re, met[3], met[4], met[5]);
-      rvc = visc_rad(pr[0], mu, vol, re, met[6], met[7], met[8]);
+      for (int f = 0; f < 9; ++f) {
+        met[f] = b.met[f * P + id];
+        sh[f][lane] = met[f];
+      }
+      const double mu = b.mu[id];
+      double vol = b.vol[id];
+      if (vol < 1e-30) vol = 1e-30;
+      {  // get_diag_coeff
+        const double ra = spec_rad(pr, gamma, met[0], met[1], met[2]);
+        const double rb = spec_rad(pr, gamma, met[3], met[4], met[5]);
+        const double rc = spec_rad(pr, gamma, met[6], met[7], met[8]);
+        double rva = 0, rvb = 0, rvc = 0;
+        if (p.nvis == 1) {
+          rva = visc_rad(pr[0], mu, vol, re, met[0], met[1], met[2]);
+          rvb = visc_rad(pr[0], mu, vol, re, met[3], met[4], met[5]);
+          rvc = visc_rad(pr[0], mu, vol, re, met[6], met[7], met[8]);
+        }
+        const double rs = ra + rb + rc + rva + rvb + rvc;
+        const double term = (1.0 / b.dt[id]) + 1.0 * 1.0 * rs;
+        sh[9][lane] = 1.0 / term;
+      }
+      // this cell seen as a neighbour: compute_flux_splitting's per-cell
+      // terms, the sound speed of calc_spectral_radius and the viscous
+      // coefficient of calc_visc_spectral_radius (same clamps)
+      double rm = pr[0], pm = pr[4];
+      const double v2 = pr[1] * pr[1] + pr[2] * pr[2] + pr[3] * pr[3];
+      if (rm < 1e-30) rm = 1e-30;
+      if (pm < 1e-30) pm = 1e-30;
+      const double c2 = gamma * pm / rm;
+      sh[15][lane] = pr[1];
+      sh[16][lane] = pr[2];
+      sh[17][lane] = pr[3];
+      sh[18][lane] = c2;
+      sh[19][lane] = (gamma / (gamma - 1.0)) * (pm / rm) + 0.5 * v2;
+      sh[20][lane] = 0.5 * (gamma - 1.0) * v2;
+      sh[21][lane] = sqrt(c2);
+      sh[22][lane] = 2.0 * mu / (fabs(re) * (rm * vol) + 1.0e-30);
     }
-    const double rs = ra + rb + rc + rva + rvb + rvc;
-    const double term = (1.0 / b.dt[id]) + 1.0 * 1.0 * rs;
-    own[sweep_field(q, c, ncp, SW_OWN, 9)] = 1.0 / term;
   }
-  // this cell seen as a neighbour: compute_flux_splitting's per-cell terms,
-  // the sound speed of calc_spectral_radius and the viscous coefficient of
-  // calc_visc_spectral_radius (same clamps)
-  double rm = pr[0], pm = pr[4];
-  const double v2 = pr[1] * pr[1] + pr[2] * pr[2] + pr[3] * pr[3];
-  if (rm < 1e-30) rm = 1e-30;
-  if (pm < 1e-30) pm = 1e-30;
-  const double c2 = gamma * pm / rm;
-  Dp[sweep_field(q, c, ncp, SW_D, 0)] = pr[1];
-  Dp[sweep_field(q, c, ncp, SW_D, 1)] = pr[2];
-  Dp[sweep_field(q, c, ncp, SW_D, 2)] = pr[3];
-  Dp[sweep_field(q, c, ncp, SW_D, 3)] = c2;
-  Dp[sweep_field(q, c, ncp, SW_D, 4)] = (gamma / (gamma - 1.0)) * (pm / rm) + 0.5 * v2;
-  Dp[sweep_field(q, c, ncp, SW_D, 5)] = 0.5 * (gamma - 1.0) * v2;
-  Dp[sweep_field(q, c, ncp, SW_D, 6)] = sqrt(c2);
-  Dp[sweep_field(q, c, ncp, SW_D, 7)] = 2.0 * mu / (fabs(re) * (rm * vol) + 1.0e-30);
+  __syncthreads();
+  const int src = sweep_slot(lane);
+  const int m = m0 + src / PT, c = c0 + src % PT;
+  if (m < b.sn[0] && c < b.sn[1]) {
+    const int ncp = b.sncp;
+    const long long rb = sweep_index(b, m, r, c) - c;
+#pragma unroll
+    for (int k = 0; k < SW_OWN; ++k)
+      S.own[rb * SW_OWN + (long long)k * ncp + c] = make_double2(sh[2 * k][src], 2 * k + 1 < 15 ? sh[2 * k + 1][src] : 0.0);
+#pragma unroll
+    for (int k = 0; k < SW_D; ++k)
+      S.D[rb * SW_D + (long long)k * ncp + c] = make_double2(sh[15 + 2 * k][src], sh[16 + 2 * k][src]);
+  }
 }
 
 // dq back to the block arrays (zero on physical-BC windows,
@@ -157,29 +204,40 @@ __global__ void __launch_bounds__(256) lu_pack_kernel(const BlockDev* __restrict
 __global__ void __launch_bounds__(256) update_kernel(const BlockDev* __restrict__ blocks,
                                                      Span SP, SweepArrays S,
                                                      double gama) {
+  __shared__ double sh[5][PT * PT];
   int bi;
   long long gi;
-  if (!span_item(SP, &bi, &gi)) return;
+  if (!span_item(SP, &bi, &gi)) return;  // whole tiles
   const BlockDev& b = blocks[bi];
-  int i, j, k, m, r, c;
-  decode_interior(b, gi, &i, &j, &k);
-  const long long id = pidx(b, i, j, k);
+  int m0, c0, r;
+  pack_tile(b, gi, &m0, &c0, &r);
+  const int lane = threadIdx.x;
+  {
+    const int src = sweep_slot(lane);
+    const int m = m0 + src / PT, c = c0 + src % PT;
+    if (m < b.sn[0] && c < b.sn[1]) {
+      const int ncp = b.sncp;
+      const long long rb = sweep_index(b, m, r, c) - c;
+      const double2 a = S.dq[rb * SW_DQ + c], e = S.dq[rb * SW_DQ + ncp + c];
+      const double g = reinterpret_cast<const double*>(S.dq)[2 * (rb * SW_DQ + 2 * (long long)ncp + c)];
+      sh[0][src] = a.x;
+      sh[1][src] = a.y;
+      sh[2][src] = e.x;
+      sh[3][src] = e.y;
+      sh[4][src] = g;
+    }
+  }
+  __syncthreads();
+  const int m = m0 + lane / PT, c = c0 + lane % PT;
+  if (!(m < b.sn[0] && c < b.sn[1])) return;
+  const long long id = natural_id(b, m, r, c);
   const long long P = b.P;
   double d[5];
-  if (b.frozen[id]) {
+  const bool fz = b.frozen[id] != 0;
 #pragma unroll
-    for (int f = 0; f < 5; ++f) {
-      d[f] = 0.0;
-      b.dq[f * P + id] = 0.0;
-    }
-  } else {
-    natural_to_sweep(b, i, j, k, &m, &r, &c);
-    const long long q = sweep_index(b, m, r, c);
-#pragma unroll
-    for (int f = 0; f < 5; ++f) {
-      d[f] = reinterpret_cast<const double*>(S.dq)[sweep_field(q, c, b.sncp, SW_DQ, f)];
-      b.dq[f * P + id] = d[f];
-    }
+  for (int f = 0; f < 5; ++f) {
+    d[f] = fz ? 0.0 : sh[f][lane];
+    b.dq[f * P + id] = d[f];
   }
   const double gm1 = gama - 1.0;
   const double rr = b.prim[id], u = b.prim[P + id], v = b.prim[2 * P + id], w = b.prim[3 * P + id];
@@ -719,11 +777,6 @@ __global__ void __launch_bounds__(TL, 1) lusgs_tile_kernel(SweepArrays S, const 
 
 __global__ void epoch_kernel(unsigned long long* epoch) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *epoch += 1;
-}
-
-void launch_lu_geom(const Launch& La, long long n, const SweepArrays& S) {
-  if (n <= 0) return;
-  hipLaunchKernelGGL(lu_geom_kernel, dim3((unsigned)La.nwg), dim3(SPAN_TB), 0, La.stream, La.blocks, La.span, S);
 }
 
 void launch_lu_pack(const Launch& La, long long n, const SweepArrays& S, const Params& p) {
