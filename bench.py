@@ -36,10 +36,10 @@ HBM_PEAK_GBS = 8000.0
 STAGES = {
     "rhs": (20, ["visc_deriv", "visc_face", "visc_diff", "inv_face", "inv_closure", "inv_diff"],
             "visc_deriv_kernel+visc_face_kernel+visc_diff_kernel+inv_face_kernel+inv_closure_kernel+inv_diff_kernel"),
-    "lu_fwd": (25, ["lu_pack", "lu_fwd"], "lu_pack_kernel+lusgs_tile_kernel<0>"),
-    "lu_bwd": (25, ["lu_bwd"], "lusgs_tile_kernel<1>"),
-    "gs_fwd": (30, ["gs_fwd"], "lusgs_tile_kernel<2>"),
-    "gs_bwd": (30, ["gs_bwd"], "lusgs_tile_kernel<3>"),
+    "lu_fwd": (25, ["lu_pack", "lu_fwd"], "lu_pack_kernel+lusgs_tile_kernel<0,true>"),
+    "lu_bwd": (25, ["lu_bwd"], "lusgs_tile_kernel<1,true>"),
+    "gs_fwd": (30, ["gs_fwd"], "lusgs_tile_kernel<2,true>"),
+    "gs_bwd": (30, ["gs_bwd"], "lusgs_tile_kernel<3,true>"),
     "update": (25, ["update", "residual"], "update_kernel+residual_kernel+residual_final_kernel"),
 }
 STEP_WORDS = sum(v[0] for v in STAGES.values())
@@ -122,7 +122,7 @@ TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "hbm_traffic_{workload}.txt")
 def _kernel_key(name):
     """Short kernel name of a rocprofv3 Kernel_Name ("void ns::k<2>(args)" -> "k<2>")."""
     n = name.split("(")[0].replace("void ", "").strip()
-    return n.split("::")[-1]
+    return n.split("::")[-1].replace(" ", "")
 
 
 def summarize_counters(fetch_csv, write_csv, cfetch_csv, cwrite_csv, steps, cmd):

@@ -115,7 +115,11 @@ static int upload_vec(orion_hip_ctx* c, T** dst, const std::vector<T>& v) {
 
 static int make_space(orion_hip_ctx* c, Space& s, const std::vector<long long>& counts) {
   s.h.assign(counts.size() + 1, 0);
-  for (size_t b = 0; b < counts.size(); ++b) s.h[b + 1] = s.h[b] + counts[b];
+  for (size_t b = 0; b < counts.size(); ++b) {
+    // kernels decode a block's item index in 32-bit arithmetic
+    if (counts[b] >= (1LL << 31)) return fail("block %d: %lld items exceed the 32-bit index range", (int)b + 1, counts[b]);
+    s.h[b + 1] = s.h[b] + counts[b];
+  }
   std::vector<int> wgb, wg0(counts.size());
   for (size_t b = 0; b < counts.size(); ++b) {
     wg0[b] = (int)wgb.size();

@@ -86,12 +86,13 @@ constexpr int PT = 16;
 
 // the tile of launch item `item` (256 items per tile)
 __device__ inline void pack_tile(const BlockDev& b, long long item, int* m0, int* c0, int* r) {
-  const int tc = (b.sn[1] + PT - 1) / PT, tm = (b.sn[0] + PT - 1) / PT;
-  long long t = item / (PT * PT);
-  *c0 = (int)(t % tc) * PT;
-  t /= tc;
-  *m0 = (int)(t % tm) * PT;
-  *r = (int)(t / tm);
+  const unsigned tc = (b.sn[1] + PT - 1) / PT, tm = (b.sn[0] + PT - 1) / PT;
+  unsigned q, rc, q2, rm;
+  udivmod((unsigned)(item / (PT * PT)), tc, &q, &rc);
+  udivmod(q, tm, &q2, &rm);
+  *c0 = (int)rc * PT;
+  *m0 = (int)rm * PT;
+  *r = (int)q2;
 }
 
 // tile position (mi * PT + ci) of the tile's u-th cell in sweep order:

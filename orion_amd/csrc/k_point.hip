@@ -24,9 +24,10 @@ __global__ void __launch_bounds__(256) state_kernel(const BlockDev* __restrict__
   long long n;
   if (span_item(SP, &bi, &n)) {
     const BlockDev& b = blocks[bi];
-    const int i = (int)(n % b.nx);
-    const long long r = n / b.nx;
-    const int j = (int)(r % b.ny), k = (int)(r / b.ny);
+    unsigned q, ui, q2, uj;
+    udivmod((unsigned)n, (unsigned)b.nx, &q, &ui);
+    udivmod(q, (unsigned)b.ny, &q2, &uj);
+    const int i = (int)ui, j = (int)uj, k = (int)q2;
     const int oi = (i < NG || i >= NG + b.idim), oj = (j < NG || j >= NG + b.jdim), ok = (k < NG || k >= NG + b.kdim);
     const int nout = oi + oj + ok;
     const long long P = b.P;

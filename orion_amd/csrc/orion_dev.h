@@ -158,43 +158,51 @@ __host__ __device__ inline int dcount(const BlockDev& b, int d) {
   return d == 0 ? b.idim : (d == 1 ? b.jdim : b.kdim);
 }
 
+// Index decodes work in 32-bit unsigned arithmetic (a block's item counts
+// are checked to fit when the iteration spaces are built): a 64-bit
+// division costs several times the instructions of a 32-bit one.
+__device__ inline void udivmod(unsigned n, unsigned d, unsigned* q, unsigned* r) {
+  *q = n / d;
+  *r = n - *q * d;
+}
+
 // Decode an index of the "faces along d" space (faces 0..n_d on every
 // interior line, i fastest in memory order) into the padded index of the
 // face's line start (position 0 along d) and the face number f.
-__device__ inline void decode_face(const BlockDev& b, int d, long long n, long long* base, int* f) {
+__device__ inline void decode_face(const BlockDev& b, int d, long long n64, long long* base, int* f) {
+  const unsigned n = (unsigned)n64;
+  unsigned q, r, q2, r2;
   if (d == 0) {
-    int nf = b.idim + 1;
-    *f = (int)(n % nf);
-    long long r = n / nf;
-    int j = (int)(r % b.jdim) + NG, k = (int)(r / b.jdim) + NG;
-    *base = pidx(b, 0, j, k);
+    udivmod(n, (unsigned)b.idim + 1u, &q, &r);
+    udivmod(q, (unsigned)b.jdim, &q2, &r2);
+    *f = (int)r;
+    *base = pidx(b, 0, (int)r2 + NG, (int)q2 + NG);
   } else if (d == 1) {
-    int i = (int)(n % b.idim) + NG;
-    long long r = n / b.idim;
-    int nf = b.jdim + 1;
-    *f = (int)(r % nf);
-    int k = (int)(r / nf) + NG;
-    *base = pidx(b, i, 0, k);
+    udivmod(n, (unsigned)b.idim, &q, &r);
+    udivmod(q, (unsigned)b.jdim + 1u, &q2, &r2);
+    *f = (int)r2;
+    *base = pidx(b, (int)r + NG, 0, (int)q2 + NG);
   } else {
-    int i = (int)(n % b.idim) + NG;
-    long long r = n / b.idim;
-    int j = (int)(r % b.jdim) + NG;
-    *f = (int)(r / b.jdim);
-    *base = pidx(b, i, j, 0);
+    udivmod(n, (unsigned)b.idim, &q, &r);
+    udivmod(q, (unsigned)b.jdim, &q2, &r2);
+    *f = (int)q2;
+    *base = pidx(b, (int)r + NG, (int)r2 + NG, 0);
   }
 }
 
 // Decode an index of the "lines along d" space (interior lines, i fastest).
-__device__ inline long long decode_line(const BlockDev& b, int d, long long n) {
+__device__ inline long long decode_line(const BlockDev& b, int d, long long n64) {
+  const unsigned n = (unsigned)n64;
+  unsigned q, r;
   if (d == 0) {
-    int j = (int)(n % b.jdim) + NG, k = (int)(n / b.jdim) + NG;
-    return pidx(b, 0, j, k);
+    udivmod(n, (unsigned)b.jdim, &q, &r);
+    return pidx(b, 0, (int)r + NG, (int)q + NG);
   } else if (d == 1) {
-    int i = (int)(n % b.idim) + NG, k = (int)(n / b.idim) + NG;
-    return pidx(b, i, 0, k);
+    udivmod(n, (unsigned)b.idim, &q, &r);
+    return pidx(b, (int)r + NG, 0, (int)q + NG);
   }
-  int i = (int)(n % b.idim) + NG, j = (int)(n / b.idim) + NG;
-  return pidx(b, i, j, 0);
+  udivmod(n, (unsigned)b.idim, &q, &r);
+  return pidx(b, (int)r + NG, (int)q + NG, 0);
 }
 __host__ __device__ inline long long nlines(const BlockDev& b, int d) {
   return d == 0 ? (long long)b.jdim * b.kdim : (d == 1 ? (long long)b.idim * b.kdim : (long long)b.idim * b.jdim);
@@ -207,11 +215,13 @@ __device__ inline long long line_id(const BlockDev& b, int d, int i, int j, int 
 }
 
 // decode an interior point index into padded (i,j,k)
-__device__ inline void decode_interior(const BlockDev& b, long long n, int* i, int* j, int* k) {
-  *i = (int)(n % b.idim) + NG;
-  long long r = n / b.idim;
-  *j = (int)(r % b.jdim) + NG;
-  *k = (int)(r / b.jdim) + NG;
+__device__ inline void decode_interior(const BlockDev& b, long long n64, int* i, int* j, int* k) {
+  unsigned q, r, q2, r2;
+  udivmod((unsigned)n64, (unsigned)b.idim, &q, &r);
+  udivmod(q, (unsigned)b.jdim, &q2, &r2);
+  *i = (int)r + NG;
+  *j = (int)r2 + NG;
+  *k = (int)q2 + NG;
 }
 
 }  // namespace orion_hip
