@@ -263,7 +263,7 @@ static long long arena_fields(const BlockDev& b, int keep_q, long long lines_tot
   auto al = [](long long x) { return (x + 31) / 32 * 32; };  // 256-byte alignment
   n += al(6 * P) + al(9 * P) + al(P) + al(5 * P) + al(5 * P) + al(P) + al(P) + al(P);
   if (keep_q) n += al(5 * P);
-  n += al(12 * P) + al(15 * P) + al(30 * lines_total) + al((P + 7) / 8);
+  n += al(12 * P) + al(15 * P) + al(30 * lines_total) + al((P + 7) / 8) + al(3 * FG_N * P);
   return n;
 }
 
@@ -633,6 +633,7 @@ int orion_hip_ctx_create(int nblocks, const int* dims, const double* xyz, int nr
       B.fcov_off[1] = 30 * nlines(B, 0);
       B.fcov_off[2] = 30 * (nlines(B, 0) + nlines(B, 1));
       B.frozen = (unsigned char*)a; a += al((P + 7) / 8);
+      B.fgeo = a; a += al(3 * FG_N * P);
     }
   }
   // regions
@@ -755,6 +756,7 @@ int orion_hip_ctx_create(int nblocks, const int* dims, const double* xyz, int nr
         hipStreamSynchronize(c->stream) != hipSuccess)
       return bail(fail("metric check failed: %s", hipGetErrorString(hipGetLastError())));
     if (bad) return bail(fail("ERROR(Jacobi<0): %llu points with volume below 1e-15 (MetricComputer.cpp:488-491)", bad));
+    for (int d = 0; d < 3; ++d) launch_face_geom(LA(c, c->face[d]), d, c->face[d].total());
   }
   if (p.nvis != 0) launch_temperature(LA(c, c->pad), c->pad.total(), p.gama, p.moo);
   if (hipStreamSynchronize(c->stream) != hipSuccess) return bail(fail("setup failed: %s", hipGetErrorString(hipGetLastError())));

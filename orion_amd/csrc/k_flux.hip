@@ -225,7 +225,12 @@ __global__ void __launch_bounds__(256) inv_face_kernel(const BlockDev* __restric
   const int ied = (L.at(0, n + 5) < 1.0e-20) ? n - 2 : n;
   double qwl[5], qwr[5], k3[3], fo[5];
   face_states(L, n, ist, ied, f, qwl, qwr);
-  metric_face(b, d, base, s, n, f, k3);
+  {
+    const double* G = b.fgeo + (long long)(d * FG_N + 10) * b.P + base + (long long)(f + 2) * s;
+    k3[0] = G[0];
+    k3[1] = G[b.P];
+    k3[2] = G[2 * b.P];
+  }
   face_flux(L, n, f, qwl, qwr, k3, gamma, fo);
   double* F = b.fx + (long long)d * 5 * b.P + base + (long long)(f + 2) * s;
 #pragma unroll
@@ -258,7 +263,10 @@ __global__ void __launch_bounds__(256) inv_closure_kernel(const BlockDev* __rest
     if (!need) continue;
     const int sl = slot(f);
     face_states(L, n, ist, ied, f, QL[sl], QR[sl]);
-    metric_face(b, d, base, s, n, f, k3[sl]);
+    const double* G = b.fgeo + (long long)(d * FG_N + 10) * b.P + base + (long long)(f + 2) * s;
+    k3[sl][0] = G[0];
+    k3[sl][1] = G[b.P];
+    k3[sl][2] = G[2 * b.P];
   }
   auto flux = [&](int st, int en) {
     for (int f = st; f <= en; ++f) face_flux(L, n, f, QL[slot(f)], QR[slot(f)], k3[slot(f)], gamma, FF[slot(f)]);
@@ -482,6 +490,30 @@ __device__ inline double half_deriv(const double* q, long long s, int n, int f, 
   return (27.0 * (c3 - c2) + -1.0 * (c4 - c1)) / 24.0;
 }
 
+// Face geometry (once, after the metrics): the central interpolation of
+// the metrics and the volume used by the viscous flux (:183-205 via
+// interp_line_to_half) and the inviscid metrics with their end formulas.
+__global__ void __launch_bounds__(256) face_geom_kernel(const BlockDev* __restrict__ blocks, Span SP, int d) {
+  int bi;
+  long long gi;
+  if (!span_item(SP, &bi, &gi)) return;
+  const BlockDev& b = blocks[bi];
+  long long base;
+  int f;
+  decode_face(b, d, gi, &base, &f);
+  const int n = dcount(b, d);
+  const long long s = dstride(b, d);
+  const long long P = b.P;
+  double* G = b.fgeo + (long long)(d * FG_N) * P + base + (long long)(f + 2) * s;
+#pragma unroll
+  for (int m = 0; m < 9; ++m) G[(long long)m * P] = half_central(b.met + (long long)m * P + base, s, f);
+  G[9 * P] = half_central(b.vol + base, s, f);
+  double k3[3];
+  metric_face(b, d, base, s, n, f, k3);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) G[(long long)(10 + c) * P] = k3[c];
+}
+
 __global__ void __launch_bounds__(256) visc_face_kernel(const BlockDev* __restrict__ blocks,
                                                         Span SP, int d,
                                                         double cp_prl) {
@@ -506,10 +538,11 @@ __global__ void __launch_bounds__(256) visc_face_kernel(const BlockDev* __restri
   }
   const double vs1 = half_central(b.mu + base, s, f);
   const double vs2 = vs1 * cp_prl;
-  const double volh = half_central(b.vol + base, s, f);
+  const double* G = b.fgeo + (long long)(d * FG_N) * P + base + (long long)(f + 2) * s;
+  const double volh = G[9 * P];
   double kx[9];
 #pragma unroll
-  for (int m = 0; m < 9; ++m) kx[m] = half_central(b.met + (long long)m * P + base, s, f);
+  for (int m = 0; m < 9; ++m) kx[m] = G[(long long)m * P];
   // cross derivatives at the face from node derivatives clamped to the
   // interior (positions f+1..f+4 -> clamp to [NG, n+NG-1])
   const int o0 = (d == 0) ? 1 : 0, o1 = (d == 2) ? 1 : 2;
@@ -603,6 +636,10 @@ void launch_inv_diff(const Launch& L, long long n) {
 void launch_visc_deriv(const Launch& L, long long n) {
   if (n <= 0) return;
   hipLaunchKernelGGL(visc_deriv_kernel, dim3((unsigned)L.nwg), dim3(SPAN_TB), 0, L.stream, L.blocks, L.span);
+}
+void launch_face_geom(const Launch& L, int d, long long n) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(face_geom_kernel, dim3((unsigned)L.nwg), dim3(SPAN_TB), 0, L.stream, L.blocks, L.span, d);
 }
 void launch_visc_face(const Launch& L, int d, long long n, double cp_prl) {
   if (n <= 0) return;

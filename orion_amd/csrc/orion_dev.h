@@ -37,6 +37,7 @@ struct BlockDev {
   double* duvwt;         // [12][P] viscous cell derivatives (scratch)
   double* fx;            // [3][5][P] face fluxes per direction (scratch)
   double* fcov;          // closure overrides, [3][lines][6][5] (scratch)
+  double* fgeo;          // [3][FG_N][P] face geometry per direction (static, see FG_N)
   unsigned char* frozen; // [P] 1 inside a physical-BC window
   long long fcov_off[3]; // offset (doubles) of each direction's overrides
   // LU-SGS sweep layout of this block: axes (march, col, row) as real axes
@@ -47,6 +48,13 @@ struct BlockDev {
   int sncp;
   long long sbase;
 };
+
+// Face geometry, interpolated once after the metrics (the grid is static)
+// and stored per direction d at the face's slot (f + 2) along d, like the
+// face fluxes: 0..8 the nine metrics and 9 the volume in the central form
+// of interp_line_to_half (viscous flux), 10..12 the direction's metrics with
+// the one-sided end formulas of the inviscid flux.
+constexpr int FG_N = 13;
 
 // LU-SGS working set in sweep layout (all blocks, n cells incl. padding).
 // A block's cell (m, r, c) along (march, row, col) lies on hyperplane

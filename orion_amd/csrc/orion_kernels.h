@@ -41,6 +41,7 @@ void launch_inv_face(const Launch& L, int d, long long n, double gamma);
 void launch_inv_closure(const Launch& L, int d, long long n, double gamma);
 void launch_inv_diff(const Launch& L, long long n);
 void launch_visc_deriv(const Launch& L, long long n);
+void launch_face_geom(const Launch& L, int d, long long n);
 void launch_visc_face(const Launch& L, int d, long long n, double cp_prl);
 void launch_visc_diff(const Launch& L, long long n, double re_inv);
 
