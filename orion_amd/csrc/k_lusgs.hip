@@ -346,14 +346,14 @@ constexpr unsigned waitcnt_vm(unsigned n) { return (n & 15u) | ((n >> 4) << 14) 
 //         partial sum its forward sweep left (5)
 //   ga    Gauss-Seidel forward: the three after-neighbour terms (15)
 //   D     the cell's packed values, handed on to its downstream neighbours
-//   xd,xq the lane's share of the upstream edge values (first wave only)
+//   xd,xq the lane's share of the upstream edge values
 template <int TYPE>
 struct StepIn {
   double own[TYPE == 3 ? 20 : 15];
   double ga[TYPE == 2 ? 15 : 1];
   double D[8];
-  double2 xd[2];
-  double xq[3];
+  double2 xd;  // the lane's share of the upstream edge values
+  double xq;
   unsigned long long pf[2];  // upstream flags, read a step before they are checked (thread 0)
 };
 
@@ -421,11 +421,10 @@ __device__ __forceinline__ void load_mid(const SweepArrays& S, long long rb, int
   }
 }
 
-// Vector memory loads every wave issues per step: own inputs, D, ga and
-// the 5 edge loads.
+// Prefetch loads every wave issues per step: own inputs, D and ga.
 template <int TYPE>
 constexpr unsigned step_loads() {
-  return (TYPE == 1 ? 5 + SW_DQ : SW_OWN) + (TYPE == 3 ? SW_PRE : 0) + SW_D + (TYPE == 2 ? SW_GA : 0) + 5;
+  return (TYPE == 1 ? 5 + SW_DQ : SW_OWN) + (TYPE == 3 ? SW_PRE : 0) + SW_D + (TYPE == 2 ? SW_GA : 0);
 }
 
 // Sweep types: 0 LU forward, 1 LU backward, 2 Gauss-Seidel forward,
@@ -471,7 +470,6 @@ __global__ void __launch_bounds__(TL, 1) lusgs_tile_kernel(SweepArrays S, const 
   constexpr int dir = FWD ? 1 : -1;
   constexpr int SL0 = FWD ? 0 : 3;  // slots of the neighbours before a cell in sweep order
   constexpr unsigned NLD = step_loads<TYPE>();
-  constexpr unsigned NEDGE = 5;  // edge loads, the last of a step's loads
   __shared__ double shA[2][NW][TL];     // (D, new dq) of each lane's cell, by step parity
   __shared__ double shX[2][2][NW][TW];  // upstream edge values, by step parity
 
@@ -504,29 +502,23 @@ __global__ void __launch_bounds__(TL, 1) lusgs_tile_kernel(SweepArrays S, const 
   auto cell_rb = [&](int t) { return cell_ok(t) ? rb0 + (long long)t * qstep : b.sbase; };
   auto cell_c = [&](int t) { return cell_ok(t) ? c : 0; };
 
-  // Upstream edge values, taken by the first wave (whose lane 0 checks the
-  // upstream progress): per step 2 x 16 cells of 4 D pairs and 5 dq words.
-  // Lane l loads D items l and l + 64 and dq items l, l + 64, l + 128; item
-  // e is (role, index, part), role 0 the cells across the upstream col
-  // edge, role 1 those across the upstream row edge.  A neighbour of a cell
-  // on plane s lies on s - dir.
-  const bool w0 = lane < 64;
-  int er[5], ec[5], xo[5];
-  bool ein[5];
+  // Upstream edge values: per step 2 x 16 cells of 4 D pairs and 5 dq
+  // words, spread over the tile's lanes: lane l < 128 loads D pair item l,
+  // lane l < 160 dq word item l.  Item e is (role, index, part), role 0 the
+  // cells across the upstream col edge, role 1 those across the upstream
+  // row edge.  The other lanes read the block's first cell and never use
+  // the values, so that every wave issues the same loads.  A neighbour of a
+  // cell on plane s lies on s - dir.  The loads follow the barrier after
+  // thread 0 has checked the upstream progress.
+  int er[2], ec[2], xo[2];
+  bool ein[2];
 #pragma unroll
-  for (int j = 0; j < 5; ++j) {
-    int role, idx, word;
-    if (j < 2) {
-      const int e = lane + 64 * j;
-      role = e >> 6;
-      idx = (e >> 2) & 15;
-      word = 2 * (e & 3);
-    } else {
-      const int e = lane + 64 * (j - 2);
-      role = e < 80 ? 0 : 1;
-      idx = (e % 80) / 5;
-      word = 8 + e % 5;
-    }
+  for (int j = 0; j < 2; ++j) {
+    const int e = lane;
+    const bool item = j == 0 ? e < 128 : e < 160;
+    const int role = j == 0 ? (e >> 6) & 1 : (e < 80 ? 0 : 1);
+    const int idx = j == 0 ? (e >> 2) & 15 : (e % 80) / 5;
+    const int word = j == 0 ? 2 * (e & 3) : 8 + e % 5;
     if (role == 0) {
       ec[j] = FWD ? T.c0 - 1 : T.c0 + TW;
       er[j] = T.r0 + idx;
@@ -534,8 +526,8 @@ __global__ void __launch_bounds__(TL, 1) lusgs_tile_kernel(SweepArrays S, const 
       er[j] = FWD ? T.r0 - 1 : T.r0 + TW;
       ec[j] = T.c0 + idx;
     }
-    ein[j] = w0 && er[j] >= 0 && er[j] < nr && ec[j] >= 0 && ec[j] < nc && (j < 4 || lane < 32);
-    xo[j] = (role * NW + word) * TW + idx;  // offset in one parity of shX
+    ein[j] = item && er[j] >= 0 && er[j] < nr && ec[j] >= 0 && ec[j] < nc;
+    xo[j] = item ? (role * NW + word) * TW + idx : -1;  // offset in one parity of shX
   }
   auto edge_rb = [&](int j, int t, int* cc) -> long long {
     const int sp = s0 + dir * (t - 1);
@@ -544,24 +536,13 @@ __global__ void __launch_bounds__(TL, 1) lusgs_tile_kernel(SweepArrays S, const 
     *cc = ok ? ec[j] : 0;
     return ok ? b.sbase + ((long long)sp * nr + er[j]) * ncp : b.sbase;
   };
-  // Every wave issues these loads (the other waves read the block's first
-  // cell and never use the values): with the same loads on every path the
-  // compiler's waits for them stay as loose as the pipeline allows.
   auto load_edges = [&](int t, StepIn<TYPE>& P) {
-#pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      int cc;
-      const long long rb = edge_rb(j, t, &cc);
-      P.xd[j] = ld_pair(S.D, rb, cc, ncp, SW_D, lane & 3);
-    }
-    const double* dqp = reinterpret_cast<const double*>(S.dq);
-#pragma unroll
-    for (int j = 0; j < 3; ++j) {
-      int cc;
-      const long long rb = edge_rb(2 + j, t, &cc);
-      P.xq[j] = __hip_atomic_load(&dqp[sweep_field(rb + cc, cc, ncp, SW_DQ, (lane + 64 * j) % 5)],
-                                  __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
+    int cc;
+    long long rb = edge_rb(0, t, &cc);
+    P.xd = ld_pair(S.D, rb, cc, ncp, SW_D, lane & 3);
+    rb = edge_rb(1, t, &cc);
+    P.xq = __hip_atomic_load(&reinterpret_cast<const double*>(S.dq)[sweep_field(rb + cc, cc, ncp, SW_DQ, lane % 5)],
+                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   };
 
   // thread 0: the upstream tiles must have finished plane s; pf holds the
@@ -619,21 +600,19 @@ __global__ void __launch_bounds__(TL, 1) lusgs_tile_kernel(SweepArrays S, const 
     const int m = m0 + dir * t;
     const bool act = cell_ok(t);
     // A
-    if (w0) {
+    {
       double* x = &shX[p][0][0][0];
-      x[xo[0]] = cur.xd[0].x;
-      x[xo[0] + TW] = cur.xd[0].y;
-      x[xo[1]] = cur.xd[1].x;
-      x[xo[1] + TW] = cur.xd[1].y;
-      x[xo[2]] = cur.xq[0];
-      x[xo[3]] = cur.xq[1];
-      if (lane < 32) x[xo[4]] = cur.xq[2];
+      if (xo[0] >= 0) {
+        x[xo[0]] = cur.xd.x;
+        x[xo[0] + TW] = cur.xd.y;
+      }
+      if (xo[1] >= 0) x[xo[1]] = cur.xq;
     }
-    // B: only the edge loads, issued after this wave's stores of step t-1,
-    // may stay outstanding (in-order counter), so those stores have completed
+    // B: this wave's memory operations have completed, its stores of step
+    // t-1 among them
     asm volatile("" ::: "memory");
     __builtin_amdgcn_sched_barrier(0);
-    __builtin_amdgcn_s_waitcnt(waitcnt_vm(NEDGE));
+    __builtin_amdgcn_s_waitcnt(waitcnt_vm(0));
     __builtin_amdgcn_sched_barrier(0);
     __syncthreads();
     // C
@@ -643,6 +622,9 @@ __global__ void __launch_bounds__(TL, 1) lusgs_tile_kernel(SweepArrays S, const 
     // the upstream flags for the check of the next step, read here so that
     // they return long before any wait of this wave needs them
     read_up(mid.pf);
+    // the edge values of step t+1 lie on plane s, which thread 0 found
+    // finished before this step's barrier
+    load_edges(t + 1, mid);
     // D: the update of every lane's cell, branch free (a lane without a cell
     // computes on the block's first cell and stores nothing), with the loads
     // of later steps interleaved: D and ga of step t+1, own inputs of t+2
@@ -706,7 +688,7 @@ __global__ void __launch_bounds__(TL, 1) lusgs_tile_kernel(SweepArrays S, const 
     }
     // one prefetch load between runs of arithmetic
 #pragma unroll
-    for (int i = 0; i < (int)(NLD - NEDGE); ++i) {
+    for (int i = 0; i < (int)NLD; ++i) {
       __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
       __builtin_amdgcn_sched_group_barrier(0x002, 24, 0);
     }
@@ -736,11 +718,10 @@ __global__ void __launch_bounds__(TL, 1) lusgs_tile_kernel(SweepArrays S, const 
     for (int f = 0; f < 8; ++f) shA[p][f][lane] = cur.D[f];
 #pragma unroll
     for (int f = 0; f < 5; ++f) shA[p][8 + f][lane] = dqn[f];
-    // E: nothing below may move above the stores (the count in B relies on it)
+    // E: the edge values of step t+2 lie on plane s(t+1)
     asm volatile("" ::: "memory");
     __builtin_amdgcn_sched_barrier(0);
-    if (t + 2 < nsteps) ensure_up(s + dir, cur.pf);  // edge values of step t+2 lie on plane s(t+1)
-    load_edges(t + 2, nxt);
+    if (t + 2 < nsteps) ensure_up(s + dir, cur.pf);
   };
 
   StepIn<TYPE> PA, PB, PC;
@@ -756,11 +737,11 @@ __global__ void __launch_bounds__(TL, 1) lusgs_tile_kernel(SweepArrays S, const 
     ensure_up(s0, none);  // covers the edge values of steps 0 and 1
   }
   read_up(PA.pf);
+  __syncthreads();
   load_edges(0, PA);
   load_own<TYPE>(S, cell_rb(0), cell_c(0), ncp, PA.own);
   load_mid<TYPE>(S, cell_rb(0), cell_c(0), ncp, PA);
   load_own<TYPE>(S, cell_rb(1), cell_c(1), ncp, PB.own);
-  load_edges(1, PB);
   // whole triples of steps: the steps past the end are idle (no cell, no
   // store, no publication) but keep the loop free of side exits, whose
   // paths would make the compiler's load waits stricter
