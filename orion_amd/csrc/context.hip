@@ -757,6 +757,7 @@ int orion_hip_ctx_create(int nblocks, const int* dims, const double* xyz, int nr
       return bail(fail("metric check failed: %s", hipGetErrorString(hipGetLastError())));
     if (bad) return bail(fail("ERROR(Jacobi<0): %llu points with volume below 1e-15 (MetricComputer.cpp:488-491)", bad));
     for (int d = 0; d < 3; ++d) launch_face_geom(LA(c, c->face[d]), d, c->face[d].total());
+    launch_lu_geom(LA(c, c->ptile), c->ptile.total(), c->sw);
   }
   if (p.nvis != 0) launch_temperature(LA(c, c->pad), c->pad.total(), p.gama, p.moo);
   if (hipStreamSynchronize(c->stream) != hipSuccess) return bail(fail("setup failed: %s", hipGetErrorString(hipGetLastError())));

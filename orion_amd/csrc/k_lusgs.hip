@@ -115,7 +115,38 @@ __device__ inline long long natural_id(const BlockDev& b, int m, int r, int c) {
          (long long)r * dstride(b, b.sax[2]);
 }
 
-constexpr int NPK = 23;  // packed words per cell: met 9, coed, rhs 5, D 8
+// Once, after the metrics: the static own pairs 0..3 (met0..met7).
+__global__ void __launch_bounds__(256) lu_geom_kernel(const BlockDev* __restrict__ blocks, Span SP, SweepArrays S) {
+  __shared__ double sh[8][PT * PT];
+  int bi;
+  long long gi;
+  if (!span_item(SP, &bi, &gi)) return;  // whole tiles: every lane of a workgroup has an item
+  const BlockDev& b = blocks[bi];
+  int m0, c0, r;
+  pack_tile(b, gi, &m0, &c0, &r);
+  const int lane = threadIdx.x;
+  {
+    const int m = m0 + lane / PT, c = c0 + lane % PT;
+    if (m < b.sn[0] && c < b.sn[1]) {
+      const long long id = natural_id(b, m, r, c);
+#pragma unroll
+      for (int f = 0; f < 8; ++f) sh[f][lane] = b.met[f * b.P + id];
+    }
+  }
+  __syncthreads();
+  const int src = sweep_slot(lane);
+  const int m = m0 + src / PT, c = c0 + src % PT;
+  if (m < b.sn[0] && c < b.sn[1]) {
+    const long long rb = sweep_index(b, m, r, c) - c;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      S.own[rb * SW_OWN + (long long)k * b.sncp + c] = make_double2(sh[2 * k][src], sh[2 * k + 1][src]);
+  }
+}
+
+// packed words per cell and update: met8, coed, rhs 5 (own pairs 4..7) and
+// D 8; own pairs 0..3 are static
+constexpr int NPK = 15;
 
 // Per update: each cell's own inputs (metrics, the diagonal coed of
 // get_diag_coeff, the rhs) and, for its neighbours, u, v, w, c^2, H, a_f,
@@ -140,16 +171,14 @@ __global__ void __launch_bounds__(256) lu_pack_kernel(const BlockDev* __restrict
       // rhs copy with the physical-BC windows zeroed (StateUpdater.cpp:557-570)
       const bool fz = b.frozen[id] != 0;
 #pragma unroll
-      for (int f = 0; f < 5; ++f) sh[10 + f][lane] = fz ? 0.0 : b.dq[f * P + id];
+      for (int f = 0; f < 5; ++f) sh[2 + f][lane] = fz ? 0.0 : b.dq[f * P + id];
       double pr[5];
 #pragma unroll
       for (int f = 0; f < 5; ++f) pr[f] = b.prim[f * P + id];
       double met[9];
 #pragma unroll
-      for (int f = 0; f < 9; ++f) {
-        met[f] = b.met[f * P + id];
-        sh[f][lane] = met[f];
-      }
+      for (int f = 0; f < 9; ++f) met[f] = b.met[f * P + id];
+      sh[0][lane] = met[8];
       const double mu = b.mu[id];
       double vol = b.vol[id];
       if (vol < 1e-30) vol = 1e-30;
@@ -165,7 +194,7 @@ __global__ void __launch_bounds__(256) lu_pack_kernel(const BlockDev* __restrict
         }
         const double rs = ra + rb + rc + rva + rvb + rvc;
         const double term = (1.0 / b.dt[id]) + 1.0 * 1.0 * rs;
-        sh[9][lane] = 1.0 / term;
+        sh[1][lane] = 1.0 / term;
       }
       // this cell seen as a neighbour: compute_flux_splitting's per-cell
       // terms, the sound speed of calc_spectral_radius and the viscous
@@ -175,14 +204,14 @@ __global__ void __launch_bounds__(256) lu_pack_kernel(const BlockDev* __restrict
       if (rm < 1e-30) rm = 1e-30;
       if (pm < 1e-30) pm = 1e-30;
       const double c2 = gamma * pm / rm;
-      sh[15][lane] = pr[1];
-      sh[16][lane] = pr[2];
-      sh[17][lane] = pr[3];
-      sh[18][lane] = c2;
-      sh[19][lane] = (gamma / (gamma - 1.0)) * (pm / rm) + 0.5 * v2;
-      sh[20][lane] = 0.5 * (gamma - 1.0) * v2;
-      sh[21][lane] = sqrt(c2);
-      sh[22][lane] = 2.0 * mu / (fabs(re) * (rm * vol) + 1.0e-30);
+      sh[7][lane] = pr[1];
+      sh[8][lane] = pr[2];
+      sh[9][lane] = pr[3];
+      sh[10][lane] = c2;
+      sh[11][lane] = (gamma / (gamma - 1.0)) * (pm / rm) + 0.5 * v2;
+      sh[12][lane] = 0.5 * (gamma - 1.0) * v2;
+      sh[13][lane] = sqrt(c2);
+      sh[14][lane] = 2.0 * mu / (fabs(re) * (rm * vol) + 1.0e-30);
     }
   }
   __syncthreads();
@@ -192,11 +221,11 @@ __global__ void __launch_bounds__(256) lu_pack_kernel(const BlockDev* __restrict
     const int ncp = b.sncp;
     const long long rb = sweep_index(b, m, r, c) - c;
 #pragma unroll
-    for (int k = 0; k < SW_OWN; ++k)
-      S.own[rb * SW_OWN + (long long)k * ncp + c] = make_double2(sh[2 * k][src], 2 * k + 1 < 15 ? sh[2 * k + 1][src] : 0.0);
+    for (int k = 0; k < 4; ++k)
+      S.own[rb * SW_OWN + (long long)(4 + k) * ncp + c] = make_double2(sh[2 * k][src], k < 3 ? sh[2 * k + 1][src] : 0.0);
 #pragma unroll
     for (int k = 0; k < SW_D; ++k)
-      S.D[rb * SW_D + (long long)k * ncp + c] = make_double2(sh[15 + 2 * k][src], sh[16 + 2 * k][src]);
+      S.D[rb * SW_D + (long long)k * ncp + c] = make_double2(sh[7 + 2 * k][src], sh[8 + 2 * k][src]);
   }
 }
 
@@ -759,6 +788,11 @@ __global__ void __launch_bounds__(TL, 1) lusgs_tile_kernel(SweepArrays S, const 
 
 __global__ void epoch_kernel(unsigned long long* epoch) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *epoch += 1;
+}
+
+void launch_lu_geom(const Launch& La, long long n, const SweepArrays& S) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(lu_geom_kernel, dim3((unsigned)La.nwg), dim3(SPAN_TB), 0, La.stream, La.blocks, La.span, S);
 }
 
 void launch_lu_pack(const Launch& La, long long n, const SweepArrays& S, const Params& p) {

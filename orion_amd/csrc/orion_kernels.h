@@ -63,6 +63,7 @@ void launch_avg_stage(const BlockDev* blocks, const AvgSrc* src, long long n, do
 void launch_avg_apply(const BlockDev* blocks, const AvgDst* dst, long long n, const double* stage, hipStream_t s);
 
 // k_lusgs.hip
+void launch_lu_geom(const Launch& La, long long n, const SweepArrays& S);
 void launch_lu_pack(const Launch& La, long long n, const SweepArrays& S, const Params& p);
 void launch_lusgs_tiles(const SweepArrays& S, const BlockDev* blocks, const SweepTile* tiles, const int* order,
                         int ntiles, unsigned long long* flags, unsigned long long* epoch, int* err, int type,
