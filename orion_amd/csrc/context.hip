@@ -63,6 +63,7 @@ struct orion_hip_ctx {
   size_t arena_bytes = 0;
   size_t map_bytes = 0;
   Space pad, intr, face[3], line[3];
+  Space fpair[3];                      // face pairs (2g, 2g+1) per line along d
   Space ptile;                         // LU-SGS pack tiles: 16 x 16 (march, col) cells of one row
   SweepArrays sw{};                    // LU-SGS working set in sweep layout
   std::vector<SweepTile> tiles;
@@ -704,6 +705,11 @@ int orion_hip_ctx_create(int nblocks, const int* dims, const double* xyz, int nr
   if (make_space(c, c->pad, cp) || make_space(c, c->intr, ci)) return bail(-1);
   for (int d = 0; d < 3; ++d)
     if (make_space(c, c->face[d], cf[d]) || make_space(c, c->line[d], cl[d])) return bail(-1);
+  for (int d = 0; d < 3; ++d) {
+    std::vector<long long> cp2(nblocks);
+    for (int b = 0; b < nblocks; ++b) cp2[b] = nlines(c->hb[b], d) * ((dcount(c->hb[b], d) + 2) / 2);
+    if (make_space(c, c->fpair[d], cp2)) return bail(-1);
+  }
   if (build_sweep(c)) return bail(-1);
   {
     std::vector<long long> ct(nblocks);
@@ -832,7 +838,7 @@ static void op_visc(orion_hip_ctx* c) {
   const double re = c->p.reynolds;
   const double re_inv = (std::fabs(re) > 1.0e-30) ? 1.0 / re : 0.0;
   launch_visc_deriv(LA(c, c->intr), c->intr.total());
-  for (int d = 0; d < 3; ++d) launch_visc_face(LA(c, c->face[d]), d, c->face[d].total(), cp_prl);
+  for (int d = 0; d < 3; ++d) launch_visc_face(LA(c, c->fpair[d]), d, c->fpair[d].total(), cp_prl);
   launch_visc_diff(LA(c, c->intr), c->intr.total(), re_inv);
 }
 
@@ -1054,7 +1060,7 @@ int orion_hip_profile_run(orion_hip_ctx* c, int nsteps, double* phase_ms, long l
       const double cp = 1.0 / ((c->p.gama - 1.0) * c->p.moo * c->p.moo);
       const double re = c->p.reynolds;
       launch_visc_deriv(LA(c, c->intr), c->intr.total()); mark(1);
-      for (int d = 0; d < 3; ++d) launch_visc_face(LA(c, c->face[d]), d, c->face[d].total(), cp / c->p.prl);
+      for (int d = 0; d < 3; ++d) launch_visc_face(LA(c, c->fpair[d]), d, c->fpair[d].total(), cp / c->p.prl);
       mark(3);
       launch_visc_diff(LA(c, c->intr), c->intr.total(), (std::fabs(re) > 1.0e-30) ? 1.0 / re : 0.0); mark(1);
       op_avg(c); mark(2);

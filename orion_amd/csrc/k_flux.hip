@@ -514,6 +514,32 @@ __global__ void __launch_bounds__(256) face_geom_kernel(const BlockDev* __restri
   for (int c = 0; c < 3; ++c) G[(long long)(10 + c) * P] = k3[c];
 }
 
+// interp_line_to_half / compute_half_deriv_line at face f with the central
+// stencil values c1..c4 (positions f+1..f+4) already loaded
+__device__ inline double half_flagged_c(const double* q, long long s, int n, int f, bool fl, bool fr, double c1,
+                                        double c2, double c3, double c4) {
+  auto Q = [&](int p) { return q[(long long)p * s]; };
+  if (fl && f == 0) return (35.0 * Q(3) + -35.0 * Q(4) + 21.0 * Q(5) + -5.0 * Q(6)) / 16.0;
+  if (fl && f == 1) return (5.0 * Q(3) + 15.0 * Q(4) + -5.0 * Q(5) + 1.0 * Q(6)) / 16.0;
+  if (fr && f == n) return (35.0 * Q(n + 2) + -35.0 * Q(n + 1) + 21.0 * Q(n) + -5.0 * Q(n - 1)) / 16.0;
+  if (fr && f == n - 1) return (5.0 * Q(n + 2) + 15.0 * Q(n + 1) + -5.0 * Q(n) + 1.0 * Q(n - 1)) / 16.0;
+  return (9.0 * (c2 + c3) + -1.0 * (c4 + c1)) / 16.0;
+}
+__device__ inline double half_deriv_c(const double* q, long long s, int n, int f, bool fl, bool fr, double c1,
+                                      double c2, double c3, double c4) {
+  auto Q = [&](int p) { return q[(long long)p * s]; };
+  if (fl && f == 0) return (-71.0 * Q(3) + 141.0 * Q(4) + -93.0 * Q(5) + 23.0 * Q(6)) / 24.0;
+  if (fl && f == 1) return (-22.0 * Q(3) + 17.0 * Q(4) + 9.0 * Q(5) + -5.0 * Q(6) + 1.0 * Q(7)) / 24.0;
+  if (fr && f == n) return -(-71.0 * Q(n + 2) + 141.0 * Q(n + 1) + -93.0 * Q(n) + 23.0 * Q(n - 1)) / 24.0;
+  if (fr && f == n - 1)
+    return -(-22.0 * Q(n + 2) + 17.0 * Q(n + 1) + 9.0 * Q(n) + -5.0 * Q(n - 1) + 1.0 * Q(n - 2)) / 24.0;
+  return (27.0 * (c3 - c2) + -1.0 * (c4 - c1)) / 24.0;
+}
+
+// Viscous face fluxes (compute_viscous_flux_1d, FluxComputer.cpp:183-252)
+// of faces f0 = 2g and f0 + 1 of one line: the two faces share their
+// stencil windows (positions f0+1..f0+5), and the face geometry is the
+// static one interpolated after the metrics.
 __global__ void __launch_bounds__(256) visc_face_kernel(const BlockDev* __restrict__ blocks,
                                                         Span SP, int d,
                                                         double cp_prl) {
@@ -522,73 +548,89 @@ __global__ void __launch_bounds__(256) visc_face_kernel(const BlockDev* __restri
   if (!span_item(SP, &bi, &gi)) return;
   const BlockDev& b = blocks[bi];
   long long base;
-  int f;
-  decode_face(b, d, gi, &base, &f);
+  int f0;
+  decode_face_pair(b, d, gi, &base, &f0);
   const int n = dcount(b, d);
   const long long s = dstride(b, d);
   const long long P = b.P;
   const bool fl = b.prim[base] < 0.0;
   const bool fr = b.prim[base + (long long)(n + 5) * s] < 0.0;
-  double uvwt[4], dmid[4];
-#pragma unroll
-  for (int v = 0; v < 4; ++v) {
-    const double* q = b.prim + (long long)(v < 3 ? v + 1 : 5) * P + base;
-    uvwt[v] = half_flagged(q, s, n, f, fl, fr);
-    dmid[v] = half_deriv(q, s, n, f, fl, fr);
-  }
-  const double vs1 = half_central(b.mu + base, s, f);
-  const double vs2 = vs1 * cp_prl;
-  const double* G = b.fgeo + (long long)(d * FG_N) * P + base + (long long)(f + 2) * s;
-  const double volh = G[9 * P];
-  double kx[9];
-#pragma unroll
-  for (int m = 0; m < 9; ++m) kx[m] = G[(long long)m * P];
-  // cross derivatives at the face from node derivatives clamped to the
-  // interior (positions f+1..f+4 -> clamp to [NG, n+NG-1])
-  const int o0 = (d == 0) ? 1 : 0, o1 = (d == 2) ? 1 : 2;
-  double duv[12];
+  // windows at positions f0+1..f0+5 (the cross derivatives at positions
+  // clamped to the interior [NG, n+NG-1])
+  double W[4][5], M[5], X[8][5];
   auto clampp = [n](int p) { return p < NG ? NG : (p > n + NG - 1 ? n + NG - 1 : p); };
-  const int p1 = clampp(f + 1), p2 = clampp(f + 2), p3 = clampp(f + 3), p4 = clampp(f + 4);
+  const int o0 = (d == 0) ? 1 : 0, o1 = (d == 2) ? 1 : 2;
 #pragma unroll
-  for (int v = 0; v < 4; ++v) {
-    duv[4 * d + v] = dmid[v];
-    const double* A = b.duvwt + (long long)(4 * o0 + v) * P + base;
-    duv[4 * o0 + v] = (9.0 * (A[p2 * s] + A[p3 * s]) + -1.0 * (A[p4 * s] + A[p1 * s])) / 16.0;
-    const double* B = b.duvwt + (long long)(4 * o1 + v) * P + base;
-    duv[4 * o1 + v] = (9.0 * (B[p2 * s] + B[p3 * s]) + -1.0 * (B[p4 * s] + B[p1 * s])) / 16.0;
+  for (int j = 0; j < 5; ++j) {
+    const long long pos = (long long)(f0 + 1 + j) * s;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) W[v][j] = b.prim[(long long)(v < 3 ? v + 1 : 5) * P + base + pos];
+    M[j] = b.mu[base + pos];
+    const long long pc = (long long)clampp(f0 + 1 + j) * s;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      X[v][j] = b.duvwt[(long long)(4 * o0 + v) * P + base + pc];
+      X[4 + v][j] = b.duvwt[(long long)(4 * o1 + v) * P + base + pc];
+    }
   }
-  // transform_derivs_to_phys (:183-205)
-  const double vinv = 1.0 / volh;
-  double gx[12];
 #pragma unroll
-  for (int m = 0; m < 3; ++m) {
-    const double k1 = kx[m], k2 = kx[m + 3], k3 = kx[m + 6];
+  for (int k = 0; k < 2; ++k) {
+    const int f = f0 + k;
+    if (f > n) break;
+    double uvwt[4], dmid[4];
 #pragma unroll
-    for (int v = 0; v < 4; ++v) gx[v * 3 + m] = (k1 * duv[v] + k2 * duv[v + 4] + k3 * duv[v + 8]) * vinv;
+    for (int v = 0; v < 4; ++v) {
+      const double* q = b.prim + (long long)(v < 3 ? v + 1 : 5) * P + base;
+      uvwt[v] = half_flagged_c(q, s, n, f, fl, fr, W[v][k], W[v][k + 1], W[v][k + 2], W[v][k + 3]);
+      dmid[v] = half_deriv_c(q, s, n, f, fl, fr, W[v][k], W[v][k + 1], W[v][k + 2], W[v][k + 3]);
+    }
+    const double vs1 = (9.0 * (M[k + 1] + M[k + 2]) + -1.0 * (M[k + 3] + M[k])) / 16.0;
+    const double vs2 = vs1 * cp_prl;
+    const double* G = b.fgeo + (long long)(d * FG_N) * P + base + (long long)(f + 2) * s;
+    const double volh = G[9 * P];
+    double kx[9];
+#pragma unroll
+    for (int m = 0; m < 9; ++m) kx[m] = G[(long long)m * P];
+    double duv[12];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      duv[4 * d + v] = dmid[v];
+      duv[4 * o0 + v] = (9.0 * (X[v][k + 1] + X[v][k + 2]) + -1.0 * (X[v][k + 3] + X[v][k])) / 16.0;
+      duv[4 * o1 + v] = (9.0 * (X[4 + v][k + 1] + X[4 + v][k + 2]) + -1.0 * (X[4 + v][k + 3] + X[4 + v][k])) / 16.0;
+    }
+    // transform_derivs_to_phys (:183-205)
+    const double vinv = 1.0 / volh;
+    double gx[12];
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      const double k1 = kx[m], k2 = kx[m + 3], k3 = kx[m + 6];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) gx[v * 3 + m] = (k1 * duv[v] + k2 * duv[v + 4] + k3 * duv[v + 8]) * vinv;
+    }
+    // compute_viscous_flux_1d (:210-252)
+    const double vscc = vs1 * (2.0 / 3.0);
+    const double dudx = gx[0], dudy = gx[1], dudz = gx[2];
+    const double dvdx = gx[3], dvdy = gx[4], dvdz = gx[5];
+    const double dwdx = gx[6], dwdy = gx[7], dwdz = gx[8];
+    const double txx = vscc * (2.0 * dudx - dvdy - dwdz);
+    const double tyy = vscc * (2.0 * dvdy - dwdz - dudx);
+    const double tzz = vscc * (2.0 * dwdz - dudx - dvdy);
+    const double txy = vs1 * (dudy + dvdx);
+    const double txz = vs1 * (dudz + dwdx);
+    const double tyz = vs1 * (dvdz + dwdy);
+    const double nx = kx[3 * d], ny = kx[3 * d + 1], nz = kx[3 * d + 2];
+    const double f1 = txx * nx + txy * ny + txz * nz;
+    const double f2 = txy * nx + tyy * ny + tyz * nz;
+    const double f3 = txz * nx + tyz * ny + tzz * nz;
+    const double work = uvwt[0] * f1 + uvwt[1] * f2 + uvwt[2] * f3;
+    const double heat = vs2 * (gx[9] * nx + gx[10] * ny + gx[11] * nz);
+    double* F = b.fx + (long long)d * 5 * P + base + (long long)(f + 2) * s;
+    F[0] = 0.0;
+    F[P] = f1;
+    F[2 * P] = f2;
+    F[3 * P] = f3;
+    F[4 * P] = work + heat;
   }
-  // compute_viscous_flux_1d (:210-252)
-  const double vscc = vs1 * (2.0 / 3.0);
-  const double dudx = gx[0], dudy = gx[1], dudz = gx[2];
-  const double dvdx = gx[3], dvdy = gx[4], dvdz = gx[5];
-  const double dwdx = gx[6], dwdy = gx[7], dwdz = gx[8];
-  const double txx = vscc * (2.0 * dudx - dvdy - dwdz);
-  const double tyy = vscc * (2.0 * dvdy - dwdz - dudx);
-  const double tzz = vscc * (2.0 * dwdz - dudx - dvdy);
-  const double txy = vs1 * (dudy + dvdx);
-  const double txz = vs1 * (dudz + dwdx);
-  const double tyz = vs1 * (dvdz + dwdy);
-  const double nx = kx[3 * d], ny = kx[3 * d + 1], nz = kx[3 * d + 2];
-  const double f1 = txx * nx + txy * ny + txz * nz;
-  const double f2 = txy * nx + tyy * ny + tyz * nz;
-  const double f3 = txz * nx + tyz * ny + tzz * nz;
-  const double work = uvwt[0] * f1 + uvwt[1] * f2 + uvwt[2] * f3;
-  const double heat = vs2 * (gx[9] * nx + gx[10] * ny + gx[11] * nz);
-  double* F = b.fx + (long long)d * 5 * P + base + (long long)(f + 2) * s;
-  F[0] = 0.0;
-  F[P] = f1;
-  F[2 * P] = f2;
-  F[3 * P] = f3;
-  F[4 * P] = work + heat;
 }
 
 __global__ void __launch_bounds__(256) visc_diff_kernel(const BlockDev* __restrict__ blocks,

@@ -198,6 +198,29 @@ __device__ inline void decode_face(const BlockDev& b, int d, long long n64, long
   }
 }
 
+// Decode an index of the "face pairs along d" space (faces 2g and 2g+1,
+// g = 0..(n_d+1)/2 on every interior line, same order as the face space).
+__device__ inline void decode_face_pair(const BlockDev& b, int d, long long n64, long long* base, int* f0) {
+  const unsigned n = (unsigned)n64;
+  unsigned q, r, q2, r2;
+  if (d == 0) {
+    udivmod(n, (unsigned)(b.idim + 2) / 2u, &q, &r);
+    udivmod(q, (unsigned)b.jdim, &q2, &r2);
+    *f0 = 2 * (int)r;
+    *base = pidx(b, 0, (int)r2 + NG, (int)q2 + NG);
+  } else if (d == 1) {
+    udivmod(n, (unsigned)b.idim, &q, &r);
+    udivmod(q, (unsigned)(b.jdim + 2) / 2u, &q2, &r2);
+    *f0 = 2 * (int)r2;
+    *base = pidx(b, (int)r + NG, 0, (int)q2 + NG);
+  } else {
+    udivmod(n, (unsigned)b.idim, &q, &r);
+    udivmod(q, (unsigned)b.jdim, &q2, &r2);
+    *f0 = 2 * (int)q2;
+    *base = pidx(b, (int)r + NG, (int)r2 + NG, 0);
+  }
+}
+
 // Decode an index of the "lines along d" space (interior lines, i fastest).
 __device__ inline long long decode_line(const BlockDev& b, int d, long long n64) {
   const unsigned n = (unsigned)n64;
