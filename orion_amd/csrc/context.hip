@@ -64,6 +64,7 @@ struct orion_hip_ctx {
   size_t map_bytes = 0;
   Space pad, intr, face[3], line[3];
   Space fpair[3];                      // face pairs (2g, 2g+1) per line along d
+  Space ltask[3];                      // six boundary-closure tasks per line along d
   Space ptile;                         // LU-SGS pack tiles: 16 x 16 (march, col) cells of one row
   SweepArrays sw{};                    // LU-SGS working set in sweep layout
   std::vector<SweepTile> tiles;
@@ -709,6 +710,8 @@ int orion_hip_ctx_create(int nblocks, const int* dims, const double* xyz, int nr
     std::vector<long long> cp2(nblocks);
     for (int b = 0; b < nblocks; ++b) cp2[b] = nlines(c->hb[b], d) * ((dcount(c->hb[b], d) + 2) / 2);
     if (make_space(c, c->fpair[d], cp2)) return bail(-1);
+    for (int b = 0; b < nblocks; ++b) cp2[b] = nlines(c->hb[b], d) * 6;
+    if (make_space(c, c->ltask[d], cp2)) return bail(-1);
   }
   if (build_sweep(c)) return bail(-1);
   {
@@ -850,7 +853,7 @@ static void op_avg(orion_hip_ctx* c) {
 static void op_inv(orion_hip_ctx* c) {
   for (int d = 0; d < 3; ++d) {
     launch_inv_face(LA(c, c->face[d]), d, c->face[d].total(), c->p.gama);
-    launch_inv_closure(LA(c, c->line[d]), d, c->line[d].total(), c->p.gama);
+    launch_inv_closure(LA(c, c->ltask[d]), d, c->ltask[d].total(), c->p.gama);
   }
   launch_inv_diff(LA(c, c->intr), c->intr.total());
 }
@@ -1069,7 +1072,7 @@ int orion_hip_profile_run(orion_hip_ctx* c, int nsteps, double* phase_ms, long l
     }
     for (int d = 0; d < 3; ++d) launch_inv_face(LA(c, c->face[d]), d, c->face[d].total(), c->p.gama);
     mark(3);
-    for (int d = 0; d < 3; ++d) launch_inv_closure(LA(c, c->line[d]), d, c->line[d].total(), c->p.gama);
+    for (int d = 0; d < 3; ++d) launch_inv_closure(LA(c, c->ltask[d]), d, c->ltask[d].total(), c->p.gama);
     mark(3);
     launch_inv_diff(LA(c, c->intr), c->intr.total()); mark(1);
     op_avg(c); mark(2);

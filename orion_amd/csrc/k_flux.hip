@@ -237,7 +237,15 @@ __global__ void __launch_bounds__(256) inv_face_kernel(const BlockDev* __restric
   for (int m = 0; m < 5; ++m) F[(long long)m * b.P] = fo[m];
 }
 
-// Boundary recalculation (InviscidFluxComputer.cpp:377-521) for one line.
+// Boundary recalculation (InviscidFluxComputer.cpp:377-521) at a line end
+// whose boundary signal is set.  Each end has three stages; each sets the
+// states of a few faces from fixed boundary stencils (the third also takes
+// the WCNS states of the end faces), recomputes their fluxes and combines
+// them into one closure value.  The stages only read line values and face
+// states, never each other's fluxes, so the six (end, stage) tasks of a
+// line run on six threads.  The right end reads face states as the left
+// end's stages left them where the two ends' faces overlap (lines of at
+// most 9 cells), as the reference, which treats the left end first, does.
 __global__ void __launch_bounds__(256) inv_closure_kernel(const BlockDev* __restrict__ blocks,
                                                          Span SP, int d,
                                                          double gamma) {
@@ -245,127 +253,120 @@ __global__ void __launch_bounds__(256) inv_closure_kernel(const BlockDev* __rest
   long long gi;
   if (!span_item(SP, &bi, &gi)) return;
   const BlockDev& b = blocks[bi];
-  const long long ln = gi;
+  // task slowest: a wavefront's lanes are lines of one task (no divergence)
+  unsigned ln, task;
+  udivmod((unsigned)gi, (unsigned)nlines(b, d), &task, &ln);
   const long long base = decode_line(b, d, ln);
   const int n = dcount(b, d);
   const long long s = dstride(b, d);
   const LineRef L{b.prim, b.P, base, s};
   const int ist = (L.at(0, 0) < 1.0e-20) ? 2 : 0;
   const int ied = (L.at(0, n + 5) < 1.0e-20) ? n - 2 : n;
-  if (ist <= 1 && ied >= n) return;
-  // private window over faces [0..5] U [n-5..n]; slot(f) is unique per face
-  constexpr int W = 12;
-  double QL[W][5], QR[W][5], FF[W][5];
-  auto slot = [n](int f) { return (n <= 11) ? f : (f <= 5 ? f : f - (n - 11)); };
-  double k3[W][3];
-  for (int f = 0; f <= n; ++f) {
-    const bool need = ((ist > 1) && f <= 5) || ((ied < n) && f >= n - 5);
-    if (!need) continue;
-    const int sl = slot(f);
-    face_states(L, n, ist, ied, f, QL[sl], QR[sl]);
+  const bool right = task >= 3;
+  if (right ? !(ied < n) : !(ist > 1)) return;
+  const int stage = (int)task % 3;
+  auto flux_at = [&](int f, const double* ql, const double* qr, double* fo) {
     const double* G = b.fgeo + (long long)(d * FG_N + 10) * b.P + base + (long long)(f + 2) * s;
-    k3[sl][0] = G[0];
-    k3[sl][1] = G[b.P];
-    k3[sl][2] = G[2 * b.P];
-  }
-  auto flux = [&](int st, int en) {
-    for (int f = st; f <= en; ++f) face_flux(L, n, f, QL[slot(f)], QR[slot(f)], k3[slot(f)], gamma, FF[slot(f)]);
+    const double k3[3] = {G[0], G[b.P], G[2 * b.P]};
+    face_flux(L, n, f, ql, qr, k3, gamma, fo);
   };
-  auto F = [&](int f, int m) { return FF[slot(f)][m]; };
-  double* out = b.fcov + b.fcov_off[d] + ln * 30;
-  if (ist > 1) {
-    double uro[5][5];
-    for (int i = 1; i <= 4; ++i)
-      for (int m = 0; m < 5; ++m) uro[i][m] = QR[slot(i)][m];
-    for (int m = 0; m < 5; ++m) {
-      const double a = L.at(m, 3), bb = L.at(m, 4), cc = L.at(m, 5), dd = L.at(m, 6);
-      QL[slot(0)][m] = (3. * a - 1. * bb) / 2.0;
-      QR[slot(0)][m] = QL[slot(0)][m];
-      QL[slot(1)][m] = (19. * a + 25. * bb - 2. * cc) / 42.0;
-      QR[slot(1)][m] = QL[slot(1)][m];
-      QL[slot(2)][m] = (4. * a + 3. * bb + 9. * cc + 2. * dd) / 18.0;
-      QR[slot(2)][m] = QL[slot(2)][m];
-      QL[slot(3)][m] = (-2. * a + 3. * bb + 3. * cc + 2. * dd) / 6.0;
-      QR[slot(3)][m] = QL[slot(3)][m];
-    }
-    flux(0, 3);
-    for (int m = 0; m < 5; ++m) out[0 * 5 + m] = (-23. * F(0, m) + 21. * F(1, m) + 3. * F(2, m) - F(3, m)) / 24.0;
-    for (int m = 0; m < 5; ++m) {
-      const double a = L.at(m, 3), bb = L.at(m, 4), cc = L.at(m, 5), dd = L.at(m, 6);
-      QL[slot(1)][m] = (3. * a + 6. * bb - cc) / 8.0;
-      QR[slot(1)][m] = QL[slot(1)][m];
-      QL[slot(2)][m] = (-30. * a + 145. * bb + 29. * cc - 8. * dd) / 136.0;
-      QR[slot(2)][m] = (2. * a + 49. * bb + 125. * cc - 40. * dd) / 136.0;
-      QL[slot(3)][m] = (-bb + 6. * cc + 3. * dd) / 8.0;
-      QR[slot(3)][m] = QL[slot(3)][m];
-      QL[slot(4)][m] = (bb + 2. * cc + 5. * dd) / 8.0;
-      QR[slot(4)][m] = QL[slot(4)][m];
-      QL[slot(5)][m] = (bb + cc + 6. * dd) / 8.0;
-      QR[slot(5)][m] = QL[slot(5)][m];
-    }
-    flux(1, 5);
-    for (int m = 0; m < 5; ++m)
-      out[1 * 5 + m] = (-22. * F(1, m) + 17. * F(2, m) + 9. * F(3, m) - 5. * F(4, m) + F(5, m)) / 24.0;
-    for (int m = 0; m < 5; ++m) {
-      const double a = L.at(m, 3), bb = L.at(m, 4), cc = L.at(m, 5), dd = L.at(m, 6);
-      QR[slot(1)][m] = uro[1][m];
-      QL[slot(2)][m] = (-29. * a + 170. * bb + 63. * cc + 12. * dd) / 216.0;
-      QR[slot(2)][m] = uro[2][m];
-      QR[slot(3)][m] = uro[3][m];
-      QR[slot(4)][m] = uro[4][m];
-    }
-    flux(1, 4);
-    for (int m = 0; m < 5; ++m) out[2 * 5 + m] = (F(1, m) - 27. * F(2, m) + 27. * F(3, m) - F(4, m)) / 24.0;
+  // the four line values next to this end, outermost first
+  double A[5], B[5], C[5], D[5];
+#pragma unroll
+  for (int m = 0; m < 5; ++m) {
+    A[m] = L.at(m, right ? n + 2 : 3);
+    B[m] = L.at(m, right ? n + 1 : 4);
+    C[m] = L.at(m, right ? n : 5);
+    D[m] = L.at(m, right ? n - 1 : 6);
   }
-  if (ied < n) {
-    const int N = n;
-    double ulo[9][5];
-    for (int i = 1; i <= 4; ++i)
-      for (int m = 0; m < 5; ++m) ulo[9 - i][m] = QL[slot(N - i)][m];
+  // face k of the stage (k = 0 the outermost face it touches)
+  auto face = [&](int k, int first) { return right ? n - first - k : first + k; };
+  double o[5];
+  if (stage == 0) {  // faces 0..3
+    double q[4][5], F[4][5];
+#pragma unroll
     for (int m = 0; m < 5; ++m) {
-      const double a = L.at(m, N + 2), bb = L.at(m, N + 1), cc = L.at(m, N), dd = L.at(m, N - 1);
-      QR[slot(N)][m] = (3. * a - 1. * bb) / 2.0;
-      QL[slot(N)][m] = QR[slot(N)][m];
-      QR[slot(N - 1)][m] = (19. * a + 25. * bb - 2. * cc) / 42.0;
-      QL[slot(N - 1)][m] = QR[slot(N - 1)][m];
-      QR[slot(N - 2)][m] = (4. * a + 3. * bb + 9. * cc + 2. * dd) / 18.0;
-      QL[slot(N - 2)][m] = QR[slot(N - 2)][m];
-      QR[slot(N - 3)][m] = (-2. * a + 3. * bb + 3. * cc + 2. * dd) / 6.0;
-      QL[slot(N - 3)][m] = QR[slot(N - 3)][m];
+      q[0][m] = (3. * A[m] - 1. * B[m]) / 2.0;
+      q[1][m] = (19. * A[m] + 25. * B[m] - 2. * C[m]) / 42.0;
+      q[2][m] = (4. * A[m] + 3. * B[m] + 9. * C[m] + 2. * D[m]) / 18.0;
+      q[3][m] = (-2. * A[m] + 3. * B[m] + 3. * C[m] + 2. * D[m]) / 6.0;
     }
-    flux(N - 3, N);
-    for (int m = 0; m < 5; ++m)
-      out[5 * 5 + m] = -(-23. * F(N, m) + 21. * F(N - 1, m) + 3. * F(N - 2, m) - F(N - 3, m)) / 24.0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) flux_at(face(k, 0), q[k], q[k], F[k]);
+#pragma unroll
     for (int m = 0; m < 5; ++m) {
-      const double a = L.at(m, N + 2), bb = L.at(m, N + 1), cc = L.at(m, N), dd = L.at(m, N - 1);
-      QR[slot(N - 1)][m] = (3. * a + 6. * bb - cc) / 8.0;
-      QL[slot(N - 1)][m] = QR[slot(N - 1)][m];
-      QR[slot(N - 2)][m] = (-30. * a + 145. * bb + 29. * cc - 8. * dd) / 136.0;
-      QL[slot(N - 2)][m] = (2. * a + 49. * bb + 125. * cc - 40. * dd) / 136.0;
-      QR[slot(N - 3)][m] = (-bb + 6. * cc + 3. * dd) / 8.0;
-      QL[slot(N - 3)][m] = QR[slot(N - 3)][m];
-      QR[slot(N - 4)][m] = (bb + 2. * cc + 5. * dd) / 8.0;
-      QL[slot(N - 4)][m] = QR[slot(N - 4)][m];
-      QR[slot(N - 5)][m] = (bb + cc + 6. * dd) / 8.0;
-      QL[slot(N - 5)][m] = QR[slot(N - 5)][m];
+      if (right) o[m] = -(-23. * F[0][m] + 21. * F[1][m] + 3. * F[2][m] - F[3][m]) / 24.0;
+      else o[m] = (-23. * F[0][m] + 21. * F[1][m] + 3. * F[2][m] - F[3][m]) / 24.0;
     }
-    flux(N - 5, N - 1);
-    for (int m = 0; m < 5; ++m)
-      out[4 * 5 + m] =
-          -(-22. * F(N - 1, m) + 17. * F(N - 2, m) + 9. * F(N - 3, m) - 5. * F(N - 4, m) + F(N - 5, m)) / 24.0;
+  } else if (stage == 1) {  // faces 1..5
+    double q[5][5], q2[5], F[5][5];
+#pragma unroll
     for (int m = 0; m < 5; ++m) {
-      const double a = L.at(m, N + 2), bb = L.at(m, N + 1), cc = L.at(m, N), dd = L.at(m, N - 1);
-      QL[slot(N - 1)][m] = ulo[8][m];
-      QR[slot(N - 2)][m] = (-29. * a + 170. * bb + 63. * cc + 12. * dd) / 216.0;
-      QL[slot(N - 2)][m] = ulo[7][m];
-      QL[slot(N - 3)][m] = ulo[6][m];
-      QL[slot(N - 4)][m] = ulo[5][m];
+      q[0][m] = (3. * A[m] + 6. * B[m] - C[m]) / 8.0;
+      q[1][m] = (-30. * A[m] + 145. * B[m] + 29. * C[m] - 8. * D[m]) / 136.0;  // the end-side state of face 2
+      q2[m] = (2. * A[m] + 49. * B[m] + 125. * C[m] - 40. * D[m]) / 136.0;     // its other state
+      q[2][m] = (-B[m] + 6. * C[m] + 3. * D[m]) / 8.0;
+      q[3][m] = (B[m] + 2. * C[m] + 5. * D[m]) / 8.0;
+      q[4][m] = (B[m] + C[m] + 6. * D[m]) / 8.0;
     }
-    flux(N - 4, N - 1);
-    for (int m = 0; m < 5; ++m)
-      out[3 * 5 + m] = -(F(N - 1, m) - 27. * F(N - 2, m) + 27. * F(N - 3, m) - F(N - 4, m)) / 24.0;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      if (k == 1) {
+        if (right) flux_at(face(k, 1), q2, q[1], F[k]);
+        else flux_at(face(k, 1), q[1], q2, F[k]);
+      } else {
+        flux_at(face(k, 1), q[k], q[k], F[k]);
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < 5; ++m) {
+      if (right) o[m] = -(-22. * F[0][m] + 17. * F[1][m] + 9. * F[2][m] - 5. * F[3][m] + F[4][m]) / 24.0;
+      else o[m] = (-22. * F[0][m] + 17. * F[1][m] + 9. * F[2][m] - 5. * F[3][m] + F[4][m]) / 24.0;
+    }
+  } else {  // faces 1..4: end-side states from the stencils, the others as before the end's first stage
+    double qe[4][5], qo[4][5], F[4][5];
+#pragma unroll
+    for (int m = 0; m < 5; ++m) {
+      qe[0][m] = (3. * A[m] + 6. * B[m] - C[m]) / 8.0;
+      qe[1][m] = (-29. * A[m] + 170. * B[m] + 63. * C[m] + 12. * D[m]) / 216.0;
+      qe[2][m] = (-B[m] + 6. * C[m] + 3. * D[m]) / 8.0;
+      qe[3][m] = (B[m] + 2. * C[m] + 5. * D[m]) / 8.0;
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int f = face(k, 1);
+      double ql[5], qr[5];
+      face_states(L, n, ist, ied, f, ql, qr);
+#pragma unroll
+      for (int m = 0; m < 5; ++m) qo[k][m] = right ? ql[m] : qr[m];
+      if (right && ist > 1 && f <= 5) {  // a face the left end's stages set last
+#pragma unroll
+        for (int m = 0; m < 5; ++m) {
+          const double a = L.at(m, 3), bb = L.at(m, 4), cc = L.at(m, 5), dd = L.at(m, 6);
+          double v;
+          if (f == 0) v = (3. * a - 1. * bb) / 2.0;
+          else if (f == 1) v = (3. * a + 6. * bb - cc) / 8.0;
+          else if (f == 2) v = (-29. * a + 170. * bb + 63. * cc + 12. * dd) / 216.0;
+          else if (f == 3) v = (-bb + 6. * cc + 3. * dd) / 8.0;
+          else if (f == 4) v = (bb + 2. * cc + 5. * dd) / 8.0;
+          else v = (bb + cc + 6. * dd) / 8.0;
+          qo[k][m] = v;
+        }
+      }
+      if (right) flux_at(f, qo[k], qe[k], F[k]);
+      else flux_at(f, qe[k], qo[k], F[k]);
+    }
+#pragma unroll
+    for (int m = 0; m < 5; ++m) {
+      if (right) o[m] = -(F[0][m] - 27. * F[1][m] + 27. * F[2][m] - F[3][m]) / 24.0;
+      else o[m] = (F[0][m] - 27. * F[1][m] + 27. * F[2][m] - F[3][m]) / 24.0;
+    }
   }
+  double* out = b.fcov + b.fcov_off[d] + (long long)ln * 30 + (right ? 5 - stage : stage) * 5;
+#pragma unroll
+  for (int m = 0; m < 5; ++m) out[m] = o[m];
 }
+
 
 // 6th-order difference with 3rd/4th-order ends
 // (compute_flux_diff_high_order :197-233 == compute_flux_diff_1d :257-296)

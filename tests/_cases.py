@@ -52,6 +52,17 @@ def isothermal_wall(nsteps=3):
     return c
 
 
+def short_duct(nsteps=3):
+    """Viscous walls at both j ends of a 9-point j extent, far-field at the i
+    ends, periodic in k: the boundary closures of a line's two ends share
+    faces (the right end reads states the left end's stages set)."""
+    ni, nj, nk = 15, 9, 7
+    c = cases.channel_wall_case(seed=6, nsteps=nsteps, dims=(ni, nj, nk))
+    c.regions[3] = cases.Region(0, 2, (1, nj, 1), (ni, nj, nk))
+    c.name = "short_duct"
+    return c
+
+
 ALL = {
     "channel_small": channel_small,
     "lattice_small": lattice_small,
@@ -59,4 +70,5 @@ ALL = {
     "bc_variety_visc": lambda nsteps=3: bc_variety(1, nsteps),
     "bc_variety_inviscid": lambda nsteps=3: bc_variety(0, nsteps),
     "isothermal_wall": isothermal_wall,
+    "short_duct": short_duct,
 }
