@@ -14,14 +14,15 @@
 //    256 lanes owns a tile; lane (row, col) marches along its pencil, and at
 //    pipeline step s processes cell m = s - row - col, so all lanes of a tile
 //    are on hyperplane s together.
-//  * Inside a tile, neighbours along col and row are lanes of the same
-//    workgroup and are exchanged through LDS (two barriers per step); along
-//    the march axis they are the lane's own previous / next cell (registers).
+//  * Inside a tile, the neighbours before a cell along col and row are lanes
+//    of the same workgroup and come through LDS (buffers alternating by step
+//    parity, one barrier per step); along the march axis it is the lane's
+//    own previous cell.
 //  * Across tiles, a tile publishes how far it has come in a progress flag
 //    and its downstream tiles wait on it (bounded spin) before reading edge
-//    values.  dq is stored with agent-scope atomic stores and each wave
-//    waits for their completion before the barrier that precedes the flag
-//    store; edge dq is read with agent-scope atomic loads.  No cache
+//    values.  dq is stored with agent-scope atomic stores, plane s is
+//    published after every wave has waited for its stores of that plane,
+//    and edge dq is read with agent-scope atomic loads.  No cache
 //    maintenance is needed (fence-based release/acquire around plain stores
 //    flushes and invalidates L2 and measured 2-8x slower).
 //    Tiles are launched in dependency order and a wait only ever targets a
@@ -35,6 +36,8 @@
 // packed values, with the reference's expressions (calc_spectral_radius
 // :175-191, calc_visc_spectral_radius :194-207, compute_flux_splitting
 // :209-264, get_diag_coeff :267-295, sweep bodies :318-365, :392-441).
+// The Gauss-Seidel sweeps take their after-neighbour terms from the sweep
+// before them (see lusgs_tile_kernel).
 #include "orion_dev.h"
 #include "orion_kernels.h"
 
