@@ -38,6 +38,8 @@
 // :209-264, get_diag_coeff :267-295, sweep bodies :318-365, :392-441).
 // The Gauss-Seidel sweeps take their after-neighbour terms from the sweep
 // before them (see lusgs_tile_kernel).
+#include <cstdlib>
+
 #include "orion_dev.h"
 #include "orion_kernels.h"
 
@@ -789,6 +791,377 @@ __global__ void __launch_bounds__(TL, 1) lusgs_tile_kernel(SweepArrays S, const 
                        __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// ---------------------------------------------------------------------------
+// tile sweep with loader waves
+// ---------------------------------------------------------------------------
+// The pipeline of lusgs_tile_kernel with the memory traffic taken off the
+// update waves.  A tile's workgroup has 256 update lanes and 256 loader
+// lanes; lane l of either kind works on pencil l.  Step t:
+//  * update lanes: the cell's inputs, the neighbours' (D, dq) and the
+//    upstream edge values come from LDS; the new dq goes to LDS;
+//  * loader lanes: put the inputs of step t+1 (fetched a step earlier) into
+//    LDS, store the dq of step t-1 (agent scope), fetch the inputs of step
+//    t+2; the first loader wave also checks the upstream progress, fetches
+//    the edge values and reads the upstream flags for the next check, and
+//    its lane 0 publishes;
+//  * one barrier ends the step.
+// Plane s(t) is published at the start of step t+3: each loader wave's
+// stores of it precede the fetch it waits for in step t+2 (the counter is
+// in order), and a barrier follows.  LDS: own inputs by
+// step parity, D by step mod 3 (a step's update reads the D of the step
+// before), dq by parity, edge values by parity.
+template <int TYPE, bool VIS>
+__global__ void __launch_bounds__(2 * TL, 1) lusgs_pc_kernel(SweepArrays S, const BlockDev* __restrict__ blocks,
+                                                            const SweepTile* __restrict__ tiles,
+                                                            const int* __restrict__ order, unsigned long long* flags,
+                                                            const unsigned long long* __restrict__ epoch, int* err,
+                                                            double gm1) {
+  constexpr bool FWD = (TYPE == 0 || TYPE == 2);
+  constexpr int dir = FWD ? 1 : -1;
+  constexpr int SL0 = FWD ? 0 : 3;
+  constexpr int NO = TYPE == 3 ? 20 : 15;                     // own input words
+  __shared__ double ownB[2][NO][TL];
+  __shared__ double Dbuf[3][8][TL];
+  __shared__ double dqB[2][5][TL];
+  __shared__ double shX[2][2][NW][TW];
+
+  const int tix = order[blockIdx.x];
+  const SweepTile T = tiles[tix];
+  const BlockDev& b = blocks[T.block];
+  const int nm = b.sn[0], nc = b.sn[1], nr = b.sn[2];
+  const int am = b.sax[0], ac = b.sax[1];
+  const bool loader = threadIdx.x >= TL;
+  const bool lw0 = loader && threadIdx.x < TL + 64;  // first loader wave
+  const int lane = threadIdx.x & (TL - 1), lc = lane % TW, lr = lane / TW;
+  const int xl = threadIdx.x & 63;
+  const int c = T.c0 + lc, r = T.r0 + lr;
+  const bool lane_ok = c < nc && r < nr;
+  const int up_c = FWD ? T.nbr[0] : T.nbr[2], up_r = FWD ? T.nbr[1] : T.nbr[3];
+  const unsigned long long ep = (*epoch) << KEY_BITS;
+  SweepTile ut[2];
+  ut[0] = up_c >= 0 ? tiles[up_c] : T;
+  ut[1] = up_r >= 0 ? tiles[up_r] : T;
+  const int nsteps = T.s_last - T.s_first + 1;
+  const int s0 = FWD ? T.s_first : T.s_last;
+  const int ncp = b.sncp;
+  const long long qstep = dir * (long long)nr * ncp;
+  const int m0 = s0 - r - c;
+  const long long rb0 = b.sbase + ((long long)s0 * nr + r) * ncp;
+  auto cell_ok = [&](int t) {
+    const int m = m0 + dir * t;
+    return lane_ok && t >= 0 && t < nsteps && m >= 0 && m < nm;
+  };
+  auto cell_rb = [&](int t) { return cell_ok(t) ? rb0 + (long long)t * qstep : b.sbase; };
+  auto cell_c = [&](int t) { return cell_ok(t) ? c : 0; };
+
+  // edge items of the first loader wave: lane x loads D items x, x + 64 and
+  // dq items x, x + 64, x + 128 (< 160); item e is (role, index, part),
+  // role 0 the cells across the upstream col edge, role 1 the row edge
+  int er[5], ec[5], xo[5];
+  bool ein[5];
+#pragma unroll
+  for (int j = 0; j < 5; ++j) {
+    int role, idx, word;
+    bool item = true;
+    if (j < 2) {
+      const int e = xl + 64 * j;
+      role = e >> 6;
+      idx = (e >> 2) & 15;
+      word = 2 * (e & 3);
+    } else {
+      const int e = xl + 64 * (j - 2);
+      item = e < 160;
+      role = e < 80 ? 0 : 1;
+      idx = (e % 80) / 5;
+      word = 8 + e % 5;
+    }
+    if (role == 0) {
+      ec[j] = FWD ? T.c0 - 1 : T.c0 + TW;
+      er[j] = T.r0 + idx;
+    } else {
+      er[j] = FWD ? T.r0 - 1 : T.r0 + TW;
+      ec[j] = T.c0 + idx;
+    }
+    ein[j] = lw0 && item && er[j] >= 0 && er[j] < nr && ec[j] >= 0 && ec[j] < nc;
+    xo[j] = item ? (role * NW + word) * TW + idx : -1;
+  }
+  auto edge_rb = [&](int j, int u, int* cc) -> long long {
+    const int sp = s0 + dir * (u - 1);
+    const int mm = sp - er[j] - ec[j];
+    const bool ok = ein[j] && u < nsteps && mm >= 0 && mm < nm;
+    *cc = ok ? ec[j] : 0;
+    return ok ? b.sbase + ((long long)sp * nr + er[j]) * ncp : b.sbase;
+  };
+
+  // loader registers: the inputs of a later step and the flags read for
+  // the next progress check
+  struct Ring {
+    double o[NO];
+    double D[8];
+    double2 xd[2];
+    double xq[3];
+    unsigned long long pf[2];
+  };
+  auto fetch = [&](int u, Ring& R) {
+    load_own<TYPE>(S, cell_rb(u), cell_c(u), ncp, R.o);
+#pragma unroll
+    for (int k = 0; k < SW_D; ++k) {
+      const double2 v = ld_pair(S.D, cell_rb(u), cell_c(u), ncp, SW_D, k);
+      R.D[2 * k] = v.x;
+      R.D[2 * k + 1] = v.y;
+    }
+    if (lw0) {
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        int cc;
+        const long long rb = edge_rb(j, u, &cc);
+        R.xd[j] = ld_pair(S.D, rb, cc, ncp, SW_D, xl & 3);
+      }
+      const double* dqp = reinterpret_cast<const double*>(S.dq);
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        int cc;
+        const long long rb = edge_rb(2 + j, u, &cc);
+        R.xq[j] = __hip_atomic_load(&dqp[sweep_field(rb + cc, cc, ncp, SW_DQ, (xl + 64 * j) % 5)], __ATOMIC_RELAXED,
+                                    __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
+  };
+  auto deposit = [&](int u, const Ring& R) {
+#pragma unroll
+    for (int f = 0; f < NO; ++f) ownB[u & 1][f][lane] = R.o[f];
+#pragma unroll
+    for (int f = 0; f < 8; ++f) Dbuf[u % 3][f][lane] = R.D[f];
+    if (lw0) {
+      double* x = &shX[u & 1][0][0][0];
+      x[xo[0]] = R.xd[0].x;
+      x[xo[0] + TW] = R.xd[0].y;
+      x[xo[1]] = R.xd[1].x;
+      x[xo[1] + TW] = R.xd[1].y;
+      x[xo[2]] = R.xq[0];
+      x[xo[3]] = R.xq[1];
+      if (xo[4] >= 0) x[xo[4]] = R.xq[2];
+    }
+  };
+  // loader lane 0: the flags for the next check, always two loads (a
+  // missing upstream reads this tile's own flag and is never checked)
+  auto read_up = [&](unsigned long long* pf) {
+    if (!(lw0 && xl == 0)) return;
+    pf[0] = __hip_atomic_load(&flags[(long long)(up_c >= 0 ? up_c : tix) * FLAG_STRIDE], __ATOMIC_RELAXED,
+                              __HIP_MEMORY_SCOPE_AGENT);
+    pf[1] = __hip_atomic_load(&flags[(long long)(up_r >= 0 ? up_r : tix) * FLAG_STRIDE], __ATOMIC_RELAXED,
+                              __HIP_MEMORY_SCOPE_AGENT);
+  };
+  // loader lane 0: the upstream tiles must have finished plane s; pf holds
+  // flag values read a step earlier, re-read (bounded spin) only when short
+  auto ensure_up = [&](int s, const unsigned long long* pf) {
+    if (!(lw0 && xl == 0)) return;
+    for (int k = 0; k < 2; ++k) {
+      const int u = k ? up_r : up_c;
+      if (u < 0) continue;
+      const unsigned long long key = need_key(FWD, ut[k], s);
+      if (key == 0) continue;
+      const unsigned long long want = ep + key;
+      unsigned long long v = pf[k];
+      long long spins = 0;
+      while (v < want) {
+        // far beyond any legal wait (a whole sweep takes milliseconds), or
+        // another tile already gave up: report and carry on, never hang
+        if (++spins > (1LL << 20) ||
+            ((spins & 255) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)) {
+          atomicOr(err, 1);
+          break;
+        }
+        if (spins > 1) __builtin_amdgcn_s_sleep(2);
+        v = __hip_atomic_load(&flags[(long long)u * FLAG_STRIDE], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
+  };
+
+  // before-neighbour of the lane's cell along real axis ax at step t: its
+  // (D, dq) and whether it exists
+  auto nbr = [&](int ax, int m, int t, double* v) -> bool {
+    constexpr int d = -dir;
+    const int p = t & 1;
+    const bool march = (ax == am);
+    const bool col = (ax == ac);
+    const int pos = col ? lc : lr, lim = col ? nc : nr, coord = col ? c : r;
+    const bool inside = march || (pos + d >= 0 && pos + d < TW);
+    const int ln = march ? lane : lane + (col ? d : d * TW);
+    const int role = col ? 0 : 1, idx = col ? lr : lc;
+    const double* bd = inside ? &Dbuf[(t + 2) % 3][0][ln] : &shX[p][role][0][idx];
+    const double* bq = inside ? &dqB[p ^ 1][0][ln] : &shX[p][role][8][idx];
+    const int st = inside ? TL : TW;
+#pragma unroll
+    for (int f = 0; f < 8; ++f) v[f] = bd[f * st];
+#pragma unroll
+    for (int f = 0; f < 5; ++f) v[8 + f] = bq[f * st];
+    return march ? (m + d >= 0 && m + d < nm) : (coord + d >= 0 && coord + d < lim);
+  };
+
+  auto load_ga = [&](int u, double* g) {
+    if constexpr (TYPE == 2) {
+      const long long rb = cell_rb(u);
+      const int cc = cell_c(u);
+#pragma unroll
+      for (int k = 0; k < 7; ++k) {
+        const double2 v = ld_pair(S.ga, rb, cc, ncp, SW_GA, k);
+        g[2 * k] = v.x;
+        g[2 * k + 1] = v.y;
+      }
+      g[14] = ld_half(S.ga, rb, cc, ncp, SW_GA, 7);
+    }
+  };
+
+  // one step of a loader lane
+  auto lstep = [&](int t, Ring& Rcur, Ring& Rprev) __attribute__((always_inline)) {
+    const int s = s0 + dir * t;
+    // plane s(t-3): every loader wave's stores of it preceded the fetch that
+    // the wave put into LDS in step t-1, and a barrier followed
+    if (lw0 && xl == 0 && t >= 3 && t - 3 < nsteps)
+      __hip_atomic_store(&flags[(long long)tix * FLAG_STRIDE], ep + plane_key(FWD, s - 3 * dir), __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+    deposit(t + 1, Rprev);
+    // the fetch just put into LDS has arrived, and with it (in-order
+    // counter) this wave's stores of step t-2, issued before it; only the
+    // flag loads issued after it may be outstanding
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_sched_barrier(0);
+    if (lw0)
+      __builtin_amdgcn_s_waitcnt(waitcnt_vm(2));
+    else
+      __builtin_amdgcn_s_waitcnt(waitcnt_vm(0));
+    __builtin_amdgcn_sched_barrier(0);
+    if (t + 2 < nsteps) ensure_up(s + dir, Rprev.pf);  // the edge values of step t+2 lie on plane s(t+1)
+    if (cell_ok(t - 1)) {
+      const long long q = rb0 + (long long)(t - 1) * qstep + c;
+      double* dqp = reinterpret_cast<double*>(S.dq);
+#pragma unroll
+      for (int f = 0; f < 5; ++f)
+        __hip_atomic_store(&dqp[sweep_field(q, c, ncp, SW_DQ, f)], dqB[(t - 1) & 1][f][lane], __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    }
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_sched_barrier(0);
+    fetch(t + 2, Rcur);
+    read_up(Rcur.pf);
+    __syncthreads();
+  };
+
+  // one step of an update lane
+  auto ustep = [&](int t, double* gcur, double* gnext) __attribute__((always_inline)) {
+    if (t < nsteps) {
+      const int m = m0 + dir * t;
+      const bool act = cell_ok(t);
+      double o[NO];
+#pragma unroll
+      for (int f = 0; f < NO; ++f) o[f] = ownB[t & 1][f][lane];
+      double r0[5];
+#pragma unroll
+      for (int f = 0; f < 5; ++f) r0[f] = TYPE == 3 ? o[15 + f] : 0.0;
+      double gout[15];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        AxisGeo ag;
+        axis_geo(o + 3 * k, ag);
+        double v[NW];
+        const bool ok = nbr(k, m, t, v);
+        double inv[5], tv[5];
+        slot_inv<SL0>(o + 3 * k, ag, v, v + 8, gm1, inv);
+        if (VIS) {
+          const double hv = 0.5 * (v[7] * ag.g2);
+#pragma unroll
+          for (int f = 0; f < 5; ++f) tv[f] = hv * v[8 + f];
+        }
+#pragma unroll
+        for (int f = 0; f < 5; ++f) {
+          if (TYPE == 3) {  // Gauss-Seidel term of a +1 neighbour: r - (inviscid - viscous)
+            const double res = VIS ? inv[f] + tv[f] * -1.0 : inv[f];
+            const double acc = r0[f] - res;
+            r0[f] = ok ? acc : r0[f];
+          } else {  // LU terms, and Gauss-Seidel terms of -1 neighbours: r + (inviscid + viscous)
+            const double res = VIS ? inv[f] + tv[f] : inv[f];
+            const double acc = r0[f] + res;
+            r0[f] = ok ? acc : r0[f];
+          }
+          if (TYPE == 1) {  // the Gauss-Seidel forward sweep's term for the same pair
+            const double g = VIS ? inv[f] + tv[f] * -1.0 : inv[f];
+            gout[5 * k + f] = ok ? g : 0.0;
+          }
+        }
+      }
+      double pre5[5];
+      if (TYPE == 2) {
+#pragma unroll
+        for (int f = 0; f < 5; ++f) pre5[f] = r0[f];
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+#pragma unroll
+          for (int f = 0; f < 5; ++f) r0[f] = r0[f] - gcur[5 * k + f];
+      }
+      const double wmig = 1.0;
+      const double* r5 = o + 10;
+      const double coed = o[9];
+#pragma unroll
+      for (int f = 0; f < 5; ++f) {
+        double x;
+        if (TYPE == 0) x = (-r5[f] + wmig * r0[f]) * coed;
+        else if (TYPE == 1) x = r5[f] - (wmig * r0[f]) * coed;
+        else x = (wmig * r0[f] + -r5[f]) * coed;
+        dqB[t & 1][f][lane] = act ? x : 0.0;
+      }
+      if (act && (TYPE == 1 || TYPE == 2)) {
+        const long long rb = rb0 + (long long)t * qstep;
+        if (TYPE == 1) {
+          double2* gp = S.ga + rb * SW_GA + c;
+#pragma unroll
+          for (int k = 0; k < SW_GA; ++k)
+            gp[(long long)k * ncp] = make_double2(gout[2 * k], 2 * k + 1 < 15 ? gout[2 * k + 1] : 0.0);
+        } else {
+          double2* pp = S.pre + rb * SW_PRE + c;
+          pp[0] = make_double2(pre5[0], pre5[1]);
+          pp[ncp] = make_double2(pre5[2], pre5[3]);
+          pp[2 * ncp] = make_double2(pre5[4], 0.0);
+        }
+      }
+      load_ga(t + 1, gnext);
+    }
+    __syncthreads();
+  };
+
+  // Both kinds of lanes run their own loop with one barrier per step (and
+  // one after the set-up), so every barrier is reached by all waves.
+  const int TT = (nsteps + 4) & ~1;  // whole pairs of steps, at least nsteps + 3
+  if (loader) {
+    Ring RA, RB;
+    {
+      const unsigned long long none[2] = {0, 0};
+      ensure_up(s0, none);  // covers the edge values of steps 0 and 1
+    }
+    read_up(RB.pf);
+    fetch(0, RA);
+    deposit(0, RA);
+    fetch(1, RB);
+    __syncthreads();
+    for (int t = 0; t < TT; t += 2) {
+      lstep(t, RA, RB);
+      lstep(t + 1, RB, RA);
+    }
+  } else {
+    double gaR[2][TYPE == 2 ? 15 : 1];  // Gauss-Seidel forward: ga by step parity
+#pragma unroll
+    for (int f = 0; f < 5; ++f) dqB[1][f][lane] = 0.0;
+#pragma unroll
+    for (int f = 0; f < 8; ++f) Dbuf[2][f][lane] = 0.0;
+    load_ga(0, gaR[0]);
+    __syncthreads();
+    for (int t = 0; t < TT; t += 2) {
+      ustep(t, gaR[0], gaR[1]);
+      ustep(t + 1, gaR[1], gaR[0]);
+    }
+  }
+}
+
 __global__ void epoch_kernel(unsigned long long* epoch) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *epoch += 1;
 }
@@ -817,6 +1190,21 @@ void launch_lusgs_tiles(const SweepArrays& S, const BlockDev* blocks, const Swee
   const double gm1 = p.gama - 1.0;
   const dim3 gr(ntiles), tb(TL);
   const bool v = p.nvis == 1;
+  static const int pc = [] { const char* e = std::getenv("ORION_LU_PC"); return e ? std::atoi(e) : 1; }();
+  if (pc) {
+    const dim3 tb2(2 * TL);
+    switch (type * 2 + (v ? 1 : 0)) {
+      case 0: hipLaunchKernelGGL((lusgs_pc_kernel<0, false>), gr, tb2, 0, s, S, blocks, tiles, order, flags, epoch, err, gm1); break;
+      case 1: hipLaunchKernelGGL((lusgs_pc_kernel<0, true>), gr, tb2, 0, s, S, blocks, tiles, order, flags, epoch, err, gm1); break;
+      case 2: hipLaunchKernelGGL((lusgs_pc_kernel<1, false>), gr, tb2, 0, s, S, blocks, tiles, order, flags, epoch, err, gm1); break;
+      case 3: hipLaunchKernelGGL((lusgs_pc_kernel<1, true>), gr, tb2, 0, s, S, blocks, tiles, order, flags, epoch, err, gm1); break;
+      case 4: hipLaunchKernelGGL((lusgs_pc_kernel<2, false>), gr, tb2, 0, s, S, blocks, tiles, order, flags, epoch, err, gm1); break;
+      case 5: hipLaunchKernelGGL((lusgs_pc_kernel<2, true>), gr, tb2, 0, s, S, blocks, tiles, order, flags, epoch, err, gm1); break;
+      case 6: hipLaunchKernelGGL((lusgs_pc_kernel<3, false>), gr, tb2, 0, s, S, blocks, tiles, order, flags, epoch, err, gm1); break;
+      default: hipLaunchKernelGGL((lusgs_pc_kernel<3, true>), gr, tb2, 0, s, S, blocks, tiles, order, flags, epoch, err, gm1); break;
+    }
+    return;
+  }
   switch (type * 2 + (v ? 1 : 0)) {
     case 0: hipLaunchKernelGGL((lusgs_tile_kernel<0, false>), gr, tb, 0, s, S, blocks, tiles, order, flags, epoch, err, gm1); break;
     case 1: hipLaunchKernelGGL((lusgs_tile_kernel<0, true>), gr, tb, 0, s, S, blocks, tiles, order, flags, epoch, err, gm1); break;
