@@ -801,7 +801,8 @@ __global__ void __launch_bounds__(TL, 1) lusgs_tile_kernel(SweepArrays S, const 
 //    upstream edge values come from LDS; the new dq goes to LDS;
 //  * loader lanes: put the inputs of step t+1 (fetched a step earlier) into
 //    LDS, store the dq of step t-1 (agent scope), fetch the inputs of step
-//    t+2; the first loader wave also checks the upstream progress, fetches
+//    t+2 (for the LU backward sweep they first store the Gauss-Seidel terms
+//    the update lanes left in the own-input slot of step t-1); the first loader wave also checks the upstream progress, fetches
 //    the edge values and reads the upstream flags for the next check, and
 //    its lane 0 publishes;
 //  * one barrier ends the step.
@@ -902,13 +903,19 @@ __global__ void __launch_bounds__(2 * TL, 1) lusgs_pc_kernel(SweepArrays S, cons
     double xq[3];
     unsigned long long pf[2];
   };
+  // Only lanes with a cell of step u load (the others' LDS entries are
+  // never read as a neighbour that exists); loader waves have no waits
+  // that count on every lane's loads.
   auto fetch = [&](int u, Ring& R) {
-    load_own<TYPE>(S, cell_rb(u), cell_c(u), ncp, R.o);
+    if (cell_ok(u)) {
+      const long long rb = rb0 + (long long)u * qstep;
+      load_own<TYPE>(S, rb, c, ncp, R.o);
 #pragma unroll
-    for (int k = 0; k < SW_D; ++k) {
-      const double2 v = ld_pair(S.D, cell_rb(u), cell_c(u), ncp, SW_D, k);
-      R.D[2 * k] = v.x;
-      R.D[2 * k + 1] = v.y;
+      for (int k = 0; k < SW_D; ++k) {
+        const double2 v = ld_pair(S.D, rb, c, ncp, SW_D, k);
+        R.D[2 * k] = v.x;
+        R.D[2 * k + 1] = v.y;
+      }
     }
     if (lw0) {
 #pragma unroll
@@ -1021,6 +1028,13 @@ __global__ void __launch_bounds__(2 * TL, 1) lusgs_pc_kernel(SweepArrays S, cons
     if (lw0 && xl == 0 && t >= 3 && t - 3 < nsteps)
       __hip_atomic_store(&flags[(long long)tix * FLAG_STRIDE], ep + plane_key(FWD, s - 3 * dir), __ATOMIC_RELAXED,
                          __HIP_MEMORY_SCOPE_AGENT);
+    if (TYPE == 1 && cell_ok(t - 1)) {  // the terms the update lanes left in the own-input slot of step t-1
+      double2* gp = S.ga + (rb0 + (long long)(t - 1) * qstep) * SW_GA + c;
+      const int q = (t - 1) & 1;
+#pragma unroll
+      for (int k = 0; k < SW_GA; ++k)
+        gp[(long long)k * ncp] = make_double2(ownB[q][2 * k][lane], 2 * k + 1 < 15 ? ownB[q][2 * k + 1][lane] : 0.0);
+    }
     deposit(t + 1, Rprev);
     // the fetch just put into LDS has arrived, and with it (in-order
     // counter) this wave's stores of step t-2, issued before it; only the
@@ -1110,19 +1124,15 @@ __global__ void __launch_bounds__(2 * TL, 1) lusgs_pc_kernel(SweepArrays S, cons
         else x = (wmig * r0[f] + -r5[f]) * coed;
         dqB[t & 1][f][lane] = act ? x : 0.0;
       }
-      if (act && (TYPE == 1 || TYPE == 2)) {
-        const long long rb = rb0 + (long long)t * qstep;
-        if (TYPE == 1) {
-          double2* gp = S.ga + rb * SW_GA + c;
+      if (TYPE == 1) {  // into the lane's own-input slot (read above); the loaders store it
 #pragma unroll
-          for (int k = 0; k < SW_GA; ++k)
-            gp[(long long)k * ncp] = make_double2(gout[2 * k], 2 * k + 1 < 15 ? gout[2 * k + 1] : 0.0);
-        } else {
-          double2* pp = S.pre + rb * SW_PRE + c;
-          pp[0] = make_double2(pre5[0], pre5[1]);
-          pp[ncp] = make_double2(pre5[2], pre5[3]);
-          pp[2 * ncp] = make_double2(pre5[4], 0.0);
-        }
+        for (int f = 0; f < 15; ++f) ownB[t & 1][f][lane] = gout[f];
+      }
+      if (act && TYPE == 2) {
+        double2* pp = S.pre + (rb0 + (long long)t * qstep) * SW_PRE + c;
+        pp[0] = make_double2(pre5[0], pre5[1]);
+        pp[ncp] = make_double2(pre5[2], pre5[3]);
+        pp[2 * ncp] = make_double2(pre5[4], 0.0);
       }
       load_ga(t + 1, gnext);
     }
