@@ -38,8 +38,6 @@
 // :209-264, get_diag_coeff :267-295, sweep bodies :318-365, :392-441).
 // The Gauss-Seidel sweeps take their after-neighbour terms from the sweep
 // before them (see lusgs_tile_kernel).
-#include <cstdlib>
-
 #include "orion_dev.h"
 #include "orion_kernels.h"
 
@@ -349,7 +347,7 @@ __device__ __forceinline__ void slot_inv(const double* n, const AxisGeo& a, cons
 }
 
 // ---------------------------------------------------------------------------
-// tile sweep
+// tile sweep: progress keys, waits and loads
 // ---------------------------------------------------------------------------
 constexpr int NW = 13;  // exchanged words per cell: D (8) + dq (5)
 
@@ -373,28 +371,13 @@ __device__ inline unsigned long long need_key(bool fwd, const SweepTile& u, int 
 // maxima, i.e. not waited on)
 constexpr unsigned waitcnt_vm(unsigned n) { return (n & 15u) | ((n >> 4) << 14) | (7u << 4) | (15u << 8); }
 
-// What a lane loads for one step of its pencil besides the neighbours'
-// values, which come through LDS:
-//   own   met0..8, coed, r5 (rhs; for the LU backward sweep the cell's dq
-//         from the forward sweep); Gauss-Seidel backward appends the
-//         partial sum its forward sweep left (5)
-//   ga    Gauss-Seidel forward: the three after-neighbour terms (15)
-//   D     the cell's packed values, handed on to its downstream neighbours
-//   xd,xq the lane's share of the upstream edge values
-template <int TYPE>
-struct StepIn {
-  double own[TYPE == 3 ? 20 : 15];
-  double ga[TYPE == 2 ? 15 : 1];
-  double D[8];
-  double2 xd;  // the lane's share of the upstream edge values
-  double xq;
-  unsigned long long pf[2];  // upstream flags, read a step before they are checked (thread 0)
-};
+// A cell's own inputs, loaded by the loader lanes: met0..8, coed, r5 (rhs;
+// for the LU backward sweep the cell's dq from the forward sweep); the
+// Gauss-Seidel backward sweep appends the partial sum its forward sweep
+// left (5).
 
-// Sweep layout pair loads of the own inputs: 16-byte loads, unconditional
-// (a lane whose cell is absent reads the block's first cell, always a valid
-// address, and never uses the values).  (rb, c): the cell's row base q - c
-// and column.
+// Sweep layout pair loads: 16-byte loads of two fields of a cell.  (rb, c):
+// the cell's row base q - c and column.
 __device__ __forceinline__ double2 ld_pair(const double2* g, long long rb, int c, int ncp, int K, int k) {
   return g[rb * K + (long long)k * ncp + c];
 }
@@ -436,31 +419,9 @@ __device__ __forceinline__ void load_own(const SweepArrays& S, long long rb, int
   }
 }
 
-template <int TYPE>
-__device__ __forceinline__ void load_mid(const SweepArrays& S, long long rb, int c, int ncp, StepIn<TYPE>& P) {
-#pragma unroll
-  for (int k = 0; k < SW_D; ++k) {
-    const double2 v = ld_pair(S.D, rb, c, ncp, SW_D, k);
-    P.D[2 * k] = v.x;
-    P.D[2 * k + 1] = v.y;
-  }
-  if constexpr (TYPE == 2) {
-#pragma unroll
-    for (int k = 0; k < 7; ++k) {
-      const double2 v = ld_pair(S.ga, rb, c, ncp, SW_GA, k);
-      P.ga[2 * k] = v.x;
-      P.ga[2 * k + 1] = v.y;
-    }
-    P.ga[14] = ld_half(S.ga, rb, c, ncp, SW_GA, 7);
-  }
-}
-
-// Prefetch loads every wave issues per step: own inputs, D and ga.
-template <int TYPE>
-constexpr unsigned step_loads() {
-  return (TYPE == 1 ? 5 + SW_DQ : SW_OWN) + (TYPE == 3 ? SW_PRE : 0) + SW_D + (TYPE == 2 ? SW_GA : 0);
-}
-
+// ---------------------------------------------------------------------------
+// tile sweep
+// ---------------------------------------------------------------------------
 // Sweep types: 0 LU forward, 1 LU backward, 2 Gauss-Seidel forward,
 // 3 Gauss-Seidel backward.
 //
@@ -479,324 +440,10 @@ constexpr unsigned step_loads() {
 //    forward sweep used for them: the forward sweep's partial sum after its
 //    first three slots is stored in pre and is the backward sweep's start.
 //
-// Pipeline of one tile.  Step t works on hyperplane s(t) = s0 + dir*t:
-//   A  the first wave puts this step's upstream edge values into LDS;
-//   B  each wave waits for its own stores of step t-1 (by count: only the
-//      loads issued after them may stay in flight); barrier;
-//   C  thread 0 publishes plane s(t-1);
-//   D  the update of each lane's cell from LDS and registers; dq is stored
-//      with agent-scope stores (ga / pre with plain ones); the lane's
-//      (D, new dq) go to LDS;
-//   E  thread 0 makes sure the upstream tiles have finished plane s(t+1),
-//      from flag values it read one step earlier (it spins only when they
-//      have not); then the loads: own inputs and edge values of step t+2,
-//      D and ga of step t+1.
-// LDS buffers alternate by step parity, so one barrier per step suffices.
-// The three StepIn buffers rotate by unrolling (no register copies, which
-// would wait for the loads still in flight).
-template <int TYPE, bool VIS>
-__global__ void __launch_bounds__(TL, 1) lusgs_tile_kernel(SweepArrays S, const BlockDev* __restrict__ blocks,
-                                                          const SweepTile* __restrict__ tiles,
-                                                          const int* __restrict__ order, unsigned long long* flags,
-                                                          const unsigned long long* __restrict__ epoch, int* err,
-                                                          double gm1) {
-  constexpr bool FWD = (TYPE == 0 || TYPE == 2);
-  constexpr int dir = FWD ? 1 : -1;
-  constexpr int SL0 = FWD ? 0 : 3;  // slots of the neighbours before a cell in sweep order
-  constexpr unsigned NLD = step_loads<TYPE>();
-  __shared__ double shA[2][NW][TL];     // (D, new dq) of each lane's cell, by step parity
-  __shared__ double shX[2][2][NW][TW];  // upstream edge values, by step parity
-
-  const int tix = order[blockIdx.x];
-  const SweepTile T = tiles[tix];
-  const BlockDev& b = blocks[T.block];
-  const int nm = b.sn[0], nc = b.sn[1], nr = b.sn[2];
-  const int am = b.sax[0], ac = b.sax[1];
-  const int lane = threadIdx.x, lc = lane % TW, lr = lane / TW;
-  const int c = T.c0 + lc, r = T.r0 + lr;
-  const bool lane_ok = c < nc && r < nr;
-  const int up_c = FWD ? T.nbr[0] : T.nbr[2], up_r = FWD ? T.nbr[1] : T.nbr[3];
-  const unsigned long long ep = (*epoch) << KEY_BITS;
-  SweepTile ut[2];  // plane ranges of the upstream tiles (clamped wait targets)
-  ut[0] = up_c >= 0 ? tiles[up_c] : T;
-  ut[1] = up_r >= 0 ? tiles[up_r] : T;
-  const int nsteps = T.s_last - T.s_first + 1;
-  const int s0 = FWD ? T.s_first : T.s_last;
-  const int ncp = b.sncp;
-  const long long qstep = dir * (long long)nr * ncp;  // one hyperplane in the sweep layout
-
-  // this lane's cell at step t: valid flag, row base q - c and column (the
-  // block's first cell when absent)
-  const int m0 = s0 - r - c;
-  const long long rb0 = b.sbase + ((long long)s0 * nr + r) * ncp;
-  auto cell_ok = [&](int t) {
-    const int m = m0 + dir * t;
-    return lane_ok && t < nsteps && m >= 0 && m < nm;
-  };
-  auto cell_rb = [&](int t) { return cell_ok(t) ? rb0 + (long long)t * qstep : b.sbase; };
-  auto cell_c = [&](int t) { return cell_ok(t) ? c : 0; };
-
-  // Upstream edge values: per step 2 x 16 cells of 4 D pairs and 5 dq
-  // words, spread over the tile's lanes: lane l < 128 loads D pair item l,
-  // lane l < 160 dq word item l.  Item e is (role, index, part), role 0 the
-  // cells across the upstream col edge, role 1 those across the upstream
-  // row edge.  The other lanes read the block's first cell and never use
-  // the values, so that every wave issues the same loads.  A neighbour of a
-  // cell on plane s lies on s - dir.  The loads follow the barrier after
-  // thread 0 has checked the upstream progress.
-  int er[2], ec[2], xo[2];
-  bool ein[2];
-#pragma unroll
-  for (int j = 0; j < 2; ++j) {
-    const int e = lane;
-    const bool item = j == 0 ? e < 128 : e < 160;
-    const int role = j == 0 ? (e >> 6) & 1 : (e < 80 ? 0 : 1);
-    const int idx = j == 0 ? (e >> 2) & 15 : (e % 80) / 5;
-    const int word = j == 0 ? 2 * (e & 3) : 8 + e % 5;
-    if (role == 0) {
-      ec[j] = FWD ? T.c0 - 1 : T.c0 + TW;
-      er[j] = T.r0 + idx;
-    } else {
-      er[j] = FWD ? T.r0 - 1 : T.r0 + TW;
-      ec[j] = T.c0 + idx;
-    }
-    ein[j] = item && er[j] >= 0 && er[j] < nr && ec[j] >= 0 && ec[j] < nc;
-    xo[j] = item ? (role * NW + word) * TW + idx : -1;  // offset in one parity of shX
-  }
-  auto edge_rb = [&](int j, int t, int* cc) -> long long {
-    const int sp = s0 + dir * (t - 1);
-    const int mm = sp - er[j] - ec[j];
-    const bool ok = ein[j] && t < nsteps && mm >= 0 && mm < nm;
-    *cc = ok ? ec[j] : 0;
-    return ok ? b.sbase + ((long long)sp * nr + er[j]) * ncp : b.sbase;
-  };
-  auto load_edges = [&](int t, StepIn<TYPE>& P) {
-    int cc;
-    long long rb = edge_rb(0, t, &cc);
-    P.xd = ld_pair(S.D, rb, cc, ncp, SW_D, lane & 3);
-    rb = edge_rb(1, t, &cc);
-    P.xq = __hip_atomic_load(&reinterpret_cast<const double*>(S.dq)[sweep_field(rb + cc, cc, ncp, SW_DQ, lane % 5)],
-                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  };
-
-  // thread 0: the upstream tiles must have finished plane s; pf holds the
-  // flag values read a step earlier, re-read (bounded spin) only when short
-  auto ensure_up = [&](int s, const unsigned long long* pf) {
-    if (lane != 0) return;
-    for (int k = 0; k < 2; ++k) {
-      const int u = k ? up_r : up_c;
-      if (u < 0) continue;
-      const unsigned long long key = need_key(FWD, ut[k], s);
-      if (key == 0) continue;
-      const unsigned long long want = ep + key;
-      unsigned long long v = pf[k];
-      long long spins = 0;
-      while (v < want) {
-        // far beyond any legal wait (a whole sweep takes milliseconds), or
-        // another tile already gave up: report and carry on, never hang
-        if (++spins > (1LL << 20) ||
-            ((spins & 255) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)) {
-          atomicOr(err, 1);
-          break;
-        }
-        if (spins > 1) __builtin_amdgcn_s_sleep(2);
-        v = __hip_atomic_load(&flags[(long long)u * FLAG_STRIDE], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      }
-    }
-  };
-  auto read_up = [&](unsigned long long* pf) {  // thread 0: flag reads for the next check
-    if (lane != 0) return;
-    if (up_c >= 0) pf[0] = __hip_atomic_load(&flags[(long long)up_c * FLAG_STRIDE], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (up_r >= 0) pf[1] = __hip_atomic_load(&flags[(long long)up_r * FLAG_STRIDE], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  };
-
-  // before-neighbour of this lane's cell along real axis ax at step parity
-  // p: its (D, dq) and whether it exists.  Along the march axis it is the
-  // lane's own cell of the previous step.  Branch free: the source (a lane's
-  // entry of shA, or the edge values in shX) is selected by address.
-  auto nbr = [&](int ax, int m, int p, double* v) -> bool {
-    constexpr int d = -dir;
-    const bool march = (ax == am);
-    const bool col = (ax == ac);
-    const int pos = col ? lc : lr, lim = col ? nc : nr, coord = col ? c : r;
-    const bool inside = march || (pos + d >= 0 && pos + d < TW);
-    const int ln = march ? lane : lane + (col ? d : d * TW);
-    const double* base = inside ? &shA[p ^ 1][0][ln] : &shX[p][col ? 0 : 1][0][col ? lr : lc];
-    const int stride = inside ? TL : TW;
-#pragma unroll
-    for (int f = 0; f < NW; ++f) v[f] = base[f * stride];
-    return march ? (m + d >= 0 && m + d < nm) : (coord + d >= 0 && coord + d < lim);
-  };
-
-  auto step = [&](int t, StepIn<TYPE>& cur, StepIn<TYPE>& mid, StepIn<TYPE>& nxt) __attribute__((always_inline)) {
-    const int p = t & 1;
-    const int s = s0 + dir * t;
-    const int m = m0 + dir * t;
-    const bool act = cell_ok(t);
-    // A
-    {
-      double* x = &shX[p][0][0][0];
-      if (xo[0] >= 0) {
-        x[xo[0]] = cur.xd.x;
-        x[xo[0] + TW] = cur.xd.y;
-      }
-      if (xo[1] >= 0) x[xo[1]] = cur.xq;
-    }
-    // B: this wave's memory operations have completed, its stores of step
-    // t-1 among them
-    asm volatile("" ::: "memory");
-    __builtin_amdgcn_sched_barrier(0);
-    __builtin_amdgcn_s_waitcnt(waitcnt_vm(0));
-    __builtin_amdgcn_sched_barrier(0);
-    __syncthreads();
-    // C
-    if (t > 0 && t <= nsteps && lane == 0)
-      __hip_atomic_store(&flags[(long long)tix * FLAG_STRIDE], ep + plane_key(FWD, s - dir), __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
-    // the upstream flags for the check of the next step, read here so that
-    // they return long before any wait of this wave needs them
-    read_up(mid.pf);
-    // the edge values of step t+1 lie on plane s, which thread 0 found
-    // finished before this step's barrier
-    load_edges(t + 1, mid);
-    // D: the update of every lane's cell, branch free (a lane without a cell
-    // computes on the block's first cell and stores nothing), with the loads
-    // of later steps interleaved: D and ga of step t+1, own inputs of t+2
-    load_mid<TYPE>(S, cell_rb(t + 1), cell_c(t + 1), ncp, mid);
-    load_own<TYPE>(S, cell_rb(t + 2), cell_c(t + 2), ncp, nxt.own);
-    const double* o = cur.own;
-    double r0[5];
-#pragma unroll
-    for (int f = 0; f < 5; ++f) r0[f] = TYPE == 3 ? o[15 + f] : 0.0;
-    double gout[15];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      AxisGeo ag;
-      axis_geo(o + 3 * k, ag);
-      double v[NW];
-      const bool ok = nbr(k, m, p, v);
-      double inv[5], tv[5];
-      slot_inv<SL0>(o + 3 * k, ag, v, v + 8, gm1, inv);
-      if (VIS) {
-        const double hv = 0.5 * (v[7] * ag.g2);
-#pragma unroll
-        for (int f = 0; f < 5; ++f) tv[f] = hv * v[8 + f];
-      }
-#pragma unroll
-      for (int f = 0; f < 5; ++f) {
-        if (TYPE == 3) {  // Gauss-Seidel term of a +1 neighbour: r - (inviscid - viscous)
-          const double res = VIS ? inv[f] + tv[f] * -1.0 : inv[f];
-          const double acc = r0[f] - res;
-          r0[f] = ok ? acc : r0[f];
-        } else {  // LU terms, and Gauss-Seidel terms of -1 neighbours: r + (inviscid + viscous)
-          const double res = VIS ? inv[f] + tv[f] : inv[f];
-          const double acc = r0[f] + res;
-          r0[f] = ok ? acc : r0[f];
-        }
-        if (TYPE == 1) {  // the Gauss-Seidel forward sweep's term for the same pair
-          const double g = VIS ? inv[f] + tv[f] * -1.0 : inv[f];
-          gout[5 * k + f] = ok ? g : 0.0;
-        }
-      }
-    }
-    double pre5[5];
-    if (TYPE == 2) {
-#pragma unroll
-      for (int f = 0; f < 5; ++f) pre5[f] = r0[f];
-#pragma unroll
-      for (int k = 0; k < 3; ++k)
-#pragma unroll
-        for (int f = 0; f < 5; ++f) r0[f] = r0[f] - cur.ga[5 * k + f];
-    }
-    const double wmig = 1.0;
-    const double* r5 = o + 10;
-    const double coed = o[9];
-    double dqn[5];
-#pragma unroll
-    for (int f = 0; f < 5; ++f) {
-      double x;
-      if (TYPE == 0) x = (-r5[f] + wmig * r0[f]) * coed;
-      else if (TYPE == 1) x = r5[f] - (wmig * r0[f]) * coed;
-      else x = (wmig * r0[f] + -r5[f]) * coed;
-      dqn[f] = act ? x : 0.0;
-    }
-    // one prefetch load between runs of arithmetic
-#pragma unroll
-    for (int i = 0; i < (int)NLD; ++i) {
-      __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
-      __builtin_amdgcn_sched_group_barrier(0x002, 24, 0);
-    }
-    // F
-    if (act) {
-      const long long rb = rb0 + (long long)t * qstep;
-      double* dqp = reinterpret_cast<double*>(S.dq);
-#pragma unroll
-      for (int f = 0; f < 5; ++f)
-        __hip_atomic_store(&dqp[sweep_field(rb + c, c, ncp, SW_DQ, f)], dqn[f], __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-      if (TYPE == 1) {
-        double2* gp = S.ga + rb * SW_GA + c;
-#pragma unroll
-        for (int k = 0; k < SW_GA; ++k)
-          gp[(long long)k * ncp] = make_double2(gout[2 * k], 2 * k + 1 < 15 ? gout[2 * k + 1] : 0.0);
-      }
-      if (TYPE == 2) {
-        double2* pp = S.pre + rb * SW_PRE + c;
-        pp[0] = make_double2(pre5[0], pre5[1]);
-        pp[ncp] = make_double2(pre5[2], pre5[3]);
-        pp[2 * ncp] = make_double2(pre5[4], 0.0);
-      }
-    }
-    // G
-#pragma unroll
-    for (int f = 0; f < 8; ++f) shA[p][f][lane] = cur.D[f];
-#pragma unroll
-    for (int f = 0; f < 5; ++f) shA[p][8 + f][lane] = dqn[f];
-    // E: the edge values of step t+2 lie on plane s(t+1)
-    asm volatile("" ::: "memory");
-    __builtin_amdgcn_sched_barrier(0);
-    if (t + 2 < nsteps) ensure_up(s + dir, cur.pf);
-  };
-
-  StepIn<TYPE> PA, PB, PC;
-#pragma unroll
-  for (int f = 0; f < NW; ++f) {
-    shA[0][f][lane] = 0.0;
-    shA[1][f][lane] = 0.0;
-  }
-  // same issue order as in the loop, so that the waits the loop needs are
-  // no stricter on entry
-  {
-    const unsigned long long none[2] = {0, 0};
-    ensure_up(s0, none);  // covers the edge values of steps 0 and 1
-  }
-  read_up(PA.pf);
-  __syncthreads();
-  load_edges(0, PA);
-  load_own<TYPE>(S, cell_rb(0), cell_c(0), ncp, PA.own);
-  load_mid<TYPE>(S, cell_rb(0), cell_c(0), ncp, PA);
-  load_own<TYPE>(S, cell_rb(1), cell_c(1), ncp, PB.own);
-  // whole triples of steps: the steps past the end are idle (no cell, no
-  // store, no publication) but keep the loop free of side exits, whose
-  // paths would make the compiler's load waits stricter
-  for (int t = 0; t < nsteps; t += 3) {
-    step(t, PA, PB, PC);
-    step(t + 1, PB, PC, PA);
-    step(t + 2, PC, PA, PB);
-  }
-  __builtin_amdgcn_s_waitcnt(0);
-  __syncthreads();
-  if (lane == 0)
-    __hip_atomic_store(&flags[(long long)tix * FLAG_STRIDE], ep + plane_key(FWD, s0 + dir * (nsteps - 1)), __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// ---------------------------------------------------------------------------
-// tile sweep with loader waves
-// ---------------------------------------------------------------------------
-// The pipeline of lusgs_tile_kernel with the memory traffic taken off the
-// update waves.  A tile's workgroup has 256 update lanes and 256 loader
-// lanes; lane l of either kind works on pencil l.  Step t:
+// Pipeline of one tile, with the memory traffic kept off the update waves:
+// a tile's workgroup has 256 update lanes and 256 loader lanes, lane l of
+// either kind working on pencil l.  Step t works on hyperplane
+// s(t) = s0 + dir*t:
 //  * update lanes: the cell's inputs, the neighbours' (D, dq) and the
 //    upstream edge values come from LDS; the new dq goes to LDS;
 //  * loader lanes: put the inputs of step t+1 (fetched a step earlier) into
@@ -812,7 +459,7 @@ __global__ void __launch_bounds__(TL, 1) lusgs_tile_kernel(SweepArrays S, const 
 // step parity, D by step mod 3 (a step's update reads the D of the step
 // before), dq by parity, edge values by parity.
 template <int TYPE, bool VIS>
-__global__ void __launch_bounds__(2 * TL, 1) lusgs_pc_kernel(SweepArrays S, const BlockDev* __restrict__ blocks,
+__global__ void __launch_bounds__(2 * TL, 1) lusgs_tile_kernel(SweepArrays S, const BlockDev* __restrict__ blocks,
                                                             const SweepTile* __restrict__ tiles,
                                                             const int* __restrict__ order, unsigned long long* flags,
                                                             const unsigned long long* __restrict__ epoch, int* err,
@@ -1198,32 +845,18 @@ void launch_lusgs_tiles(const SweepArrays& S, const BlockDev* blocks, const Swee
   if (ntiles <= 0) return;
   hipLaunchKernelGGL(epoch_kernel, dim3(1), dim3(64), 0, s, epoch);
   const double gm1 = p.gama - 1.0;
-  const dim3 gr(ntiles), tb(TL);
+  const dim3 gr(ntiles);
   const bool v = p.nvis == 1;
-  static const int pc = [] { const char* e = std::getenv("ORION_LU_PC"); return e ? std::atoi(e) : 1; }();
-  if (pc) {
-    const dim3 tb2(2 * TL);
-    switch (type * 2 + (v ? 1 : 0)) {
-      case 0: hipLaunchKernelGGL((lusgs_pc_kernel<0, false>), gr, tb2, 0, s, S, blocks, tiles, order, flags, epoch, err, gm1); break;
-      case 1: hipLaunchKernelGGL((lusgs_pc_kernel<0, true>), gr, tb2, 0, s, S, blocks, tiles, order, flags, epoch, err, gm1); break;
-      case 2: hipLaunchKernelGGL((lusgs_pc_kernel<1, false>), gr, tb2, 0, s, S, blocks, tiles, order, flags, epoch, err, gm1); break;
-      case 3: hipLaunchKernelGGL((lusgs_pc_kernel<1, true>), gr, tb2, 0, s, S, blocks, tiles, order, flags, epoch, err, gm1); break;
-      case 4: hipLaunchKernelGGL((lusgs_pc_kernel<2, false>), gr, tb2, 0, s, S, blocks, tiles, order, flags, epoch, err, gm1); break;
-      case 5: hipLaunchKernelGGL((lusgs_pc_kernel<2, true>), gr, tb2, 0, s, S, blocks, tiles, order, flags, epoch, err, gm1); break;
-      case 6: hipLaunchKernelGGL((lusgs_pc_kernel<3, false>), gr, tb2, 0, s, S, blocks, tiles, order, flags, epoch, err, gm1); break;
-      default: hipLaunchKernelGGL((lusgs_pc_kernel<3, true>), gr, tb2, 0, s, S, blocks, tiles, order, flags, epoch, err, gm1); break;
-    }
-    return;
-  }
+  const dim3 tb2(2 * TL);
   switch (type * 2 + (v ? 1 : 0)) {
-    case 0: hipLaunchKernelGGL((lusgs_tile_kernel<0, false>), gr, tb, 0, s, S, blocks, tiles, order, flags, epoch, err, gm1); break;
-    case 1: hipLaunchKernelGGL((lusgs_tile_kernel<0, true>), gr, tb, 0, s, S, blocks, tiles, order, flags, epoch, err, gm1); break;
-    case 2: hipLaunchKernelGGL((lusgs_tile_kernel<1, false>), gr, tb, 0, s, S, blocks, tiles, order, flags, epoch, err, gm1); break;
-    case 3: hipLaunchKernelGGL((lusgs_tile_kernel<1, true>), gr, tb, 0, s, S, blocks, tiles, order, flags, epoch, err, gm1); break;
-    case 4: hipLaunchKernelGGL((lusgs_tile_kernel<2, false>), gr, tb, 0, s, S, blocks, tiles, order, flags, epoch, err, gm1); break;
-    case 5: hipLaunchKernelGGL((lusgs_tile_kernel<2, true>), gr, tb, 0, s, S, blocks, tiles, order, flags, epoch, err, gm1); break;
-    case 6: hipLaunchKernelGGL((lusgs_tile_kernel<3, false>), gr, tb, 0, s, S, blocks, tiles, order, flags, epoch, err, gm1); break;
-    default: hipLaunchKernelGGL((lusgs_tile_kernel<3, true>), gr, tb, 0, s, S, blocks, tiles, order, flags, epoch, err, gm1); break;
+    case 0: hipLaunchKernelGGL((lusgs_tile_kernel<0, false>), gr, tb2, 0, s, S, blocks, tiles, order, flags, epoch, err, gm1); break;
+    case 1: hipLaunchKernelGGL((lusgs_tile_kernel<0, true>), gr, tb2, 0, s, S, blocks, tiles, order, flags, epoch, err, gm1); break;
+    case 2: hipLaunchKernelGGL((lusgs_tile_kernel<1, false>), gr, tb2, 0, s, S, blocks, tiles, order, flags, epoch, err, gm1); break;
+    case 3: hipLaunchKernelGGL((lusgs_tile_kernel<1, true>), gr, tb2, 0, s, S, blocks, tiles, order, flags, epoch, err, gm1); break;
+    case 4: hipLaunchKernelGGL((lusgs_tile_kernel<2, false>), gr, tb2, 0, s, S, blocks, tiles, order, flags, epoch, err, gm1); break;
+    case 5: hipLaunchKernelGGL((lusgs_tile_kernel<2, true>), gr, tb2, 0, s, S, blocks, tiles, order, flags, epoch, err, gm1); break;
+    case 6: hipLaunchKernelGGL((lusgs_tile_kernel<3, false>), gr, tb2, 0, s, S, blocks, tiles, order, flags, epoch, err, gm1); break;
+    default: hipLaunchKernelGGL((lusgs_tile_kernel<3, true>), gr, tb2, 0, s, S, blocks, tiles, order, flags, epoch, err, gm1); break;
   }
 }
 
