@@ -65,6 +65,7 @@ struct orion_hip_ctx {
   Space pad, intr, face[3], line[3];
   Space fpair[3];                      // face pairs (2g, 2g+1) per line along d
   Space ltask[3];                      // six boundary-closure tasks per line along d
+  Space dtile;                         // flux differences: 16 x 16 (i, j) nodes of a k plane
   Space ptile;                         // LU-SGS pack tiles: 16 x 16 (march, col) cells of one row
   SweepArrays sw{};                    // LU-SGS working set in sweep layout
   std::vector<SweepTile> tiles;
@@ -713,6 +714,14 @@ int orion_hip_ctx_create(int nblocks, const int* dims, const double* xyz, int nr
     for (int b = 0; b < nblocks; ++b) cp2[b] = nlines(c->hb[b], d) * 6;
     if (make_space(c, c->ltask[d], cp2)) return bail(-1);
   }
+  {
+    std::vector<long long> ct(nblocks);
+    for (int b = 0; b < nblocks; ++b) {
+      const BlockDev& B = c->hb[b];
+      ct[b] = (long long)((B.idim + 15) / 16) * ((B.jdim + 15) / 16) * B.kdim * 256;
+    }
+    if (make_space(c, c->dtile, ct)) return bail(-1);
+  }
   if (build_sweep(c)) return bail(-1);
   {
     std::vector<long long> ct(nblocks);
@@ -842,7 +851,7 @@ static void op_visc(orion_hip_ctx* c) {
   const double re_inv = (std::fabs(re) > 1.0e-30) ? 1.0 / re : 0.0;
   launch_visc_deriv(LA(c, c->intr), c->intr.total());
   for (int d = 0; d < 3; ++d) launch_visc_face(LA(c, c->fpair[d]), d, c->fpair[d].total(), cp_prl);
-  launch_visc_diff(LA(c, c->intr), c->intr.total(), re_inv);
+  launch_visc_diff(LA(c, c->dtile), c->dtile.total(), re_inv);
 }
 
 static void op_avg(orion_hip_ctx* c) {
@@ -855,7 +864,7 @@ static void op_inv(orion_hip_ctx* c) {
     launch_inv_face(LA(c, c->face[d]), d, c->face[d].total(), c->p.gama);
     launch_inv_closure(LA(c, c->ltask[d]), d, c->ltask[d].total(), c->p.gama);
   }
-  launch_inv_diff(LA(c, c->intr), c->intr.total());
+  launch_inv_diff(LA(c, c->dtile), c->dtile.total());
 }
 
 static void op_sweep(orion_hip_ctx* c, int type) {
@@ -1065,7 +1074,7 @@ int orion_hip_profile_run(orion_hip_ctx* c, int nsteps, double* phase_ms, long l
       launch_visc_deriv(LA(c, c->intr), c->intr.total()); mark(1);
       for (int d = 0; d < 3; ++d) launch_visc_face(LA(c, c->fpair[d]), d, c->fpair[d].total(), cp / c->p.prl);
       mark(3);
-      launch_visc_diff(LA(c, c->intr), c->intr.total(), (std::fabs(re) > 1.0e-30) ? 1.0 / re : 0.0); mark(1);
+      launch_visc_diff(LA(c, c->dtile), c->dtile.total(), (std::fabs(re) > 1.0e-30) ? 1.0 / re : 0.0); mark(1);
       op_avg(c); mark(2);
     } else {
       mark(0); mark(0); mark(0); mark(0);
@@ -1074,7 +1083,7 @@ int orion_hip_profile_run(orion_hip_ctx* c, int nsteps, double* phase_ms, long l
     mark(3);
     for (int d = 0; d < 3; ++d) launch_inv_closure(LA(c, c->ltask[d]), d, c->ltask[d].total(), c->p.gama);
     mark(3);
-    launch_inv_diff(LA(c, c->intr), c->intr.total()); mark(1);
+    launch_inv_diff(LA(c, c->dtile), c->dtile.total()); mark(1);
     op_avg(c); mark(2);
     launch_lu_pack(LA(c, c->ptile), c->ptile.total(), c->sw, c->p); mark(1);
     for (int t = 0; t < 4; ++t) { op_sweep(c, t); mark(2); }

@@ -371,51 +371,7 @@ __global__ void __launch_bounds__(256) inv_closure_kernel(const BlockDev* __rest
 // 6th-order difference with 3rd/4th-order ends
 // (compute_flux_diff_high_order :197-233 == compute_flux_diff_1d :257-296)
 // for interior node t (0-based along the line); F(f) is face f.
-__device__ inline double diff_node(const double* F0, long long s, int n, int t) {
-  auto F = [&](int f) { return F0[(long long)(f + 2) * s]; };
-  if (t >= 2 && t < n - 2)
-    return (2250.0 * (F(t + 1) - F(t)) + -125.0 * (F(t + 2) - F(t - 1)) + 9.0 * (F(t + 3) - F(t - 2))) / 1920.0;
-  if (t == 1) return (F(0) - 27.0 * F(1) + 27.0 * F(2) - F(3)) / 24.0;
-  if (t == n - 2) return -(F(n) - 27.0 * F(n - 1) + 27.0 * F(n - 2) - F(n - 3)) / 24.0;
-  if (t == 0) return (-23.0 * F(0) + 21.0 * F(1) + 3.0 * F(2) - F(3)) / 24.0;
-  return -(-23.0 * F(n) + 21.0 * F(n - 1) + 3.0 * F(n - 2) - F(n - 3)) / 24.0;
-}
 
-__global__ void __launch_bounds__(256) inv_diff_kernel(const BlockDev* __restrict__ blocks,
-                                                       Span SP) {
-  int bi;
-  long long gi;
-  if (!span_item(SP, &bi, &gi)) return;
-  const BlockDev& b = blocks[bi];
-  int i, j, k;
-  decode_interior(b, gi, &i, &j, &k);
-  const long long id = pidx(b, i, j, k);
-  double acc[5];
-#pragma unroll
-  for (int m = 0; m < 5; ++m) acc[m] = b.dq[(long long)m * b.P + id];
-  const int pos[3] = {i, j, k};
-  for (int d = 0; d < 3; ++d) {
-    const int n = dcount(b, d);
-    const long long s = dstride(b, d);
-    const int t = pos[d] - NG;
-    const long long lbase = id - (long long)pos[d] * s;
-    const bool lflag = b.prim[lbase] < 1.0e-20;
-    const bool rflag = b.prim[lbase + (long long)(n + 5) * s] < 1.0e-20;
-    const double* cov = nullptr;
-    if (lflag && t <= 2) cov = b.fcov + b.fcov_off[d] + line_id(b, d, i, j, k) * 30 + t * 5;
-    else if (rflag && t >= n - 3) cov = b.fcov + b.fcov_off[d] + line_id(b, d, i, j, k) * 30 + (3 + t - (n - 3)) * 5;
-    // the difference is formed for every node (its loads do not wait for
-    // the flags) and replaced by the closure value where the line has one
-#pragma unroll
-    for (int m = 0; m < 5; ++m) {
-      double fc = diff_node(b.fx + ((long long)d * 5 + m) * b.P + lbase, s, n, t);
-      if (cov) fc = cov[m];
-      acc[m] += fc;
-    }
-  }
-#pragma unroll
-  for (int m = 0; m < 5; ++m) b.dq[(long long)m * b.P + id] = acc[m];
-}
 
 // ---------------------------------------------------------------------------
 // Viscous RHS
@@ -634,29 +590,101 @@ __global__ void __launch_bounds__(256) visc_face_kernel(const BlockDev* __restri
   }
 }
 
-__global__ void __launch_bounds__(256) visc_diff_kernel(const BlockDev* __restrict__ blocks,
-                                                        Span SP, double re_inv) {
+
+template <class FF>
+__device__ __forceinline__ double diff_node_f(const FF& F, int n, int t) {
+  if (t >= 2 && t < n - 2)
+    return (2250.0 * (F(t + 1) - F(t)) + -125.0 * (F(t + 2) - F(t - 1)) + 9.0 * (F(t + 3) - F(t - 2))) / 1920.0;
+  if (t == 1) return (F(0) - 27.0 * F(1) + 27.0 * F(2) - F(3)) / 24.0;
+  if (t == n - 2) return -(F(n) - 27.0 * F(n - 1) + 27.0 * F(n - 2) - F(n - 3)) / 24.0;
+  if (t == 0) return (-23.0 * F(0) + 21.0 * F(1) + 3.0 * F(2) - F(3)) / 24.0;
+  return -(-23.0 * F(n) + 21.0 * F(n - 1) + 3.0 * F(n - 2) - F(n - 3)) / 24.0;
+}
+
+// Flux differences of the inviscid (INV, closure overrides included) or the
+// viscous face fluxes into dq, per direction i, j, k in that order
+// (compute_flux_diff_1d / compute_flux_diff_high_order; viscous: dq -=
+// diff / Re, inviscid: dq += diff).  A workgroup takes 16 x 16 (i, j) nodes
+// of one k plane; the i and j stencils of the tile (21 face slots per row
+// or column) are staged through LDS once, the k stencils are loaded per
+// node.  Node t of a line reads faces t-2..t+3, which sit at the padded
+// slots t..t+5.
+template <bool INV>
+__global__ void __launch_bounds__(256) diff_kernel(const BlockDev* __restrict__ blocks, Span SP, double re_inv) {
+  constexpr int DT = 16, DW = DT + 5;
+  __shared__ double X[5][DT][DW];  // i slots i0-3 .. i0+17 of each row of the tile
+  __shared__ double Y[5][DW][DT];  // j slots j0-3 .. j0+17 of each column
   int bi;
   long long gi;
-  if (!span_item(SP, &bi, &gi)) return;
+  if (!span_item(SP, &bi, &gi)) return;  // whole tiles: every lane of a workgroup has an item
   const BlockDev& b = blocks[bi];
-  int i, j, k;
-  decode_interior(b, gi, &i, &j, &k);
+  const long long P = b.P;
+  const unsigned tci = (unsigned)(b.idim + DT - 1) / DT, tcj = (unsigned)(b.jdim + DT - 1) / DT;
+  unsigned q, ti, q2, tj;
+  udivmod((unsigned)(gi / 256), tci, &q, &ti);
+  udivmod(q, tcj, &q2, &tj);
+  const int k = (int)q2 + NG;
+  const int i0 = NG + DT * (int)ti, j0 = NG + DT * (int)tj;
+  const int lane = threadIdx.x, li = lane % DT, lj = lane / DT;
+  const int i = i0 + li, j = j0 + lj;
+  for (int e = lane; e < 5 * DT * DW; e += 256) {
+    const int m = e / (DT * DW), r = e % (DT * DW);
+    {
+      const int row = r / DW, col = r % DW;
+      const int jj = j0 + row, ii = i0 - 3 + col;
+      const bool ok = jj < NG + b.jdim && ii < b.nx;
+      X[m][row][col] = ok ? b.fx[(long long)m * P + pidx(b, ii, jj, k)] : 0.0;
+    }
+    {
+      const int row = r / DT, col = r % DT;
+      const int ii = i0 + col, jj = j0 - 3 + row;
+      const bool ok = ii < NG + b.idim && jj < b.ny;
+      Y[m][row][col] = ok ? b.fx[(long long)(5 + m) * P + pidx(b, ii, jj, k)] : 0.0;
+    }
+  }
+  __syncthreads();
+  if (!(i < NG + b.idim && j < NG + b.jdim)) return;
   const long long id = pidx(b, i, j, k);
   double acc[5];
 #pragma unroll
-  for (int m = 0; m < 5; ++m) acc[m] = b.dq[(long long)m * b.P + id];
+  for (int m = 0; m < 5; ++m) acc[m] = b.dq[(long long)m * P + id];
   const int pos[3] = {i, j, k};
+#pragma unroll
   for (int d = 0; d < 3; ++d) {
     const int n = dcount(b, d);
     const long long s = dstride(b, d);
-    const long long lbase = id - (long long)pos[d] * s;
     const int t = pos[d] - NG;
+    const long long lbase = id - (long long)pos[d] * s;
+    const double* cov = nullptr;
+    if (INV) {
+      const bool lflag = b.prim[lbase] < 1.0e-20;
+      const bool rflag = b.prim[lbase + (long long)(n + 5) * s] < 1.0e-20;
+      if (lflag && t <= 2) cov = b.fcov + b.fcov_off[d] + line_id(b, d, i, j, k) * 30 + t * 5;
+      else if (rflag && t >= n - 3) cov = b.fcov + b.fcov_off[d] + line_id(b, d, i, j, k) * 30 + (3 + t - (n - 3)) * 5;
+    }
 #pragma unroll
-    for (int m = 0; m < 5; ++m) acc[m] -= re_inv * diff_node(b.fx + ((long long)d * 5 + m) * b.P + lbase, s, n, t);
+    for (int m = 0; m < 5; ++m) {
+      double fc;
+      if (d == 0) {
+        fc = diff_node_f([&](int f) { return X[m][lj][f + 2 - (i0 - 3)]; }, n, t);
+      } else if (d == 1) {
+        fc = diff_node_f([&](int f) { return Y[m][f + 2 - (j0 - 3)][li]; }, n, t);
+      } else {
+        const double* F0 = b.fx + (long long)(10 + m) * P + lbase;
+        fc = diff_node_f([&](int f) { return F0[(long long)(f + 2) * s]; }, n, t);
+      }
+      if (INV) {
+        // the difference is formed for every node and replaced by the
+        // closure value where the line has one
+        if (cov) fc = cov[m];
+        acc[m] += fc;
+      } else {
+        acc[m] -= re_inv * fc;
+      }
+    }
   }
 #pragma unroll
-  for (int m = 0; m < 5; ++m) b.dq[(long long)m * b.P + id] = acc[m];
+  for (int m = 0; m < 5; ++m) b.dq[(long long)m * P + id] = acc[m];
 }
 
 // ---------------------------------------------------------------------------
@@ -674,7 +702,7 @@ void launch_inv_closure(const Launch& L, int d, long long n, double gamma) {
 }
 void launch_inv_diff(const Launch& L, long long n) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(inv_diff_kernel, dim3((unsigned)L.nwg), dim3(SPAN_TB), 0, L.stream, L.blocks, L.span);
+  hipLaunchKernelGGL(diff_kernel<true>, dim3((unsigned)L.nwg), dim3(SPAN_TB), 0, L.stream, L.blocks, L.span, 0.0);
 }
 void launch_visc_deriv(const Launch& L, long long n) {
   if (n <= 0) return;
@@ -690,7 +718,7 @@ void launch_visc_face(const Launch& L, int d, long long n, double cp_prl) {
 }
 void launch_visc_diff(const Launch& L, long long n, double re_inv) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(visc_diff_kernel, dim3((unsigned)L.nwg), dim3(SPAN_TB), 0, L.stream, L.blocks, L.span, re_inv);
+  hipLaunchKernelGGL(diff_kernel<false>, dim3((unsigned)L.nwg), dim3(SPAN_TB), 0, L.stream, L.blocks, L.span, re_inv);
 }
 
 }  // namespace orion_hip
