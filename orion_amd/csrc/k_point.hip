@@ -137,11 +137,16 @@ __global__ void __launch_bounds__(RES_TB) residual_kernel(const BlockDev* __rest
   __shared__ double smax[RES_TB];
   __shared__ long long skey[RES_TB];
   const long long total = off[nb];
-  const long long stride = (long long)gridDim.x * RES_TB;
+  // one contiguous chunk of the interior nodes per workgroup: the block
+  // changes only at block boundaries inside a chunk
+  const long long chunk = ((total + gridDim.x - 1) / gridDim.x + RES_TB - 1) / RES_TB * RES_TB;
+  const long long g0 = (long long)blockIdx.x * chunk;
+  const long long g1 = g0 + chunk < total ? g0 + chunk : total;
   double sum = 0.0;
   MaxLoc best{-1.0, 0x7fffffffffffffffLL};
-  for (long long g = (long long)blockIdx.x * RES_TB + threadIdx.x; g < total; g += stride) {
-    const int bi = find_block(off, nb, g);
+  int bi = find_block(off, nb, g0 < total ? g0 : 0);
+  for (long long g = g0 + threadIdx.x; g < g1; g += RES_TB) {
+    while (g >= off[bi + 1]) ++bi;
     const BlockDev& b = blocks[bi];
     int i, j, k;
     decode_interior(b, g - off[bi], &i, &j, &k);
