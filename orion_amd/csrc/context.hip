@@ -55,6 +55,8 @@ struct orion_hip_ctx {
   int device = 0;
   hipStream_t own_stream = nullptr;
   hipStream_t stream = nullptr;
+  hipStream_t side_stream = nullptr;  // inviscid face fluxes beside the viscous chain (captured step)
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   int nb = 0;
   std::vector<BlockDev> hb;
   BlockDev* db = nullptr;
@@ -134,8 +136,8 @@ static int make_space(orion_hip_ctx* c, Space& s, const std::vector<long long>& 
   return upload_vec(c, &s.d, s.h) || upload_vec(c, &s.wgb, wgb) || upload_vec(c, &s.wg0, wg0);
 }
 
-static Launch LA(orion_hip_ctx* c, const Space& s) {
-  return Launch{c->stream, c->db, s.d, c->nb, Span{s.d, s.wgb, s.wg0, c->nb}, s.nwg};
+static Launch LA(orion_hip_ctx* c, const Space& s, hipStream_t st = nullptr) {
+  return Launch{st ? st : c->stream, c->db, s.d, c->nb, Span{s.d, s.wgb, s.wg0, c->nb}, s.nwg};
 }
 
 // ---------------------------------------------------------------------------
@@ -260,13 +262,14 @@ static void init_params(Params& p, const orion_inputs* in) {
   p.gcl = in->gcl;
 }
 
-static long long arena_fields(const BlockDev& b, int keep_q, long long lines_total) {
+static long long arena_fields(const BlockDev& b, int keep_q, int viscous, long long lines_total) {
   const long long P = b.P;
   long long n = 0;
   auto al = [](long long x) { return (x + 31) / 32 * 32; };  // 256-byte alignment
   n += al(6 * P) + al(9 * P) + al(P) + al(5 * P) + al(5 * P) + al(P) + al(P) + al(P);
   if (keep_q) n += al(5 * P);
   n += al(12 * P) + al(15 * P) + al(30 * lines_total) + al((P + 7) / 8) + al(3 * FG_N * P);
+  if (viscous) n += al(15 * P);
   return n;
 }
 
@@ -565,6 +568,9 @@ void orion_hip_ctx_destroy(orion_hip_ctx* c) {
   if (c->arena) hipFree(c->arena);
   if (c->gexec) hipGraphExecDestroy(c->gexec);
   if (c->graph) hipGraphDestroy(c->graph);
+  if (c->ev_fork) hipEventDestroy(c->ev_fork);
+  if (c->ev_join) hipEventDestroy(c->ev_join);
+  if (c->side_stream) hipStreamDestroy(c->side_stream);
   if (c->own_stream) hipStreamDestroy(c->own_stream);
   delete c;
 }
@@ -581,9 +587,13 @@ int orion_hip_ctx_create(int nblocks, const int* dims, const double* xyz, int nr
   };
   c->device = g_device;
   if (hipSetDevice(c->device) != hipSuccess) return bail(fail("hipSetDevice(%d) failed", c->device));
-  // default-priority stream
-  if (hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess)
+  // default-priority streams
+  if (hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&c->side_stream, hipStreamNonBlocking) != hipSuccess)
     return bail(fail("hipStreamCreate failed"));
+  if (hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess)
+    return bail(fail("hipEventCreate failed"));
   c->stream = c->own_stream;
   c->nb = nblocks;
   c->keep_q = keep_q;
@@ -608,7 +618,7 @@ int orion_hip_ctx_create(int nblocks, const int* dims, const double* xyz, int nr
   }
   // arena
   long long total = 0;
-  for (int b = 0; b < nblocks; ++b) total += arena_fields(c->hb[b], keep_q, lines_tot[b]);
+  for (int b = 0; b < nblocks; ++b) total += arena_fields(c->hb[b], keep_q, c->p.nvis == 1, lines_tot[b]);
   c->arena_bytes = (size_t)total * sizeof(double);
   if (hipMalloc((void**)&c->arena, c->arena_bytes) != hipSuccess)
     return bail(fail("arena hipMalloc(%zu bytes) failed", c->arena_bytes));
@@ -637,6 +647,10 @@ int orion_hip_ctx_create(int nblocks, const int* dims, const double* xyz, int nr
       B.fcov_off[2] = 30 * (nlines(B, 0) + nlines(B, 1));
       B.frozen = (unsigned char*)a; a += al((P + 7) / 8);
       B.fgeo = a; a += al(3 * FG_N * P);
+      // the inviscid fluxes get their own array when a viscous flux runs
+      // beside them; otherwise they reuse fx
+      B.fxi = B.fx;
+      if (c->p.nvis == 1) { B.fxi = a; a += al(15 * P); }
     }
   }
   // regions
@@ -859,11 +873,17 @@ static void op_avg(orion_hip_ctx* c) {
   launch_avg_apply(c->db, c->d_avg_dst, c->n_avg_dst, c->d_stage, c->stream);
 }
 
-static void op_inv(orion_hip_ctx* c) {
+// Inviscid face fluxes and boundary closures: they read the flow state
+// and the static face geometry only, and write fxi and fcov.
+static void op_inv_faces(orion_hip_ctx* c, hipStream_t st) {
   for (int d = 0; d < 3; ++d) {
-    launch_inv_face(LA(c, c->face[d]), d, c->face[d].total(), c->p.gama);
-    launch_inv_closure(LA(c, c->ltask[d]), d, c->ltask[d].total(), c->p.gama);
+    launch_inv_face(LA(c, c->face[d], st), d, c->face[d].total(), c->p.gama);
+    launch_inv_closure(LA(c, c->ltask[d], st), d, c->ltask[d].total(), c->p.gama);
   }
+}
+
+static void op_inv(orion_hip_ctx* c) {
+  op_inv_faces(c, c->stream);
   launch_inv_diff(LA(c, c->dtile), c->dtile.total());
 }
 
@@ -986,10 +1006,21 @@ static void step_body(orion_hip_ctx* c) {
   op_bc(c);
   launch_state(LA(c, c->pad), c->pad.total(), c->p, 3, c->d_dtmin, c->d_step);
   if (c->p.nvis == 1) {
+    // The inviscid face fluxes (FP64-bound) run on the side stream beside
+    // the viscous chain (load-bound); they have their own flux array and
+    // join before the inviscid difference, the first kernel after them
+    // that touches dq.  dq is thus updated in the operators' order.
+    hipEventRecord(c->ev_fork, c->stream);
+    hipStreamWaitEvent(c->side_stream, c->ev_fork, 0);
+    op_inv_faces(c, c->side_stream);
+    hipEventRecord(c->ev_join, c->side_stream);
     op_visc(c);
     op_avg(c);
+    hipStreamWaitEvent(c->stream, c->ev_join, 0);
+    launch_inv_diff(LA(c, c->dtile), c->dtile.total());
+  } else {
+    op_inv(c);
   }
-  op_inv(c);
   op_avg(c);
   op_update(c);
   launch_residual(LA(c, c->intr), c->intr.total(), 5 * c->intr.total(), c->d_part, c->d_pidx, c->nparts, c->d_res,

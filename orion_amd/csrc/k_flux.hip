@@ -232,7 +232,7 @@ __global__ void __launch_bounds__(256) inv_face_kernel(const BlockDev* __restric
     k3[2] = G[2 * b.P];
   }
   face_flux(L, n, f, qwl, qwr, k3, gamma, fo);
-  double* F = b.fx + (long long)d * 5 * b.P + base + (long long)(f + 2) * s;
+  double* F = b.fxi + (long long)d * 5 * b.P + base + (long long)(f + 2) * s;
 #pragma unroll
   for (int m = 0; m < 5; ++m) F[(long long)m * b.P] = fo[m];
 }
@@ -627,19 +627,20 @@ __global__ void __launch_bounds__(256) diff_kernel(const BlockDev* __restrict__ 
   const int i0 = NG + DT * (int)ti, j0 = NG + DT * (int)tj;
   const int lane = threadIdx.x, li = lane % DT, lj = lane / DT;
   const int i = i0 + li, j = j0 + lj;
+  const double* __restrict__ FX = INV ? b.fxi : b.fx;
   for (int e = lane; e < 5 * DT * DW; e += 256) {
     const int m = e / (DT * DW), r = e % (DT * DW);
     {
       const int row = r / DW, col = r % DW;
       const int jj = j0 + row, ii = i0 - 3 + col;
       const bool ok = jj < NG + b.jdim && ii < b.nx;
-      X[m][row][col] = ok ? b.fx[(long long)m * P + pidx(b, ii, jj, k)] : 0.0;
+      X[m][row][col] = ok ? FX[(long long)m * P + pidx(b, ii, jj, k)] : 0.0;
     }
     {
       const int row = r / DT, col = r % DT;
       const int ii = i0 + col, jj = j0 - 3 + row;
       const bool ok = ii < NG + b.idim && jj < b.ny;
-      Y[m][row][col] = ok ? b.fx[(long long)(5 + m) * P + pidx(b, ii, jj, k)] : 0.0;
+      Y[m][row][col] = ok ? FX[(long long)(5 + m) * P + pidx(b, ii, jj, k)] : 0.0;
     }
   }
   __syncthreads();
@@ -670,7 +671,7 @@ __global__ void __launch_bounds__(256) diff_kernel(const BlockDev* __restrict__ 
       } else if (d == 1) {
         fc = diff_node_f([&](int f) { return Y[m][f + 2 - (j0 - 3)][li]; }, n, t);
       } else {
-        const double* F0 = b.fx + (long long)(10 + m) * P + lbase;
+        const double* F0 = FX + (long long)(10 + m) * P + lbase;
         fc = diff_node_f([&](int f) { return F0[(long long)(f + 2) * s]; }, n, t);
       }
       if (INV) {

@@ -35,7 +35,8 @@ struct BlockDev {
   double* mu;            // [P]
   double* q;             // [5][P] conservative (write-only; may be null)
   double* duvwt;         // [12][P] viscous cell derivatives (scratch)
-  double* fx;            // [3][5][P] face fluxes per direction (scratch)
+  double* fx;            // [3][5][P] viscous face fluxes per direction (scratch)
+  double* fxi;           // [3][5][P] inviscid face fluxes (scratch; the same array as fx without viscosity)
   double* fcov;          // closure overrides, [3][lines][6][5] (scratch)
   double* fgeo;          // [3][FG_N][P] face geometry per direction (static, see FG_N)
   unsigned char* frozen; // [P] 1 inside a physical-BC window
