@@ -40,12 +40,14 @@
 // before them (see lusgs_tile_kernel).
 #include "orion_dev.h"
 #include "orion_kernels.h"
+#include <type_traits>
 
 namespace orion_hip {
 
 constexpr int TW = 16;         // tile width along col and row
 constexpr int TL = TW * TW;    // lanes per tile workgroup
 constexpr int KEY_BITS = 16;   // progress flag: epoch << KEY_BITS | plane key
+constexpr int CPOL_SC1 = 16;   // cache-policy operand of a memory builtin: the sc1 bit (gfx950)
 
 __device__ inline double spec_rad(const double* pr, double g, double kx, double ky, double kz) {
   double rho = pr[0], p = pr[4];
@@ -371,11 +373,6 @@ __device__ inline unsigned long long need_key(bool fwd, const SweepTile& u, int 
 // maxima, i.e. not waited on)
 constexpr unsigned waitcnt_vm(unsigned n) { return (n & 15u) | ((n >> 4) << 14) | (7u << 4) | (15u << 8); }
 
-// A cell's own inputs, loaded by the loader lanes: met0..8, coed, r5 (rhs;
-// for the LU backward sweep the cell's dq from the forward sweep); the
-// Gauss-Seidel backward sweep appends the partial sum its forward sweep
-// left (5).
-
 // Sweep layout pair loads: 16-byte loads of two fields of a cell.  (rb, c):
 // the cell's row base q - c and column.
 __device__ __forceinline__ double2 ld_pair(const double2* g, long long rb, int c, int ncp, int K, int k) {
@@ -389,34 +386,23 @@ __device__ __forceinline__ double ld_half(const double2* g, long long rb, int c,
   return reinterpret_cast<const double*>(g)[2 * (rb * K + (long long)k * ncp + c)];
 }
 
-template <int TYPE>
-__device__ __forceinline__ void load_own(const SweepArrays& S, long long rb, int c, int ncp, double* o) {
-  double2 v[10];
-  double last, lastp = 0.0;
-  if constexpr (TYPE == 1) {
-#pragma unroll
-    for (int k = 0; k < 5; ++k) v[k] = ld_pair(S.own, rb, c, ncp, SW_OWN, k);
-#pragma unroll
-    for (int k = 0; k < 2; ++k) v[5 + k] = ld_pair(S.dq, rb, c, ncp, SW_DQ, k);
-    last = ld_half(S.dq, rb, c, ncp, SW_DQ, 2);
-  } else {
-#pragma unroll
-    for (int k = 0; k < 7; ++k) v[k] = ld_pair(S.own, rb, c, ncp, SW_OWN, k);
-    last = ld_half(S.own, rb, c, ncp, SW_OWN, 7);
-  }
-  if constexpr (TYPE == 3) {
-#pragma unroll
-    for (int k = 0; k < 2; ++k) v[7 + k] = ld_pair(S.pre, rb, c, ncp, SW_PRE, k);
-    lastp = ld_half(S.pre, rb, c, ncp, SW_PRE, 2);
-  }
-#pragma unroll
-  for (int i = 0; i < 14; ++i) o[i] = (i & 1) ? v[i >> 1].y : v[i >> 1].x;
-  o[14] = last;
-  if constexpr (TYPE == 3) {
-#pragma unroll
-    for (int i = 0; i < 4; ++i) o[15 + i] = (i & 1) ? v[7 + (i >> 1)].y : v[7 + (i >> 1)].x;
-    o[19] = lastp;
-  }
+// Direct global-to-LDS load of one sweep-layout pair per lane: lane l of
+// the wave writes its 16 bytes to wbase[l], where wbase (the wave's first
+// slot) is the same for every lane of the wave.  Lanes that are switched off
+// write nothing.
+__device__ __forceinline__ void dma_pair(const double2* g, long long rb, int c, int ncp, int K, int k,
+                                         double2* wbase) {
+  __builtin_amdgcn_global_load_lds((const void*)(g + rb * K + (long long)k * ncp + c),
+                                   (__attribute__((address_space(3))) void*)wbase, 16, 0, 0);
+}
+
+// the same with the cache policy of an agent-scope atomic load (sc1: the
+// load is served coherently for the whole device), for values that other
+// workgroups store while this one runs
+__device__ __forceinline__ void dma_pair_coherent(const double2* g, long long rb, int c, int ncp, int K, int k,
+                                                  double2* wbase) {
+  __builtin_amdgcn_global_load_lds((const void*)(g + rb * K + (long long)k * ncp + c),
+                                   (__attribute__((address_space(3))) void*)wbase, 16, 0, CPOL_SC1);
 }
 
 // ---------------------------------------------------------------------------
@@ -446,18 +432,21 @@ __device__ __forceinline__ void load_own(const SweepArrays& S, long long rb, int
 // s(t) = s0 + dir*t:
 //  * update lanes: the cell's inputs, the neighbours' (D, dq) and the
 //    upstream edge values come from LDS; the new dq goes to LDS;
-//  * loader lanes: put the inputs of step t+1 (fetched a step earlier) into
-//    LDS, store the dq of step t-1 (agent scope), fetch the inputs of step
-//    t+2 (for the LU backward sweep they first store the Gauss-Seidel terms
-//    the update lanes left in the own-input slot of step t-1); the first loader wave also checks the upstream progress, fetches
-//    the edge values and reads the upstream flags for the next check, and
-//    its lane 0 publishes;
-//  * one barrier ends the step.
-// Plane s(t) is published at the start of step t+3: each loader wave's
-// stores of it precede the fetch it waits for in step t+2 (the counter is
-// in order), and a barrier follows.  LDS: own inputs by
-// step parity, D by step mod 3 (a step's update reads the D of the step
-// before), dq by parity, edge values by parity.
+//  * loader lanes: store the dq of step t-1 (agent scope; for the LU
+//    backward sweep first the Gauss-Seidel terms the update lanes left in
+//    the own-input slot of step t-1), then load the own inputs of step t+1
+//    and the D of step t straight into LDS (global_load_lds, no registers);
+//    the first loader wave also puts the edge values of step t+1 (fetched a
+//    step earlier) into LDS, checks the upstream progress, fetches the edge
+//    values of step t+2 and reads the upstream flags for the next check,
+//    and its lane 0 publishes;
+//  * each loader wave waits for its direct loads; one barrier ends the step.
+// Plane s(t) is published at the start of step t+2: each loader wave's
+// stores of it precede the direct loads it waited for at the end of step
+// t+1 (the counter is in order), and a barrier followed.  LDS: own inputs
+// by step parity, D by step parity (a step's update reads the D of the step
+// before while the loaders fill the step's own), dq by parity, edge values
+// by parity; own inputs and D are kept as the global pairs.
 template <int TYPE, bool VIS>
 __global__ void __launch_bounds__(2 * TL, 1) lusgs_tile_kernel(SweepArrays S, const BlockDev* __restrict__ blocks,
                                                             const SweepTile* __restrict__ tiles,
@@ -467,11 +456,14 @@ __global__ void __launch_bounds__(2 * TL, 1) lusgs_tile_kernel(SweepArrays S, co
   constexpr bool FWD = (TYPE == 0 || TYPE == 2);
   constexpr int dir = FWD ? 1 : -1;
   constexpr int SL0 = FWD ? 0 : 3;
-  constexpr int NO = TYPE == 3 ? 20 : 15;                     // own input words
-  __shared__ double ownB[2][NO][TL];
-  __shared__ double Dbuf[3][8][TL];
-  __shared__ double dqB[2][5][TL];
-  __shared__ double shX[2][2][NW][TW];
+  constexpr int NO = TYPE == 3 ? 20 : 15;  // own input words
+  constexpr int NOP = TYPE == 3 ? 11 : 8;  // own input pairs
+  __shared__ double2 ownP[2][NOP][TL];
+  __shared__ double2 Dp[2][SW_D][TL];
+  constexpr int NQ = TYPE == 2 ? 10 : 5;  // new dq, then (Gauss-Seidel forward) the partial sums
+  __shared__ double dqB[2][NQ][TL];
+  __shared__ double2 shXD[3][2][TW][SW_D];  // edge D by step mod 3: [role][index][pair]
+  __shared__ double2 shXQ[3][128];           // edge dq by step mod 3: [role][index][pair] (96 used)
 
   const int tix = order[blockIdx.x];
   const SweepTile T = tiles[tix];
@@ -482,6 +474,7 @@ __global__ void __launch_bounds__(2 * TL, 1) lusgs_tile_kernel(SweepArrays S, co
   const bool lw0 = loader && threadIdx.x < TL + 64;  // first loader wave
   const int lane = threadIdx.x & (TL - 1), lc = lane % TW, lr = lane / TW;
   const int xl = threadIdx.x & 63;
+  const int wb = lane & ~63;  // the wave's first pencil: LDS base of its direct loads
   const int c = T.c0 + lc, r = T.r0 + lr;
   const bool lane_ok = c < nc && r < nr;
   const int up_c = FWD ? T.nbr[0] : T.nbr[2], up_r = FWD ? T.nbr[1] : T.nbr[3];
@@ -502,26 +495,30 @@ __global__ void __launch_bounds__(2 * TL, 1) lusgs_tile_kernel(SweepArrays S, co
   auto cell_rb = [&](int t) { return cell_ok(t) ? rb0 + (long long)t * qstep : b.sbase; };
   auto cell_c = [&](int t) { return cell_ok(t) ? c : 0; };
 
-  // edge items of the first loader wave: lane x loads D items x, x + 64 and
-  // dq items x, x + 64, x + 128 (< 160); item e is (role, index, part),
-  // role 0 the cells across the upstream col edge, role 1 the row edge
-  int er[5], ec[5], xo[5];
-  bool ein[5];
+  // Edge values, loaded by the first loader wave straight into LDS: the D
+  // pairs (x, x + 64: role x >> 6, index (x >> 2) & 15, pair x & 3) and the
+  // dq pairs (items x, x + 64; those below 96: role i / 48, index
+  // (i % 48) / 3, pair i % 3) of the cells across the upstream col edge
+  // (role 0) and row edge (role 1).  shXD[b][role][index][pair] and
+  // shXQ[b][item] put lane x of each load at 16 x bytes from the load's
+  // base.
+  int er[4], ec[4];
+  bool ein[4];
+  int ek[4];
 #pragma unroll
-  for (int j = 0; j < 5; ++j) {
-    int role, idx, word;
+  for (int j = 0; j < 4; ++j) {
+    int role, idx;
     bool item = true;
     if (j < 2) {
-      const int e = xl + 64 * j;
-      role = e >> 6;
-      idx = (e >> 2) & 15;
-      word = 2 * (e & 3);
+      role = j;
+      idx = xl >> 2;
+      ek[j] = xl & 3;
     } else {
-      const int e = xl + 64 * (j - 2);
-      item = e < 160;
-      role = e < 80 ? 0 : 1;
-      idx = (e % 80) / 5;
-      word = 8 + e % 5;
+      const int i = xl + 64 * (j - 2);
+      item = i < 96;
+      role = i < 48 ? 0 : 1;
+      idx = (i % 48) / 3;
+      ek[j] = i % 3;
     }
     if (role == 0) {
       ec[j] = FWD ? T.c0 - 1 : T.c0 + TW;
@@ -531,8 +528,9 @@ __global__ void __launch_bounds__(2 * TL, 1) lusgs_tile_kernel(SweepArrays S, co
       ec[j] = T.c0 + idx;
     }
     ein[j] = lw0 && item && er[j] >= 0 && er[j] < nr && ec[j] >= 0 && ec[j] < nc;
-    xo[j] = item ? (role * NW + word) * TW + idx : -1;
   }
+  // the edge cell of item j for step u lies on plane s(u-1); out-of-range
+  // items read the block's first cell, which no lane reads as a neighbour
   auto edge_rb = [&](int j, int u, int* cc) -> long long {
     const int sp = s0 + dir * (u - 1);
     const int mm = sp - er[j] - ec[j];
@@ -540,76 +538,65 @@ __global__ void __launch_bounds__(2 * TL, 1) lusgs_tile_kernel(SweepArrays S, co
     *cc = ok ? ec[j] : 0;
     return ok ? b.sbase + ((long long)sp * nr + er[j]) * ncp : b.sbase;
   };
-
-  // loader registers: the inputs of a later step and the flags read for
-  // the next progress check
-  struct Ring {
-    double o[NO];
-    double D[8];
-    double2 xd[2];
-    double xq[3];
-    unsigned long long pf[2];
-  };
-  // Only lanes with a cell of step u load (the others' LDS entries are
-  // never read as a neighbour that exists); loader waves have no waits
-  // that count on every lane's loads.
-  auto fetch = [&](int u, Ring& R) {
-    if (cell_ok(u)) {
-      const long long rb = rb0 + (long long)u * qstep;
-      load_own<TYPE>(S, rb, c, ncp, R.o);
+  // first loader wave: the edge values of step u into LDS, four loads on
+  // every lane (items 96..127 read the block's first cell into slots no lane
+  // reads); dq with the coherent cache policy, as it was stored by other
+  // tiles during this sweep
+  auto dma_edges = [&](int u) {
+    const int e = u % 3;
 #pragma unroll
-      for (int k = 0; k < SW_D; ++k) {
-        const double2 v = ld_pair(S.D, rb, c, ncp, SW_D, k);
-        R.D[2 * k] = v.x;
-        R.D[2 * k + 1] = v.y;
-      }
+    for (int j = 0; j < 2; ++j) {
+      int cc;
+      const long long rb = edge_rb(j, u, &cc);
+      dma_pair(S.D, rb, cc, ncp, SW_D, ek[j], &shXD[e][j][0][0]);
     }
-    if (lw0) {
 #pragma unroll
-      for (int j = 0; j < 2; ++j) {
-        int cc;
-        const long long rb = edge_rb(j, u, &cc);
-        R.xd[j] = ld_pair(S.D, rb, cc, ncp, SW_D, xl & 3);
-      }
-      const double* dqp = reinterpret_cast<const double*>(S.dq);
-#pragma unroll
-      for (int j = 0; j < 3; ++j) {
-        int cc;
-        const long long rb = edge_rb(2 + j, u, &cc);
-        R.xq[j] = __hip_atomic_load(&dqp[sweep_field(rb + cc, cc, ncp, SW_DQ, (xl + 64 * j) % 5)], __ATOMIC_RELAXED,
-                                    __HIP_MEMORY_SCOPE_AGENT);
-      }
+    for (int j = 2; j < 4; ++j) {
+      int cc;
+      const long long rb = edge_rb(j, u, &cc);
+      dma_pair_coherent(S.dq, rb, cc, ncp, SW_DQ, ek[j], &shXQ[e][64 * (j - 2)]);
     }
   };
-  auto deposit = [&](int u, const Ring& R) {
+  // the own inputs of the cell of step u into ownP[u & 1], as pairs: met0..8,
+  // coed, r5 (rhs; for the LU backward sweep the cell's dq from the forward
+  // sweep), then for the Gauss-Seidel backward sweep the partial sum its
+  // forward sweep left
+  auto dma_own = [&](int u) {
+    const long long rb = rb0 + (long long)u * qstep;
+    double2(*dst)[TL] = ownP[u & 1];
+    if constexpr (TYPE == 1) {
 #pragma unroll
-    for (int f = 0; f < NO; ++f) ownB[u & 1][f][lane] = R.o[f];
+      for (int k = 0; k < 5; ++k) dma_pair(S.own, rb, c, ncp, SW_OWN, k, &dst[k][wb]);
 #pragma unroll
-    for (int f = 0; f < 8; ++f) Dbuf[u % 3][f][lane] = R.D[f];
-    if (lw0) {
-      double* x = &shX[u & 1][0][0][0];
-      x[xo[0]] = R.xd[0].x;
-      x[xo[0] + TW] = R.xd[0].y;
-      x[xo[1]] = R.xd[1].x;
-      x[xo[1] + TW] = R.xd[1].y;
-      x[xo[2]] = R.xq[0];
-      x[xo[3]] = R.xq[1];
-      if (xo[4] >= 0) x[xo[4]] = R.xq[2];
+      for (int k = 0; k < 3; ++k) dma_pair(S.dq, rb, c, ncp, SW_DQ, k, &dst[5 + k][wb]);
+    } else {
+#pragma unroll
+      for (int k = 0; k < SW_OWN; ++k) dma_pair(S.own, rb, c, ncp, SW_OWN, k, &dst[k][wb]);
+    }
+    if constexpr (TYPE == 3) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) dma_pair(S.pre, rb, c, ncp, SW_PRE, k, &dst[SW_OWN + k][wb]);
     }
   };
-  // loader lane 0: the flags for the next check, always two loads (a
-  // missing upstream reads this tile's own flag and is never checked)
+  auto dma_D = [&](int u) {
+    const long long rb = rb0 + (long long)u * qstep;
+#pragma unroll
+    for (int k = 0; k < SW_D; ++k) dma_pair(S.D, rb, c, ncp, SW_D, k, &Dp[u & 1][k][wb]);
+  };
+  // first loader wave: the flags for the next check, two loads on every
+  // lane (a missing upstream reads this tile's own flag and is never
+  // checked)
   auto read_up = [&](unsigned long long* pf) {
-    if (!(lw0 && xl == 0)) return;
     pf[0] = __hip_atomic_load(&flags[(long long)(up_c >= 0 ? up_c : tix) * FLAG_STRIDE], __ATOMIC_RELAXED,
                               __HIP_MEMORY_SCOPE_AGENT);
     pf[1] = __hip_atomic_load(&flags[(long long)(up_r >= 0 ? up_r : tix) * FLAG_STRIDE], __ATOMIC_RELAXED,
                               __HIP_MEMORY_SCOPE_AGENT);
   };
-  // loader lane 0: the upstream tiles must have finished plane s; pf holds
-  // flag values read a step earlier, re-read (bounded spin) only when short
-  auto ensure_up = [&](int s, const unsigned long long* pf) {
-    if (!(lw0 && xl == 0)) return;
+  // first loader wave: the upstream tiles must have finished plane s; pf
+  // holds flag values read a step earlier, re-read (bounded spin) only when
+  // short.  Every lane of the wave runs the check (all read the same flag
+  // words), so the flag registers are consumed on every path.
+  auto ensure_up = [&](int s, unsigned long long* pf) {
     for (int k = 0; k < 2; ++k) {
       const int u = k ? up_r : up_c;
       if (u < 0) continue;
@@ -623,12 +610,13 @@ __global__ void __launch_bounds__(2 * TL, 1) lusgs_tile_kernel(SweepArrays S, co
         // another tile already gave up: report and carry on, never hang
         if (++spins > (1LL << 20) ||
             ((spins & 255) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)) {
-          atomicOr(err, 1);
+          if (xl == 0) atomicOr(err, 1);
           break;
         }
         if (spins > 1) __builtin_amdgcn_s_sleep(2);
         v = __hip_atomic_load(&flags[(long long)u * FLAG_STRIDE], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
+      pf[k] = v;
     }
   };
 
@@ -643,11 +631,16 @@ __global__ void __launch_bounds__(2 * TL, 1) lusgs_tile_kernel(SweepArrays S, co
     const bool inside = march || (pos + d >= 0 && pos + d < TW);
     const int ln = march ? lane : lane + (col ? d : d * TW);
     const int role = col ? 0 : 1, idx = col ? lr : lc;
-    const double* bd = inside ? &Dbuf[(t + 2) % 3][0][ln] : &shX[p][role][0][idx];
-    const double* bq = inside ? &dqB[p ^ 1][0][ln] : &shX[p][role][8][idx];
-    const int st = inside ? TL : TW;
+    const int e = t % 3;
+    const double2* bd = inside ? &Dp[p ^ 1][0][ln] : &shXD[e][role][idx][0];
+    const double* bq = inside ? &dqB[p ^ 1][0][ln] : reinterpret_cast<const double*>(&shXQ[e][(role * TW + idx) * 3]);
+    const int st = inside ? TL : 1;
 #pragma unroll
-    for (int f = 0; f < 8; ++f) v[f] = bd[f * st];
+    for (int k = 0; k < SW_D; ++k) {
+      const double2 x = bd[k * st];
+      v[2 * k] = x.x;
+      v[2 * k + 1] = x.y;
+    }
 #pragma unroll
     for (int f = 0; f < 5; ++f) v[8 + f] = bq[f * st];
     return march ? (m + d >= 0 && m + d < nm) : (coord + d >= 0 && coord + d < lim);
@@ -667,33 +660,27 @@ __global__ void __launch_bounds__(2 * TL, 1) lusgs_tile_kernel(SweepArrays S, co
     }
   };
 
-  // one step of a loader lane
-  auto lstep = [&](int t, Ring& Rcur, Ring& Rprev) __attribute__((always_inline)) {
+  // one step of a loader lane; W0: the first loader wave, which runs its
+  // own loop and also keeps the tile's progress: it publishes, checks the
+  // upstream flags it read in the step before, and loads the edge values of
+  // step t+2 last, so that they stay in flight across the barrier.
+  auto lstep = [&](auto w0, int t, unsigned long long* pf) __attribute__((always_inline)) {
+    constexpr bool W0 = decltype(w0)::value;
     const int s = s0 + dir * t;
-    // plane s(t-3): every loader wave's stores of it preceded the fetch that
-    // the wave put into LDS in step t-1, and a barrier followed
-    if (lw0 && xl == 0 && t >= 3 && t - 3 < nsteps)
-      __hip_atomic_store(&flags[(long long)tix * FLAG_STRIDE], ep + plane_key(FWD, s - 3 * dir), __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
+    if constexpr (W0) {
+      // plane s(t-2): every loader wave's stores of it preceded the direct
+      // loads that the wave waited for before the barrier ending step t-1
+      if (xl == 0 && t >= 2 && t - 2 < nsteps)
+        __hip_atomic_store(&flags[(long long)tix * FLAG_STRIDE], ep + plane_key(FWD, s - 2 * dir), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+      if (t + 2 < nsteps) ensure_up(s + dir, pf);  // the edge values of step t+2 lie on plane s(t+1)
+    }
     if (TYPE == 1 && cell_ok(t - 1)) {  // the terms the update lanes left in the own-input slot of step t-1
       double2* gp = S.ga + (rb0 + (long long)(t - 1) * qstep) * SW_GA + c;
       const int q = (t - 1) & 1;
 #pragma unroll
-      for (int k = 0; k < SW_GA; ++k)
-        gp[(long long)k * ncp] = make_double2(ownB[q][2 * k][lane], 2 * k + 1 < 15 ? ownB[q][2 * k + 1][lane] : 0.0);
+      for (int k = 0; k < SW_GA; ++k) gp[(long long)k * ncp] = ownP[q][k][lane];
     }
-    deposit(t + 1, Rprev);
-    // the fetch just put into LDS has arrived, and with it (in-order
-    // counter) this wave's stores of step t-2, issued before it; only the
-    // flag loads issued after it may be outstanding
-    asm volatile("" ::: "memory");
-    __builtin_amdgcn_sched_barrier(0);
-    if (lw0)
-      __builtin_amdgcn_s_waitcnt(waitcnt_vm(2));
-    else
-      __builtin_amdgcn_s_waitcnt(waitcnt_vm(0));
-    __builtin_amdgcn_sched_barrier(0);
-    if (t + 2 < nsteps) ensure_up(s + dir, Rprev.pf);  // the edge values of step t+2 lie on plane s(t+1)
     if (cell_ok(t - 1)) {
       const long long q = rb0 + (long long)(t - 1) * qstep + c;
       double* dqp = reinterpret_cast<double*>(S.dq);
@@ -701,12 +688,43 @@ __global__ void __launch_bounds__(2 * TL, 1) lusgs_tile_kernel(SweepArrays S, co
       for (int f = 0; f < 5; ++f)
         __hip_atomic_store(&dqp[sweep_field(q, c, ncp, SW_DQ, f)], dqB[(t - 1) & 1][f][lane], __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
+      if constexpr (TYPE == 2) {  // the partial sums the update lanes left beside the new dq
+        double2* pp = S.pre + (q - c) * SW_PRE + c;
+        const double* w = &dqB[(t - 1) & 1][5][lane];
+        pp[0] = make_double2(w[0], w[TL]);
+        pp[ncp] = make_double2(w[2 * TL], w[3 * TL]);
+        pp[2 * ncp] = make_double2(w[4 * TL], 0.0);
+      }
     }
+    // the stores above took their values from LDS before they issued, so
+    // the direct loads may now refill ownP[(t+1)&1]
     asm volatile("" ::: "memory");
     __builtin_amdgcn_sched_barrier(0);
-    fetch(t + 2, Rcur);
-    read_up(Rcur.pf);
-    __syncthreads();
+    if constexpr (W0) read_up(pf);  // checked at the next step
+    if (cell_ok(t + 1)) dma_own(t + 1);
+    if (cell_ok(t)) dma_D(t);
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_sched_barrier(0);
+    if constexpr (W0) {
+      dma_edges(t + 2);
+      asm volatile("" ::: "memory");
+      __builtin_amdgcn_sched_barrier(0);
+      // everything but the four edge loads just issued has completed: the
+      // stores, the flags and the direct loads of step t+1 (and the edge
+      // loads of step t+1, issued a step earlier); LDS accesses too.  The
+      // barrier carries no fence, which would wait for the edge loads.
+      __builtin_amdgcn_s_waitcnt(waitcnt_vm(4) & ~(15u << 8));
+      __builtin_amdgcn_sched_barrier(0);
+      __builtin_amdgcn_s_barrier();
+      asm volatile("" ::: "memory");
+      __builtin_amdgcn_sched_barrier(0);
+    } else {
+      // the direct loads have reached LDS, and with them (in-order counter)
+      // every load and store the wave issued before them
+      __builtin_amdgcn_s_waitcnt(waitcnt_vm(0));
+      __builtin_amdgcn_sched_barrier(0);
+      __syncthreads();
+    }
   };
 
   // one step of an update lane
@@ -716,11 +734,24 @@ __global__ void __launch_bounds__(2 * TL, 1) lusgs_tile_kernel(SweepArrays S, co
       const bool act = cell_ok(t);
       double o[NO];
 #pragma unroll
-      for (int f = 0; f < NO; ++f) o[f] = ownB[t & 1][f][lane];
+      for (int k = 0; k < SW_OWN; ++k) {
+        const double2 x = ownP[t & 1][k][lane];
+        o[2 * k] = x.x;
+        if (2 * k + 1 < 15) o[2 * k + 1] = x.y;
+      }
+      if constexpr (TYPE == 3) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          const double2 x = ownP[t & 1][SW_OWN + k][lane];
+          o[15 + 2 * k] = x.x;
+          if (2 * k + 1 < 5) o[15 + 2 * k + 1] = x.y;
+        }
+      }
       double r0[5];
 #pragma unroll
       for (int f = 0; f < 5; ++f) r0[f] = TYPE == 3 ? o[15 + f] : 0.0;
-      double gout[15];
+      double gout[16];
+      gout[15] = 0.0;
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
         AxisGeo ag;
@@ -773,13 +804,11 @@ __global__ void __launch_bounds__(2 * TL, 1) lusgs_tile_kernel(SweepArrays S, co
       }
       if (TYPE == 1) {  // into the lane's own-input slot (read above); the loaders store it
 #pragma unroll
-        for (int f = 0; f < 15; ++f) ownB[t & 1][f][lane] = gout[f];
+        for (int k = 0; k < SW_GA; ++k) ownP[t & 1][k][lane] = make_double2(gout[2 * k], gout[2 * k + 1]);
       }
-      if (act && TYPE == 2) {
-        double2* pp = S.pre + (rb0 + (long long)t * qstep) * SW_PRE + c;
-        pp[0] = make_double2(pre5[0], pre5[1]);
-        pp[ncp] = make_double2(pre5[2], pre5[3]);
-        pp[2 * ncp] = make_double2(pre5[4], 0.0);
+      if constexpr (TYPE == 2) {  // the loaders store them with the dq
+#pragma unroll
+        for (int f = 0; f < 5; ++f) dqB[t & 1][5 + f][lane] = pre5[f];
       }
       load_ga(t + 1, gnext);
     }
@@ -788,28 +817,31 @@ __global__ void __launch_bounds__(2 * TL, 1) lusgs_tile_kernel(SweepArrays S, co
 
   // Both kinds of lanes run their own loop with one barrier per step (and
   // one after the set-up), so every barrier is reached by all waves.
-  const int TT = (nsteps + 4) & ~1;  // whole pairs of steps, at least nsteps + 3
+  const int TT = (nsteps + 3) & ~1;  // whole pairs of steps, at least nsteps + 2
   if (loader) {
-    Ring RA, RB;
-    {
-      const unsigned long long none[2] = {0, 0};
-      ensure_up(s0, none);  // covers the edge values of steps 0 and 1
+    unsigned long long pf[2] = {0, 0};
+    if (lw0) {
+      ensure_up(s0 - dir, pf);  // the edge values of step 0 lie on plane s(-1)
+      dma_edges(0);
+      ensure_up(s0, pf);  // those of step 1 on plane s(0)
+      dma_edges(1);
+      read_up(pf);
     }
-    read_up(RB.pf);
-    fetch(0, RA);
-    deposit(0, RA);
-    fetch(1, RB);
+    if (cell_ok(0)) dma_own(0);
+    __builtin_amdgcn_s_waitcnt(waitcnt_vm(0));
     __syncthreads();
-    for (int t = 0; t < TT; t += 2) {
-      lstep(t, RA, RB);
-      lstep(t + 1, RB, RA);
+    if (lw0) {
+      for (int t = 0; t < TT; ++t) lstep(std::true_type{}, t, pf);
+    } else {
+      for (int t = 0; t < TT; ++t) lstep(std::false_type{}, t, pf);
     }
+    __builtin_amdgcn_s_waitcnt(waitcnt_vm(0));
   } else {
     double gaR[2][TYPE == 2 ? 15 : 1];  // Gauss-Seidel forward: ga by step parity
 #pragma unroll
     for (int f = 0; f < 5; ++f) dqB[1][f][lane] = 0.0;
 #pragma unroll
-    for (int f = 0; f < 8; ++f) Dbuf[2][f][lane] = 0.0;
+    for (int k = 0; k < SW_D; ++k) Dp[1][k][lane] = make_double2(0.0, 0.0);
     load_ga(0, gaR[0]);
     __syncthreads();
     for (int t = 0; t < TT; t += 2) {
