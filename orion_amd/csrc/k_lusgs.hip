@@ -456,15 +456,19 @@ __global__ void __launch_bounds__(2 * TL, 1) lusgs_tile_kernel(SweepArrays S, co
   constexpr bool FWD = (TYPE == 0 || TYPE == 2);
   constexpr int dir = FWD ? 1 : -1;
   constexpr int SL0 = FWD ? 0 : 3;
-  constexpr int NO = TYPE == 3 ? 20 : 15;  // own input words
-  constexpr int NOP = TYPE == 3 ? 11 : 8;  // own input pairs
-  __shared__ double2 ownP[2][NOP][TL];
+  constexpr int NO = 15;  // own input words
+  // own inputs loaded two steps ahead (three buffers), except in the
+  // Gauss-Seidel forward sweep, whose LDS holds one buffer less
+  constexpr bool OWN2 = TYPE != 2;
+  constexpr int NOB = OWN2 ? 3 : 2;
+  __shared__ double2 ownP[NOB][SW_OWN][TL];
   __shared__ double2 Dp[2][SW_D][TL];
   constexpr int NQ = TYPE == 2 ? 10 : 5;  // new dq, then (Gauss-Seidel forward) the partial sums
   __shared__ double dqB[2][NQ][TL];
   __shared__ double2 shXD[3][2][TW][SW_D];  // edge D by step mod 3: [role][index][pair]
   __shared__ double2 shXQ[3][128];           // edge dq by step mod 3: [role][index][pair] (96 used)
 
+  auto ob = [](int u) { return OWN2 ? (u + 3) % 3 : u & 1; };  // own buffer of step u >= -1
   const int tix = order[blockIdx.x];
   const SweepTile T = tiles[tix];
   const BlockDev& b = blocks[T.block];
@@ -557,25 +561,22 @@ __global__ void __launch_bounds__(2 * TL, 1) lusgs_tile_kernel(SweepArrays S, co
       dma_pair_coherent(S.dq, rb, cc, ncp, SW_DQ, ek[j], &shXQ[e][64 * (j - 2)]);
     }
   };
-  // the own inputs of the cell of step u into ownP[u & 1], as pairs: met0..8,
-  // coed, r5 (rhs; for the LU backward sweep the cell's dq from the forward
-  // sweep), then for the Gauss-Seidel backward sweep the partial sum its
-  // forward sweep left
+  // the own inputs of the cell of step u into ownP[ob(u)], as pairs:
+  // met0..8, coed, r5 (rhs; for the LU backward sweep the cell's dq from the
+  // forward sweep).  Eight loads on every lane: a lane without a cell reads
+  // the block's first cell into its own slot, which nothing reads.
   auto dma_own = [&](int u) {
-    const long long rb = rb0 + (long long)u * qstep;
-    double2(*dst)[TL] = ownP[u & 1];
+    const long long rb = cell_rb(u);
+    const int cc = cell_c(u);
+    double2(*dst)[TL] = ownP[ob(u)];
     if constexpr (TYPE == 1) {
 #pragma unroll
-      for (int k = 0; k < 5; ++k) dma_pair(S.own, rb, c, ncp, SW_OWN, k, &dst[k][wb]);
+      for (int k = 0; k < 5; ++k) dma_pair(S.own, rb, cc, ncp, SW_OWN, k, &dst[k][wb]);
 #pragma unroll
-      for (int k = 0; k < 3; ++k) dma_pair(S.dq, rb, c, ncp, SW_DQ, k, &dst[5 + k][wb]);
+      for (int k = 0; k < 3; ++k) dma_pair(S.dq, rb, cc, ncp, SW_DQ, k, &dst[5 + k][wb]);
     } else {
 #pragma unroll
-      for (int k = 0; k < SW_OWN; ++k) dma_pair(S.own, rb, c, ncp, SW_OWN, k, &dst[k][wb]);
-    }
-    if constexpr (TYPE == 3) {
-#pragma unroll
-      for (int k = 0; k < 3; ++k) dma_pair(S.pre, rb, c, ncp, SW_PRE, k, &dst[SW_OWN + k][wb]);
+      for (int k = 0; k < SW_OWN; ++k) dma_pair(S.own, rb, cc, ncp, SW_OWN, k, &dst[k][wb]);
     }
   };
   auto dma_D = [&](int u) {
@@ -646,7 +647,20 @@ __global__ void __launch_bounds__(2 * TL, 1) lusgs_tile_kernel(SweepArrays S, co
     return march ? (m + d >= 0 && m + d < nm) : (coord + d >= 0 && coord + d < lim);
   };
 
+  // update lanes, a step ahead into registers: the Gauss-Seidel forward
+  // sweep's ga terms, the Gauss-Seidel backward sweep's starting sums
   auto load_ga = [&](int u, double* g) {
+    if constexpr (TYPE == 3) {
+      const long long rb = cell_rb(u);
+      const int cc = cell_c(u);
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        const double2 v = ld_pair(S.pre, rb, cc, ncp, SW_PRE, k);
+        g[2 * k] = v.x;
+        g[2 * k + 1] = v.y;
+      }
+      g[4] = ld_half(S.pre, rb, cc, ncp, SW_PRE, 2);
+    }
     if constexpr (TYPE == 2) {
       const long long rb = cell_rb(u);
       const int cc = cell_c(u);
@@ -677,7 +691,7 @@ __global__ void __launch_bounds__(2 * TL, 1) lusgs_tile_kernel(SweepArrays S, co
     }
     if (TYPE == 1 && cell_ok(t - 1)) {  // the terms the update lanes left in the own-input slot of step t-1
       double2* gp = S.ga + (rb0 + (long long)(t - 1) * qstep) * SW_GA + c;
-      const int q = (t - 1) & 1;
+      const int q = ob(t - 1);
 #pragma unroll
       for (int k = 0; k < SW_GA; ++k) gp[(long long)k * ncp] = ownP[q][k][lane];
     }
@@ -697,30 +711,35 @@ __global__ void __launch_bounds__(2 * TL, 1) lusgs_tile_kernel(SweepArrays S, co
       }
     }
     // the stores above took their values from LDS before they issued, so
-    // the direct loads may now refill ownP[(t+1)&1]
+    // the direct loads may now refill the own buffer of step t-1
     asm volatile("" ::: "memory");
     __builtin_amdgcn_sched_barrier(0);
     if constexpr (W0) read_up(pf);  // checked at the next step
-    if (cell_ok(t + 1)) dma_own(t + 1);
     if (cell_ok(t)) dma_D(t);
+    if constexpr (OWN2) {
+      dma_own(t + 2);
+    } else {
+      if (cell_ok(t + 1)) dma_own(t + 1);
+    }
     asm volatile("" ::: "memory");
     __builtin_amdgcn_sched_barrier(0);
-    if constexpr (W0) {
-      dma_edges(t + 2);
-      asm volatile("" ::: "memory");
-      __builtin_amdgcn_sched_barrier(0);
-      // everything but the four edge loads just issued has completed: the
-      // stores, the flags and the direct loads of step t+1 (and the edge
-      // loads of step t+1, issued a step earlier); LDS accesses too.  The
-      // barrier carries no fence, which would wait for the edge loads.
-      __builtin_amdgcn_s_waitcnt(waitcnt_vm(4) & ~(15u << 8));
+    if constexpr (W0) dma_edges(t + 2);
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_sched_barrier(0);
+    // Loads that may stay in flight across the barrier: the own inputs of
+    // step t+2 (eight loads) and the first wave's edge values of step t+2
+    // (four), the last issued.  Everything before them has completed (the
+    // counter is in order): the stores, the flags, the D of step t and the
+    // own inputs of step t+1 (issued a step earlier); LDS accesses too.  The
+    // barrier carries no fence, which would wait for every direct load.
+    constexpr unsigned NFLY = (OWN2 ? SW_OWN : 0) + (W0 ? 4 : 0);
+    if constexpr (NFLY > 0) {
+      __builtin_amdgcn_s_waitcnt(waitcnt_vm(NFLY) & ~(15u << 8));
       __builtin_amdgcn_sched_barrier(0);
       __builtin_amdgcn_s_barrier();
       asm volatile("" ::: "memory");
       __builtin_amdgcn_sched_barrier(0);
     } else {
-      // the direct loads have reached LDS, and with them (in-order counter)
-      // every load and store the wave issued before them
       __builtin_amdgcn_s_waitcnt(waitcnt_vm(0));
       __builtin_amdgcn_sched_barrier(0);
       __syncthreads();
@@ -735,21 +754,13 @@ __global__ void __launch_bounds__(2 * TL, 1) lusgs_tile_kernel(SweepArrays S, co
       double o[NO];
 #pragma unroll
       for (int k = 0; k < SW_OWN; ++k) {
-        const double2 x = ownP[t & 1][k][lane];
+        const double2 x = ownP[ob(t)][k][lane];
         o[2 * k] = x.x;
         if (2 * k + 1 < 15) o[2 * k + 1] = x.y;
       }
-      if constexpr (TYPE == 3) {
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-          const double2 x = ownP[t & 1][SW_OWN + k][lane];
-          o[15 + 2 * k] = x.x;
-          if (2 * k + 1 < 5) o[15 + 2 * k + 1] = x.y;
-        }
-      }
       double r0[5];
 #pragma unroll
-      for (int f = 0; f < 5; ++f) r0[f] = TYPE == 3 ? o[15 + f] : 0.0;
+      for (int f = 0; f < 5; ++f) r0[f] = TYPE == 3 ? gcur[f] : 0.0;
       double gout[16];
       gout[15] = 0.0;
 #pragma unroll
@@ -804,7 +815,7 @@ __global__ void __launch_bounds__(2 * TL, 1) lusgs_tile_kernel(SweepArrays S, co
       }
       if (TYPE == 1) {  // into the lane's own-input slot (read above); the loaders store it
 #pragma unroll
-        for (int k = 0; k < SW_GA; ++k) ownP[t & 1][k][lane] = make_double2(gout[2 * k], gout[2 * k + 1]);
+        for (int k = 0; k < SW_GA; ++k) ownP[ob(t)][k][lane] = make_double2(gout[2 * k], gout[2 * k + 1]);
       }
       if constexpr (TYPE == 2) {  // the loaders store them with the dq
 #pragma unroll
@@ -827,7 +838,8 @@ __global__ void __launch_bounds__(2 * TL, 1) lusgs_tile_kernel(SweepArrays S, co
       dma_edges(1);
       read_up(pf);
     }
-    if (cell_ok(0)) dma_own(0);
+    dma_own(0);
+    if constexpr (OWN2) dma_own(1);
     __builtin_amdgcn_s_waitcnt(waitcnt_vm(0));
     __syncthreads();
     if (lw0) {
@@ -837,7 +849,7 @@ __global__ void __launch_bounds__(2 * TL, 1) lusgs_tile_kernel(SweepArrays S, co
     }
     __builtin_amdgcn_s_waitcnt(waitcnt_vm(0));
   } else {
-    double gaR[2][TYPE == 2 ? 15 : 1];  // Gauss-Seidel forward: ga by step parity
+    double gaR[2][TYPE == 2 ? 15 : TYPE == 3 ? 5 : 1];  // ga (GS forward) or starting sums (GS backward) by step parity
 #pragma unroll
     for (int f = 0; f < 5; ++f) dqB[1][f][lane] = 0.0;
 #pragma unroll
