@@ -40,7 +40,7 @@ STAGES = {
     "lu_bwd": (25, ["lu_bwd"], "lusgs_tile_kernel<1,true>"),
     "gs_fwd": (30, ["gs_fwd"], "lusgs_tile_kernel<2,true>"),
     "gs_bwd": (30, ["gs_bwd"], "lusgs_tile_kernel<3,true>"),
-    "update": (25, ["update", "residual"], "update_kernel+residual_kernel+residual_final_kernel"),
+    "update": (25, ["update", "residual"], "update_kernel+residual_final_kernel"),
 }
 STEP_WORDS = sum(v[0] for v in STAGES.values())
 # The dominant kernel: the LU-SGS tile sweep, four launches per step (LU

@@ -87,6 +87,8 @@ struct orion_hip_ctx {
   double* d_stage = nullptr;
   double* d_part = nullptr;
   long long* d_pidx = nullptr;
+  double* d_upart = nullptr;      // residual partials of the update kernel: sums, then maxima
+  long long* d_ukey = nullptr;    // and the maxima's keys (one per update workgroup)
   int nparts = 1;
   double* d_res = nullptr;  // 4 doubles per residual slot
   int res_cap = 0;
@@ -748,7 +750,10 @@ int orion_hip_ctx_create(int nblocks, const int* dims, const double* xyz, int nr
   if (upload_vec(c, &c->db, c->hb)) return bail(-1);
   if (build_topology(c, R)) return bail(-1);
   c->nparts = residual_parts(c->intr.total());
+  if (c->ptile.nwg >= (1LL << 31)) return bail(fail("too many update workgroups"));
   if (dalloc(c, (void**)&c->d_part, sizeof(double) * 2 * 1024) || dalloc(c, (void**)&c->d_pidx, sizeof(long long) * 1024) ||
+      dalloc(c, (void**)&c->d_upart, sizeof(double) * 2 * c->ptile.nwg) ||
+      dalloc(c, (void**)&c->d_ukey, sizeof(long long) * c->ptile.nwg) ||
       dalloc(c, (void**)&c->d_count, sizeof(unsigned long long) * 2) || dalloc(c, (void**)&c->d_step, sizeof(int)))
     return bail(-1);
   if (const char* e = std::getenv("ORION_HIP_GRAPH")) c->use_graph = std::atoi(e) != 0;
@@ -896,7 +901,14 @@ static void op_sweep(orion_hip_ctx* c, int type) {
 static void op_update(orion_hip_ctx* c) {
   launch_lu_pack(LA(c, c->ptile), c->ptile.total(), c->sw, c->p);
   for (int t = 0; t < 4; ++t) op_sweep(c, t);
-  launch_update(LA(c, c->ptile), c->ptile.total(), c->sw, c->p.gama);
+  launch_update(LA(c, c->ptile), c->ptile.total(), c->sw, c->p.gama, c->intr.d, c->d_upart,
+                c->d_upart + c->ptile.nwg, c->d_ukey);
+}
+
+// the residual of the update op_update just made, from its partials
+static void op_update_residual(orion_hip_ctx* c, double* out, const int* step) {
+  launch_residual_final(c->d_upart, c->d_upart + c->ptile.nwg, c->d_ukey, (int)c->ptile.nwg, 5 * c->intr.total(), out,
+                        step, c->stream);
 }
 
 // a sweep whose wait for an upstream tile ran out has left wrong values
@@ -1023,8 +1035,7 @@ static void step_body(orion_hip_ctx* c) {
   }
   op_avg(c);
   op_update(c);
-  launch_residual(LA(c, c->intr), c->intr.total(), 5 * c->intr.total(), c->d_part, c->d_pidx, c->nparts, c->d_res,
-                  c->d_step);
+  op_update_residual(c, c->d_res, c->d_step);
   launch_step_advance(c->d_step, c->stream);
 }
 
@@ -1118,10 +1129,11 @@ int orion_hip_profile_run(orion_hip_ctx* c, int nsteps, double* phase_ms, long l
     op_avg(c); mark(2);
     launch_lu_pack(LA(c, c->ptile), c->ptile.total(), c->sw, c->p); mark(1);
     for (int t = 0; t < 4; ++t) { op_sweep(c, t); mark(2); }
-    launch_update(LA(c, c->ptile), c->ptile.total(), c->sw, c->p.gama); mark(1);
-    launch_residual(LA(c, c->intr), c->intr.total(), 5 * c->intr.total(), c->d_part, c->d_pidx, c->nparts,
-                    c->d_res + 4 * s, nullptr);
-    mark(2);
+    launch_update(LA(c, c->ptile), c->ptile.total(), c->sw, c->p.gama, c->intr.d, c->d_upart,
+                  c->d_upart + c->ptile.nwg, c->d_ukey);
+    mark(1);
+    op_update_residual(c, c->d_res + 4 * s, nullptr);
+    mark(1);
     if (check_launch("profile_run")) return -1;
     HIPCHK(hipEventSynchronize(ev[k]));
     for (int i = 0; i < k; ++i) {

@@ -236,13 +236,28 @@ __global__ void __launch_bounds__(256) lu_pack_kernel(const BlockDev* __restrict
 
 // dq back to the block arrays (zero on physical-BC windows,
 // StateUpdater.cpp:577) and update_limited (:470-542).
+// The update also forms the residual's terms (ResidualMonitor.cpp, as in
+// residual_kernel) of its cells from the dq it stores: each workgroup leaves
+// the sum of squares and the largest term with its key (interior index x 5
+// + component) in part[blockIdx.x], for residual_final_kernel.
 __global__ void __launch_bounds__(256) update_kernel(const BlockDev* __restrict__ blocks,
                                                      Span SP, SweepArrays S,
-                                                     double gama) {
+                                                     double gama, const long long* __restrict__ ioff,
+                                                     double* psum, double* pmax, long long* pkey) {
   __shared__ double sh[5][PT * PT];
+  __shared__ double ssum[PT * PT];
+  __shared__ double smax[PT * PT];
+  __shared__ long long skey[PT * PT];
   int bi;
   long long gi;
-  if (!span_item(SP, &bi, &gi)) return;  // whole tiles
+  if (!span_item(SP, &bi, &gi)) {  // whole tiles: the workgroup has no tile
+    if (threadIdx.x == 0) {
+      psum[blockIdx.x] = 0.0;
+      pmax[blockIdx.x] = -1.0;
+      pkey[blockIdx.x] = 0x7fffffffffffffffLL;
+    }
+    return;
+  }
   const BlockDev& b = blocks[bi];
   int m0, c0, r;
   pack_tile(b, gi, &m0, &c0, &r);
@@ -264,41 +279,82 @@ __global__ void __launch_bounds__(256) update_kernel(const BlockDev* __restrict_
   }
   __syncthreads();
   const int m = m0 + lane / PT, c = c0 + lane % PT;
-  if (!(m < b.sn[0] && c < b.sn[1])) return;
-  const long long id = natural_id(b, m, r, c);
-  const long long P = b.P;
-  double d[5];
-  const bool fz = b.frozen[id] != 0;
+  double rsum = 0.0, rmax = -1.0;
+  long long rkey = 0x7fffffffffffffffLL;
+  if (m < b.sn[0] && c < b.sn[1]) {
+    const long long id = natural_id(b, m, r, c);
+    const long long P = b.P;
+    double d[5];
+    const bool fz = b.frozen[id] != 0;
 #pragma unroll
-  for (int f = 0; f < 5; ++f) {
-    d[f] = fz ? 0.0 : sh[f][lane];
-    b.dq[f * P + id] = d[f];
+    for (int f = 0; f < 5; ++f) {
+      d[f] = fz ? 0.0 : sh[f][lane];
+      b.dq[f * P + id] = d[f];
+    }
+    {  // the residual's terms |dq| / (dt vol), same clamps as residual_kernel
+      double vol = b.vol[id], dt = b.dt[id];
+      if (vol < 1e-30) vol = 1e-30;
+      if (dt < 1e-30) dt = 1e-30;
+      const double inv = 1.0 / (dt * vol);
+      auto along = [&](int ax) { return b.sax[0] == ax ? m : (b.sax[1] == ax ? c : r); };
+      const long long g = ioff[bi] + ((long long)along(2) * b.jdim + along(1)) * b.idim + along(0);
+#pragma unroll
+      for (int f = 0; f < 5; ++f) {
+        const double x = fabs(d[f]) * inv;
+        rsum += x * x;
+        if (x > rmax) {  // components in increasing key order: the first of equal terms stays
+          rmax = x;
+          rkey = g * 5 + f;
+        }
+      }
+    }
+    const double gm1 = gama - 1.0;
+    const double rr = b.prim[id], u = b.prim[P + id], v = b.prim[2 * P + id], w = b.prim[3 * P + id];
+    const double pp = b.prim[4 * P + id];
+    const double E = pp / gm1 + 0.5 * rr * (u * u + v * v + w * w);
+    const double q0 = rr + d[0], q1 = rr * u + d[1], q2 = rr * v + d[2], q3 = rr * w + d[3], q4 = E + d[4];
+    double rn = q0;
+    double ir = 1.0 / rn;
+    const double a = q1 * ir, bb = q2 * ir, cc = q3 * ir;
+    const double pn = gm1 * (q4 - 0.5 * rn * (a * a + bb * bb + cc * cc));
+    const double dpp = fabs((pn - pp) / pp);
+    const double x = -0.3 + dpp;
+    const double term = (0.0 < x) ? x : 0.0;  // std::max(0.0, x)
+    const double Fa = 1.0 / (1.0 + 2.0 * term);
+    const double f0 = rr + d[0] * Fa, f1 = rr * u + d[1] * Fa, f2 = rr * v + d[2] * Fa, f3 = rr * w + d[3] * Fa;
+    const double f4 = E + d[4] * Fa;
+    rn = f0;
+    ir = 1.0 / rn;
+    const double un = f1 * ir, vn = f2 * ir, wn = f3 * ir;
+    const double pf = gm1 * (f4 - 0.5 * rn * (un * un + vn * vn + wn * wn));
+    if (!(rn <= 1e-6 || pf <= 1e-6)) {
+      b.prim[id] = rn;
+      b.prim[P + id] = un;
+      b.prim[2 * P + id] = vn;
+      b.prim[3 * P + id] = wn;
+      b.prim[4 * P + id] = pf;
+    }
   }
-  const double gm1 = gama - 1.0;
-  const double rr = b.prim[id], u = b.prim[P + id], v = b.prim[2 * P + id], w = b.prim[3 * P + id];
-  const double pp = b.prim[4 * P + id];
-  const double E = pp / gm1 + 0.5 * rr * (u * u + v * v + w * w);
-  const double q0 = rr + d[0], q1 = rr * u + d[1], q2 = rr * v + d[2], q3 = rr * w + d[3], q4 = E + d[4];
-  double rn = q0;
-  double ir = 1.0 / rn;
-  const double a = q1 * ir, bb = q2 * ir, cc = q3 * ir;
-  const double pn = gm1 * (q4 - 0.5 * rn * (a * a + bb * bb + cc * cc));
-  const double dpp = fabs((pn - pp) / pp);
-  const double x = -0.3 + dpp;
-  const double term = (0.0 < x) ? x : 0.0;  // std::max(0.0, x)
-  const double Fa = 1.0 / (1.0 + 2.0 * term);
-  const double f0 = rr + d[0] * Fa, f1 = rr * u + d[1] * Fa, f2 = rr * v + d[2] * Fa, f3 = rr * w + d[3] * Fa;
-  const double f4 = E + d[4] * Fa;
-  rn = f0;
-  ir = 1.0 / rn;
-  const double un = f1 * ir, vn = f2 * ir, wn = f3 * ir;
-  const double pf = gm1 * (f4 - 0.5 * rn * (un * un + vn * vn + wn * wn));
-  if (!(rn <= 1e-6 || pf <= 1e-6)) {
-    b.prim[id] = rn;
-    b.prim[P + id] = un;
-    b.prim[2 * P + id] = vn;
-    b.prim[3 * P + id] = wn;
-    b.prim[4 * P + id] = pf;
+  ssum[lane] = rsum;
+  smax[lane] = rmax;
+  skey[lane] = rkey;
+  __syncthreads();
+  for (int h = PT * PT / 2; h > 0; h >>= 1) {
+    if (lane < h) {
+      ssum[lane] = ssum[lane] + ssum[lane + h];
+      const double ov = smax[lane + h];
+      const long long ok = skey[lane + h];
+      if (ov > smax[lane] || (ov == smax[lane] && ok < skey[lane])) {
+        smax[lane] = ov;
+        skey[lane] = ok;
+      }
+    }
+    __syncthreads();
+  }
+  if (lane == 0) {
+    psum[blockIdx.x] = ssum[0];
+    pmax[blockIdx.x] = smax[0];
+    pkey[blockIdx.x] = skey[0];
   }
 }
 
@@ -877,10 +933,11 @@ void launch_lu_pack(const Launch& La, long long n, const SweepArrays& S, const P
   hipLaunchKernelGGL(lu_pack_kernel, dim3((unsigned)La.nwg), dim3(SPAN_TB), 0, La.stream, La.blocks, La.span, S, p);
 }
 
-void launch_update(const Launch& La, long long n, const SweepArrays& S, double gamma) {
+void launch_update(const Launch& La, long long n, const SweepArrays& S, double gamma, const long long* ioff,
+                   double* psum, double* pmax, long long* pkey) {
   if (n <= 0) return;
   hipLaunchKernelGGL(update_kernel, dim3((unsigned)La.nwg), dim3(SPAN_TB), 0, La.stream, La.blocks, La.span, S,
-                     gamma);
+                     gamma, ioff, psum, pmax, pkey);
 }
 
 void launch_lusgs_tiles(const SweepArrays& S, const BlockDev* blocks, const SweepTile* tiles, const int* order,

@@ -193,9 +193,17 @@ __global__ void __launch_bounds__(RES_MAXPARTS) residual_final_kernel(const doub
   __shared__ double smax[RES_MAXPARTS];
   __shared__ long long skey[RES_MAXPARTS];
   const int t = threadIdx.x;
-  ssum[t] = t < nparts ? partial[t] : 0.0;
-  smax[t] = t < nparts ? pmax[t] : -1.0;
-  skey[t] = t < nparts ? pidx_[t] : 0x7fffffffffffffffLL;
+  // partials t, t + 1024, ... in order, then a tree over the threads
+  double sum = 0.0;
+  MaxLoc best{-1.0, 0x7fffffffffffffffLL};
+  for (int q = t; q < nparts; q += RES_MAXPARTS) {
+    sum = q == t ? partial[q] : sum + partial[q];
+    const MaxLoc c{pmax[q], pidx_[q]};
+    if (better(c, best)) best = c;
+  }
+  ssum[t] = sum;
+  smax[t] = best.v;
+  skey[t] = best.key;
   __syncthreads();
   for (int s = RES_MAXPARTS / 2; s > 0; s >>= 1) {
     if (t < s) {
@@ -258,6 +266,11 @@ void launch_residual(const Launch& L, long long n, long long total5, double* par
                      partial + RES_MAXPARTS, pidx_);
   hipLaunchKernelGGL(residual_final_kernel, dim3(1), dim3(RES_MAXPARTS), 0, L.stream, partial, partial + RES_MAXPARTS,
                      pidx_, nparts, total5, out, step);
+}
+void launch_residual_final(const double* psum, const double* pmax, const long long* pkey, int nparts, long long total5,
+                           double* out, const int* step, hipStream_t s) {
+  hipLaunchKernelGGL(residual_final_kernel, dim3(1), dim3(RES_MAXPARTS), 0, s, psum, pmax, pkey, nparts, total5, out,
+                     step);
 }
 void launch_fill(double* p, long long n, double v, hipStream_t s) {
   if (n <= 0) return;

@@ -52,6 +52,9 @@ void launch_step_advance(int* step, hipStream_t s);
 void launch_residual(const Launch& L, long long n, long long total5, double* partial, long long* pidx,
                      int nparts, double* out, const int* step);
 int residual_parts(long long n);
+// the final reduction alone, over partials left by another kernel (any count)
+void launch_residual_final(const double* psum, const double* pmax, const long long* pkey, int nparts, long long total5,
+                           double* out, const int* step, hipStream_t s);
 void launch_fill(double* p, long long n, double v, hipStream_t s);
 void launch_temperature(const Launch& L, long long n, double gama, double moo);
 
@@ -68,7 +71,10 @@ void launch_lu_pack(const Launch& La, long long n, const SweepArrays& S, const P
 void launch_lusgs_tiles(const SweepArrays& S, const BlockDev* blocks, const SweepTile* tiles, const int* order,
                         int ntiles, unsigned long long* flags, unsigned long long* epoch, int* err, int type,
                         const Params& p, hipStream_t s);
-void launch_update(const Launch& La, long long n, const SweepArrays& S, double gamma);
+// ioff: prefix of the interior node counts; psum/pmax/pkey: one residual
+// partial per workgroup of the launch
+void launch_update(const Launch& La, long long n, const SweepArrays& S, double gamma, const long long* ioff,
+                   double* psum, double* pmax, long long* pkey);
 
 // k_metrics.hip
 void launch_metrics(const Launch& L, long long n, int gcl, double* tmp, hipStream_t s);
