@@ -862,7 +862,10 @@ static void op_bc(orion_hip_ctx* c) {
     launch_bc(c->db, c->d_bc + c->bc_first[rk], c->bc_count[rk], c->bc_total[rk], c->p, c->stream);
 }
 
-static void op_visc(orion_hip_ctx* c) {
+// first: the step's first right-hand-side kernel, which writes dq whole
+// (the step's state kernel skipped the zeroing); the per-operator calls
+// accumulate into dq as the reference does
+static void op_visc(orion_hip_ctx* c, bool first = false) {
   if (c->p.nvis != 1) return;
   const double cp = 1.0 / ((c->p.gama - 1.0) * c->p.moo * c->p.moo);
   const double cp_prl = cp / c->p.prl;
@@ -870,7 +873,7 @@ static void op_visc(orion_hip_ctx* c) {
   const double re_inv = (std::fabs(re) > 1.0e-30) ? 1.0 / re : 0.0;
   launch_visc_deriv(LA(c, c->intr), c->intr.total());
   for (int d = 0; d < 3; ++d) launch_visc_face(LA(c, c->fpair[d]), d, c->fpair[d].total(), cp_prl);
-  launch_visc_diff(LA(c, c->dtile), c->dtile.total(), re_inv);
+  launch_visc_diff(LA(c, c->dtile), c->dtile.total(), re_inv, first);
 }
 
 static void op_avg(orion_hip_ctx* c) {
@@ -887,9 +890,9 @@ static void op_inv_faces(orion_hip_ctx* c, hipStream_t st) {
   }
 }
 
-static void op_inv(orion_hip_ctx* c) {
+static void op_inv(orion_hip_ctx* c, bool first = false) {
   op_inv_faces(c, c->stream);
-  launch_inv_diff(LA(c, c->dtile), c->dtile.total());
+  launch_inv_diff(LA(c, c->dtile), c->dtile.total(), first);
 }
 
 static void op_sweep(orion_hip_ctx* c, int type) {
@@ -1016,7 +1019,8 @@ int orion_hip_residual(orion_hip_ctx* c, orion_residual_stats* out) {
 static void step_body(orion_hip_ctx* c) {
   op_halo(c);
   op_bc(c);
-  launch_state(LA(c, c->pad), c->pad.total(), c->p, 3, c->d_dtmin, c->d_step);
+  // mode 4: the first right-hand-side kernel below writes dq whole
+  launch_state(LA(c, c->pad), c->pad.total(), c->p, 3 | 4, c->d_dtmin, c->d_step);
   if (c->p.nvis == 1) {
     // The inviscid face fluxes (FP64-bound) run on the side stream beside
     // the viscous chain (load-bound); they have their own flux array and
@@ -1026,12 +1030,12 @@ static void step_body(orion_hip_ctx* c) {
     hipStreamWaitEvent(c->side_stream, c->ev_fork, 0);
     op_inv_faces(c, c->side_stream);
     hipEventRecord(c->ev_join, c->side_stream);
-    op_visc(c);
+    op_visc(c, true);
     op_avg(c);
     hipStreamWaitEvent(c->stream, c->ev_join, 0);
-    launch_inv_diff(LA(c, c->dtile), c->dtile.total());
+    launch_inv_diff(LA(c, c->dtile), c->dtile.total(), false);
   } else {
-    op_inv(c);
+    op_inv(c, true);
   }
   op_avg(c);
   op_update(c);
@@ -1109,14 +1113,14 @@ int orion_hip_profile_run(orion_hip_ctx* c, int nsteps, double* phase_ms, long l
     hipEventRecord(ev[0], c->stream);
     op_halo(c); mark(1);
     op_bc(c); mark((long long)c->bc_first.size());
-    launch_state(LA(c, c->pad), c->pad.total(), c->p, 3, c->d_dtmin + s, nullptr); mark(1);
+    launch_state(LA(c, c->pad), c->pad.total(), c->p, 3 | 4, c->d_dtmin + s, nullptr); mark(1);
     if (c->p.nvis == 1) {
       const double cp = 1.0 / ((c->p.gama - 1.0) * c->p.moo * c->p.moo);
       const double re = c->p.reynolds;
       launch_visc_deriv(LA(c, c->intr), c->intr.total()); mark(1);
       for (int d = 0; d < 3; ++d) launch_visc_face(LA(c, c->fpair[d]), d, c->fpair[d].total(), cp / c->p.prl);
       mark(3);
-      launch_visc_diff(LA(c, c->dtile), c->dtile.total(), (std::fabs(re) > 1.0e-30) ? 1.0 / re : 0.0); mark(1);
+      launch_visc_diff(LA(c, c->dtile), c->dtile.total(), (std::fabs(re) > 1.0e-30) ? 1.0 / re : 0.0, true); mark(1);
       op_avg(c); mark(2);
     } else {
       mark(0); mark(0); mark(0); mark(0);
@@ -1125,7 +1129,7 @@ int orion_hip_profile_run(orion_hip_ctx* c, int nsteps, double* phase_ms, long l
     mark(3);
     for (int d = 0; d < 3; ++d) launch_inv_closure(LA(c, c->ltask[d]), d, c->ltask[d].total(), c->p.gama);
     mark(3);
-    launch_inv_diff(LA(c, c->dtile), c->dtile.total()); mark(1);
+    launch_inv_diff(LA(c, c->dtile), c->dtile.total(), c->p.nvis != 1); mark(1);
     op_avg(c); mark(2);
     launch_lu_pack(LA(c, c->ptile), c->ptile.total(), c->sw, c->p); mark(1);
     for (int t = 0; t < 4; ++t) { op_sweep(c, t); mark(2); }

@@ -608,8 +608,9 @@ __device__ __forceinline__ double diff_node_f(const FF& F, int n, int t) {
 // of one k plane; the i and j stencils of the tile (21 face slots per row
 // or column) are staged through LDS once, the k stencils are loaded per
 // node.  Node t of a line reads faces t-2..t+3, which sit at the padded
-// slots t..t+5.
-template <bool INV>
+// slots t..t+5.  FIRST: the step's first right-hand-side kernel, which starts
+// from dq = 0 (what update_flow_states leaves) instead of reading it.
+template <bool INV, bool FIRST>
 __global__ void __launch_bounds__(256) diff_kernel(const BlockDev* __restrict__ blocks, Span SP, double re_inv) {
   constexpr int DT = 16, DW = DT + 5;
   __shared__ double X[5][DT][DW];  // i slots i0-3 .. i0+17 of each row of the tile
@@ -648,7 +649,7 @@ __global__ void __launch_bounds__(256) diff_kernel(const BlockDev* __restrict__ 
   const long long id = pidx(b, i, j, k);
   double acc[5];
 #pragma unroll
-  for (int m = 0; m < 5; ++m) acc[m] = b.dq[(long long)m * P + id];
+  for (int m = 0; m < 5; ++m) acc[m] = FIRST ? 0.0 : b.dq[(long long)m * P + id];
   const int pos[3] = {i, j, k};
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
@@ -701,9 +702,14 @@ void launch_inv_closure(const Launch& L, int d, long long n, double gamma) {
   if (n <= 0) return;
   hipLaunchKernelGGL(inv_closure_kernel, dim3((unsigned)L.nwg), dim3(SPAN_TB), 0, L.stream, L.blocks, L.span, d, gamma);
 }
-void launch_inv_diff(const Launch& L, long long n) {
+void launch_inv_diff(const Launch& L, long long n, bool first) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(diff_kernel<true>, dim3((unsigned)L.nwg), dim3(SPAN_TB), 0, L.stream, L.blocks, L.span, 0.0);
+  if (first)
+    hipLaunchKernelGGL((diff_kernel<true, true>), dim3((unsigned)L.nwg), dim3(SPAN_TB), 0, L.stream, L.blocks, L.span,
+                       0.0);
+  else
+    hipLaunchKernelGGL((diff_kernel<true, false>), dim3((unsigned)L.nwg), dim3(SPAN_TB), 0, L.stream, L.blocks, L.span,
+                       0.0);
 }
 void launch_visc_deriv(const Launch& L, long long n) {
   if (n <= 0) return;
@@ -717,9 +723,14 @@ void launch_visc_face(const Launch& L, int d, long long n, double cp_prl) {
   if (n <= 0) return;
   hipLaunchKernelGGL(visc_face_kernel, dim3((unsigned)L.nwg), dim3(SPAN_TB), 0, L.stream, L.blocks, L.span, d, cp_prl);
 }
-void launch_visc_diff(const Launch& L, long long n, double re_inv) {
+void launch_visc_diff(const Launch& L, long long n, double re_inv, bool first) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(diff_kernel<false>, dim3((unsigned)L.nwg), dim3(SPAN_TB), 0, L.stream, L.blocks, L.span, re_inv);
+  if (first)
+    hipLaunchKernelGGL((diff_kernel<false, true>), dim3((unsigned)L.nwg), dim3(SPAN_TB), 0, L.stream, L.blocks,
+                       L.span, re_inv);
+  else
+    hipLaunchKernelGGL((diff_kernel<false, false>), dim3((unsigned)L.nwg), dim3(SPAN_TB), 0, L.stream, L.blocks,
+                       L.span, re_inv);
 }
 
 }  // namespace orion_hip

@@ -60,7 +60,7 @@ __global__ void __launch_bounds__(256) state_kernel(const BlockDev* __restrict__
         const double ke = 0.5 * rc * (u * u + v * v + w * w);
         b.q[4 * P + id] = pp * (1.0 / (p.gama - 1.0)) + ke;
       }
-      if (nout == 0) {
+      if (nout == 0 && !(mode & 4)) {  // mode 4: the first right-hand-side kernel writes dq whole
 #pragma unroll
         for (int m = 0; m < 5; ++m) b.dq[m * P + id] = 0.0;
       }

@@ -39,11 +39,13 @@ struct AvgDst {
 // k_flux.hip
 void launch_inv_face(const Launch& L, int d, long long n, double gamma);
 void launch_inv_closure(const Launch& L, int d, long long n, double gamma);
-void launch_inv_diff(const Launch& L, long long n);
+// first: the step's first right-hand-side kernel, starting from dq = 0
+// without reading it (the step's state kernel then skips the zeroing)
+void launch_inv_diff(const Launch& L, long long n, bool first);
 void launch_visc_deriv(const Launch& L, long long n);
 void launch_face_geom(const Launch& L, int d, long long n);
 void launch_visc_face(const Launch& L, int d, long long n, double cp_prl);
-void launch_visc_diff(const Launch& L, long long n, double re_inv);
+void launch_visc_diff(const Launch& L, long long n, double re_inv, bool first);
 
 // k_point.hip
 void launch_state(const Launch& L, long long n, const Params& p, int mode, unsigned long long* dtmin_base,
