@@ -20,26 +20,45 @@ namespace orion_hip {
 
 constexpr int NG = ORION_NG;
 
+// A device pointer held in a descriptor table: on the device it is typed as
+// a global-address-space pointer, so that the loads and stores made through
+// it (and through pointers derived from it) are global instructions with a
+// scalar base, not flat ones (which also count against the LDS/scalar
+// counter and need 64-bit addresses in vector registers).  It converts to
+// and from a plain pointer.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define ORION_GLOBAL_AS __attribute__((address_space(1)))
+#else
+#define ORION_GLOBAL_AS
+#endif
+template <class T>
+struct GPtr {
+  ORION_GLOBAL_AS T* p;
+  GPtr() = default;
+  __host__ __device__ GPtr(T* x) : p((ORION_GLOBAL_AS T*)x) {}
+  __host__ __device__ operator T*() const { return (T*)p; }
+};
+
 // Fields of one block (device pointers into the arena).
 struct BlockDev {
   int idim, jdim, kdim;  // interior sizes
   int nx, ny, nz;        // padded sizes
   long long P;           // nx*ny*nz
-  double* prim;          // [6][P] rho,u,v,w,p,T
-  double* met;           // [9][P]
-  double* vol;           // [P]
-  double* dq;            // [5][P]
-  double* rhs;           // [5][P] LU-SGS right-hand side copy
-  double* dt;            // [P]
-  double* c;             // [P]
-  double* mu;            // [P]
-  double* q;             // [5][P] conservative (write-only; may be null)
-  double* duvwt;         // [12][P] viscous cell derivatives (scratch)
-  double* fx;            // [3][5][P] viscous face fluxes per direction (scratch)
-  double* fxi;           // [3][5][P] inviscid face fluxes (scratch; the same array as fx without viscosity)
-  double* fcov;          // closure overrides, [3][lines][6][5] (scratch)
-  double* fgeo;          // [3][FG_N][P] face geometry per direction (static, see FG_N)
-  unsigned char* frozen; // [P] 1 inside a physical-BC window
+  GPtr<double> prim;          // [6][P] rho,u,v,w,p,T
+  GPtr<double> met;           // [9][P]
+  GPtr<double> vol;           // [P]
+  GPtr<double> dq;            // [5][P]
+  GPtr<double> rhs;           // [5][P] LU-SGS right-hand side copy
+  GPtr<double> dt;            // [P]
+  GPtr<double> c;             // [P]
+  GPtr<double> mu;            // [P]
+  GPtr<double> q;             // [5][P] conservative (write-only; may be null)
+  GPtr<double> duvwt;         // [12][P] viscous cell derivatives (scratch)
+  GPtr<double> fx;            // [3][5][P] viscous face fluxes per direction (scratch)
+  GPtr<double> fxi;           // [3][5][P] inviscid face fluxes (scratch; the same array as fx without viscosity)
+  GPtr<double> fcov;          // closure overrides, [3][lines][6][5] (scratch)
+  GPtr<double> fgeo;          // [3][FG_N][P] face geometry per direction (static, see FG_N)
+  GPtr<unsigned char> frozen; // [P] 1 inside a physical-BC window
   long long fcov_off[3]; // offset (doubles) of each direction's overrides
   // LU-SGS sweep layout of this block: axes (march, col, row) as real axes
   // 0=i,1=j,2=k, their interior sizes, the padded column count and the
