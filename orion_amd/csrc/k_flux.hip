@@ -98,8 +98,39 @@ __device__ inline void roe_flux(const double* ql, const double* qr, double nx, d
   for (int m = 0; m < 5; ++m) f[m] = 0.5 * (fl[m] + fr[m] - df[m]);
 }
 
+// Correctly rounded quotients a / d over one denominator d.  The reciprocal
+// is refined once from the hardware estimate by two Newton-Raphson steps,
+// r <- r + r (1 - d r), and each quotient is corrected with its exact
+// residual (Markstein): q = a r, e = fma(-d, q, a), q' = fma(e, r, q).
+// These are the steps of the correctly rounded division sequence without its
+// exponent pre-scaling and special-value fix-up, which change nothing while
+// d, a and a / d stay well inside the normal range (the callers check).
+// So each quotient is the IEEE a / d, and quotients sharing d share the
+// reciprocal.
+struct SharedDiv {
+  double d, r;
+  __device__ __forceinline__ explicit SharedDiv(double den) : d(den) {
+    double x = __builtin_amdgcn_rcp(den);
+    double e = __builtin_fma(-den, x, 1.0);
+    x = __builtin_fma(x, e, x);
+    e = __builtin_fma(-den, x, 1.0);
+    r = __builtin_fma(x, e, x);
+  }
+  __device__ __forceinline__ double operator()(double a) const {
+    const double q = a * r;
+    const double e = __builtin_fma(-d, q, a);
+    return __builtin_fma(e, r, q);
+  }
+};
+
 // WCNS-E-5 left/right states of one variable at the face between line
 // positions p+2 and p+3, from q[p..p+5] (InviscidFluxComputer.cpp:268-314).
+// SHARED: with smoothness indicators below 1e100 every divisor lies in
+// [1e-12, 1e200] and every weight in [6e-202, 1], so the weights are formed
+// with SharedDiv (one reciprocal for the three normalised weights of a
+// side); otherwise (never for a physical state; NaN included), and without
+// SHARED (kernels whose register budget is tight), with '/'.
+template <bool SHARED>
 __device__ inline void weno5(double qm2, double qm1, double q0, double qp1, double qp2, double qp3, double* ql,
                              double* qr) {
   const double CL1 = 1.0 / 16.0, CL2 = 10.0 / 16.0, CL3 = 5.0 / 16.0, EPS = 1.0e-6;
@@ -110,21 +141,35 @@ __device__ inline void weno5(double qm2, double qm1, double q0, double qp1, doub
   const double g2 = 0.5 * (qp1 - qm1);
   const double g3 = 0.5 * (-3.0 * q0 + 4.0 * qp1 - qp2);
   const double is1 = g1 * g1 + s1 * s1, is2 = g2 * g2 + s2 * s2, is3 = g3 * g3 + s3 * s3;
-  const double w1 = CL1 / ((EPS + is1) * (EPS + is1));
-  const double w2 = CL2 / ((EPS + is2) * (EPS + is2));
-  const double w3 = CL3 / ((EPS + is3) * (EPS + is3));
-  const double sw = w1 + w2 + w3;
-  *ql = q0 + 0.125 * ((w1 / sw) * (s1 + 4.0 * g1) + (w2 / sw) * (s2 + 4.0 * g2) + (w3 / sw) * (s3 + 4.0 * g3));
   const double r1s = s2, r2s = s3, r3s = qp1 - 2.0 * qp2 + qp3;
   const double h1 = 0.5 * (qm1 - 4.0 * q0 + 3.0 * qp1);
   const double h2 = 0.5 * (qp2 - q0);
   const double h3 = 0.5 * (-3.0 * qp1 + 4.0 * qp2 - qp3);
   const double js1 = h1 * h1 + r1s * r1s, js2 = h2 * h2 + r2s * r2s, js3 = h3 * h3 + r3s * r3s;
-  const double v1 = CL3 / ((EPS + js1) * (EPS + js1));
-  const double v2 = CL2 / ((EPS + js2) * (EPS + js2));
-  const double v3 = CL1 / ((EPS + js3) * (EPS + js3));
-  const double sv = v1 + v2 + v3;
-  *qr = qp1 + 0.125 * ((v1 / sv) * (r1s - 4.0 * h1) + (v2 / sv) * (r2s - 4.0 * h2) + (v3 / sv) * (r3s - 4.0 * h3));
+  const double ismax = fmax(fmax(fmax(is1, is2), fmax(is3, js1)), fmax(js2, js3));
+  if (SHARED && ismax < 1.0e100) {
+    const double w1 = SharedDiv((EPS + is1) * (EPS + is1))(CL1);
+    const double w2 = SharedDiv((EPS + is2) * (EPS + is2))(CL2);
+    const double w3 = SharedDiv((EPS + is3) * (EPS + is3))(CL3);
+    const SharedDiv sw(w1 + w2 + w3);
+    *ql = q0 + 0.125 * (sw(w1) * (s1 + 4.0 * g1) + sw(w2) * (s2 + 4.0 * g2) + sw(w3) * (s3 + 4.0 * g3));
+    const double v1 = SharedDiv((EPS + js1) * (EPS + js1))(CL3);
+    const double v2 = SharedDiv((EPS + js2) * (EPS + js2))(CL2);
+    const double v3 = SharedDiv((EPS + js3) * (EPS + js3))(CL1);
+    const SharedDiv sv(v1 + v2 + v3);
+    *qr = qp1 + 0.125 * (sv(v1) * (r1s - 4.0 * h1) + sv(v2) * (r2s - 4.0 * h2) + sv(v3) * (r3s - 4.0 * h3));
+  } else {
+    const double w1 = CL1 / ((EPS + is1) * (EPS + is1));
+    const double w2 = CL2 / ((EPS + is2) * (EPS + is2));
+    const double w3 = CL3 / ((EPS + is3) * (EPS + is3));
+    const double sw = w1 + w2 + w3;
+    *ql = q0 + 0.125 * ((w1 / sw) * (s1 + 4.0 * g1) + (w2 / sw) * (s2 + 4.0 * g2) + (w3 / sw) * (s3 + 4.0 * g3));
+    const double v1 = CL3 / ((EPS + js1) * (EPS + js1));
+    const double v2 = CL2 / ((EPS + js2) * (EPS + js2));
+    const double v3 = CL1 / ((EPS + js3) * (EPS + js3));
+    const double sv = v1 + v2 + v3;
+    *qr = qp1 + 0.125 * ((v1 / sv) * (r1s - 4.0 * h1) + (v2 / sv) * (r2s - 4.0 * h2) + (v3 / sv) * (r3s - 4.0 * h3));
+  }
 }
 
 struct LineRef {
@@ -141,10 +186,11 @@ struct LineRef {
 // loads then need not wait for the boundary flags -- and the faces outside
 // [ist, ied] (f < 2 or f > n - 2) have both states replaced below, exactly
 // as the reference leaves them to the fixed stencils.
+template <bool SHARED>
 __device__ inline void face_states(const LineRef& L, int n, int ist, int ied, int f, double* qwl, double* qwr) {
 #pragma unroll
   for (int m = 0; m < 5; ++m)
-    weno5(L.at(m, f), L.at(m, f + 1), L.at(m, f + 2), L.at(m, f + 3), L.at(m, f + 4), L.at(m, f + 5), &qwl[m],
+    weno5<SHARED>(L.at(m, f), L.at(m, f + 1), L.at(m, f + 2), L.at(m, f + 3), L.at(m, f + 4), L.at(m, f + 5), &qwl[m],
           &qwr[m]);
   if (ist > 1 && f <= 2) {
 #pragma unroll
@@ -224,7 +270,7 @@ __global__ void __launch_bounds__(256) inv_face_kernel(const BlockDev* __restric
   const int ist = (L.at(0, 0) < 1.0e-20) ? 2 : 0;
   const int ied = (L.at(0, n + 5) < 1.0e-20) ? n - 2 : n;
   double qwl[5], qwr[5], k3[3], fo[5];
-  face_states(L, n, ist, ied, f, qwl, qwr);
+  face_states<true>(L, n, ist, ied, f, qwl, qwr);
   {
     const double* G = b.fgeo + (long long)(d * FG_N + 10) * b.P + base + (long long)(f + 2) * s;
     k3[0] = G[0];
@@ -336,7 +382,7 @@ __global__ void __launch_bounds__(256) inv_closure_kernel(const BlockDev* __rest
     for (int k = 0; k < 4; ++k) {
       const int f = face(k, 1);
       double ql[5], qr[5];
-      face_states(L, n, ist, ied, f, ql, qr);
+      face_states<false>(L, n, ist, ied, f, ql, qr);
 #pragma unroll
       for (int m = 0; m < 5; ++m) qo[k][m] = right ? ql[m] : qr[m];
       if (right && ist > 1 && f <= 5) {  // a face the left end's stages set last
