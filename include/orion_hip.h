@@ -136,6 +136,44 @@ int orion_hip_run(orion_hip_ctx* ctx, int nsteps, orion_residual_stats* hist);
 #define ORION_PHASES 18
 int orion_hip_profile_run(orion_hip_ctx* ctx, int nsteps, double* phase_ms, long long* phase_launches);
 
+/* Output (postprocess::PostProcess, src/postprocess/PostProcess.hpp:10-33).
+ *
+ * The point fields that PostProcess::write_block_vts_binary writes
+ * (src/postprocess/PostProcess.cpp:183-240) for one block, formed on the
+ * device from the current state and copied back as float: Density,
+ * Pressure, Temperature and Mach over the interior nodes (i fastest), then
+ * Velocity as (u, v, w) per node.  count = 7 * idim * jdim * kdim.
+ * Synchronous. */
+int orion_hip_output_fields(orion_hip_ctx* ctx, int block, float* host, long long count);
+/* PostProcess::write_solution (src/postprocess/PostProcess.cpp:44-123) for
+ * every block of the context at step `step`: the directory and file stem
+ * come from flowname (NULL or "" means "output/flow"), the directory is
+ * created when missing, then <dir>/<stem>_<step:06>_block_<b>.vts per block
+ * and <dir>/<stem>_<step:06>.vtm.  xyz: the grid as passed to
+ * orion_hip_ctx_create (x, y, z of block 0, then block 1, ...). */
+int orion_hip_write_solution(orion_hip_ctx* ctx, const char* flowname, int step, const double* xyz);
+/* Host only (no device use): one block's VTS file as
+ * write_block_vts_binary writes it (src/postprocess/PostProcess.cpp:126-252)
+ * from its grid (x, y, z: idim * jdim * kdim each, i fastest) and the fields
+ * laid out as orion_hip_output_fields returns them. */
+int orion_hip_write_vts(const char* path, int idim, int jdim, int kdim, const double* x, const double* y,
+                        const double* z, const float* fields);
+/* Host only: the multiblock index of a step (write_global_vtm,
+ * src/postprocess/PostProcess.cpp:254-283). */
+int orion_hip_write_vtm(const char* output_dir, const char* stem, int nblocks, int step);
+
+/* Restart, which the reference does not have (it always starts from the
+ * freestream, src/preprocess/InitialCondition.cpp:10-18, and
+ * OrionSolver::solve starts at step 0, src/core/OrionSolver.cpp:132).
+ * The state file holds a header (magic "ORIONST1", block count, the
+ * interior sizes of every block, the step) and then every block's padded
+ * prim (6 variables, the reference's layout, doubles).  prim alone fixes
+ * the next pseudo-time step, so a context of the same grid and inputs that
+ * loads the file continues bit for bit as the one that saved it.  Loading
+ * checks the block count and sizes.  Both synchronise. */
+int orion_hip_save_state(orion_hip_ctx* ctx, const char* path, long long step);
+int orion_hip_load_state(orion_hip_ctx* ctx, const char* path, long long* step);
+
 int orion_hip_synchronize(orion_hip_ctx* ctx);
 /* Use a caller-owned hipStream_t (NULL restores the context's own stream). */
 int orion_hip_set_stream(orion_hip_ctx* ctx, void* stream);

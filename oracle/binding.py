@@ -74,6 +74,8 @@ def lib():
         L.ob_run.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double)]
         L.ob_set_threads.argtypes = [C.c_int]
         L.ob_lusgs_sweeps.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double)]
+        L.ob_output_fields.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_float)]
+        L.ob_output_fields.restype = None
         _lib = L
     return _lib
 
@@ -138,6 +140,13 @@ class OracleCase:
             raise RuntimeError("oracle run: " + lib().ob_last_error().decode())
         return hist.reshape(n, 2)
 
+    def output_fields(self, b: int) -> np.ndarray:
+        """The VTS point fields of block b (PostProcess.cpp:183-240), float32."""
+        i, j, k = (int(v) for v in self.case.dims[b])
+        out = np.empty(7 * i * j * k, dtype=np.float32)
+        lib().ob_output_fields(self._h, b, out.ctypes.data_as(C.POINTER(C.c_float)))
+        return out
+
     def lusgs_sweeps(self, b: int, rhs: np.ndarray) -> None:
         rhs = np.ascontiguousarray(rhs, dtype=np.float64)
         lib().ob_lusgs_sweeps(self._h, b, rhs.ctypes.data_as(C.POINTER(C.c_double)))
@@ -151,15 +160,17 @@ def have_reference() -> bool:
     return os.path.exists(REF_HARNESS)
 
 
-def run_reference(case, casedir: str, outdir: str, nsteps: int, dump_steps: int = 0):
+def run_reference(case, casedir: str, outdir: str, nsteps: int, dump_steps: int = 0, vts: bool = False):
     """Write ``case`` in the reference's formats and run the reference
     solver's operators for ``nsteps``.  Returns (hist[n,8], reader) where
-    reader(stage_file) -> list of per-block dicts of fields."""
+    reader(stage_file) -> list of per-block dicts of fields.  With ``vts``
+    the reference's PostProcess::write_solution also runs after the last
+    step, writing into the case's flowname directory under casedir."""
     case.write(casedir)
     os.makedirs(outdir, exist_ok=True)
     env = dict(os.environ)
     env.pop("LD_LIBRARY_PATH", None)
-    subprocess.run([REF_HARNESS, casedir, str(nsteps), outdir, str(dump_steps)], check=True,
+    subprocess.run([REF_HARNESS, casedir, str(nsteps), outdir, str(dump_steps)] + (["vts"] if vts else []), check=True,
                    stdout=subprocess.DEVNULL, env=env)
     hist = np.fromfile(os.path.join(outdir, "hist.bin")).reshape(nsteps, 8)
 

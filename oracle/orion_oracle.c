@@ -2055,3 +2055,30 @@ int ob_run(ob_case* c, int nsteps, double* hist) {
   }
   return 0;
 }
+
+/* PostProcess::write_block_vts_binary's extraction loops
+ * (src/postprocess/PostProcess.cpp:174-240): k, j, i over the interior,
+ * each double value cast to float; Mach from prim with gama
+ * (params.inflow.gama), sqrt(v2 / c2) where c2 > 1e-30, else 0. */
+void ob_output_fields(const ob_case* c, int b, float* out) {
+  const ob_block* k = &c->blocks[b];
+  const long n = (long)k->idim * k->jdim * k->kdim;
+  const double gamma = c->p.gama;
+  long g = 0;
+  for (int kk = 0; kk < k->kdim; ++kk)
+    for (int j = 0; j < k->jdim; ++j)
+      for (int i = 0; i < k->idim; ++i, ++g) {
+        const long id = IX(k, i + NG, j + NG, kk + NG);
+        const double r = k->prim[id], u = k->prim[k->P + id], v = k->prim[2 * k->P + id];
+        const double w = k->prim[3 * k->P + id], p = k->prim[4 * k->P + id];
+        out[g] = (float)r;
+        out[n + g] = (float)p;
+        out[2 * n + g] = (float)k->prim[5 * k->P + id];
+        const double v2 = u * u + v * v + w * w;
+        const double c2 = gamma * p / r;
+        out[3 * n + g] = (float)((c2 > 1e-30) ? sqrt(v2 / c2) : 0.0);
+        out[4 * n + 3 * g] = (float)u;
+        out[4 * n + 3 * g + 1] = (float)v;
+        out[4 * n + 3 * g + 2] = (float)w;
+      }
+}

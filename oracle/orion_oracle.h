@@ -125,6 +125,12 @@ int ob_run(ob_case* c, int nsteps, double* hist);
 /* LU-SGS building blocks exposed for the lexicographic-vs-wavefront test. */
 void ob_lusgs_sweeps(ob_case* c, int b, const double* rhs_soa);
 
+/* The point fields PostProcess::write_block_vts_binary writes for block b
+ * (src/postprocess/PostProcess.cpp:183-240), as float: Density, Pressure,
+ * Temperature, Mach (idim*jdim*kdim each, interior nodes, i fastest), then
+ * Velocity as (u, v, w) per node -- 7 * idim*jdim*kdim values in out. */
+void ob_output_fields(const ob_case* c, int b, float* out);
+
 #ifdef __cplusplus
 }
 #endif

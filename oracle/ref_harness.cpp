@@ -10,11 +10,15 @@
 // the reference has no way to express a perturbed initial field
 // (src/preprocess/InitialCondition.cpp:7-86).
 //
-// usage: ref_harness <casedir> <nsteps> <outdir> [dump_steps]
+// usage: ref_harness <casedir> <nsteps> <outdir> [dump_steps] [vts]
 //   outdir/hist.bin   nsteps x 8 doubles: rms, max, max_loc[5], phydtime
 //   outdir/s<step>_<stage>.bin   for step <= dump_steps, every block's
 //                     [prim 6P][dq 5P][c P][mu P][dt P] after each stage
 //   outdir/final.bin  same layout after the last step
+//   with "vts": PostProcess::write_solution (src/postprocess/PostProcess.cpp:44-123)
+//                     after the last step, as OrionSolver::solve ends
+//                     (src/core/OrionSolver.cpp:225), into the case's
+//                     flowname directory (relative to casedir)
 #include "bc/BCDistributor.hpp"
 #include "bc/BCPreprocess.hpp"
 #include "bc/BCReader.hpp"
@@ -27,6 +31,7 @@
 #include "preprocess/FlowField.hpp"
 #include "preprocess/Inflow.hpp"
 #include "preprocess/InitialCondition.hpp"
+#include "postprocess/PostProcess.hpp"
 #include "solver/FluxComputer.hpp"
 #include "solver/HaloExchanger.hpp"
 #include "solver/InviscidFluxComputer.hpp"
@@ -67,6 +72,7 @@ int main(int argc, char** argv) {
   const int nsteps = std::atoi(argv[2]);
   std::string outdir = argv[3];
   const int dump_steps = argc > 4 ? std::atoi(argv[4]) : 0;
+  const bool vts = argc > 5 && std::string(argv[5]) == "vts";
   if (outdir[0] != '/') {
     char cwd[4096];
     if (!getcwd(cwd, sizeof cwd)) return 2;
@@ -145,6 +151,7 @@ int main(int argc, char** argv) {
     std::fclose(f);
   }
   dump_state(outdir + "/final.bin", fs);
+  if (vts) postprocess::PostProcess::write_solution(fs, grid, params, nsteps);
   MPI_Finalize();
   return 0;
 }

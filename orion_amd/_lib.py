@@ -88,6 +88,13 @@ def load():
         "orion_hip_set_stream": ([P, C.c_void_p], C.c_int),
         "orion_hip_device_bytes": ([P], C.c_size_t),
         "orion_hip_interior_nodes": ([P], C.c_longlong),
+        "orion_hip_output_fields": ([P, C.c_int, C.POINTER(C.c_float), C.c_longlong], C.c_int),
+        "orion_hip_write_solution": ([P, C.c_char_p, C.c_int, C.POINTER(C.c_double)], C.c_int),
+        "orion_hip_write_vts": ([C.c_char_p, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double),
+                                 C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_float)], C.c_int),
+        "orion_hip_write_vtm": ([C.c_char_p, C.c_char_p, C.c_int, C.c_int], C.c_int),
+        "orion_hip_save_state": ([P, C.c_char_p, C.c_longlong], C.c_int),
+        "orion_hip_load_state": ([P, C.c_char_p, C.POINTER(C.c_longlong)], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)

@@ -13,7 +13,10 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <filesystem>
+#include <iomanip>
 #include <map>
+#include <sstream>
 #include <string>
 #include <vector>
 
@@ -26,7 +29,9 @@ using namespace orion_hip;
 static thread_local std::string g_err;
 static thread_local int g_device = 0;
 
-static int fail(const char* fmt, ...) {
+// records the message returned by orion_hip_last_error; returns -1
+namespace orion_hip {
+int fail(const char* fmt, ...) {
   char buf[1024];
   va_list ap;
   va_start(ap, fmt);
@@ -35,6 +40,7 @@ static int fail(const char* fmt, ...) {
   g_err = buf;
   return -1;
 }
+}  // namespace orion_hip
 
 #define HIPCHK(x)                                                                              \
   do {                                                                                         \
@@ -1152,6 +1158,111 @@ int orion_hip_profile_run(orion_hip_ctx* c, int nsteps, double* phase_ms, long l
     if (phase_ms) phase_ms[i] = acc[i];
     if (phase_launches) phase_launches[i] = nl[i];
   }
+  return 0;
+}
+
+int orion_hip_output_fields(orion_hip_ctx* c, int b, float* host, long long count) {
+  if (!c || b < 0 || b >= c->nb) return fail("output_fields: bad block %d", b);
+  const BlockDev& B = c->hb[b];
+  const long long n = (long long)B.idim * B.jdim * B.kdim;
+  if (count != 7 * n) return fail("output_fields: block %d has %lld values, got %lld", b, 7 * n, count);
+  // staged in the block's face-flux scratch (15 P doubles >= 7 n floats),
+  // which every step rewrites before reading
+  float* dev = reinterpret_cast<float*>((double*)B.fx);
+  launch_output(c->db, b, n, dev, c->p.gama, c->stream);
+  if (check_launch("output_fields")) return -1;
+  HIPCHK(hipMemcpyAsync(host, dev, sizeof(float) * 7 * n, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int orion_hip_write_solution(orion_hip_ctx* c, const char* flowname, int step, const double* xyz) {
+  if (!c || !xyz) return fail("write_solution: null context or grid");
+  // output directory and file stem (PostProcess.cpp:53-71)
+  std::string prefix = flowname ? flowname : "";
+  if (prefix.empty()) prefix = "output/flow";
+  const std::filesystem::path path(prefix);
+  std::string dir = path.parent_path().string();
+  const std::string stem = path.filename().string();
+  if (dir.empty()) dir = ".";
+  {
+    std::error_code ec;
+    if (!std::filesystem::exists(dir, ec)) std::filesystem::create_directories(dir, ec);
+  }
+  const double* g = xyz;
+  std::vector<float> f;
+  for (int b = 0; b < c->nb; ++b) {
+    const BlockDev& B = c->hb[b];
+    const long long n = (long long)B.idim * B.jdim * B.kdim;
+    f.resize(7 * n);
+    if (orion_hip_output_fields(c, b, f.data(), 7 * n)) return -1;
+    std::ostringstream name;
+    name << dir << "/" << stem << "_" << std::setfill('0') << std::setw(6) << step << "_block_" << b << ".vts";
+    if (orion_hip_write_vts(name.str().c_str(), B.idim, B.jdim, B.kdim, g, g + n, g + 2 * n, f.data())) return -1;
+    g += 3 * n;
+  }
+  return orion_hip_write_vtm(dir.c_str(), stem.c_str(), c->nb, step);
+}
+
+// state file: magic, block count, the interior sizes of every block, the
+// step, then every block's padded prim
+static const char kStateMagic[8] = {'O', 'R', 'I', 'O', 'N', 'S', 'T', '1'};
+
+int orion_hip_save_state(orion_hip_ctx* c, const char* path, long long step) {
+  if (!c || !path) return fail("save_state: null context or path");
+  FILE* f = std::fopen(path, "wb");
+  if (!f) return fail("save_state: cannot open %s", path);
+  bool ok = std::fwrite(kStateMagic, 1, 8, f) == 8;
+  const long long nb = c->nb;
+  ok = ok && std::fwrite(&nb, sizeof nb, 1, f) == 1;
+  for (int b = 0; ok && b < c->nb; ++b) {
+    const int d[3] = {c->hb[b].idim, c->hb[b].jdim, c->hb[b].kdim};
+    ok = std::fwrite(d, sizeof(int), 3, f) == 3;
+  }
+  ok = ok && std::fwrite(&step, sizeof step, 1, f) == 1;
+  std::vector<double> h;
+  for (int b = 0; ok && b < c->nb; ++b) {
+    h.resize(6 * c->hb[b].P);
+    if (orion_hip_download(c, b, ORION_FIELD_PRIM, h.data(), (long long)h.size())) {
+      std::fclose(f);
+      return -1;
+    }
+    ok = std::fwrite(h.data(), sizeof(double), h.size(), f) == h.size();
+  }
+  if (std::fclose(f) != 0) ok = false;
+  return ok ? 0 : fail("save_state: write to %s failed", path);
+}
+
+int orion_hip_load_state(orion_hip_ctx* c, const char* path, long long* step) {
+  if (!c || !path) return fail("load_state: null context or path");
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return fail("load_state: cannot open %s", path);
+  auto bad = [&](const char* why) {
+    std::fclose(f);
+    return fail("load_state: %s: %s", path, why);
+  };
+  char magic[8];
+  long long nb = 0, st = 0;
+  if (std::fread(magic, 1, 8, f) != 8 || std::memcmp(magic, kStateMagic, 8) != 0) return bad("not a state file");
+  if (std::fread(&nb, sizeof nb, 1, f) != 1 || nb != c->nb) return bad("block count differs from the context's");
+  for (int b = 0; b < c->nb; ++b) {
+    int d[3];
+    if (std::fread(d, sizeof(int), 3, f) != 3) return bad("truncated header");
+    if (d[0] != c->hb[b].idim || d[1] != c->hb[b].jdim || d[2] != c->hb[b].kdim)
+      return bad("block sizes differ from the context's");
+  }
+  if (std::fread(&st, sizeof st, 1, f) != 1) return bad("truncated header");
+  std::vector<double> h;
+  for (int b = 0; b < c->nb; ++b) {
+    h.resize(6 * c->hb[b].P);
+    if (std::fread(h.data(), sizeof(double), h.size(), f) != h.size()) return bad("truncated fields");
+    if (orion_hip_upload(c, b, ORION_FIELD_PRIM, h.data(), (long long)h.size())) {
+      std::fclose(f);
+      return -1;
+    }
+  }
+  std::fclose(f);
+  if (step) *step = st;
   return 0;
 }
 

@@ -249,6 +249,35 @@ __global__ void __launch_bounds__(256) temperature_kernel(const BlockDev* __rest
   b.prim[5 * b.P + id] = (moo * moo) * a2;
 }
 
+// The point fields that PostProcess::write_block_vts_binary writes
+// (src/postprocess/PostProcess.cpp:183-240), on the interior nodes of block
+// bi, i fastest, as float: Density, Pressure, Temperature and Mach (n each),
+// then Velocity as (u, v, w) per node.  Mach is sqrt(v2 / c2) with
+// c2 = gamma p / rho, 0 where c2 <= 1e-30, in double, then rounded.
+__global__ void __launch_bounds__(256) output_kernel(const BlockDev* __restrict__ blocks, int bi, float* out,
+                                                     double gamma) {
+  const BlockDev& b = blocks[bi];
+  const long long n = (long long)b.idim * b.jdim * b.kdim;
+  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n) return;
+  unsigned q, ui, q2, uj;
+  udivmod((unsigned)g, (unsigned)b.idim, &q, &ui);
+  udivmod(q, (unsigned)b.jdim, &q2, &uj);
+  const long long P = b.P;
+  const long long id = pidx(b, (int)ui + NG, (int)uj + NG, (int)q2 + NG);
+  const double r = b.prim[id], u = b.prim[P + id], v = b.prim[2 * P + id], w = b.prim[3 * P + id];
+  const double p = b.prim[4 * P + id], t = b.prim[5 * P + id];
+  out[g] = (float)r;
+  out[n + g] = (float)p;
+  out[2 * n + g] = (float)t;
+  const double v2 = u * u + v * v + w * w;
+  const double c2 = gamma * p / r;
+  out[3 * n + g] = (float)((c2 > 1e-30) ? sqrt(v2 / c2) : 0.0);
+  out[4 * n + 3 * g] = (float)u;
+  out[4 * n + 3 * g + 1] = (float)v;
+  out[4 * n + 3 * g + 2] = (float)w;
+}
+
 static inline unsigned grid_of(long long n, int tb) { return (unsigned)((n + tb - 1) / tb); }
 
 void launch_state(const Launch& L, long long n, const Params& p, int mode, unsigned long long* dtmin_base,
@@ -271,6 +300,10 @@ void launch_residual_final(const double* psum, const double* pmax, const long lo
                            double* out, const int* step, hipStream_t s) {
   hipLaunchKernelGGL(residual_final_kernel, dim3(1), dim3(RES_MAXPARTS), 0, s, psum, pmax, pkey, nparts, total5, out,
                      step);
+}
+void launch_output(const BlockDev* blocks, int bi, long long n, float* out, double gamma, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(output_kernel, dim3(grid_of(n, 256)), dim3(256), 0, s, blocks, bi, out, gamma);
 }
 void launch_fill(double* p, long long n, double v, hipStream_t s) {
   if (n <= 0) return;

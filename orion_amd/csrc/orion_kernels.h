@@ -54,6 +54,8 @@ void launch_step_advance(int* step, hipStream_t s);
 void launch_residual(const Launch& L, long long n, long long total5, double* partial, long long* pidx,
                      int nparts, double* out, const int* step);
 int residual_parts(long long n);
+// the VTS point fields of block bi (n interior nodes) as float, 7 n values
+void launch_output(const BlockDev* blocks, int bi, long long n, float* out, double gamma, hipStream_t s);
 // the final reduction alone, over partials left by another kernel (any count)
 void launch_residual_final(const double* psum, const double* pmax, const long long* pkey, int nparts, long long total5,
                            double* out, const int* step, hipStream_t s);

@@ -379,6 +379,16 @@ void FlowFieldSet::upload(int block, int field, const std::vector<double>& host)
 
 void FlowFieldSet::synchronize() const { check(orion_hip_synchronize(ctx_), "synchronize"); }
 
+void FlowFieldSet::save_state(const std::string& path, long long step) const {
+  check(orion_hip_save_state(ctx_, path.c_str(), step), "save_state");
+}
+
+long long FlowFieldSet::load_state(const std::string& path) {
+  long long step = 0;
+  check(orion_hip_load_state(ctx_, path.c_str(), &step), "load_state");
+  return step;
+}
+
 void update_temperature(const core::Params&, FlowFieldSet& fs) {
   check(orion_hip_update_temperature(fs.ctx()), "update_temperature");
 }
@@ -451,4 +461,16 @@ std::vector<orion_residual_stats> run(preprocess::FlowFieldSet& fs, int nsteps) 
 }
 
 }  // namespace solver
+
+namespace postprocess {
+
+void PostProcess::write_solution(const preprocess::FlowFieldSet& fs, const mesh::MultiBlockGrid& grid,
+                                 const core::Params& params, int step) {
+  std::vector<double> xyz;
+  for (const auto& b : grid.blocks)
+    for (const auto* a : {&b.x, &b.y, &b.z}) xyz.insert(xyz.end(), a->begin(), a->end());
+  check(orion_hip_write_solution(fs.ctx(), params.filename.flowname.c_str(), step, xyz.data()), "write_solution");
+}
+
+}  // namespace postprocess
 }  // namespace orion_amd

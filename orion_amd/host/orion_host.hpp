@@ -151,6 +151,10 @@ class FlowFieldSet {
   std::vector<double> download(int block, int field) const;
   void upload(int block, int field, const std::vector<double>& host);
   void synchronize() const;
+  // restart state (every block's padded prim; no counterpart in the
+  // reference): save after `step` steps, load returns the saved step
+  void save_state(const std::string& path, long long step) const;
+  long long load_state(const std::string& path);
 
  private:
   orion_hip_ctx* ctx_ = nullptr;
@@ -215,4 +219,18 @@ ResidualStats step(bc::BCData& bc, preprocess::FlowFieldSet& fs, core::Params& p
 std::vector<orion_residual_stats> run(preprocess::FlowFieldSet& fs, int nsteps);
 
 }  // namespace solver
+
+namespace postprocess {
+
+// postprocess::PostProcess (src/postprocess/PostProcess.hpp:10-33)
+class PostProcess {
+ public:
+  // write_solution (src/postprocess/PostProcess.cpp:44-123): the fields are
+  // formed on the device and written, with the grid's points, as one VTS
+  // per block and the VTM index under params.filename.flowname
+  static void write_solution(const preprocess::FlowFieldSet& fs, const mesh::MultiBlockGrid& grid,
+                             const core::Params& params, int step);
+};
+
+}  // namespace postprocess
 }  // namespace orion_amd

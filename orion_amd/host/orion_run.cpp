@@ -4,6 +4,7 @@
 // directory (src/main.cpp, src/core/OrionSolver.cpp:61-223).
 //
 //   orion_run CASEDIR [NSTEPS] [--ops] [--hist FILE] [--device N] [--check]
+//             [--restart FILE] [--save FILE] [--no-output]
 //
 //   NSTEPS   steps to run (default: control.nomax of param.dat)
 //   --ops    call the operators one by one (as OrionSolver::solve does)
@@ -11,6 +12,12 @@
 //   --hist   write one record of 8 doubles per step: rms, max, the 5 entries
 //            of max_loc, phydtime
 //   --check  parse the case only and print a summary (no device use)
+//   --restart  start from a state file written by --save (instead of the
+//            initial field); the steps continue its step count
+//   --save   write the state after the last step
+//   --no-output  skip the solution files; by default, like the reference
+//            (OrionSolver.cpp:225), the VTS/VTM files of the last step are
+//            written under the case's flowname (relative to CASEDIR)
 //
 // An optional CASEDIR/init.bin overrides the initial primitive field
 // (6 x padded points per block, doubles, the reference's layout), after
@@ -32,14 +39,17 @@ int main(int argc, char** argv) {
   }
   const std::string dir = argv[1];
   int nsteps = -1, device = 0;
-  bool ops = false, check_only = false;
-  std::string hist;
+  bool ops = false, check_only = false, output = true;
+  std::string hist, restart, save;
   for (int a = 2; a < argc; ++a) {
     const std::string s = argv[a];
     if (s == "--ops") ops = true;
     else if (s == "--check") check_only = true;
     else if (s == "--hist" && a + 1 < argc) hist = argv[++a];
     else if (s == "--device" && a + 1 < argc) device = std::atoi(argv[++a]);
+    else if (s == "--restart" && a + 1 < argc) restart = argv[++a];
+    else if (s == "--save" && a + 1 < argc) save = argv[++a];
+    else if (s == "--no-output") output = false;
     else nsteps = std::atoi(s.c_str());
   }
   try {
@@ -62,7 +72,10 @@ int main(int argc, char** argv) {
       return 0;
     }
     preprocess::FlowFieldSet fs(grid, bcd, params, device);
-    {
+    long long step0 = 0;
+    if (!restart.empty()) {
+      step0 = fs.load_state(restart);
+    } else {
       std::ifstream init(dir + "/init.bin", std::ios::binary);
       if (init.good()) {
         for (int b = 0; b < fs.nblocks(); ++b) {
@@ -79,7 +92,7 @@ int main(int argc, char** argv) {
       solver::HaloExchanger halo;
       for (int n = 0; n < nsteps; ++n) {
         const solver::ResidualStats r = solver::step(bcd, fs, params, halo);
-        std::printf("step %6d  rms %.10e  max %.10e  at block %d (%d,%d,%d) var %d\n", n + 1, r.rms_residual,
+        std::printf("step %6lld  rms %.10e  max %.10e  at block %d (%d,%d,%d) var %d\n", step0 + n + 1, r.rms_residual,
                     r.max_residual, r.max_loc[0] + 1, r.max_loc[1], r.max_loc[2], r.max_loc[3], r.max_loc[4]);
         rec.insert(rec.end(), {r.rms_residual, r.max_residual, (double)r.max_loc[0], (double)r.max_loc[1],
                                (double)r.max_loc[2], (double)r.max_loc[3], (double)r.max_loc[4], params.step.phydtime});
@@ -88,11 +101,21 @@ int main(int argc, char** argv) {
       const auto h = solver::run(fs, nsteps);
       for (int n = 0; n < nsteps; ++n) {
         const auto& r = h[n];
-        std::printf("step %6d  rms %.10e  max %.10e  at block %d (%d,%d,%d) var %d\n", n + 1, r.rms_residual,
+        std::printf("step %6lld  rms %.10e  max %.10e  at block %d (%d,%d,%d) var %d\n", step0 + n + 1, r.rms_residual,
                     r.max_residual, r.max_loc[0] + 1, r.max_loc[1], r.max_loc[2], r.max_loc[3], r.max_loc[4]);
         rec.insert(rec.end(), {r.rms_residual, r.max_residual, (double)r.max_loc[0], (double)r.max_loc[1],
                                (double)r.max_loc[2], (double)r.max_loc[3], (double)r.max_loc[4], r.phydtime});
       }
+    }
+    const long long last = step0 + nsteps;
+    if (!save.empty()) fs.save_state(save, last);
+    if (output) {
+      // relative flownames are relative to the case directory, the
+      // reference's working directory
+      core::Params out = params;
+      if (!out.filename.flowname.empty() && out.filename.flowname[0] != '/')
+        out.filename.flowname = dir + "/" + out.filename.flowname;
+      postprocess::PostProcess::write_solution(fs, grid, out, (int)last);
     }
     if (!hist.empty()) {
       std::ofstream out(hist, std::ios::binary);

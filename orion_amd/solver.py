@@ -57,6 +57,7 @@ def make_params(case) -> SimpleNamespace:
         flowtype=SimpleNamespace(nvis=int(inp.get("nvis", 1))),
         control=SimpleNamespace(method=int(inp.get("method", 1)), nomax=case.nsteps),
         gcl_cic=SimpleNamespace(gcl=int(inp.get("gcl", 1))),
+        filename=SimpleNamespace(flowname="output/"),  # what Case.write puts in param.dat
     )
 
 
@@ -145,6 +146,30 @@ class FlowFieldSet:
     def set_stream(self, stream_ptr: Optional[int]) -> None:
         check(_lib.load().orion_hip_set_stream(self.ctx, C.c_void_p(stream_ptr or 0)), "set_stream")
 
+    # ---------------------------------------------------------------- output
+    def output_fields(self, b: int) -> np.ndarray:
+        """The VTS point fields of block b, formed on the device as float32:
+        Density, Pressure, Temperature, Mach (n interior nodes each, i
+        fastest), then Velocity (u, v, w per node); 7 n values
+        (PostProcess.cpp:183-240)."""
+        i, j, k = (int(v) for v in self.case.dims[b])
+        out = np.empty(7 * i * j * k, dtype=np.float32)
+        check(_lib.load().orion_hip_output_fields(self.ctx, b, out.ctypes.data_as(C.POINTER(C.c_float)), out.size),
+              "output_fields")
+        return out
+
+    def save_state(self, path: str, step: int = 0) -> None:
+        """Write the restart state (every block's padded prim, after a header
+        with the block sizes and ``step``)."""
+        check(_lib.load().orion_hip_save_state(self.ctx, str(path).encode(), int(step)), "save_state")
+
+    def load_state(self, path: str) -> int:
+        """Restore a state written by save_state for the same grid; returns
+        its step."""
+        st = C.c_longlong()
+        check(_lib.load().orion_hip_load_state(self.ctx, str(path).encode(), C.byref(st)), "load_state")
+        return int(st.value)
+
 
 # ---------------------------------------------------------------------------
 # operators (same names as the reference's classes)
@@ -198,6 +223,21 @@ class ResidualMonitor:
         s = ResidualStats()
         check(_lib.load().orion_hip_residual(fs.ctx, C.byref(s)), "residual")
         return _stats(s)
+
+
+class PostProcess:
+    """postprocess::PostProcess (src/postprocess/PostProcess.hpp:10-33)."""
+
+    @staticmethod
+    def write_solution(fs: FlowFieldSet, grid, params, step: int) -> None:
+        """PostProcess::write_solution (src/postprocess/PostProcess.cpp:44-123):
+        one VTS file per block and the VTM index, named from
+        params.filename.flowname.  ``grid`` is the case (its grid
+        coordinates are the files' points)."""
+        flowname = getattr(getattr(params, "filename", None), "flowname", "") or ""
+        xyz = np.ascontiguousarray(grid.xyz_flat(), dtype=np.float64)
+        check(_lib.load().orion_hip_write_solution(fs.ctx, flowname.encode(), int(step),
+                                                   xyz.ctypes.data_as(C.POINTER(C.c_double))), "write_solution")
 
 
 class OrionSolver:

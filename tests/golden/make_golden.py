@@ -17,11 +17,14 @@ outputs as small .npz files:
                         order)
   bc_variety.npz        freestream/extrapolation/symmetry/slip-wall block
                         (nvis=0), 10 steps: history and final fields
+  vts_bc_variety.npz    the files PostProcess::write_solution writes after
+                        3 steps of the viscous bc_variety block (the VTS of
+                        the block and the VTM index), byte for byte
 
 Inputs are regenerated from the seeded generators in orion_amd.cases, so
 each fixture records only the generator call and the reference's outputs.
 
-usage: python tests/golden/make_golden.py
+usage: python tests/golden/make_golden.py [fixture ...]   (default: all)
 """
 import json
 import os
@@ -53,8 +56,27 @@ def run(case, nsteps, dump_steps=0):
     return hist, read
 
 
-def fixtures():
+def vts_fixture():
+    c = _cases.bc_variety(nvis=1, nsteps=3)
+    tmp = tempfile.mkdtemp(prefix="orion_golden_")
+    casedir = os.path.join(tmp, "case")
+    binding.run_reference(c, casedir, os.path.join(tmp, "out"), 3, vts=True)
+    outdir = os.path.join(casedir, "output")
+    names = sorted(os.listdir(outdir))
+    d = dict(generator="tests._cases.bc_variety(nvis=1, nsteps=3); PostProcess::write_solution after step 3",
+             names=np.array(names))
+    for n, name in enumerate(names):
+        with open(os.path.join(outdir, name), "rb") as f:
+            d["file_%d" % n] = np.frombuffer(f.read(), dtype=np.uint8)
+    return d
+
+
+def fixtures(only=None):
     out = {}
+    if only and "vts_bc_variety" in only:
+        out["vts_bc_variety"] = vts_fixture()
+    if only and not set(only) - {"vts_bc_variety"}:
+        return out
     c = cases.channel_wall_case(seed=1, nsteps=200)
     hist, read = run(c, 200)
     fin = read("final.bin")[0]
@@ -92,13 +114,16 @@ def fixtures():
     fin = read("final.bin")[0]
     out["bc_variety"] = dict(generator="tests._cases.bc_variety(nvis=0, nsteps=10)", hist=hist,
                              prim=interior(fin["prim"]), dq=interior(fin["dq"]))
+    out["vts_bc_variety"] = vts_fixture()
+    if only:
+        out = {k: v for k, v in out.items() if k in only}
     return out
 
 
 def main():
     if not binding.have_reference():
         raise SystemExit("oracle/_ref/ref_harness is missing: run make -C oracle ref (needs /root/reference)")
-    for name, d in fixtures().items():
+    for name, d in fixtures(sys.argv[1:] or None).items():
         path = os.path.join(HERE, name + ".npz")
         gen = d.pop("generator")
         np.savez_compressed(path, generator=np.array(gen), **d)
