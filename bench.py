@@ -20,6 +20,7 @@ C oracle (a port of the reference) timed on this host on a bounded sample.
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -197,6 +198,22 @@ def stage_traffic(workload, groups):
     return out
 
 
+def serving_run(nreq, slots):
+    """The serving form (orion_amd.serving.measure): a seeded Poisson stream
+    of steady channel solves, each to a 1e-8 RMS residual drop relative to
+    its first step (at most 3000 steps), served ``slots`` at a time with
+    continuous admission, and the same stream solved one request at a time
+    (the sequential plain-GPU baseline); with the check that every request's
+    residual history is the same in both runs.  It runs as a child process,
+    because the hardware queue count it asks HIP for (one per resident
+    request) must be set before HIP starts in a process."""
+    cmd = [sys.executable, "-u", "-m", "orion_amd.serving", "--requests", str(nreq), "--slots", str(slots)]
+    p = subprocess.run(cmd, cwd=os.path.dirname(os.path.abspath(__file__)), stdout=subprocess.PIPE, timeout=900)
+    if p.returncode != 0:
+        raise RuntimeError("serving measurement failed with exit status %d" % p.returncode)
+    return json.loads(p.stdout.decode().strip().splitlines()[-1])
+
+
 def cpu_baseline(case, threads, seconds):
     """The C oracle (a restatement of the reference path, OpenMP over blocks)
     on the same case: whole pseudo-steps until ``seconds`` have passed."""
@@ -225,6 +242,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
+    ap.add_argument("--serve-requests", type=int, default=16)
+    ap.add_argument("--serve-slots", type=int, default=16)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (HBM counter calibration)")
     ap.add_argument("--summarize-counters", nargs=4, metavar=("FETCH", "WRITE", "COPY_FETCH", "COPY_WRITE"),
@@ -295,6 +315,8 @@ def main():
         "residual_first_last": [hist[0].rms_residual, hist[-1].rms_residual],
     }
     s.fs.close()
+    if not a.no_serve:
+        out["serving"] = serving_run(a.serve_requests, a.serve_slots)
     if not a.no_cpu:
         threads = a.cpu_threads or min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(case, threads, a.cpu_seconds)

@@ -88,6 +88,22 @@ const char* orion_hip_last_error(void);
 int orion_hip_ctx_create(int nblocks, const int* dims, const double* xyz, int nregions,
                          const orion_region_desc* regions, const orion_inputs* in, int keep_q,
                          orion_hip_ctx** out);
+/* No counterpart in the reference (one solver per process there): a context
+ * that batches nreq independent solves, for serving many small requests.
+ * Request r owns req_nblocks[r] consecutive blocks (their sum is nblocks)
+ * and has its own inputs, inputs[r]; dims, xyz and regions are laid out as
+ * for orion_hip_ctx_create over all blocks (region block numbers over all
+ * blocks), and an interface may not join blocks of two requests.  The
+ * requests must share nvis, method, csrv, ntmst and gcl.  Every kernel of a
+ * step covers all requests; each request's results are those it has in a
+ * context of its own, bit for bit.  orion_hip_run then reports nsteps x
+ * nreq residuals; the per-operator residual and time step calls refuse a
+ * batch.  orion_hip_ctx_create is the batch of one request. */
+int orion_hip_batch_create(int nreq, const int* req_nblocks, int nblocks, const int* dims, const double* xyz,
+                           int nregions, const orion_region_desc* regions, const orion_inputs* inputs, int keep_q,
+                           orion_hip_ctx** out);
+/* Requests in the context (1 unless batched). */
+int orion_hip_requests(const orion_hip_ctx* ctx);
 void orion_hip_ctx_destroy(orion_hip_ctx* ctx);
 
 /* Device pointer and element count of one block field (OrionArray::devicePtr,
@@ -123,8 +139,9 @@ int orion_hip_residual(orion_hip_ctx* ctx, orion_residual_stats* out);
 
 /* nsteps pseudo-time steps in the order of OrionSolver::solve
  * (src/core/OrionSolver.cpp:139-209, without the debug block at :156-201).
- * hist (nsteps entries, may be NULL) receives each step's residual; it is
- * read back once, after the last step. */
+ * hist (nsteps x requests entries, step-major, may be NULL) receives each
+ * step's residual per request, max_loc's block numbered within the
+ * request; it is read back once, after the last step. */
 int orion_hip_run(orion_hip_ctx* ctx, int nsteps, orion_residual_stats* hist);
 
 /* Measurement helper with no counterpart in the reference: runs nsteps
@@ -177,6 +194,14 @@ int orion_hip_load_state(orion_hip_ctx* ctx, const char* path, long long* step);
 int orion_hip_synchronize(orion_hip_ctx* ctx);
 /* Use a caller-owned hipStream_t (NULL restores the context's own stream). */
 int orion_hip_set_stream(orion_hip_ctx* ctx, void* stream);
+/* No counterpart in the reference.  With viscosity the step runs the
+ * inviscid face fluxes on a second stream beside the viscous chain (on, the
+ * default): that fills a GPU that one context has to itself.  Off runs the
+ * step on one stream, for callers that keep many contexts busy at once
+ * (their steps then overlap each other instead, without a second stream per
+ * context sharing the hardware queues).  The kernels and the order of the
+ * updates of dq are the same either way, so are the results. */
+int orion_hip_set_overlap(orion_hip_ctx* ctx, int on);
 /* Device bytes held by the context (arena plus maps). */
 size_t orion_hip_device_bytes(const orion_hip_ctx* ctx);
 /* Interior node count (the unit of cell-steps). */

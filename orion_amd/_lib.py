@@ -9,6 +9,7 @@ from __future__ import annotations
 import ctypes as C
 import os
 import re
+import threading
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_build", "liborion_hip.so")
@@ -46,6 +47,7 @@ class OrionHipError(RuntimeError):
 
 
 _lib = None
+_load_lock = threading.Lock()  # the serving workers may load from several threads
 
 
 def declared_symbols():
@@ -59,6 +61,13 @@ def load():
     global _lib
     if _lib is not None:
         return _lib
+    with _load_lock:
+        if _lib is None:
+            _lib = _open()
+    return _lib
+
+
+def _open():
     if not os.path.exists(LIB_PATH):
         raise OrionHipError("HIP library not built: %s is missing (run python orion_amd/build.py)" % LIB_PATH)
     L = C.CDLL(LIB_PATH)
@@ -68,6 +77,9 @@ def load():
         "orion_hip_last_error": ([], C.c_char_p),
         "orion_hip_ctx_create": ([C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double), C.c_int,
                                   C.POINTER(RegionDesc), C.POINTER(Inputs), C.c_int, C.POINTER(P)], C.c_int),
+        "orion_hip_batch_create": ([C.c_int, C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double),
+                                   C.c_int, C.POINTER(RegionDesc), C.POINTER(Inputs), C.c_int, C.POINTER(P)], C.c_int),
+        "orion_hip_requests": ([P], C.c_int),
         "orion_hip_ctx_destroy": ([P], None),
         "orion_hip_field": ([P, C.c_int, C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_longlong)], C.c_int),
         "orion_hip_upload": ([P, C.c_int, C.c_int, C.POINTER(C.c_double), C.c_longlong], C.c_int),
@@ -86,6 +98,7 @@ def load():
         "orion_hip_profile_run": ([P, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_longlong)], C.c_int),
         "orion_hip_synchronize": ([P], C.c_int),
         "orion_hip_set_stream": ([P, C.c_void_p], C.c_int),
+        "orion_hip_set_overlap": ([P, C.c_int], C.c_int),
         "orion_hip_device_bytes": ([P], C.c_size_t),
         "orion_hip_interior_nodes": ([P], C.c_longlong),
         "orion_hip_output_fields": ([P, C.c_int, C.POINTER(C.c_float), C.c_longlong], C.c_int),
@@ -100,7 +113,6 @@ def load():
         fn = getattr(L, name)
         fn.argtypes = args
         fn.restype = res
-    _lib = L
     return L
 
 

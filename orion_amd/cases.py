@@ -417,3 +417,47 @@ def uneven_lattice_case(seed: int = 11, nsteps: int = 1, target: float = 8.0e6) 
 
 def scaled_channel(ni: int, nj: int, nk: int, **kw) -> Case:
     return channel_wall_case(dims=(ni, nj, nk), **kw)
+
+
+def bump_channel_case(seed: int, nodes: float, moo: float, reynolds: float, bump: float, *,
+                      upper_wall: bool = False, nsteps: int = 1, cfl: float = 5.0) -> Case:
+    """A steady channel-bump request of the serving stream: one block of
+    about ``nodes`` interior nodes (aspect 4 : 2 : 1 in i, j, k), x in
+    [0, 2], a cosine bump of height ``bump`` on the lower wall over
+    x in [0.6, 1.4], wall stretching 1.05 in j, height 1; viscous adiabatic
+    lower wall, far-field inflow and outflow, upper wall viscous or
+    far-field, periodic in k.  Single blocks, because the reference's
+    multi-block solutions do not converge (see lattice_case)."""
+    n = max(float(nodes), 8.0 * 7 * 7 * 7)
+    nk = max(7, int(round((n / 8.0) ** (1.0 / 3.0))))
+    nj = max(9, 2 * nk)
+    ni = max(13, int(round(n / (nj * nk))))
+    x = np.linspace(0.0, 2.0, ni)
+    t = np.clip((x - 0.6) / 0.8, 0.0, 1.0)
+    ylow = bump * 0.5 * (1.0 - np.cos(2.0 * np.pi * t))
+    g = 1.05 ** np.arange(nj - 1)
+    eta = np.concatenate([[0.0], np.cumsum(g)]) / g.sum()  # 0..1, clustered at the wall
+    dz = 2.0 / (ni - 1)
+    z = np.arange(nk) * dz
+    Z, E, X = np.meshgrid(z, eta, x, indexing="ij")
+    YL = np.broadcast_to(ylow[None, None, :], X.shape)
+    Y = YL + E * (1.0 - YL)
+    xyz = [np.stack([X, Y, Z])]
+    regs = [
+        Region(0, 4, (1, 1, 1), (1, nj, nk)),
+        Region(0, 4, (ni, 1, 1), (ni, nj, nk)),
+        Region(0, 2, (1, 1, 1), (ni, 1, nk)),
+        Region(0, 2 if upper_wall else 4, (1, nj, 1), (ni, nj, nk)),
+        Region(0, -1, (1, 1, 1), (ni, nj, 1), (1, 1, nk), (ni, nj, nk), nbt=1, ibcwin=6),
+        Region(0, -1, (1, 1, nk), (ni, nj, nk), (1, 1, 1), (ni, nj, 1), nbt=1, ibcwin=5),
+    ]
+    # seeded Gaussian density bump of 1e-2, as in channel_wall_case
+    rng = np.random.default_rng(seed)
+    x0 = np.array([rng.uniform(0.4, 1.6), rng.uniform(0.3, 0.7), z[nk // 2]])
+    sig = rng.uniform(0.15, 0.3)
+    r2 = (X - x0[0]) ** 2 + (Y - x0[1]) ** 2 + (Z - x0[2]) ** 2
+    rho = 1.0 + 1.0e-2 * np.exp(-r2 / sig ** 2)
+    inputs = dict(DEFAULT_INPUTS, moo=float(moo), reynolds=float(reynolds), cfl=float(cfl))
+    return Case("%s_%d_%dx%dx%d" % ("bump" if bump > 0 else "channel", seed, ni, nj, nk),
+                np.array([[ni, nj, nk]], dtype=np.int32), xyz, regs,
+                inputs, nsteps, [rho])

@@ -16,6 +16,7 @@
 #include <filesystem>
 #include <iomanip>
 #include <map>
+#include <mutex>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -28,6 +29,14 @@ using namespace orion_hip;
 
 static thread_local std::string g_err;
 static thread_local int g_device = 0;
+
+// Contexts may be used from several host threads, one context per thread
+// (the serving workers).  Creating or destroying a context makes
+// synchronous allocations and copies on the legacy stream, which HIP
+// refuses while another thread captures a graph; these and the capture of
+// a context's step are therefore serialised.  (Creation destroys on its
+// error paths, hence recursive.)
+static std::recursive_mutex g_capture_mutex;
 
 // records the message returned by orion_hip_last_error; returns -1
 namespace orion_hip {
@@ -66,13 +75,23 @@ struct orion_hip_ctx {
   int nb = 0;
   std::vector<BlockDev> hb;
   BlockDev* db = nullptr;
-  Params p{};
+  Params p{};                          // request 0's inputs (the shared ones for a batch)
+  // requests batched in the context: blocks req_first[r] .. req_first[r+1]
+  // are request r's; each has its own residual and time step per step
+  int nreq = 1;
+  std::vector<int> req_first;
+  int* d_rseg = nullptr;               // [nreq+1] each request's update partials
+  long long* d_rtot = nullptr;         // [nreq] 5 x each request's interior nodes
+  int* d_rdone = nullptr;              // residual workgroups finished (left at 0)
   char* arena = nullptr;
   size_t arena_bytes = 0;
   size_t map_bytes = 0;
   Space pad, intr, face[3], line[3];
   Space fpair[3];                      // face pairs (2g, 2g+1) per line along d
   Space ltask[3];                      // six boundary-closure tasks per line along d
+  // the same three spaces each joined over the directions (virtual blocks
+  // d * nb + b): one launch per step for all directions
+  Space face3, fpair3, ltask3;
   Space dtile;                         // flux differences: 16 x 16 (i, j) nodes of a k plane
   Space ptile;                         // LU-SGS pack tiles: 16 x 16 (march, col) cells of one row
   SweepArrays sw{};                    // LU-SGS working set in sweep layout
@@ -108,6 +127,7 @@ struct orion_hip_ctx {
   hipGraphExec_t gexec = nullptr;
   hipStream_t graph_stream = nullptr;
   int use_graph = 1;
+  bool overlap = true;  // inviscid face fluxes on side_stream (viscous steps)
 };
 
 static int dalloc(orion_hip_ctx* c, void** p, size_t bytes) {
@@ -119,10 +139,18 @@ static int dalloc(orion_hip_ctx* c, void** p, size_t bytes) {
   return 0;
 }
 
+// Every set-up copy and fill goes to the context's stream, in order after
+// what was queued there before (the arena's zeroing among it): the stream
+// does not block on the legacy stream, so a legacy-stream copy could land
+// before an earlier fill of the same memory that is still queued behind
+// other contexts' work.  The wait keeps the host vector alive until read.
 template <class T>
 static int upload_vec(orion_hip_ctx* c, T** dst, const std::vector<T>& v) {
   if (dalloc(c, (void**)dst, sizeof(T) * std::max<size_t>(v.size(), 1))) return -1;
-  if (!v.empty()) HIPCHK(hipMemcpy(*dst, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+  if (!v.empty()) {
+    HIPCHK(hipMemcpyAsync(*dst, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
   return 0;
 }
 
@@ -146,6 +174,10 @@ static int make_space(orion_hip_ctx* c, Space& s, const std::vector<long long>& 
 
 static Launch LA(orion_hip_ctx* c, const Space& s, hipStream_t st = nullptr) {
   return Launch{st ? st : c->stream, c->db, s.d, c->nb, Span{s.d, s.wgb, s.wg0, c->nb}, s.nwg};
+}
+// a space joined over the three directions (3 nb virtual blocks)
+static Launch LA3(orion_hip_ctx* c, const Space& s, hipStream_t st = nullptr) {
+  return Launch{st ? st : c->stream, c->db, s.d, 3 * c->nb, Span{s.d, s.wgb, s.wg0, 3 * c->nb}, s.nwg};
 }
 
 // ---------------------------------------------------------------------------
@@ -469,7 +501,9 @@ static int build_topology(orion_hip_ctx* c, const std::vector<std::vector<Reg>>&
           for (int i = rg.d.s_st[0] - 1 + NG; i <= rg.d.s_ed[0] - 1 + NG; ++i)
             if (i >= 0 && j >= 0 && k >= 0 && i < B.nx && j < B.ny && k < B.nz) mask[pidx(B, i, j, k)] = 1;
     }
-    HIPCHK(hipMemcpy(B.frozen, mask.data(), B.P, hipMemcpyHostToDevice));
+    // the mask lies in the arena, whose zeroing is queued on the stream
+    HIPCHK(hipMemcpyAsync(B.frozen, mask.data(), B.P, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
   }
   return 0;
 }
@@ -541,14 +575,14 @@ static int build_sweep(orion_hip_ctx* c) {
       dalloc(c, (void**)&c->d_flags, sizeof(unsigned long long) * FLAG_STRIDE * std::max(nt, 1)) ||
       dalloc(c, (void**)&c->d_epoch, sizeof(unsigned long long)) || dalloc(c, (void**)&c->d_err, sizeof(int)))
     return -1;
-  HIPCHK(hipMemset(c->sw.dq, 0, sizeof(double2) * SW_DQ * n));
-  HIPCHK(hipMemset(c->sw.own, 0, sizeof(double2) * SW_OWN * n));
-  HIPCHK(hipMemset(c->sw.D, 0, sizeof(double2) * SW_D * n));
-  HIPCHK(hipMemset(c->sw.ga, 0, sizeof(double2) * SW_GA * n));
-  HIPCHK(hipMemset(c->sw.pre, 0, sizeof(double2) * SW_PRE * n));
-  HIPCHK(hipMemset(c->d_flags, 0, sizeof(unsigned long long) * FLAG_STRIDE * std::max(nt, 1)));
-  HIPCHK(hipMemset(c->d_epoch, 0, sizeof(unsigned long long)));
-  HIPCHK(hipMemset(c->d_err, 0, sizeof(int)));
+  HIPCHK(hipMemsetAsync(c->sw.dq, 0, sizeof(double2) * SW_DQ * n, c->stream));
+  HIPCHK(hipMemsetAsync(c->sw.own, 0, sizeof(double2) * SW_OWN * n, c->stream));
+  HIPCHK(hipMemsetAsync(c->sw.D, 0, sizeof(double2) * SW_D * n, c->stream));
+  HIPCHK(hipMemsetAsync(c->sw.ga, 0, sizeof(double2) * SW_GA * n, c->stream));
+  HIPCHK(hipMemsetAsync(c->sw.pre, 0, sizeof(double2) * SW_PRE * n, c->stream));
+  HIPCHK(hipMemsetAsync(c->d_flags, 0, sizeof(unsigned long long) * FLAG_STRIDE * std::max(nt, 1), c->stream));
+  HIPCHK(hipMemsetAsync(c->d_epoch, 0, sizeof(unsigned long long), c->stream));
+  HIPCHK(hipMemsetAsync(c->d_err, 0, sizeof(int), c->stream));
   return 0;
 }
 
@@ -570,6 +604,7 @@ int orion_hip_backend_init(int device) {
 
 void orion_hip_ctx_destroy(orion_hip_ctx* c) {
   if (!c) return;
+  std::lock_guard<std::recursive_mutex> guard(g_capture_mutex);
   hipSetDevice(c->device);
   if (c->stream) hipStreamSynchronize(c->stream);
   for (void* p : c->allocs) hipFree(p);
@@ -583,11 +618,13 @@ void orion_hip_ctx_destroy(orion_hip_ctx* c) {
   delete c;
 }
 
-int orion_hip_ctx_create(int nblocks, const int* dims, const double* xyz, int nregions,
-                         const orion_region_desc* regions, const orion_inputs* in, int keep_q,
-                         orion_hip_ctx** out) {
+static int create_ctx(int nreq, const int* req_nblocks, int nblocks, const int* dims, const double* xyz,
+                      int nregions, const orion_region_desc* regions, const orion_inputs* in, int keep_q,
+                      orion_hip_ctx** out) {
   *out = nullptr;
   if (nblocks <= 0) return fail("nblocks must be positive");
+  if (nreq <= 0 || !req_nblocks || !in) return fail("a context needs at least one request");
+  std::lock_guard<std::recursive_mutex> guard(g_capture_mutex);
   orion_hip_ctx* c = new orion_hip_ctx();
   auto bail = [&](int rc) {
     orion_hip_ctx_destroy(c);
@@ -606,6 +643,21 @@ int orion_hip_ctx_create(int nblocks, const int* dims, const double* xyz, int nr
   c->nb = nblocks;
   c->keep_q = keep_q;
   init_params(c->p, in);
+  // requests: their blocks and inputs; what selects kernels or is a
+  // constant of the discretisation must agree across a batch
+  c->nreq = nreq;
+  c->req_first.assign(nreq + 1, 0);
+  std::vector<Params> preq(nreq);
+  for (int r = 0; r < nreq; ++r) {
+    if (req_nblocks[r] <= 0) return bail(fail("request %d has no blocks", r));
+    c->req_first[r + 1] = c->req_first[r] + req_nblocks[r];
+    init_params(preq[r], &in[r]);
+    const Params& a = preq[r];
+    const Params& z = preq[0];
+    if (a.nvis != z.nvis || a.method != z.method || a.csrv != z.csrv || a.ntmst != z.ntmst || a.gcl != z.gcl)
+      return bail(fail("request %d: a batch's requests must share nvis, method, csrv, ntmst and gcl", r));
+  }
+  if (c->req_first[nreq] != nblocks) return bail(fail("the requests' block counts do not add up to nblocks"));
   // blocks
   c->hb.resize(nblocks);
   std::vector<long long> lines_tot(nblocks);
@@ -623,6 +675,13 @@ int orion_hip_ctx_create(int nblocks, const int* dims, const double* xyz, int nr
     B.nz = B.kdim + 2 * NG;
     B.P = (long long)B.nx * B.ny * B.nz;
     lines_tot[b] = nlines(B, 0) + nlines(B, 1) + nlines(B, 2);
+    int r = 0;
+    while (c->req_first[r + 1] <= b) ++r;
+    B.req = r;
+    B.par = preq[r];
+    const double cp = 1.0 / ((B.par.gama - 1.0) * B.par.moo * B.par.moo);
+    B.cp_prl = cp / B.par.prl;
+    B.re_inv = (std::fabs(B.par.reynolds) > 1.0e-30) ? 1.0 / B.par.reynolds : 0.0;
   }
   // arena
   long long total = 0;
@@ -685,6 +744,8 @@ int orion_hip_ctx_create(int nblocks, const int* dims, const double* xyz, int nr
         return bail(fail("block %d region %d: window outside the block", d.block + 1, r + 1));
       if (bt < 0) {
         if (d.nbt < 1 || d.nbt > nblocks) return bail(fail("block %d region %d: bad target block", d.block + 1, r + 1));
+        if (c->hb[d.nbt - 1].req != c->hb[d.block].req)
+          return bail(fail("block %d region %d: an interface joins blocks of two requests", d.block + 1, r + 1));
         const BlockDev& T = c->hb[d.nbt - 1];
         const int tm[3] = {T.idim, T.jdim, T.kdim};
         const int ta = std::abs(d.t_st[m]), te = std::abs(d.t_ed[m]);
@@ -737,6 +798,16 @@ int orion_hip_ctx_create(int nblocks, const int* dims, const double* xyz, int nr
     if (make_space(c, c->ltask[d], cp2)) return bail(-1);
   }
   {
+    std::vector<long long> f3, p3, l3;
+    for (int d = 0; d < 3; ++d)
+      for (int b = 0; b < nblocks; ++b) {
+        f3.push_back(cf[d][b]);
+        p3.push_back(c->fpair[d].h[b + 1] - c->fpair[d].h[b]);
+        l3.push_back(c->ltask[d].h[b + 1] - c->ltask[d].h[b]);
+      }
+    if (make_space(c, c->face3, f3) || make_space(c, c->fpair3, p3) || make_space(c, c->ltask3, l3)) return bail(-1);
+  }
+  {
     std::vector<long long> ct(nblocks);
     for (int b = 0; b < nblocks; ++b) {
       const BlockDev& B = c->hb[b];
@@ -762,14 +833,33 @@ int orion_hip_ctx_create(int nblocks, const int* dims, const double* xyz, int nr
       dalloc(c, (void**)&c->d_ukey, sizeof(long long) * c->ptile.nwg) ||
       dalloc(c, (void**)&c->d_count, sizeof(unsigned long long) * 2) || dalloc(c, (void**)&c->d_step, sizeof(int)))
     return bail(-1);
+  {
+    // each request's update partials (whole workgroups per block, in tile
+    // order) and residual term count
+    std::vector<int> seg(nreq + 1, 0);
+    std::vector<long long> tot(nreq, 0);
+    for (int r = 0; r < nreq; ++r) {
+      long long w = 0;
+      for (int b = c->req_first[r]; b < c->req_first[r + 1]; ++b)
+        w += (c->ptile.h[b + 1] - c->ptile.h[b] + SPAN_TB - 1) / SPAN_TB;
+      seg[r + 1] = seg[r] + (int)w;
+      tot[r] = 5 * (c->intr.h[c->req_first[r + 1]] - c->intr.h[c->req_first[r]]);
+    }
+    if (seg[nreq] != c->ptile.nwg) return bail(fail("internal: request partials do not cover the update"));
+    if (upload_vec(c, &c->d_rseg, seg) || upload_vec(c, &c->d_rtot, tot) || dalloc(c, (void**)&c->d_rdone, sizeof(int)))
+      return bail(-1);
+    if (hipMemsetAsync(c->d_rdone, 0, sizeof(int), c->stream) != hipSuccess) return bail(fail("memset"));
+  }
   if (const char* e = std::getenv("ORION_HIP_GRAPH")) c->use_graph = std::atoi(e) != 0;
-  // initial state (InitialCondition.cpp:49-80): freestream prim everywhere
+  // initial state (InitialCondition.cpp:49-80): freestream prim everywhere,
+  // each block with its request's freestream
   const Params& p = c->p;
-  const double prim0[6] = {p.roo, p.uoo, p.voo, p.woo, p.poo, p.too};
-  const double v2 = p.uoo * p.uoo + p.voo * p.voo + p.woo * p.woo;
-  const double q0[5] = {p.roo, p.roo * p.uoo, p.roo * p.voo, p.roo * p.woo, p.poo / (p.gama - 1.0) + 0.5 * p.roo * v2};
   for (int b = 0; b < nblocks; ++b) {
     const BlockDev& B = c->hb[b];
+    const Params& f = B.par;
+    const double prim0[6] = {f.roo, f.uoo, f.voo, f.woo, f.poo, f.too};
+    const double v2 = f.uoo * f.uoo + f.voo * f.voo + f.woo * f.woo;
+    const double q0[5] = {f.roo, f.roo * f.uoo, f.roo * f.voo, f.roo * f.woo, f.poo / (f.gama - 1.0) + 0.5 * f.roo * v2};
     for (int m = 0; m < 6; ++m) launch_fill(B.prim + m * B.P, B.P, prim0[m], c->stream);
     if (B.q)
       for (int m = 0; m < 5; ++m) launch_fill(B.q + m * B.P, B.P, q0[m], c->stream);
@@ -802,11 +892,26 @@ int orion_hip_ctx_create(int nblocks, const int* dims, const double* xyz, int nr
     for (int d = 0; d < 3; ++d) launch_face_geom(LA(c, c->face[d]), d, c->face[d].total());
     launch_lu_geom(LA(c, c->ptile), c->ptile.total(), c->sw);
   }
-  if (p.nvis != 0) launch_temperature(LA(c, c->pad), c->pad.total(), p.gama, p.moo);
+  if (p.nvis != 0) launch_temperature(LA(c, c->pad), c->pad.total());
   if (hipStreamSynchronize(c->stream) != hipSuccess) return bail(fail("setup failed: %s", hipGetErrorString(hipGetLastError())));
   *out = c;
   return 0;
 }
+
+int orion_hip_ctx_create(int nblocks, const int* dims, const double* xyz, int nregions,
+                         const orion_region_desc* regions, const orion_inputs* in, int keep_q,
+                         orion_hip_ctx** out) {
+  return create_ctx(1, &nblocks, nblocks, dims, xyz, nregions, regions, in, keep_q, out);
+}
+
+int orion_hip_batch_create(int nreq, const int* req_nblocks, int nblocks, const int* dims, const double* xyz,
+                           int nregions, const orion_region_desc* regions, const orion_inputs* inputs, int keep_q,
+                           orion_hip_ctx** out) {
+  if (!out) return fail("batch_create: null output");
+  return create_ctx(nreq, req_nblocks, nblocks, dims, xyz, nregions, regions, inputs, keep_q, out);
+}
+
+int orion_hip_requests(const orion_hip_ctx* c) { return c ? c->nreq : 0; }
 
 static int field_info(orion_hip_ctx* c, int b, int f, double** ptr, long long* n) {
   if (!c || b < 0 || b >= c->nb) return fail("bad context or block");
@@ -857,7 +962,7 @@ static int check_launch(const char* what) {
 
 int orion_hip_update_temperature(orion_hip_ctx* c) {
   if (c->p.nvis == 0) return 0;
-  launch_temperature(LA(c, c->pad), c->pad.total(), c->p.gama, c->p.moo);
+  launch_temperature(LA(c, c->pad), c->pad.total());
   return check_launch("update_temperature");
 }
 
@@ -865,7 +970,7 @@ static void op_halo(orion_hip_ctx* c) { launch_halo(c->d_halo, c->n_halo, (doubl
 
 static void op_bc(orion_hip_ctx* c) {
   for (size_t rk = 0; rk < c->bc_first.size(); ++rk)
-    launch_bc(c->db, c->d_bc + c->bc_first[rk], c->bc_count[rk], c->bc_total[rk], c->p, c->stream);
+    launch_bc(c->db, c->d_bc + c->bc_first[rk], c->bc_count[rk], c->bc_total[rk], c->stream);
 }
 
 // first: the step's first right-hand-side kernel, which writes dq whole
@@ -873,13 +978,9 @@ static void op_bc(orion_hip_ctx* c) {
 // accumulate into dq as the reference does
 static void op_visc(orion_hip_ctx* c, bool first = false) {
   if (c->p.nvis != 1) return;
-  const double cp = 1.0 / ((c->p.gama - 1.0) * c->p.moo * c->p.moo);
-  const double cp_prl = cp / c->p.prl;
-  const double re = c->p.reynolds;
-  const double re_inv = (std::fabs(re) > 1.0e-30) ? 1.0 / re : 0.0;
   launch_visc_deriv(LA(c, c->intr), c->intr.total());
-  for (int d = 0; d < 3; ++d) launch_visc_face(LA(c, c->fpair[d]), d, c->fpair[d].total(), cp_prl);
-  launch_visc_diff(LA(c, c->dtile), c->dtile.total(), re_inv, first);
+  launch_visc_face(LA3(c, c->fpair3), -1, c->fpair3.total());
+  launch_visc_diff(LA(c, c->dtile), c->dtile.total(), first);
 }
 
 static void op_avg(orion_hip_ctx* c) {
@@ -890,10 +991,8 @@ static void op_avg(orion_hip_ctx* c) {
 // Inviscid face fluxes and boundary closures: they read the flow state
 // and the static face geometry only, and write fxi and fcov.
 static void op_inv_faces(orion_hip_ctx* c, hipStream_t st) {
-  for (int d = 0; d < 3; ++d) {
-    launch_inv_face(LA(c, c->face[d], st), d, c->face[d].total(), c->p.gama);
-    launch_inv_closure(LA(c, c->ltask[d], st), d, c->ltask[d].total(), c->p.gama);
-  }
+  launch_inv_face(LA3(c, c->face3, st), -1, c->face3.total(), c->p.gama);
+  launch_inv_closure(LA3(c, c->ltask3, st), -1, c->ltask3.total(), c->p.gama);
 }
 
 static void op_inv(orion_hip_ctx* c, bool first = false) {
@@ -908,16 +1007,17 @@ static void op_sweep(orion_hip_ctx* c, int type) {
 }
 
 static void op_update(orion_hip_ctx* c) {
-  launch_lu_pack(LA(c, c->ptile), c->ptile.total(), c->sw, c->p);
+  launch_lu_pack(LA(c, c->ptile), c->ptile.total(), c->sw);
   for (int t = 0; t < 4; ++t) op_sweep(c, t);
   launch_update(LA(c, c->ptile), c->ptile.total(), c->sw, c->p.gama, c->intr.d, c->d_upart,
-                c->d_upart + c->ptile.nwg, c->d_ukey);
+                c->d_upart + c->ptile.nwg, c->d_ukey, c->d_epoch);
 }
 
 // the residual of the update op_update just made, from its partials
-static void op_update_residual(orion_hip_ctx* c, double* out, const int* step) {
-  launch_residual_final(c->d_upart, c->d_upart + c->ptile.nwg, c->d_ukey, (int)c->ptile.nwg, 5 * c->intr.total(), out,
-                        step, c->stream);
+// (one row per request: out + 4 (step nreq + r))
+static void op_update_residual(orion_hip_ctx* c, double* out, int* step) {
+  launch_residual_final(c->d_upart, c->d_upart + c->ptile.nwg, c->d_ukey, c->d_rseg, c->d_rtot, c->nreq, out, step,
+                        c->d_rdone, c->stream);
 }
 
 // a sweep whose wait for an upstream tile ran out has left wrong values
@@ -929,11 +1029,13 @@ static int check_sweeps(orion_hip_ctx* c) {
   return 0;
 }
 
+// per-step outputs of n steps: one row per (step, request)
 static int ensure_res(orion_hip_ctx* c, int n) {
   if (n <= c->res_cap) return 0;
   // grow: old buffers stay owned by the context until destroy
-  if (dalloc(c, (void**)&c->d_res, sizeof(double) * 4 * n)) return -1;
-  if (dalloc(c, (void**)&c->d_dtmin, sizeof(unsigned long long) * n)) return -1;
+  const size_t rows = (size_t)n * c->nreq;
+  if (dalloc(c, (void**)&c->d_res, sizeof(double) * 4 * rows)) return -1;
+  if (dalloc(c, (void**)&c->d_dtmin, sizeof(unsigned long long) * rows)) return -1;
   c->res_cap = n;
   if (c->gexec) {  // the captured step points at the old buffers
     hipGraphExecDestroy(c->gexec);
@@ -942,17 +1044,19 @@ static int ensure_res(orion_hip_ctx* c, int n) {
   return 0;
 }
 
-static void decode_stats(const orion_hip_ctx* c, const double* r, unsigned long long dtbits, orion_residual_stats* o) {
+// row of request req: the location's block is numbered within the request
+static void decode_stats(const orion_hip_ctx* c, const double* r, unsigned long long dtbits, orion_residual_stats* o,
+                         int req = 0) {
   o->rms_residual = r[0];
   o->max_residual = r[1];
   const long long key = (long long)r[2];
   const long long g = key / 5;
   const int m = (int)(key % 5);
-  int b = 0;
-  while (b + 1 < c->nb && c->intr.h[b + 1] <= g) ++b;
+  int b = c->req_first[req];
+  while (b + 1 < c->req_first[req + 1] && c->intr.h[b + 1] <= g) ++b;
   const BlockDev& B = c->hb[b];
   const long long n = g - c->intr.h[b];
-  o->max_loc[0] = b;
+  o->max_loc[0] = b - c->req_first[req];
   o->max_loc[1] = (int)(n % B.idim) + 1;
   o->max_loc[2] = (int)((n / B.idim) % B.jdim) + 1;
   o->max_loc[3] = (int)(n / ((long long)B.idim * B.jdim)) + 1;
@@ -971,16 +1075,17 @@ int orion_hip_apply_physical_bc(orion_hip_ctx* c) {
   return check_launch("apply_physical_bc");
 }
 int orion_hip_update_flow_states(orion_hip_ctx* c) {
-  launch_state(LA(c, c->pad), c->pad.total(), c->p, 1, nullptr, nullptr);
+  launch_state(LA(c, c->pad), c->pad.total(), 1, nullptr, nullptr, c->nreq);
   return check_launch("update_flow_states");
 }
 int orion_hip_calculate_time_step(orion_hip_ctx* c, double* phydtime) {
+  if (c->nreq != 1) return fail("calculate_time_step: a batch reports each request's time step through orion_hip_run");
   if (ensure_res(c, 1)) return -1;
   const double init = 1000.0;
   unsigned long long bits;
   std::memcpy(&bits, &init, sizeof bits);
   HIPCHK(hipMemcpyAsync(c->d_dtmin, &bits, sizeof bits, hipMemcpyHostToDevice, c->stream));
-  launch_state(LA(c, c->pad), c->pad.total(), c->p, 2, c->d_dtmin, nullptr);
+  launch_state(LA(c, c->pad), c->pad.total(), 2, c->d_dtmin, nullptr, c->nreq);
   if (check_launch("calculate_time_step")) return -1;
   if (phydtime) {
     HIPCHK(hipMemcpyAsync(&bits, c->d_dtmin, sizeof bits, hipMemcpyDeviceToHost, c->stream));
@@ -1006,9 +1111,9 @@ int orion_hip_update_solution(orion_hip_ctx* c) {
   return check_launch("update_solution");
 }
 int orion_hip_residual(orion_hip_ctx* c, orion_residual_stats* out) {
+  if (c->nreq != 1) return fail("residual: a batch reports each request's residual through orion_hip_run");
   if (ensure_res(c, 1)) return -1;
-  launch_residual(LA(c, c->intr), c->intr.total(), 5 * c->intr.total(), c->d_part, c->d_pidx, c->nparts, c->d_res,
-                  nullptr);
+  launch_residual(LA(c, c->intr), c->intr.total(), 5 * c->intr.total(), c->d_part, c->d_pidx, c->nparts, c->d_res);
   if (check_launch("residual")) return -1;
   double r[4];
   unsigned long long bits = 0;
@@ -1026,8 +1131,12 @@ static void step_body(orion_hip_ctx* c) {
   op_halo(c);
   op_bc(c);
   // mode 4: the first right-hand-side kernel below writes dq whole
-  launch_state(LA(c, c->pad), c->pad.total(), c->p, 3 | 4, c->d_dtmin, c->d_step);
-  if (c->p.nvis == 1) {
+  launch_state(LA(c, c->pad), c->pad.total(), 3 | 4, c->d_dtmin, c->d_step, c->nreq);
+  if (c->p.nvis == 1 && !c->overlap) {
+    op_visc(c, true);
+    op_avg(c);
+    op_inv(c, false);
+  } else if (c->p.nvis == 1) {
     // The inviscid face fluxes (FP64-bound) run on the side stream beside
     // the viscous chain (load-bound); they have their own flux array and
     // join before the inviscid difference, the first kernel after them
@@ -1045,8 +1154,7 @@ static void step_body(orion_hip_ctx* c) {
   }
   op_avg(c);
   op_update(c);
-  op_update_residual(c, c->d_res, c->d_step);
-  launch_step_advance(c->d_step, c->stream);
+  op_update_residual(c, c->d_res, c->d_step);  // and advances *d_step
 }
 
 static int ensure_graph(orion_hip_ctx* c) {
@@ -1059,6 +1167,7 @@ static int ensure_graph(orion_hip_ctx* c) {
     hipGraphDestroy(c->graph);
     c->graph = nullptr;
   }
+  std::lock_guard<std::recursive_mutex> guard(g_capture_mutex);
   HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
   step_body(c);
   hipGraph_t g = nullptr;
@@ -1073,13 +1182,14 @@ static int ensure_graph(orion_hip_ctx* c) {
 int orion_hip_run(orion_hip_ctx* c, int nsteps, orion_residual_stats* hist) {
   if (nsteps <= 0) return 0;
   if (ensure_res(c, nsteps)) return -1;
+  const size_t rows = (size_t)nsteps * c->nreq;
   {
     const double init = 1000.0;
     unsigned long long bits;
     std::memcpy(&bits, &init, sizeof bits);
-    std::vector<unsigned long long> v(nsteps, bits);
+    std::vector<unsigned long long> v(rows, bits);
     const int zero = 0;
-    HIPCHK(hipMemcpyAsync(c->d_dtmin, v.data(), sizeof(unsigned long long) * nsteps, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->d_dtmin, v.data(), sizeof(unsigned long long) * rows, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(c->d_step, &zero, sizeof zero, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
   }
@@ -1091,12 +1201,12 @@ int orion_hip_run(orion_hip_ctx* c, int nsteps, orion_residual_stats* hist) {
   }
   if (check_launch("run") || check_sweeps(c)) return -1;
   if (hist) {
-    std::vector<double> r(4 * (size_t)nsteps);
-    std::vector<unsigned long long> bits(nsteps);
-    HIPCHK(hipMemcpyAsync(r.data(), c->d_res, sizeof(double) * 4 * nsteps, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipMemcpyAsync(bits.data(), c->d_dtmin, sizeof(unsigned long long) * nsteps, hipMemcpyDeviceToHost, c->stream));
+    std::vector<double> r(4 * rows);
+    std::vector<unsigned long long> bits(rows);
+    HIPCHK(hipMemcpyAsync(r.data(), c->d_res, sizeof(double) * 4 * rows, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(bits.data(), c->d_dtmin, sizeof(unsigned long long) * rows, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    for (int s = 0; s < nsteps; ++s) decode_stats(c, &r[4 * s], bits[s], &hist[s]);
+    for (size_t w = 0; w < rows; ++w) decode_stats(c, &r[4 * w], bits[w], &hist[w], (int)(w % c->nreq));
   }
   return 0;
 }
@@ -1119,30 +1229,28 @@ int orion_hip_profile_run(orion_hip_ctx* c, int nsteps, double* phase_ms, long l
     hipEventRecord(ev[0], c->stream);
     op_halo(c); mark(1);
     op_bc(c); mark((long long)c->bc_first.size());
-    launch_state(LA(c, c->pad), c->pad.total(), c->p, 3 | 4, c->d_dtmin + s, nullptr); mark(1);
+    launch_state(LA(c, c->pad), c->pad.total(), 3 | 4, c->d_dtmin + (long long)s * c->nreq, nullptr, c->nreq); mark(1);
     if (c->p.nvis == 1) {
-      const double cp = 1.0 / ((c->p.gama - 1.0) * c->p.moo * c->p.moo);
-      const double re = c->p.reynolds;
       launch_visc_deriv(LA(c, c->intr), c->intr.total()); mark(1);
-      for (int d = 0; d < 3; ++d) launch_visc_face(LA(c, c->fpair[d]), d, c->fpair[d].total(), cp / c->p.prl);
-      mark(3);
-      launch_visc_diff(LA(c, c->dtile), c->dtile.total(), (std::fabs(re) > 1.0e-30) ? 1.0 / re : 0.0, true); mark(1);
+      launch_visc_face(LA3(c, c->fpair3), -1, c->fpair3.total());
+      mark(1);
+      launch_visc_diff(LA(c, c->dtile), c->dtile.total(), true); mark(1);
       op_avg(c); mark(2);
     } else {
       mark(0); mark(0); mark(0); mark(0);
     }
-    for (int d = 0; d < 3; ++d) launch_inv_face(LA(c, c->face[d]), d, c->face[d].total(), c->p.gama);
-    mark(3);
-    for (int d = 0; d < 3; ++d) launch_inv_closure(LA(c, c->ltask[d]), d, c->ltask[d].total(), c->p.gama);
-    mark(3);
+    launch_inv_face(LA3(c, c->face3), -1, c->face3.total(), c->p.gama);
+    mark(1);
+    launch_inv_closure(LA3(c, c->ltask3), -1, c->ltask3.total(), c->p.gama);
+    mark(1);
     launch_inv_diff(LA(c, c->dtile), c->dtile.total(), c->p.nvis != 1); mark(1);
     op_avg(c); mark(2);
-    launch_lu_pack(LA(c, c->ptile), c->ptile.total(), c->sw, c->p); mark(1);
-    for (int t = 0; t < 4; ++t) { op_sweep(c, t); mark(2); }
+    launch_lu_pack(LA(c, c->ptile), c->ptile.total(), c->sw); mark(1);
+    for (int t = 0; t < 4; ++t) { op_sweep(c, t); mark(1); }
     launch_update(LA(c, c->ptile), c->ptile.total(), c->sw, c->p.gama, c->intr.d, c->d_upart,
-                  c->d_upart + c->ptile.nwg, c->d_ukey);
+                  c->d_upart + c->ptile.nwg, c->d_ukey, c->d_epoch);
     mark(1);
-    op_update_residual(c, c->d_res + 4 * s, nullptr);
+    op_update_residual(c, c->d_res + 4 * (long long)s * c->nreq, nullptr);
     mark(1);
     if (check_launch("profile_run")) return -1;
     HIPCHK(hipEventSynchronize(ev[k]));
@@ -1273,6 +1381,14 @@ int orion_hip_synchronize(orion_hip_ctx* c) {
 
 int orion_hip_set_stream(orion_hip_ctx* c, void* s) {
   c->stream = s ? (hipStream_t)s : c->own_stream;
+  return 0;
+}
+
+int orion_hip_set_overlap(orion_hip_ctx* c, int on) {
+  if (!c) return fail("set_overlap: null context");
+  if ((on != 0) == c->overlap) return 0;
+  c->overlap = on != 0;
+  c->graph_stream = nullptr;  // the captured step has the other form: capture again
   return 0;
 }
 

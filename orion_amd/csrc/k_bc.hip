@@ -61,7 +61,7 @@ __device__ inline void dif_average(const BlockDev& b, const int* o, int dir, int
 }
 
 __global__ void __launch_bounds__(256) bc_kernel(const BlockDev* __restrict__ blocks, const BCTask* __restrict__ tasks,
-                                                 int ntasks, long long total, Params p) {
+                                                 int ntasks, long long total) {
   const long long g = xcd_thread();
   if (g >= total) return;
   int t = 0;
@@ -76,6 +76,7 @@ __global__ void __launch_bounds__(256) bc_kernel(const BlockDev* __restrict__ bl
   }
   const BCTask T = tasks[t];
   const BlockDev& b = blocks[T.block];
+  const Params& p = b.par;  // the inputs of the block's request
   const long long e = g - T.start;
   const int ni = T.hi[0] - T.lo[0] + 1, nj = T.hi[1] - T.lo[1] + 1;
   const int i = T.lo[0] + (int)(e % ni);
@@ -327,9 +328,9 @@ void launch_halo(const long long* pairs, long long n, double* arena, hipStream_t
   if (n <= 0) return;
   hipLaunchKernelGGL(halo_kernel, dim3(grid_of(n, 256)), dim3(256), 0, s, pairs, n, arena);
 }
-void launch_bc(const BlockDev* blocks, const BCTask* tasks, int ntasks, long long n, const Params& p, hipStream_t s) {
+void launch_bc(const BlockDev* blocks, const BCTask* tasks, int ntasks, long long n, hipStream_t s) {
   if (n <= 0 || ntasks <= 0) return;
-  hipLaunchKernelGGL(bc_kernel, dim3(grid_of(n, 256)), dim3(256), 0, s, blocks, tasks, ntasks, n, p);
+  hipLaunchKernelGGL(bc_kernel, dim3(grid_of(n, 256)), dim3(256), 0, s, blocks, tasks, ntasks, n);
 }
 void launch_avg_stage(const BlockDev* blocks, const AvgSrc* src, long long n, double* stage, hipStream_t s) {
   if (n <= 0) return;

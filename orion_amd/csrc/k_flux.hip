@@ -260,6 +260,11 @@ __global__ void __launch_bounds__(256) inv_face_kernel(const BlockDev* __restric
   int bi;
   long long gi;
   if (!span_item(SP, &bi, &gi)) return;
+  if (d < 0) {  // the three directions in one launch: virtual block d * nb + b
+    const int nbr = SP.nb / 3;
+    d = bi / nbr;
+    bi -= d * nbr;
+  }
   const BlockDev& b = blocks[bi];
   long long base;
   int f;
@@ -298,6 +303,11 @@ __global__ void __launch_bounds__(256) inv_closure_kernel(const BlockDev* __rest
   int bi;
   long long gi;
   if (!span_item(SP, &bi, &gi)) return;
+  if (d < 0) {  // the three directions in one launch: virtual block d * nb + b
+    const int nbr = SP.nb / 3;
+    d = bi / nbr;
+    bi -= d * nbr;
+  }
   const BlockDev& b = blocks[bi];
   // task slowest: a wavefront's lanes are lines of one task (no divergence)
   unsigned ln, task;
@@ -544,12 +554,17 @@ __device__ inline double half_deriv_c(const double* q, long long s, int n, int f
 // stencil windows (positions f0+1..f0+5), and the face geometry is the
 // static one interpolated after the metrics.
 __global__ void __launch_bounds__(256) visc_face_kernel(const BlockDev* __restrict__ blocks,
-                                                        Span SP, int d,
-                                                        double cp_prl) {
+                                                        Span SP, int d) {
   int bi;
   long long gi;
   if (!span_item(SP, &bi, &gi)) return;
+  if (d < 0) {  // the three directions in one launch: virtual block d * nb + b
+    const int nbr = SP.nb / 3;
+    d = bi / nbr;
+    bi -= d * nbr;
+  }
   const BlockDev& b = blocks[bi];
+  const double cp_prl = b.cp_prl;
   long long base;
   int f0;
   decode_face_pair(b, d, gi, &base, &f0);
@@ -657,7 +672,7 @@ __device__ __forceinline__ double diff_node_f(const FF& F, int n, int t) {
 // slots t..t+5.  FIRST: the step's first right-hand-side kernel, which starts
 // from dq = 0 (what update_flow_states leaves) instead of reading it.
 template <bool INV, bool FIRST>
-__global__ void __launch_bounds__(256) diff_kernel(const BlockDev* __restrict__ blocks, Span SP, double re_inv) {
+__global__ void __launch_bounds__(256) diff_kernel(const BlockDev* __restrict__ blocks, Span SP) {
   constexpr int DT = 16, DW = DT + 5;
   __shared__ double X[5][DT][DW];  // i slots i0-3 .. i0+17 of each row of the tile
   __shared__ double Y[5][DW][DT];  // j slots j0-3 .. j0+17 of each column
@@ -665,6 +680,7 @@ __global__ void __launch_bounds__(256) diff_kernel(const BlockDev* __restrict__ 
   long long gi;
   if (!span_item(SP, &bi, &gi)) return;  // whole tiles: every lane of a workgroup has an item
   const BlockDev& b = blocks[bi];
+  const double re_inv = b.re_inv;
   const long long P = b.P;
   const unsigned tci = (unsigned)(b.idim + DT - 1) / DT, tcj = (unsigned)(b.jdim + DT - 1) / DT;
   unsigned q, ti, q2, tj;
@@ -751,11 +767,9 @@ void launch_inv_closure(const Launch& L, int d, long long n, double gamma) {
 void launch_inv_diff(const Launch& L, long long n, bool first) {
   if (n <= 0) return;
   if (first)
-    hipLaunchKernelGGL((diff_kernel<true, true>), dim3((unsigned)L.nwg), dim3(SPAN_TB), 0, L.stream, L.blocks, L.span,
-                       0.0);
+    hipLaunchKernelGGL((diff_kernel<true, true>), dim3((unsigned)L.nwg), dim3(SPAN_TB), 0, L.stream, L.blocks, L.span);
   else
-    hipLaunchKernelGGL((diff_kernel<true, false>), dim3((unsigned)L.nwg), dim3(SPAN_TB), 0, L.stream, L.blocks, L.span,
-                       0.0);
+    hipLaunchKernelGGL((diff_kernel<true, false>), dim3((unsigned)L.nwg), dim3(SPAN_TB), 0, L.stream, L.blocks, L.span);
 }
 void launch_visc_deriv(const Launch& L, long long n) {
   if (n <= 0) return;
@@ -765,18 +779,18 @@ void launch_face_geom(const Launch& L, int d, long long n) {
   if (n <= 0) return;
   hipLaunchKernelGGL(face_geom_kernel, dim3((unsigned)L.nwg), dim3(SPAN_TB), 0, L.stream, L.blocks, L.span, d);
 }
-void launch_visc_face(const Launch& L, int d, long long n, double cp_prl) {
+void launch_visc_face(const Launch& L, int d, long long n) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(visc_face_kernel, dim3((unsigned)L.nwg), dim3(SPAN_TB), 0, L.stream, L.blocks, L.span, d, cp_prl);
+  hipLaunchKernelGGL(visc_face_kernel, dim3((unsigned)L.nwg), dim3(SPAN_TB), 0, L.stream, L.blocks, L.span, d);
 }
-void launch_visc_diff(const Launch& L, long long n, double re_inv, bool first) {
+void launch_visc_diff(const Launch& L, long long n, bool first) {
   if (n <= 0) return;
   if (first)
     hipLaunchKernelGGL((diff_kernel<false, true>), dim3((unsigned)L.nwg), dim3(SPAN_TB), 0, L.stream, L.blocks,
-                       L.span, re_inv);
+                       L.span);
   else
     hipLaunchKernelGGL((diff_kernel<false, false>), dim3((unsigned)L.nwg), dim3(SPAN_TB), 0, L.stream, L.blocks,
-                       L.span, re_inv);
+                       L.span);
 }
 
 }  // namespace orion_hip

@@ -46,7 +46,7 @@ namespace orion_hip {
 
 constexpr int TW = 16;         // tile width along col and row
 constexpr int TL = TW * TW;    // lanes per tile workgroup
-constexpr int KEY_BITS = 16;   // progress flag: epoch << KEY_BITS | plane key
+constexpr int KEY_BITS = 16;   // progress flag: sweep serial << KEY_BITS | plane key
 constexpr int CPOL_SC1 = 16;   // cache-policy operand of a memory builtin: the sc1 bit (gfx950)
 
 __device__ inline double spec_rad(const double* pr, double g, double kx, double ky, double kz) {
@@ -157,13 +157,13 @@ constexpr int NPK = 15;
 // get_diag_coeff, the rhs) and, for its neighbours, u, v, w, c^2, H, a_f,
 // c and the viscous coefficient.
 __global__ void __launch_bounds__(256) lu_pack_kernel(const BlockDev* __restrict__ blocks,
-                                                      Span SP, SweepArrays S,
-                                                      Params p) {
+                                                      Span SP, SweepArrays S) {
   __shared__ double sh[NPK][PT * PT];
   int bi;
   long long gi;
   if (!span_item(SP, &bi, &gi)) return;  // whole tiles: every lane of a workgroup has an item
   const BlockDev& b = blocks[bi];
+  const Params& p = b.par;  // the inputs of the block's request
   int m0, c0, r;
   pack_tile(b, gi, &m0, &c0, &r);
   const int lane = threadIdx.x;
@@ -243,18 +243,25 @@ __global__ void __launch_bounds__(256) lu_pack_kernel(const BlockDev* __restrict
 __global__ void __launch_bounds__(256) update_kernel(const BlockDev* __restrict__ blocks,
                                                      Span SP, SweepArrays S,
                                                      double gama, const long long* __restrict__ ioff,
-                                                     double* psum, double* pmax, long long* pkey) {
+                                                     double* psum, double* pmax, long long* pkey,
+                                                     unsigned long long* epoch) {
   __shared__ double sh[5][PT * PT];
   __shared__ double ssum[PT * PT];
   __shared__ double smax[PT * PT];
   __shared__ long long skey[PT * PT];
+  // the step's four sweeps are done (they precede this kernel on the
+  // stream): the next step's take the next four serials
+  if (blockIdx.x == 0 && threadIdx.x == 0) *epoch += 1;
   int bi;
   long long gi;
+  // partials in tile order (the span's workgroup index): a block's, and a
+  // request's, are contiguous and ordered alike in any context
+  const unsigned wq = xcd_remap(blockIdx.x, gridDim.x);
   if (!span_item(SP, &bi, &gi)) {  // whole tiles: the workgroup has no tile
     if (threadIdx.x == 0) {
-      psum[blockIdx.x] = 0.0;
-      pmax[blockIdx.x] = -1.0;
-      pkey[blockIdx.x] = 0x7fffffffffffffffLL;
+      psum[wq] = 0.0;
+      pmax[wq] = -1.0;
+      pkey[wq] = 0x7fffffffffffffffLL;
     }
     return;
   }
@@ -352,9 +359,9 @@ __global__ void __launch_bounds__(256) update_kernel(const BlockDev* __restrict_
     __syncthreads();
   }
   if (lane == 0) {
-    psum[blockIdx.x] = ssum[0];
-    pmax[blockIdx.x] = smax[0];
-    pkey[blockIdx.x] = skey[0];
+    psum[wq] = ssum[0];
+    pmax[wq] = smax[0];
+    pkey[wq] = skey[0];
   }
 }
 
@@ -538,7 +545,10 @@ __global__ void __launch_bounds__(2 * TL, 1) lusgs_tile_kernel(SweepArrays S, co
   const int c = T.c0 + lc, r = T.r0 + lr;
   const bool lane_ok = c < nc && r < nr;
   const int up_c = FWD ? T.nbr[0] : T.nbr[2], up_r = FWD ? T.nbr[1] : T.nbr[3];
-  const unsigned long long ep = (*epoch) << KEY_BITS;
+  // the sweep's serial: four sweeps per step epoch (the update kernel after
+  // them advances the epoch), so serials rise from launch to launch and a
+  // flag left by an earlier sweep is always short of this one's keys
+  const unsigned long long ep = ((*epoch) * 4 + TYPE + 1) << KEY_BITS;
   SweepTile ut[2];
   ut[0] = up_c >= 0 ? tiles[up_c] : T;
   ut[1] = up_r >= 0 ? tiles[up_r] : T;
@@ -919,32 +929,27 @@ __global__ void __launch_bounds__(2 * TL, 1) lusgs_tile_kernel(SweepArrays S, co
   }
 }
 
-__global__ void epoch_kernel(unsigned long long* epoch) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) *epoch += 1;
-}
-
 void launch_lu_geom(const Launch& La, long long n, const SweepArrays& S) {
   if (n <= 0) return;
   hipLaunchKernelGGL(lu_geom_kernel, dim3((unsigned)La.nwg), dim3(SPAN_TB), 0, La.stream, La.blocks, La.span, S);
 }
 
-void launch_lu_pack(const Launch& La, long long n, const SweepArrays& S, const Params& p) {
+void launch_lu_pack(const Launch& La, long long n, const SweepArrays& S) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(lu_pack_kernel, dim3((unsigned)La.nwg), dim3(SPAN_TB), 0, La.stream, La.blocks, La.span, S, p);
+  hipLaunchKernelGGL(lu_pack_kernel, dim3((unsigned)La.nwg), dim3(SPAN_TB), 0, La.stream, La.blocks, La.span, S);
 }
 
 void launch_update(const Launch& La, long long n, const SweepArrays& S, double gamma, const long long* ioff,
-                   double* psum, double* pmax, long long* pkey) {
+                   double* psum, double* pmax, long long* pkey, unsigned long long* epoch) {
   if (n <= 0) return;
   hipLaunchKernelGGL(update_kernel, dim3((unsigned)La.nwg), dim3(SPAN_TB), 0, La.stream, La.blocks, La.span, S,
-                     gamma, ioff, psum, pmax, pkey);
+                     gamma, ioff, psum, pmax, pkey, epoch);
 }
 
 void launch_lusgs_tiles(const SweepArrays& S, const BlockDev* blocks, const SweepTile* tiles, const int* order,
                         int ntiles, unsigned long long* flags, unsigned long long* epoch, int* err, int type,
                         const Params& p, hipStream_t s) {
   if (ntiles <= 0) return;
-  hipLaunchKernelGGL(epoch_kernel, dim3(1), dim3(64), 0, s, epoch);
   const double gm1 = p.gama - 1.0;
   const dim3 gr(ntiles);
   const bool v = p.nvis == 1;

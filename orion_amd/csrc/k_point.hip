@@ -16,14 +16,15 @@
 namespace orion_hip {
 
 __global__ void __launch_bounds__(256) state_kernel(const BlockDev* __restrict__ blocks,
-                                                    Span SP, Params p, int mode,
-                                                    unsigned long long* dtmin_base, const int* step) {
+                                                    Span SP, int mode, unsigned long long* dtmin_base,
+                                                    const int* step, int nreq) {
   __shared__ double smin[256];
   double mydt = 1000.0;
   int bi;
   long long n;
   if (span_item(SP, &bi, &n)) {
     const BlockDev& b = blocks[bi];
+    const Params& p = b.par;  // the inputs of the block's request
     unsigned q, ui, q2, uj;
     udivmod((unsigned)n, (unsigned)b.nx, &q, &ui);
     udivmod(q, (unsigned)b.ny, &q2, &uj);
@@ -96,7 +97,9 @@ __global__ void __launch_bounds__(256) state_kernel(const BlockDev* __restrict__
     }
   }
   if (!dtmin_base) return;
-  unsigned long long* dtmin_bits = dtmin_base + (step ? *step : 0);
+  // slot (step, request): a workgroup holds items of one block
+  const int req = blocks[SP.wgb[xcd_remap(blockIdx.x, gridDim.x)]].req;
+  unsigned long long* dtmin_bits = dtmin_base + (long long)(step ? *step : 0) * nreq + req;
   smin[threadIdx.x] = mydt;
   __syncthreads();
   for (int s = 128; s > 0; s >>= 1) {
@@ -185,19 +188,26 @@ __global__ void __launch_bounds__(RES_TB) residual_kernel(const BlockDev* __rest
   }
 }
 
+// Workgroup r reduces request r's partials, seg[r] .. seg[r + 1] (all
+// nparts when seg is null), over tot5[r] (total5) terms into row
+// (*step) * nreq + r of out.  With step: the last workgroup to finish (done
+// counts them, and is left at 0) advances *step, after every workgroup has
+// read it.
 __global__ void __launch_bounds__(RES_MAXPARTS) residual_final_kernel(const double* partial, const double* pmax,
-                                                                      const long long* pidx_, int nparts,
+                                                                      const long long* pidx_, const int* seg,
+                                                                      int nparts, const long long* tot5,
                                                                       long long total5, double* out_base,
-                                                                      const int* step) {
+                                                                      int* step, int* done, int nreq) {
   __shared__ double ssum[RES_MAXPARTS];
   __shared__ double smax[RES_MAXPARTS];
   __shared__ long long skey[RES_MAXPARTS];
-  const int t = threadIdx.x;
-  // partials t, t + 1024, ... in order, then a tree over the threads
+  const int t = threadIdx.x, r = blockIdx.x;
+  const int q0 = seg ? seg[r] : 0, q1 = seg ? seg[r + 1] : nparts;
+  // partials q0 + t, q0 + t + 1024, ... in order, then a tree over the threads
   double sum = 0.0;
   MaxLoc best{-1.0, 0x7fffffffffffffffLL};
-  for (int q = t; q < nparts; q += RES_MAXPARTS) {
-    sum = q == t ? partial[q] : sum + partial[q];
+  for (int q = q0 + t; q < q1; q += RES_MAXPARTS) {
+    sum = q == q0 + t ? partial[q] : sum + partial[q];
     const MaxLoc c{pmax[q], pidx_[q]};
     if (better(c, best)) best = c;
   }
@@ -217,15 +227,18 @@ __global__ void __launch_bounds__(RES_MAXPARTS) residual_final_kernel(const doub
     __syncthreads();
   }
   if (t == 0) {
-    double* out = out_base + 4 * (step ? *step : 0);
-    out[0] = total5 > 0 ? sqrt(ssum[0] / (double)total5) : 0.0;
+    const int sidx = step ? *step : 0;
+    const long long t5 = tot5 ? tot5[r] : total5;
+    double* out = out_base + 4 * ((long long)sidx * nreq + r);
+    out[0] = t5 > 0 ? sqrt(ssum[0] / (double)t5) : 0.0;
     out[1] = smax[0];
     out[2] = (double)skey[0];
+    if (step && atomicAdd(done, 1) == nreq - 1) {
+      // every workgroup has read *step (its value addressed the stores above)
+      *done = 0;
+      *step = sidx + 1;
+    }
   }
-}
-
-__global__ void step_advance_kernel(int* step) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) *step += 1;
 }
 
 __global__ void fill_kernel(double* p, long long n, double v) {
@@ -235,12 +248,12 @@ __global__ void fill_kernel(double* p, long long n, double v) {
 
 // update_temperature, src/preprocess/InitialCondition.cpp:91-147
 __global__ void __launch_bounds__(256) temperature_kernel(const BlockDev* __restrict__ blocks,
-                                                          Span SP, double gama,
-                                                          double moo) {
+                                                          Span SP) {
   int bi;
   long long gi;
   if (!span_item(SP, &bi, &gi)) return;
   const BlockDev& b = blocks[bi];
+  const double gama = b.par.gama, moo = b.par.moo;
   const long long id = gi;
   double rho = b.prim[id];
   const double p = b.prim[4 * b.P + id];
@@ -280,26 +293,23 @@ __global__ void __launch_bounds__(256) output_kernel(const BlockDev* __restrict_
 
 static inline unsigned grid_of(long long n, int tb) { return (unsigned)((n + tb - 1) / tb); }
 
-void launch_state(const Launch& L, long long n, const Params& p, int mode, unsigned long long* dtmin_base,
-                  const int* step) {
+void launch_state(const Launch& L, long long n, int mode, unsigned long long* dtmin_base, const int* step,
+                  int nreq) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(state_kernel, dim3((unsigned)L.nwg), dim3(SPAN_TB), 0, L.stream, L.blocks, L.span, p, mode,
-                     dtmin_base, step);
-}
-void launch_step_advance(int* step, hipStream_t s) {
-  hipLaunchKernelGGL(step_advance_kernel, dim3(1), dim3(64), 0, s, step);
+  hipLaunchKernelGGL(state_kernel, dim3((unsigned)L.nwg), dim3(SPAN_TB), 0, L.stream, L.blocks, L.span, mode,
+                     dtmin_base, step, nreq);
 }
 void launch_residual(const Launch& L, long long n, long long total5, double* partial, long long* pidx_, int nparts,
-                     double* out, const int* step) {
+                     double* out) {
   hipLaunchKernelGGL(residual_kernel, dim3(nparts), dim3(RES_TB), 0, L.stream, L.blocks, L.off, L.nb, partial,
                      partial + RES_MAXPARTS, pidx_);
   hipLaunchKernelGGL(residual_final_kernel, dim3(1), dim3(RES_MAXPARTS), 0, L.stream, partial, partial + RES_MAXPARTS,
-                     pidx_, nparts, total5, out, step);
+                     pidx_, nullptr, nparts, nullptr, total5, out, nullptr, nullptr, 1);
 }
-void launch_residual_final(const double* psum, const double* pmax, const long long* pkey, int nparts, long long total5,
-                           double* out, const int* step, hipStream_t s) {
-  hipLaunchKernelGGL(residual_final_kernel, dim3(1), dim3(RES_MAXPARTS), 0, s, psum, pmax, pkey, nparts, total5, out,
-                     step);
+void launch_residual_final(const double* psum, const double* pmax, const long long* pkey, const int* seg,
+                           const long long* tot5, int nreq, double* out, int* step, int* done, hipStream_t s) {
+  hipLaunchKernelGGL(residual_final_kernel, dim3(nreq), dim3(RES_MAXPARTS), 0, s, psum, pmax, pkey, seg, 0, tot5, 0LL,
+                     out, step, done, nreq);
 }
 void launch_output(const BlockDev* blocks, int bi, long long n, float* out, double gamma, hipStream_t s) {
   if (n <= 0) return;
@@ -309,10 +319,9 @@ void launch_fill(double* p, long long n, double v, hipStream_t s) {
   if (n <= 0) return;
   hipLaunchKernelGGL(fill_kernel, dim3(grid_of(n, 256)), dim3(256), 0, s, p, n, v);
 }
-void launch_temperature(const Launch& L, long long n, double gama, double moo) {
+void launch_temperature(const Launch& L, long long n) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(temperature_kernel, dim3((unsigned)L.nwg), dim3(SPAN_TB), 0, L.stream, L.blocks, L.span, gama,
-                     moo);
+  hipLaunchKernelGGL(temperature_kernel, dim3((unsigned)L.nwg), dim3(SPAN_TB), 0, L.stream, L.blocks, L.span);
 }
 
 }  // namespace orion_hip

@@ -39,6 +39,12 @@ struct GPtr {
   __host__ __device__ operator T*() const { return (T*)p; }
 };
 
+struct Params {
+  double gama, prl, moo, reynolds, visc, tref, twall, cfl;
+  double roo, uoo, voo, woo, poo, too;
+  int csrv, nvis, method, ntmst, gcl;
+};
+
 // Fields of one block (device pointers into the arena).
 struct BlockDev {
   int idim, jdim, kdim;  // interior sizes
@@ -67,6 +73,15 @@ struct BlockDev {
   int sn[3];
   int sncp;
   long long sbase;
+  // The block's request: its index in a context that batches several
+  // independent solves (0 when the context holds one), its inputs, and the
+  // viscous coefficients the host derives from them.  Kernels take every
+  // per-request value from here, so blocks of different requests share a
+  // launch.
+  int req;
+  Params par;
+  double cp_prl;  // cp / Pr, cp = 1 / ((gama - 1) moo^2)
+  double re_inv;  // 1 / Re, 0 for |Re| <= 1e-30
 };
 
 // Face geometry, interpolated once after the metrics (the grid is static)
@@ -118,12 +133,6 @@ struct SweepTile {
 __host__ __device__ inline long long sweep_index(const BlockDev& b, int m, int r, int c) {
   return b.sbase + ((long long)(m + r + c) * b.sn[2] + r) * b.sncp + c;
 }
-
-struct Params {
-  double gama, prl, moo, reynolds, visc, tref, twall, cfl;
-  double roo, uoo, voo, woo, poo, too;
-  int csrv, nvis, method, ntmst, gcl;
-};
 
 __host__ __device__ inline long long pidx(const BlockDev& b, int i, int j, int k) {
   return (long long)i + (long long)b.nx * ((long long)j + (long long)b.ny * (long long)k);

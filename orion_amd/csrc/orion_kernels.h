@@ -37,6 +37,8 @@ struct AvgDst {
 };
 
 // k_flux.hip
+// face kernels: d = 0, 1, 2 for one direction's space; d = -1 for a space of
+// 3 nb virtual blocks d * nb + b holding the three directions' spaces
 void launch_inv_face(const Launch& L, int d, long long n, double gamma);
 void launch_inv_closure(const Launch& L, int d, long long n, double gamma);
 // first: the step's first right-hand-side kernel, starting from dq = 0
@@ -44,41 +46,43 @@ void launch_inv_closure(const Launch& L, int d, long long n, double gamma);
 void launch_inv_diff(const Launch& L, long long n, bool first);
 void launch_visc_deriv(const Launch& L, long long n);
 void launch_face_geom(const Launch& L, int d, long long n);
-void launch_visc_face(const Launch& L, int d, long long n, double cp_prl);
-void launch_visc_diff(const Launch& L, long long n, double re_inv, bool first);
+void launch_visc_face(const Launch& L, int d, long long n);
+void launch_visc_diff(const Launch& L, long long n, bool first);
 
 // k_point.hip
-void launch_state(const Launch& L, long long n, const Params& p, int mode, unsigned long long* dtmin_base,
-                  const int* step);
-void launch_step_advance(int* step, hipStream_t s);
+// dtmin slot of (step, request): dtmin_base[*step * nreq + request]
+void launch_state(const Launch& L, long long n, int mode, unsigned long long* dtmin_base, const int* step,
+                  int nreq);
+// into row 0 of out
 void launch_residual(const Launch& L, long long n, long long total5, double* partial, long long* pidx,
-                     int nparts, double* out, const int* step);
+                     int nparts, double* out);
 int residual_parts(long long n);
 // the VTS point fields of block bi (n interior nodes) as float, 7 n values
 void launch_output(const BlockDev* blocks, int bi, long long n, float* out, double gamma, hipStream_t s);
-// the final reduction alone, over partials left by another kernel (any count)
-void launch_residual_final(const double* psum, const double* pmax, const long long* pkey, int nparts, long long total5,
-                           double* out, const int* step, hipStream_t s);
+// the final reduction alone, over partials left by another kernel (any
+// count): one workgroup per request r over partials seg[r] .. seg[r + 1],
+// row (*step) * nreq + r of out; then *step advances (done: a zeroed counter)
+void launch_residual_final(const double* psum, const double* pmax, const long long* pkey, const int* seg,
+                           const long long* tot5, int nreq, double* out, int* step, int* done, hipStream_t s);
 void launch_fill(double* p, long long n, double v, hipStream_t s);
-void launch_temperature(const Launch& L, long long n, double gama, double moo);
+void launch_temperature(const Launch& L, long long n);
 
 // k_bc.hip
 void launch_halo(const long long* pairs, long long n, double* arena, hipStream_t s);
-void launch_bc(const BlockDev* blocks, const BCTask* tasks, int ntasks, long long n, const Params& p,
-               hipStream_t s);
+void launch_bc(const BlockDev* blocks, const BCTask* tasks, int ntasks, long long n, hipStream_t s);
 void launch_avg_stage(const BlockDev* blocks, const AvgSrc* src, long long n, double* stage, hipStream_t s);
 void launch_avg_apply(const BlockDev* blocks, const AvgDst* dst, long long n, const double* stage, hipStream_t s);
 
 // k_lusgs.hip
 void launch_lu_geom(const Launch& La, long long n, const SweepArrays& S);
-void launch_lu_pack(const Launch& La, long long n, const SweepArrays& S, const Params& p);
+void launch_lu_pack(const Launch& La, long long n, const SweepArrays& S);
 void launch_lusgs_tiles(const SweepArrays& S, const BlockDev* blocks, const SweepTile* tiles, const int* order,
                         int ntiles, unsigned long long* flags, unsigned long long* epoch, int* err, int type,
                         const Params& p, hipStream_t s);
 // ioff: prefix of the interior node counts; psum/pmax/pkey: one residual
 // partial per workgroup of the launch
 void launch_update(const Launch& La, long long n, const SweepArrays& S, double gamma, const long long* ioff,
-                   double* psum, double* pmax, long long* pkey);
+                   double* psum, double* pmax, long long* pkey, unsigned long long* epoch);
 
 // k_metrics.hip
 void launch_metrics(const Launch& L, long long n, int gcl, double* tmp, hipStream_t s);

@@ -61,6 +61,34 @@ def make_params(case) -> SimpleNamespace:
     )
 
 
+def region_descs(cases, offsets=None):
+    """The C-ABI region array of one or several cases' regions, in order,
+    with block numbers (owner, interface target) shifted by each case's
+    block offset."""
+    cases = list(cases)
+    offsets = offsets if offsets is not None else [0] * len(cases)
+    total = sum(len(c.regions) for c in cases)
+    regs = (RegionDesc * max(1, total))()
+    n = 0
+    for case, off in zip(cases, offsets):
+        for r in case.regions:
+            ints = r.as_ints()
+            regs[n].block, regs[n].bctype = ints[0] + off, ints[1]
+            for m in range(3):
+                regs[n].s_st[m] = ints[2 + m]
+                regs[n].s_ed[m] = ints[5 + m]
+                regs[n].t_st[m] = ints[8 + m]
+                regs[n].t_ed[m] = ints[11 + m]
+            regs[n].nbt = ints[14] + off if ints[14] > 0 else ints[14]
+            regs[n].ibcwin = ints[15]
+            n += 1
+    return regs, total
+
+
+def case_inputs(case) -> Inputs:
+    return Inputs(**{k: case.inputs[k] for k, _ in Inputs._fields_})
+
+
 class FlowFieldSet:
     """Device mirror of preprocess::FlowFieldSet (src/preprocess/FlowField.hpp:73-87)
     for one case: fields, metrics and the compiled boundary topology."""
@@ -76,17 +104,8 @@ class FlowFieldSet:
         self.bc = case.regions
         dims = np.ascontiguousarray(case.dims, dtype=np.int32).reshape(-1)
         xyz = np.ascontiguousarray(case.xyz_flat(), dtype=np.float64)
-        regs = (RegionDesc * max(1, len(case.regions)))()
-        for n, r in enumerate(case.regions):
-            ints = r.as_ints()
-            regs[n].block, regs[n].bctype = ints[0], ints[1]
-            for m in range(3):
-                regs[n].s_st[m] = ints[2 + m]
-                regs[n].s_ed[m] = ints[5 + m]
-                regs[n].t_st[m] = ints[8 + m]
-                regs[n].t_ed[m] = ints[11 + m]
-            regs[n].nbt, regs[n].ibcwin = ints[14], ints[15]
-        inp = Inputs(**{k: case.inputs[k] for k, _ in Inputs._fields_})
+        regs, _ = region_descs([case])
+        inp = case_inputs(case)
         h = C.c_void_p()
         check(L.orion_hip_ctx_create(case.nblocks, dims.ctypes.data_as(C.POINTER(C.c_int)),
                                      xyz.ctypes.data_as(C.POINTER(C.c_double)), len(case.regions), regs,
@@ -145,6 +164,12 @@ class FlowFieldSet:
 
     def set_stream(self, stream_ptr: Optional[int]) -> None:
         check(_lib.load().orion_hip_set_stream(self.ctx, C.c_void_p(stream_ptr or 0)), "set_stream")
+
+    def set_overlap(self, on: bool) -> None:
+        """With viscosity, run the inviscid face fluxes on a second stream
+        beside the viscous chain (the default) or the whole step on one
+        stream (many contexts busy at once); the results are the same."""
+        check(_lib.load().orion_hip_set_overlap(self.ctx, int(bool(on))), "set_overlap")
 
     # ---------------------------------------------------------------- output
     def output_fields(self, b: int) -> np.ndarray:
