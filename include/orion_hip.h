@@ -113,6 +113,12 @@ int orion_hip_field(orion_hip_ctx* ctx, int block, int field, double** dev_ptr, 
  * fromDevice, src/core/OrionArray.hpp:146-155); synchronous. */
 int orion_hip_upload(orion_hip_ctx* ctx, int block, int field, const double* host, long long count);
 int orion_hip_download(orion_hip_ctx* ctx, int block, int field, double* host, long long count);
+/* No counterpart in the reference: one block's solution state (prim, which
+ * alone fixes the next step, see orion_hip_save_state) copied from a block
+ * of another context on the same device, without a host round trip; for
+ * rebuilding a serving batch around requests that stay.  The blocks must
+ * have the same sizes.  Synchronous. */
+int orion_hip_copy_block_state(orion_hip_ctx* dst, int dst_block, orion_hip_ctx* src, int src_block);
 
 /* preprocess::update_temperature (src/preprocess/InitialCondition.cpp:91-147). */
 int orion_hip_update_temperature(orion_hip_ctx* ctx);

@@ -121,6 +121,8 @@ struct orion_hip_ctx {
   int dtmin_cap = 0;
   unsigned long long* d_count = nullptr;
   std::vector<void*> allocs;
+  char* pool = nullptr;  // current chunk for small device tables
+  size_t pool_used = 0, pool_size = 0;
   int keep_q = 0;
   int* d_step = nullptr;             // step index for per-step outputs
   hipGraph_t graph = nullptr;        // one captured pseudo-time step
@@ -130,8 +132,29 @@ struct orion_hip_ctx {
   bool overlap = true;  // inviscid face fluxes on side_stream (viscous steps)
 };
 
+// Device allocations of a context, freed with it.  Tables up to 64 KB share
+// 1 MB chunks (256-byte aligned), so a context makes a few dozen fewer
+// hipMalloc and hipFree calls: hipFree waits for the device, which a
+// serving batch that is rebuilt while the device works would feel.
 static int dalloc(orion_hip_ctx* c, void** p, size_t bytes) {
   if (bytes == 0) bytes = 8;
+  constexpr size_t SMALL = 64 << 10, CHUNK = 1 << 20, ALIGN = 256;
+  if (bytes <= SMALL) {
+    const size_t need = (bytes + ALIGN - 1) / ALIGN * ALIGN;
+    if (!c->pool || c->pool_used + need > c->pool_size) {
+      void* q = nullptr;
+      hipError_t e = hipMalloc(&q, CHUNK);
+      if (e != hipSuccess) return fail("hipMalloc(%zu) failed: %s", CHUNK, hipGetErrorString(e));
+      c->allocs.push_back(q);
+      c->map_bytes += CHUNK;
+      c->pool = (char*)q;
+      c->pool_used = 0;
+      c->pool_size = CHUNK;
+    }
+    *p = c->pool + c->pool_used;
+    c->pool_used += need;
+    return 0;
+  }
   hipError_t e = hipMalloc(p, bytes);
   if (e != hipSuccess) return fail("hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
   c->allocs.push_back(*p);
@@ -602,6 +625,30 @@ int orion_hip_backend_init(int device) {
   return 0;
 }
 
+// Streams of destroyed contexts, per device, handed to the next contexts
+// (creating a stream takes milliseconds, which a serving batch rebuilt at
+// every chunk boundary would pay twice); used under g_capture_mutex.
+static std::map<int, std::vector<hipStream_t>> g_stream_pool;
+
+static int take_stream(int device, hipStream_t* s) {
+  std::vector<hipStream_t>& v = g_stream_pool[device];
+  if (!v.empty()) {
+    *s = v.back();
+    v.pop_back();
+    return 0;
+  }
+  return hipStreamCreateWithFlags(s, hipStreamNonBlocking) == hipSuccess ? 0 : -1;
+}
+
+static void give_stream(int device, hipStream_t s) {
+  if (!s) return;
+  if (hipStreamSynchronize(s) != hipSuccess) {
+    hipStreamDestroy(s);
+    return;
+  }
+  g_stream_pool[device].push_back(s);
+}
+
 void orion_hip_ctx_destroy(orion_hip_ctx* c) {
   if (!c) return;
   std::lock_guard<std::recursive_mutex> guard(g_capture_mutex);
@@ -613,9 +660,36 @@ void orion_hip_ctx_destroy(orion_hip_ctx* c) {
   if (c->graph) hipGraphDestroy(c->graph);
   if (c->ev_fork) hipEventDestroy(c->ev_fork);
   if (c->ev_join) hipEventDestroy(c->ev_join);
-  if (c->side_stream) hipStreamDestroy(c->side_stream);
-  if (c->own_stream) hipStreamDestroy(c->own_stream);
+  give_stream(c->device, c->side_stream);
+  give_stream(c->device, c->own_stream);
   delete c;
+}
+
+// The requests of a context, checked on the host before any device work:
+// block counts that add up, the inputs that select kernels or are
+// constants of the discretisation shared, and no interface between two
+// requests' blocks.
+static int check_requests(int nreq, const int* req_nblocks, int nblocks, int nregions,
+                          const orion_region_desc* regions, const orion_inputs* in) {
+  if (nreq <= 0 || !req_nblocks || !in) return fail("a context needs at least one request");
+  std::vector<int> owner;
+  for (int r = 0; r < nreq; ++r) {
+    if (req_nblocks[r] <= 0) return fail("request %d has no blocks", r);
+    owner.insert(owner.end(), req_nblocks[r], r);
+    const orion_inputs& a = in[r];
+    const orion_inputs& z = in[0];
+    if (a.nvis != z.nvis || a.method != z.method || a.csrv != z.csrv || a.ntmst != z.ntmst || a.gcl != z.gcl)
+      return fail("request %d: a batch's requests must share nvis, method, csrv, ntmst and gcl", r);
+  }
+  if ((int)owner.size() != nblocks) return fail("the requests' block counts do not add up to nblocks");
+  for (int q = 0; q < nregions; ++q) {
+    const orion_region_desc& d = regions[q];
+    if (d.bctype >= 0 || d.block < 0 || d.block >= nblocks || d.nbt < 1 || d.nbt > nblocks) continue;
+    if (owner[d.nbt - 1] != owner[d.block])
+      return fail("block %d region %d: an interface joins blocks of requests %d and %d", d.block + 1, q + 1,
+                  owner[d.block], owner[d.nbt - 1]);
+  }
+  return 0;
 }
 
 static int create_ctx(int nreq, const int* req_nblocks, int nblocks, const int* dims, const double* xyz,
@@ -623,7 +697,8 @@ static int create_ctx(int nreq, const int* req_nblocks, int nblocks, const int* 
                       orion_hip_ctx** out) {
   *out = nullptr;
   if (nblocks <= 0) return fail("nblocks must be positive");
-  if (nreq <= 0 || !req_nblocks || !in) return fail("a context needs at least one request");
+  if (nregions > 0 && !regions) return fail("null regions");
+  if (check_requests(nreq, req_nblocks, nblocks, nregions, regions, in)) return -1;
   std::lock_guard<std::recursive_mutex> guard(g_capture_mutex);
   orion_hip_ctx* c = new orion_hip_ctx();
   auto bail = [&](int rc) {
@@ -633,8 +708,7 @@ static int create_ctx(int nreq, const int* req_nblocks, int nblocks, const int* 
   c->device = g_device;
   if (hipSetDevice(c->device) != hipSuccess) return bail(fail("hipSetDevice(%d) failed", c->device));
   // default-priority streams
-  if (hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess ||
-      hipStreamCreateWithFlags(&c->side_stream, hipStreamNonBlocking) != hipSuccess)
+  if (take_stream(c->device, &c->own_stream) || take_stream(c->device, &c->side_stream))
     return bail(fail("hipStreamCreate failed"));
   if (hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess)
@@ -649,15 +723,9 @@ static int create_ctx(int nreq, const int* req_nblocks, int nblocks, const int* 
   c->req_first.assign(nreq + 1, 0);
   std::vector<Params> preq(nreq);
   for (int r = 0; r < nreq; ++r) {
-    if (req_nblocks[r] <= 0) return bail(fail("request %d has no blocks", r));
     c->req_first[r + 1] = c->req_first[r] + req_nblocks[r];
     init_params(preq[r], &in[r]);
-    const Params& a = preq[r];
-    const Params& z = preq[0];
-    if (a.nvis != z.nvis || a.method != z.method || a.csrv != z.csrv || a.ntmst != z.ntmst || a.gcl != z.gcl)
-      return bail(fail("request %d: a batch's requests must share nvis, method, csrv, ntmst and gcl", r));
   }
-  if (c->req_first[nreq] != nblocks) return bail(fail("the requests' block counts do not add up to nblocks"));
   // blocks
   c->hb.resize(nblocks);
   std::vector<long long> lines_tot(nblocks);
@@ -744,8 +812,6 @@ static int create_ctx(int nreq, const int* req_nblocks, int nblocks, const int* 
         return bail(fail("block %d region %d: window outside the block", d.block + 1, r + 1));
       if (bt < 0) {
         if (d.nbt < 1 || d.nbt > nblocks) return bail(fail("block %d region %d: bad target block", d.block + 1, r + 1));
-        if (c->hb[d.nbt - 1].req != c->hb[d.block].req)
-          return bail(fail("block %d region %d: an interface joins blocks of two requests", d.block + 1, r + 1));
         const BlockDev& T = c->hb[d.nbt - 1];
         const int tm[3] = {T.idim, T.jdim, T.kdim};
         const int ta = std::abs(d.t_st[m]), te = std::abs(d.t_ed[m]);
@@ -941,6 +1007,24 @@ int orion_hip_upload(orion_hip_ctx* c, int b, int f, const double* host, long lo
   if (count != n) return fail("upload: field %d of block %d holds %lld values, got %lld", f, b, n, count);
   HIPCHK(hipMemcpyAsync(p, host, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int orion_hip_copy_block_state(orion_hip_ctx* dst, int dst_block, orion_hip_ctx* src, int src_block) {
+  if (!dst || !src) return fail("copy_block_state: null context");
+  if (dst->device != src->device) return fail("copy_block_state: the contexts are on different devices");
+  double *pd, *ps;
+  long long nd, ns;
+  if (field_info(dst, dst_block, ORION_FIELD_PRIM, &pd, &nd) || field_info(src, src_block, ORION_FIELD_PRIM, &ps, &ns))
+    return -1;
+  const BlockDev &D = dst->hb[dst_block], &S = src->hb[src_block];
+  if (D.idim != S.idim || D.jdim != S.jdim || D.kdim != S.kdim)
+    return fail("copy_block_state: block sizes differ (%d,%d,%d) vs (%d,%d,%d)", D.idim, D.jdim, D.kdim, S.idim,
+                S.jdim, S.kdim);
+  // the source's queued steps first, then the copy in the destination's order
+  HIPCHK(hipStreamSynchronize(src->stream));
+  HIPCHK(hipMemcpyAsync(pd, ps, sizeof(double) * nd, hipMemcpyDeviceToDevice, dst->stream));
+  HIPCHK(hipStreamSynchronize(dst->stream));
   return 0;
 }
 

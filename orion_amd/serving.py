@@ -198,8 +198,10 @@ class Batch:
     """Resident requests batched in one device context
     (orion_hip_batch_create): each kernel launch of a step covers all of
     them, and each request's results are those it has in a context of its
-    own.  ``states[k]``: request k's saved prim per block (a request carried
-    over from an earlier batch), or None for a request starting at step 0."""
+    own.  ``states[k]`` for a request carried over from an earlier batch:
+    ``(batch, j)``, request j of that (still open) batch, whose state is
+    copied on the device, or its prim per block as host arrays; None for a
+    request starting at step 0."""
 
     def __init__(self, case_list: Sequence[cases.Case], states: Optional[Sequence] = None, device: int = 0):
         import ctypes as C
@@ -237,7 +239,12 @@ class Batch:
         if fresh:
             _lib.check(L.orion_hip_update_temperature(self.ctx), "update_temperature")
         for k, st in enumerate(states):
-            if st is not None:
+            if isinstance(st, tuple):
+                src, j = st
+                for b in range(self.cases[k].nblocks):
+                    _lib.check(L.orion_hip_copy_block_state(self.ctx, self.first[k] + b, src.ctx, src.first[j] + b),
+                               "copy_block_state")
+            elif st is not None:
                 for b, a in enumerate(st):
                     self._prim_io(k, b, a, upload=True)
 
@@ -280,7 +287,7 @@ class BatchServer:
     (``Batch``) that advances them all ``chunk`` pseudo-steps at a time.  At
     each chunk boundary finished requests leave and arrived ones are
     admitted (arrival order, within ``slots`` and the HBM budget); the batch
-    is then rebuilt with the carried-over requests' state.  Every request
+    is then rebuilt, the staying requests' state copied over on the device.  Every request
     advances in whole chunks from its step 0, as under ``Server``, so it
     stops at the same step with the same history."""
 
@@ -298,6 +305,7 @@ class BatchServer:
         self.peak_resident = 0
         self.peak_bytes = 0
         self.rebuilds = 0
+        self.rebuild_s = 0.0  # host time spent rebuilding batches
 
     def serve(self, requests: Sequence[Request]) -> List[Result]:
         """Run the stream in real time; returns the results in completion order."""
@@ -322,19 +330,23 @@ class BatchServer:
                     used += footprint(req.case)
                     admitted.append({"req": req, "start": now, "rms": []})
                 if admitted or leaving:
-                    # one rebuild per boundary: carried-over state, then the newcomers
-                    states = [batch.prim(k) for k in stay] if batch is not None else []
-                    if batch is not None:
-                        batch.close()
-                        batch = None
+                    # one rebuild per boundary: the staying requests' state
+                    # copied on the device from the old batch, then the newcomers
+                    states = [(batch, k) for k in stay] if batch is not None else []
                     resident = [resident[k] for k in stay] + admitted
                     leaving = set()
+                    new = None
+                    tb = time.perf_counter()
                     if resident:
-                        batch = self.make_batch([m["req"].case for m in resident], states + [None] * len(admitted),
-                                                self.device)
+                        new = self.make_batch([m["req"].case for m in resident], states + [None] * len(admitted),
+                                              self.device)
                         self.rebuilds += 1
                         self.peak_resident = max(self.peak_resident, len(resident))
                         self.peak_bytes = max(self.peak_bytes, used)
+                    if batch is not None:
+                        batch.close()
+                    batch = new
+                    self.rebuild_s += time.perf_counter() - tb
                 if not resident:
                     time.sleep(max(0.0, pending[0].arrival - (time.perf_counter() - t0)))
                     continue
@@ -384,15 +396,15 @@ def measure(nreq: int = 16, slots: int = 16, seed: int = 2024) -> dict:
     t0 = time.perf_counter()
     srv_ = BatchServer(slots=slots)
     srv = srv_.serve(reqs)
-    print("[serving] batched, %d slots: %.1f s, %d batch rebuilds" % (slots, time.perf_counter() - t0, srv_.rebuilds),
-          file=sys.stderr, flush=True)
+    print("[serving] batched, %d slots: %.1f s, %d batch rebuilds (%.2f s)" % (
+        slots, time.perf_counter() - t0, srv_.rebuilds, srv_.rebuild_s), file=sys.stderr, flush=True)
     by = {r.rid: r for r in seq}
     same = all(np.array_equal(by[r.rid].rms, r.rms) for r in srv)
     a, b = summarize(srv), summarize(seq)
     return {"workload": "%d steady channel solves (1 block, %d-%d nodes, M 0.5-0.85, Re 1e3-1e6), Poisson arrivals "
                         "20/s, done at a 1e-8 RMS drop or 3000 steps" % (nreq, lo, hi),
             "slots": slots, "form": "one batch context of the resident requests, rebuilt at 50-step boundaries",
-            "batch_rebuilds": srv_.rebuilds, "peak_resident": srv_.peak_resident,
+            "batch_rebuilds": srv_.rebuilds, "rebuild_s": srv_.rebuild_s, "peak_resident": srv_.peak_resident,
             "served": a, "sequential": b, "solves_per_s_vs_sequential": a["solves_per_s"] / b["solves_per_s"],
             "same_histories_alone_and_mixed": bool(same)}
 

@@ -72,12 +72,12 @@ def test_sequential_baseline_runs_back_to_back():
 
 class _FakeBatch:
     """Stand-in batch: request k's residual is r0_k * q_k**s at its own step
-    s, which its state (what ``prim`` hands to the next batch) carries, so a
+    s, which the next batch takes over from this one (``(batch, k)``), so a
     request's history does not depend on the batch it runs in."""
 
     def __init__(self, cases, states, device, log, dt=0.003):
         self.cases, self.dt = list(cases), dt
-        self.steps = [int(st[0][0]) if st is not None else 0 for st in states]
+        self.steps = [st[0].steps[st[1]] if st is not None else 0 for st in states]
         log.append([c.name for c in self.cases])
 
     @staticmethod
@@ -129,6 +129,46 @@ def test_batched_server_checks_its_arguments():
         serving.BatchServer(slots=1, hbm_budget=1.0).serve(_stream(1))
 
 
+def test_batch_create_checks_requests_on_the_host():
+    """orion_hip_batch_create refuses, before any device work (so on the
+    CPU), requests without blocks or whose block counts do not add up,
+    requests that differ in an input shared by the batch, and an interface
+    that joins two requests' blocks."""
+    import ctypes as C
+    from orion_amd import _lib
+    from orion_amd.solver import case_inputs, region_descs
+    L = _lib.load()
+    cs = [r.case for r in serving.channel_stream(2, seed=3, nodes=(2.0e3, 3.0e3))]
+    dims = np.ascontiguousarray(np.concatenate([np.asarray(c.dims, dtype=np.int32).reshape(-1) for c in cs]))
+    xyz = np.ascontiguousarray(np.concatenate([np.asarray(c.xyz_flat(), dtype=np.float64) for c in cs]))
+    regs, nregs = region_descs(cs, [0, 1])
+
+    def call(nreq, counts, inps, regions=regs):
+        h = C.c_void_p()
+        rc = L.orion_hip_batch_create(nreq, (C.c_int * 2)(*counts), 2, dims.ctypes.data_as(C.POINTER(C.c_int)),
+                                      xyz.ctypes.data_as(C.POINTER(C.c_double)), nregs, regions, inps, 0,
+                                      C.byref(h))
+        assert h.value is None
+        return rc, L.orion_hip_last_error().decode()
+
+    good = (_lib.Inputs * 2)(*[case_inputs(c) for c in cs])
+    rc, msg = call(0, (1, 1), good)
+    assert rc != 0 and "at least one request" in msg
+    rc, msg = call(2, (1, 2), good)
+    assert rc != 0 and "do not add up" in msg
+    rc, msg = call(2, (0, 2), good)
+    assert rc != 0 and "has no blocks" in msg
+    mixed = (_lib.Inputs * 2)(*[case_inputs(c) for c in cs])
+    mixed[1].nvis = 0
+    rc, msg = call(2, (1, 1), mixed)
+    assert rc != 0 and "must share" in msg
+    cross, _ = region_descs(cs, [0, 1])
+    q = next(i for i in range(nregs) if cross[i].block == 1 and cross[i].bctype < 0)
+    cross[q].nbt = 1  # request 1's periodic interface pointed at request 0's block
+    rc, msg = call(2, (1, 1), good, cross)
+    assert rc != 0 and "joins blocks of requests 1 and 0" in msg
+
+
 def test_oversized_request_is_refused():
     reqs = _stream(1)
     with pytest.raises(ValueError):
@@ -169,3 +209,27 @@ def test_same_result_alone_and_batched():
         assert np.array_equal(a.rms, m.rms), rid
         for pa, pm in zip(a.prim, m.prim):
             assert np.array_equal(pa, pm), rid
+
+
+@pytest.mark.gpu
+def test_batch_refuses_per_request_calls_it_cannot_answer():
+    """A batch reports residuals per request only through orion_hip_run, and
+    a state copy needs blocks of equal sizes."""
+    import ctypes as C
+    from orion_amd import _lib
+    cs = [r.case for r in serving.channel_stream(2, seed=17, nodes=(5.0e3, 9.0e3))]
+    print("\n[gpu] serving: batch refusals", flush=True)
+    L = _lib.load()
+    b = serving.Batch(cs)
+    try:
+        assert L.orion_hip_requests(b.ctx) == 2
+        st = _lib.ResidualStats()
+        assert L.orion_hip_residual(b.ctx, C.byref(st)) != 0
+        assert "through orion_hip_run" in L.orion_hip_last_error().decode()
+        assert L.orion_hip_calculate_time_step(b.ctx, None) != 0
+        assert b.run(3).shape == (3, 2)
+        same = tuple(cs[0].dims[0]) == tuple(cs[1].dims[0])
+        rc = L.orion_hip_copy_block_state(b.ctx, 0, b.ctx, 1)
+        assert (rc == 0) == same
+    finally:
+        b.close()
