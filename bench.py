@@ -237,6 +237,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--tune-steps", type=int, default=3,
+                    help="steps per trial of the sweep-schedule autotuner after the warmup (0: no tuning)")
+    ap.add_argument("--schedule", type=int, default=-1,
+                    help="fix the sweep schedule (0 diagonal, 1 critical path) instead of tuning")
     ap.add_argument("--workload", default="uneven64")
     ap.add_argument("--profile-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
@@ -273,6 +277,16 @@ def main():
           file=sys.stderr, flush=True)
     if a.warmup > 0:
         s.solve(a.warmup)
+    tuning = None
+    if a.schedule >= 0:
+        s.fs.set_schedule(a.schedule)
+    elif a.tune_steps > 0:
+        # untimed, like the warmup; the state is restored after each trial
+        chosen, ms = s.fs.tune_schedule(a.tune_steps)
+        tuning = {"steps_per_trial": a.tune_steps,
+                  "ms_per_step": {n: round(v, 4) for n, v in zip(s.fs.SCHEDULES, ms)}}
+        print("[bench] sweep schedule tuning: %s -> %s" % (json.dumps(tuning["ms_per_step"]), s.fs.SCHEDULES[chosen]),
+              file=sys.stderr, flush=True)
     s.fs.synchronize()
     t0 = time.perf_counter()
     hist = s.solve(a.steps)
@@ -307,7 +321,9 @@ def main():
         "config": {"workload": case.name, "blocks": case.nblocks, "interior_nodes": N,
                    "device_bytes": s.fs.device_bytes(),
                    "scheme": "WCNS-E-5+Roe, 6th-order viscous, LU-SGS+2 SGS sweeps", "precision": "fp64",
-                   "moo": case.inputs["moo"], "reynolds": case.inputs["reynolds"], "cfl": case.inputs["cfl"]},
+                   "moo": case.inputs["moo"], "reynolds": case.inputs["reynolds"], "cfl": case.inputs["cfl"],
+                   "sweep_schedule": s.fs.SCHEDULES[s.fs.schedule()]},
+        "schedule_tuning": tuning,
         "roofline": roof,
         "stage_rooflines": stages,
         "step_roofline": step_roof,

@@ -159,6 +159,24 @@ int orion_hip_run(orion_hip_ctx* ctx, int nsteps, orion_residual_stats* hist);
 #define ORION_PHASES 18
 int orion_hip_profile_run(orion_hip_ctx* ctx, int nsteps, double* phase_ms, long long* phase_launches);
 
+/* No counterpart in the reference: the launch order of the LU-SGS tiles.
+ * 0 diagonal (by tile diagonal over all blocks; the default), 1 critical
+ * path first (by the planes left to the end of the tile's block sweep,
+ * longest first).  Every order launches a tile after the tiles it waits
+ * for, and the results are the same under every order; they differ in how
+ * soon the longest chains get CUs when the tiles outnumber them. */
+#define ORION_SCHEDULES 2
+int orion_hip_set_schedule(orion_hip_ctx* ctx, int schedule);
+int orion_hip_get_schedule(const orion_hip_ctx* ctx);
+/* The autotuner of the sweep schedule: from the current state, times
+ * `steps` pseudo-steps (after one untimed step) under each schedule,
+ * restoring the state after each trial, and keeps the fastest.  chosen
+ * (may be NULL) gets its index, ms_per_step (ORION_SCHEDULES entries, may
+ * be NULL) each schedule's time.  The solution state afterwards is the one
+ * before the call, bit for bit (prim alone fixes the next step).
+ * Synchronous; holds a device copy of prim while it runs. */
+int orion_hip_tune_schedule(orion_hip_ctx* ctx, int steps, int* chosen, double* ms_per_step);
+
 /* Output (postprocess::PostProcess, src/postprocess/PostProcess.hpp:10-33).
  *
  * The point fields that PostProcess::write_block_vts_binary writes

@@ -100,6 +100,9 @@ def _open():
         "orion_hip_synchronize": ([P], C.c_int),
         "orion_hip_set_stream": ([P, C.c_void_p], C.c_int),
         "orion_hip_set_overlap": ([P, C.c_int], C.c_int),
+        "orion_hip_set_schedule": ([P, C.c_int], C.c_int),
+        "orion_hip_get_schedule": ([P], C.c_int),
+        "orion_hip_tune_schedule": ([P, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)], C.c_int),
         "orion_hip_device_bytes": ([P], C.c_size_t),
         "orion_hip_interior_nodes": ([P], C.c_longlong),
         "orion_hip_output_fields": ([P, C.c_int, C.POINTER(C.c_float), C.c_longlong], C.c_int),

@@ -392,24 +392,26 @@ def eight_block_case(seed: int = 7, nsteps: int = 500, block=(129, 65, 65)) -> C
 
 
 def uneven_sizes(seed: int, n: int = 4, lo: int = 17, hi: int = 129, target: float = 8.0e6,
-                 tol: float = 0.02):
+                 tol: float = 0.02, tries: int = 100000):
     """Per-axis lattice sizes drawn log-uniformly from [lo, hi]; the seed is
-    advanced until the total node count is within ``tol`` of ``target``.
+    advanced until the total node count is within ``tol`` of ``target``
+    (at most ``tries`` seeds: a target far from what [lo, hi] gives raises).
     Returns (sizes_x, sizes_y, sizes_z, seed_used)."""
-    s = seed
-    while True:
+    for s in range(seed, seed + tries):
         rng = np.random.default_rng(s)
         sz = np.rint(np.exp(rng.uniform(np.log(lo), np.log(hi), size=(3, n)))).astype(int)
         tot = float(sz[0].sum() * sz[1].sum() * sz[2].sum())
         if abs(tot - target) <= tol * target:
             return list(sz[0]), list(sz[1]), list(sz[2]), s
-        s += 1
+    raise ValueError("no seed in [%d, %d) gives %.3g nodes from sizes in [%d, %d]" % (seed, seed + tries, target, lo, hi))
 
 
-def uneven_lattice_case(seed: int = 11, nsteps: int = 1, target: float = 8.0e6) -> Case:
-    """64 uneven blocks (4x4x4 lattice, sizes log-U[17,129], about 8M nodes),
-    same physics as the channel case (M=0.5, Re=5e3, CFL 5)."""
-    sx, sy, sz, used = uneven_sizes(seed, target=target)
+def uneven_lattice_case(seed: int = 11, nsteps: int = 1, target: float = 8.0e6, lo: int = 17,
+                        hi: int = 129) -> Case:
+    """64 uneven blocks (4x4x4 lattice, sizes log-U[lo, hi], about ``target``
+    nodes; by default 17-129 and 8M), same physics as the channel case
+    (M=0.5, Re=5e3, CFL 5)."""
+    sx, sy, sz, used = uneven_sizes(seed, lo=lo, hi=hi, target=target)
     c = lattice_case(sx, sy, sz, seed=used, moo=0.5, reynolds=5.0e3, cfl=5.0, nsteps=nsteps,
                      name="uneven64_seed%d" % used)
     return c

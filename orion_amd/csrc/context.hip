@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <chrono>
 #include <filesystem>
 #include <iomanip>
 #include <map>
@@ -98,6 +99,8 @@ struct orion_hip_ctx {
   std::vector<SweepTile> tiles;
   SweepTile* d_tiles = nullptr;
   int* d_order[2] = {nullptr, nullptr};  // tile launch order: forward, backward sweeps
+  std::vector<int> sched_order[ORION_SCHEDULES][2];  // candidate orders (host)
+  int schedule = 0;                      // the one in d_order
   unsigned long long* d_flags = nullptr; // per-tile progress
   unsigned long long* d_epoch = nullptr; // sweep counter
   int* d_err = nullptr;                  // set by a sweep whose wait ran out
@@ -581,13 +584,40 @@ static int build_sweep(orion_hip_ctx* c) {
   }
   n = (n + 31) / 32 * 32;
   const int nt = (int)c->tiles.size();
-  std::vector<int> ord(nt);
-  for (int i = 0; i < nt; ++i) ord[i] = i;
-  auto diag = [&](int i) { return c->tiles[i].r0 / TWD + c->tiles[i].c0 / TWD; };
-  std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return diag(a) < diag(b); });
-  if (upload_vec(c, &c->d_order[0], ord)) return -1;
-  std::reverse(ord.begin(), ord.end());
-  if (upload_vec(c, &c->d_order[1], ord)) return -1;
+  // Launch orders of the tiles (forward sweeps, backward sweeps).  Every
+  // order launches a tile after the upstream tiles it waits for, so the
+  // persistent grid drains whatever the dispatch.
+  //   0 diagonal: by tile diagonal R + C over all blocks, reversed for the
+  //     backward sweeps;
+  //   1 critical path first: by the planes left from the tile's start to
+  //     the end of its block's sweep (forward: last plane of the block minus
+  //     the tile's first; backward: the tile's last plane), longest first,
+  //     so that the tiles of the longest chains take the CUs first when the
+  //     tiles outnumber them.  Upstream tiles have more planes left.
+  {
+    std::vector<int> ord(nt);
+    for (int i = 0; i < nt; ++i) ord[i] = i;
+    auto diag = [&](int i) { return c->tiles[i].r0 / TWD + c->tiles[i].c0 / TWD; };
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return diag(a) < diag(b); });
+    c->sched_order[0][0] = ord;
+    std::reverse(ord.begin(), ord.end());
+    c->sched_order[0][1] = ord;
+    auto last_plane = [&](int i) {
+      const BlockDev& B = c->hb[c->tiles[i].block];
+      return B.sn[0] + B.sn[1] + B.sn[2] - 3;
+    };
+    for (int i = 0; i < nt; ++i) ord[i] = i;
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) {
+      return last_plane(a) - c->tiles[a].s_first > last_plane(b) - c->tiles[b].s_first;
+    });
+    c->sched_order[1][0] = ord;
+    for (int i = 0; i < nt; ++i) ord[i] = i;
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return c->tiles[a].s_last > c->tiles[b].s_last; });
+    c->sched_order[1][1] = ord;
+  }
+  c->schedule = 0;
+  if (upload_vec(c, &c->d_order[0], c->sched_order[0][0])) return -1;
+  if (upload_vec(c, &c->d_order[1], c->sched_order[0][1])) return -1;
   if (upload_vec(c, &c->d_tiles, c->tiles)) return -1;
   c->sw.n = n;
   if (dalloc(c, (void**)&c->sw.own, sizeof(double2) * SW_OWN * n) ||
@@ -1466,6 +1496,68 @@ int orion_hip_synchronize(orion_hip_ctx* c) {
 int orion_hip_set_stream(orion_hip_ctx* c, void* s) {
   c->stream = s ? (hipStream_t)s : c->own_stream;
   return 0;
+}
+
+int orion_hip_set_schedule(orion_hip_ctx* c, int which) {
+  if (!c || which < 0 || which >= ORION_SCHEDULES) return fail("set_schedule: no schedule %d", which);
+  if (which == c->schedule) return 0;
+  // the captured step reads the order arrays in place: only their contents change
+  for (int k = 0; k < 2; ++k) {
+    const std::vector<int>& v = c->sched_order[which][k];
+    if (!v.empty())
+      HIPCHK(hipMemcpyAsync(c->d_order[k], v.data(), sizeof(int) * v.size(), hipMemcpyHostToDevice, c->stream));
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->schedule = which;
+  return 0;
+}
+
+int orion_hip_get_schedule(const orion_hip_ctx* c) { return c ? c->schedule : -1; }
+
+int orion_hip_tune_schedule(orion_hip_ctx* c, int steps, int* chosen, double* ms_per_step) {
+  if (!c || steps <= 0) return fail("tune_schedule: bad context or step count");
+  // the state to return to after every trial: each block's prim
+  std::vector<long long> at(c->nb + 1, 0);
+  for (int b = 0; b < c->nb; ++b) at[b + 1] = at[b] + 6 * c->hb[b].P;
+  double* saved = nullptr;
+  if (hipMalloc((void**)&saved, sizeof(double) * at[c->nb]) != hipSuccess)
+    return fail("tune_schedule: no memory for the state copy");
+  auto copy_state = [&](bool save) -> int {
+    for (int b = 0; b < c->nb; ++b) {
+      double* prim = c->hb[b].prim;
+      double* dst = save ? saved + at[b] : prim;
+      const double* src = save ? prim : saved + at[b];
+      if (hipMemcpyAsync(dst, src, sizeof(double) * 6 * c->hb[b].P, hipMemcpyDeviceToDevice, c->stream) != hipSuccess)
+        return fail("tune_schedule: state copy failed");
+    }
+    return hipStreamSynchronize(c->stream) == hipSuccess ? 0 : fail("tune_schedule: state copy failed");
+  };
+  int rc = copy_state(true);
+  int best = c->schedule;
+  double best_ms = 0.0;
+  for (int k = 0; rc == 0 && k < ORION_SCHEDULES; ++k) {
+    rc = orion_hip_set_schedule(c, k);
+    if (rc == 0) rc = orion_hip_run(c, 1, nullptr);  // untimed: capture and warm the step
+    double ms = 0.0;
+    if (rc == 0) {
+      const auto t0 = std::chrono::steady_clock::now();
+      rc = orion_hip_run(c, steps, nullptr);  // synchronises at the end
+      ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / steps;
+    }
+    if (rc == 0) rc = copy_state(false);
+    if (rc == 0) {
+      if (ms_per_step) ms_per_step[k] = ms;
+      if (k == 0 || ms < best_ms) {
+        best = k;
+        best_ms = ms;
+      }
+    }
+  }
+  if (rc == 0) rc = orion_hip_set_schedule(c, best);
+  hipStreamSynchronize(c->stream);
+  hipFree(saved);
+  if (rc == 0 && chosen) *chosen = best;
+  return rc;
 }
 
 int orion_hip_set_overlap(orion_hip_ctx* c, int on) {

@@ -171,6 +171,25 @@ class FlowFieldSet:
         stream (many contexts busy at once); the results are the same."""
         check(_lib.load().orion_hip_set_overlap(self.ctx, int(bool(on))), "set_overlap")
 
+    SCHEDULES = ("diagonal", "critical_path")
+
+    def set_schedule(self, which: int) -> None:
+        """The LU-SGS tile launch order (0 diagonal, 1 critical path first);
+        the results are the same under every order."""
+        check(_lib.load().orion_hip_set_schedule(self.ctx, int(which)), "set_schedule")
+
+    def schedule(self) -> int:
+        return int(_lib.load().orion_hip_get_schedule(self.ctx))
+
+    def tune_schedule(self, steps: int = 3):
+        """Time ``steps`` pseudo-steps under each schedule from the current
+        state (which is restored), keep the fastest; returns (index,
+        ms per step of each schedule)."""
+        chosen = C.c_int()
+        ms = (C.c_double * len(self.SCHEDULES))()
+        check(_lib.load().orion_hip_tune_schedule(self.ctx, int(steps), C.byref(chosen), ms), "tune_schedule")
+        return int(chosen.value), list(ms)
+
     # ---------------------------------------------------------------- output
     def output_fields(self, b: int) -> np.ndarray:
         """The VTS point fields of block b, formed on the device as float32:

@@ -1,0 +1,64 @@
+"""The LU-SGS tile launch orders and their autotuner.
+
+Every schedule launches a tile after the tiles it waits for, and each
+cell's arithmetic does not depend on the order, so the results must be the
+same bit for bit under every schedule; the tuner runs trial steps and must
+leave the solution state as it found it.
+"""
+import numpy as np
+import pytest
+
+from orion_amd import cases
+
+
+def _case():
+    # 64 blocks of 17-26 nodes per axis, up to four tiles per block: the orders differ
+    return cases.uneven_lattice_case(seed=11, nsteps=4, target=6.0e5, lo=17, hi=26)
+
+
+@pytest.mark.gpu
+def test_schedules_give_identical_results():
+    from orion_amd.solver import OrionSolver
+    case = _case()
+    print("\n[gpu] sweep schedules: %d blocks, %d nodes" % (case.nblocks, case.interior_nodes()), flush=True)
+    runs = []
+    for k in (0, 1):
+        s = OrionSolver(case)
+        s.fs.set_schedule(k)
+        assert s.fs.schedule() == k
+        h = s.solve(4)
+        runs.append(([x.rms_residual for x in h], [x.max_loc for x in h],
+                     [s.fs.field(b, "prim") for b in range(case.nblocks)]))
+        s.fs.close()
+    assert runs[0][0] == runs[1][0] and runs[0][1] == runs[1][1]
+    for a, b in zip(runs[0][2], runs[1][2]):
+        assert np.array_equal(a, b)
+
+
+@pytest.mark.gpu
+def test_tuning_restores_the_state():
+    from orion_amd.solver import OrionSolver
+    case = _case()
+    print("\n[gpu] sweep schedule tuning", flush=True)
+    a = OrionSolver(case)
+    a.solve(2)
+    before = [a.fs.field(b, "prim") for b in range(case.nblocks)]
+    chosen, ms = a.fs.tune_schedule(2)
+    assert chosen in (0, 1) and a.fs.schedule() == chosen and all(m > 0 for m in ms)
+    for b in range(case.nblocks):
+        assert np.array_equal(a.fs.field(b, "prim"), before[b]), b
+    # and the solve goes on as one that was never tuned
+    ref = OrionSolver(case)
+    ref.solve(2)
+    ha, hr = a.solve(2), ref.solve(2)
+    assert [x.rms_residual for x in ha] == [x.rms_residual for x in hr]
+    for s in (a, ref):
+        s.fs.close()
+
+
+def test_uneven_sizes_search_is_bounded():
+    sx, sy, sz, _ = cases.uneven_sizes(11, lo=17, hi=26, target=6.0e5)
+    assert abs(sum(sx) * sum(sy) * sum(sz) - 6.0e5) <= 0.02 * 6.0e5
+    assert min(sx + sy + sz) >= 17 and max(sx + sy + sz) <= 26
+    with pytest.raises(ValueError):  # unreachable from sizes 17-129
+        cases.uneven_sizes(11, target=6.0e5, tries=500)
