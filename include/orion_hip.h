@@ -160,9 +160,9 @@ int orion_hip_run(orion_hip_ctx* ctx, int nsteps, orion_residual_stats* hist);
 int orion_hip_profile_run(orion_hip_ctx* ctx, int nsteps, double* phase_ms, long long* phase_launches);
 
 /* No counterpart in the reference: the launch order of the LU-SGS tiles.
- * 0 diagonal (by tile diagonal over all blocks; the default), 1 critical
- * path first (by the planes left to the end of the tile's block sweep,
- * longest first).  Every order launches a tile after the tiles it waits
+ * 0 diagonal (by tile diagonal over all blocks), 1 critical path first
+ * (by the planes left to the end of the tile's block sweep, longest first;
+ * the default).  Every order launches a tile after the tiles it waits
  * for, and the results are the same under every order; they differ in how
  * soon the longest chains get CUs when the tiles outnumber them. */
 #define ORION_SCHEDULES 2

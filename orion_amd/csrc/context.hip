@@ -615,9 +615,9 @@ static int build_sweep(orion_hip_ctx* c) {
     std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return c->tiles[a].s_last > c->tiles[b].s_last; });
     c->sched_order[1][1] = ord;
   }
-  c->schedule = 0;
-  if (upload_vec(c, &c->d_order[0], c->sched_order[0][0])) return -1;
-  if (upload_vec(c, &c->d_order[1], c->sched_order[0][1])) return -1;
+  c->schedule = 1;  // critical path first until a tuner says otherwise
+  if (upload_vec(c, &c->d_order[0], c->sched_order[1][0])) return -1;
+  if (upload_vec(c, &c->d_order[1], c->sched_order[1][1])) return -1;
   if (upload_vec(c, &c->d_tiles, c->tiles)) return -1;
   c->sw.n = n;
   if (dalloc(c, (void**)&c->sw.own, sizeof(double2) * SW_OWN * n) ||

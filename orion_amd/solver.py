@@ -174,8 +174,8 @@ class FlowFieldSet:
     SCHEDULES = ("diagonal", "critical_path")
 
     def set_schedule(self, which: int) -> None:
-        """The LU-SGS tile launch order (0 diagonal, 1 critical path first);
-        the results are the same under every order."""
+        """The LU-SGS tile launch order (0 diagonal, 1 critical path first,
+        the default); the results are the same under every order."""
         check(_lib.load().orion_hip_set_schedule(self.ctx, int(which)), "set_schedule")
 
     def schedule(self) -> int:
