@@ -14,8 +14,11 @@ with interface averaging, LU-SGS update, residual).  value = interior nodes x
 timed steps / wall time of the timed steps, with all state resident in HBM.
 
 Prints one JSON line.  The roofline entry is the dominant kernel family,
-timed live with HIP events on the solver's stream; the CPU baseline is the
-C oracle (a port of the reference) timed on this host on a bounded sample.
+timed live with HIP events on the solver's stream.  The CPU baseline is the
+reference's own MPI CPU path (the reference solver built from its sources,
+oracle/_ref) on 16 ranks of this host for a few steps; the C oracle (a port
+of the reference, OpenMP) is timed beside it as cpu_port.  Both samples are
+bounded.
 """
 import argparse
 import json
@@ -214,6 +217,25 @@ def serving_run(nreq, slots):
     return json.loads(p.stdout.decode().strip().splitlines()[-1])
 
 
+def cpu_reference(case, ranks, nsteps):
+    """The reference's MPI CPU path (its own solver built from its sources,
+    oracle/_ref) on ``ranks`` processes of this host: ``nsteps`` whole
+    pseudo-steps of the same case, set-up excluded.  None without the
+    build."""
+    import tempfile
+    from oracle import binding
+    if not (binding.have_reference() and os.path.exists(binding.MPIEXEC)):
+        return None
+    with tempfile.TemporaryDirectory() as d:
+        steps, secs, nproc = binding.time_reference(case, ranks, nsteps, d)
+    print("[bench] reference cpu path: %d steps in %.1f s on %d MPI ranks" % (steps, secs, nproc), file=sys.stderr,
+          flush=True)
+    return {"value": case.interior_nodes() * steps / secs, "unit": "cell-steps/s", "cores": nproc, "kind": "reference",
+            "sample": "%d pseudo-steps of the same %d-block, %d-node case (from the freestream) by the reference "
+                      "solver built from its own sources, %d MPI ranks, step loop timed between barriers (%.1f s)" % (
+                          steps, case.nblocks, case.interior_nodes(), nproc, secs)}
+
+
 def cpu_baseline(case, threads, seconds):
     """The C oracle (a restatement of the reference path, OpenMP over blocks)
     on the same case: whole pseudo-steps until ``seconds`` have passed."""
@@ -246,6 +268,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--ref-steps", type=int, default=3, help="steps of the reference CPU baseline (0: port only)")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
     ap.add_argument("--serve-requests", type=int, default=16)
     ap.add_argument("--serve-slots", type=int, default=16)
@@ -335,7 +358,12 @@ def main():
         out["serving"] = serving_run(a.serve_requests, a.serve_slots)
     if not a.no_cpu:
         threads = a.cpu_threads or min(16, os.cpu_count() or 1)
-        out["cpu_baseline"] = cpu_baseline(case, threads, a.cpu_seconds)
+        port = cpu_baseline(case, threads, a.cpu_seconds)
+        ref = cpu_reference(case, threads, a.ref_steps) if a.ref_steps > 0 else None
+        out["cpu_baseline"] = ref or port
+        if ref:
+            out["cpu_port"] = port
+            out["gpu_over_cpu_port"] = value / port["value"]
         out["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]
     print(json.dumps(out), flush=True)
     return 0

@@ -160,6 +160,27 @@ def have_reference() -> bool:
     return os.path.exists(REF_HARNESS)
 
 
+MPIEXEC = os.environ.get("ORION_MPIEXEC", "/opt/conda/bin/mpiexec")  # the MPI the reference build links
+
+
+def time_reference(case, ranks: int, nsteps: int, workdir: str):
+    """The reference solver's own CPU path (oracle/_ref/ref_harness in its
+    timing mode) on ``ranks`` MPI processes of this host: ``nsteps``
+    pseudo-steps of ``case`` from the freestream, timed between barriers
+    around the step loop (set-up excluded).  Returns (steps, seconds,
+    ranks)."""
+    casedir, outdir = os.path.join(workdir, "case"), os.path.join(workdir, "out")
+    case.write(casedir)
+    os.makedirs(outdir, exist_ok=True)
+    env = dict(os.environ)
+    env.pop("LD_LIBRARY_PATH", None)
+    subprocess.run([MPIEXEC, "-launcher", "fork", "-n", str(int(ranks)), REF_HARNESS, casedir, str(int(nsteps)), outdir,
+                    "0", "time"], check=True, stdout=subprocess.DEVNULL, env=env, timeout=900)
+    with open(os.path.join(outdir, "time.txt")) as f:
+        steps, secs, nproc = f.read().split()
+    return int(steps), float(secs), int(nproc)
+
+
 def run_reference(case, casedir: str, outdir: str, nsteps: int, dump_steps: int = 0, vts: bool = False):
     """Write ``case`` in the reference's formats and run the reference
     solver's operators for ``nsteps``.  Returns (hist[n,8], reader) where

@@ -65,7 +65,7 @@ static void dump_state(const std::string& path, preprocess::FlowFieldSet& fs) {
 
 int main(int argc, char** argv) {
   if (argc < 4) {
-    std::fprintf(stderr, "usage: %s casedir nsteps outdir [dump_steps]\n", argv[0]);
+    std::fprintf(stderr, "usage: %s casedir nsteps outdir [dump_steps [vts|time]]\n", argv[0]);
     return 2;
   }
   const std::string casedir = argv[1];
@@ -73,6 +73,10 @@ int main(int argc, char** argv) {
   std::string outdir = argv[3];
   const int dump_steps = argc > 4 ? std::atoi(argv[4]) : 0;
   const bool vts = argc > 5 && std::string(argv[5]) == "vts";
+  // time: the CPU baseline, on any number of MPI ranks -- the freestream
+  // start, no dumps; rank 0 writes outdir/time.txt (steps, seconds of the
+  // step loop between barriers, ranks) and nothing else is written
+  const bool timing = argc > 5 && std::string(argv[5]) == "time";
   if (outdir[0] != '/') {
     char cwd[4096];
     if (!getcwd(cwd, sizeof cwd)) return 2;
@@ -99,7 +103,7 @@ int main(int argc, char** argv) {
   preprocess::initialize_flow(params, fs);
   preprocess::update_temperature(params, fs);
 
-  if (FILE* f = std::fopen("init.bin", "rb")) {
+  if (FILE* f = timing ? nullptr : std::fopen("init.bin", "rb")) {
     for (int nb : fs.local_block_ids) {
       auto& bf = fs.blocks[nb];
       if (std::fread(bf.prim.hostPtr(), sizeof(double), bf.prim.size(), f) != bf.prim.size()) {
@@ -113,8 +117,10 @@ int main(int argc, char** argv) {
 
   solver::HaloExchanger halo;
   std::vector<double> hist;
+  MPI_Barrier(MPI_COMM_WORLD);
+  const double t0 = MPI_Wtime();
   for (int step = 1; step <= nsteps; ++step) {
-    const bool dump = step <= dump_steps;
+    const bool dump = !timing && step <= dump_steps;
     auto stage = [&](const char* name) {
       if (dump) dump_state(outdir + "/s" + std::to_string(step) + "_" + name + ".bin", fs);
     };
@@ -143,6 +149,19 @@ int main(int argc, char** argv) {
     hist.push_back(st.max_residual);
     for (int m = 0; m < 5; ++m) hist.push_back((double)st.max_loc[m]);
     hist.push_back(params.step.phydtime);
+  }
+  MPI_Barrier(MPI_COMM_WORLD);
+  const double t1 = MPI_Wtime();
+  if (timing) {
+    if (core::Runtime::myid == 0) {
+      std::string p = outdir + "/time.txt";
+      FILE* f = std::fopen(p.c_str(), "w");
+      if (!f) { std::perror(p.c_str()); return 2; }
+      std::fprintf(f, "%d %.6f %d\n", nsteps, t1 - t0, core::Runtime::nproc);
+      std::fclose(f);
+    }
+    MPI_Finalize();
+    return 0;
   }
   {
     std::string p = outdir + "/hist.bin";
