@@ -226,7 +226,18 @@ class Batch:
                                             xyz.ctypes.data_as(C.POINTER(C.c_double)), nreg, regs, inps, 0,
                                             C.byref(h)), "batch_create")
         self.ctx = h
+        try:
+            self._set_up(states, L)
+        except Exception:
+            self.close()
+            raise
+
+    def _set_up(self, states, L) -> None:
+        from . import _lib
+        n = len(self.cases)
         states = list(states) if states is not None else [None] * n
+        if len(states) != n:
+            raise ValueError("%d states for %d requests" % (len(states), n))
         # requests starting at step 0: their initial field, then the
         # temperature, as a context of their own sets them up; carried-over
         # requests: their saved prim (temperature included) after that
@@ -280,6 +291,12 @@ class Batch:
             from . import _lib
             _lib.load().orion_hip_ctx_destroy(self.ctx)
             self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class BatchServer:
