@@ -540,7 +540,7 @@ static int build_topology(orion_hip_ctx* c, const std::vector<std::vector<Reg>>&
 // tile's upstream neighbours come first: by increasing (row tile + col tile)
 // for the forward sweeps, by decreasing for the backward ones.
 static int build_sweep(orion_hip_ctx* c) {
-  const int TWD = 16;
+  const int TWD = SWEEP_TW, TRD = SWEEP_TR;
   long long n = 0;
   c->tiles.clear();
   for (int b = 0; b < c->nb; ++b) {
@@ -564,15 +564,15 @@ static int build_sweep(orion_hip_ctx* c) {
     const long long planes = (long long)d[0] + d[1] + d[2] - 2;
     if (planes >= 60000) return fail("block %d: %lld hyperplanes exceed the sweep limit", b + 1, planes);
     n += planes * B.sn[2] * B.sncp;
-    const int tc = (B.sn[1] + TWD - 1) / TWD, tr = (B.sn[2] + TWD - 1) / TWD;
+    const int tc = (B.sn[1] + TWD - 1) / TWD, tr = (B.sn[2] + TRD - 1) / TRD;
     const int first = (int)c->tiles.size();
     for (int R = 0; R < tr; ++R)
       for (int C = 0; C < tc; ++C) {
         SweepTile t{};
         t.block = b;
-        t.r0 = R * TWD;
+        t.r0 = R * TRD;
         t.c0 = C * TWD;
-        const int rhi = std::min(t.r0 + TWD - 1, B.sn[2] - 1), chi = std::min(t.c0 + TWD - 1, B.sn[1] - 1);
+        const int rhi = std::min(t.r0 + TRD - 1, B.sn[2] - 1), chi = std::min(t.c0 + TWD - 1, B.sn[1] - 1);
         t.s_first = t.r0 + t.c0;
         t.s_last = rhi + chi + B.sn[0] - 1;
         t.nbr[0] = C > 0 ? first + R * tc + C - 1 : -1;
@@ -597,7 +597,7 @@ static int build_sweep(orion_hip_ctx* c) {
   {
     std::vector<int> ord(nt);
     for (int i = 0; i < nt; ++i) ord[i] = i;
-    auto diag = [&](int i) { return c->tiles[i].r0 / TWD + c->tiles[i].c0 / TWD; };
+    auto diag = [&](int i) { return c->tiles[i].r0 / TRD + c->tiles[i].c0 / TWD; };
     std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return diag(a) < diag(b); });
     c->sched_order[0][0] = ord;
     std::reverse(ord.begin(), ord.end());

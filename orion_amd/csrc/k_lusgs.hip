@@ -44,8 +44,9 @@
 
 namespace orion_hip {
 
-constexpr int TW = 16;         // tile width along col and row
-constexpr int TL = TW * TW;    // lanes per tile workgroup
+constexpr int TW = SWEEP_TW;   // tile width along col
+constexpr int TR = SWEEP_TR;   // tile height along row
+constexpr int TL = TW * TR;    // lanes per tile workgroup (of each kind)
 constexpr int KEY_BITS = 16;   // progress flag: sweep serial << KEY_BITS | plane key
 constexpr int CPOL_SC1 = 16;   // cache-policy operand of a memory builtin: the sc1 bit (gfx950)
 
@@ -590,11 +591,12 @@ __global__ void __launch_bounds__(2 * TL, 1) lusgs_tile_kernel(SweepArrays S, co
       idx = (i % 48) / 3;
       ek[j] = i % 3;
     }
-    if (role == 0) {
+    if (role == 0) {  // the col edge has TR cells
+      item = item && idx < TR;
       ec[j] = FWD ? T.c0 - 1 : T.c0 + TW;
       er[j] = T.r0 + idx;
     } else {
-      er[j] = FWD ? T.r0 - 1 : T.r0 + TW;
+      er[j] = FWD ? T.r0 - 1 : T.r0 + TR;
       ec[j] = T.c0 + idx;
     }
     ein[j] = lw0 && item && er[j] >= 0 && er[j] < nr && ec[j] >= 0 && ec[j] < nc;
@@ -695,7 +697,7 @@ __global__ void __launch_bounds__(2 * TL, 1) lusgs_tile_kernel(SweepArrays S, co
     const bool march = (ax == am);
     const bool col = (ax == ac);
     const int pos = col ? lc : lr, lim = col ? nc : nr, coord = col ? c : r;
-    const bool inside = march || (pos + d >= 0 && pos + d < TW);
+    const bool inside = march || (pos + d >= 0 && pos + d < (col ? TW : TR));
     const int ln = march ? lane : lane + (col ? d : d * TW);
     const int role = col ? 0 : 1, idx = col ? lr : lc;
     const int e = t % 3;

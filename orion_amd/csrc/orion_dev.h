@@ -122,7 +122,15 @@ __host__ __device__ inline long long sweep_field(long long q, int c, int ncp, in
 // unrelated tiles in one line would make those accesses contend.
 constexpr int FLAG_STRIDE = 32;
 
-// One tile of the LU-SGS pipeline: 16 x 16 (row, col) pencils of a block.
+// LU-SGS tiles: SWEEP_TW columns by SWEEP_TR rows of pencils (the block
+// layout pads columns to whole multiples of SWEEP_TW).  16 x 8 tiles were
+// measured slower than 16 x 16 on the 64-block mesh: a step's time is set
+// by the update lanes' per-cell chain, not by the tile's bytes, and the
+// extra tile diagonals add lag.
+constexpr int SWEEP_TW = 16;
+constexpr int SWEEP_TR = 16;
+
+// One tile of the LU-SGS pipeline: SWEEP_TR x SWEEP_TW (row, col) pencils of a block.
 struct SweepTile {
   int block;
   int r0, c0;           // first row and column
