@@ -80,6 +80,7 @@ class Result:
     nodes: int
     rms: np.ndarray  # residual history
     prim: Optional[List[np.ndarray]] = None  # final fields, when asked for
+    error: Optional[str] = None  # why the request was refused (it then ran no step)
 
     @property
     def latency(self) -> float:
@@ -355,8 +356,29 @@ class BatchServer:
                     new = None
                     tb = time.perf_counter()
                     if resident:
-                        new = self.make_batch([m["req"].case for m in resident], states + [None] * len(admitted),
-                                              self.device)
+                        try:
+                            new = self.make_batch([m["req"].case for m in resident],
+                                                  states + [None] * len(admitted), self.device)
+                        except Exception:
+                            if not admitted:
+                                raise
+                            # a newcomer the device refuses must not take the others down:
+                            # admit them one at a time, refusing those that fail alone
+                            keep = resident[:len(stay)]
+                            for m in admitted:
+                                try:
+                                    self.make_batch([x["req"].case for x in keep + [m]],
+                                                    states + [None] * (len(keep) - len(stay) + 1), self.device).close()
+                                    keep.append(m)
+                                except Exception as e:  # noqa: BLE001 -- reported in the result
+                                    now = time.perf_counter() - t0
+                                    results.append(Result(m["req"].rid, m["req"].arrival, now, now, 0, False,
+                                                          m["req"].case.interior_nodes(), np.zeros(0), None, str(e)))
+                            resident = keep
+                            used = sum(footprint(m["req"].case) for m in resident)
+                            new = (self.make_batch([m["req"].case for m in resident],
+                                                   states + [None] * (len(resident) - len(stay)), self.device)
+                                   if resident else None)
                         self.rebuilds += 1
                         self.peak_resident = max(self.peak_resident, len(resident))
                         self.peak_bytes = max(self.peak_bytes, used)
@@ -365,7 +387,8 @@ class BatchServer:
                     batch = new
                     self.rebuild_s += time.perf_counter() - tb
                 if not resident:
-                    time.sleep(max(0.0, pending[0].arrival - (time.perf_counter() - t0)))
+                    if pending:
+                        time.sleep(max(0.0, pending[0].arrival - (time.perf_counter() - t0)))
                     continue
                 hist = batch.run(self.chunk)
                 finish = time.perf_counter() - t0
@@ -386,10 +409,14 @@ class BatchServer:
 
 def summarize(results: Sequence[Result]) -> dict:
     """solves/s = completed / (last completion - first arrival); latency =
-    completion - arrival, p50 and p99."""
+    completion - arrival, p50 and p99; refused requests are counted apart."""
+    refused = sum(r.error is not None for r in results)
+    results = [r for r in results if r.error is None]
+    if not results:
+        return {"requests": 0, "refused": refused}
     lat = np.array([r.latency for r in results])
     span = max(r.finish for r in results) - min(r.arrival for r in results)
-    return {"requests": len(results), "converged": int(sum(r.converged for r in results)),
+    return {"requests": len(results), "refused": refused, "converged": int(sum(r.converged for r in results)),
             "solves_per_s": len(results) / span if span > 0 else float("nan"),
             "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
             "mean_steps": float(np.mean([r.steps for r in results])),

@@ -122,6 +122,32 @@ def test_batched_histories_carry_across_rebuilds():
         assert got.start >= got.arrival - 1e-6
 
 
+def test_batched_server_refuses_a_bad_request_and_serves_the_rest():
+    """A request the device refuses (its batch cannot be built) comes back
+    with the error and no steps; the others are served as usual."""
+    reqs = serving.channel_stream(5, seed=9, rate=300.0, nodes=(2.0e3, 6.0e3))
+    bad = reqs[2].case.name
+
+    def make(cs, st, dev):
+        if any(c.name == bad for c in cs):
+            raise RuntimeError("refused " + bad)
+        return _FakeBatch(cs, st, dev, [])
+
+    srv = serving.BatchServer(slots=3, chunk=5, drop=1e-4, max_steps=200, make_batch=make)
+    res = {r.rid: r for r in srv.serve(reqs)}
+    assert sorted(res) == list(range(5))
+    assert res[2].error and "refused" in res[2].error and res[2].steps == 0 and not res[2].converged
+    for r in reqs:
+        if r.rid == 2:
+            continue
+        r0, q = _FakeBatch.law(r.case)
+        got = res[r.rid]
+        assert got.error is None and got.converged
+        assert np.array_equal(got.rms, r0 * q ** np.arange(got.steps))
+    s = serving.summarize(list(res.values()))
+    assert s["refused"] == 1 and s["requests"] == 4 and s["converged"] == 4
+
+
 def test_batched_server_checks_its_arguments():
     with pytest.raises(ValueError):
         serving.BatchServer(chunk=30, max_steps=100)
