@@ -424,33 +424,40 @@ def summarize(results: Sequence[Result]) -> dict:
             "span_s": span}
 
 
-def measure(nreq: int = 16, slots: int = 16, seed: int = 2024) -> dict:
+def measure(nreq: int = 16, slots: int = 16, seed: int = 2024, sequential: bool = True) -> dict:
     """The serving measurement: the seeded stream ``channel_stream(nreq,
     seed)`` solved one request at a time (the sequential baseline) and then
     served ``slots`` at a time; with the check that every request's
-    residual history is the same in both runs."""
+    residual history is the same in both runs.  sequential=False: the
+    served run alone."""
     reqs = channel_stream(nreq, seed=seed)
     lo = min(r.case.interior_nodes() for r in reqs)
     hi = max(r.case.interior_nodes() for r in reqs)
     print("[serving] %d requests, %d-%d nodes, GPU_MAX_HW_QUEUES=%s" % (
         nreq, lo, hi, os.environ.get("GPU_MAX_HW_QUEUES", "(HIP default)")), file=sys.stderr, flush=True)
-    t0 = time.perf_counter()
-    seq = Server(slots=1).serve(reqs)
-    print("[serving] sequential %.1f s" % (time.perf_counter() - t0), file=sys.stderr, flush=True)
+    seq = None
+    if sequential:
+        t0 = time.perf_counter()
+        seq = Server(slots=1).serve(reqs)
+        print("[serving] sequential %.1f s" % (time.perf_counter() - t0), file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     srv_ = BatchServer(slots=slots)
     srv = srv_.serve(reqs)
     print("[serving] batched, %d slots: %.1f s, %d batch rebuilds (%.2f s)" % (
         slots, time.perf_counter() - t0, srv_.rebuilds, srv_.rebuild_s), file=sys.stderr, flush=True)
-    by = {r.rid: r for r in seq}
-    same = all(np.array_equal(by[r.rid].rms, r.rms) for r in srv)
-    a, b = summarize(srv), summarize(seq)
+    a = summarize(srv)
+    b = summarize(seq) if seq is not None else None
+    same = None
+    if seq is not None:
+        by = {r.rid: r for r in seq}
+        same = all(np.array_equal(by[r.rid].rms, r.rms) for r in srv)
     return {"workload": "%d steady channel solves (1 block, %d-%d nodes, M 0.5-0.85, Re 1e3-1e6), Poisson arrivals "
                         "20/s, done at a 1e-8 RMS drop or 3000 steps" % (nreq, lo, hi),
             "slots": slots, "form": "one batch context of the resident requests, rebuilt at 50-step boundaries",
             "batch_rebuilds": srv_.rebuilds, "rebuild_s": srv_.rebuild_s, "peak_resident": srv_.peak_resident,
-            "served": a, "sequential": b, "solves_per_s_vs_sequential": a["solves_per_s"] / b["solves_per_s"],
-            "same_histories_alone_and_mixed": bool(same)}
+            "served": a, "sequential": b,
+            "solves_per_s_vs_sequential": a["solves_per_s"] / b["solves_per_s"] if b else None,
+            "same_histories_alone_and_mixed": same}
 
 
 def main(argv=None) -> int:
@@ -459,9 +466,10 @@ def main(argv=None) -> int:
     ap.add_argument("--slots", type=int, default=16)
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--hw-queues", type=int, default=HW_QUEUES)
+    ap.add_argument("--batched-only", action="store_true", help="skip the sequential baseline")
     a = ap.parse_args(argv)
     prepare_process(a.hw_queues)
-    print(json.dumps(measure(a.requests, a.slots, a.seed)), flush=True)
+    print(json.dumps(measure(a.requests, a.slots, a.seed, sequential=not a.batched_only)), flush=True)
     return 0
 
 
