@@ -39,7 +39,8 @@ HBM_PEAK_GBS = 8000.0
 # kernels that implement it and the profiled phases they run in.
 STAGES = {
     "rhs": (20, ["visc_deriv", "visc_face", "visc_diff", "inv_face", "inv_closure", "inv_diff"],
-            "visc_deriv_kernel+visc_face_kernel+visc_diff_kernel+inv_face_kernel+inv_closure_kernel+inv_diff_kernel"),
+            "visc_deriv_kernel+visc_face_kernel+diff_kernel<false,true>+inv_face_kernel+inv_closure_kernel"
+            "+diff_kernel<true,false>"),
     "lu_fwd": (25, ["lu_pack", "lu_fwd"], "lu_pack_kernel+lusgs_tile_kernel<0,true>"),
     "lu_bwd": (25, ["lu_bwd"], "lusgs_tile_kernel<1,true>"),
     "gs_fwd": (30, ["gs_fwd"], "lusgs_tile_kernel<2,true>"),
