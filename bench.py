@@ -227,8 +227,13 @@ def cpu_reference(case, ranks, nsteps):
     from oracle import binding
     if not (binding.have_reference() and os.path.exists(binding.MPIEXEC)):
         return None
-    with tempfile.TemporaryDirectory() as d:
-        steps, secs, nproc = binding.time_reference(case, ranks, nsteps, d)
+    try:
+        with tempfile.TemporaryDirectory() as d:
+            steps, secs, nproc = binding.time_reference(case, ranks, nsteps, d)
+    except Exception as e:  # noqa: BLE001 -- the port then stands as the CPU baseline
+        print("[bench] reference cpu path unavailable (%s); the port is the cpu baseline" % e, file=sys.stderr,
+              flush=True)
+        return None
     print("[bench] reference cpu path: %d steps in %.1f s on %d MPI ranks" % (steps, secs, nproc), file=sys.stderr,
           flush=True)
     return {"value": case.interior_nodes() * steps / secs, "unit": "cell-steps/s", "cores": nproc, "kind": "reference",
