@@ -212,10 +212,14 @@ def serving_run(nreq, slots):
     because the hardware queue count it asks HIP for (one per resident
     request) must be set before HIP starts in a process."""
     cmd = [sys.executable, "-u", "-m", "orion_amd.serving", "--requests", str(nreq), "--slots", str(slots)]
-    p = subprocess.run(cmd, cwd=os.path.dirname(os.path.abspath(__file__)), stdout=subprocess.PIPE, timeout=900)
-    if p.returncode != 0:
-        raise RuntimeError("serving measurement failed with exit status %d" % p.returncode)
-    return json.loads(p.stdout.decode().strip().splitlines()[-1])
+    try:
+        p = subprocess.run(cmd, cwd=os.path.dirname(os.path.abspath(__file__)), stdout=subprocess.PIPE, timeout=900)
+        if p.returncode != 0:
+            raise RuntimeError("exit status %d" % p.returncode)
+        return json.loads(p.stdout.decode().strip().splitlines()[-1])
+    except Exception as e:  # noqa: BLE001 -- reported in the line; the plain measurement stands
+        print("[bench] serving measurement failed: %s" % e, file=sys.stderr, flush=True)
+        return {"error": str(e)}
 
 
 def cpu_reference(case, ranks, nsteps):
