@@ -279,12 +279,18 @@ class Batch:
         """Request k's prim, per block."""
         return [self._prim_io(k, b, None, upload=False) for b in range(self.cases[k].nblocks)]
 
-    def run(self, nsteps: int) -> np.ndarray:
-        """nsteps pseudo-steps of every request: the RMS residuals, (nsteps, requests)."""
+    def run(self, nsteps: int, stats: bool = False):
+        """nsteps pseudo-steps of every request: the RMS residuals, (nsteps,
+        requests); with ``stats``, each step's statistics per request
+        instead (a list of steps, each a list of ResidualStatsPy, max_loc's
+        block numbered within the request)."""
         from . import _lib
+        from .solver import _stats
         n = len(self.cases)
         hist = (_lib.ResidualStats * (nsteps * n))()
         _lib.check(_lib.load().orion_hip_run(self.ctx, nsteps, hist), "run")
+        if stats:
+            return [[_stats(hist[s * n + k]) for k in range(n)] for s in range(nsteps)]
         return np.array([h.rms_residual for h in hist]).reshape(nsteps, n)
 
     def close(self) -> None:

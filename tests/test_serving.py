@@ -259,3 +259,35 @@ def test_batch_refuses_per_request_calls_it_cannot_answer():
         assert (rc == 0) == same
     finally:
         b.close()
+
+
+@pytest.mark.gpu
+def test_batch_of_multiblock_requests_equals_each_alone():
+    """Three requests in one batch -- a 4-block lattice (interfaces, halo and
+    averaging maps), a channel, and a 4-block supersonic lattice at another
+    Mach number -- give each request's own statistics (residuals, maximum
+    and its location numbered within the request, time step) and fields, bit
+    for bit."""
+    from tests import _cases
+    from orion_amd.solver import OrionSolver
+    cs = [_cases.lattice_small(nsteps=6), _cases.channel_small(nsteps=6), _cases.lattice_supersonic(nsteps=6)]
+    print("\n[gpu] serving: batch of multi-block requests", flush=True)
+    alone = []
+    for c in cs:
+        s = OrionSolver(c)
+        h = s.solve(6)
+        alone.append((h, [s.fs.field(b, "prim") for b in range(c.nblocks)]))
+        s.fs.close()
+    b = serving.Batch(cs)
+    try:
+        st = b.run(6, stats=True)
+        for k, c in enumerate(cs):
+            h, prims = alone[k]
+            for n in range(6):
+                x, y = st[n][k], h[n]
+                assert (x.rms_residual, x.max_residual, x.max_loc, x.phydtime) == \
+                    (y.rms_residual, y.max_residual, y.max_loc, y.phydtime), (k, n)
+            for blk, p in enumerate(b.prim(k)):
+                assert np.array_equal(p, prims[blk]), (k, blk)
+    finally:
+        b.close()
