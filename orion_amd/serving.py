@@ -45,7 +45,9 @@ from . import cases
 
 # HBM held by one context per padded point: the arena (fields, scratch,
 # static face geometry, the inviscid flux array) plus the LU-SGS sweep
-# arrays, about 1.7 KB on the 64-block workload; admission counts 2 KB
+# arrays.  Measured (orion_hip_device_bytes) on the channel requests of
+# channel_stream: 1.33-1.46 KB, and 1.39 KB for a batch of sixteen;
+# admission counts 2 KB
 BYTES_PER_PADDED_POINT = 2048
 
 # hardware queues asked for by a serving process: one per resident request
