@@ -4,28 +4,28 @@ A request is one steady solve (a case) that arrives at a given time and is
 done when its RMS residual has dropped by ``drop`` relative to its first
 step (the reference's monitor, ResidualMonitor::compute,
 src/solver/ResidualMonitor.cpp:11-138), or after ``max_steps`` steps.
+Requests are admitted in arrival order as soon as one has arrived, a slot
+is free and its HBM footprint fits the budget beside the requests already
+resident.  A request's result does not depend on what else is resident.
 
-The server keeps up to ``slots`` requests on the device at once.  Requests
-are admitted in arrival order as soon as one has arrived, a slot is free and
-its HBM footprint fits the budget beside the requests already resident.
-Each admitted request owns a device context (its fields, scratch and
-streams, created on admission and freed on completion) and is advanced by a
-host worker thread in chunks of ``chunk`` pseudo-steps, replayed from the
-context's captured step; the workers' chunks run concurrently on the GPU,
-so small solves that each fill a fraction of the 256 CUs overlap.  A
-request's work and result do not depend on what else is resident: its
-context is its own and every kernel is deterministic.
+``BatchServer`` (the served form): the resident requests share one batch
+context (``Batch``, orion_hip_batch_create), so every kernel launch of a
+step covers all of them.  The batch advances ``chunk`` pseudo-steps at a
+time; at each chunk boundary finished requests leave, arrived ones join,
+and the batch is rebuilt with the staying requests' state copied over on
+the device.  Each request's arithmetic and reduction order are those of a
+context of its own, so its results are too, bit for bit.
 
-``Server(slots=1)`` is the sequential baseline: the same requests solved
-one after another in arrival order.
-
-Concurrency on the device: each context's step is one stream of kernels
-(orion_hip_set_overlap off: the single-context side stream would double
-the streams), and HIP maps the streams of a process onto GPU_MAX_HW_QUEUES
-hardware queues (4 by default); kernels of streams that share a queue run
-one after another.  A serving process therefore sets GPU_MAX_HW_QUEUES to
-HW_QUEUES before HIP starts (``prepare_process``, or ``python -m
-orion_amd.serving``), so that every resident request has a queue of its own.
+``Server``: one context per resident request, each advanced by a host
+thread; their steps overlap on the GPU.  It is the simpler form and,
+with one slot, the sequential baseline: the same requests solved one after
+another in arrival order.  With several slots each context's step is one
+stream of kernels (orion_hip_set_overlap off), and HIP maps a process's
+streams onto GPU_MAX_HW_QUEUES hardware queues (4 by default), kernels of
+streams that share a queue running one after another; a process using it
+sets GPU_MAX_HW_QUEUES to HW_QUEUES before HIP starts (``prepare_process``,
+as ``python -m orion_amd.serving`` does).  Many small contexts share the
+device's launch throughput, which is why the batch is the served form.
 """
 from __future__ import annotations
 
