@@ -1244,26 +1244,32 @@ int orion_hip_residual(orion_hip_ctx* c, orion_residual_stats* out) {
 static void step_body(orion_hip_ctx* c) {
   op_halo(c);
   op_bc(c);
-  // mode 4: the first right-hand-side kernel below writes dq whole
-  launch_state(LA(c, c->pad), c->pad.total(), 3 | 4, c->d_dtmin, c->d_step, c->nreq);
+  // mode 4 of the state kernel: the first right-hand-side kernel after it
+  // writes dq whole
   if (c->p.nvis == 1 && !c->overlap) {
+    launch_state(LA(c, c->pad), c->pad.total(), 3 | 4, c->d_dtmin, c->d_step, c->nreq);
     op_visc(c, true);
     op_avg(c);
     op_inv(c, false);
   } else if (c->p.nvis == 1) {
     // The inviscid face fluxes (FP64-bound) run on the side stream beside
-    // the viscous chain (load-bound); they have their own flux array and
-    // join before the inviscid difference, the first kernel after them
-    // that touches dq.  dq is thus updated in the operators' order.
+    // the state kernel and the viscous chain (load-bound).  They read prim's
+    // first five fields and the static face geometry, which neither writes
+    // (the state kernel writes T, c, mu, q and dt), write their own flux
+    // arrays, and join before the inviscid difference, the first kernel
+    // after them that touches dq.  dq is thus updated in the operators'
+    // order.
     hipEventRecord(c->ev_fork, c->stream);
     hipStreamWaitEvent(c->side_stream, c->ev_fork, 0);
     op_inv_faces(c, c->side_stream);
     hipEventRecord(c->ev_join, c->side_stream);
+    launch_state(LA(c, c->pad), c->pad.total(), 3 | 4, c->d_dtmin, c->d_step, c->nreq);
     op_visc(c, true);
     op_avg(c);
     hipStreamWaitEvent(c->stream, c->ev_join, 0);
     launch_inv_diff(LA(c, c->dtile), c->dtile.total(), false);
   } else {
+    launch_state(LA(c, c->pad), c->pad.total(), 3 | 4, c->d_dtmin, c->d_step, c->nreq);
     op_inv(c, true);
   }
   op_avg(c);
