@@ -120,3 +120,22 @@ def test_bench_stage_kernels_are_in_the_traffic_summary():
     assert sorted(t) == sorted(groups), sorted(set(groups) - set(t))
     for name, v in t.items():
         assert v["bytes_per_launch"] > 0, name
+
+
+def test_new_entry_points_refuse_a_null_context():
+    """The entry points without a reference counterpart check their context
+    before any device work, so a null one fails with a message (no GPU)."""
+    import ctypes as C
+    from orion_amd import _lib
+    L = _lib.load()
+    ms = (C.c_double * 2)()
+    chosen = C.c_int()
+    for name, call in [
+        ("set_schedule", lambda: L.orion_hip_set_schedule(None, 0)),
+        ("tune_schedule", lambda: L.orion_hip_tune_schedule(None, 3, C.byref(chosen), ms)),
+        ("set_overlap", lambda: L.orion_hip_set_overlap(None, 1)),
+        ("copy_block_state", lambda: L.orion_hip_copy_block_state(None, 0, None, 0)),
+    ]:
+        assert call() != 0, name
+        assert L.orion_hip_last_error().decode(), name
+    assert L.orion_hip_get_schedule(None) == -1 and L.orion_hip_requests(None) == 0
