@@ -62,6 +62,8 @@ def make_case(name, nsteps):
     from orion_amd import cases
     if name == "uneven64":
         return cases.uneven_lattice_case(seed=11, nsteps=nsteps)
+    if name == "columns64":
+        return cases.column_case(seed=3, nsteps=nsteps)
     if name == "eight_block":
         return cases.eight_block_case(seed=7, nsteps=nsteps)
     if name == "channel":
@@ -222,28 +224,61 @@ def serving_run(nreq, slots):
         return {"error": str(e)}
 
 
-def cpu_reference(case, ranks, nsteps):
+def host_info():
+    """Host cores as the bench sees them and the CPU model."""
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    return {"nproc": avail, "cpu_count": os.cpu_count(), "cpu_model": model}
+
+
+def binding_max_ranks(case, most):
+    from oracle import binding
+    return binding.max_ranks(case, most)
+
+
+def cpu_reference(case, ranks, nsteps, gpu_hist=None):
     """The reference's MPI CPU path (its own solver built from its sources,
-    oracle/_ref) on ``ranks`` processes of this host: ``nsteps`` whole
-    pseudo-steps of the same case, set-up excluded.  None without the
-    build."""
+    oracle/_ref) on ``ranks`` processes of this host, the blocks spread
+    over them: ``nsteps`` whole pseudo-steps of the same case from the same
+    seeded start state, set-up excluded.  With ``gpu_hist`` (the GPU's
+    residuals of its first steps from that state) the two histories are
+    compared.  None without the build."""
     import tempfile
     from oracle import binding
     if not (binding.have_reference() and os.path.exists(binding.MPIEXEC)):
         return None
     try:
         with tempfile.TemporaryDirectory() as d:
-            steps, secs, nproc = binding.time_reference(case, ranks, nsteps, d)
+            steps, secs, nproc, hist = binding.time_reference(case, ranks, nsteps, d)
     except Exception as e:  # noqa: BLE001 -- the port then stands as the CPU baseline
         print("[bench] reference cpu path unavailable (%s); the port is the cpu baseline" % e, file=sys.stderr,
               flush=True)
         return None
     print("[bench] reference cpu path: %d steps in %.1f s on %d MPI ranks" % (steps, secs, nproc), file=sys.stderr,
           flush=True)
-    return {"value": case.interior_nodes() * steps / secs, "unit": "cell-steps/s", "cores": nproc, "kind": "reference",
-            "sample": "%d pseudo-steps of the same %d-block, %d-node case (from the freestream) by the reference "
-                      "solver built from its own sources, %d MPI ranks, step loop timed between barriers (%.1f s)" % (
-                          steps, case.nblocks, case.interior_nodes(), nproc, secs)}
+    out = {"value": case.interior_nodes() * steps / secs, "unit": "cell-steps/s", "cores": nproc, "kind": "reference",
+           "sample": "%d pseudo-steps of the same %d-block, %d-node case from the same seeded start state, by the "
+                     "reference solver built from its own sources on %d MPI ranks (blocks spread over them, "
+                     "largest block %.1f%% of the nodes), step loop timed between barriers (%.1f s)" % (
+                         steps, case.nblocks, case.interior_nodes(), nproc,
+                         100.0 * float(max(case.dims.prod(axis=1))) / case.interior_nodes(), secs)}
+    out.update(host_info())
+    if gpu_hist:
+        n = min(len(gpu_hist), steps)
+        rel = [abs(gpu_hist[i] / hist[i, 0] - 1.0) for i in range(n)]
+        out["history_check"] = {"steps": n, "reference_rms": [float(v) for v in hist[:n, 0]],
+                                "gpu_rms": [float(v) for v in gpu_hist[:n]], "max_rel_diff": max(rel) if rel else None}
+    return out
 
 
 def cpu_baseline(case, threads, seconds):
@@ -259,9 +294,11 @@ def cpu_baseline(case, threads, seconds):
         if dt >= seconds:
             break
     print("[bench] cpu baseline: %d steps in %.1f s on %d threads" % (n, dt, threads), file=sys.stderr, flush=True)
-    return {"value": case.interior_nodes() * n / dt, "unit": "cell-steps/s", "cores": threads, "kind": "port",
-            "sample": "%d full pseudo-steps of the same %d-block, %d-node case on %d OpenMP threads (%.1f s)" % (
-                n, case.nblocks, case.interior_nodes(), threads, dt)}
+    out = {"value": case.interior_nodes() * n / dt, "unit": "cell-steps/s", "cores": threads, "kind": "port",
+           "sample": "%d full pseudo-steps of the same %d-block, %d-node case on %d OpenMP threads (%.1f s)" % (
+               n, case.nblocks, case.interior_nodes(), threads, dt)}
+    out.update(host_info())
+    return out
 
 
 def main():
@@ -269,6 +306,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--runs", type=int, default=5, help="timed runs of --steps steps each; value is their median")
     ap.add_argument("--tune-steps", type=int, default=3,
                     help="steps per trial of the sweep-schedule autotuner after the warmup (0: no tuning)")
     ap.add_argument("--schedule", type=int, default=-1,
@@ -308,8 +346,9 @@ def main():
     t_setup = time.perf_counter() - t_setup
     print("[bench] %s: %d blocks, %d interior nodes, setup %.1f s" % (case.name, case.nblocks, N, t_setup),
           file=sys.stderr, flush=True)
+    warm_hist = []
     if a.warmup > 0:
-        s.solve(a.warmup)
+        warm_hist = [h.rms_residual for h in s.solve(a.warmup)]
     tuning = None
     if a.schedule >= 0:
         s.fs.set_schedule(a.schedule)
@@ -320,14 +359,23 @@ def main():
                   "ms_per_step": {n: round(v, 4) for n, v in zip(s.fs.SCHEDULES, ms)}}
         print("[bench] sweep schedule tuning: %s -> %s" % (json.dumps(tuning["ms_per_step"]), s.fs.SCHEDULES[chosen]),
               file=sys.stderr, flush=True)
-    s.fs.synchronize()
-    t0 = time.perf_counter()
-    hist = s.solve(a.steps)
-    s.fs.synchronize()
-    t = time.perf_counter() - t0
+    # a.runs timed runs of exactly a.steps steps, each from the same state
+    # (prim alone fixes the next step); value is their median
+    start = [s.fs.field(b, "prim") for b in range(case.nblocks)] if a.runs > 1 else None
+    run_t = []
+    for r in range(a.runs):
+        if r > 0:
+            for b in range(case.nblocks):
+                s.fs.set_field(b, "prim", start[b])
+        s.fs.synchronize()
+        t0 = time.perf_counter()
+        hist = s.solve(a.steps)
+        s.fs.synchronize()
+        run_t.append(time.perf_counter() - t0)
+        print("[bench] run %d: %d steps in %.4f s, rms %.6e -> %.6e" % (
+            r, a.steps, run_t[-1], hist[0].rms_residual, hist[-1].rms_residual), file=sys.stderr, flush=True)
+    t = sorted(run_t)[len(run_t) // 2]
     value = N * a.steps / t
-    print("[bench] %d steps in %.3f s, rms %.6e -> %.6e" % (a.steps, t, hist[0].rms_residual, hist[-1].rms_residual),
-          file=sys.stderr, flush=True)
     phase = profile_phases(s.fs, a.profile_steps)
     phases = {k: round(v[0], 4) for k, v in phase.items()}
     print("[bench] phase ms/step: " + json.dumps(phases), file=sys.stderr, flush=True)
@@ -362,6 +410,8 @@ def main():
         "step_roofline": step_roof,
         "phase_ms_per_step": phases,
         "residual_first_last": [hist[0].rms_residual, hist[-1].rms_residual],
+        "runs_ms_per_step": [round(1e3 * v / a.steps, 4) for v in run_t],
+        "timing": "median of %d timed runs of %d steps, each from the same state" % (a.runs, a.steps),
     }
     s.fs.close()
     if not a.no_serve:
@@ -369,7 +419,7 @@ def main():
     if not a.no_cpu:
         threads = a.cpu_threads or min(16, os.cpu_count() or 1)
         port = cpu_baseline(case, threads, a.cpu_seconds)
-        ref = cpu_reference(case, threads, a.ref_steps) if a.ref_steps > 0 else None
+        ref = cpu_reference(case, binding_max_ranks(case, threads), a.ref_steps, warm_hist) if a.ref_steps > 0 else None
         out["cpu_baseline"] = ref or port
         if ref:
             out["cpu_port"] = port

@@ -76,6 +76,11 @@ def lib():
         L.ob_lusgs_sweeps.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double)]
         L.ob_output_fields.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_float)]
         L.ob_output_fields.restype = None
+        L.ob_guard_counts.argtypes = [C.POINTER(C.c_long)]
+        L.ob_guard_counts.restype = None
+        L.ob_guard_reset.restype = None
+        L.ob_set_accurate_sum.argtypes = [C.c_int]
+        L.ob_set_accurate_sum.restype = None
         _lib = L
     return _lib
 
@@ -84,9 +89,14 @@ class OracleCase:
     """One case held by the C oracle; fields are exposed as numpy views in
     OrionArray layout: (nvar, nz, ny, nx), i fastest."""
 
-    def __init__(self, case, threads: int = 1, apply_init: bool = True):
+    def __init__(self, case, threads: int = 1, apply_init: bool = True, accurate_sum: bool = False):
+        """accurate_sum: form the RMS residual with a compensated sum (about
+        the exact sum) rather than the reference's loop-order sum, whose
+        rounding error reaches 1e-12 relative at millions of nodes; the
+        setting is process-wide and lasts until the next OracleCase."""
         L = lib()
         L.ob_set_threads(int(threads))
+        L.ob_set_accurate_sum(1 if accurate_sum else 0)
         self.case = case
         dims = np.ascontiguousarray(case.dims, dtype=np.int32).reshape(-1)
         xyz = np.ascontiguousarray(case.xyz_flat())
@@ -152,6 +162,17 @@ class OracleCase:
         lib().ob_lusgs_sweeps(self._h, b, rhs.ctypes.data_as(C.POINTER(C.c_double)))
 
 
+def guard_counts(reset: bool = False):
+    """Times the reference's guard branches were taken in the oracle since
+    the last reset: (left fallback, right fallback, damped update, kept
+    state)."""
+    out = (C.c_long * 4)()
+    lib().ob_guard_counts(out)
+    if reset:
+        lib().ob_guard_reset()
+    return tuple(out)
+
+
 # --------------------------------------------------------------------------
 # reference harness (oracle/_ref/ref_harness, this container only)
 # --------------------------------------------------------------------------
@@ -160,51 +181,94 @@ def have_reference() -> bool:
     return os.path.exists(REF_HARNESS)
 
 
+# ref_harness history columns: rms, max, max_loc[5], phydtime, compensated rms
+HIST_COLS = 9
 MPIEXEC = os.environ.get("ORION_MPIEXEC", "/opt/conda/bin/mpiexec")  # the MPI the reference build links
+
+
+def max_ranks(case, most: int) -> int:
+    """The most MPI ranks (at most ``most``) the reference can run ``case``
+    on: a placement of its blocks must exist in which no block sends two
+    interface messages to one other rank (Case.block_pids)."""
+    for r in range(max(1, min(int(most), case.nblocks)), 0, -1):
+        try:
+            case.block_pids(r)
+            return r
+        except ValueError:
+            continue
+    return 1
+
+
+def _harness_cmd(ranks: int):
+    ranks = max(1, int(ranks))
+    if ranks == 1:
+        return [REF_HARNESS]
+    return [MPIEXEC, "-launcher", "fork", "-n", str(ranks), REF_HARNESS]
 
 
 def time_reference(case, ranks: int, nsteps: int, workdir: str):
     """The reference solver's own CPU path (oracle/_ref/ref_harness in its
-    timing mode) on ``ranks`` MPI processes of this host: ``nsteps``
-    pseudo-steps of ``case`` from the freestream, timed between barriers
-    around the step loop (set-up excluded).  Returns (steps, seconds,
-    ranks)."""
+    timing mode) on ``ranks`` MPI processes of this host, the blocks spread
+    over the ranks (Case.block_pids): ``nsteps`` pseudo-steps of ``case``
+    from its seeded start state, timed between barriers around the step
+    loop (set-up excluded).  Returns (steps, seconds, ranks, hist[n, 9])."""
     casedir, outdir = os.path.join(workdir, "case"), os.path.join(workdir, "out")
-    case.write(casedir)
+    case.write(casedir, nprocs=ranks)
     os.makedirs(outdir, exist_ok=True)
     env = dict(os.environ)
     env.pop("LD_LIBRARY_PATH", None)
-    subprocess.run([MPIEXEC, "-launcher", "fork", "-n", str(int(ranks)), REF_HARNESS, casedir, str(int(nsteps)), outdir,
-                    "0", "time"], check=True, stdout=subprocess.DEVNULL, env=env, timeout=900)
+    subprocess.run([MPIEXEC, "-launcher", "fork", "-n", str(max(1, int(ranks))), REF_HARNESS, casedir,
+                    str(int(nsteps)), outdir, "0", "time"],
+                   check=True, stdout=subprocess.DEVNULL, env=env, timeout=1800)
     with open(os.path.join(outdir, "time.txt")) as f:
         steps, secs, nproc = f.read().split()
-    return int(steps), float(secs), int(nproc)
+    hist = np.fromfile(os.path.join(outdir, "hist.bin")).reshape(int(steps), HIST_COLS)
+    return int(steps), float(secs), int(nproc), hist
 
 
-def run_reference(case, casedir: str, outdir: str, nsteps: int, dump_steps: int = 0, vts: bool = False):
+def run_reference(case, casedir: str, outdir: str, nsteps: int, dump_steps: int = 0, vts: bool = False,
+                  ranks: int = 1):
     """Write ``case`` in the reference's formats and run the reference
-    solver's operators for ``nsteps``.  Returns (hist[n,8], reader) where
-    reader(stage_file) -> list of per-block dicts of fields.  With ``vts``
-    the reference's PostProcess::write_solution also runs after the last
-    step, writing into the case's flowname directory under casedir."""
-    case.write(casedir)
+    solver's operators for ``nsteps`` (on ``ranks`` MPI processes, the
+    blocks spread over them).  Returns (hist[n,9], reader) where
+    reader(stage_file) -> list of per-block dicts of fields ("final.bin"
+    also reads the per-block files of a run on several ranks).  With
+    ``vts`` the reference's PostProcess::write_solution also runs after the
+    last step, writing into the case's flowname directory under casedir.
+    Stage dumps and ``vts`` need one rank."""
+    ranks = max(1, int(ranks))
+    if ranks > 1 and (dump_steps or vts):
+        raise ValueError("stage dumps and VTS output need a single-rank run")
+    case.write(casedir, nprocs=ranks)
     os.makedirs(outdir, exist_ok=True)
     env = dict(os.environ)
     env.pop("LD_LIBRARY_PATH", None)
-    subprocess.run([REF_HARNESS, casedir, str(nsteps), outdir, str(dump_steps)] + (["vts"] if vts else []), check=True,
-                   stdout=subprocess.DEVNULL, env=env)
-    hist = np.fromfile(os.path.join(outdir, "hist.bin")).reshape(nsteps, 8)
+    subprocess.run(_harness_cmd(ranks) + [casedir, str(nsteps), outdir, str(dump_steps)] + (["vts"] if vts else []),
+                   check=True, stdout=subprocess.DEVNULL, env=env)
+    hist = np.fromfile(os.path.join(outdir, "hist.bin")).reshape(nsteps, HIST_COLS)
+
+    def block(raw, pos, b):
+        nx, ny, nz = case.padded(b)
+        P = nx * ny * nz
+        blk = {}
+        for key, nv in (("prim", 6), ("dq", 5), ("c", 1), ("mu", 1), ("dt", 1)):
+            blk[key] = raw[pos:pos + nv * P].reshape(nv, nz, ny, nx)
+            pos += nv * P
+        return blk, pos
 
     def read(name: str):
+        if ranks > 1 and name == "final.bin":
+            out = []
+            for b in range(case.nblocks):
+                raw = np.fromfile(os.path.join(outdir, "final_b%d.bin" % b))
+                blk, pos = block(raw, 0, b)
+                assert pos == raw.size
+                out.append(blk)
+            return out
         raw = np.fromfile(os.path.join(outdir, name))
         out, pos = [], 0
         for b in range(case.nblocks):
-            nx, ny, nz = case.padded(b)
-            P = nx * ny * nz
-            blk = {}
-            for key, nv in (("prim", 6), ("dq", 5), ("c", 1), ("mu", 1), ("dt", 1)):
-                blk[key] = raw[pos:pos + nv * P].reshape(nv, nz, ny, nx)
-                pos += nv * P
+            blk, pos = block(raw, pos, b)
             out.append(blk)
         assert pos == raw.size
         return out

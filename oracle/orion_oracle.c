@@ -64,6 +64,20 @@ struct ob_case {
 
 static __thread char g_err[512];
 const char* ob_last_error(void) { return g_err; }
+
+/* How often the reference's guard branches were taken (test
+ * instrumentation; no effect on results): 0 inviscid left-state fallback,
+ * 1 right-state fallback (InviscidFluxComputer.cpp:346-355), 2 update
+ * damped by the pressure-change limiter, 3 update refused and the old
+ * state kept (StateUpdater.cpp:502-538). */
+static long g_guard[4];
+static void guard_hit(int which) { __atomic_fetch_add(&g_guard[which], 1, __ATOMIC_RELAXED); }
+void ob_guard_counts(long* out) {
+  for (int i = 0; i < 4; ++i) out[i] = __atomic_load_n(&g_guard[i], __ATOMIC_RELAXED);
+}
+void ob_guard_reset(void) {
+  for (int i = 0; i < 4; ++i) __atomic_store_n(&g_guard[i], 0, __ATOMIC_RELAXED);
+}
 static int fail(const char* fmt, ...) {
   va_list ap;
   va_start(ap, fmt);
@@ -1595,10 +1609,12 @@ static void flux_line(int ni, const double* q, const double* mh, double* fc, dou
     if (a_[0] <= 1e-6 || a_[4] <= 1e-6) {                                     \
       int ix = (i - 1 > 0 ? i - 1 : 0) + 2;                                   \
       for (int m = 0; m < nl; ++m) a_[m] = q[ix * 6 + m];                     \
+      guard_hit(0);                                                           \
     }                                                                         \
     if (b_[0] <= 1e-6 || b_[4] <= 1e-6) {                                     \
       int ix = (ni - 1 < i ? ni - 1 : i) + 2 + 1;                             \
       for (int m = 0; m < nl; ++m) b_[m] = q[ix * 6 + m];                     \
+      guard_hit(1);                                                           \
     }                                                                         \
     roe(a_, b_, mh[i * 5 + 0], mh[i * 5 + 1], mh[i * 5 + 2], mh[i * 5 + 3], efix, gamma, o_); \
     for (int m = 0; m < nl; ++m) f[i * nl + m] = o_[m];                       \
@@ -1944,6 +1960,7 @@ static void update_limited(ob_block* k, const ob_params* p) {
         double dpp = fabs((pn - pp) / pp);
         double term = dmax(0.0, -0.3 + dpp);
         double Fa = 1.0 / (1.0 + 2.0 * term);
+        if (term > 0.0) guard_hit(2);
         double qf[5] = {r + d0 * Fa, r * u + d1 * Fa, r * v + d2 * Fa, r * w + d3 * Fa, E + d4 * Fa};
         rn = qf[0];
         ir = 1.0 / rn;
@@ -1951,6 +1968,7 @@ static void update_limited(ob_block* k, const ob_params* p) {
         double pf = gm1 * (qf[4] - 0.5 * rn * (un * un + vn * vn + wn * wn));
         if (rn <= r_min || pf <= p_min) {
           /* keep the old state */
+          guard_hit(3);
         } else {
           k->prim[id] = rn;
           k->prim[P + id] = un;
@@ -1988,10 +2006,20 @@ void ob_update_solution(ob_case* c) {
  * Residual (src/solver/ResidualMonitor.cpp:11-138)
  * ======================================================================== */
 
+/* Test option (not the reference's arithmetic): form the RMS residual's
+ * sum of squares with Neumaier's compensated summation instead of the
+ * reference's loop-order sum.  At millions of nodes the loop-order sum
+ * carries a rounding error of order 1e-12 relative, larger than the parity
+ * bar; the compensated sum is the exact sum to about 1e-16, which a
+ * tree-ordered device sum approaches too.  The maximum and its location
+ * are unaffected. */
+static int g_accurate_sum = 0;
+void ob_set_accurate_sum(int on) { g_accurate_sum = on; }
+
 ob_residual ob_residual_compute(const ob_case* c) {
   ob_residual s;
   memset(&s, 0, sizeof s);
-  double sum = 0.0;
+  double sum = 0.0, comp = 0.0;
   long long tot = 0;
   double mx = -1.0;
   int loc[5] = {-1, -1, -1, -1, -1};
@@ -2007,7 +2035,13 @@ ob_residual ob_residual_compute(const ob_case* c) {
           double inv = 1.0 / (dt * vol);
           for (int m = 0; m < 5; ++m) {
             double r = fabs(k->dq[m * k->P + id]) * inv;
-            sum += r * r;
+            if (g_accurate_sum) {
+              const double x = r * r, t = sum + x;
+              comp += fabs(sum) >= fabs(x) ? (sum - t) + x : (x - t) + sum;
+              sum = t;
+            } else {
+              sum += r * r;
+            }
             if (r > mx) {
               mx = r;
               loc[0] = b;
@@ -2022,6 +2056,7 @@ ob_residual ob_residual_compute(const ob_case* c) {
   }
   s.max = mx;
   memcpy(s.max_loc, loc, sizeof loc);
+  if (g_accurate_sum) sum += comp;
   if (tot > 0) s.rms = sqrt(sum / tot);
   return s;
 }

@@ -131,6 +131,18 @@ void ob_lusgs_sweeps(ob_case* c, int b, const double* rhs_soa);
  * Velocity as (u, v, w) per node -- 7 * idim*jdim*kdim values in out. */
 void ob_output_fields(const ob_case* c, int b, float* out);
 
+/* Test instrumentation: how often the reference's guard branches were taken
+ * since the last reset -- out[0] inviscid left-state and out[1] right-state
+ * low-order fallbacks (src/solver/InviscidFluxComputer.cpp:346-355),
+ * out[2] updates damped by the pressure-change limiter and out[3] updates
+ * refused (old state kept) (src/solver/StateUpdater.cpp:502-538). */
+void ob_guard_counts(long* out);
+/* Test option: the RMS residual's sum of squares by compensated summation
+ * (about the exact sum) instead of the reference's loop order, whose
+ * rounding error reaches 1e-12 relative at millions of nodes. */
+void ob_set_accurate_sum(int on);
+void ob_guard_reset(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -10,11 +10,19 @@
 // the reference has no way to express a perturbed initial field
 // (src/preprocess/InitialCondition.cpp:7-86).
 //
-// usage: ref_harness <casedir> <nsteps> <outdir> [dump_steps] [vts]
-//   outdir/hist.bin   nsteps x 8 doubles: rms, max, max_loc[5], phydtime
+// usage: ref_harness <casedir> <nsteps> <outdir> [dump_steps] [vts|time]
+// (under mpiexec -n R for R ranks; the BC file's pids place the blocks)
+//   outdir/hist.bin   nsteps x 9 doubles: rms, max, max_loc[5], phydtime,
+//                     and the RMS again from a compensated sum of the same
+//                     terms (the harness's own addition: the reference's
+//                     loop-order sum is off the exact sum by up to ~1e-12
+//                     relative at millions of nodes) (rank 0; reduced over
+//                     all ranks)
 //   outdir/s<step>_<stage>.bin   for step <= dump_steps, every block's
 //                     [prim 6P][dq 5P][c P][mu P][dt P] after each stage
-//   outdir/final.bin  same layout after the last step
+//                     (one rank only)
+//   outdir/final.bin  same layout after the last step (one rank), or
+//   outdir/final_b<n>.bin  block n's fields, written by its owner (R > 1)
 //   with "vts": PostProcess::write_solution (src/postprocess/PostProcess.cpp:44-123)
 //                     after the last step, as OrionSolver::solve ends
 //                     (src/core/OrionSolver.cpp:225), into the case's
@@ -42,12 +50,42 @@
 #include <mpi.h>
 #include <unistd.h>
 
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <string>
 #include <vector>
 
 using namespace orion;
+
+// ResidualMonitor::compute's RMS (src/solver/ResidualMonitor.cpp:38-102)
+// with Neumaier's compensated summation of the same squared terms.
+static double rms_compensated(preprocess::FlowFieldSet& fs) {
+  double sum = 0.0, comp = 0.0, cnt = 0.0;
+  for (int nb : fs.local_block_ids) {
+    auto& bf = fs.blocks[nb];
+    const int ng = fs.ng;
+    const int nx = (int)bf.prim.dims()[0], ny = (int)bf.prim.dims()[1], nz = (int)bf.prim.dims()[2];
+    for (int k = ng; k < nz - ng; ++k)
+      for (int j = ng; j < ny - ng; ++j)
+        for (int i = ng; i < nx - ng; ++i) {
+          double vol = bf.vol(i, j, k), dt = bf.dt(i, j, k);
+          if (vol < 1e-30) vol = 1e-30;
+          if (dt < 1e-30) dt = 1e-30;
+          const double inv = 1.0 / (dt * vol);
+          for (int m = 0; m < 5; ++m) {
+            const double r = std::fabs(bf.dq(i, j, k, m)) * inv;
+            const double x = r * r, t = sum + x;
+            comp += std::fabs(sum) >= std::fabs(x) ? (sum - t) + x : (x - t) + sum;
+            sum = t;
+          }
+          cnt += 5.0;
+        }
+  }
+  double loc[3] = {sum, comp, cnt}, tot[3];
+  MPI_Allreduce(loc, tot, 3, MPI_DOUBLE, MPI_SUM, MPI_COMM_WORLD);
+  return tot[2] > 0.0 ? std::sqrt((tot[0] + tot[1]) / tot[2]) : 0.0;
+}
 
 static void dump_state(const std::string& path, preprocess::FlowFieldSet& fs) {
   FILE* f = std::fopen(path.c_str(), "wb");
@@ -73,9 +111,9 @@ int main(int argc, char** argv) {
   std::string outdir = argv[3];
   const int dump_steps = argc > 4 ? std::atoi(argv[4]) : 0;
   const bool vts = argc > 5 && std::string(argv[5]) == "vts";
-  // time: the CPU baseline, on any number of MPI ranks -- the freestream
-  // start, no dumps; rank 0 writes outdir/time.txt (steps, seconds of the
-  // step loop between barriers, ranks) and nothing else is written
+  // time: the CPU baseline, on any number of MPI ranks -- the same start
+  // state (init.bin), no dumps; rank 0 writes outdir/time.txt (steps,
+  // seconds of the step loop between barriers, ranks) and hist.bin
   const bool timing = argc > 5 && std::string(argv[5]) == "time";
   if (outdir[0] != '/') {
     char cwd[4096];
@@ -103,10 +141,18 @@ int main(int argc, char** argv) {
   preprocess::initialize_flow(params, fs);
   preprocess::update_temperature(params, fs);
 
-  if (FILE* f = timing ? nullptr : std::fopen("init.bin", "rb")) {
+  if (FILE* f = std::fopen("init.bin", "rb")) {
+    // init.bin holds every block in order; each rank reads its own blocks
+    std::vector<long> first(grid.nblocks + 1, 0);
+    for (int nb = 0; nb < grid.nblocks; ++nb) {
+      const auto& g = grid.blocks[nb];
+      first[nb + 1] = first[nb] + 6L * (g.idim + 2 * fs.ng) * (g.jdim + 2 * fs.ng) * (g.kdim + 2 * fs.ng);
+    }
     for (int nb : fs.local_block_ids) {
       auto& bf = fs.blocks[nb];
-      if (std::fread(bf.prim.hostPtr(), sizeof(double), bf.prim.size(), f) != bf.prim.size()) {
+      if (std::fseek(f, first[nb] * (long)sizeof(double), SEEK_SET) != 0 ||
+          (long)bf.prim.size() != first[nb + 1] - first[nb] ||
+          std::fread(bf.prim.hostPtr(), sizeof(double), bf.prim.size(), f) != bf.prim.size()) {
         std::fprintf(stderr, "init.bin too short\n");
         return 2;
       }
@@ -149,9 +195,17 @@ int main(int argc, char** argv) {
     hist.push_back(st.max_residual);
     for (int m = 0; m < 5; ++m) hist.push_back((double)st.max_loc[m]);
     hist.push_back(params.step.phydtime);
+    hist.push_back(timing ? 0.0 : rms_compensated(fs));
   }
   MPI_Barrier(MPI_COMM_WORLD);
   const double t1 = MPI_Wtime();
+  if (core::Runtime::myid == 0) {
+    std::string p = outdir + "/hist.bin";
+    FILE* f = std::fopen(p.c_str(), "wb");
+    if (!f) { std::perror(p.c_str()); return 2; }
+    std::fwrite(hist.data(), sizeof(double), hist.size(), f);
+    std::fclose(f);
+  }
   if (timing) {
     if (core::Runtime::myid == 0) {
       std::string p = outdir + "/time.txt";
@@ -163,13 +217,22 @@ int main(int argc, char** argv) {
     MPI_Finalize();
     return 0;
   }
-  {
-    std::string p = outdir + "/hist.bin";
-    FILE* f = std::fopen(p.c_str(), "wb");
-    std::fwrite(hist.data(), sizeof(double), hist.size(), f);
-    std::fclose(f);
+  if (core::Runtime::nproc == 1) {
+    dump_state(outdir + "/final.bin", fs);
+  } else {
+    for (int nb : fs.local_block_ids) {
+      std::string p = outdir + "/final_b" + std::to_string(nb) + ".bin";
+      FILE* f = std::fopen(p.c_str(), "wb");
+      if (!f) { std::perror(p.c_str()); return 2; }
+      auto& bf = fs.blocks[nb];
+      std::fwrite(bf.prim.hostPtr(), sizeof(double), bf.prim.size(), f);
+      std::fwrite(bf.dq.hostPtr(), sizeof(double), bf.dq.size(), f);
+      std::fwrite(bf.c.hostPtr(), sizeof(double), bf.c.size(), f);
+      std::fwrite(bf.mu.hostPtr(), sizeof(double), bf.mu.size(), f);
+      std::fwrite(bf.dt.hostPtr(), sizeof(double), bf.dt.size(), f);
+      std::fclose(f);
+    }
   }
-  dump_state(outdir + "/final.bin", fs);
   if (vts) postprocess::PostProcess::write_solution(fs, grid, params, nsteps);
   MPI_Finalize();
   return 0;

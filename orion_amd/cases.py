@@ -67,6 +67,7 @@ class Case:
     inputs: Dict[str, float]
     nsteps: int = 1
     init_rho: Optional[List[np.ndarray]] = None  # per block (kdim, jdim, idim) interior rho
+    init_p: Optional[List[np.ndarray]] = None  # per block (kdim, jdim, idim) interior p (None: freestream)
 
     @property
     def nblocks(self) -> int:
@@ -109,13 +110,82 @@ class Case:
         prim[:] = self.freestream()[:, None, None, None]
         if self.init_rho is not None:
             prim[0, NG:-NG, NG:-NG, NG:-NG] = self.init_rho[b]
+        if self.init_p is not None:
+            prim[4, NG:-NG, NG:-NG, NG:-NG] = self.init_p[b]
         return prim
 
     # ------------------------------------------------------------------ writers
-    def write(self, casedir: str) -> None:
+    def block_pids(self, nprocs: int = 1, tries: int = 400) -> List[int]:
+        """1-based owner rank of every block for a run of the reference on
+        ``nprocs`` MPI ranks.
+
+        The reference tags an interface-residual message with its source
+        block (HaloExchanger.cpp:274,303), so two messages from one block to
+        two blocks of the same other rank can meet the wrong receives (and
+        abort with a truncation error where their sizes differ).  A valid
+        placement gives every block at most one interface partner on each
+        other rank.  Blocks are placed largest first onto the least-loaded
+        rank that keeps the placement valid, with seeded random
+        perturbations of both orders over ``tries`` attempts; the most
+        balanced valid placement is kept.  Raises ValueError when none is
+        found."""
+        import random
+        nprocs = max(1, int(nprocs))
+        nb = self.nblocks
+        if nprocs == 1:
+            return [1] * nb
+        if nprocs > nb:
+            raise ValueError("%d ranks for %d blocks" % (nprocs, nb))
+        nbrs = [set() for _ in range(nb)]
+        for r in self.regions:
+            if r.bctype < 0 and r.nbt - 1 != r.block:
+                nbrs[r.block].add(r.nbt - 1)
+        sizes = [int(v) for v in np.prod(self.dims.astype(np.int64), axis=1)]
+        mean = sum(sizes) / nprocs
+        bound = max(1.0, max(sizes) / mean)
+
+        def clash(rank, s):
+            seen = set()
+            for x in nbrs[s]:
+                if rank[x] >= 0 and rank[x] != rank[s]:
+                    if rank[x] in seen:
+                        return True
+                    seen.add(rank[x])
+            return False
+
+        rng = random.Random(12345)
+        best = None
+        for t in range(tries):
+            jit = 0.0 if t == 0 else 0.3
+            rank, load = [-1] * nb, [0] * nprocs
+            order = sorted(range(nb), key=lambda b: (-sizes[b] * (1.0 + jit * rng.random()), b))
+            for b in order:
+                for r in sorted(range(nprocs), key=lambda r: (load[r] + jit * 0.5 * mean * rng.random(), r)):
+                    rank[b] = r
+                    if not any(rank[s] >= 0 and clash(rank, s) for s in [b] + sorted(nbrs[b])):
+                        load[r] += sizes[b]
+                        break
+                    rank[b] = -1
+                if rank[b] < 0:
+                    break
+            if min(rank) < 0 or any(clash(rank, s) for s in range(nb)):
+                continue
+            imb = max(load) / mean
+            if best is None or imb < best[0]:
+                best = (imb, rank)
+            if imb <= bound * 1.02:
+                break
+        if best is None:
+            raise ValueError("no placement of %d blocks on %d ranks without a repeated message tag" % (nb, nprocs))
+        return [r + 1 for r in best[1]]
+
+    def write(self, casedir: str, nprocs: int = 1) -> None:
+        """The case's files; ``nprocs`` > 1 spreads the blocks over that many
+        pids in the BC file (block_pids), which the reference's BCReader
+        keeps when the run has as many ranks (BCReader.cpp:51-54,80-82)."""
         os.makedirs(casedir, exist_ok=True)
         write_plot3d(os.path.join(casedir, "grid.x"), self)
-        write_bc(os.path.join(casedir, "grid.inp"), self)
+        write_bc(os.path.join(casedir, "grid.inp"), self, self.block_pids(nprocs), max(1, int(nprocs)))
         write_param(os.path.join(casedir, "param.dat"), self)
         with open(os.path.join(casedir, "init.bin"), "wb") as f:
             for b in range(self.nblocks):
@@ -163,15 +233,17 @@ def read_plot3d(path: str):
     return dims, xyz
 
 
-def write_bc(path: str, case: Case) -> None:
-    """The 1976-marker BC file (BCReader.cpp:41-176), all blocks on pid 1."""
+def write_bc(path: str, case: Case, pids: Optional[Sequence[int]] = None, nprocs: int = 1) -> None:
+    """The 1976-marker BC file (BCReader.cpp:41-176): ``nprocs`` processes
+    in the file and each block's 1-based pid (all on pid 1 by default)."""
+    pids = list(pids) if pids is not None else [1] * case.nblocks
     by_block: Dict[int, List[Region]] = {b: [] for b in range(case.nblocks)}
     for r in case.regions:
         by_block[r.block].append(r)
-    lines = ["    1976", "       1", f"{case.nblocks:8d}"]
+    lines = ["    1976", f"{int(nprocs):8d}", f"{case.nblocks:8d}"]
     for b in range(case.nblocks):
         i, j, k = (int(v) for v in case.dims[b])
-        lines.append("       1")
+        lines.append(f"{int(pids[b]):8d}")
         lines.append(f"{i:8d}{j:8d}{k:8d}")
         lines.append(f" blk-{b + 1}")
         lines.append(f"{len(by_block[b]):8d}")
@@ -463,3 +535,224 @@ def bump_channel_case(seed: int, nodes: float, moo: float, reynolds: float, bump
     return Case("%s_%d_%dx%dx%d" % ("bump" if bump > 0 else "channel", seed, ni, nj, nk),
                 np.array([[ni, nj, nk]], dtype=np.int32), xyz, regs,
                 inputs, nsteps, [rho])
+
+
+def _flag_pair(rng, own, other, axes):
+    """Seeded re-expression of one interface window pair that keeps its
+    (identity) orientation, in forms the reference's BC files use
+    (testcases/NS/grid_64proc_out.inp:12-15): a tangential axis written with
+    negative indices on both sides (-1 .. -n), or run from its high end to
+    its low end on both sides (n .. 1).  Never both on one axis: the
+    reference's fill_bc_flag loops over the signed window bounds
+    (FlowField.cpp:164-196), and -n .. -1 writes outside its arrays.
+    own/other: [s_st, s_ed] of the two windows (lists of 3)."""
+    for ax in axes:
+        u = rng.random()
+        if u < 0.45:  # negative flag on both sides
+            for w in (own, other):
+                w[0][ax], w[1][ax] = -w[0][ax], -w[1][ax]
+        elif u < 0.7:  # reversed on both sides
+            for w in (own, other):
+                w[0][ax], w[1][ax] = w[1][ax], w[0][ax]
+
+
+def _halo_count(win, dims):
+    """Points of one side of the reference's halo message for an interface
+    window (get_memory_window, HaloExchanger.cpp:17-64): the tangential
+    ranges grow by one node at each end that is not a block edge."""
+    n = 1
+    for m in range(3):
+        a, b = abs(win[0][m]), abs(win[1][m])
+        if a == b:
+            continue
+        lo, hi = min(a, b), max(a, b)
+        n *= (hi - lo + 1) + (lo > 1) + (hi < dims[m])
+    return n
+
+
+def column_case(seed: int = 3, ncol=(4, 4), per_column: int = 4, lo: int = 17, hi: int = 129,
+                target: Optional[float] = 8.0e6, tol: float = 0.02, tries: int = 100000, *, amp: float = 0.05,
+                moo: float = 0.5, reynolds: float = 5.0e3, cfl: float = 5.0, noise: float = 1.0e-3,
+                nsteps: int = 1, flags: bool = True, name: str = "columns") -> Case:
+    """Blocks on a seeded adjacency graph with partial-face windows.
+
+    ncol[0] x ncol[1] columns in (j, k); column (b, c) has j size sy[b] and
+    k size sz[c] (each log-uniform in [lo, hi]) and is split along i into
+    ``per_column`` blocks.  The block edges of neighbouring columns
+    interleave strictly: a global sequence of 2 (per_column + 1) positions
+    is laid out with gaps log-uniform in [lo/2, hi/2]; columns with b + c
+    even put their edges near the even positions, the others near the odd
+    ones, each edge moved by its own seeded jitter.  So block i sizes fall
+    near [lo, hi], columns start and end at different i, and every block
+    face meets the neighbouring column's blocks in partial windows, none
+    containing another.  That matters to the reference's halo, which grows
+    each side's window by one node at every end that is not a block edge
+    and unpacks without a map (HaloExchanger.cpp:17-64,161-172): with
+    interleaved edges each window pair has one such end per side, the two
+    messages have the same length, and the variables stay aligned.  (A
+    window that contains its partner's, or edges that coincide, give
+    messages of different lengths and the reference then mixes variables;
+    the generator checks that no pair does.)  The seed is advanced until
+    the node count is within ``tol`` of ``target`` (None: the first seed).
+    Blocks share their face node planes.
+
+    Regions of each block, face by face (imin, imax, jmin, jmax, kmin,
+    kmax) and along each face by position:
+      * i faces: full-face 1:1 windows to the column's next block; the
+        column's two ends are far-field (4);
+      * j and k faces toward a neighbouring column: a 1:1 window for every
+        block of that column overlapping this one by at least two nodes,
+        and the parts of the face beyond the neighbouring column's ends a
+        physical region of type 4 or 6 (seeded), at least two nodes wide
+        (it shares its first or last node line with the window beside it);
+      * the global j=1 face is a viscous wall (2), other outer faces
+        far-field (4).
+    With ``flags`` each interface pair is re-expressed, seeded, with
+    negative tangential indices or reversed ranges on both sides
+    (_flag_pair); the orientation stays the identity.  Nodes are Cartesian
+    plus a seeded sinusoidal perturbation of ``amp`` x spacing."""
+    nj_col, nk_col = ncol
+    npos = 2 * (per_column + 1)
+    for s in range(seed, seed + tries):
+        rng = np.random.default_rng(s)
+        logu = lambda a, b, n: np.exp(rng.uniform(np.log(a), np.log(b), size=n))  # noqa: E731
+        sy = np.rint(logu(lo, hi, nj_col)).astype(int)
+        sz = np.rint(logu(lo, hi, nk_col)).astype(int)
+        gaps = np.rint(logu(lo / 2.0, hi / 2.0, npos - 1)).astype(int)
+        gaps = np.maximum(gaps, 8)
+        G = np.concatenate([[0], np.cumsum(gaps)])
+        # jitter of each position: strictly inside half the smaller gap beside it,
+        # keeping edges of neighbouring columns at least 3 nodes apart
+        room = np.minimum(np.concatenate([[gaps[0]], gaps]), np.concatenate([gaps, [gaps[-1]]]))
+        jmax = np.maximum((room - 3) // 2, 0)
+        edges = np.zeros((nj_col, nk_col, per_column + 1), dtype=int)
+        for b in range(nj_col):
+            for c in range(nk_col):
+                par = (b + c) % 2
+                for k in range(per_column + 1):
+                    q = 2 * k + par
+                    edges[b, c, k] = G[q] + rng.integers(-jmax[q], jmax[q] + 1)
+        edges -= edges.min()
+        sx = edges[:, :, 1:] - edges[:, :, :-1] + 1
+        tot = sum(int(sx[b, c].sum() - (per_column - 1)) * int(sy[b]) * int(sz[c])
+                  for b in range(nj_col) for c in range(nk_col))
+        if target is None or abs(tot - target) <= tol * target:
+            used = s
+            break
+    else:
+        raise ValueError("no seed in [%d, %d) gives %.3g nodes" % (seed, seed + tries, target))
+    rng = np.random.default_rng(used + 1000003)
+    oy = np.concatenate([[0], np.cumsum(sy - 1)])
+    oz = np.concatenate([[0], np.cumsum(sz - 1)])
+    NX = int(edges.max()) + 1
+    NY, NZ = int(oy[-1]) + 1, int(oz[-1]) + 1
+    h = np.array([2.0 / (NX - 1), 1.0 / (NY - 1), 1.0 / (NZ - 1)])
+    phases = rng.uniform(0.0, 2 * np.pi, size=(3, 3))
+    waves = rng.integers(1, 4, size=(3, 3))
+
+    def coords(I, J, K):
+        out = []
+        base = [I * h[0], J * h[1], K * h[2]]
+        for m in range(3):
+            arg = (2 * np.pi * (waves[m, 0] * I / (NX - 1) + waves[m, 1] * J / (NY - 1)
+                                + waves[m, 2] * K / (NZ - 1)) + phases[m, 0])
+            out.append(base[m] + amp * h[m] * np.sin(arg) * np.cos(phases[m, 1] + 0.5 * arg))
+        return out
+
+    blocks = [(a, b, c) for c in range(nk_col) for b in range(nj_col) for a in range(per_column)]
+    index = {abc: n for n, abc in enumerate(blocks)}
+    dims = np.array([[sx[b, c, a], sy[b], sz[c]] for (a, b, c) in blocks], dtype=np.int32)
+    xyz, rhos = [], []
+    for n, (a, b, c) in enumerate(blocks):
+        ni, nj, nk = (int(v) for v in dims[n])
+        K, J, I = np.meshgrid(oz[c] + np.arange(nk), oy[b] + np.arange(nj), edges[b, c, a] + np.arange(ni),
+                              indexing="ij")
+        xyz.append(np.stack(coords(I.astype(float), J.astype(float), K.astype(float))))
+        rhos.append(1.0 + noise * rng.uniform(-1.0, 1.0, size=(nk, nj, ni)))
+
+    FACE = {"imin": 0, "imax": 1, "jmin": 2, "jmax": 3, "kmin": 4, "kmax": 5}
+    entries: List[List[tuple]] = [[] for _ in blocks]  # (face, position, region)
+
+    def window(n, face, rng_t=None):
+        d = [int(v) for v in dims[n]]
+        ax = {"i": 0, "j": 1, "k": 2}[face[0]]
+        st, ed = [1, 1, 1], list(d)
+        st[ax] = ed[ax] = 1 if face.endswith("min") else d[ax]
+        if rng_t is not None:
+            st[0], ed[0] = rng_t
+        return [st, ed], ax
+
+    pairs = []
+
+    def add_pair(n, face_n, m, face_m, rng_n=None, rng_m=None):
+        wn, ax = window(n, face_n, rng_n)
+        wm, _ = window(m, face_m, rng_m)
+        if _halo_count(wn, dims[n]) != _halo_count(wm, dims[m]):
+            raise AssertionError("halo messages of different lengths between blocks %d and %d" % (n + 1, m + 1))
+        if flags:
+            _flag_pair(rng, wn, wm, [t for t in range(3) if t != ax])
+        rn = Region(n, -1, tuple(wn[0]), tuple(wn[1]), tuple(wm[0]), tuple(wm[1]), nbt=m + 1)
+        rm = Region(m, -1, tuple(wm[0]), tuple(wm[1]), tuple(wn[0]), tuple(wn[1]), nbt=n + 1)
+        entries[n].append((FACE[face_n], rng_n[0] if rng_n else 0, rn))
+        entries[m].append((FACE[face_m], rng_m[0] if rng_m else 0, rm))
+        pairs.append((rn, rm))
+
+    def add_phys(n, face, kind, rng_t=None):
+        w, _ = window(n, face, rng_t)
+        entries[n].append((FACE[face], rng_t[0] if rng_t else 0, Region(n, kind, tuple(w[0]), tuple(w[1]))))
+
+    unmatched_kind = lambda: 4 if rng.random() < 0.5 else 6  # noqa: E731
+    covered = {}  # (block, face) -> i ranges (local, 1-based) of its windows
+    for n, (a, b, c) in enumerate(blocks):
+        if a == 0:
+            add_phys(n, "imin", 4)
+        if a + 1 < per_column:
+            add_pair(n, "imax", index[(a + 1, b, c)], "imin")
+        else:
+            add_phys(n, "imax", 4)
+        a0 = int(edges[b, c, a])
+        a1 = a0 + int(dims[n][0]) - 1
+        for face, other, (b2, c2) in (("jmax", "jmin", (b + 1, c)), ("kmax", "kmin", (b, c + 1))):
+            if b2 >= nj_col or c2 >= nk_col:
+                continue
+            for a2 in range(per_column):
+                m = index[(a2, b2, c2)]
+                e0 = int(edges[b2, c2, a2])
+                e1 = e0 + int(dims[m][0]) - 1
+                o0, o1 = max(a0, e0), min(a1, e1)
+                if o1 - o0 >= 1:
+                    rn, rm = (o0 - a0 + 1, o1 - a0 + 1), (o0 - e0 + 1, o1 - e0 + 1)
+                    add_pair(n, face, m, other, rn, rm)
+                    covered.setdefault((n, face), []).append(rn)
+                    covered.setdefault((m, other), []).append(rm)
+    for n, (a, b, c) in enumerate(blocks):
+        ni = int(dims[n][0])
+        for face, outer, kind in (("jmin", b == 0, 2), ("jmax", b + 1 == nj_col, 4),
+                                  ("kmin", c == 0, 4), ("kmax", c + 1 == nk_col, 4)):
+            if outer:
+                add_phys(n, face, kind)
+                continue
+            got = sorted(covered.get((n, face), []))
+            if not got:
+                add_phys(n, face, unmatched_kind())
+                continue
+            if got[0][0] > 1:
+                add_phys(n, face, unmatched_kind(), (1, max(2, got[0][0])))
+            for (l0, l1), (r0, r1) in zip(got, got[1:]):
+                if r0 > l1:
+                    raise AssertionError("gap inside a face of block %d" % (n + 1))
+            if got[-1][1] < ni:
+                add_phys(n, face, unmatched_kind(), (min(got[-1][1], ni - 1), ni))
+    regions: List[Region] = []
+    for n in range(len(blocks)):
+        entries[n].sort(key=lambda e: (e[0], e[1]))
+        regions.extend(r for _, _, r in entries[n])
+    where, count = {}, {}
+    for r in regions:
+        count[r.block] = count.get(r.block, 0) + 1
+        where[id(r)] = count[r.block]
+    for rn, rm in pairs:
+        rn.ibcwin = where[id(rm)]
+        rm.ibcwin = where[id(rn)]
+    inputs = dict(DEFAULT_INPUTS, moo=moo, reynolds=reynolds, cfl=cfl)
+    return Case("%s_seed%d" % (name, used), dims, xyz, regions, inputs, nsteps, rhos)

@@ -14,6 +14,7 @@
 //               partner face node, per staging entry.
 //   avg_apply   phase 3 (:330-397): dq <- 0.5*(stage*vol + dq), entries of a
 //               face node applied in region order.
+#include "dd_pow.h"
 #include "orion_dev.h"
 #include "orion_kernels.h"
 
@@ -224,8 +225,8 @@ __global__ void __launch_bounds__(256) bc_kernel(const BlockDev* __restrict__ bl
       const double vin = PR(b, nn[0], nn[1], nn[2], 2), win = PR(b, nn[0], nn[1], nn[2], 3);
       const double pin = PR(b, nn[0], nn[1], nn[2], 4);
       const double cin = sqrt(gama * pin / rin);
-      const double sin_ = pin / pow(rin, gama);
-      const double sinf = pinf / pow(rinf, gama);
+      const double sin_ = pin / orion_dd::pow_cr(rin, gama);
+      const double sinf = pinf / orion_dd::pow_cr(rinf, gama);
       const double vnin = uin * nx + vin * ny + win * nz;
       const double vninf = uinf * nx + vinf * ny + winf * nz;
       const double vel = sqrt(uin * uin + vin * vin + win * win);
@@ -242,8 +243,8 @@ __global__ void __launch_bounds__(256) bc_kernel(const BlockDev* __restrict__ bl
         } else {
           sb = sinf; ur = uinf; vr = vinf; wr = winf; vnr = vninf;
         }
-        rb = pow(cb * cb / (gama * sb), 1.0 / gm1);
-        pb = sb * pow(rb, gama);
+        rb = orion_dd::pow_cr(cb * cb / (gama * sb), 1.0 / gm1);
+        pb = sb * orion_dd::pow_cr(rb, gama);
         ub = ur + (vnb - vnr) * nx;
         vb = vr + (vnb - vnr) * ny;
         wb = wr + (vnb - vnr) * nz;

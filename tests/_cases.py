@@ -63,6 +63,91 @@ def short_duct(nsteps=3):
     return c
 
 
+def columns_small(nsteps=3):
+    """2 x 2 columns of 3 blocks each with staggered i sizes: partial-face
+    interface windows, unmatched face parts of type 4 and 6, and windows
+    written with negative and reversed tangential ranges."""
+    return cases.column_case(seed=5, ncol=(2, 2), per_column=3, lo=7, hi=14, target=None, nsteps=nsteps,
+                             name="columns_small")
+
+
+def rotated_pair(nsteps=3):
+    """Two blocks joined through A's imax face and B's imin face with B's
+    axes turned: B's j runs along A's k and B's k along A's j, reversed.
+    The window pair is written as the reference's BC files write such
+    interfaces (testcases/NS/grid_64proc_out.inp:12-15): a negative range
+    on one tangential axis and a reversed one, so analyze_bc_connect
+    (BCPreprocess.cpp:16-192) must swap the axes and flip a sign.  The
+    halo copies without an orientation map (HaloExchanger.cpp:161-172),
+    which the reference does too."""
+    na, nj, nk = 9, 8, 7
+    nb_i = 10
+    h = 1.0 / 8
+    # global node (I, J, K); A: I = i, J = j, K = k; B: I = na-1+iB, J = nj-1-kB, K = jB (0-based)
+    def xyz_of(I, J, K):
+        X = I * h + 0.03 * h * np.sin(1.3 * J + 0.7 * K)
+        Y = J * h * 0.9 + 0.02 * h * np.cos(0.9 * I - 0.4 * K)
+        Z = K * h * 1.1 + 0.02 * h * np.sin(0.5 * I + 1.1 * J)
+        return np.stack([X, Y, Z])
+    K, J, I = np.meshgrid(np.arange(nk), np.arange(nj), np.arange(na), indexing="ij")
+    xa = xyz_of(I.astype(float), J.astype(float), K.astype(float))
+    # B dims (nb_i, nk, nj): index order (kB, jB, iB)
+    KB, JB, IB = np.meshgrid(np.arange(nj), np.arange(nk), np.arange(nb_i), indexing="ij")
+    xb = xyz_of((na - 1 + IB).astype(float), (nj - 1 - KB).astype(float), JB.astype(float))
+    regs = [
+        Region(0, 4, (1, 1, 1), (1, nj, nk)),
+        Region(0, -1, (na, 1, -1), (na, nj, -nk), (1, -1, nj), (1, -nk, 1), nbt=2, ibcwin=1),
+        Region(0, 2, (1, 1, 1), (na, 1, nk)),
+        Region(0, 4, (1, nj, 1), (na, nj, nk)),
+        Region(0, 3, (1, 1, 1), (na, nj, 1)),
+        Region(0, 6, (1, 1, nk), (na, nj, nk)),
+        Region(1, -1, (1, -1, nj), (1, -nk, 1), (na, 1, -1), (na, nj, -nk), nbt=1, ibcwin=2),
+        Region(1, 4, (nb_i, 1, 1), (nb_i, nk, nj)),
+        Region(1, 3, (1, 1, 1), (nb_i, 1, nj)),
+        Region(1, 6, (1, nk, 1), (nb_i, nk, nj)),
+        Region(1, 4, (1, 1, 1), (nb_i, nk, 1)),
+        Region(1, 2, (1, 1, nj), (nb_i, nk, nj)),
+    ]
+    rng = np.random.default_rng(33)
+    rhos = [1.0 + 1e-2 * rng.uniform(-1, 1, size=(nk, nj, na)), 1.0 + 1e-2 * rng.uniform(-1, 1, size=(nj, nk, nb_i))]
+    inputs = dict(DEFAULT_INPUTS, moo=0.6, reynolds=3.0e3, cfl=3.0)
+    return Case("rotated_pair", np.array([[na, nj, nk], [nb_i, nk, nj]], dtype=np.int32), [xa, xb], regs, inputs,
+                nsteps, rhos)
+
+
+def near_vacuum(nsteps=3):
+    """A near-vacuum blob (density and pressure down to 1e-7 of their
+    freestream values) in a block bounded by walls, symmetry and
+    extrapolation: reconstructed states at or below 1e-6 take the
+    inviscid low-order fallback (InviscidFluxComputer.cpp:346-355), and
+    the update meets both guards of update_limited -- the pressure-change
+    damping and the refusal that keeps the old state
+    (StateUpdater.cpp:502-538)."""
+    c = bc_variety(nvis=1, nsteps=nsteps)
+    ni, nj, nk = (int(v) for v in c.dims[0])
+    x, y, z = c.xyz[0]
+    r2 = (x - x.mean()) ** 2 + (y - y.mean()) ** 2 + (z - z.mean()) ** 2
+    blob = np.minimum(1.0, 2.0 * np.exp(-r2 / (0.18 ** 2)))  # flat core at the floor
+    depth = 1.0 - 1.0e-7
+    c.init_rho = [1.0 - depth * blob]
+    p_inf = 1.0 / (1.4 * c.inputs["moo"] ** 2)
+    c.init_p = [p_inf * (1.0 - depth * blob)]
+    c.name = "near_vacuum"
+    return c
+
+
+def inputs_variant(nsteps=3):
+    """The boundary-variety block with the inputs no other case changes:
+    gcl=0 metrics (MetricComputer.cpp:103-256 without the conservative
+    GCL form), angles of attack and sideslip (Inflow.cpp:300-459; the
+    freestream boundary reads them) and csrv=2 in the viscous spectral
+    radius of the time step (TimeIntegrator.cpp:81-82)."""
+    c = bc_variety(nvis=1, nsteps=nsteps)
+    c.inputs = dict(c.inputs, gcl=0, attack=4.0, sideslip=2.5, csrv=2)
+    c.name = "inputs_variant"
+    return c
+
+
 ALL = {
     "channel_small": channel_small,
     "lattice_small": lattice_small,
@@ -71,4 +156,8 @@ ALL = {
     "bc_variety_inviscid": lambda nsteps=3: bc_variety(0, nsteps),
     "isothermal_wall": isothermal_wall,
     "short_duct": short_duct,
+    "columns_small": columns_small,
+    "rotated_pair": rotated_pair,
+    "near_vacuum": near_vacuum,
+    "inputs_variant": inputs_variant,
 }

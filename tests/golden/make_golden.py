@@ -20,6 +20,17 @@ outputs as small .npz files:
   vts_bc_variety.npz    the files PostProcess::write_solution writes after
                         3 steps of the viscous bc_variety block (the VTS of
                         the block and the VTM index), byte for byte
+  full_eight_block.npz  the 8 x 129x65x65 curvilinear lattice (M 0.8,
+  full_uneven64.npz     Re 1e5, CFL 10), the 64-block uneven lattice of
+  full_columns64.npz    about 7.85M nodes and the 64-block adjacency-graph
+                        mesh of about 8.05M nodes with partial-face windows
+                        (both M 0.5, Re 5e3, CFL 5), 20 steps on 8 MPI
+                        ranks or as many as the mesh allows (written only
+                        when named on the command line): history, per block and variable the
+                        sum, absolute sum and sum of squares of the final
+                        interior prim and dq, and every 997th interior value
+                        of prim and dq (norms and samples: the full fields
+                        are gigabytes)
 
 Inputs are regenerated from the seeded generators in orion_amd.cases, so
 each fixture records only the generator call and the reference's outputs.
@@ -71,11 +82,49 @@ def vts_fixture():
     return d
 
 
+FULL = {
+    "full_eight_block": "cases.eight_block_case(seed=7, nsteps=20)",
+    "full_uneven64": "cases.uneven_lattice_case(seed=11, nsteps=20)",
+    "full_columns64": "cases.column_case(seed=3, nsteps=20)",
+}
+FULL_STEPS = 20
+FULL_RANKS = 8
+SAMPLE_STRIDE = 997
+
+
+def full_norms(fin, nblocks):
+    """Per block: sums, absolute sums and sums of squares of the interior
+    prim (6) and dq (5), and a strided sample of each."""
+    d = {}
+    for b in range(nblocks):
+        for key in ("prim", "dq"):
+            a = interior(fin[b][key])
+            d["%s_sum_%d" % (key, b)] = a.sum(axis=(1, 2, 3))
+            d["%s_abs_%d" % (key, b)] = np.abs(a).sum(axis=(1, 2, 3))
+            d["%s_sq_%d" % (key, b)] = (a * a).sum(axis=(1, 2, 3))
+            d["%s_sample_%d" % (key, b)] = a.reshape(a.shape[0], -1)[:, ::SAMPLE_STRIDE].copy()
+    return d
+
+
+def full_fixture(name):
+    c = eval(FULL[name])  # noqa: S307 -- one of the two generator calls above
+    tmp = tempfile.mkdtemp(prefix="orion_golden_")
+    ranks = binding.max_ranks(c, FULL_RANKS)
+    hist, read = binding.run_reference(c, os.path.join(tmp, "case"), os.path.join(tmp, "out"), FULL_STEPS,
+                                       ranks=ranks)
+    d = dict(generator=FULL[name], ranks=np.array(ranks), hist=hist, stride=np.array(SAMPLE_STRIDE))
+    d.update(full_norms(read("final.bin"), c.nblocks))
+    return d
+
+
 def fixtures(only=None):
     out = {}
     if only and "vts_bc_variety" in only:
         out["vts_bc_variety"] = vts_fixture()
-    if only and not set(only) - {"vts_bc_variety"}:
+    for name in FULL:
+        if only and name in only:
+            out[name] = full_fixture(name)
+    if only and not set(only) - {"vts_bc_variety"} - set(FULL):
         return out
     c = cases.channel_wall_case(seed=1, nsteps=200)
     hist, read = run(c, 200)
@@ -131,7 +180,8 @@ def main():
     with open(os.path.join(HERE, "README.json"), "w") as f:
         json.dump({"written_by": "tests/golden/make_golden.py",
                    "source": "reference solver built from its own sources (oracle/_ref/ref_harness)",
-                   "hist_columns": ["rms", "max", "max_loc_block0", "i", "j", "k", "var", "phydtime"]}, f,
+                   "hist_columns": ["rms", "max", "max_loc_block0", "i", "j", "k", "var", "phydtime",
+                                    "rms_compensated (full_* fixtures)"]}, f,
                   indent=1)
 
 

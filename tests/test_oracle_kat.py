@@ -271,3 +271,45 @@ def test_inviscid_quadratic_exact():
         got = dq[m][sl]
         exp = np.broadcast_to(want[m][sl[2]][None, None, :], got.shape)
         assert np.abs(got - exp).max() < 1e-12, (m, np.abs(got - exp).max())
+
+
+def test_guard_paths_are_taken():
+    """The near-vacuum case takes every guard branch of the reference's
+    step (counted in the oracle): the inviscid low-order fallback on both
+    sides of a face (InviscidFluxComputer.cpp:346-355), the limiter's
+    damping and its refusal that keeps the old state
+    (StateUpdater.cpp:502-538).  Its parity with the reference build and
+    with the GPU is checked with the other cases."""
+    from oracle import binding
+    from tests import _cases
+    c = _cases.near_vacuum(nsteps=2)
+    binding.guard_counts(reset=True)
+    o = binding.OracleCase(c)
+    o.run(2)
+    left, right, damped, kept = binding.guard_counts(reset=True)
+    assert left > 0 and right > 0 and damped > 0 and kept > 0, (left, right, damped, kept)
+
+
+def test_inputs_variant_changes_what_it_selects():
+    """gcl=0, attack/sideslip and csrv=2 each change the state of the
+    boundary-variety block, so the parity cases that set them pin those
+    branches (MetricComputer.cpp:103-256, Inflow.cpp:300-459,
+    TimeIntegrator.cpp:81-82)."""
+    from oracle.binding import OracleCase
+    from tests import _cases
+    base = _cases.bc_variety(nvis=1, nsteps=1)
+    ob = OracleCase(base)
+    for key, val in (("gcl", 0), ("attack", 4.0), ("csrv", 2)):
+        c = _cases.bc_variety(nvis=1, nsteps=1)
+        c.inputs = dict(c.inputs, **{key: val})
+        o = OracleCase(c)
+        if key == "gcl":
+            assert not np.array_equal(o.field(0, "metrics"), ob.field(0, "metrics"))
+        o.op("exchange_bc")
+        o.op("apply_physical_bc")
+        o.op("update_flow_states")
+        o.op("calculate_time_step")
+        ob2 = OracleCase(base)
+        for op in ("exchange_bc", "apply_physical_bc", "update_flow_states", "calculate_time_step"):
+            ob2.op(op)
+        assert not np.array_equal(o.field(0, "dt"), ob2.field(0, "dt")), key

@@ -44,7 +44,8 @@ def test_stages_bit_exact(name):
                     assert bad == 0, "%s step %d %s block %d %s: %d values differ" % (name, step, stage, b, key, bad)
 
 
-@pytest.mark.parametrize("name", ["channel_small", "lattice_small", "bc_variety_visc"])
+@pytest.mark.parametrize("name", ["channel_small", "lattice_small", "bc_variety_visc", "columns_small",
+                                  "rotated_pair"])
 def test_history_bit_exact(name):
     n = 25
     case = _cases.ALL[name](nsteps=n)
