@@ -21,6 +21,8 @@
 //   outdir/s<step>_<stage>.bin   for step <= dump_steps, every block's
 //                     [prim 6P][dq 5P][c P][mu P][dt P] after each stage
 //                     (one rank only)
+//   outdir/params.bin  moo, reynolds, tref, visc, poo, uoo, voo, woo, pref,
+//                     rref as init_inflow derived them (rank 0)
 //   outdir/final.bin  same layout after the last step (one rank), or
 //   outdir/final_b<n>.bin  block n's fields, written by its owner (R > 1)
 //   with "vts": PostProcess::write_solution (src/postprocess/PostProcess.cpp:44-123)
@@ -138,6 +140,16 @@ int main(int argc, char** argv) {
   mesh::compute_grid_metrics(grid, fs, params);
   mesh::check_grid_metrics(grid, fs, bcd);
   preprocess::init_inflow(params, fs);
+  if (core::Runtime::myid == 0) {
+    // the inflow values the step reads, as init_inflow derived them
+    const auto& F = params.inflow;
+    const double vals[10] = {F.moo, F.reynolds, F.tref, F.visc, F.poo, F.uoo, F.voo, F.woo, F.pref, F.rref};
+    std::string p = outdir + "/params.bin";
+    if (FILE* f = std::fopen(p.c_str(), "wb")) {
+      std::fwrite(vals, sizeof(double), 10, f);
+      std::fclose(f);
+    }
+  }
   preprocess::initialize_flow(params, fs);
   preprocess::update_temperature(params, fs);
 

@@ -179,12 +179,13 @@ class Case:
             raise ValueError("no placement of %d blocks on %d ranks without a repeated message tag" % (nb, nprocs))
         return [r + 1 for r in best[1]]
 
-    def write(self, casedir: str, nprocs: int = 1) -> None:
+    def write(self, casedir: str, nprocs: int = 1, ascii_grid: bool = False) -> None:
         """The case's files; ``nprocs`` > 1 spreads the blocks over that many
         pids in the BC file (block_pids), which the reference's BCReader
-        keeps when the run has as many ranks (BCReader.cpp:51-54,80-82)."""
+        keeps when the run has as many ranks (BCReader.cpp:51-54,80-82);
+        ``ascii_grid`` writes the grid in the ASCII form."""
         os.makedirs(casedir, exist_ok=True)
-        write_plot3d(os.path.join(casedir, "grid.x"), self)
+        (write_plot3d_ascii if ascii_grid else write_plot3d)(os.path.join(casedir, "grid.x"), self)
         write_bc(os.path.join(casedir, "grid.inp"), self, self.block_pids(nprocs), max(1, int(nprocs)))
         write_param(os.path.join(casedir, "param.dat"), self)
         with open(os.path.join(casedir, "init.bin"), "wb") as f:
@@ -206,6 +207,22 @@ def write_plot3d(path: str, case: Case) -> None:
             _rec(f, np.asarray(case.dims[b], dtype="<i4").tobytes())
         for b in range(case.nblocks):
             _rec(f, np.ascontiguousarray(case.xyz[b], dtype="<f8").tobytes())
+
+
+def write_plot3d_ascii(path: str, case: Case) -> None:
+    """The ASCII grid form the reference falls back to
+    (Plot3DReader.cpp:162-208): block count, the sizes, then every block's
+    x, y and z, with 17 significant digits (a round trip to the same
+    doubles)."""
+    with open(path, "w") as f:
+        f.write("%d\n" % case.nblocks)
+        for b in range(case.nblocks):
+            f.write("%d %d %d\n" % tuple(int(v) for v in case.dims[b]))
+        for b in range(case.nblocks):
+            for c in range(3):
+                vals = case.xyz[b][c].reshape(-1)
+                for s in range(0, vals.size, 4):
+                    f.write(" ".join("%.17g" % v for v in vals[s:s + 4]) + "\n")
 
 
 def read_plot3d(path: str):
@@ -273,10 +290,10 @@ def write_param(path: str, case: Case) -> None:
   sideslip = {d(p['sideslip'])}
   reynolds = {d(p['reynolds'])}
   twall = {d(p['twall'])}
-  height = -1.0
+  height = {d(p.get('height', -1.0))}
   tref = {d(p['tref'])}
-  pref = -1.0
-  rref = 1.225
+  pref = {d(p.get('pref', -1.0))}
+  rref = {d(p.get('rref', 1.225))}
   ndim = 3
 /
 $force

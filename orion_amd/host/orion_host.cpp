@@ -148,16 +148,109 @@ void Params::load_namelist_file(const std::string& path) {
   }
 }
 
+// call_air (src/preprocess/Inflow.cpp:54-265): the standard atmosphere's
+// temperature (K), pressure (Pa), density (kg/m^3) and speed of sound at
+// h_km.  The layer table and the expression forms are the reference's, so
+// the values agree to the bit: layers with a temperature gradient
+//   al = (t_top - t_lo) / span,  t = t_base + al (h - h_lo),
+//   p = p_base (t / t_base)^(-g / (r al)),  rho = rho_base (t / t_base)^(-1 - g / (r al)),
+// isothermal layers p = p_base exp(-g (h - h_lo) / (r t)) and the same for rho.
+// Above 700 km no layer applies: t and p keep the values passed in and the
+// density is 0, as in the reference.
+static void standard_atmosphere(double h_km, double& t, double& p, double& den, double& a) {
+  const double h = h_km * 1000.0;
+  const double r = 287.053, g0 = 9.80665, rp = 6.37111e6;
+  const double g = std::pow(rp / (rp + h), 2) * g0;
+  struct Layer {
+    double top;        // the layer holds h <= top
+    bool iso;          // isothermal
+    double t_lo, t_hi; // gradient: al = (t_hi - t_lo) / span
+    double span, h_lo;
+    double t_base;     // temperature at h_lo (the base of the power law)
+    double p_base, rho_base;
+  };
+  static const Layer L[] = {
+      {11019.0, false, 288.15, 216.65, 11019.0, 0.0, 288.15, 10.1325e2, 1.225},
+      {20063.0, true, 216.65, 216.65, 0.0, 11019.0, 216.65, 2.2632e2, 3.6392e-1},
+      {32162.0, false, 216.65, 228.65, 32162.0 - 20063.0, 20063.0, 216.65, 5.4747e1, 8.8035e-2},
+      {47350.0, false, 228.65, 270.65, 47350.0 - 32162.0, 32162.0, 228.65, 8.6789, 1.3225e-2},
+      {52429.0, true, 270.65, 270.65, 0.0, 47350.0, 270.65, 1.1090, 1.4275e-3},
+      {61591.0, false, 270.65, 252.65, 61591.0 - 52429.0, 52429.0, 270.65, 5.8997e-1, 7.5943e-4},
+      {79994.0, false, 252.65, 180.65, 79994.0 - 61591.0, 61591.0, 252.65, 1.8209e-1, 2.5109e-4},
+      {90000.0, true, 180.65, 180.65, 0.0, 79994.0, 180.65, 1.0376e-2, 2.0010e-5},
+      {100000.0, false, 180.65, 210.02, 10000.0, 90000.0, 180.65, 1.6437e-3, 3.4165e-6},
+      {110000.0, false, 210.02, 257.00, 10000.0, 100000.0, 210.02, 3.0070e-4, 5.6044e-7},
+      {120000.0, false, 257.00, 349.49, 10000.0, 110000.0, 257.00, 7.3527e-5, 9.7081e-8},
+      {150000.0, false, 349.49, 892.79, 30000.0, 120000.0, 349.49, 2.5209e-5, 2.2222e-8},
+      {160000.0, false, 892.79, 1022.20, 10000.0, 150000.0, 892.79, 5.0599e-6, 2.0752e-9},
+      {170000.0, false, 1022.20, 1103.40, 10000.0, 160000.0, 1022.20, 3.6929e-6, 1.2336e-9},
+      {190000.0, false, 1103.40, 1205.40, 20000.0, 170000.0, 1103.40, 2.7915e-6, 7.8155e-10},
+      {230000.0, false, 1205.40, 1322.30, 40000.0, 190000.0, 1205.40, 1.6845e-6, 3.5807e-10},
+      {300000.0, false, 1322.30, 1432.10, 70000.0, 230000.0, 1322.30, 6.7138e-7, 1.5640e-10},
+      {400000.0, false, 1432.10, 1487.40, 100000.0, 300000.0, 1432.10, 1.8828e-7, 1.9159e-11},
+      {500000.0, false, 1487.40, 1499.20, 100000.0, 400000.0, 1487.40, 4.0278e-8, 2.8028e-12},
+      {600000.0, false, 1499.20, 1506.10, 100000.0, 500000.0, 1499.20, 1.0949e-8, 5.2148e-13},
+      {700000.0, false, 1506.10, 1507.60, 100000.0, 600000.0, 1506.10, 3.4475e-9, 1.1367e-13},
+  };
+  double rho = 0.0;
+  for (const Layer& y : L) {
+    if (!(h <= y.top)) continue;
+    if (y.iso) {
+      t = y.t_base;
+      p = y.p_base * std::exp(-g * (h - y.h_lo) / (r * y.t_base));
+      rho = y.rho_base * std::exp(-g * (h - y.h_lo) / (r * y.t_base));
+    } else {
+      const double al = (y.t_hi - y.t_lo) / y.span;
+      t = y.t_base + al * (h - y.h_lo);
+      p = y.p_base * std::pow(t / y.t_base, -g / (r * al));
+      rho = y.rho_base * std::pow(t / y.t_base, -1.0 - g / (r * al));
+    }
+    break;
+  }
+  a = std::sqrt(1.4 * r * t);
+  p = p * 100.0;
+  den = rho;
+}
+
+Params::Reference Params::derive_reference() const {
+  // setup_perfect_gas_air and the species sums (Inflow.cpp:15-46,326-360):
+  // N2 and O2 at 79 % and 21 %, molar masses in kg/mol
+  const double ws[2] = {28.0 * 1.0e-3, 32.0 * 1.0e-3}, cn[2] = {0.79, 0.21};
+  double mref1 = 0.0;
+  for (int s = 0; s < 2; ++s) mref1 += cn[s] * (1.0 / ws[s]);
+  const double rjmk = 8.31434;  // Params.hpp:19
+  const double R_spec = rjmk * mref1;
+  Reference o{inflow.reynolds, inflow.tref, inflow.pref, inflow.rref};
+  if (inflow.height < 0.0) {
+    // Inflow.cpp:362-390
+    const bool p_given = o.pref > 0.0, t_given = o.tref > 0.0, r_given = o.rref > 0.0;
+    require(!(p_given && t_given && r_given), "pressure, temperature and density cannot all be given (Inflow.cpp:369)");
+    if (p_given && t_given && !r_given) o.rref = o.pref / (o.tref * R_spec);
+    else if (r_given && t_given && !p_given) o.pref = o.rref * o.tref * R_spec;
+    else if (p_given && r_given && !t_given) o.tref = o.pref / (o.rref * R_spec);
+    else throw std::runtime_error("invalid combination of pressure, temperature and density (Inflow.cpp:388)");
+    // Inflow.cpp:323-325 with params.inflow.lfref = lref = 1
+    o.reynolds = (o.reynolds / 1.0) * 1.0;
+  } else {
+    // Inflow.cpp:393-407,411-415
+    double a = 0.0, den = 0.0;
+    standard_atmosphere(inflow.height, o.tref, o.pref, den, a);
+    o.rref = o.pref / (o.tref * R_spec);
+    const double ccon = (1.0 + 110.4 / 273.15) / (1.0 + 110.4 / o.tref);
+    const double visloo = std::sqrt(o.tref / 273.15) * ccon * 1.715e-5;
+    const double vref = a * inflow.moo;
+    o.reynolds = o.rref * vref * 1.0 / visloo;
+  }
+  return o;
+}
+
 orion_inputs Params::to_inputs() const {
-  require(inflow.height < 0.0, "height >= 0 selects the standard atmosphere, which this build does not implement");
-  const bool p_given = inflow.pref > 0.0, t_given = inflow.tref > 0.0, r_given = inflow.rref > 0.0;
-  require(!(p_given && t_given && r_given), "pressure, temperature and density cannot all be given (Inflow.cpp:367)");
-  require(t_given, "this build needs tref > 0 (the temperature sets the Sutherland constant 110.4/tref)");
+  const Reference ref = derive_reference();
+  require(ref.tref > 0.0, "the reference temperature must be positive (the Sutherland constant is 110.4/tref)");
   orion_inputs in{};
   in.moo = inflow.moo;
-  // Inflow.cpp:323-325
-  in.reynolds = (std::fabs(force.lfref) > 1e-30) ? (inflow.reynolds / force.lfref) * force.lref : inflow.reynolds;
-  in.tref = inflow.tref;
+  in.reynolds = ref.reynolds;
+  in.tref = ref.tref;
   in.twall = inflow.twall;
   in.cfl = step.cfl;
   in.attack = inflow.attack;
@@ -181,6 +274,11 @@ static std::vector<unsigned char> record(std::ifstream& in, const std::string& p
   uint32_t n1 = 0, n2 = 0;
   in.read(reinterpret_cast<char*>(&n1), 4);
   require(in.good(), path + ": end of file or bad record header");
+  const std::streampos here = in.tellg();
+  in.seekg(0, std::ios::end);
+  const std::streamoff left = in.tellg() - here;
+  in.seekg(here);
+  require((std::streamoff)n1 + 4 <= left, path + ": record longer than the file");
   std::vector<unsigned char> buf(n1);
   in.read(reinterpret_cast<char*>(buf.data()), n1);
   in.read(reinterpret_cast<char*>(&n2), 4);
@@ -189,7 +287,41 @@ static std::vector<unsigned char> record(std::ifstream& in, const std::string& p
   return buf;
 }
 
+// read_ascii_simple (src/mesh/Plot3DReader.cpp:162-208)
+static MultiBlockGrid read_plot3d_ascii(const std::string& path) {
+  std::ifstream in(path);
+  require(in.good(), "Failed to open ASCII grid file: " + path);
+  MultiBlockGrid g;
+  in >> g.nblocks;
+  require(in.good() && g.nblocks > 0, path + ": invalid block count in ASCII");
+  g.blocks.resize(g.nblocks);
+  for (auto& b : g.blocks) {
+    in >> b.idim >> b.jdim >> b.kdim;
+    require(in.good() && b.idim > 0 && b.jdim > 0 && b.kdim > 0, path + ": bad block sizes in ASCII");
+  }
+  for (auto& b : g.blocks) {
+    const size_t n = (size_t)b.npoints();
+    for (std::vector<double>* c : {&b.x, &b.y, &b.z}) {
+      c->resize(n);
+      for (size_t i = 0; i < n; ++i) in >> (*c)[i];
+    }
+    require(!in.fail(), path + ": bad coordinates in ASCII");
+  }
+  return g;
+}
+
+static MultiBlockGrid read_plot3d_binary(const std::string& path);
+
 MultiBlockGrid read_plot3d(const std::string& path) {
+  // the binary form first, the ASCII form if that fails (Plot3DReader.cpp:210-224)
+  try {
+    return read_plot3d_binary(path);
+  } catch (const std::exception&) {
+    return read_plot3d_ascii(path);
+  }
+}
+
+static MultiBlockGrid read_plot3d_binary(const std::string& path) {
   std::ifstream in(path, std::ios::binary);
   require(in.good(), "Failed to open grid file: " + path);
   MultiBlockGrid g;

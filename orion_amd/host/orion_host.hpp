@@ -64,11 +64,23 @@ struct Params {
   // path does not read are accepted and ignored.
   void load_namelist_file(const std::string& path);
 
-  // The inputs of the device context, derived as preprocess::init_inflow
-  // does (src/preprocess/Inflow.cpp:300-459) for the custom-atmosphere
-  // path (height < 0) with the reference temperature given: Reynolds number
-  // scaled by lref/lfref, Sutherland constant 110.4/tref.  The standard-
-  // atmosphere path (height >= 0) is outside this build and throws.
+  // The reference state as preprocess::init_inflow derives it
+  // (src/preprocess/Inflow.cpp:300-410): with height >= 0 the standard
+  // atmosphere at that height (km) gives temperature, pressure and speed of
+  // sound, the density follows from the gas constant of the perfect-gas air
+  // model, and the Reynolds number from the reference velocity moo * a and
+  // the Sutherland viscosity; with height < 0 two of pref, tref and rref
+  // give the third, and the Reynolds number is the input one.  (init_inflow
+  // scales it by params.inflow.lref / lfref, which the namelist never sets,
+  // so by 1: the $force lengths do not enter.)
+  struct Reference {
+    double reynolds, tref, pref, rref;
+  };
+  Reference derive_reference() const;
+
+  // The inputs of the device context: the derived Reynolds number and
+  // reference temperature (Sutherland constant 110.4 / tref) with the
+  // namelist's other inputs.
   orion_inputs to_inputs() const;
 };
 
@@ -87,9 +99,12 @@ struct MultiBlockGrid {
   std::vector<Block> blocks;
 };
 
-// Plot3DReader (src/mesh/Plot3DReader.cpp:17-156): Fortran sequential
-// records with 32-bit markers -- block count, one (i,j,k) record per block,
-// then one record of x, y, z per block in float or double.
+// Plot3DReader::read (src/mesh/Plot3DReader.cpp:17-224): Fortran
+// sequential records with 32-bit markers -- block count, one (i,j,k) record
+// per block, then one record of x, y, z per block in float or double; a
+// file that does not read as that is read as the ASCII form (block count,
+// the sizes, then every block's x, y and z as numbers separated by white
+// space), as the reference falls back.
 MultiBlockGrid read_plot3d(const std::string& path);
 
 }  // namespace mesh

@@ -11,7 +11,8 @@
 //            instead of the captured step sequence
 //   --hist   write one record of 8 doubles per step: rms, max, the 5 entries
 //            of max_loc, phydtime
-//   --check  parse the case only and print a summary (no device use)
+//   --check  parse the case only and print a summary (no device use): sizes,
+//            the derived inflow reference state and a coordinate checksum
 //   --restart  start from a state file written by --save (instead of the
 //            initial field); the steps continue its step count
 //   --save   write the state after the last step
@@ -64,11 +65,19 @@ int main(int argc, char** argv) {
     if (check_only) {
       long long nodes = 0;
       size_t nreg = 0;
-      for (const auto& b : grid.blocks) nodes += b.npoints();
+      double xyz_sum = 0.0;  // a checksum of the coordinates as read
+      for (const auto& b : grid.blocks) {
+        nodes += b.npoints();
+        for (const std::vector<double>* c : {&b.x, &b.y, &b.z})
+          for (double v : *c) xyz_sum += v;
+      }
       for (const auto& b : bcd.block_bc) nreg += b.regions.size();
+      const core::Params::Reference ref = params.derive_reference();
       std::printf("{\"blocks\": %d, \"nodes\": %lld, \"regions\": %zu, \"moo\": %.17g, \"reynolds\": %.17g, "
-                  "\"tref\": %.17g, \"cfl\": %.17g, \"nvis\": %d, \"gcl\": %d, \"steps\": %d}\n",
-                  grid.nblocks, nodes, nreg, in.moo, in.reynolds, in.tref, in.cfl, in.nvis, in.gcl, nsteps);
+                  "\"tref\": %.17g, \"pref\": %.17g, \"rref\": %.17g, \"cfl\": %.17g, \"nvis\": %d, \"gcl\": %d, "
+                  "\"steps\": %d, \"xyz_sum\": %.17g}\n",
+                  grid.nblocks, nodes, nreg, in.moo, in.reynolds, in.tref, ref.pref, ref.rref, in.cfl, in.nvis, in.gcl,
+                  nsteps, xyz_sum);
       return 0;
     }
     preprocess::FlowFieldSet fs(grid, bcd, params, device);

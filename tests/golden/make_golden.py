@@ -17,6 +17,10 @@ outputs as small .npz files:
                         order)
   bc_variety.npz        freestream/extrapolation/symmetry/slip-wall block
                         (nvis=0), 10 steps: history and final fields
+  inflow_reference.npz  the reference state init_inflow derives (Reynolds
+                        number, temperature, pressure, density) at 24
+                        standard-atmosphere heights and for the custom
+                        pressure / temperature / density combinations
   vts_bc_variety.npz    the files PostProcess::write_solution writes after
                         3 steps of the viscous bc_variety block (the VTS of
                         the block and the VTM index), byte for byte
@@ -117,14 +121,32 @@ def full_fixture(name):
     return d
 
 
+def inflow_fixture():
+    """The reference state init_inflow derives (ref_harness params.bin) for
+    every case of tests/test_case_io.py: standard-atmosphere heights and the
+    custom pressure / temperature / density combinations."""
+    from tests import test_case_io as io
+    labels, vals = [], {k: [] for k in ("reynolds", "tref", "pref", "rref")}
+    for label, inputs in io.inflow_cases():
+        tmp = tempfile.mkdtemp(prefix="orion_golden_")
+        ref = io.reference_params(io.case_with(inputs), tmp)
+        labels.append(label)
+        for k in vals:
+            vals[k].append(ref[k])
+    return dict(generator="tests.test_case_io.inflow_cases()", labels=np.array(labels),
+                **{k: np.array(v) for k, v in vals.items()})
+
+
 def fixtures(only=None):
     out = {}
+    if only and "inflow_reference" in only:
+        out["inflow_reference"] = inflow_fixture()
     if only and "vts_bc_variety" in only:
         out["vts_bc_variety"] = vts_fixture()
     for name in FULL:
         if only and name in only:
             out[name] = full_fixture(name)
-    if only and not set(only) - {"vts_bc_variety"} - set(FULL):
+    if only and not set(only) - {"vts_bc_variety", "inflow_reference"} - set(FULL):
         return out
     c = cases.channel_wall_case(seed=1, nsteps=200)
     hist, read = run(c, 200)
