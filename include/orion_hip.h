@@ -215,6 +215,24 @@ int orion_hip_write_vtm(const char* output_dir, const char* stem, int nblocks, i
 int orion_hip_save_state(orion_hip_ctx* ctx, const char* path, long long step);
 int orion_hip_load_state(orion_hip_ctx* ctx, const char* path, long long* step);
 
+/* No counterpart in the reference; for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  solver::ResidualMonitor::compute
+ * (src/solver/ResidualMonitor.hpp:20) over caller-held device fields, with
+ * the kernels of orion_hip_residual, allocating nothing: its scratch (a
+ * block table, partial sums and maxima, the result) is a fixed device array
+ * of about 130 KB that each GPU holds until the process ends.  nblocks
+ * blocks (1 to ORION_NOALLOC_MAX_BLOCKS, each under 2^31 interior nodes);
+ * dims holds 3 * nblocks interior sizes; dq[b], dt[b] and vol[b] are device
+ * pointers to block b's fields in the OrionArray layout of this header
+ * (dq: 5 variables, dt holding dt/vol as calculate_time_step leaves it).
+ * stream: a hipStream_t (NULL: the null stream).  out->phydtime is 0 (the
+ * time step forms it, not the monitor).  Synchronises the stream.  Calls
+ * on one GPU share its scratch: calls that overlap on one GPU race. */
+#define ORION_NOALLOC_MAX_BLOCKS 256
+int orion_hip_residual_noalloc(int nblocks, const int* dims, const double* const* dq, const double* const* dt,
+                               const double* const* vol, void* stream, orion_residual_stats* out);
+
 int orion_hip_synchronize(orion_hip_ctx* ctx);
 /* Use a caller-owned hipStream_t (NULL restores the context's own stream). */
 int orion_hip_set_stream(orion_hip_ctx* ctx, void* stream);

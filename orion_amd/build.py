@@ -20,7 +20,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(BUILD, "liborion_hip.so")
-SOURCES = ["context.hip", "io.hip", "k_flux.hip", "k_point.hip", "k_bc.hip", "k_lusgs.hip", "k_metrics.hip"]
+SOURCES = ["context.hip", "io.hip", "k_flux.hip", "k_point.hip", "k_bc.hip", "k_lusgs.hip", "k_metrics.hip",
+           "noalloc.hip"]
 HOST = os.path.join(HERE, "host")
 HOST_SOURCES = ["orion_host.cpp", "orion_run.cpp"]
 RUNNER = os.path.join(BUILD, "orion_run")
