@@ -85,6 +85,8 @@ def _open():
         "orion_hip_upload": ([P, C.c_int, C.c_int, C.POINTER(C.c_double), C.c_longlong], C.c_int),
         "orion_hip_download": ([P, C.c_int, C.c_int, C.POINTER(C.c_double), C.c_longlong], C.c_int),
         "orion_hip_copy_block_state": ([P, C.c_int, P, C.c_int], C.c_int),
+        "orion_hip_residual_noalloc": ([C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
+                                        C.POINTER(C.c_void_p), C.c_void_p, C.POINTER(ResidualStats)], C.c_int),
         "orion_hip_update_temperature": ([P], C.c_int),
         "orion_hip_exchange_bc": ([P], C.c_int),
         "orion_hip_apply_physical_bc": ([P], C.c_int),
