@@ -42,8 +42,11 @@ def test_reads_case_files(name, tmp_path):
 def test_namelist_forms(tmp_path):
     """Fortran forms the reference's reader accepts (Params.cpp:12-100,
     :238-268): '&' groups, comma-separated items, '!' comments, 'd'
-    exponents, quoted strings; and the Reynolds scaling by lref/lfref
-    (Inflow.cpp:323-325)."""
+    exponents, quoted strings.  The $force lengths do not scale the
+    Reynolds number: init_inflow scales by params.inflow.lref / lfref
+    (Inflow.cpp:323-325), which the namelist never sets (the $force values
+    go to params.force, Params.cpp:150-153), so by 1 -- checked against
+    the reference build where it is present."""
     case = _cases.channel_small(nsteps=3)
     case.write(str(tmp_path))
     text = open(tmp_path / "param.dat").read()
@@ -52,7 +55,16 @@ def test_namelist_forms(tmp_path):
     text = text.replace("gridname = 'grid.x'", "gridname = \"grid.x\"  ! quoted")
     open(tmp_path / "param.dat", "w").write(text)
     got = _check(str(tmp_path))
-    assert got["reynolds"] == (5.0e3 / 2.0) * 0.5
+    assert got["reynolds"] == 5.0e3
+    from oracle import binding
+    if binding.have_reference():
+        out = tmp_path / "ref"
+        out.mkdir()
+        env = dict(os.environ)
+        env.pop("LD_LIBRARY_PATH", None)
+        subprocess.run([binding.REF_HARNESS, str(tmp_path), "0", str(out), "0"], check=True,
+                       stdout=subprocess.DEVNULL, env=env)
+        assert np.fromfile(str(out / "params.bin"))[1] == got["reynolds"]
 
 
 def test_rejects_what_the_reference_rejects(tmp_path):
@@ -64,9 +76,10 @@ def test_rejects_what_the_reference_rejects(tmp_path):
     assert r.returncode == 1 and "marker" in r.stderr
     case.write(str(tmp_path))
     p = open(tmp_path / "param.dat").read()
-    open(tmp_path / "param.dat", "w").write(p.replace("height = -1.0", "height = 10.0"))
+    # pressure, temperature and density all given (Inflow.cpp:367-370)
+    open(tmp_path / "param.dat", "w").write(p.replace("pref = -1.0", "pref = 101325.0"))
     r = subprocess.run([_runner(), str(tmp_path), "--check"], capture_output=True, text=True, timeout=60)
-    assert r.returncode == 1 and "standard atmosphere" in r.stderr
+    assert r.returncode == 1 and "cannot all be given" in r.stderr
 
 
 @pytest.mark.gpu
