@@ -1,24 +1,34 @@
 #!/usr/bin/env python3
 """Benchmark: FP64 cell-steps/s of ORION's pseudo-time step on one MI355X.
 
-Workload (default): the uneven 64-block mesh of about 8M interior nodes
-(4x4x4 lattice, per-axis block sizes log-uniform in [17,129], seeded), M=0.5,
-Re=5e3, CFL 5, WCNS-E-5 + Roe inviscid flux, 4th/6th-order viscous flux,
-LU-SGS with two extra symmetric Gauss-Seidel sweeps, FP64, synthetic
-seeded initial field.  ``--workload eight_block`` runs 8 blocks of 129x65x65
-(M=0.8, Re=1e5, CFL 10) instead.
+Workload (default): the 64-block mesh of about 8.05M interior nodes on a
+seeded block-adjacency graph with partial-face interface windows
+(cases.column_case(seed=3): 4x4 columns of 4 blocks, staggered block
+edges), M=0.5, Re=5e3, CFL 5, WCNS-E-5 + Roe inviscid flux, 4th/6th-order
+viscous flux, LU-SGS with two extra symmetric Gauss-Seidel sweeps, FP64,
+synthetic seeded initial field.  ``--workload uneven64`` runs the 4x4x4
+lattice of uneven blocks (7.85M nodes), ``--workload eight_block`` 8 blocks
+of 129x65x65 (M=0.8, Re=1e5, CFL 10).
 
 A step is one full pseudo-time step over every block (halo, boundary
 conditions, flow states, time step, viscous and inviscid right-hand sides
 with interface averaging, LU-SGS update, residual).  value = interior nodes x
 timed steps / wall time of the timed steps, with all state resident in HBM.
 
+value is the median of 5 timed runs of --steps steps, each from the same
+state, in the batched form (every kernel over all blocks, the step replayed
+as a captured graph); the plain per-block form (a launch per block per
+kernel) is timed beside it from the same state and its results compared.
+
 Prints one JSON line.  The roofline entry is the dominant kernel family,
 timed live with HIP events on the solver's stream.  The CPU baseline is the
 reference's own MPI CPU path (the reference solver built from its sources,
-oracle/_ref) on 16 ranks of this host for a few steps; the C oracle (a port
-of the reference, OpenMP) is timed beside it as cpu_port.  Both samples are
-bounded.
+oracle/_ref) on as many ranks of this host as the mesh allows (up to 16;
+the reference's interface-residual messages are tagged by source block, so
+a rank may hold at most one interface partner of any block of another
+rank), for a few steps from the same start state, its residuals compared
+with the GPU's; the C oracle (a port of the reference, OpenMP) is timed
+beside it as cpu_port.  Both samples are bounded.
 """
 import argparse
 import json
@@ -26,6 +36,8 @@ import os
 import subprocess
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -307,16 +319,18 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--runs", type=int, default=5, help="timed runs of --steps steps each; value is their median")
+    ap.add_argument("--per-block-steps", type=int, default=5,
+                    help="steps of the plain per-block form timed beside the batched form (0: none)")
     ap.add_argument("--tune-steps", type=int, default=3,
                     help="steps per trial of the sweep-schedule autotuner after the warmup (0: no tuning)")
     ap.add_argument("--schedule", type=int, default=-1,
                     help="fix the sweep schedule (0 diagonal, 1 critical path) instead of tuning")
-    ap.add_argument("--workload", default="uneven64")
+    ap.add_argument("--workload", default="columns64")
     ap.add_argument("--profile-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
-    ap.add_argument("--ref-steps", type=int, default=3, help="steps of the reference CPU baseline (0: port only)")
+    ap.add_argument("--ref-steps", type=int, default=2, help="steps of the reference CPU baseline (0: port only)")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
     ap.add_argument("--serve-requests", type=int, default=16)
     ap.add_argument("--serve-slots", type=int, default=16)
@@ -376,6 +390,37 @@ def main():
             r, a.steps, run_t[-1], hist[0].rms_residual, hist[-1].rms_residual), file=sys.stderr, flush=True)
     t = sorted(run_t)[len(run_t) // 2]
     value = N * a.steps / t
+    plain = None
+    if a.per_block_steps > 0 and start is not None:
+        # the plain per-block form (a launch per block per kernel, no captured
+        # step) from the same state, against the batched run from that state
+        n = min(a.per_block_steps, a.steps)
+        for b in range(case.nblocks):
+            s.fs.set_field(b, "prim", start[b])
+        ref = s.solve(n)
+        end_batched = [s.fs.field(b, "prim") for b in range(case.nblocks)]
+        for b in range(case.nblocks):
+            s.fs.set_field(b, "prim", start[b])
+        s.fs.set_per_block(True)
+        s.solve(1)  # untimed: first launches of the form
+        for b in range(case.nblocks):
+            s.fs.set_field(b, "prim", start[b])
+        s.fs.synchronize()
+        t0 = time.perf_counter()
+        hp = s.solve(n)
+        s.fs.synchronize()
+        tp = time.perf_counter() - t0
+        same = ([h.rms_residual for h in hp] == [h.rms_residual for h in ref]
+                and all(np.array_equal(s.fs.field(b, "prim"), end_batched[b]) for b in range(case.nblocks)))
+        s.fs.set_per_block(False)
+        for b in range(case.nblocks):
+            s.fs.set_field(b, "prim", start[b])
+        plain = {"form": "plain per-block: every kernel launched once per block, in block order, one stream, "
+                         "no captured step", "steps": n, "ms_per_step": 1e3 * tp / n, "value": N * n / tp,
+                 "unit": "cell-steps/s", "batched_over_plain": (N * a.steps / t) / (N * n / tp),
+                 "same_results_as_batched": bool(same)}
+        print("[bench] plain per-block form: %.3f ms/step (batched %.3f), same results: %s" % (
+            1e3 * tp / n, 1e3 * t / a.steps, same), file=sys.stderr, flush=True)
     phase = profile_phases(s.fs, a.profile_steps)
     phases = {k: round(v[0], 4) for k, v in phase.items()}
     print("[bench] phase ms/step: " + json.dumps(phases), file=sys.stderr, flush=True)
@@ -412,6 +457,7 @@ def main():
         "residual_first_last": [hist[0].rms_residual, hist[-1].rms_residual],
         "runs_ms_per_step": [round(1e3 * v / a.steps, 4) for v in run_t],
         "timing": "median of %d timed runs of %d steps, each from the same state" % (a.runs, a.steps),
+        "plain_per_block": plain,
     }
     s.fs.close()
     if not a.no_serve:

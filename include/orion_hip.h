@@ -244,6 +244,17 @@ int orion_hip_set_stream(orion_hip_ctx* ctx, void* stream);
  * context sharing the hardware queues).  The kernels and the order of the
  * updates of dq are the same either way, so are the results. */
 int orion_hip_set_overlap(orion_hip_ctx* ctx, int on);
+/* No counterpart in the reference.  On: the plain per-block form -- every
+ * kernel of a step launched once per block, in block order, as the
+ * reference's operators loop over their blocks (e.g.
+ * StateUpdater::update_solution, src/solver/StateUpdater.cpp:544-580), on
+ * one stream and without the captured step; the boundary, halo and
+ * averaging gathers stay one launch each.  Off (the default): the batched
+ * form -- each kernel covers every block, the LU-SGS tiles of all blocks
+ * share one launch per sweep, and orion_hip_run replays the step captured
+ * as a graph.  The two forms give the same results bit for bit;
+ * orion_hip_profile_run measures the batched form only. */
+int orion_hip_set_per_block(orion_hip_ctx* ctx, int on);
 /* Device bytes held by the context (arena plus maps). */
 size_t orion_hip_device_bytes(const orion_hip_ctx* ctx);
 /* Interior node count (the unit of cell-steps). */

@@ -85,6 +85,7 @@ def _open():
         "orion_hip_upload": ([P, C.c_int, C.c_int, C.POINTER(C.c_double), C.c_longlong], C.c_int),
         "orion_hip_download": ([P, C.c_int, C.c_int, C.POINTER(C.c_double), C.c_longlong], C.c_int),
         "orion_hip_copy_block_state": ([P, C.c_int, P, C.c_int], C.c_int),
+        "orion_hip_set_per_block": ([P, C.c_int], C.c_int),
         "orion_hip_residual_noalloc": ([C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
                                         C.POINTER(C.c_void_p), C.c_void_p, C.POINTER(ResidualStats)], C.c_int),
         "orion_hip_update_temperature": ([P], C.c_int),

@@ -60,6 +60,7 @@ int fail(const char* fmt, ...) {
 
 struct Space {
   std::vector<long long> h;  // prefix offsets (nb+1)
+  std::vector<int> hwg0;     // first workgroup of each block (host copy)
   long long* d = nullptr;
   int* wgb = nullptr;        // block of each workgroup of the padded space
   int* wg0 = nullptr;        // first workgroup of each block
@@ -133,6 +134,12 @@ struct orion_hip_ctx {
   hipStream_t graph_stream = nullptr;
   int use_graph = 1;
   bool overlap = true;  // inviscid face fluxes on side_stream (viscous steps)
+  // plain per-block launches: every kernel once per block, in block order,
+  // no captured step (orion_hip_set_per_block)
+  bool per_block = false;
+  int* d_zeros = nullptr;               // zero workgroup tables of one-block spans
+  std::vector<int> tile_first, tile_count;  // each block's tiles in `tiles`
+  int* d_border[2] = {nullptr, nullptr};   // per-block tile orders at the tiles' positions
 };
 
 // Device allocations of a context, freed with it.  Tables up to 64 KB share
@@ -195,7 +202,20 @@ static int make_space(orion_hip_ctx* c, Space& s, const std::vector<long long>& 
   }
   if (wgb.size() >= (size_t)INT32_MAX) return fail("iteration space too large");
   s.nwg = (long long)wgb.size();
+  s.hwg0 = wg0;
   return upload_vec(c, &s.d, s.h) || upload_vec(c, &s.wgb, wgb) || upload_vec(c, &s.wg0, wg0);
+}
+
+// workgroups and items of block b in space s
+static long long nwg_of(const Space& s, int b) {
+  const long long hi = b + 1 < (int)s.hwg0.size() ? s.hwg0[b + 1] : s.nwg;
+  return hi - s.hwg0[b];
+}
+static long long items_of(const Space& s, int b) { return s.h[b + 1] - s.h[b]; }
+// one block of a space as a launch of its own: the block table and the
+// prefix start at block b, and every workgroup maps to block 0 of that view
+static Launch LAB(orion_hip_ctx* c, const Space& s, int b) {
+  return Launch{c->stream, c->db + b, s.d + b, 1, Span{s.d + b, c->d_zeros, c->d_zeros, 1}, nwg_of(s, b)};
 }
 
 static Launch LA(orion_hip_ctx* c, const Space& s, hipStream_t st = nullptr) {
@@ -615,6 +635,27 @@ static int build_sweep(orion_hip_ctx* c) {
     std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return c->tiles[a].s_last > c->tiles[b].s_last; });
     c->sched_order[1][1] = ord;
   }
+  // per-block launches: each block's tiles by diagonal, at the positions of
+  // its tiles (forward sweeps; reversed for the backward ones)
+  {
+    c->tile_first.assign(c->nb, 0);
+    c->tile_count.assign(c->nb, 0);
+    std::vector<int> f(nt), bk(nt);
+    for (int i = nt - 1; i >= 0; --i) c->tile_first[c->tiles[i].block] = i;
+    for (int i = 0; i < nt; ++i) c->tile_count[c->tiles[i].block]++;
+    for (int b = 0; b < c->nb; ++b) {
+      const int f0 = c->tile_first[b], n = c->tile_count[b];
+      std::vector<int> o(n);
+      for (int i = 0; i < n; ++i) o[i] = f0 + i;
+      auto diag = [&](int i) { return c->tiles[i].r0 / TRD + c->tiles[i].c0 / TWD; };
+      std::stable_sort(o.begin(), o.end(), [&](int a, int b2) { return diag(a) < diag(b2); });
+      for (int i = 0; i < n; ++i) {
+        f[f0 + i] = o[i];
+        bk[f0 + i] = o[n - 1 - i];
+      }
+    }
+    if (upload_vec(c, &c->d_border[0], f) || upload_vec(c, &c->d_border[1], bk)) return -1;
+  }
   c->schedule = 1;  // critical path first until a tuner says otherwise
   if (upload_vec(c, &c->d_order[0], c->sched_order[1][0])) return -1;
   if (upload_vec(c, &c->d_order[1], c->sched_order[1][1])) return -1;
@@ -920,6 +961,20 @@ static int create_ctx(int nreq, const int* req_nblocks, int nblocks, const int* 
     }
     if (make_space(c, c->ptile, ct)) return bail(-1);
   }
+  {
+    // the zero workgroup tables of one-block launches: as long as the most
+    // workgroups any block has in any space
+    long long most = 1;
+    std::vector<const Space*> all = {&c->pad, &c->intr, &c->dtile, &c->ptile};
+    for (int d = 0; d < 3; ++d) {
+      all.push_back(&c->face[d]);
+      all.push_back(&c->fpair[d]);
+      all.push_back(&c->ltask[d]);
+    }
+    for (const Space* sp : all)
+      for (int b = 0; b < nblocks; ++b) most = std::max(most, nwg_of(*sp, b));
+    if (upload_vec(c, &c->d_zeros, std::vector<int>((size_t)most, 0))) return bail(-1);
+  }
   if (upload_vec(c, &c->db, c->hb)) return bail(-1);
   if (build_topology(c, R)) return bail(-1);
   c->nparts = residual_parts(c->intr.total());
@@ -1080,6 +1135,15 @@ int orion_hip_update_temperature(orion_hip_ctx* c) {
   return check_launch("update_temperature");
 }
 
+// the state kernel over the padded points (per block when per_block)
+static void op_state(orion_hip_ctx* c, int mode, unsigned long long* dtmin, const int* step) {
+  if (c->per_block) {
+    for (int b = 0; b < c->nb; ++b) launch_state(LAB(c, c->pad, b), items_of(c->pad, b), mode, dtmin, step, c->nreq);
+    return;
+  }
+  launch_state(LA(c, c->pad), c->pad.total(), mode, dtmin, step, c->nreq);
+}
+
 static void op_halo(orion_hip_ctx* c) { launch_halo(c->d_halo, c->n_halo, (double*)c->arena, c->stream); }
 
 static void op_bc(orion_hip_ctx* c) {
@@ -1092,6 +1156,14 @@ static void op_bc(orion_hip_ctx* c) {
 // accumulate into dq as the reference does
 static void op_visc(orion_hip_ctx* c, bool first = false) {
   if (c->p.nvis != 1) return;
+  if (c->per_block) {
+    for (int b = 0; b < c->nb; ++b) {
+      launch_visc_deriv(LAB(c, c->intr, b), items_of(c->intr, b));
+      for (int d = 0; d < 3; ++d) launch_visc_face(LAB(c, c->fpair[d], b), d, items_of(c->fpair[d], b));
+      launch_visc_diff(LAB(c, c->dtile, b), items_of(c->dtile, b), first);
+    }
+    return;
+  }
   launch_visc_deriv(LA(c, c->intr), c->intr.total());
   launch_visc_face(LA3(c, c->fpair3), -1, c->fpair3.total());
   launch_visc_diff(LA(c, c->dtile), c->dtile.total(), first);
@@ -1110,6 +1182,16 @@ static void op_inv_faces(orion_hip_ctx* c, hipStream_t st) {
 }
 
 static void op_inv(orion_hip_ctx* c, bool first = false) {
+  if (c->per_block) {
+    for (int b = 0; b < c->nb; ++b) {
+      for (int d = 0; d < 3; ++d) {
+        launch_inv_face(LAB(c, c->face[d], b), d, items_of(c->face[d], b), c->p.gama);
+        launch_inv_closure(LAB(c, c->ltask[d], b), d, items_of(c->ltask[d], b), c->p.gama);
+      }
+      launch_inv_diff(LAB(c, c->dtile, b), items_of(c->dtile, b), first);
+    }
+    return;
+  }
   op_inv_faces(c, c->stream);
   launch_inv_diff(LA(c, c->dtile), c->dtile.total(), first);
 }
@@ -1121,6 +1203,23 @@ static void op_sweep(orion_hip_ctx* c, int type) {
 }
 
 static void op_update(orion_hip_ctx* c) {
+  if (c->per_block) {
+    // the reference's update_solution loop: pack, four sweeps and the
+    // limited update of one block, then the next (StateUpdater.cpp:544-580);
+    // the partials go to the block's own slots of the step's partials
+    for (int b = 0; b < c->nb; ++b) {
+      launch_lu_pack(LAB(c, c->ptile, b), items_of(c->ptile, b), c->sw);
+      for (int t = 0; t < 4; ++t) {
+        const int bwd = (t == 1 || t == 3) ? 1 : 0;
+        launch_lusgs_tiles(c->sw, c->db, c->d_tiles, c->d_border[bwd] + c->tile_first[b], c->tile_count[b],
+                           c->d_flags, c->d_epoch, c->d_err, t, c->p, c->stream);
+      }
+      const long long w0 = c->ptile.hwg0[b];
+      launch_update(LAB(c, c->ptile, b), items_of(c->ptile, b), c->sw, c->p.gama, c->intr.d + b, c->d_upart + w0,
+                    c->d_upart + c->ptile.nwg + w0, c->d_ukey + w0, c->d_epoch);
+    }
+    return;
+  }
   launch_lu_pack(LA(c, c->ptile), c->ptile.total(), c->sw);
   for (int t = 0; t < 4; ++t) op_sweep(c, t);
   launch_update(LA(c, c->ptile), c->ptile.total(), c->sw, c->p.gama, c->intr.d, c->d_upart,
@@ -1189,7 +1288,7 @@ int orion_hip_apply_physical_bc(orion_hip_ctx* c) {
   return check_launch("apply_physical_bc");
 }
 int orion_hip_update_flow_states(orion_hip_ctx* c) {
-  launch_state(LA(c, c->pad), c->pad.total(), 1, nullptr, nullptr, c->nreq);
+  op_state(c, 1, nullptr, nullptr);
   return check_launch("update_flow_states");
 }
 int orion_hip_calculate_time_step(orion_hip_ctx* c, double* phydtime) {
@@ -1199,7 +1298,7 @@ int orion_hip_calculate_time_step(orion_hip_ctx* c, double* phydtime) {
   unsigned long long bits;
   std::memcpy(&bits, &init, sizeof bits);
   HIPCHK(hipMemcpyAsync(c->d_dtmin, &bits, sizeof bits, hipMemcpyHostToDevice, c->stream));
-  launch_state(LA(c, c->pad), c->pad.total(), 2, c->d_dtmin, nullptr, c->nreq);
+  op_state(c, 2, c->d_dtmin, nullptr);
   if (check_launch("calculate_time_step")) return -1;
   if (phydtime) {
     HIPCHK(hipMemcpyAsync(&bits, c->d_dtmin, sizeof bits, hipMemcpyDeviceToHost, c->stream));
@@ -1244,6 +1343,21 @@ int orion_hip_residual(orion_hip_ctx* c, orion_residual_stats* out) {
 static void step_body(orion_hip_ctx* c) {
   op_halo(c);
   op_bc(c);
+  if (c->per_block) {
+    // one stream, the operators in the reference's order, each per block
+    op_state(c, 3 | 4, c->d_dtmin, c->d_step);
+    if (c->p.nvis == 1) {
+      op_visc(c, true);
+      op_avg(c);
+      op_inv(c, false);
+    } else {
+      op_inv(c, true);
+    }
+    op_avg(c);
+    op_update(c);
+    op_update_residual(c, c->d_res, c->d_step);
+    return;
+  }
   // mode 4 of the state kernel: the first right-hand-side kernel after it
   // writes dq whole
   if (c->p.nvis == 1 && !c->overlap) {
@@ -1313,7 +1427,7 @@ int orion_hip_run(orion_hip_ctx* c, int nsteps, orion_residual_stats* hist) {
     HIPCHK(hipMemcpyAsync(c->d_step, &zero, sizeof zero, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
   }
-  if (c->use_graph) {
+  if (c->use_graph && !c->per_block) {
     if (ensure_graph(c)) return -1;
     for (int s = 0; s < nsteps; ++s) HIPCHK(hipGraphLaunch(c->gexec, c->stream));
   } else {
@@ -1333,6 +1447,7 @@ int orion_hip_run(orion_hip_ctx* c, int nsteps, orion_residual_stats* hist) {
 
 int orion_hip_profile_run(orion_hip_ctx* c, int nsteps, double* phase_ms, long long* phase_launches) {
   if (nsteps <= 0) return 0;
+  if (c->per_block) return fail("profile_run times the all-blocks launches; switch per-block launches off first");
   if (ensure_res(c, nsteps)) return -1;
   const int NP = ORION_PHASES;
   hipEvent_t ev[ORION_PHASES + 1];
@@ -1571,6 +1686,12 @@ int orion_hip_set_overlap(orion_hip_ctx* c, int on) {
   if ((on != 0) == c->overlap) return 0;
   c->overlap = on != 0;
   c->graph_stream = nullptr;  // the captured step has the other form: capture again
+  return 0;
+}
+
+int orion_hip_set_per_block(orion_hip_ctx* c, int on) {
+  if (!c) return fail("set_per_block: null context");
+  c->per_block = on != 0;
   return 0;
 }
 

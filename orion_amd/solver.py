@@ -171,6 +171,13 @@ class FlowFieldSet:
         stream (many contexts busy at once); the results are the same."""
         check(_lib.load().orion_hip_set_overlap(self.ctx, int(bool(on))), "set_overlap")
 
+    def set_per_block(self, on: bool) -> None:
+        """The plain per-block form (every kernel once per block, in block
+        order, no captured step) or the batched form (the default: each
+        kernel over all blocks, the step replayed as a graph); the results
+        are the same bit for bit."""
+        check(_lib.load().orion_hip_set_per_block(self.ctx, int(bool(on))), "set_per_block")
+
     SCHEDULES = ("diagonal", "critical_path")
 
     def set_schedule(self, which: int) -> None:

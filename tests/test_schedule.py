@@ -1,4 +1,5 @@
-"""The LU-SGS tile launch orders and their autotuner.
+"""The LU-SGS tile launch orders, their autotuner, and the plain per-block
+launch form.
 
 Every schedule launches a tile after the tiles it waits for, and each
 cell's arithmetic does not depend on the order, so the results must be the
@@ -9,6 +10,7 @@ import numpy as np
 import pytest
 
 from orion_amd import cases
+from tests import _cases
 
 
 def _case():
@@ -62,3 +64,29 @@ def test_uneven_sizes_search_is_bounded():
     assert min(sx + sy + sz) >= 17 and max(sx + sy + sz) <= 26
     with pytest.raises(ValueError):  # unreachable from sizes 17-129
         cases.uneven_sizes(11, target=6.0e5, tries=500)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["columns_small", "lattice_small", "bc_variety_inviscid"])
+def test_per_block_form_equals_batched(name):
+    """The plain per-block form (a launch per block per kernel, no captured
+    step) and the batched form (all blocks per launch, captured step) give
+    the same residual histories and fields bit for bit, from the same state
+    and through a switch mid-run."""
+    from orion_amd.solver import OrionSolver
+    case = _cases.ALL[name](nsteps=6)
+    print("\n[gpu] per-block vs batched:", name, flush=True)
+    a, b = OrionSolver(case), OrionSolver(case)
+    b.fs.set_per_block(True)
+    ha, hb = a.solve(3), b.solve(3)
+    b.fs.set_per_block(False)
+    a.fs.set_per_block(True)
+    ha += a.solve(3)
+    hb += b.solve(3)
+    assert [h.rms_residual for h in ha] == [h.rms_residual for h in hb]
+    assert [h.max_loc for h in ha] == [h.max_loc for h in hb]
+    for blk in range(case.nblocks):
+        for f in ("prim", "dq", "dt"):
+            assert np.array_equal(a.fs.field(blk, f), b.fs.field(blk, f)), (blk, f)
+    a.fs.close()
+    b.fs.close()
