@@ -87,7 +87,14 @@ class Case:
         return np.asarray([r.as_ints() for r in self.regions], dtype=np.int32).reshape(-1, REGION_INTS)
 
     def xyz_flat(self) -> np.ndarray:
-        return np.concatenate([a.reshape(-1) for a in self.xyz]).astype(np.float64)
+        """Every block's x, y, z in order (cached: a serving batch rebuilt
+        around a request asks for it again)."""
+        key = (id(self.xyz), len(self.xyz))
+        cached = getattr(self, "_xyz_cache", None)
+        if cached is None or cached[0] != key:
+            cached = (key, np.concatenate([a.reshape(-1) for a in self.xyz]).astype(np.float64))
+            object.__setattr__(self, "_xyz_cache", cached)
+        return cached[1]
 
     def freestream(self):
         """Freestream primitive state (Inflow.cpp init_inflow, custom atmosphere)."""
@@ -773,3 +780,156 @@ def column_case(seed: int = 3, ncol=(4, 4), per_column: int = 4, lo: int = 17, h
         rm.ibcwin = where[id(rn)]
     inputs = dict(DEFAULT_INPUTS, moo=moo, reynolds=reynolds, cfl=cfl)
     return Case("%s_seed%d" % (name, used), dims, xyz, regions, inputs, nsteps, rhos)
+
+
+def split_along_i(case: Case, nblocks: int, name: Optional[str] = None) -> Case:
+    """One block cut along i into ``nblocks`` blocks that share their cut
+    planes (point-matched), joined by 1:1 full-face windows.  The physical
+    regions and the block's self-interfaces (periodic windows of a constant
+    k or j plane) are split with it; the i-end regions go to the first and
+    last blocks.  Each piece keeps at least 9 nodes along i."""
+    if case.nblocks != 1:
+        raise ValueError("split_along_i takes a single-block case")
+    ni, nj, nk = (int(v) for v in case.dims[0])
+    nb = max(1, min(int(nblocks), (ni - 1) // 8))
+    cuts = [int(round(q)) for q in np.linspace(0, ni - 1, nb + 1)]
+    x = case.xyz[0]
+    rho = case.init_rho[0] if case.init_rho is not None else None
+    pres = case.init_p[0] if case.init_p is not None else None
+    dims, xyz, rhos, ps = [], [], [], []
+    for b in range(nb):
+        i0, i1 = cuts[b], cuts[b + 1]
+        dims.append([i1 - i0 + 1, nj, nk])
+        xyz.append(np.ascontiguousarray(x[:, :, :, i0:i1 + 1]))
+        if rho is not None:
+            rhos.append(np.ascontiguousarray(rho[:, :, i0:i1 + 1]))
+        if pres is not None:
+            ps.append(np.ascontiguousarray(pres[:, :, i0:i1 + 1]))
+    per: List[List[Region]] = [[] for _ in range(nb)]
+    for r in case.regions:
+        st, ed = list(r.s_st), list(r.s_ed)
+        if st[0] == ed[0]:  # an i-end face
+            b = 0 if st[0] == 1 else nb - 1
+            if b == nb - 1:
+                st[0] = ed[0] = dims[b][0]
+            if r.bctype < 0:
+                raise ValueError("split_along_i: interfaces on the i ends are not split")
+            per[b].append(Region(b, r.bctype, tuple(st), tuple(ed)))
+            continue
+        for b in range(nb):
+            i0 = cuts[b]
+            s2, e2 = list(st), list(ed)
+            s2[0], e2[0] = 1, dims[b][0]
+            if r.bctype < 0:  # a self-interface of the block: of each piece with itself
+                t2, u2 = list(r.t_st), list(r.t_ed)
+                t2[0], u2[0] = 1, dims[b][0]
+                per[b].append(Region(b, -1, tuple(s2), tuple(e2), tuple(t2), tuple(u2), nbt=b + 1, ibcwin=0))
+            else:
+                per[b].append(Region(b, r.bctype, tuple(s2), tuple(e2)))
+    # the cut planes
+    for b in range(nb - 1):
+        d0, d1 = dims[b], dims[b + 1]
+        ra = Region(b, -1, (d0[0], 1, 1), (d0[0], nj, nk), (1, 1, 1), (1, nj, nk), nbt=b + 2)
+        rb = Region(b + 1, -1, (1, 1, 1), (1, nj, nk), (d0[0], 1, 1), (d0[0], nj, nk), nbt=b + 1)
+        per[b].append(ra)
+        per[b + 1].insert(0, rb)
+    # face order within each block: imin, imax, jmin, jmax, kmin, kmax
+    def face_key(r: Region, d):
+        for ax in range(3):
+            if r.s_st[ax] == r.s_ed[ax]:
+                return 2 * ax + (0 if r.s_st[ax] == 1 and d[ax] > 1 else 1)
+        return 6
+    for b in range(nb):
+        per[b].sort(key=lambda r: face_key(r, dims[b]))
+    # partner windows: interfaces between pieces and each piece's self-interfaces
+    for b in range(nb):
+        for r in per[b]:
+            if r.bctype >= 0:
+                continue
+            m = r.nbt - 1
+            for w, pr in enumerate(per[m]):
+                if pr is not r or m == b:
+                    if (pr.bctype < 0 and pr.nbt == b + 1 and tuple(pr.s_st) == tuple(r.t_st)
+                            and tuple(pr.s_ed) == tuple(r.t_ed)):
+                        r.ibcwin = w + 1
+                        break
+            else:
+                raise RuntimeError("split_along_i: unmatched interface")
+    regions = [r for regs in per for r in regs]
+    return Case(name or "%s_x%d" % (case.name, nb), np.array(dims, dtype=np.int32), xyz, regions,
+                dict(case.inputs), case.nsteps, rhos or None, ps or None)
+
+
+def bump_request(seed: int, nodes: float, nblocks: int, moo: float, reynolds: float, bump: float,
+                 upper_wall: bool, nsteps: int = 1, cfl: float = 5.0) -> Case:
+    """A steady channel-bump request: bump_channel_case of about ``nodes``
+    interior nodes cut along i into ``nblocks`` point-matched blocks."""
+    c = bump_channel_case(seed, nodes, moo, reynolds, bump, upper_wall=upper_wall, nsteps=nsteps, cfl=cfl)
+    return split_along_i(c, nblocks) if nblocks > 1 else c
+
+
+def capacity_case(seed: int = 5, nsteps: int = 1, nbi: int = 4, nbk: int = 12, block=(257, 129, 21),
+                  bump: float = 0.08, moo: float = 0.85, reynolds: float = 1.0e6, cfl: float = 20.0,
+                  amp: float = 0.05) -> Case:
+    """The capacity workload: nbi x nbk blocks of ``block`` nodes (48 of
+    257x129x21, 33.4M nodes) forming a transonic channel over a cosine bump
+    (height ``bump`` on the lower wall over the middle of the channel):
+    blocks cut along i and k, joined by point-matched 1:1 full-face
+    windows; viscous lower wall, far-field inflow, outflow and upper
+    boundary, symmetry at the two k ends; nodes perturbed by a seeded
+    sinusoid of ``amp`` x spacing (zero on the lower wall and the k ends);
+    M 0.85, Re 1e6, CFL 20, seeded 1e-3 density noise."""
+    bi, bj, bk = block
+    NI, NK = nbi * (bi - 1) + 1, nbk * (bk - 1) + 1
+    rng = np.random.default_rng(seed)
+    ph = rng.uniform(0.0, 2 * np.pi, size=3)
+    Lx, Lz = 4.0, 4.0 * (NK - 1) / (NI - 1)
+    g = 1.03 ** np.arange(bj - 1)
+    eta = np.concatenate([[0.0], np.cumsum(g)]) / g.sum()
+
+    def coords(I, J, K):
+        x = I * (Lx / (NI - 1))
+        z = K * (Lz / (NK - 1))
+        t = np.clip((x - 1.5) / 1.0, 0.0, 1.0)
+        ylow = bump * 0.5 * (1.0 - np.cos(2.0 * np.pi * t))
+        e = eta[J.astype(int)]
+        y = ylow + e * (1.0 - ylow)
+        damp = e * (1.0 - e) * np.sin(np.pi * K / (NK - 1))
+        hx, hy = Lx / (NI - 1), 1.0 / (bj - 1)
+        x = x + amp * hx * damp * np.sin(2 * np.pi * 3 * I / (NI - 1) + ph[0])
+        y = y + amp * hy * damp * np.sin(2 * np.pi * 2 * K / (NK - 1) + ph[1])
+        return x, y, z
+
+    blocks = [(a, c) for c in range(nbk) for a in range(nbi)]
+    dims = np.array([[bi, bj, bk]] * len(blocks), dtype=np.int32)
+    xyz, rhos, per = [], [], []
+    for n, (a, c) in enumerate(blocks):
+        K, J, I = np.meshgrid(c * (bk - 1) + np.arange(bk), np.arange(bj), a * (bi - 1) + np.arange(bi),
+                              indexing="ij")
+        xyz.append(np.stack(coords(I.astype(float), J, K.astype(float))))
+        rhos.append(1.0 + 1.0e-3 * rng.uniform(-1.0, 1.0, size=(bk, bj, bi)))
+    index = {abc: n for n, abc in enumerate(blocks)}
+    for n, (a, c) in enumerate(blocks):
+        regs = []
+        for face, nbr, ax, fix, kind in (("imin", (a - 1, c), 0, 1, 4), ("imax", (a + 1, c), 0, bi, 4),
+                                         ("jmin", None, 1, 1, 2), ("jmax", None, 1, bj, 4),
+                                         ("kmin", (a, c - 1), 2, 1, 3), ("kmax", (a, c + 1), 2, bk, 3)):
+            st, ed = [1, 1, 1], [bi, bj, bk]
+            st[ax] = ed[ax] = fix
+            if nbr is not None and nbr in index:
+                tst, ted = list(st), list(ed)
+                tst[ax] = ted[ax] = (dims[0][ax] if fix == 1 else 1)
+                regs.append(Region(n, -1, tuple(st), tuple(ed), tuple(tst), tuple(ted), nbt=index[nbr] + 1))
+            else:
+                regs.append(Region(n, kind, tuple(st), tuple(ed)))
+        per.append(regs)
+    for n, regs in enumerate(per):
+        for r in regs:
+            if r.bctype < 0:
+                for w, pr in enumerate(per[r.nbt - 1]):
+                    if pr.bctype < 0 and pr.nbt == n + 1 and tuple(pr.s_st) == tuple(r.t_st):
+                        r.ibcwin = w + 1
+                        break
+    regions = [r for regs in per for r in regs]
+    inputs = dict(DEFAULT_INPUTS, moo=moo, reynolds=reynolds, cfl=cfl)
+    return Case("capacity_%dx%dx%dx%d" % (len(blocks), bi, bj, bk), dims, xyz, regions, inputs, nsteps, rhos)

@@ -20,6 +20,7 @@
 #include <mutex>
 #include <sstream>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/orion_hip.h"
@@ -367,59 +368,81 @@ static int build_topology(orion_hip_ctx* c, const std::vector<std::vector<Reg>>&
     bool used;
   };
   std::vector<Msg> msgs;
+  std::unordered_map<long long, std::vector<int>> by_tag;  // tag -> messages in send order
   for (int b = 0; b < nb; ++b)
     for (int r = 0; r < (int)R[b].size(); ++r)
-      if (R[b][r].d.nbt > 0) msgs.push_back({R[b][r].d.nbt * 1000 + R[b][r].d.ibcwin, b, r, false});
-  struct HP {
-    long long dst, order, src;
-  };
-  std::vector<HP> hp;
-  long long order = 0;
+      if (R[b][r].d.nbt > 0) {
+        by_tag[(long long)R[b][r].d.nbt * 1000 + R[b][r].d.ibcwin].push_back((int)msgs.size());
+        msgs.push_back({R[b][r].d.nbt * 1000 + R[b][r].d.ibcwin, b, r, false});
+      }
+  // Every ghost write of every receiving region, in the order the
+  // reference performs them (receiving blocks, their regions, then m, k, j,
+  // i of the window); a later write to the same ghost replaces an earlier
+  // one.  Destinations only repeat within one receiving block (where its
+  // windows overlap), so each block's writes are reduced through an index of
+  // its own destinations, last writer kept, in linear time.
   auto arena_off = [&](int b, int m, long long id) {
     return (long long)((c->hb[b].prim - (double*)c->arena) + (long long)m * c->hb[b].P + id);
   };
-  for (int b = 0; b < nb; ++b)
+  long long maxP = 1;
+  for (int b = 0; b < nb; ++b) maxP = std::max(maxP, c->hb[b].P);
+  std::vector<long long> where((size_t)(5 * maxP), -1);  // block-local destination -> its entry in pairs
+  std::vector<long long> touched;
+  std::vector<long long> pairs;
+  for (int b = 0; b < nb; ++b) {
+    touched.clear();
     for (int r = 0; r < (int)R[b].size(); ++r) {
       const Reg& rg = R[b][r];
       if (rg.d.nbt <= 0) continue;
       const int tag = (b + 1) * 1000 + (r + 1);
       Msg* M = nullptr;
-      for (auto& m : msgs)
-        if (!m.used && m.tag == tag) { M = &m; break; }
+      auto it = by_tag.find(tag);
+      if (it != by_tag.end())
+        for (int q : it->second)
+          if (!msgs[q].used) { M = &msgs[q]; break; }
       if (!M) return fail("halo: no partner message for block %d window %d", b + 1, r + 1);
       M->used = true;
       const BlockDev& A = c->hb[M->block];
+      const BlockDev& B = c->hb[b];
       int sb[3], se[3], rb[3], re[3];
       memory_window(R[M->block][M->reg], A, 0, sb, se);
-      memory_window(rg, c->hb[b], 1, rb, re);
-      const int sn[3] = {se[0] - sb[0] + 1, se[1] - sb[1] + 1, se[2] - sb[2] + 1};
-      const long long npts = (long long)sn[0] * sn[1] * sn[2];
+      memory_window(rg, B, 1, rb, re);
+      const long long npts = (long long)(se[0] - sb[0] + 1) * (se[1] - sb[1] + 1) * (se[2] - sb[2] + 1);
       const long long count = 5 * npts;
       long long ptr = 0;
+      // the sender's packing order (m, k, j, i over its window) walked
+      // alongside the receiver's unpacking order
+      int sm = 0, si = sb[0], sj = sb[1], sk = sb[2];
       for (int m = 0; m < 5; ++m)
         for (int k = rb[2]; k <= re[2]; ++k)
           for (int j = rb[1]; j <= re[1]; ++j)
             for (int i = rb[0]; i <= re[0]; ++i) {
               if (ptr >= count) continue;
-              const long long sm = ptr / npts, sl = ptr % npts;
-              const int si = sb[0] + (int)(sl % sn[0]);
-              const int sj = sb[1] + (int)((sl / sn[0]) % sn[1]);
-              const int sk = sb[2] + (int)(sl / ((long long)sn[0] * sn[1]));
               ++ptr;
-              hp.push_back({arena_off(b, m, pidx(c->hb[b], i, j, k)), order++,
-                            arena_off(M->block, (int)sm, pidx(A, si, sj, sk))});
+              const long long local = (long long)m * B.P + pidx(B, i, j, k);
+              const long long src = arena_off(M->block, sm, pidx(A, si, sj, sk));
+              if (++si > se[0]) {
+                si = sb[0];
+                if (++sj > se[1]) {
+                  sj = sb[1];
+                  if (++sk > se[2]) {
+                    sk = sb[2];
+                    ++sm;
+                  }
+                }
+              }
+              long long& w = where[(size_t)local];
+              if (w >= 0) {
+                pairs[2 * w + 1] = src;
+              } else {
+                w = (long long)(pairs.size() / 2);
+                pairs.push_back(arena_off(b, m, pidx(B, i, j, k)));
+                pairs.push_back(src);
+                touched.push_back(local);
+              }
             }
     }
-  // last writer wins: keep the highest order per destination
-  std::sort(hp.begin(), hp.end(), [](const HP& x, const HP& y) {
-    return x.dst < y.dst || (x.dst == y.dst && x.order < y.order);
-  });
-  std::vector<long long> pairs;
-  pairs.reserve(2 * hp.size());
-  for (size_t q = 0; q < hp.size(); ++q) {
-    if (q + 1 < hp.size() && hp[q + 1].dst == hp[q].dst) continue;
-    pairs.push_back(hp[q].dst);
-    pairs.push_back(hp[q].src);
+    for (long long t : touched) where[(size_t)t] = -1;
   }
   c->n_halo = (long long)(pairs.size() / 2);
   if (upload_vec(c, &c->d_halo, pairs)) return -1;
@@ -463,11 +486,11 @@ static int build_topology(orion_hip_ctx* c, const std::vector<std::vector<Reg>>&
   if (upload_vec(c, &c->d_bc, tasks)) return -1;
 
   // ---------------- interface averaging maps ----------------
-  struct Ent {
-    int sb;
-    long long sidx;
-  };
-  std::vector<std::vector<std::pair<int, long long>>> filler(nb);  // per (b, r): (A, X)
+  // Every face node of every interface window W (block b) takes the dq/vol
+  // that the region X packed for it (X of block a targets W; the last such
+  // packer wins), at the partner node W's map names.  A node on several
+  // windows is averaged once per window, in region order: entries sorted by
+  // (block, node, region order).
   std::map<std::pair<int, int>, std::pair<int, int>> fill_of;
   for (int a = 0; a < nb; ++a)
     for (int x = 0; x < (int)R[a].size(); ++x) {
@@ -475,8 +498,14 @@ static int build_topology(orion_hip_ctx* c, const std::vector<std::vector<Reg>>&
       if (X.d.bctype >= 0) continue;
       fill_of[{X.d.nbt - 1, X.d.ibcwin - 1}] = {a, x};  // last packer wins
     }
-  std::map<std::pair<int, long long>, std::vector<Ent>> dst_entries;
-  std::vector<std::pair<int, long long>> dst_order;
+  struct AE {
+    int b;
+    long long didx, seq;
+    int a;
+    long long sidx;
+  };
+  std::vector<AE> ae;
+  long long seq = 0;
   for (int b = 0; b < nb; ++b)
     for (int w = 0; w < (int)R[b].size(); ++w) {
       const Reg& W = R[b][w];
@@ -511,24 +540,26 @@ static int build_topology(orion_hip_ctx* c, const std::vector<std::vector<Reg>>&
             const int xi = (int)(rem % xn[0]), xj = (int)((rem / xn[0]) % xn[1]), xk = (int)(rem / ((long long)xn[0] * xn[1]));
             const long long sidx = pidx(c->hb[a], xlo[0] - 1 + NG + xi, xlo[1] - 1 + NG + xj, xlo[2] - 1 + NG + xk);
             const long long didx = pidx(c->hb[b], lo[0] - 1 + NG + i, lo[1] - 1 + NG + j, lo[2] - 1 + NG + k);
-            auto key = std::make_pair(b, didx);
-            auto& lst = dst_entries[key];
-            if (lst.empty()) dst_order.push_back(key);
-            lst.push_back({a, sidx});
+            ae.push_back({b, didx, seq++, a, sidx});
           }
     }
+  std::sort(ae.begin(), ae.end(), [](const AE& x, const AE& y) {
+    return x.b < y.b || (x.b == y.b && (x.didx < y.didx || (x.didx == y.didx && x.seq < y.seq)));
+  });
   std::vector<AvgSrc> src;
   std::vector<AvgDst> dst;
-  std::sort(dst_order.begin(), dst_order.end());
-  for (auto& key : dst_order) {
-    auto& lst = dst_entries[key];
-    AvgDst D{};
-    D.block = key.first;
-    D.idx = key.second;
-    D.start = (long long)src.size();
-    D.count = (int)lst.size();
-    for (auto& e : lst) src.push_back(AvgSrc{e.sb, e.sidx});
-    dst.push_back(D);
+  src.reserve(ae.size());
+  for (size_t q = 0; q < ae.size(); ++q) {
+    if (q == 0 || ae[q].b != ae[q - 1].b || ae[q].didx != ae[q - 1].didx) {
+      AvgDst D{};
+      D.block = ae[q].b;
+      D.idx = ae[q].didx;
+      D.start = (long long)src.size();
+      D.count = 0;
+      dst.push_back(D);
+    }
+    dst.back().count++;
+    src.push_back(AvgSrc{ae[q].a, ae[q].sidx});
   }
   c->n_avg_src = (long long)src.size();
   c->n_avg_dst = (long long)dst.size();
@@ -766,6 +797,19 @@ static int check_requests(int nreq, const int* req_nblocks, int nblocks, int nre
 static int create_ctx(int nreq, const int* req_nblocks, int nblocks, const int* dims, const double* xyz,
                       int nregions, const orion_region_desc* regions, const orion_inputs* in, int keep_q,
                       orion_hip_ctx** out) {
+  // ORION_HIP_TIMING=1: the set-up's phases on stderr (each waits for the
+  // device work queued before it)
+  static const bool timed = std::getenv("ORION_HIP_TIMING") && std::atoi(std::getenv("ORION_HIP_TIMING")) != 0;
+  auto tlast = std::chrono::steady_clock::now();
+  hipStream_t tstream = nullptr;
+  auto timing = [&](const char* what) {
+    if (!timed) return;
+    if (tstream) hipStreamSynchronize(tstream);
+    const auto now = std::chrono::steady_clock::now();
+    std::fprintf(stderr, "[orion_hip] set-up %-40s %8.1f ms\n", what,
+                 std::chrono::duration<double, std::milli>(now - tlast).count());
+    tlast = now;
+  };
   *out = nullptr;
   if (nblocks <= 0) return fail("nblocks must be positive");
   if (nregions > 0 && !regions) return fail("null regions");
@@ -785,6 +829,8 @@ static int create_ctx(int nreq, const int* req_nblocks, int nblocks, const int* 
       hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess)
     return bail(fail("hipEventCreate failed"));
   c->stream = c->own_stream;
+  tstream = c->stream;
+  timing("streams");
   c->nb = nblocks;
   c->keep_q = keep_q;
   init_params(c->p, in);
@@ -829,6 +875,7 @@ static int create_ctx(int nreq, const int* req_nblocks, int nblocks, const int* 
   if (hipMalloc((void**)&c->arena, c->arena_bytes) != hipSuccess)
     return bail(fail("arena hipMalloc(%zu bytes) failed", c->arena_bytes));
   if (hipMemsetAsync(c->arena, 0, c->arena_bytes, c->stream) != hipSuccess) return bail(fail("arena memset failed"));
+  timing("arena allocation and zeroing");
   {
     double* a = (double*)c->arena;
     auto al = [](long long x) { return (x + 31) / 32 * 32; };
@@ -952,7 +999,9 @@ static int create_ctx(int nreq, const int* req_nblocks, int nblocks, const int* 
     }
     if (make_space(c, c->dtile, ct)) return bail(-1);
   }
+  timing("regions and iteration spaces");
   if (build_sweep(c)) return bail(-1);
+  timing("sweep layout and tile orders");
   {
     std::vector<long long> ct(nblocks);
     for (int b = 0; b < nblocks; ++b) {
@@ -977,6 +1026,7 @@ static int create_ctx(int nreq, const int* req_nblocks, int nblocks, const int* 
   }
   if (upload_vec(c, &c->db, c->hb)) return bail(-1);
   if (build_topology(c, R)) return bail(-1);
+  timing("boundary topology (halo, BC, averaging, masks)");
   c->nparts = residual_parts(c->intr.total());
   if (c->ptile.nwg >= (1LL << 31)) return bail(fail("too many update workgroups"));
   if (dalloc(c, (void**)&c->d_part, sizeof(double) * 2 * 1024) || dalloc(c, (void**)&c->d_pidx, sizeof(long long) * 1024) ||
@@ -1017,21 +1067,19 @@ static int create_ctx(int nreq, const int* req_nblocks, int nblocks, const int* 
   }
   // grid -> metrics on the device
   {
+    // each block's coordinates as given (no padding) to a staging part of
+    // its face-flux scratch, then into the padded layout on the device
     long long off = 0;
     for (int b = 0; b < nblocks; ++b) {
       const BlockDev& B = c->hb[b];
       const long long n = (long long)B.idim * B.jdim * B.kdim;
-      std::vector<double> pad(3 * B.P, 0.0);
-      for (int m = 0; m < 3; ++m)
-        for (int k = 0; k < B.kdim; ++k)
-          for (int j = 0; j < B.jdim; ++j)
-            for (int i = 0; i < B.idim; ++i)
-              pad[m * B.P + pidx(B, i + NG, j + NG, k + NG)] = xyz[off + m * n + ((long long)k * B.jdim + j) * B.idim + i];
-      off += 3 * n;
-      if (hipMemcpyAsync(B.fx, pad.data(), sizeof(double) * 3 * B.P, hipMemcpyHostToDevice, c->stream) != hipSuccess)
+      if (hipMemcpyAsync(B.fx + 3 * B.P, xyz + off, sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->stream) !=
+          hipSuccess)
         return bail(fail("grid upload failed"));
-      if (hipStreamSynchronize(c->stream) != hipSuccess) return bail(fail("grid upload sync failed"));
+      off += 3 * n;
     }
+    launch_grid_scatter(LA(c, c->intr), c->intr.total());
+    timing("grid upload");
     launch_metrics(LA(c, c->intr), c->intr.total(), p.gcl, nullptr, c->stream);
     if (hipMemsetAsync(c->d_count, 0, sizeof(unsigned long long), c->stream) != hipSuccess) return bail(fail("memset"));
     launch_count_small_vol(LA(c, c->intr), c->intr.total(), c->d_count);
@@ -1045,6 +1093,7 @@ static int create_ctx(int nreq, const int* req_nblocks, int nblocks, const int* 
   }
   if (p.nvis != 0) launch_temperature(LA(c, c->pad), c->pad.total());
   if (hipStreamSynchronize(c->stream) != hipSuccess) return bail(fail("setup failed: %s", hipGetErrorString(hipGetLastError())));
+  timing("metrics, face geometry, initial state");
   *out = c;
   return 0;
 }

@@ -138,7 +138,30 @@ __global__ void __launch_bounds__(256) small_vol_kernel(const BlockDev* __restri
   if (!(b.vol[pidx(b, i, j, k)] >= 1.0e-15)) atomicAdd(count, 1ULL);
 }
 
+// The node coordinates of each block, copied compactly (x, y, z, each
+// idim*jdim*kdim values, i fastest) to fx + 3P, spread into the padded
+// layout at fx[0 .. 3P) that the metric kernels read.  The arena starts
+// zeroed, so the ghost points stay 0 as the reference's grid arrays have
+// them.  Source and destination do not overlap.
+__global__ void __launch_bounds__(256) grid_scatter_kernel(const BlockDev* __restrict__ blocks, Span SP) {
+  int bi;
+  long long gi;
+  if (!span_item(SP, &bi, &gi)) return;
+  const BlockDev& b = blocks[bi];
+  int i, j, k;
+  decode_interior(b, gi, &i, &j, &k);
+  const long long n = (long long)b.idim * b.jdim * b.kdim;
+  const double* stage = b.fx + 3 * b.P;
+  const long long id = pidx(b, i, j, k);
+  for (int m = 0; m < 3; ++m) b.fx[m * b.P + id] = stage[m * n + gi];
+}
+
 static inline unsigned grid_of(long long n, int tb) { return (unsigned)((n + tb - 1) / tb); }
+
+void launch_grid_scatter(const Launch& L, long long n) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(grid_scatter_kernel, dim3((unsigned)L.nwg), dim3(SPAN_TB), 0, L.stream, L.blocks, L.span);
+}
 
 void launch_metrics(const Launch& L, long long n, int gcl, double*, hipStream_t) {
   if (n <= 0) return;

@@ -87,5 +87,6 @@ void launch_update(const Launch& La, long long n, const SweepArrays& S, double g
 // k_metrics.hip
 void launch_metrics(const Launch& L, long long n, int gcl, double* tmp, hipStream_t s);
 void launch_count_small_vol(const Launch& L, long long n, unsigned long long* count);
+void launch_grid_scatter(const Launch& L, long long n);
 
 }  // namespace orion_hip

@@ -113,19 +113,53 @@ def channel_stream(n: int, seed: int = 2024, rate: float = 20.0, nodes=(1.0e4, 1
     return out
 
 
-def _device_solver(case: cases.Case, overlap: bool = True):
+def bump_stream(n: int, seed: int = 4242, rate: float = 20.0, nodes=(2.0e5, 8.0e6), blocks=(1, 16),
+                moo=(0.3, 0.85), reynolds=(1.0e3, 1.0e6), bump=(0.0, 0.1)) -> List[Request]:
+    """A seeded Poisson stream of ``n`` steady channel-bump solves in the
+    shape of the surveyed serving workload: arrival rate ``rate`` per
+    second; per request log-uniform interior nodes in ``nodes``, a block
+    count uniform in ``blocks`` (the channel cut along i, each piece at least
+    9 nodes long), uniform Mach number, log-uniform Reynolds number, a
+    cosine bump of uniform height on the lower wall, the upper boundary a
+    wall or far-field (cases.bump_request)."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, size=n))
+    t -= t[0]
+    out = []
+    for r in range(n):
+        nn = float(np.exp(rng.uniform(np.log(nodes[0]), np.log(nodes[1]))))
+        nb = int(rng.integers(blocks[0], blocks[1] + 1))
+        m = float(rng.uniform(*moo))
+        re = float(np.exp(rng.uniform(np.log(reynolds[0]), np.log(reynolds[1]))))
+        h = float(rng.uniform(*bump))
+        upper = bool(rng.integers(0, 2))
+        out.append(Request(r, float(t[r]), cases.bump_request(seed * 1000 + r, nn, nb, m, re, h, upper)))
+    return out
+
+
+def _device_solver(case: cases.Case, overlap: bool = True, per_block: bool = False):
     from .solver import OrionSolver
     s = OrionSolver(case)
     s.fs.set_overlap(overlap)
+    if per_block:
+        s.fs.set_per_block(True)
     return s
+
+
+def _plain_solver(case: cases.Case, overlap: bool = True):
+    """The plain per-block form: a launch per block per kernel, one stream,
+    no captured step."""
+    return _device_solver(case, False, per_block=True)
 
 
 class Server:
     def __init__(self, slots: int = 8, hbm_budget: float = 200.0e9, chunk: int = 50, drop: float = 1.0e-8,
                  max_steps: int = 3000, make_solver: Callable = _device_solver, keep_fields: bool = False,
-                 overlap: Optional[bool] = None):
+                 overlap: Optional[bool] = None, progress: bool = False):
         """overlap: the side stream of each context (orion_hip_set_overlap);
-        None: on for one slot, off for several."""
+        None: on for one slot, off for several.  progress: a line on stderr
+        per finished request."""
+        self.progress = progress
         if slots < 1 or chunk < 1:
             raise ValueError("slots and chunk must be positive")
         self.overlap = (slots == 1) if overlap is None else bool(overlap)
@@ -194,6 +228,10 @@ class Server:
                     with lock:
                         used -= footprint(req.case)
                     results.append(f.result())
+                    if self.progress:
+                        r = results[-1]
+                        print("[serving]   request %d done at %.1f s: %d steps, %d nodes" % (
+                            r.rid, r.finish, r.steps, r.nodes), file=sys.stderr, flush=True)
         return results
 
 
@@ -211,6 +249,7 @@ class Batch:
         from . import _lib
         from .solver import case_inputs, region_descs
         L = _lib.load()
+        t_in = time.perf_counter()
         _lib.check(L.orion_hip_backend_init(int(device)), "backend_init")
         self.cases = list(case_list)
         n = len(self.cases)
@@ -224,16 +263,19 @@ class Batch:
         regs, nreg = region_descs(self.cases, self.first)
         inps = (_lib.Inputs * n)(*[case_inputs(c) for c in self.cases])
         h = C.c_void_p()
+        t_cr = time.perf_counter()
         _lib.check(L.orion_hip_batch_create(n, nbl.ctypes.data_as(C.POINTER(C.c_int)), int(nbl.sum()),
                                             dims.ctypes.data_as(C.POINTER(C.c_int)),
                                             xyz.ctypes.data_as(C.POINTER(C.c_double)), nreg, regs, inps, 0,
                                             C.byref(h)), "batch_create")
         self.ctx = h
+        t_su = time.perf_counter()
         try:
             self._set_up(states, L)
         except Exception:
             self.close()
             raise
+        self.timing = {"inputs": t_cr - t_in, "create": t_su - t_cr, "setup": time.perf_counter() - t_su}
 
     def _set_up(self, states, L) -> None:
         from . import _lib
@@ -319,8 +361,15 @@ class BatchServer:
 
     def __init__(self, slots: int = 16, hbm_budget: float = 200.0e9, chunk: int = 50, drop: float = 1.0e-8,
                  max_steps: int = 3000, keep_fields: bool = False, device: int = 0,
-                 make_batch: Callable = None):
-        """make_batch(cases, states, device): the batch factory (``Batch``)."""
+                 make_batch: Callable = None, progress: bool = False, refuse_oversize: bool = False):
+        """make_batch(cases, states, device): the batch factory (``Batch``);
+        progress: a line on stderr per batch rebuild; refuse_oversize: a
+        request larger than the whole budget is refused (a result with its
+        error) instead of failing the stream."""
+        self.progress = progress
+        self.refuse_oversize = refuse_oversize
+        self.delayed_by_budget = 0  # admissions that waited for HBM (a slot was free)
+        self.host_stashes = 0  # rebuilds whose staying state went through host memory
         if slots < 1 or chunk < 1:
             raise ValueError("slots and chunk must be positive")
         if max_steps % chunk:
@@ -332,14 +381,22 @@ class BatchServer:
         self.peak_bytes = 0
         self.rebuilds = 0
         self.rebuild_s = 0.0  # host time spent rebuilding batches
+        # where the rebuild time goes: host inputs, context creation, state set-up, old batch release
+        self.rebuild_parts = {"inputs": 0.0, "create": 0.0, "setup": 0.0, "release": 0.0}
 
     def serve(self, requests: Sequence[Request]) -> List[Result]:
         """Run the stream in real time; returns the results in completion order."""
         pending = sorted(requests, key=lambda r: (r.arrival, r.rid))
-        for r in pending:
-            if footprint(r.case) > self.hbm_budget:
-                raise ValueError("request %d needs %.1f GB, more than the budget" % (r.rid, footprint(r.case) / 1e9))
         results: List[Result] = []
+        for r in list(pending):
+            if footprint(r.case) > self.hbm_budget:
+                msg = "request %d needs %.1f GB, more than the budget" % (r.rid, footprint(r.case) / 1e9)
+                if not self.refuse_oversize:
+                    raise ValueError(msg)
+                pending.remove(r)
+                results.append(Result(r.rid, r.arrival, r.arrival, r.arrival, 0, False, r.case.interior_nodes(),
+                                      np.zeros(0), None, msg))
+        waited: set = set()
         resident: List[dict] = []  # in batch order: {req, start, rms}
         leaving: set = set()  # indices into resident that finished at the last boundary
         batch: Optional[Batch] = None
@@ -355,10 +412,26 @@ class BatchServer:
                     req = pending.pop(0)
                     used += footprint(req.case)
                     admitted.append({"req": req, "start": now, "rms": []})
+                if (pending and pending[0].arrival <= now and len(stay) + len(admitted) < self.slots
+                        and pending[0].rid not in waited):
+                    waited.add(pending[0].rid)  # arrived, a slot free, but its HBM does not fit yet
+                    self.delayed_by_budget += 1
                 if admitted or leaving:
                     # one rebuild per boundary: the staying requests' state
-                    # copied on the device from the old batch, then the newcomers
-                    states = [(batch, k) for k in stay] if batch is not None else []
+                    # copied on the device from the old batch, then the
+                    # newcomers.  The old batch and the new one are both on the
+                    # device until the copies are done; when the two would not
+                    # fit the budget together, the staying requests' state
+                    # goes through host memory and the old batch is released
+                    # first.
+                    old_bytes = sum(footprint(m["req"].case) for m in resident) if batch is not None else 0
+                    if batch is not None and old_bytes + used > self.hbm_budget:
+                        states = [batch.prim(k) for k in stay]
+                        self.host_stashes += 1
+                        batch.close()
+                        batch = None
+                    else:
+                        states = [(batch, k) for k in stay] if batch is not None else []
                     resident = [resident[k] for k in stay] + admitted
                     leaving = set()
                     new = None
@@ -390,8 +463,17 @@ class BatchServer:
                         self.rebuilds += 1
                         self.peak_resident = max(self.peak_resident, len(resident))
                         self.peak_bytes = max(self.peak_bytes, used)
+                        if self.progress:
+                            print("[serving]   batch %d at %.1f s: %d resident, %.1f GB admitted, %d pending" % (
+                                self.rebuilds, time.perf_counter() - t0, len(resident), used / 1e9, len(pending)),
+                                file=sys.stderr, flush=True)
+                    if new is not None and hasattr(new, "timing"):
+                        for k, v in new.timing.items():
+                            self.rebuild_parts[k] += v
+                    tr = time.perf_counter()
                     if batch is not None:
                         batch.close()
+                    self.rebuild_parts["release"] += time.perf_counter() - tr
                     batch = new
                     self.rebuild_s += time.perf_counter() - tb
                 if not resident:
@@ -468,6 +550,153 @@ def measure(nreq: int = 16, slots: int = 16, seed: int = 2024, sequential: bool 
             "same_histories_alone_and_mixed": same}
 
 
+def measure_bumps(nreq: int = 32, slots: int = 16, seed: int = 4242, max_steps: int = 200,
+                  hbm_budget: float = 200.0e9, plain: bool = True, batched_sequential: bool = False) -> dict:
+    """The serving measurement at the surveyed shape: ``bump_stream(nreq,
+    seed)`` (0.2M-8M nodes, 1-16 blocks, M 0.3-0.85, Re 1e3-1e6, bumps up
+    to 0.1) served ``slots`` at a time from one batch context, against the
+    same requests solved one after another in the plain per-block form
+    (the sequential plain GPU solves) and, with ``batched_sequential``, one
+    after another in the batched form.  A request stops at a 1e-8 RMS drop
+    relative to its first step or after ``max_steps`` steps, whichever
+    comes first; every request's residual history must be the same in
+    every run."""
+    reqs = bump_stream(nreq, seed=seed)
+    nodes = [r.case.interior_nodes() for r in reqs]
+    print("[serving] %d bump requests, %d-%d nodes, %d-%d blocks, stop at 1e-8 drop or %d steps" % (
+        nreq, min(nodes), max(nodes), min(r.case.nblocks for r in reqs), max(r.case.nblocks for r in reqs),
+        max_steps), file=sys.stderr, flush=True)
+    out = {"workload": "%d steady channel-bump solves (%d-%d nodes, 1-16 blocks cut along i, M 0.3-0.85, "
+                       "Re 1e3-1e6 log-uniform, cosine bumps of height 0-0.1, upper wall or far-field), Poisson "
+                       "arrivals 20/s, seed %d" % (nreq, min(nodes), max(nodes), seed),
+           "stop_rule": "RMS residual drop of 1e-8 relative to the first step, or %d steps, whichever comes first"
+                        % max_steps,
+           "slots": slots, "hbm_budget_bytes": hbm_budget,
+           "form": "one batch context of the resident requests, rebuilt at 50-step boundaries"}
+    t0 = time.perf_counter()
+    srv_ = BatchServer(slots=slots, max_steps=max_steps, hbm_budget=hbm_budget, progress=True)
+    srv = srv_.serve(reqs)
+    print("[serving] batched, %d slots: %.1f s, %d batch rebuilds (%.2f s)" % (
+        slots, time.perf_counter() - t0, srv_.rebuilds, srv_.rebuild_s), file=sys.stderr, flush=True)
+    out.update(batch_rebuilds=srv_.rebuilds, rebuild_s=srv_.rebuild_s, peak_resident=srv_.peak_resident,
+               peak_admitted_bytes=srv_.peak_bytes, served=summarize(srv))
+    out["rebuild_breakdown_s"] = dict(srv_.rebuild_parts)
+    by = {r.rid: r for r in srv}
+    out["requests_final_rms_not_finite"] = int(sum(not np.all(np.isfinite(r.rms)) for r in srv))
+    same = True
+    differ = {}
+    for key, on, make in (("sequential_plain", plain, _plain_solver),
+                          ("sequential_batched_form", batched_sequential, _device_solver)):
+        if not on:
+            continue
+        t0 = time.perf_counter()
+        seq = Server(slots=1, max_steps=max_steps, make_solver=make, hbm_budget=hbm_budget, progress=True).serve(reqs)
+        print("[serving] %s %.1f s" % (key, time.perf_counter() - t0), file=sys.stderr, flush=True)
+        out[key] = summarize(seq)
+        out["solves_per_s_vs_" + key] = out["served"]["solves_per_s"] / out[key]["solves_per_s"]
+        # histories bit for bit (a diverged request's non-finite entries compare equal to themselves)
+        bad = [r.rid for r in seq if not np.array_equal(by[r.rid].rms, r.rms, equal_nan=True)]
+        if bad:
+            first = {}
+            for rid in bad[:4]:
+                a, b = by[rid].rms, next(x.rms for x in seq if x.rid == rid)
+                n = min(len(a), len(b))
+                d = np.nonzero(~((a[:n] == b[:n]) | (np.isnan(a[:n]) & np.isnan(b[:n]))))[0]
+                first[rid] = int(d[0]) if d.size else -1
+            differ[key] = {"requests": bad, "first_differing_step": first}
+        same = same and not bad
+    out["same_histories_alone_and_mixed"] = same if (plain or batched_sequential) else None
+    if differ:
+        out["differing"] = differ
+    return out
+
+
+def device_memory(device: int = 0):
+    """(free, total) bytes of the device, from the HIP runtime."""
+    import ctypes as C
+    from . import _lib
+    _lib.load()
+    hip = C.CDLL("libamdhip64.so")
+    free, total = C.c_size_t(), C.c_size_t()
+    if hip.hipSetDevice(int(device)) != 0 or hip.hipMemGetInfo(C.byref(free), C.byref(total)) != 0:
+        raise RuntimeError("hipMemGetInfo failed")
+    return int(free.value), int(total.value)
+
+
+STEP_BYTES = 1240  # algorithmic bytes per interior cell-step (the step's roofline)
+HBM_PEAK = 8.0e12
+
+
+def measure_capacity(nreq: int = 32, slots: int = 32, seed: int = 4242, max_steps: int = 200, alone_steps: int = 10,
+                     chunk: int = 10, margin: float = 8.0e9) -> dict:
+    """The capacity workload (cases.capacity_case: 48 blocks of 257x129x21,
+    33.4M nodes, transonic bump channel, CFL 20) resident beside the served
+    bump stream (measure_bumps' requests).  The capacity case runs on its
+    own host thread and stream: first ``alone_steps`` steps alone, then in
+    chunks of ``chunk`` steps for as long as the stream is served; the
+    stream's admission budget is the device memory left beside it less
+    ``margin``.  Reports the HBM held, the admissions the budget delayed or
+    refused, the capacity case's step time and step roofline alone and under
+    contention, and the stream's solves/s and latencies."""
+    from .solver import OrionSolver
+    cap = cases.capacity_case()
+    print("[capacity] %s: %d nodes" % (cap.name, cap.interior_nodes()), file=sys.stderr, flush=True)
+    free0, total = device_memory()
+    s = OrionSolver(cap)
+    cap_bytes = s.fs.device_bytes()
+    s.solve(2)
+    s.fs.synchronize()
+    t0 = time.perf_counter()
+    s.solve(alone_steps)
+    s.fs.synchronize()
+    alone_ms = 1e3 * (time.perf_counter() - t0) / alone_steps
+    print("[capacity] alone: %.2f ms/step, %.1f GB on the device" % (alone_ms, cap_bytes / 1e9), file=sys.stderr,
+          flush=True)
+    free1, _ = device_memory()
+    budget = float(free1 - margin)
+    reqs = bump_stream(nreq, seed=seed)
+    stop = threading.Event()
+    chunks: List[float] = []
+    err: List[str] = []
+
+    def drive():
+        try:
+            while not stop.is_set():
+                ta = time.perf_counter()
+                s.solve(chunk)
+                s.fs.synchronize()
+                chunks.append(1e3 * (time.perf_counter() - ta) / chunk)
+        except Exception as e:  # noqa: BLE001 -- reported
+            err.append(str(e))
+
+    th = threading.Thread(target=drive, daemon=True)
+    th.start()
+    srv_ = BatchServer(slots=slots, max_steps=max_steps, hbm_budget=budget, progress=True, refuse_oversize=True)
+    t1 = time.perf_counter()
+    res = srv_.serve(reqs)
+    span = time.perf_counter() - t1
+    stop.set()
+    th.join()
+    n = cap.interior_nodes()
+    cont = float(np.mean(chunks)) if chunks else float("nan")
+    out = {"workload": "%s (%d nodes, M 0.85, Re 1e6, CFL 20) resident beside %d channel-bump requests "
+                       "(bump_stream seed %d, stop at a 1e-8 drop or %d steps)" % (cap.name, n, nreq, seed, max_steps),
+           "device_total_bytes": total, "device_free_before_bytes": free0, "capacity_device_bytes": cap_bytes,
+           "serving_budget_bytes": budget, "peak_admitted_bytes": srv_.peak_bytes,
+           "peak_resident_requests": srv_.peak_resident,
+           "peak_hbm_fraction": (cap_bytes + srv_.peak_bytes) / total,
+           "admissions_delayed_by_budget": srv_.delayed_by_budget, "refused": sum(r.error is not None for r in res),
+           "rebuilds": srv_.rebuilds, "rebuilds_through_host": srv_.host_stashes, "rebuild_s": srv_.rebuild_s,
+           "served": summarize(res), "serving_span_s": span,
+           "capacity_alone_ms_per_step": alone_ms, "capacity_contended_ms_per_step": cont,
+           "capacity_steps_under_contention": chunk * len(chunks),
+           "capacity_step_roofline_alone": n * STEP_BYTES / (alone_ms * 1e-3) / HBM_PEAK,
+           "capacity_step_roofline_contended": n * STEP_BYTES / (cont * 1e-3) / HBM_PEAK if chunks else None,
+           "capacity_error": err[0] if err else None}
+    s.fs.close()
+    return out
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description="serve a seeded stream of steady channel solves; prints one JSON line")
     ap.add_argument("--requests", type=int, default=16)
@@ -475,9 +704,24 @@ def main(argv=None) -> int:
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--hw-queues", type=int, default=HW_QUEUES)
     ap.add_argument("--batched-only", action="store_true", help="skip the sequential baseline")
+    ap.add_argument("--stream", choices=("channels", "bumps"), default="channels",
+                    help="channels: small single-block channels to a 1e-8 drop; bumps: the surveyed shape "
+                         "(0.2M-8M nodes, 1-16 blocks)")
+    ap.add_argument("--max-steps", type=int, default=200, help="bumps: the step cap of the stop rule")
+    ap.add_argument("--capacity", action="store_true",
+                    help="the 33.4M-node capacity case resident beside the bump stream (measure_capacity)")
+    ap.add_argument("--batched-sequential", action="store_true",
+                    help="bumps: also solve the requests one after another in the batched form")
     a = ap.parse_args(argv)
     prepare_process(a.hw_queues)
-    print(json.dumps(measure(a.requests, a.slots, a.seed, sequential=not a.batched_only)), flush=True)
+    if a.capacity:
+        res = measure_capacity(a.requests, a.slots, max_steps=a.max_steps)
+    elif a.stream == "bumps":
+        res = measure_bumps(a.requests, a.slots, seed=a.seed if a.seed != 2024 else 4242, max_steps=a.max_steps,
+                            plain=not a.batched_only, batched_sequential=a.batched_sequential)
+    else:
+        res = measure(a.requests, a.slots, a.seed, sequential=not a.batched_only)
+    print(json.dumps(res), flush=True)
     return 0
 
 

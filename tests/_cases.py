@@ -148,6 +148,13 @@ def inputs_variant(nsteps=3):
     return c
 
 
+def bump_split(nsteps=3):
+    """A channel-bump request of the serving stream cut along i into three
+    point-matched blocks, periodic in k (each piece's own self-interface),
+    lower and upper viscous walls, far-field ends."""
+    return cases.bump_request(5, 9000, 3, 0.7, 4.0e3, 0.08, True, nsteps=nsteps)
+
+
 ALL = {
     "channel_small": channel_small,
     "lattice_small": lattice_small,
@@ -160,4 +167,5 @@ ALL = {
     "rotated_pair": rotated_pair,
     "near_vacuum": near_vacuum,
     "inputs_variant": inputs_variant,
+    "bump_split": bump_split,
 }

@@ -291,3 +291,38 @@ def test_batch_of_multiblock_requests_equals_each_alone():
                 assert np.array_equal(p, prims[blk]), (k, blk)
     finally:
         b.close()
+
+
+@pytest.mark.gpu
+def test_batch_with_a_million_node_request_equals_each_alone():
+    """A batch holding a channel-bump request of about 1.2M nodes in 6
+    blocks (the serving stream's shape), a 3-block bump request at another
+    Mach and Reynolds number and a single-block channel: each request's
+    residuals, maximum, its location and time step per step, and its final
+    fields, equal its run alone bit for bit."""
+    from orion_amd import cases as C
+    from orion_amd.solver import OrionSolver
+    cs = [C.bump_request(11, 1.2e6, 6, 0.8, 3.0e5, 0.06, False, nsteps=4),
+          C.bump_request(12, 5.0e4, 3, 0.4, 2.0e3, 0.1, True, nsteps=4),
+          C.bump_request(13, 2.0e4, 1, 0.6, 1.0e4, 0.0, False, nsteps=4)]
+    assert cs[0].interior_nodes() >= 1_000_000
+    print("\n[gpu] serving: batch with a %d-node request" % cs[0].interior_nodes(), flush=True)
+    alone = []
+    for c in cs:
+        s = OrionSolver(c)
+        h = s.solve(4)
+        alone.append((h, [s.fs.field(b, "prim") for b in range(c.nblocks)]))
+        s.fs.close()
+    b = serving.Batch(cs)
+    try:
+        st = b.run(4, stats=True)
+        for k, c in enumerate(cs):
+            h, prims = alone[k]
+            for n in range(4):
+                x, y = st[n][k], h[n]
+                assert (x.rms_residual, x.max_residual, x.max_loc, x.phydtime) == \
+                    (y.rms_residual, y.max_residual, y.max_loc, y.phydtime), (k, n)
+            for blk, p in enumerate(b.prim(k)):
+                assert np.array_equal(p, prims[blk]), (k, blk)
+    finally:
+        b.close()
