@@ -216,18 +216,23 @@ def stage_traffic(workload, groups):
     return out
 
 
-def serving_run(nreq, slots):
-    """The serving form (orion_amd.serving.measure): a seeded Poisson stream
-    of steady channel solves, each to a 1e-8 RMS residual drop relative to
-    its first step (at most 3000 steps), served ``slots`` at a time with
-    continuous admission, and the same stream solved one request at a time
-    (the sequential plain-GPU baseline); with the check that every request's
-    residual history is the same in both runs.  It runs as a child process,
+def serving_run(nreq, slots, stream="bumps", max_steps=200):
+    """The serving form (orion_amd.serving): by default the surveyed shape
+    (measure_bumps): a seeded Poisson stream of channel-bump solves of
+    0.2M-8M nodes in 1-16 blocks, each stopped at a 1e-8 RMS residual drop
+    relative to its first step or after ``max_steps`` steps, served
+    ``slots`` at a time with continuous admission from one batch context,
+    and the same stream solved one request at a time in the plain per-block
+    form (the sequential plain-GPU baseline); with the check that every
+    request's residual history is the same in both runs.  stream
+    "channels": small single-block channels to a 1e-8 drop (measure).
+    It runs as a child process,
     because the hardware queue count it asks HIP for (one per resident
     request) must be set before HIP starts in a process."""
-    cmd = [sys.executable, "-u", "-m", "orion_amd.serving", "--requests", str(nreq), "--slots", str(slots)]
+    cmd = [sys.executable, "-u", "-m", "orion_amd.serving", "--requests", str(nreq), "--slots", str(slots),
+           "--stream", stream, "--max-steps", str(max_steps)]
     try:
-        p = subprocess.run(cmd, cwd=os.path.dirname(os.path.abspath(__file__)), stdout=subprocess.PIPE, timeout=900)
+        p = subprocess.run(cmd, cwd=os.path.dirname(os.path.abspath(__file__)), stdout=subprocess.PIPE, timeout=1200)
         if p.returncode != 0:
             raise RuntimeError("exit status %d" % p.returncode)
         return json.loads(p.stdout.decode().strip().splitlines()[-1])
@@ -332,7 +337,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--ref-steps", type=int, default=2, help="steps of the reference CPU baseline (0: port only)")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
-    ap.add_argument("--serve-requests", type=int, default=16)
+    ap.add_argument("--serve-requests", type=int, default=32)
+    ap.add_argument("--serve-stream", choices=("bumps", "channels"), default="bumps")
+    ap.add_argument("--serve-max-steps", type=int, default=200)
     ap.add_argument("--serve-slots", type=int, default=16)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (HBM counter calibration)")
@@ -461,7 +468,7 @@ def main():
     }
     s.fs.close()
     if not a.no_serve:
-        out["serving"] = serving_run(a.serve_requests, a.serve_slots)
+        out["serving"] = serving_run(a.serve_requests, a.serve_slots, a.serve_stream, a.serve_max_steps)
     if not a.no_cpu:
         threads = a.cpu_threads or min(16, os.cpu_count() or 1)
         port = cpu_baseline(case, threads, a.cpu_seconds)
