@@ -327,7 +327,7 @@ def main():
     ap.add_argument("--per-block-steps", type=int, default=5,
                     help="steps of the plain per-block form timed beside the batched form (0: none)")
     ap.add_argument("--tune-steps", type=int, default=3,
-                    help="steps per trial of the sweep-schedule autotuner after the warmup (0: no tuning)")
+                    help="steps per trial of the launch-shape autotuner after the warmup (0: no tuning)")
     ap.add_argument("--schedule", type=int, default=-1,
                     help="fix the sweep schedule (0 diagonal, 1 critical path) instead of tuning")
     ap.add_argument("--workload", default="columns64")
@@ -374,12 +374,10 @@ def main():
     if a.schedule >= 0:
         s.fs.set_schedule(a.schedule)
     elif a.tune_steps > 0:
-        # untimed, like the warmup; the state is restored after each trial
-        chosen, ms = s.fs.tune_schedule(a.tune_steps)
-        tuning = {"steps_per_trial": a.tune_steps,
-                  "ms_per_step": {n: round(v, 4) for n, v in zip(s.fs.SCHEDULES, ms)}}
-        print("[bench] sweep schedule tuning: %s -> %s" % (json.dumps(tuning["ms_per_step"]), s.fs.SCHEDULES[chosen]),
-              file=sys.stderr, flush=True)
+        # untimed, like the warmup; the state is restored after each trial:
+        # the sweep schedule, then the viscous stage's tiling
+        tuning = s.fs.tune(a.tune_steps)
+        print("[bench] launch-shape tuning: %s" % json.dumps(tuning), file=sys.stderr, flush=True)
     # a.runs timed runs of exactly a.steps steps, each from the same state
     # (prim alone fixes the next step); value is their median
     start = [s.fs.field(b, "prim") for b in range(case.nblocks)] if a.runs > 1 else None
@@ -455,7 +453,8 @@ def main():
                    "device_bytes": s.fs.device_bytes(),
                    "scheme": "WCNS-E-5+Roe, 6th-order viscous, LU-SGS+2 SGS sweeps", "precision": "fp64",
                    "moo": case.inputs["moo"], "reynolds": case.inputs["reynolds"], "cfl": case.inputs["cfl"],
-                   "sweep_schedule": s.fs.SCHEDULES[s.fs.schedule()]},
+                   "sweep_schedule": s.fs.SCHEDULES[s.fs.schedule()],
+                   "visc_tiles": dict(zip(("segment", "grid"), s.fs.visc_tiles()))},
         "schedule_tuning": tuning,
         "roofline": roof,
         "stage_rooflines": stages,

@@ -177,6 +177,24 @@ int orion_hip_get_schedule(const orion_hip_ctx* ctx);
  * Synchronous; holds a device copy of prim while it runs. */
 int orion_hip_tune_schedule(orion_hip_ctx* ctx, int steps, int* chosen, double* ms_per_step);
 
+/* No counterpart in the reference: the tiling of the viscous face-flux
+ * stage, one persistent launch of `grid` workgroups that take items from a
+ * block work queue -- along j and k, 256 lines by `segment` faces, each
+ * lane marching along its line; along i, 256 face pairs.  Results do not
+ * depend on the tiling.  Defaults: segment 32, grid 2048. */
+int orion_hip_set_visc_tiles(orion_hip_ctx* ctx, int segment, int grid);
+int orion_hip_get_visc_tiles(const orion_hip_ctx* ctx, int* segment, int* grid);
+/* The autotuner of the step's launch shapes: from the current state, times
+ * `steps` pseudo-steps (after one untimed step) per candidate, restoring the
+ * state after each, and keeps the fastest by coordinate search -- the LU-SGS
+ * tile schedule (ORION_SCHEDULES), then the viscous segment (8, 16, 32, 64),
+ * then its grid (1024, 2048, 4096 workgroups).  Outputs (each may be NULL):
+ * the chosen schedule, segment and grid, the best ms per step, the trials
+ * run.  The solution state afterwards is the one before the call, bit for
+ * bit.  Synchronous; holds a device copy of prim while it runs. */
+int orion_hip_tune(orion_hip_ctx* ctx, int steps, int* schedule, int* segment, int* grid, double* best_ms,
+                   int* trials);
+
 /* Output (postprocess::PostProcess, src/postprocess/PostProcess.hpp:10-33).
  *
  * The point fields that PostProcess::write_block_vts_binary writes

@@ -86,6 +86,10 @@ def _open():
         "orion_hip_download": ([P, C.c_int, C.c_int, C.POINTER(C.c_double), C.c_longlong], C.c_int),
         "orion_hip_copy_block_state": ([P, C.c_int, P, C.c_int], C.c_int),
         "orion_hip_set_per_block": ([P, C.c_int], C.c_int),
+        "orion_hip_set_visc_tiles": ([P, C.c_int, C.c_int], C.c_int),
+        "orion_hip_get_visc_tiles": ([P, C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
+        "orion_hip_tune": ([P, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                            C.POINTER(C.c_double), C.POINTER(C.c_int)], C.c_int),
         "orion_hip_residual_noalloc": ([C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
                                         C.POINTER(C.c_void_p), C.c_void_p, C.POINTER(ResidualStats)], C.c_int),
         "orion_hip_update_temperature": ([P], C.c_int),

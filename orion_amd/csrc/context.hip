@@ -141,6 +141,15 @@ struct orion_hip_ctx {
   int* d_zeros = nullptr;               // zero workgroup tables of one-block spans
   std::vector<int> tile_first, tile_count;  // each block's tiles in `tiles`
   int* d_border[2] = {nullptr, nullptr};   // per-block tile orders at the tiles' positions
+  // the viscous face-flux work queue (one persistent launch per step):
+  // items, their count, the ticket counter, faces per marching item and
+  // the launch's workgroups
+  ViscItem* d_vitems = nullptr;
+  long long vitems_cap = 0;
+  int n_vitems = 0;
+  int* d_vhead = nullptr;
+  int visc_seg = 32;
+  int visc_grid = 2048;
 };
 
 // Device allocations of a context, freed with it.  Tables up to 64 KB share
@@ -585,6 +594,47 @@ static int build_topology(orion_hip_ctx* c, const std::vector<std::vector<Reg>>&
   return 0;
 }
 
+// The viscous face-flux work queue with `seg` faces per marching item:
+// along j and k, 256 lines (i fastest) by seg faces; along i, 256 face pairs.
+// The marching items come first (they are the longer ones).
+static std::vector<ViscItem> visc_items(const orion_hip_ctx* c, int seg) {
+  std::vector<ViscItem> v;
+  for (int d = 1; d < 3; ++d)
+    for (int b = 0; b < c->nb; ++b) {
+      const BlockDev& B = c->hb[b];
+      const long long lines = (long long)B.idim * (d == 1 ? B.kdim : B.jdim);
+      const int faces = dcount(B, d) + 1;
+      for (long long a = 0; a < lines; a += 256)
+        for (int f = 0; f < faces; f += seg)
+          v.push_back(ViscItem{b, d, a, std::min(lines, a + 256), f, std::min(faces, f + seg)});
+    }
+  for (int b = 0; b < c->nb; ++b) {
+    const BlockDev& B = c->hb[b];
+    const long long pairs = nlines(B, 0) * ((B.idim + 2) / 2);
+    for (long long a = 0; a < pairs; a += 256) v.push_back(ViscItem{b, 0, a, std::min(pairs, a + 256), 0, 0});
+  }
+  return v;
+}
+
+static int set_visc_queue(orion_hip_ctx* c, int seg, int grid) {
+  if (seg < 1 || grid < 1) return fail("viscous queue: segment and grid must be positive");
+  std::vector<ViscItem> v = visc_items(c, seg);
+  if (v.size() >= (size_t)INT32_MAX) return fail("viscous queue: too many items");
+  if ((long long)v.size() > c->vitems_cap) {
+    // sized once for the shortest segment the tuner tries, so later changes reuse it
+    const long long cap = std::max<long long>((long long)v.size(), (long long)visc_items(c, 8).size());
+    if (dalloc(c, (void**)&c->d_vitems, sizeof(ViscItem) * cap)) return -1;
+    c->vitems_cap = cap;
+  }
+  HIPCHK(hipMemcpyAsync(c->d_vitems, v.data(), sizeof(ViscItem) * v.size(), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->n_vitems = (int)v.size();
+  c->visc_seg = seg;
+  c->visc_grid = grid;
+  c->graph_stream = nullptr;  // the captured step holds the old count: capture again
+  return 0;
+}
+
 // LU-SGS sweep layout (k_lusgs.hip): per block the march axis is the longest
 // one and col is i whenever i is not the march axis; the (col, row)
 // cross-section is cut into 16 x 16 tiles.  Tiles are launched so that every
@@ -1025,6 +1075,9 @@ static int create_ctx(int nreq, const int* req_nblocks, int nblocks, const int* 
     if (upload_vec(c, &c->d_zeros, std::vector<int>((size_t)most, 0))) return bail(-1);
   }
   if (upload_vec(c, &c->db, c->hb)) return bail(-1);
+  if (c->p.nvis == 1) {
+    if (dalloc(c, (void**)&c->d_vhead, sizeof(int)) || set_visc_queue(c, c->visc_seg, c->visc_grid)) return bail(-1);
+  }
   if (build_topology(c, R)) return bail(-1);
   timing("boundary topology (halo, BC, averaging, masks)");
   c->nparts = residual_parts(c->intr.total());
@@ -1213,8 +1266,8 @@ static void op_visc(orion_hip_ctx* c, bool first = false) {
     }
     return;
   }
-  launch_visc_deriv(LA(c, c->intr), c->intr.total());
-  launch_visc_face(LA3(c, c->fpair3), -1, c->fpair3.total());
+  launch_visc_deriv(LA(c, c->intr), c->intr.total(), c->d_vhead);
+  launch_visc_face_queue(c->db, c->d_vitems, c->n_vitems, c->d_vhead, c->visc_grid, c->stream);
   launch_visc_diff(LA(c, c->dtile), c->dtile.total(), first);
 }
 
@@ -1515,8 +1568,8 @@ int orion_hip_profile_run(orion_hip_ctx* c, int nsteps, double* phase_ms, long l
     op_bc(c); mark((long long)c->bc_first.size());
     launch_state(LA(c, c->pad), c->pad.total(), 3 | 4, c->d_dtmin + (long long)s * c->nreq, nullptr, c->nreq); mark(1);
     if (c->p.nvis == 1) {
-      launch_visc_deriv(LA(c, c->intr), c->intr.total()); mark(1);
-      launch_visc_face(LA3(c, c->fpair3), -1, c->fpair3.total());
+      launch_visc_deriv(LA(c, c->intr), c->intr.total(), c->d_vhead); mark(1);
+      launch_visc_face_queue(c->db, c->d_vitems, c->n_vitems, c->d_vhead, c->visc_grid, c->stream);
       mark(1);
       launch_visc_diff(LA(c, c->dtile), c->dtile.total(), true); mark(1);
       op_avg(c); mark(2);
@@ -1684,37 +1737,58 @@ int orion_hip_set_schedule(orion_hip_ctx* c, int which) {
 
 int orion_hip_get_schedule(const orion_hip_ctx* c) { return c ? c->schedule : -1; }
 
-int orion_hip_tune_schedule(orion_hip_ctx* c, int steps, int* chosen, double* ms_per_step) {
-  if (!c || steps <= 0) return fail("tune_schedule: bad context or step count");
-  // the state to return to after every trial: each block's prim
-  std::vector<long long> at(c->nb + 1, 0);
-  for (int b = 0; b < c->nb; ++b) at[b + 1] = at[b] + 6 * c->hb[b].P;
+// Trials of the autotuners: each runs one untimed step (capture and warm-up)
+// and `steps` timed steps from the state saved at the start, then restores
+// it (prim alone fixes the next step).
+struct TrialState {
+  orion_hip_ctx* c;
+  std::vector<long long> at;
   double* saved = nullptr;
-  if (hipMalloc((void**)&saved, sizeof(double) * at[c->nb]) != hipSuccess)
-    return fail("tune_schedule: no memory for the state copy");
-  auto copy_state = [&](bool save) -> int {
+  explicit TrialState(orion_hip_ctx* ctx) : c(ctx), at(ctx->nb + 1, 0) {
+    for (int b = 0; b < c->nb; ++b) at[b + 1] = at[b] + 6 * c->hb[b].P;
+  }
+  ~TrialState() {
+    if (saved) {
+      hipStreamSynchronize(c->stream);
+      hipFree(saved);
+    }
+  }
+  int copy(bool save) {
     for (int b = 0; b < c->nb; ++b) {
       double* prim = c->hb[b].prim;
       double* dst = save ? saved + at[b] : prim;
       const double* src = save ? prim : saved + at[b];
       if (hipMemcpyAsync(dst, src, sizeof(double) * 6 * c->hb[b].P, hipMemcpyDeviceToDevice, c->stream) != hipSuccess)
-        return fail("tune_schedule: state copy failed");
+        return fail("tuning: state copy failed");
     }
-    return hipStreamSynchronize(c->stream) == hipSuccess ? 0 : fail("tune_schedule: state copy failed");
-  };
-  int rc = copy_state(true);
-  int best = c->schedule;
-  double best_ms = 0.0;
-  for (int k = 0; rc == 0 && k < ORION_SCHEDULES; ++k) {
-    rc = orion_hip_set_schedule(c, k);
-    if (rc == 0) rc = orion_hip_run(c, 1, nullptr);  // untimed: capture and warm the step
-    double ms = 0.0;
+    return hipStreamSynchronize(c->stream) == hipSuccess ? 0 : fail("tuning: state copy failed");
+  }
+  int begin() {
+    if (hipMalloc((void**)&saved, sizeof(double) * at[c->nb]) != hipSuccess)
+      return fail("tuning: no memory for the state copy");
+    return copy(true);
+  }
+  int trial(int steps, double* ms) {
+    int rc = orion_hip_run(c, 1, nullptr);
     if (rc == 0) {
       const auto t0 = std::chrono::steady_clock::now();
       rc = orion_hip_run(c, steps, nullptr);  // synchronises at the end
-      ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / steps;
+      *ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / steps;
     }
-    if (rc == 0) rc = copy_state(false);
+    return rc == 0 ? copy(false) : rc;
+  }
+};
+
+int orion_hip_tune_schedule(orion_hip_ctx* c, int steps, int* chosen, double* ms_per_step) {
+  if (!c || steps <= 0) return fail("tune_schedule: bad context or step count");
+  TrialState T(c);
+  int rc = T.begin();
+  int best = c->schedule;
+  double best_ms = 0.0;
+  for (int k = 0; rc == 0 && k < ORION_SCHEDULES; ++k) {
+    double ms = 0.0;
+    rc = orion_hip_set_schedule(c, k);
+    if (rc == 0) rc = T.trial(steps, &ms);
     if (rc == 0) {
       if (ms_per_step) ms_per_step[k] = ms;
       if (k == 0 || ms < best_ms) {
@@ -1724,9 +1798,81 @@ int orion_hip_tune_schedule(orion_hip_ctx* c, int steps, int* chosen, double* ms
     }
   }
   if (rc == 0) rc = orion_hip_set_schedule(c, best);
-  hipStreamSynchronize(c->stream);
-  hipFree(saved);
   if (rc == 0 && chosen) *chosen = best;
+  return rc;
+}
+
+int orion_hip_set_visc_tiles(orion_hip_ctx* c, int segment, int grid) {
+  if (!c) return fail("set_visc_tiles: null context");
+  if (c->p.nvis != 1) return 0;  // no viscous flux, nothing to tile
+  if (segment == c->visc_seg && grid == c->visc_grid) return 0;
+  return set_visc_queue(c, segment, grid);
+}
+
+int orion_hip_get_visc_tiles(const orion_hip_ctx* c, int* segment, int* grid) {
+  if (!c) return fail("get_visc_tiles: null context");
+  if (segment) *segment = c->visc_seg;
+  if (grid) *grid = c->visc_grid;
+  return 0;
+}
+
+int orion_hip_tune(orion_hip_ctx* c, int steps, int* schedule, int* segment, int* grid, double* best_ms,
+                   int* trials) {
+  if (!c || steps <= 0) return fail("tune: bad context or step count");
+  TrialState T(c);
+  int rc = T.begin();
+  double best = -1.0;
+  int n = 0;
+  auto measure = [&](double* ms) {
+    const int r = T.trial(steps, ms);
+    ++n;
+    return r;
+  };
+  // the sweep schedule
+  int bs = c->schedule;
+  for (int k = 0; rc == 0 && k < ORION_SCHEDULES; ++k) {
+    double ms = 0.0;
+    rc = orion_hip_set_schedule(c, k);
+    if (rc == 0) rc = measure(&ms);
+    if (rc == 0 && (best < 0.0 || ms < best)) {
+      best = ms;
+      bs = k;
+    }
+  }
+  if (rc == 0) rc = orion_hip_set_schedule(c, bs);
+  // the viscous faces per marching item, then the persistent grid
+  if (c->p.nvis == 1) {
+    int bseg = c->visc_seg, bgrid = c->visc_grid;
+    static const int SEGS[] = {8, 16, 32, 64};
+    static const int GRIDS[] = {1024, 2048, 4096};
+    for (int seg : SEGS) {
+      if (rc != 0 || seg == bseg) continue;
+      double ms = 0.0;
+      rc = orion_hip_set_visc_tiles(c, seg, bgrid);
+      if (rc == 0) rc = measure(&ms);
+      if (rc == 0 && ms < best) {
+        best = ms;
+        bseg = seg;
+      }
+    }
+    for (int g : GRIDS) {
+      if (rc != 0 || g == bgrid) continue;
+      double ms = 0.0;
+      rc = orion_hip_set_visc_tiles(c, bseg, g);
+      if (rc == 0) rc = measure(&ms);
+      if (rc == 0 && ms < best) {
+        best = ms;
+        bgrid = g;
+      }
+    }
+    if (rc == 0) rc = orion_hip_set_visc_tiles(c, bseg, bgrid);
+  }
+  if (rc == 0) {
+    if (schedule) *schedule = c->schedule;
+    orion_hip_get_visc_tiles(c, segment, grid);
+    if (best_ms) *best_ms = best;
+    if (trials) *trials = n;
+  }
   return rc;
 }
 

@@ -450,7 +450,9 @@ __device__ inline double node_deriv(const double* q, long long s, int n, int i1,
 }
 
 __global__ void __launch_bounds__(256) visc_deriv_kernel(const BlockDev* __restrict__ blocks,
-                                                         Span SP) {
+                                                         Span SP, int* queue_head) {
+  // the face-flux work queue, which runs next, starts from its first item
+  if (queue_head && blockIdx.x == 0 && threadIdx.x == 0) *queue_head = 0;
   int bi;
   long long gi;
   if (!span_item(SP, &bi, &gi)) return;
@@ -549,22 +551,77 @@ __device__ inline double half_deriv_c(const double* q, long long s, int n, int f
   return (27.0 * (c3 - c2) + -1.0 * (c4 - c1)) / 24.0;
 }
 
-// Viscous face fluxes (compute_viscous_flux_1d, FluxComputer.cpp:183-252)
-// of faces f0 = 2g and f0 + 1 of one line: the two faces share their
-// stencil windows (positions f0+1..f0+5), and the face geometry is the
-// static one interpolated after the metrics.
-__global__ void __launch_bounds__(256) visc_face_kernel(const BlockDev* __restrict__ blocks,
-                                                        Span SP, int d) {
-  int bi;
-  long long gi;
-  if (!span_item(SP, &bi, &gi)) return;
-  if (d < 0) {  // the three directions in one launch: virtual block d * nb + b
-    const int nbr = SP.nb / 3;
-    d = bi / nbr;
-    bi -= d * nbr;
+// One viscous face flux (compute_viscous_flux_1d, FluxComputer.cpp:183-252)
+// at face f of a line along d, from the windows at positions f+1..f+4:
+// Wv(v, j) of u, v, w, T, Mv(j) of mu and Xv(c, j) of the cross
+// derivatives (c < 4: along o0, c >= 4: along o1, at positions clamped to
+// the interior), j = 0..3; the face geometry is the static one
+// interpolated after the metrics.  Writes the five face fluxes of face f.
+template <class WF, class MF, class XF>
+__device__ __forceinline__ void visc_face_at(const BlockDev& b, int d, long long base, long long s, int n, int f,
+                                             bool fl, bool fr, const WF& Wv, const MF& Mv, const XF& Xv) {
+  const long long P = b.P;
+  const int o0 = (d == 0) ? 1 : 0, o1 = (d == 2) ? 1 : 2;
+  double uvwt[4], dmid[4];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const double* q = b.prim + (long long)(v < 3 ? v + 1 : 5) * P + base;
+    uvwt[v] = half_flagged_c(q, s, n, f, fl, fr, Wv(v, 0), Wv(v, 1), Wv(v, 2), Wv(v, 3));
+    dmid[v] = half_deriv_c(q, s, n, f, fl, fr, Wv(v, 0), Wv(v, 1), Wv(v, 2), Wv(v, 3));
   }
-  const BlockDev& b = blocks[bi];
-  const double cp_prl = b.cp_prl;
+  const double vs1 = (9.0 * (Mv(1) + Mv(2)) + -1.0 * (Mv(3) + Mv(0))) / 16.0;
+  const double vs2 = vs1 * b.cp_prl;
+  const double* G = b.fgeo + (long long)(d * FG_N) * P + base + (long long)(f + 2) * s;
+  const double volh = G[9 * P];
+  double kx[9];
+#pragma unroll
+  for (int m = 0; m < 9; ++m) kx[m] = G[(long long)m * P];
+  double duv[12];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    duv[4 * d + v] = dmid[v];
+    duv[4 * o0 + v] = (9.0 * (Xv(v, 1) + Xv(v, 2)) + -1.0 * (Xv(v, 3) + Xv(v, 0))) / 16.0;
+    duv[4 * o1 + v] = (9.0 * (Xv(4 + v, 1) + Xv(4 + v, 2)) + -1.0 * (Xv(4 + v, 3) + Xv(4 + v, 0))) / 16.0;
+  }
+  // transform_derivs_to_phys (:183-205)
+  const double vinv = 1.0 / volh;
+  double gx[12];
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    const double k1 = kx[m], k2 = kx[m + 3], k3 = kx[m + 6];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) gx[v * 3 + m] = (k1 * duv[v] + k2 * duv[v + 4] + k3 * duv[v + 8]) * vinv;
+  }
+  // compute_viscous_flux_1d (:210-252)
+  const double vscc = vs1 * (2.0 / 3.0);
+  const double dudx = gx[0], dudy = gx[1], dudz = gx[2];
+  const double dvdx = gx[3], dvdy = gx[4], dvdz = gx[5];
+  const double dwdx = gx[6], dwdy = gx[7], dwdz = gx[8];
+  const double txx = vscc * (2.0 * dudx - dvdy - dwdz);
+  const double tyy = vscc * (2.0 * dvdy - dwdz - dudx);
+  const double tzz = vscc * (2.0 * dwdz - dudx - dvdy);
+  const double txy = vs1 * (dudy + dvdx);
+  const double txz = vs1 * (dudz + dwdx);
+  const double tyz = vs1 * (dvdz + dwdy);
+  const double nx = kx[3 * d], ny = kx[3 * d + 1], nz = kx[3 * d + 2];
+  const double f1 = txx * nx + txy * ny + txz * nz;
+  const double f2 = txy * nx + tyy * ny + tyz * nz;
+  const double f3 = txz * nx + tyz * ny + tzz * nz;
+  const double work = uvwt[0] * f1 + uvwt[1] * f2 + uvwt[2] * f3;
+  const double heat = vs2 * (gx[9] * nx + gx[10] * ny + gx[11] * nz);
+  double* F = b.fx + (long long)d * 5 * P + base + (long long)(f + 2) * s;
+  F[0] = 0.0;
+  F[P] = f1;
+  F[2 * P] = f2;
+  F[3 * P] = f3;
+  F[4 * P] = work + heat;
+}
+
+__device__ __forceinline__ int clamp_interior(int p, int n) { return p < NG ? NG : (p > n + NG - 1 ? n + NG - 1 : p); }
+
+// Faces f0 = 2g and f0 + 1 of one line along d: the two faces share their
+// stencil windows (positions f0+1..f0+5).
+__device__ __forceinline__ void visc_face_pair(const BlockDev& b, int d, long long gi) {
   long long base;
   int f0;
   decode_face_pair(b, d, gi, &base, &f0);
@@ -573,10 +630,7 @@ __global__ void __launch_bounds__(256) visc_face_kernel(const BlockDev* __restri
   const long long P = b.P;
   const bool fl = b.prim[base] < 0.0;
   const bool fr = b.prim[base + (long long)(n + 5) * s] < 0.0;
-  // windows at positions f0+1..f0+5 (the cross derivatives at positions
-  // clamped to the interior [NG, n+NG-1])
   double W[4][5], M[5], X[8][5];
-  auto clampp = [n](int p) { return p < NG ? NG : (p > n + NG - 1 ? n + NG - 1 : p); };
   const int o0 = (d == 0) ? 1 : 0, o1 = (d == 2) ? 1 : 2;
 #pragma unroll
   for (int j = 0; j < 5; ++j) {
@@ -584,7 +638,7 @@ __global__ void __launch_bounds__(256) visc_face_kernel(const BlockDev* __restri
 #pragma unroll
     for (int v = 0; v < 4; ++v) W[v][j] = b.prim[(long long)(v < 3 ? v + 1 : 5) * P + base + pos];
     M[j] = b.mu[base + pos];
-    const long long pc = (long long)clampp(f0 + 1 + j) * s;
+    const long long pc = (long long)clamp_interior(f0 + 1 + j, n) * s;
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       X[v][j] = b.duvwt[(long long)(4 * o0 + v) * P + base + pc];
@@ -595,59 +649,99 @@ __global__ void __launch_bounds__(256) visc_face_kernel(const BlockDev* __restri
   for (int k = 0; k < 2; ++k) {
     const int f = f0 + k;
     if (f > n) break;
-    double uvwt[4], dmid[4];
+    visc_face_at(b, d, base, s, n, f, fl, fr, [&](int v, int j) { return W[v][k + j]; },
+                 [&](int j) { return M[k + j]; }, [&](int c, int j) { return X[c][k + j]; });
+  }
+}
+
+// Viscous face fluxes over a span of face pairs (one launch per direction,
+// or, d < 0, the three directions in one launch: virtual block d * nb + b).
+__global__ void __launch_bounds__(256) visc_face_kernel(const BlockDev* __restrict__ blocks,
+                                                        Span SP, int d) {
+  int bi;
+  long long gi;
+  if (!span_item(SP, &bi, &gi)) return;
+  if (d < 0) {
+    const int nbr = SP.nb / 3;
+    d = bi / nbr;
+    bi -= d * nbr;
+  }
+  visc_face_pair(blocks[bi], d, gi);
+}
+
+// Faces f0 .. f1-1 of one line along d (j or k) in order: the windows of
+// the thirteen stencil arrays slide one position per face, so every value
+// is loaded once, and the 64 lanes of a wavefront hold neighbouring lines
+// (loads of consecutive i).
+__device__ __forceinline__ void visc_face_march(const BlockDev& b, int d, long long line, int f0, int f1) {
+  unsigned q, r;
+  udivmod((unsigned)line, (unsigned)b.idim, &q, &r);
+  const long long base = (d == 1) ? pidx(b, (int)r + NG, 0, (int)q + NG) : pidx(b, (int)r + NG, (int)q + NG, 0);
+  const int n = dcount(b, d);
+  const long long s = dstride(b, d);
+  const long long P = b.P;
+  const bool fl = b.prim[base] < 0.0;
+  const bool fr = b.prim[base + (long long)(n + 5) * s] < 0.0;
+  const int o0 = (d == 0) ? 1 : 0, o1 = (d == 2) ? 1 : 2;
+  double W[4][4], M[4], X[8][4];
+  auto load = [&](int slot, int p) {
+    const long long pos = (long long)p * s;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) W[v][slot] = b.prim[(long long)(v < 3 ? v + 1 : 5) * P + base + pos];
+    M[slot] = b.mu[base + pos];
+    const long long pc = (long long)clamp_interior(p, n) * s;
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-      const double* q = b.prim + (long long)(v < 3 ? v + 1 : 5) * P + base;
-      uvwt[v] = half_flagged_c(q, s, n, f, fl, fr, W[v][k], W[v][k + 1], W[v][k + 2], W[v][k + 3]);
-      dmid[v] = half_deriv_c(q, s, n, f, fl, fr, W[v][k], W[v][k + 1], W[v][k + 2], W[v][k + 3]);
+      X[v][slot] = b.duvwt[(long long)(4 * o0 + v) * P + base + pc];
+      X[4 + v][slot] = b.duvwt[(long long)(4 * o1 + v) * P + base + pc];
     }
-    const double vs1 = (9.0 * (M[k + 1] + M[k + 2]) + -1.0 * (M[k + 3] + M[k])) / 16.0;
-    const double vs2 = vs1 * cp_prl;
-    const double* G = b.fgeo + (long long)(d * FG_N) * P + base + (long long)(f + 2) * s;
-    const double volh = G[9 * P];
-    double kx[9];
+  };
 #pragma unroll
-    for (int m = 0; m < 9; ++m) kx[m] = G[(long long)m * P];
-    double duv[12];
+  for (int j = 0; j < 4; ++j) load(j, f0 + 1 + j);
+  for (int f = f0; f < f1; ++f) {
+    visc_face_at(b, d, base, s, n, f, fl, fr, [&](int v, int j) { return W[v][j]; }, [&](int j) { return M[j]; },
+                 [&](int c, int j) { return X[c][j]; });
+    if (f + 1 < f1) {
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      duv[4 * d + v] = dmid[v];
-      duv[4 * o0 + v] = (9.0 * (X[v][k + 1] + X[v][k + 2]) + -1.0 * (X[v][k + 3] + X[v][k])) / 16.0;
-      duv[4 * o1 + v] = (9.0 * (X[4 + v][k + 1] + X[4 + v][k + 2]) + -1.0 * (X[4 + v][k + 3] + X[4 + v][k])) / 16.0;
+      for (int j = 0; j < 3; ++j) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) W[v][j] = W[v][j + 1];
+        M[j] = M[j + 1];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) X[c][j] = X[c][j + 1];
+      }
+      load(3, f + 5);
     }
-    // transform_derivs_to_phys (:183-205)
-    const double vinv = 1.0 / volh;
-    double gx[12];
-#pragma unroll
-    for (int m = 0; m < 3; ++m) {
-      const double k1 = kx[m], k2 = kx[m + 3], k3 = kx[m + 6];
-#pragma unroll
-      for (int v = 0; v < 4; ++v) gx[v * 3 + m] = (k1 * duv[v] + k2 * duv[v + 4] + k3 * duv[v + 8]) * vinv;
+  }
+}
+
+// The viscous face fluxes of every block and direction as one persistent
+// launch fed from a block work queue: each workgroup takes the next item
+// (an atomic ticket on *head) until the queue is empty.  An item is
+//   d = 0: 256 consecutive face pairs of block b's lines along i (one pair
+//          per lane, as visc_face_kernel);
+//   d = 1, 2: 256 consecutive lines along j or k (lane = line, i fastest)
+//          over faces f0 .. f1-1, each lane marching along its line.
+// *head is zeroed by the derivative kernel that runs before this one.
+__global__ void __launch_bounds__(256) visc_face_queue_kernel(const BlockDev* __restrict__ blocks,
+                                                              const ViscItem* __restrict__ items, int nitems,
+                                                              int* head) {
+  __shared__ int s_item;
+  for (;;) {
+    if (threadIdx.x == 0) s_item = atomicAdd(head, 1);
+    __syncthreads();
+    const int it = s_item;
+    __syncthreads();
+    if (it >= nitems) return;
+    const ViscItem Q = items[it];
+    const BlockDev& b = blocks[Q.block];
+    const long long g = Q.a0 + threadIdx.x;
+    if (g < Q.a1) {
+      if (Q.d == 0)
+        visc_face_pair(b, 0, g);
+      else
+        visc_face_march(b, Q.d, g, Q.f0, Q.f1);
     }
-    // compute_viscous_flux_1d (:210-252)
-    const double vscc = vs1 * (2.0 / 3.0);
-    const double dudx = gx[0], dudy = gx[1], dudz = gx[2];
-    const double dvdx = gx[3], dvdy = gx[4], dvdz = gx[5];
-    const double dwdx = gx[6], dwdy = gx[7], dwdz = gx[8];
-    const double txx = vscc * (2.0 * dudx - dvdy - dwdz);
-    const double tyy = vscc * (2.0 * dvdy - dwdz - dudx);
-    const double tzz = vscc * (2.0 * dwdz - dudx - dvdy);
-    const double txy = vs1 * (dudy + dvdx);
-    const double txz = vs1 * (dudz + dwdx);
-    const double tyz = vs1 * (dvdz + dwdy);
-    const double nx = kx[3 * d], ny = kx[3 * d + 1], nz = kx[3 * d + 2];
-    const double f1 = txx * nx + txy * ny + txz * nz;
-    const double f2 = txy * nx + tyy * ny + tyz * nz;
-    const double f3 = txz * nx + tyz * ny + tzz * nz;
-    const double work = uvwt[0] * f1 + uvwt[1] * f2 + uvwt[2] * f3;
-    const double heat = vs2 * (gx[9] * nx + gx[10] * ny + gx[11] * nz);
-    double* F = b.fx + (long long)d * 5 * P + base + (long long)(f + 2) * s;
-    F[0] = 0.0;
-    F[P] = f1;
-    F[2 * P] = f2;
-    F[3 * P] = f3;
-    F[4 * P] = work + heat;
   }
 }
 
@@ -771,9 +865,16 @@ void launch_inv_diff(const Launch& L, long long n, bool first) {
   else
     hipLaunchKernelGGL((diff_kernel<true, false>), dim3((unsigned)L.nwg), dim3(SPAN_TB), 0, L.stream, L.blocks, L.span);
 }
-void launch_visc_deriv(const Launch& L, long long n) {
+void launch_visc_deriv(const Launch& L, long long n, int* queue_head) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(visc_deriv_kernel, dim3((unsigned)L.nwg), dim3(SPAN_TB), 0, L.stream, L.blocks, L.span);
+  hipLaunchKernelGGL(visc_deriv_kernel, dim3((unsigned)L.nwg), dim3(SPAN_TB), 0, L.stream, L.blocks, L.span,
+                     queue_head);
+}
+void launch_visc_face_queue(const BlockDev* blocks, const ViscItem* items, int nitems, int* head, int grid,
+                            hipStream_t s) {
+  if (nitems <= 0) return;
+  const int g = grid < nitems ? grid : nitems;
+  hipLaunchKernelGGL(visc_face_queue_kernel, dim3((unsigned)g), dim3(256), 0, s, blocks, items, nitems, head);
 }
 void launch_face_geom(const Launch& L, int d, long long n) {
   if (n <= 0) return;

@@ -44,7 +44,19 @@ void launch_inv_closure(const Launch& L, int d, long long n, double gamma);
 // first: the step's first right-hand-side kernel, starting from dq = 0
 // without reading it (the step's state kernel then skips the zeroing)
 void launch_inv_diff(const Launch& L, long long n, bool first);
-void launch_visc_deriv(const Launch& L, long long n);
+// queue_head: when not null, the viscous face work queue's ticket counter,
+// zeroed by this launch
+void launch_visc_deriv(const Launch& L, long long n, int* queue_head = nullptr);
+// one item of the viscous face-flux work queue (visc_face_queue_kernel):
+// d = 0: face pairs [a0, a1) of block's lines along i; d = 1, 2: lines
+// [a0, a1) along j or k (i fastest), faces [f0, f1)
+struct ViscItem {
+  int block, d;
+  long long a0, a1;
+  int f0, f1;
+};
+void launch_visc_face_queue(const BlockDev* blocks, const ViscItem* items, int nitems, int* head, int grid,
+                            hipStream_t s);
 void launch_face_geom(const Launch& L, int d, long long n);
 void launch_visc_face(const Launch& L, int d, long long n);
 void launch_visc_diff(const Launch& L, long long n, bool first);

@@ -178,6 +178,27 @@ class FlowFieldSet:
         are the same bit for bit."""
         check(_lib.load().orion_hip_set_per_block(self.ctx, int(bool(on))), "set_per_block")
 
+    def set_visc_tiles(self, segment: int, grid: int) -> None:
+        """The viscous face-flux stage's tiling: faces per marching item and
+        workgroups of its persistent launch (results do not depend on it)."""
+        check(_lib.load().orion_hip_set_visc_tiles(self.ctx, int(segment), int(grid)), "set_visc_tiles")
+
+    def visc_tiles(self):
+        seg, grid = C.c_int(), C.c_int()
+        check(_lib.load().orion_hip_get_visc_tiles(self.ctx, C.byref(seg), C.byref(grid)), "get_visc_tiles")
+        return seg.value, grid.value
+
+    def tune(self, steps: int = 3) -> dict:
+        """The launch-shape autotuner (orion_hip_tune): sweep schedule, then
+        viscous segment, then its grid, each kept when faster; the state is
+        restored after every trial."""
+        sch, seg, grid, n = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        ms = C.c_double()
+        check(_lib.load().orion_hip_tune(self.ctx, int(steps), C.byref(sch), C.byref(seg), C.byref(grid),
+                                         C.byref(ms), C.byref(n)), "tune")
+        return {"schedule": self.SCHEDULES[sch.value], "visc_segment": seg.value, "visc_grid": grid.value,
+                "ms_per_step": ms.value, "trials": n.value, "steps_per_trial": int(steps)}
+
     SCHEDULES = ("diagonal", "critical_path")
 
     def set_schedule(self, which: int) -> None:
