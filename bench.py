@@ -51,7 +51,7 @@ HBM_PEAK_GBS = 8000.0
 # kernels that implement it and the profiled phases they run in.
 STAGES = {
     "rhs": (20, ["visc_deriv", "visc_face", "visc_diff", "inv_face", "inv_closure", "inv_diff"],
-            "visc_deriv_kernel+visc_face_kernel+diff_kernel<false,true>+inv_face_kernel+inv_closure_kernel"
+            "visc_deriv_kernel+visc_face_queue_kernel+diff_kernel<false,true>+inv_face_kernel+inv_closure_kernel"
             "+diff_kernel<true,false>"),
     "lu_fwd": (25, ["lu_pack", "lu_fwd"], "lu_pack_kernel+lusgs_tile_kernel<0,true>"),
     "lu_bwd": (25, ["lu_bwd"], "lusgs_tile_kernel<1,true>"),
@@ -158,7 +158,7 @@ def _kernel_key(name):
     return n.split("::")[-1].replace(" ", "")
 
 
-def summarize_counters(fetch_csv, write_csv, cfetch_csv, cwrite_csv, steps, cmd):
+def summarize_counters(fetch_csv, write_csv, cfetch_csv, cwrite_csv, steps, cmd, commit=""):
     """Per-kernel HBM bytes per step from two rocprofv3 --pmc runs
     (FETCH_SIZE, WRITE_SIZE, KiB per dispatch), each scaled by what the same
     counter reads for a plain device copy of known size."""
@@ -183,6 +183,7 @@ def summarize_counters(fetch_csv, write_csv, cfetch_csv, cwrite_csv, steps, cmd)
     f, w = load(fetch_csv), load(write_csv)
     lines = ["# HBM traffic per kernel (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE)",
              "# command: " + cmd,
+             "# commit: " + (commit or "unknown"),
              "# calibration: a %d-byte device copy reads FETCH_SIZE x %.4f and WRITE_SIZE x %.4f bytes" % (
                  copy_bytes, sf, sw),
              "# columns: kernel dispatches_per_step fetch_bytes_per_step write_bytes_per_step",
@@ -201,7 +202,10 @@ def stage_traffic(workload, groups):
     if not os.path.exists(path):
         return {}
     per = {}
+    commit = "unknown"
     for line in open(path):
+        if line.startswith("# commit: "):
+            commit = line.split(":", 1)[1].strip()
         parts = line.split()
         if line.startswith("#") or len(parts) != 4:
             continue
@@ -212,7 +216,8 @@ def stage_traffic(workload, groups):
             continue
         nbytes = sum(per[k][1] for k in ks)
         launches = sum(per[k][0] for k in ks)
-        out[name] = {"bytes_per_step": nbytes, "bytes_per_launch": nbytes / max(launches, 1)}
+        out[name] = {"bytes_per_step": nbytes, "bytes_per_launch": nbytes / max(launches, 1),
+                     "source": "%s (counters of the tree at commit %s)" % (os.path.relpath(path, ROOT), commit)}
     return out
 
 
@@ -347,11 +352,13 @@ def main():
                     help="write the per-kernel HBM traffic summary from rocprofv3 counter CSVs and exit")
     ap.add_argument("--counter-steps", type=int, default=2, help="steps covered by the counter run")
     ap.add_argument("--counter-command", default="", help="command recorded in the summary")
+    ap.add_argument("--counter-commit", default="", help="commit of the tree the counters were measured on")
     a = ap.parse_args()
     if a.profile_steps < 1:
         ap.error("--profile-steps must be at least 1: the rooflines come from the per-phase timings")
     if a.summarize_counters:
-        sys.stdout.write(summarize_counters(*a.summarize_counters, a.counter_steps, a.counter_command))
+        sys.stdout.write(summarize_counters(*a.summarize_counters, a.counter_steps, a.counter_command,
+                                            a.counter_commit))
         return 0
     if a.gpus != 1:
         print("bench.py: this build measures one GPU only (the path shards by block across GPUs, which is "
@@ -436,6 +443,7 @@ def main():
     dom = stage_traffic(a.workload, {"dominant": DOMINANT[3]})
     if dom:
         roof["traffic"] = dom["dominant"]["bytes_per_launch"]
+        roof["traffic_source"] = dom["dominant"]["source"]
     out = {
         "metric": METRIC,
         "value": value,
