@@ -53,7 +53,8 @@ STAGES = {
     "rhs": (20, ["visc_deriv", "visc_face", "visc_diff", "inv_face", "inv_closure", "inv_diff"],
             "visc_deriv_kernel+visc_face_queue_kernel+diff_kernel<false,true>+inv_face_kernel+inv_closure_kernel"
             "+diff_kernel<true,false>"),
-    "lu_fwd": (25, ["lu_pack", "lu_fwd"], "lu_pack_kernel+lusgs_tile_kernel<0,true>"),
+    "lu_fwd": (25, ["lu_pack_state", "lu_pack", "lu_fwd"],
+               "lu_pack_kernel<1>+lu_pack_kernel<2>+lusgs_tile_kernel<0,true>"),
     "lu_bwd": (25, ["lu_bwd"], "lusgs_tile_kernel<1,true>"),
     "gs_fwd": (30, ["gs_fwd"], "lusgs_tile_kernel<2,true>"),
     "gs_bwd": (30, ["gs_bwd"], "lusgs_tile_kernel<3,true>"),
@@ -150,6 +151,7 @@ def copy_calibration(mib):
 
 
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "hbm_traffic_{workload}.txt")
+DEFAULT_WORKLOAD = "columns64"
 
 
 def _kernel_key(name):
@@ -335,7 +337,7 @@ def main():
                     help="steps per trial of the launch-shape autotuner after the warmup (0: no tuning)")
     ap.add_argument("--schedule", type=int, default=-1,
                     help="fix the sweep schedule (0 diagonal, 1 critical path) instead of tuning")
-    ap.add_argument("--workload", default="columns64")
+    ap.add_argument("--workload", default=DEFAULT_WORKLOAD)
     ap.add_argument("--profile-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)

@@ -154,9 +154,9 @@ int orion_hip_run(orion_hip_ctx* ctx, int nsteps, orion_residual_stats* hist);
  * like orion_hip_run with HIP events between the kernels of each phase and
  * returns, per phase, the summed device time (ms) and the number of kernel
  * launches.  Phases, in order: halo, bc, state+dt, visc_deriv, visc_face,
- * visc_diff, avg1, inv_face, inv_closure, inv_diff, avg2, lu_pack, lu_fwd,
- * lu_bwd, gs_fwd, gs_bwd, update, residual. */
-#define ORION_PHASES 18
+ * visc_diff, avg1, inv_face, inv_closure, lu_pack_state, inv_diff, avg2,
+ * lu_pack, lu_fwd, lu_bwd, gs_fwd, gs_bwd, update, residual. */
+#define ORION_PHASES 19
 int orion_hip_profile_run(orion_hip_ctx* ctx, int nsteps, double* phase_ms, long long* phase_launches);
 
 /* No counterpart in the reference: the launch order of the LU-SGS tiles.

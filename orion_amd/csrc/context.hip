@@ -74,7 +74,7 @@ struct orion_hip_ctx {
   hipStream_t own_stream = nullptr;
   hipStream_t stream = nullptr;
   hipStream_t side_stream = nullptr;  // inviscid face fluxes beside the viscous chain (captured step)
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_state = nullptr;
   int nb = 0;
   std::vector<BlockDev> hb;
   BlockDev* db = nullptr;
@@ -664,6 +664,9 @@ static int build_sweep(orion_hip_ctx* c) {
     B.sbase = n;
     const long long planes = (long long)d[0] + d[1] + d[2] - 2;
     if (planes >= 60000) return fail("block %d: %lld hyperplanes exceed the sweep limit", b + 1, planes);
+    // the sweep stores a block's dq through a buffer resource with 32-bit byte offsets
+    if (planes * B.sn[2] * ((B.sn[1] + TWD - 1) / TWD * TWD) * SW_DQ * 16 >= (1LL << 32))
+      return fail("block %d: its LU-SGS sweep arrays exceed 4 GiB", b + 1);
     n += planes * B.sn[2] * B.sncp;
     const int tc = (B.sn[1] + TWD - 1) / TWD, tr = (B.sn[2] + TRD - 1) / TRD;
     const int first = (int)c->tiles.size();
@@ -812,6 +815,7 @@ void orion_hip_ctx_destroy(orion_hip_ctx* c) {
   if (c->graph) hipGraphDestroy(c->graph);
   if (c->ev_fork) hipEventDestroy(c->ev_fork);
   if (c->ev_join) hipEventDestroy(c->ev_join);
+  if (c->ev_state) hipEventDestroy(c->ev_state);
   give_stream(c->device, c->side_stream);
   give_stream(c->device, c->own_stream);
   delete c;
@@ -876,7 +880,8 @@ static int create_ctx(int nreq, const int* req_nblocks, int nblocks, const int* 
   if (take_stream(c->device, &c->own_stream) || take_stream(c->device, &c->side_stream))
     return bail(fail("hipStreamCreate failed"));
   if (hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess)
+      hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_state, hipEventDisableTiming) != hipSuccess)
     return bail(fail("hipEventCreate failed"));
   c->stream = c->own_stream;
   tstream = c->stream;
@@ -909,6 +914,9 @@ static int create_ctx(int nreq, const int* req_nblocks, int nblocks, const int* 
     B.ny = B.jdim + 2 * NG;
     B.nz = B.kdim + 2 * NG;
     B.P = (long long)B.nx * B.ny * B.nz;
+    if (B.P >= MAX_BLOCK_POINTS)
+      return bail(fail("block %d: %lld padded points (kernels address a block's fields with 32-bit offsets: "
+                       "fewer than %lld accepted)", b, B.P, MAX_BLOCK_POINTS));
     lines_tot[b] = nlines(B, 0) + nlines(B, 1) + nlines(B, 2);
     int r = 0;
     while (c->req_first[r + 1] <= b) ++r;
@@ -1304,13 +1312,16 @@ static void op_sweep(orion_hip_ctx* c, int type) {
                      c->d_err, type, c->p, c->stream);
 }
 
-static void op_update(orion_hip_ctx* c) {
+// state_packed: the step already launched the state part of the pack (on
+// the side stream, beside the right-hand side)
+static void op_update(orion_hip_ctx* c, bool state_packed = false) {
   if (c->per_block) {
     // the reference's update_solution loop: pack, four sweeps and the
     // limited update of one block, then the next (StateUpdater.cpp:544-580);
     // the partials go to the block's own slots of the step's partials
     for (int b = 0; b < c->nb; ++b) {
-      launch_lu_pack(LAB(c, c->ptile, b), items_of(c->ptile, b), c->sw);
+      launch_lu_pack(LAB(c, c->ptile, b), items_of(c->ptile, b), c->sw, 1);
+      launch_lu_pack(LAB(c, c->ptile, b), items_of(c->ptile, b), c->sw, 2);
       for (int t = 0; t < 4; ++t) {
         const int bwd = (t == 1 || t == 3) ? 1 : 0;
         launch_lusgs_tiles(c->sw, c->db, c->d_tiles, c->d_border[bwd] + c->tile_first[b], c->tile_count[b],
@@ -1322,7 +1333,8 @@ static void op_update(orion_hip_ctx* c) {
     }
     return;
   }
-  launch_lu_pack(LA(c, c->ptile), c->ptile.total(), c->sw);
+  if (!state_packed) launch_lu_pack(LA(c, c->ptile), c->ptile.total(), c->sw, 1);
+  launch_lu_pack(LA(c, c->ptile), c->ptile.total(), c->sw, 2);
   for (int t = 0; t < 4; ++t) op_sweep(c, t);
   launch_update(LA(c, c->ptile), c->ptile.total(), c->sw, c->p.gama, c->intr.d, c->d_upart,
                 c->d_upart + c->ptile.nwg, c->d_ukey, c->d_epoch);
@@ -1474,16 +1486,25 @@ static void step_body(orion_hip_ctx* c) {
     // (the state kernel writes T, c, mu, q and dt), write their own flux
     // arrays, and join before the inviscid difference, the first kernel
     // after them that touches dq.  dq is thus updated in the operators'
-    // order.
+    // order.  After them the side stream packs the LU-SGS state part (it
+    // reads the state kernel's mu and dt, so it waits for that kernel, and
+    // writes only the sweep arrays, which nothing reads before the sweeps).
     hipEventRecord(c->ev_fork, c->stream);
     hipStreamWaitEvent(c->side_stream, c->ev_fork, 0);
     op_inv_faces(c, c->side_stream);
-    hipEventRecord(c->ev_join, c->side_stream);
     launch_state(LA(c, c->pad), c->pad.total(), 3 | 4, c->d_dtmin, c->d_step, c->nreq);
+    hipEventRecord(c->ev_state, c->stream);
+    hipStreamWaitEvent(c->side_stream, c->ev_state, 0);
+    launch_lu_pack(LA(c, c->ptile, c->side_stream), c->ptile.total(), c->sw, 1);
+    hipEventRecord(c->ev_join, c->side_stream);
     op_visc(c, true);
     op_avg(c);
     hipStreamWaitEvent(c->stream, c->ev_join, 0);
     launch_inv_diff(LA(c, c->dtile), c->dtile.total(), false);
+    op_avg(c);
+    op_update(c, true);
+    op_update_residual(c, c->d_res, c->d_step);  // and advances *d_step
+    return;
   } else {
     launch_state(LA(c, c->pad), c->pad.total(), 3 | 4, c->d_dtmin, c->d_step, c->nreq);
     op_inv(c, true);
@@ -1580,9 +1601,10 @@ int orion_hip_profile_run(orion_hip_ctx* c, int nsteps, double* phase_ms, long l
     mark(1);
     launch_inv_closure(LA3(c, c->ltask3), -1, c->ltask3.total(), c->p.gama);
     mark(1);
+    launch_lu_pack(LA(c, c->ptile), c->ptile.total(), c->sw, 1); mark(1);
     launch_inv_diff(LA(c, c->dtile), c->dtile.total(), c->p.nvis != 1); mark(1);
     op_avg(c); mark(2);
-    launch_lu_pack(LA(c, c->ptile), c->ptile.total(), c->sw); mark(1);
+    launch_lu_pack(LA(c, c->ptile), c->ptile.total(), c->sw, 2); mark(1);
     for (int t = 0; t < 4; ++t) { op_sweep(c, t); mark(1); }
     launch_update(LA(c, c->ptile), c->ptile.total(), c->sw, c->p.gama, c->intr.d, c->d_upart,
                   c->d_upart + c->ptile.nwg, c->d_ukey, c->d_epoch);

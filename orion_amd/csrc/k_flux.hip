@@ -530,19 +530,20 @@ __global__ void __launch_bounds__(256) face_geom_kernel(const BlockDev* __restri
 }
 
 // interp_line_to_half / compute_half_deriv_line at face f with the central
-// stencil values c1..c4 (positions f+1..f+4) already loaded
-__device__ inline double half_flagged_c(const double* q, long long s, int n, int f, bool fl, bool fr, double c1,
-                                        double c2, double c3, double c4) {
-  auto Q = [&](int p) { return q[(long long)p * s]; };
+// stencil values c1..c4 (positions f+1..f+4) already loaded; Q(p) loads the
+// line's value at position p (the boundary forms only)
+template <class QF>
+__device__ __forceinline__ double half_flagged_c(const QF& Q, int n, int f, bool fl, bool fr, double c1, double c2,
+                                                 double c3, double c4) {
   if (fl && f == 0) return (35.0 * Q(3) + -35.0 * Q(4) + 21.0 * Q(5) + -5.0 * Q(6)) / 16.0;
   if (fl && f == 1) return (5.0 * Q(3) + 15.0 * Q(4) + -5.0 * Q(5) + 1.0 * Q(6)) / 16.0;
   if (fr && f == n) return (35.0 * Q(n + 2) + -35.0 * Q(n + 1) + 21.0 * Q(n) + -5.0 * Q(n - 1)) / 16.0;
   if (fr && f == n - 1) return (5.0 * Q(n + 2) + 15.0 * Q(n + 1) + -5.0 * Q(n) + 1.0 * Q(n - 1)) / 16.0;
   return (9.0 * (c2 + c3) + -1.0 * (c4 + c1)) / 16.0;
 }
-__device__ inline double half_deriv_c(const double* q, long long s, int n, int f, bool fl, bool fr, double c1,
-                                      double c2, double c3, double c4) {
-  auto Q = [&](int p) { return q[(long long)p * s]; };
+template <class QF>
+__device__ __forceinline__ double half_deriv_c(const QF& Q, int n, int f, bool fl, bool fr, double c1, double c2,
+                                               double c3, double c4) {
   if (fl && f == 0) return (-71.0 * Q(3) + 141.0 * Q(4) + -93.0 * Q(5) + 23.0 * Q(6)) / 24.0;
   if (fl && f == 1) return (-22.0 * Q(3) + 17.0 * Q(4) + 9.0 * Q(5) + -5.0 * Q(6) + 1.0 * Q(7)) / 24.0;
   if (fr && f == n) return -(-71.0 * Q(n + 2) + 141.0 * Q(n + 1) + -93.0 * Q(n) + 23.0 * Q(n - 1)) / 24.0;
@@ -551,31 +552,42 @@ __device__ inline double half_deriv_c(const double* q, long long s, int n, int f
   return (27.0 * (c3 - c2) + -1.0 * (c4 - c1)) / 24.0;
 }
 
+// prim field of viscous variable v (u, v, w, T = prim 1, 2, 3, 5)
+__device__ __forceinline__ constexpr unsigned uvwt_field(int v) { return v < 3 ? (unsigned)v + 1u : 5u; }
+
 // One viscous face flux (compute_viscous_flux_1d, FluxComputer.cpp:183-252)
-// at face f of a line along d, from the windows at positions f+1..f+4:
+// at face f of a line along D, from the windows at positions f+1..f+4:
 // Wv(v, j) of u, v, w, T, Mv(j) of mu and Xv(c, j) of the cross
 // derivatives (c < 4: along o0, c >= 4: along o1, at positions clamped to
 // the interior), j = 0..3; the face geometry is the static one
 // interpolated after the metrics.  Writes the five face fluxes of face f.
-template <class WF, class MF, class XF>
-__device__ __forceinline__ void visc_face_at(const BlockDev& b, int d, long long base, long long s, int n, int f,
+// base is the padded index of the line's position 0 and s the stride
+// along D (32-bit: field offsets are formed with boff).  The direction is a
+// template parameter: with it the derivative and metric arrays are indexed
+// by constants and stay in registers (a run-time index puts them in
+// scratch memory).
+template <int D, class WF, class MF, class XF>
+__device__ __forceinline__ void visc_face_at(const BlockDev& b, unsigned base, unsigned s, int n, int f,
                                              bool fl, bool fr, const WF& Wv, const MF& Mv, const XF& Xv) {
-  const long long P = b.P;
-  const int o0 = (d == 0) ? 1 : 0, o1 = (d == 2) ? 1 : 2;
+  constexpr int d = D;
+  const unsigned P = (unsigned)b.P;
+  constexpr int o0 = (d == 0) ? 1 : 0, o1 = (d == 2) ? 1 : 2;
+  const double* prim = b.prim;
   double uvwt[4], dmid[4];
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
-    const double* q = b.prim + (long long)(v < 3 ? v + 1 : 5) * P + base;
-    uvwt[v] = half_flagged_c(q, s, n, f, fl, fr, Wv(v, 0), Wv(v, 1), Wv(v, 2), Wv(v, 3));
-    dmid[v] = half_deriv_c(q, s, n, f, fl, fr, Wv(v, 0), Wv(v, 1), Wv(v, 2), Wv(v, 3));
+    auto Q = [&](int p) { return ldo(prim, boff(uvwt_field(v), P, base + (unsigned)p * s)); };
+    uvwt[v] = half_flagged_c(Q, n, f, fl, fr, Wv(v, 0), Wv(v, 1), Wv(v, 2), Wv(v, 3));
+    dmid[v] = half_deriv_c(Q, n, f, fl, fr, Wv(v, 0), Wv(v, 1), Wv(v, 2), Wv(v, 3));
   }
   const double vs1 = (9.0 * (Mv(1) + Mv(2)) + -1.0 * (Mv(3) + Mv(0))) / 16.0;
   const double vs2 = vs1 * b.cp_prl;
-  const double* G = b.fgeo + (long long)(d * FG_N) * P + base + (long long)(f + 2) * s;
-  const double volh = G[9 * P];
+  const double* G = b.fgeo + (long long)(d * FG_N) * b.P;  // this direction's face geometry
+  const unsigned g = base + (unsigned)(f + 2) * s;
+  const double volh = ldo(G, boff(9, P, g));
   double kx[9];
 #pragma unroll
-  for (int m = 0; m < 9; ++m) kx[m] = G[(long long)m * P];
+  for (int m = 0; m < 9; ++m) kx[m] = ldo(G, boff(m, P, g));
   double duv[12];
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
@@ -609,48 +621,57 @@ __device__ __forceinline__ void visc_face_at(const BlockDev& b, int d, long long
   const double f3 = txz * nx + tyz * ny + tzz * nz;
   const double work = uvwt[0] * f1 + uvwt[1] * f2 + uvwt[2] * f3;
   const double heat = vs2 * (gx[9] * nx + gx[10] * ny + gx[11] * nz);
-  double* F = b.fx + (long long)d * 5 * P + base + (long long)(f + 2) * s;
-  F[0] = 0.0;
-  F[P] = f1;
-  F[2 * P] = f2;
-  F[3 * P] = f3;
-  F[4 * P] = work + heat;
+  double* F = b.fx + (long long)d * 5 * b.P;  // this direction's face fluxes
+  sto(F, boff(0, P, g), 0.0);
+  sto(F, boff(1, P, g), f1);
+  sto(F, boff(2, P, g), f2);
+  sto(F, boff(3, P, g), f3);
+  sto(F, boff(4, P, g), work + heat);
 }
 
 __device__ __forceinline__ int clamp_interior(int p, int n) { return p < NG ? NG : (p > n + NG - 1 ? n + NG - 1 : p); }
 
-// Faces f0 = 2g and f0 + 1 of one line along d: the two faces share their
-// stencil windows (positions f0+1..f0+5).
-__device__ __forceinline__ void visc_face_pair(const BlockDev& b, int d, long long gi) {
-  long long base;
-  int f0;
-  decode_face_pair(b, d, gi, &base, &f0);
-  const int n = dcount(b, d);
-  const long long s = dstride(b, d);
-  const long long P = b.P;
-  const bool fl = b.prim[base] < 0.0;
-  const bool fr = b.prim[base + (long long)(n + 5) * s] < 0.0;
-  double W[4][5], M[5], X[8][5];
-  const int o0 = (d == 0) ? 1 : 0, o1 = (d == 2) ? 1 : 2;
+// the stencil values of line position p: u, v, w, T and mu at p, the cross
+// derivatives at p clamped to the interior
+template <int D>
+__device__ __forceinline__ void visc_window_load(const BlockDev& b, unsigned base, unsigned s, int n, int p,
+                                                 double* W4, double& M, double* X8) {
+  constexpr int o0 = (D == 0) ? 1 : 0, o1 = (D == 2) ? 1 : 2;
+  const unsigned P = (unsigned)b.P;
+  const unsigned e = base + (unsigned)p * s;
 #pragma unroll
-  for (int j = 0; j < 5; ++j) {
-    const long long pos = (long long)(f0 + 1 + j) * s;
+  for (int v = 0; v < 4; ++v) W4[v] = ldo(b.prim, boff(uvwt_field(v), P, e));
+  M = ldo(b.mu, e << 3);
+  const unsigned ec = base + (unsigned)clamp_interior(p, n) * s;
 #pragma unroll
-    for (int v = 0; v < 4; ++v) W[v][j] = b.prim[(long long)(v < 3 ? v + 1 : 5) * P + base + pos];
-    M[j] = b.mu[base + pos];
-    const long long pc = (long long)clamp_interior(f0 + 1 + j, n) * s;
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      X[v][j] = b.duvwt[(long long)(4 * o0 + v) * P + base + pc];
-      X[4 + v][j] = b.duvwt[(long long)(4 * o1 + v) * P + base + pc];
-    }
+  for (int v = 0; v < 4; ++v) {
+    X8[v] = ldo(b.duvwt, boff(4 * o0 + v, P, ec));
+    X8[4 + v] = ldo(b.duvwt, boff(4 * o1 + v, P, ec));
   }
+}
+
+// Faces f0 = 2g and f0 + 1 of one line along D: the two faces share their
+// stencil windows (positions f0+1..f0+5).
+template <int D>
+__device__ __forceinline__ void visc_face_pair(const BlockDev& b, long long gi) {
+  constexpr int d = D;
+  long long base64;
+  int f0;
+  decode_face_pair(b, d, gi, &base64, &f0);
+  const unsigned base = (unsigned)base64;
+  const int n = dcount(b, d);
+  const unsigned s = (unsigned)dstride(b, d);
+  const bool fl = ldo(b.prim, base << 3) < 0.0;
+  const bool fr = ldo(b.prim, (base + (unsigned)(n + 5) * s) << 3) < 0.0;
+  double W[5][4], M[5], X[5][8];
+#pragma unroll
+  for (int j = 0; j < 5; ++j) visc_window_load<D>(b, base, s, n, f0 + 1 + j, W[j], M[j], X[j]);
 #pragma unroll
   for (int k = 0; k < 2; ++k) {
     const int f = f0 + k;
     if (f > n) break;
-    visc_face_at(b, d, base, s, n, f, fl, fr, [&](int v, int j) { return W[v][k + j]; },
-                 [&](int j) { return M[k + j]; }, [&](int c, int j) { return X[c][k + j]; });
+    visc_face_at<D>(b, base, s, n, f, fl, fr, [&](int v, int j) { return W[k + j][v]; },
+                    [&](int j) { return M[k + j]; }, [&](int c, int j) { return X[k + j][c]; });
   }
 }
 
@@ -666,51 +687,45 @@ __global__ void __launch_bounds__(256) visc_face_kernel(const BlockDev* __restri
     d = bi / nbr;
     bi -= d * nbr;
   }
-  visc_face_pair(blocks[bi], d, gi);
+  if (d == 0)
+    visc_face_pair<0>(blocks[bi], gi);
+  else if (d == 1)
+    visc_face_pair<1>(blocks[bi], gi);
+  else
+    visc_face_pair<2>(blocks[bi], gi);
 }
 
-// Faces f0 .. f1-1 of one line along d (j or k) in order: the windows of
+// Faces f0 .. f1-1 of one line along D (j or k) in order: the windows of
 // the thirteen stencil arrays slide one position per face, so every value
 // is loaded once, and the 64 lanes of a wavefront hold neighbouring lines
 // (loads of consecutive i).
-__device__ __forceinline__ void visc_face_march(const BlockDev& b, int d, long long line, int f0, int f1) {
+template <int D>
+__device__ __forceinline__ void visc_face_march(const BlockDev& b, long long line, int f0, int f1) {
+  constexpr int d = D;
   unsigned q, r;
   udivmod((unsigned)line, (unsigned)b.idim, &q, &r);
-  const long long base = (d == 1) ? pidx(b, (int)r + NG, 0, (int)q + NG) : pidx(b, (int)r + NG, (int)q + NG, 0);
+  const unsigned base =
+      (unsigned)((d == 1) ? pidx(b, (int)r + NG, 0, (int)q + NG) : pidx(b, (int)r + NG, (int)q + NG, 0));
   const int n = dcount(b, d);
-  const long long s = dstride(b, d);
-  const long long P = b.P;
-  const bool fl = b.prim[base] < 0.0;
-  const bool fr = b.prim[base + (long long)(n + 5) * s] < 0.0;
-  const int o0 = (d == 0) ? 1 : 0, o1 = (d == 2) ? 1 : 2;
-  double W[4][4], M[4], X[8][4];
-  auto load = [&](int slot, int p) {
-    const long long pos = (long long)p * s;
+  const unsigned s = (unsigned)dstride(b, d);
+  const bool fl = ldo(b.prim, base << 3) < 0.0;
+  const bool fr = ldo(b.prim, (base + (unsigned)(n + 5) * s) << 3) < 0.0;
+  double W[4][4], M[4], X[4][8];
 #pragma unroll
-    for (int v = 0; v < 4; ++v) W[v][slot] = b.prim[(long long)(v < 3 ? v + 1 : 5) * P + base + pos];
-    M[slot] = b.mu[base + pos];
-    const long long pc = (long long)clamp_interior(p, n) * s;
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      X[v][slot] = b.duvwt[(long long)(4 * o0 + v) * P + base + pc];
-      X[4 + v][slot] = b.duvwt[(long long)(4 * o1 + v) * P + base + pc];
-    }
-  };
-#pragma unroll
-  for (int j = 0; j < 4; ++j) load(j, f0 + 1 + j);
+  for (int j = 0; j < 4; ++j) visc_window_load<D>(b, base, s, n, f0 + 1 + j, W[j], M[j], X[j]);
   for (int f = f0; f < f1; ++f) {
-    visc_face_at(b, d, base, s, n, f, fl, fr, [&](int v, int j) { return W[v][j]; }, [&](int j) { return M[j]; },
-                 [&](int c, int j) { return X[c][j]; });
+    visc_face_at<D>(b, base, s, n, f, fl, fr, [&](int v, int j) { return W[j][v]; }, [&](int j) { return M[j]; },
+                    [&](int c, int j) { return X[j][c]; });
     if (f + 1 < f1) {
 #pragma unroll
       for (int j = 0; j < 3; ++j) {
 #pragma unroll
-        for (int v = 0; v < 4; ++v) W[v][j] = W[v][j + 1];
+        for (int v = 0; v < 4; ++v) W[j][v] = W[j + 1][v];
         M[j] = M[j + 1];
 #pragma unroll
-        for (int c = 0; c < 8; ++c) X[c][j] = X[c][j + 1];
+        for (int c = 0; c < 8; ++c) X[j][c] = X[j + 1][c];
       }
-      load(3, f + 5);
+      visc_window_load<D>(b, base, s, n, f + 5, W[3], M[3], X[3]);
     }
   }
 }
@@ -730,7 +745,7 @@ __global__ void __launch_bounds__(256) visc_face_queue_kernel(const BlockDev* __
   for (;;) {
     if (threadIdx.x == 0) s_item = atomicAdd(head, 1);
     __syncthreads();
-    const int it = s_item;
+    const int it = __builtin_amdgcn_readfirstlane(s_item);  // the same for the whole workgroup
     __syncthreads();
     if (it >= nitems) return;
     const ViscItem Q = items[it];
@@ -738,9 +753,11 @@ __global__ void __launch_bounds__(256) visc_face_queue_kernel(const BlockDev* __
     const long long g = Q.a0 + threadIdx.x;
     if (g < Q.a1) {
       if (Q.d == 0)
-        visc_face_pair(b, 0, g);
+        visc_face_pair<0>(b, g);
+      else if (Q.d == 1)
+        visc_face_march<1>(b, g, Q.f0, Q.f1);
       else
-        visc_face_march(b, Q.d, g, Q.f0, Q.f1);
+        visc_face_march<2>(b, g, Q.f0, Q.f1);
     }
   }
 }

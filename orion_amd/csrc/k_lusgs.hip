@@ -20,7 +20,8 @@
 //    own previous cell.
 //  * Across tiles, a tile publishes how far it has come in a progress flag
 //    and its downstream tiles wait on it (bounded spin) before reading edge
-//    values.  dq is stored with agent-scope atomic stores, plane s is
+//    values.  dq is stored with the cache policy of agent-scope atomic
+//    stores (16-byte pair stores), plane s is
 //    published after every wave has waited for its stores of that plane,
 //    and edge dq is read with agent-scope atomic loads.  No cache
 //    maintenance is needed (fence-based release/acquire around plain stores
@@ -30,8 +31,9 @@
 //    that exceeds its bound records an error instead of hanging.
 //
 // Everything in a cell's update that does not depend on dq is either packed
-// once per update by lu_pack (the diagonal 1/(1/dt + sum of radii) and each
-// cell's u, v, w, c^2, H, a_f, c and viscous coefficient) or recomputed per
+// once per update by lu_pack's state part (the diagonal 1/(1/dt + sum of
+// radii) and each cell's u, v, w, c^2, H, a_f, c and viscous coefficient;
+// its rhs part copies the right-hand side) or recomputed per
 // neighbour slot in the sweep from this cell's metrics and the neighbour's
 // packed values, with the reference's expressions (calc_spectral_radius
 // :175-191, calc_visc_spectral_radius :194-207, compute_flux_splitting
@@ -150,15 +152,16 @@ __global__ void __launch_bounds__(256) lu_geom_kernel(const BlockDev* __restrict
   }
 }
 
-// packed words per cell and update: met8, coed, rhs 5 (own pairs 4..7) and
-// D 8; own pairs 0..3 are static
-constexpr int NPK = 15;
-
-// Per update: each cell's own inputs (metrics, the diagonal coed of
-// get_diag_coeff, the rhs) and, for its neighbours, u, v, w, c^2, H, a_f,
-// c and the viscous coefficient.
+// Per update, in two parts.  PART 1 (the state part, which depends only on
+// the flow state, the time step and the grid, so it can run beside the
+// right-hand side): each cell's met8 and diagonal coed of get_diag_coeff
+// (own pair 4) and, for its neighbours, u, v, w, c^2, H, a_f, c and the
+// viscous coefficient (D).  PART 2 (after the right-hand side): the rhs
+// copy (own pairs 5..7).
+template <int PART>
 __global__ void __launch_bounds__(256) lu_pack_kernel(const BlockDev* __restrict__ blocks,
                                                       Span SP, SweepArrays S) {
+  constexpr int NPK = PART == 1 ? 10 : 5;  // staged words per cell
   __shared__ double sh[NPK][PT * PT];
   int bi;
   long long gi;
@@ -173,51 +176,54 @@ __global__ void __launch_bounds__(256) lu_pack_kernel(const BlockDev* __restrict
     if (m < b.sn[0] && c < b.sn[1]) {
       const long long P = b.P;
       const long long id = natural_id(b, m, r, c);
-      const double gamma = p.gama, re = p.reynolds;
-      // rhs copy with the physical-BC windows zeroed (StateUpdater.cpp:557-570)
-      const bool fz = b.frozen[id] != 0;
+      if constexpr (PART == 2) {
+        // rhs copy with the physical-BC windows zeroed (StateUpdater.cpp:557-570)
+        const bool fz = b.frozen[id] != 0;
 #pragma unroll
-      for (int f = 0; f < 5; ++f) sh[2 + f][lane] = fz ? 0.0 : b.dq[f * P + id];
-      double pr[5];
+        for (int f = 0; f < 5; ++f) sh[f][lane] = fz ? 0.0 : b.dq[f * P + id];
+      } else {
+        const double gamma = p.gama, re = p.reynolds;
+        double pr[5];
 #pragma unroll
-      for (int f = 0; f < 5; ++f) pr[f] = b.prim[f * P + id];
-      double met[9];
+        for (int f = 0; f < 5; ++f) pr[f] = b.prim[f * P + id];
+        double met[9];
 #pragma unroll
-      for (int f = 0; f < 9; ++f) met[f] = b.met[f * P + id];
-      sh[0][lane] = met[8];
-      const double mu = b.mu[id];
-      double vol = b.vol[id];
-      if (vol < 1e-30) vol = 1e-30;
-      {  // get_diag_coeff
-        const double ra = spec_rad(pr, gamma, met[0], met[1], met[2]);
-        const double rb = spec_rad(pr, gamma, met[3], met[4], met[5]);
-        const double rc = spec_rad(pr, gamma, met[6], met[7], met[8]);
-        double rva = 0, rvb = 0, rvc = 0;
-        if (p.nvis == 1) {
-          rva = visc_rad(pr[0], mu, vol, re, met[0], met[1], met[2]);
-          rvb = visc_rad(pr[0], mu, vol, re, met[3], met[4], met[5]);
-          rvc = visc_rad(pr[0], mu, vol, re, met[6], met[7], met[8]);
+        for (int f = 0; f < 9; ++f) met[f] = b.met[f * P + id];
+        sh[0][lane] = met[8];
+        const double mu = b.mu[id];
+        double vol = b.vol[id];
+        if (vol < 1e-30) vol = 1e-30;
+        {  // get_diag_coeff
+          const double ra = spec_rad(pr, gamma, met[0], met[1], met[2]);
+          const double rb = spec_rad(pr, gamma, met[3], met[4], met[5]);
+          const double rc = spec_rad(pr, gamma, met[6], met[7], met[8]);
+          double rva = 0, rvb = 0, rvc = 0;
+          if (p.nvis == 1) {
+            rva = visc_rad(pr[0], mu, vol, re, met[0], met[1], met[2]);
+            rvb = visc_rad(pr[0], mu, vol, re, met[3], met[4], met[5]);
+            rvc = visc_rad(pr[0], mu, vol, re, met[6], met[7], met[8]);
+          }
+          const double rs = ra + rb + rc + rva + rvb + rvc;
+          const double term = (1.0 / b.dt[id]) + 1.0 * 1.0 * rs;
+          sh[1][lane] = 1.0 / term;
         }
-        const double rs = ra + rb + rc + rva + rvb + rvc;
-        const double term = (1.0 / b.dt[id]) + 1.0 * 1.0 * rs;
-        sh[1][lane] = 1.0 / term;
+        // this cell seen as a neighbour: compute_flux_splitting's per-cell
+        // terms, the sound speed of calc_spectral_radius and the viscous
+        // coefficient of calc_visc_spectral_radius (same clamps)
+        double rm = pr[0], pm = pr[4];
+        const double v2 = pr[1] * pr[1] + pr[2] * pr[2] + pr[3] * pr[3];
+        if (rm < 1e-30) rm = 1e-30;
+        if (pm < 1e-30) pm = 1e-30;
+        const double c2 = gamma * pm / rm;
+        sh[2][lane] = pr[1];
+        sh[3][lane] = pr[2];
+        sh[4][lane] = pr[3];
+        sh[5][lane] = c2;
+        sh[6][lane] = (gamma / (gamma - 1.0)) * (pm / rm) + 0.5 * v2;
+        sh[7][lane] = 0.5 * (gamma - 1.0) * v2;
+        sh[8][lane] = sqrt(c2);
+        sh[9][lane] = 2.0 * mu / (fabs(re) * (rm * vol) + 1.0e-30);
       }
-      // this cell seen as a neighbour: compute_flux_splitting's per-cell
-      // terms, the sound speed of calc_spectral_radius and the viscous
-      // coefficient of calc_visc_spectral_radius (same clamps)
-      double rm = pr[0], pm = pr[4];
-      const double v2 = pr[1] * pr[1] + pr[2] * pr[2] + pr[3] * pr[3];
-      if (rm < 1e-30) rm = 1e-30;
-      if (pm < 1e-30) pm = 1e-30;
-      const double c2 = gamma * pm / rm;
-      sh[7][lane] = pr[1];
-      sh[8][lane] = pr[2];
-      sh[9][lane] = pr[3];
-      sh[10][lane] = c2;
-      sh[11][lane] = (gamma / (gamma - 1.0)) * (pm / rm) + 0.5 * v2;
-      sh[12][lane] = 0.5 * (gamma - 1.0) * v2;
-      sh[13][lane] = sqrt(c2);
-      sh[14][lane] = 2.0 * mu / (fabs(re) * (rm * vol) + 1.0e-30);
     }
   }
   __syncthreads();
@@ -226,12 +232,16 @@ __global__ void __launch_bounds__(256) lu_pack_kernel(const BlockDev* __restrict
   if (m < b.sn[0] && c < b.sn[1]) {
     const int ncp = b.sncp;
     const long long rb = sweep_index(b, m, r, c) - c;
+    if constexpr (PART == 2) {
 #pragma unroll
-    for (int k = 0; k < 4; ++k)
-      S.own[rb * SW_OWN + (long long)(4 + k) * ncp + c] = make_double2(sh[2 * k][src], k < 3 ? sh[2 * k + 1][src] : 0.0);
+      for (int k = 0; k < 3; ++k)
+        S.own[rb * SW_OWN + (long long)(5 + k) * ncp + c] = make_double2(sh[2 * k][src], k < 2 ? sh[2 * k + 1][src] : 0.0);
+    } else {
+      S.own[rb * SW_OWN + 4LL * ncp + c] = make_double2(sh[0][src], sh[1][src]);
 #pragma unroll
-    for (int k = 0; k < SW_D; ++k)
-      S.D[rb * SW_D + (long long)k * ncp + c] = make_double2(sh[7 + 2 * k][src], sh[8 + 2 * k][src]);
+      for (int k = 0; k < SW_D; ++k)
+        S.D[rb * SW_D + (long long)k * ncp + c] = make_double2(sh[2 + 2 * k][src], sh[3 + 2 * k][src]);
+    }
   }
 }
 
@@ -460,6 +470,23 @@ __device__ __forceinline__ void dma_pair(const double2* g, long long rb, int c, 
                                    (__attribute__((address_space(3))) void*)wbase, 16, 0, 0);
 }
 
+// A block's dq pairs as a buffer resource, for 16-byte stores with the cache
+// policy of an agent-scope atomic store (sc1: written through to where
+// every XCD reads it).  Base: the block's first cell; the host checks that
+// a block's sweep arrays stay below 4 GiB, so byte offsets fit 32 bits.
+// Word 3 is the gfx9 raw-buffer format (32-bit data, no swizzle).
+typedef unsigned v4u32 __attribute__((ext_vector_type(4)));
+typedef double v2f64 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t dq_resource(const SweepArrays& S, const BlockDev& b) {
+  const long long bytes = (long long)(b.sn[0] + b.sn[1] + b.sn[2] - 2) * b.sn[2] * b.sncp * SW_DQ * 16;
+  return __builtin_amdgcn_make_buffer_rsrc(S.dq + b.sbase * SW_DQ, 0, (int)(unsigned)bytes, 0x00020000);
+}
+// one cell's pair (x, y) at byte offset off of the resource, coherent for the device
+__device__ __forceinline__ void store_pair_coherent(__amdgpu_buffer_rsrc_t r, unsigned off, double x, double y) {
+  const v2f64 v = {x, y};
+  __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(v4u32, v), r, off, 0, CPOL_SC1);
+}
+
 // the same with the cache policy of an agent-scope atomic load (sc1: the
 // load is served coherently for the whole device), for values that other
 // workgroups store while this one runs
@@ -556,6 +583,7 @@ __global__ void __launch_bounds__(2 * TL, 1) lusgs_tile_kernel(SweepArrays S, co
   const int nsteps = T.s_last - T.s_first + 1;
   const int s0 = FWD ? T.s_first : T.s_last;
   const int ncp = b.sncp;
+  const __amdgpu_buffer_rsrc_t dq_rsrc = dq_resource(S, b);
   const long long qstep = dir * (long long)nr * ncp;
   const int m0 = s0 - r - c;
   const long long rb0 = b.sbase + ((long long)s0 * nr + r) * ncp;
@@ -765,11 +793,15 @@ __global__ void __launch_bounds__(2 * TL, 1) lusgs_tile_kernel(SweepArrays S, co
     }
     if (cell_ok(t - 1)) {
       const long long q = rb0 + (long long)(t - 1) * qstep + c;
-      double* dqp = reinterpret_cast<double*>(S.dq);
-#pragma unroll
-      for (int f = 0; f < 5; ++f)
-        __hip_atomic_store(&dqp[sweep_field(q, c, ncp, SW_DQ, f)], dqB[(t - 1) & 1][f][lane], __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
+      // the cell's three dq pairs (the last one's second half is padding)
+      // as whole 16-byte stores: half-pair stores written through cost
+      // twice the write traffic
+      const unsigned off = (unsigned)(((q - c - b.sbase) * SW_DQ + c) * 16);
+      const unsigned kst = (unsigned)ncp * 16u;
+      const double* w = &dqB[(t - 1) & 1][0][lane];
+      store_pair_coherent(dq_rsrc, off, w[0], w[TL]);
+      store_pair_coherent(dq_rsrc, off + kst, w[2 * TL], w[3 * TL]);
+      store_pair_coherent(dq_rsrc, off + 2u * kst, w[4 * TL], 0.0);
       if constexpr (TYPE == 2) {  // the partial sums the update lanes left beside the new dq
         double2* pp = S.pre + (q - c) * SW_PRE + c;
         const double* w = &dqB[(t - 1) & 1][5][lane];
@@ -936,9 +968,12 @@ void launch_lu_geom(const Launch& La, long long n, const SweepArrays& S) {
   hipLaunchKernelGGL(lu_geom_kernel, dim3((unsigned)La.nwg), dim3(SPAN_TB), 0, La.stream, La.blocks, La.span, S);
 }
 
-void launch_lu_pack(const Launch& La, long long n, const SweepArrays& S) {
+void launch_lu_pack(const Launch& La, long long n, const SweepArrays& S, int part) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(lu_pack_kernel, dim3((unsigned)La.nwg), dim3(SPAN_TB), 0, La.stream, La.blocks, La.span, S);
+  if (part == 1)
+    hipLaunchKernelGGL(lu_pack_kernel<1>, dim3((unsigned)La.nwg), dim3(SPAN_TB), 0, La.stream, La.blocks, La.span, S);
+  else
+    hipLaunchKernelGGL(lu_pack_kernel<2>, dim3((unsigned)La.nwg), dim3(SPAN_TB), 0, La.stream, La.blocks, La.span, S);
 }
 
 void launch_update(const Launch& La, long long n, const SweepArrays& S, double gamma, const long long* ioff,

@@ -39,6 +39,23 @@ struct GPtr {
   __host__ __device__ operator T*() const { return (T*)p; }
 };
 
+// Accesses at a 32-bit byte offset from a wave-uniform global base.  A
+// block's field arrays hold at most 15 fields of P doubles and blocks are
+// held to P < 2^25 padded points (checked when a context is built), so a
+// word's byte offset from its array's base fits in 32 bits.  Written as
+// (uniform base) + (32-bit offset), an access is one global instruction
+// with a scalar base and a 32-bit vector offset; a 64-bit index instead
+// needs a 64-bit address (two registers and a 64-bit add) per access.
+constexpr long long MAX_BLOCK_POINTS = 1LL << 25;
+__device__ __forceinline__ double ldo(const double* base, unsigned byte_off) {
+  return *(const ORION_GLOBAL_AS double*)((const ORION_GLOBAL_AS char*)base + byte_off);
+}
+__device__ __forceinline__ void sto(double* base, unsigned byte_off, double v) {
+  *(ORION_GLOBAL_AS double*)((ORION_GLOBAL_AS char*)base + byte_off) = v;
+}
+// byte offset of element e of field k in an array of fields of P words
+__device__ __forceinline__ unsigned boff(unsigned k, unsigned P, unsigned e) { return (k * P + e) << 3; }
+
 struct Params {
   double gama, prl, moo, reynolds, visc, tref, twall, cfl;
   double roo, uoo, voo, woo, poo, too;

@@ -87,7 +87,8 @@ void launch_avg_apply(const BlockDev* blocks, const AvgDst* dst, long long n, co
 
 // k_lusgs.hip
 void launch_lu_geom(const Launch& La, long long n, const SweepArrays& S);
-void launch_lu_pack(const Launch& La, long long n, const SweepArrays& S);
+// part 1: met8, coed and D (state only); part 2: the rhs copy
+void launch_lu_pack(const Launch& La, long long n, const SweepArrays& S, int part);
 void launch_lusgs_tiles(const SweepArrays& S, const BlockDev* blocks, const SweepTile* tiles, const int* order,
                         int ntiles, unsigned long long* flags, unsigned long long* epoch, int* err, int type,
                         const Params& p, hipStream_t s);

@@ -1,7 +1,7 @@
 """Why the reference's multi-block solutions diverge where single blocks
 converge: its interface residual averaging runs once per interface window,
 in place (HaloExchanger::average_interface_residuals, phase 3,
-src/solver/HaloExchanger.cpp:319-395, from packs all taken first, :197-262).
+src/solver/HaloExchanger.cpp:327-395, from packs all taken first, :196-280).
 A node that lies on two interface windows of its block -- a block edge
 where two interfaces meet, or where an interface meets a periodic
 self-interface -- is averaged twice: its residual becomes

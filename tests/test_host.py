@@ -111,12 +111,13 @@ def test_bench_refuses_multi_gpu():
 
 def test_bench_stage_kernels_are_in_the_traffic_summary():
     """Every kernel the bench's stages and its dominant entry name appears in
-    the committed counter summary, so no stage silently loses its traffic."""
+    the committed counter summary of the default workload, so no stage
+    silently loses its traffic."""
     sys.path.insert(0, ROOT)
     import bench
     groups = {k: v[2].split("+") for k, v in bench.STAGES.items()}
     groups["dominant"] = bench.DOMINANT[3]
-    t = bench.stage_traffic("uneven64", groups)
+    t = bench.stage_traffic(bench.DEFAULT_WORKLOAD, groups)
     assert sorted(t) == sorted(groups), sorted(set(groups) - set(t))
     for name, v in t.items():
         assert v["bytes_per_launch"] > 0, name
