@@ -621,8 +621,9 @@ __device__ __forceinline__ void visc_face_at(const BlockDev& b, unsigned base, u
   const double f3 = txz * nx + tyz * ny + tzz * nz;
   const double work = uvwt[0] * f1 + uvwt[1] * f2 + uvwt[2] * f3;
   const double heat = vs2 * (gx[9] * nx + gx[10] * ny + gx[11] * nz);
-  double* F = b.fx + (long long)d * 5 * b.P;  // this direction's face fluxes
-  sto(F, boff(0, P, g), 0.0);
+  // this direction's face fluxes; the mass component is zero, is never
+  // stored, and the viscous difference skips it (dq0 - Re^-1 0 == dq0)
+  double* F = b.fx + (long long)d * 5 * b.P;
   sto(F, boff(1, P, g), f1);
   sto(F, boff(2, P, g), f2);
   sto(F, boff(3, P, g), f3);
@@ -782,11 +783,17 @@ __device__ __forceinline__ double diff_node_f(const FF& F, int n, int t) {
 // node.  Node t of a line reads faces t-2..t+3, which sit at the padded
 // slots t..t+5.  FIRST: the step's first right-hand-side kernel, which starts
 // from dq = 0 (what update_flow_states leaves) instead of reading it.
+// The viscous mass flux is identically zero (compute_viscous_flux_1d sets
+// it to 0): its difference is +-0, and dq0 - Re^-1 (+-0) is dq0 bit for bit
+// for every dq0 the step produces (dq starts from +0), so the viscous
+// difference leaves component 0 as it is and the face kernel never stores
+// it.
 template <bool INV, bool FIRST>
 __global__ void __launch_bounds__(256) diff_kernel(const BlockDev* __restrict__ blocks, Span SP) {
   constexpr int DT = 16, DW = DT + 5;
-  __shared__ double X[5][DT][DW];  // i slots i0-3 .. i0+17 of each row of the tile
-  __shared__ double Y[5][DW][DT];  // j slots j0-3 .. j0+17 of each column
+  constexpr int M0 = INV ? 0 : 1, NM = 5 - M0;  // flux components differenced
+  __shared__ double X[NM][DT][DW];  // i slots i0-3 .. i0+17 of each row of the tile
+  __shared__ double Y[NM][DW][DT];  // j slots j0-3 .. j0+17 of each column
   int bi;
   long long gi;
   if (!span_item(SP, &bi, &gi)) return;  // whole tiles: every lane of a workgroup has an item
@@ -802,19 +809,19 @@ __global__ void __launch_bounds__(256) diff_kernel(const BlockDev* __restrict__ 
   const int lane = threadIdx.x, li = lane % DT, lj = lane / DT;
   const int i = i0 + li, j = j0 + lj;
   const double* __restrict__ FX = INV ? b.fxi : b.fx;
-  for (int e = lane; e < 5 * DT * DW; e += 256) {
+  for (int e = lane; e < NM * DT * DW; e += 256) {
     const int m = e / (DT * DW), r = e % (DT * DW);
     {
       const int row = r / DW, col = r % DW;
       const int jj = j0 + row, ii = i0 - 3 + col;
       const bool ok = jj < NG + b.jdim && ii < b.nx;
-      X[m][row][col] = ok ? FX[(long long)m * P + pidx(b, ii, jj, k)] : 0.0;
+      X[m][row][col] = ok ? FX[(long long)(M0 + m) * P + pidx(b, ii, jj, k)] : 0.0;
     }
     {
       const int row = r / DT, col = r % DT;
       const int ii = i0 + col, jj = j0 - 3 + row;
       const bool ok = ii < NG + b.idim && jj < b.ny;
-      Y[m][row][col] = ok ? FX[(long long)(5 + m) * P + pidx(b, ii, jj, k)] : 0.0;
+      Y[m][row][col] = ok ? FX[(long long)(5 + M0 + m) * P + pidx(b, ii, jj, k)] : 0.0;
     }
   }
   __syncthreads();
@@ -822,7 +829,7 @@ __global__ void __launch_bounds__(256) diff_kernel(const BlockDev* __restrict__ 
   const long long id = pidx(b, i, j, k);
   double acc[5];
 #pragma unroll
-  for (int m = 0; m < 5; ++m) acc[m] = FIRST ? 0.0 : b.dq[(long long)m * P + id];
+  for (int m = 0; m < 5; ++m) acc[m] = FIRST ? 0.0 : (m < M0 ? 0.0 : b.dq[(long long)m * P + id]);
   const int pos[3] = {i, j, k};
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
@@ -838,12 +845,12 @@ __global__ void __launch_bounds__(256) diff_kernel(const BlockDev* __restrict__ 
       else if (rflag && t >= n - 3) cov = b.fcov + b.fcov_off[d] + line_id(b, d, i, j, k) * 30 + (3 + t - (n - 3)) * 5;
     }
 #pragma unroll
-    for (int m = 0; m < 5; ++m) {
+    for (int m = M0; m < 5; ++m) {
       double fc;
       if (d == 0) {
-        fc = diff_node_f([&](int f) { return X[m][lj][f + 2 - (i0 - 3)]; }, n, t);
+        fc = diff_node_f([&](int f) { return X[m - M0][lj][f + 2 - (i0 - 3)]; }, n, t);
       } else if (d == 1) {
-        fc = diff_node_f([&](int f) { return Y[m][f + 2 - (j0 - 3)][li]; }, n, t);
+        fc = diff_node_f([&](int f) { return Y[m - M0][f + 2 - (j0 - 3)][li]; }, n, t);
       } else {
         const double* F0 = FX + (long long)(10 + m) * P + lbase;
         fc = diff_node_f([&](int f) { return F0[(long long)(f + 2) * s]; }, n, t);
@@ -858,8 +865,11 @@ __global__ void __launch_bounds__(256) diff_kernel(const BlockDev* __restrict__ 
       }
     }
   }
+  // component 0 of the viscous difference: written (as +0) only by the
+  // step's first kernel, otherwise left as it is
 #pragma unroll
-  for (int m = 0; m < 5; ++m) b.dq[(long long)m * P + id] = acc[m];
+  for (int m = 0; m < 5; ++m)
+    if (m >= M0 || FIRST) b.dq[(long long)m * P + id] = acc[m];
 }
 
 // ---------------------------------------------------------------------------

@@ -155,13 +155,14 @@ __global__ void __launch_bounds__(256) lu_geom_kernel(const BlockDev* __restrict
 // Per update, in two parts.  PART 1 (the state part, which depends only on
 // the flow state, the time step and the grid, so it can run beside the
 // right-hand side): each cell's met8 and diagonal coed of get_diag_coeff
-// (own pair 4) and, for its neighbours, u, v, w, c^2, H, a_f, c and the
-// viscous coefficient (D).  PART 2 (after the right-hand side): the rhs
+// (own pair 4) and, for its neighbours, u, v, w, c^2, H and the viscous
+// coefficient (D; the sweep recomputes a_f and c from them with the same
+// expressions, so each value is the one packed before).  PART 2 (after the right-hand side): the rhs
 // copy (own pairs 5..7).
 template <int PART>
 __global__ void __launch_bounds__(256) lu_pack_kernel(const BlockDev* __restrict__ blocks,
                                                       Span SP, SweepArrays S) {
-  constexpr int NPK = PART == 1 ? 10 : 5;  // staged words per cell
+  constexpr int NPK = PART == 1 ? 8 : 5;  // staged words per cell
   __shared__ double sh[NPK][PT * PT];
   int bi;
   long long gi;
@@ -220,9 +221,7 @@ __global__ void __launch_bounds__(256) lu_pack_kernel(const BlockDev* __restrict
         sh[4][lane] = pr[3];
         sh[5][lane] = c2;
         sh[6][lane] = (gamma / (gamma - 1.0)) * (pm / rm) + 0.5 * v2;
-        sh[7][lane] = 0.5 * (gamma - 1.0) * v2;
-        sh[8][lane] = sqrt(c2);
-        sh[9][lane] = 2.0 * mu / (fabs(re) * (rm * vol) + 1.0e-30);
+        sh[7][lane] = 2.0 * mu / (fabs(re) * (rm * vol) + 1.0e-30);
       }
     }
   }
@@ -393,13 +392,18 @@ __device__ __forceinline__ void axis_geo(const double* n, AxisGeo& a) {
 }
 
 // n: this cell's metrics of the slot's axis; Dn, d: the neighbour's packed
-// values and dq
+// values and dq.  a_f = 0.5 (gamma - 1) |v|^2 and c = sqrt(c^2) are formed
+// here as compute_flux_splitting and calc_spectral_radius form them (and as
+// the pack formed them before: the same operations on the same values).
 template <int SL>
 __device__ __forceinline__ void slot_inv(const double* n, const AxisGeo& a, const double* Dn, const double* d,
                                          double gm1, double* res) {
   constexpr int sgn = SL < 3 ? 1 : -1;
   const double nx = n[0], ny = n[1], nz = n[2];
-  const double um = Dn[0], vm = Dn[1], wm = Dn[2], c2 = Dn[3], H = Dn[4], af = Dn[5], cm = Dn[6];
+  const double um = Dn[0], vm = Dn[1], wm = Dn[2], c2 = Dn[3], H = Dn[4];
+  const double v2 = um * um + vm * vm + wm * wm;
+  const double af = 0.5 * gm1 * v2;
+  const double cm = sqrt(c2);
   const double U = um * nx + vm * ny + wm * nz + 0.0;
   const double r = fabs(U) + cm * a.sq;
   const double ct = nx * um + ny * vm + nz * wm + 0.0;
@@ -425,7 +429,7 @@ __device__ __forceinline__ void slot_inv(const double* n, const AxisGeo& a, cons
 // ---------------------------------------------------------------------------
 // tile sweep: progress keys, waits and loads
 // ---------------------------------------------------------------------------
-constexpr int NW = 13;  // exchanged words per cell: D (8) + dq (5)
+constexpr int NW = 11;  // exchanged words per cell: D (6) + dq (5)
 
 __device__ inline unsigned long long plane_key(bool fwd, int s) {
   return fwd ? (unsigned long long)(s + 1) : (unsigned long long)((1 << KEY_BITS) - s);
@@ -556,7 +560,7 @@ __global__ void __launch_bounds__(2 * TL, 1) lusgs_tile_kernel(SweepArrays S, co
   __shared__ double2 Dp[2][SW_D][TL];
   constexpr int NQ = TYPE == 2 ? 10 : 5;  // new dq, then (Gauss-Seidel forward) the partial sums
   __shared__ double dqB[2][NQ][TL];
-  __shared__ double2 shXD[3][2][TW][SW_D];  // edge D by step mod 3: [role][index][pair]
+  __shared__ double2 shXD[3][128];           // edge D by step mod 3: [role][index][pair] (96 used)
   __shared__ double2 shXQ[3][128];           // edge dq by step mod 3: [role][index][pair] (96 used)
 
   auto ob = [](int u) { return OWN2 ? (u + 3) % 3 : u & 1; };  // own buffer of step u >= -1
@@ -595,48 +599,39 @@ __global__ void __launch_bounds__(2 * TL, 1) lusgs_tile_kernel(SweepArrays S, co
   auto cell_c = [&](int t) { return cell_ok(t) ? c : 0; };
 
   // Edge values, loaded by the first loader wave straight into LDS: the D
-  // pairs (x, x + 64: role x >> 6, index (x >> 2) & 15, pair x & 3) and the
-  // dq pairs (items x, x + 64; those below 96: role i / 48, index
-  // (i % 48) / 3, pair i % 3) of the cells across the upstream col edge
-  // (role 0) and row edge (role 1).  shXD[b][role][index][pair] and
-  // shXQ[b][item] put lane x of each load at 16 x bytes from the load's
-  // base.
-  int er[4], ec[4];
-  bool ein[4];
-  int ek[4];
+  // pairs and the dq pairs (three pairs each per cell; items x and x + 64 of
+  // each, those below 96: role i / 48, index (i % 48) / 3, pair i % 3) of
+  // the cells across the upstream col edge (role 0) and row edge (role 1).
+  // shXD[b][item] and shXQ[b][item] put lane x of each load at 16 x bytes
+  // from the load's base; item h of the lane is x + 64 h.
+  int er[2], ec[2];
+  bool ein[2];
+  int ek[2];
 #pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    int role, idx;
-    bool item = true;
-    if (j < 2) {
-      role = j;
-      idx = xl >> 2;
-      ek[j] = xl & 3;
-    } else {
-      const int i = xl + 64 * (j - 2);
-      item = i < 96;
-      role = i < 48 ? 0 : 1;
-      idx = (i % 48) / 3;
-      ek[j] = i % 3;
-    }
+  for (int h = 0; h < 2; ++h) {
+    const int i = xl + 64 * h;
+    bool item = i < 96;
+    const int role = i < 48 ? 0 : 1;
+    const int idx = (i % 48) / 3;
+    ek[h] = i % 3;
     if (role == 0) {  // the col edge has TR cells
       item = item && idx < TR;
-      ec[j] = FWD ? T.c0 - 1 : T.c0 + TW;
-      er[j] = T.r0 + idx;
+      ec[h] = FWD ? T.c0 - 1 : T.c0 + TW;
+      er[h] = T.r0 + idx;
     } else {
-      er[j] = FWD ? T.r0 - 1 : T.r0 + TR;
-      ec[j] = T.c0 + idx;
+      er[h] = FWD ? T.r0 - 1 : T.r0 + TR;
+      ec[h] = T.c0 + idx;
     }
-    ein[j] = lw0 && item && er[j] >= 0 && er[j] < nr && ec[j] >= 0 && ec[j] < nc;
+    ein[h] = lw0 && item && er[h] >= 0 && er[h] < nr && ec[h] >= 0 && ec[h] < nc;
   }
-  // the edge cell of item j for step u lies on plane s(u-1); out-of-range
+  // the edge cell of item h for step u lies on plane s(u-1); out-of-range
   // items read the block's first cell, which no lane reads as a neighbour
-  auto edge_rb = [&](int j, int u, int* cc) -> long long {
+  auto edge_rb = [&](int h, int u, int* cc) -> long long {
     const int sp = s0 + dir * (u - 1);
-    const int mm = sp - er[j] - ec[j];
-    const bool ok = ein[j] && u < nsteps && mm >= 0 && mm < nm;
-    *cc = ok ? ec[j] : 0;
-    return ok ? b.sbase + ((long long)sp * nr + er[j]) * ncp : b.sbase;
+    const int mm = sp - er[h] - ec[h];
+    const bool ok = ein[h] && u < nsteps && mm >= 0 && mm < nm;
+    *cc = ok ? ec[h] : 0;
+    return ok ? b.sbase + ((long long)sp * nr + er[h]) * ncp : b.sbase;
   };
   // first loader wave: the edge values of step u into LDS, four loads on
   // every lane (items 96..127 read the block's first cell into slots no lane
@@ -645,16 +640,16 @@ __global__ void __launch_bounds__(2 * TL, 1) lusgs_tile_kernel(SweepArrays S, co
   auto dma_edges = [&](int u) {
     const int e = u % 3;
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
+    for (int h = 0; h < 2; ++h) {
       int cc;
-      const long long rb = edge_rb(j, u, &cc);
-      dma_pair(S.D, rb, cc, ncp, SW_D, ek[j], &shXD[e][j][0][0]);
+      const long long rb = edge_rb(h, u, &cc);
+      dma_pair(S.D, rb, cc, ncp, SW_D, ek[h], &shXD[e][64 * h]);
     }
 #pragma unroll
-    for (int j = 2; j < 4; ++j) {
+    for (int h = 0; h < 2; ++h) {
       int cc;
-      const long long rb = edge_rb(j, u, &cc);
-      dma_pair_coherent(S.dq, rb, cc, ncp, SW_DQ, ek[j], &shXQ[e][64 * (j - 2)]);
+      const long long rb = edge_rb(h, u, &cc);
+      dma_pair_coherent(S.dq, rb, cc, ncp, SW_DQ, ek[h], &shXQ[e][64 * h]);
     }
   };
   // the own inputs of the cell of step u into ownP[ob(u)], as pairs:
@@ -729,7 +724,7 @@ __global__ void __launch_bounds__(2 * TL, 1) lusgs_tile_kernel(SweepArrays S, co
     const int ln = march ? lane : lane + (col ? d : d * TW);
     const int role = col ? 0 : 1, idx = col ? lr : lc;
     const int e = t % 3;
-    const double2* bd = inside ? &Dp[p ^ 1][0][ln] : &shXD[e][role][idx][0];
+    const double2* bd = inside ? &Dp[p ^ 1][0][ln] : &shXD[e][(role * TW + idx) * SW_D];
     const double* bq = inside ? &dqB[p ^ 1][0][ln] : reinterpret_cast<const double*>(&shXQ[e][(role * TW + idx) * 3]);
     const int st = inside ? TL : 1;
 #pragma unroll
@@ -739,7 +734,7 @@ __global__ void __launch_bounds__(2 * TL, 1) lusgs_tile_kernel(SweepArrays S, co
       v[2 * k + 1] = x.y;
     }
 #pragma unroll
-    for (int f = 0; f < 5; ++f) v[8 + f] = bq[f * st];
+    for (int f = 0; f < 5; ++f) v[2 * SW_D + f] = bq[f * st];
     return march ? (m + d >= 0 && m + d < nm) : (coord + d >= 0 && coord + d < lim);
   };
 
@@ -870,11 +865,11 @@ __global__ void __launch_bounds__(2 * TL, 1) lusgs_tile_kernel(SweepArrays S, co
         double v[NW];
         const bool ok = nbr(k, m, t, v);
         double inv[5], tv[5];
-        slot_inv<SL0>(o + 3 * k, ag, v, v + 8, gm1, inv);
+        slot_inv<SL0>(o + 3 * k, ag, v, v + 2 * SW_D, gm1, inv);
         if (VIS) {
-          const double hv = 0.5 * (v[7] * ag.g2);
+          const double hv = 0.5 * (v[2 * SW_D - 1] * ag.g2);
 #pragma unroll
-          for (int f = 0; f < 5; ++f) tv[f] = hv * v[8 + f];
+          for (int f = 0; f < 5; ++f) tv[f] = hv * v[2 * SW_D + f];
         }
 #pragma unroll
         for (int f = 0; f < 5; ++f) {

@@ -116,7 +116,7 @@ constexpr int FG_N = 13;
 // of K pairs lies at (q - c) * K + k * ncol_padded + c, i.e. every row holds
 // K planes of pairs and consecutive columns are consecutive pairs.
 constexpr int SW_OWN = 8;  // met0..met8, coed, rhs0..rhs4 (+1 unused)
-constexpr int SW_D = 4;    // u, v, w, c2, H, af, c, 2mu/(|Re| rho vol + 1e-30)
+constexpr int SW_D = 3;    // u, v, w, c2, H, 2mu/(|Re| rho vol + 1e-30) (a_f and c are recomputed)
 constexpr int SW_DQ = 3;   // dq0..dq4 (+1 unused)
 constexpr int SW_GA = 8;   // Gauss-Seidel forward: terms of the +1 neighbours along i, j, k (3 x 5, +1 unused)
 constexpr int SW_PRE = 3;  // Gauss-Seidel backward: sum of the -1 neighbours' terms (5, +1 unused)
