@@ -14,6 +14,7 @@
 //           m in [sqrt(1/2), sqrt(2)), atanh by its odd series in s^2;
 //   x^y   = exp(y log x) = 2^k exp(t)^1024,  t = (y log x - k ln2)/1024,
 //           exp(t) by its Taylor series, then ten double-double squarings.
+// The series' coefficients 1/(2k+1) and 1/n are double-double constants.
 // Only IEEE +, -, *, / and fma appear, each correctly rounded, so the host
 // and the device compute the same bits.  Non-positive, infinite and NaN x
 // go to the library pow (the IEEE special cases).
@@ -74,6 +75,51 @@ ORION_HD inline dd div(dd a, dd b) {
   return add(quick_two_sum(q1, q2), dd{q3, 0.0});
 }
 
+// 1/(2k+1), k = 0..23, and 1/n, n = 1..10, as double-doubles (hi the
+// nearest double, lo the nearest double to the remainder: exact to about
+// 2^-106), so that the series multiply where they would divide.
+#if defined(__HIPCC__)
+#define ORION_DD_CONST __device__ __constant__
+#else
+#define ORION_DD_CONST
+#endif
+static ORION_DD_CONST const double RCP_ODD[24][2] = {
+    {0x1.0000000000000p+0, 0x0.0p+0},
+    {0x1.5555555555555p-2, 0x1.5555555555555p-56},
+    {0x1.999999999999ap-3, -0x1.999999999999ap-57},
+    {0x1.2492492492492p-3, 0x1.2492492492492p-57},
+    {0x1.c71c71c71c71cp-4, 0x1.c71c71c71c71cp-58},
+    {0x1.745d1745d1746p-4, -0x1.745d1745d1746p-59},
+    {0x1.3b13b13b13b14p-4, -0x1.3b13b13b13b14p-58},
+    {0x1.1111111111111p-4, 0x1.1111111111111p-60},
+    {0x1.e1e1e1e1e1e1ep-5, 0x1.e1e1e1e1e1e1ep-61},
+    {0x1.af286bca1af28p-5, 0x1.af286bca1af28p-59},
+    {0x1.8618618618618p-5, 0x1.8618618618618p-59},
+    {0x1.642c8590b2164p-5, 0x1.642c8590b2164p-60},
+    {0x1.47ae147ae147bp-5, -0x1.eb851eb851eb8p-61},
+    {0x1.2f684bda12f68p-5, 0x1.2f684bda12f68p-59},
+    {0x1.1a7b9611a7b96p-5, 0x1.1a7b9611a7b96p-61},
+    {0x1.0842108421084p-5, 0x1.0842108421084p-60},
+    {0x1.f07c1f07c1f08p-6, -0x1.f07c1f07c1f08p-61},
+    {0x1.d41d41d41d41dp-6, 0x1.0750750750750p-60},
+    {0x1.bacf914c1bad0p-6, -0x1.bacf914c1bad0p-60},
+    {0x1.a41a41a41a41ap-6, 0x1.0690690690690p-60},
+    {0x1.8f9c18f9c18fap-6, -0x1.f3831f3831f38p-61},
+    {0x1.7d05f417d05f4p-6, 0x1.7d05f417d05f4p-62},
+    {0x1.6c16c16c16c17p-6, -0x1.f49f49f49f49fp-61},
+    {0x1.5c9882b931057p-6, 0x1.310572620ae4cp-61}};
+static ORION_DD_CONST const double RCP_N[10][2] = {
+    {0x1.0000000000000p+0, 0x0.0p+0},
+    {0x1.0000000000000p-1, 0x0.0p+0},
+    {0x1.5555555555555p-2, 0x1.5555555555555p-56},
+    {0x1.0000000000000p-2, 0x0.0p+0},
+    {0x1.999999999999ap-3, -0x1.999999999999ap-57},
+    {0x1.5555555555555p-3, 0x1.5555555555555p-57},
+    {0x1.2492492492492p-3, 0x1.2492492492492p-57},
+    {0x1.0000000000000p-3, 0x0.0p+0},
+    {0x1.c71c71c71c71cp-4, 0x1.c71c71c71c71cp-58},
+    {0x1.999999999999ap-4, -0x1.999999999999ap-58}};
+
 // natural log of x > 0 (finite, normal)
 ORION_HD inline dd log_dd(double x) {
   int e;
@@ -86,8 +132,8 @@ ORION_HD inline dd log_dd(double x) {
   const dd s2 = mul(s, s);
   // sum_{k=0..K} s2^k / (2k + 1), |s2| < 0.0295: K = 23 leaves < 2^-115
   const int K = 23;
-  dd p = div(dd{1.0, 0.0}, dd{2.0 * K + 1.0, 0.0});
-  for (int k = K - 1; k >= 0; --k) p = add(mul(p, s2), div(dd{1.0, 0.0}, dd{2.0 * k + 1.0, 0.0}));
+  dd p = dd{RCP_ODD[K][0], RCP_ODD[K][1]};
+  for (int k = K - 1; k >= 0; --k) p = add(mul(p, s2), dd{RCP_ODD[k][0], RCP_ODD[k][1]});
   const dd atanh2 = mul_d(mul(s, p), 2.0);
   const dd ln2{0x1.62e42fefa39efp-1, 0x1.abc9e3b39803fp-56};
   return add(mul_d(ln2, (double)e), atanh2);
@@ -106,7 +152,7 @@ ORION_HD inline double pow_cr(double x, double y) {
   t.lo *= 0x1p-10;
   // exp(t), |t| < 3.4e-4: Taylor to t^10/10!
   dd e{1.0, 0.0};
-  for (int n = 10; n >= 1; --n) e = add(dd{1.0, 0.0}, div(mul(e, t), dd{(double)n, 0.0}));
+  for (int n = 10; n >= 1; --n) e = add(dd{1.0, 0.0}, mul(mul(e, t), dd{RCP_N[n - 1][0], RCP_N[n - 1][1]}));
   for (int q = 0; q < 10; ++q) e = mul(e, e);
   const dd n = quick_two_sum(e.hi, e.lo);  // hi = round(hi + lo)
   return ldexp(n.hi, (int)k);

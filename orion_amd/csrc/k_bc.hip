@@ -225,7 +225,8 @@ __global__ void __launch_bounds__(256) bc_kernel(const BlockDev* __restrict__ bl
       const double vin = PR(b, nn[0], nn[1], nn[2], 2), win = PR(b, nn[0], nn[1], nn[2], 3);
       const double pin = PR(b, nn[0], nn[1], nn[2], 4);
       const double cin = sqrt(gama * pin / rin);
-      const double sin_ = pin / orion_dd::pow_cr(rin, gama);
+      // s_in = p_in / rho_in^gamma is formed only where it is used (subsonic
+      // outflow): the same value, one double-double power fewer elsewhere
       const double sinf = pinf / orion_dd::pow_cr(rinf, gama);
       const double vnin = uin * nx + vin * ny + win * nz;
       const double vninf = uinf * nx + vinf * ny + winf * nz;
@@ -239,7 +240,7 @@ __global__ void __launch_bounds__(256) bc_kernel(const BlockDev* __restrict__ bl
         const double cb = 0.25 * gm1 * (Rp - Rm);
         double sb, ur, vr, wr, vnr;
         if (vnb > 0.0) {
-          sb = sin_; ur = uin; vr = vin; wr = win; vnr = vnin;
+          sb = pin / orion_dd::pow_cr(rin, gama); ur = uin; vr = vin; wr = win; vnr = vnin;
         } else {
           sb = sinf; ur = uinf; vr = vinf; wr = winf; vnr = vninf;
         }
