@@ -53,22 +53,33 @@ STAGES = {
     "rhs": (20, ["visc_deriv", "visc_face", "visc_diff", "inv_face", "inv_closure", "inv_diff"],
             "visc_deriv_kernel+visc_face_queue_kernel+diff_kernel<false,true>+inv_face_kernel+inv_closure_kernel"
             "+diff_kernel<true,false>"),
-    "lu_fwd": (25, ["lu_pack_state", "lu_pack", "lu_fwd"],
-               "lu_pack_kernel<1>+lu_pack_kernel<2>+lusgs_tile_kernel<0,true>"),
-    "lu_bwd": (25, ["lu_bwd"], "lusgs_tile_kernel<1,true>"),
-    "gs_fwd": (30, ["gs_fwd"], "lusgs_tile_kernel<2,true>"),
-    "gs_bwd": (30, ["gs_bwd"], "lusgs_tile_kernel<3,true>"),
+    # a layout copy in two parts: no algorithmic words of its own
+    "lu_pack": (0, ["lu_pack_state", "lu_pack"], "lu_pack_kernel<1>+lu_pack_kernel<2>"),
+    # the four sweeps (LU forward 25, LU backward 25, GS forward 30, GS
+    # backward 30 words): one chained launch, or one launch per sweep
+    "lu_sweeps": (110, ["lu_fwd", "lu_bwd", "gs_fwd", "gs_bwd", "lu_chain"], "lusgs_chain_kernel<true>"),
     "update": (25, ["update", "residual"], "update_kernel+residual_final_kernel"),
 }
+SWEEP_KERNELS = {True: "lusgs_chain_kernel<true>",
+                 False: "lusgs_tile_kernel<0,true>+lusgs_tile_kernel<1,true>+lusgs_tile_kernel<2,true>"
+                        "+lusgs_tile_kernel<3,true>"}
 STEP_WORDS = sum(v[0] for v in STAGES.values())
-# The dominant kernel: the LU-SGS tile sweep, four launches per step (LU
-# forward / backward, GS forward / backward) that take the largest share of
-# the step of any kernel and run alone (the captured step overlaps only the
-# face-flux kernels).  Its algorithmic words are those of the four sweep
-# stages (25 + 25 + 30 + 30); lu_pack, a layout copy, is given none.
-DOMINANT = ("lusgs_tile_kernel", 110, ["lu_fwd", "lu_bwd", "gs_fwd", "gs_bwd"],
-            ["lusgs_tile_kernel<0,true>", "lusgs_tile_kernel<1,true>", "lusgs_tile_kernel<2,true>",
-             "lusgs_tile_kernel<3,true>"])
+# The dominant kernel: the LU-SGS sweeps (LU forward / backward, GS forward
+# / backward), one chained launch per step (or four, one per sweep, under
+# the per-sweep schedules), which take the largest share of the step of any
+# kernel and run alone (the captured step overlaps only the face-flux and
+# pack kernels).  Its algorithmic words are those of the four sweep stages
+# (25 + 25 + 30 + 30); lu_pack, a layout copy, is given none.
+DOMINANT = ("lusgs_chain_kernel", 110, STAGES["lu_sweeps"][1], STAGES["lu_sweeps"][2].split("+"))
+
+
+def use_sweep_kernels(chained):
+    """Name the sweep kernels of the schedule in use in the stage and
+    dominant-kernel tables."""
+    global DOMINANT
+    k = SWEEP_KERNELS[bool(chained)]
+    STAGES["lu_sweeps"] = STAGES["lu_sweeps"][:2] + (k,)
+    DOMINANT = ("lusgs_chain_kernel" if chained else "lusgs_tile_kernel",) + DOMINANT[1:3] + (k.split("+"),)
 
 
 def make_case(name, nsteps):
@@ -106,21 +117,24 @@ def roofline(phase, N, ms_per_step):
         t_ms = sum(phase[p][0] for p in phases)
         launches = sum(phase[p][1] for p in phases)
         nbytes = words * 8.0 * N
-        gbs = nbytes / (t_ms * 1e-3) / 1e9
+        gbs = nbytes / (t_ms * 1e-3) / 1e9 if t_ms > 0 else 0.0
         stages[name] = {"kernels": kernels, "ms": round(t_ms, 4), "launches": launches,
                         "algorithmic_bytes": nbytes, "achieved_GBs": round(gbs, 1),
                         "frac": round(gbs / HBM_PEAK_GBS, 5)}
-    # one sweep launch per phase (each phase also holds the sweep's 5 us
-    # epoch-counter kernel)
+    # the four sweeps: one launch per sweep phase, or one chained launch
+    # (phase lu_chain) that runs all four
     kname, words, kphases, _ = DOMINANT
     t_ms = sum(phase[p][0] for p in kphases)
-    launches = len(kphases)
+    launches = int(round(sum(phase[p][1] for p in kphases))) or len(kphases)
+    chained = phase.get("lu_chain", (0.0, 0))[1] > 0
     per_launch = words * 8.0 * N / launches
     avg_s = t_ms * 1e-3 / launches
     gbs = per_launch / avg_s / 1e9
     roof = {
         "bound": "hbm",
-        "kernel": "%s (LU forward, LU backward, GS forward, GS backward sweeps)" % kname,
+        "kernel": ("%s (LU forward, LU backward, GS forward and GS backward sweeps as one launch)" % kname
+                   if chained else
+                   "%s (LU forward, LU backward, GS forward, GS backward sweeps)" % kname),
         "achieved": round(gbs, 1),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
@@ -336,7 +350,7 @@ def main():
     ap.add_argument("--tune-steps", type=int, default=3,
                     help="steps per trial of the launch-shape autotuner after the warmup (0: no tuning)")
     ap.add_argument("--schedule", type=int, default=-1,
-                    help="fix the sweep schedule (0 diagonal, 1 critical path) instead of tuning")
+                    help="fix the sweep schedule (0 diagonal, 1 critical path, 2 chained) instead of tuning")
     ap.add_argument("--workload", default=DEFAULT_WORKLOAD)
     ap.add_argument("--profile-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
@@ -435,6 +449,7 @@ def main():
                  "same_results_as_batched": bool(same)}
         print("[bench] plain per-block form: %.3f ms/step (batched %.3f), same results: %s" % (
             1e3 * tp / n, 1e3 * t / a.steps, same), file=sys.stderr, flush=True)
+    use_sweep_kernels(s.fs.schedule() == 2)
     phase = profile_phases(s.fs, a.profile_steps)
     phases = {k: round(v[0], 4) for k, v in phase.items()}
     print("[bench] phase ms/step: " + json.dumps(phases), file=sys.stderr, flush=True)
@@ -442,7 +457,7 @@ def main():
     traffic = stage_traffic(a.workload, {k: v[2].split("+") for k, v in STAGES.items()})
     for name, tr in traffic.items():
         stages[name]["traffic_bytes_per_launch"] = tr["bytes_per_launch"]
-    dom = stage_traffic(a.workload, {"dominant": DOMINANT[3]})
+    dom = stage_traffic(a.workload, {"dominant": DOMINANT[3]}) if DOMINANT[3] else {}
     if dom:
         roof["traffic"] = dom["dominant"]["bytes_per_launch"]
         roof["traffic_source"] = dom["dominant"]["source"]

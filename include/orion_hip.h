@@ -155,17 +155,24 @@ int orion_hip_run(orion_hip_ctx* ctx, int nsteps, orion_residual_stats* hist);
  * returns, per phase, the summed device time (ms) and the number of kernel
  * launches.  Phases, in order: halo, bc, state+dt, visc_deriv, visc_face,
  * visc_diff, avg1, inv_face, inv_closure, lu_pack_state, inv_diff, avg2,
- * lu_pack, lu_fwd, lu_bwd, gs_fwd, gs_bwd, update, residual. */
-#define ORION_PHASES 19
+ * lu_pack, lu_fwd, lu_bwd, gs_fwd, gs_bwd, lu_chain (the four sweeps as
+ * one launch under the chained schedule; the four before it are then
+ * empty), update, residual. */
+#define ORION_PHASES 20
 int orion_hip_profile_run(orion_hip_ctx* ctx, int nsteps, double* phase_ms, long long* phase_launches);
 
 /* No counterpart in the reference: the launch order of the LU-SGS tiles.
  * 0 diagonal (by tile diagonal over all blocks), 1 critical path first
  * (by the planes left to the end of the tile's block sweep, longest first;
- * the default).  Every order launches a tile after the tiles it waits
+ * the default), both one launch per sweep; 2 chained: the four sweeps as
+ * one launch of (tile, sweep) tasks, a tile's sweep waiting for its own
+ * previous sweep, in the order a greedy list schedule on the device's CUs
+ * starts them (unavailable, with an error, for blocks whose LU-SGS arrays
+ * exceed 4 GiB).  Every order launches a task after the tasks it waits
  * for, and the results are the same under every order; they differ in how
  * soon the longest chains get CUs when the tiles outnumber them. */
-#define ORION_SCHEDULES 2
+#define ORION_SCHEDULES 3
+#define ORION_SCHEDULE_CHAINED 2
 int orion_hip_set_schedule(orion_hip_ctx* ctx, int schedule);
 int orion_hip_get_schedule(const orion_hip_ctx* ctx);
 /* The autotuner of the sweep schedule: from the current state, times

@@ -39,8 +39,8 @@ class ResidualStats(C.Structure):
 
 
 PHASES = ["halo", "bc", "state_dt", "visc_deriv", "visc_face", "visc_diff", "avg1", "inv_face", "inv_closure",
-          "lu_pack_state", "inv_diff", "avg2", "lu_pack", "lu_fwd", "lu_bwd", "gs_fwd", "gs_bwd", "update",
-          "residual"]
+          "lu_pack_state", "inv_diff", "avg2", "lu_pack", "lu_fwd", "lu_bwd", "gs_fwd", "gs_bwd", "lu_chain",
+          "update", "residual"]
 
 
 class OrionHipError(RuntimeError):

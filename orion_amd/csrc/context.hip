@@ -101,8 +101,10 @@ struct orion_hip_ctx {
   std::vector<SweepTile> tiles;
   SweepTile* d_tiles = nullptr;
   int* d_order[2] = {nullptr, nullptr};  // tile launch order: forward, backward sweeps
-  std::vector<int> sched_order[ORION_SCHEDULES][2];  // candidate orders (host)
-  int schedule = 0;                      // the one in d_order
+  std::vector<int> sched_order[ORION_SCHEDULES][2];  // candidate orders (host; empty for the chained schedule)
+  int schedule = 0;                      // the one in d_order, or ORION_SCHEDULE_CHAINED
+  int* d_chain = nullptr;                // chained launch: tasks tile << 2 | sweep, topological order
+  int n_chain = 0;                       // its task count (0: the chained schedule is unavailable)
   unsigned long long* d_flags = nullptr; // per-tile progress
   unsigned long long* d_epoch = nullptr; // sweep counter
   int* d_err = nullptr;                  // set by a sweep whose wait ran out
@@ -635,6 +637,132 @@ static int set_visc_queue(orion_hip_ctx* c, int seg, int grid) {
   return 0;
 }
 
+// The chained launch of the four sweeps (lusgs_chain_kernel): one task per
+// tile and sweep, dispatched in the order a greedy list schedule on the
+// device's CUs (one tile workgroup per CU) starts them.  Units: pipeline
+// steps.  A task runs its tile's planes plus two steps of publication lag;
+// it may start once its tile's previous sweep has finished and the tiles it
+// waits for in its own sweep have published the plane before its first
+// (the upstream tile's start + the planes up to it + 3).  Among the tasks
+// that may start, the one with the longest remaining path to the end of the
+// fourth sweep goes first.  Each task starts strictly after the tasks it
+// waits for, so the start order is topological: every wait in the launch
+// targets a workgroup dispatched earlier.  The chained schedule needs each
+// block's share of the Gauss-Seidel term array (its largest) addressable
+// with 32-bit offsets; otherwise it is unavailable.
+static int build_chain(orion_hip_ctx* c) {
+  c->n_chain = 0;
+  const int nt = (int)c->tiles.size();
+  for (int b = 0; b < c->nb; ++b) {
+    const BlockDev& B = c->hb[b];
+    const long long cells = (long long)(B.sn[0] + B.sn[1] + B.sn[2] - 2) * B.sn[2] * B.sncp;
+    if (cells * SW_GA * 16 >= (1LL << 32)) return 0;  // unavailable, not an error
+  }
+  if (nt == 0) return 0;
+  int ncu = 0;
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess || ncu <= 0)
+    ncu = 256;
+  const int NTK = 4 * nt;
+  struct Edge { int to; long long off; };
+  std::vector<std::vector<Edge>> succ(NTK);
+  std::vector<std::vector<std::pair<int, long long>>> pred(NTK);
+  auto dur = [&](int t) { const SweepTile& T = c->tiles[t]; return (long long)(T.s_last - T.s_first + 1) + 2; };
+  auto add = [&](int a, int b, long long off) {
+    succ[a].push_back({b, off});
+    pred[b].push_back({a, off});
+  };
+  for (int x = 0; x < nt; ++x) {
+    const SweepTile& X = c->tiles[x];
+    for (int k = 0; k < 4; ++k) {
+      const int id = 4 * x + k;
+      if (k > 0) add(4 * x + k - 1, id, dur(x));
+      const bool fwd = (k == 0 || k == 2);
+      for (int e = 0; e < 2; ++e) {
+        const int u = fwd ? X.nbr[e] : X.nbr[2 + e];
+        if (u < 0) continue;
+        const SweepTile& U = c->tiles[u];
+        if (fwd) {
+          const int p = X.s_first - 1;  // the plane X's first edge values lie on
+          if (p < U.s_first) continue;
+          add(4 * u + k, id, (long long)(std::min(p, U.s_last) - U.s_first) + 3);
+        } else {
+          const int p = X.s_last + 1;
+          if (p > U.s_last) continue;
+          add(4 * u + k, id, (long long)(U.s_last - std::max(p, U.s_first)) + 3);
+        }
+      }
+    }
+  }
+  // remaining path to the end of the last sweep, in reverse topological
+  // order (sweep 3 first; within a sweep, tiles in reverse dependency order)
+  std::vector<long long> bl(NTK, 0);
+  std::vector<int> topo;  // a topological order of the tasks: by sweep, then by tile diagonal
+  {
+    std::vector<int> ord(nt);
+    for (int i = 0; i < nt; ++i) ord[i] = i;
+    auto diag = [&](int i) { return c->tiles[i].r0 / SWEEP_TR + c->tiles[i].c0 / SWEEP_TW; };
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return diag(a) < diag(b); });
+    for (int k = 0; k < 4; ++k) {
+      const bool fwd = (k == 0 || k == 2);
+      for (int i = 0; i < nt; ++i) topo.push_back(4 * ord[fwd ? i : nt - 1 - i] + k);
+    }
+  }
+  for (int i = NTK - 1; i >= 0; --i) {
+    const int a = topo[i];
+    long long v = dur(a / 4);
+    for (const Edge& e : succ[a]) v = std::max(v, e.off + bl[e.to]);
+    bl[a] = v;
+  }
+  // greedy list schedule
+  std::vector<long long> start(NTK, -1), avail(NTK, 0);
+  std::vector<int> npred(NTK), left(NTK);
+  for (int i = 0; i < NTK; ++i) npred[i] = left[i] = (int)pred[i].size();
+  std::vector<int> ready;  // all predecessors started; may start at avail
+  for (int i = 0; i < NTK; ++i)
+    if (npred[i] == 0) ready.push_back(i);
+  std::vector<long long> running;  // end times of running tasks
+  std::vector<int> order;
+  order.reserve(NTK);
+  long long now = 0;
+  while ((int)order.size() < NTK) {
+    // free the slots of tasks that have ended
+    running.erase(std::remove_if(running.begin(), running.end(), [&](long long e) { return e <= now; }),
+                  running.end());
+    // start what may start, longest remaining path first
+    std::vector<int> cand;
+    for (int t : ready)
+      if (avail[t] <= now) cand.push_back(t);
+    std::stable_sort(cand.begin(), cand.end(), [&](int a, int b) { return bl[a] > bl[b]; });
+    bool started = false;
+    for (int t : cand) {
+      if ((int)running.size() >= ncu) break;
+      start[t] = now;
+      order.push_back(t);
+      running.push_back(now + dur(t / 4));
+      ready.erase(std::find(ready.begin(), ready.end(), t));
+      for (const Edge& e : succ[t]) {
+        avail[e.to] = std::max(avail[e.to], now + e.off);
+        if (--left[e.to] == 0) ready.push_back(e.to);
+      }
+      started = true;
+    }
+    if (started) continue;
+    // advance to the next event: a slot freeing or a ready task becoming available
+    long long next = -1;
+    for (long long e : running)
+      if (next < 0 || e < next) next = e;
+    for (int t : ready)
+      if (avail[t] > now && (next < 0 || avail[t] < next)) next = avail[t];
+    if (next < 0 || next <= now) return fail("chained sweep schedule: no progress at step %lld", now);
+    now = next;
+  }
+  std::vector<int> tasks(NTK);
+  for (int i = 0; i < NTK; ++i) tasks[i] = (order[i] / 4) << 2 | (order[i] % 4);
+  if (upload_vec(c, &c->d_chain, tasks)) return -1;
+  c->n_chain = NTK;
+  return 0;
+}
+
 // LU-SGS sweep layout (k_lusgs.hip): per block the march axis is the longest
 // one and col is i whenever i is not the march axis; the (col, row)
 // cross-section is cut into 16 x 16 tiles.  Tiles are launched so that every
@@ -740,7 +868,11 @@ static int build_sweep(orion_hip_ctx* c) {
     }
     if (upload_vec(c, &c->d_border[0], f) || upload_vec(c, &c->d_border[1], bk)) return -1;
   }
-  c->schedule = 1;  // critical path first until a tuner says otherwise
+  if (build_chain(c)) return -1;
+  // the four sweeps chained where they can be, else critical path first,
+  // until a tuner says otherwise (d_order holds the critical-path orders
+  // either way)
+  c->schedule = c->n_chain > 0 ? ORION_SCHEDULE_CHAINED : 1;
   if (upload_vec(c, &c->d_order[0], c->sched_order[1][0])) return -1;
   if (upload_vec(c, &c->d_order[1], c->sched_order[1][1])) return -1;
   if (upload_vec(c, &c->d_tiles, c->tiles)) return -1;
@@ -1312,6 +1444,16 @@ static void op_sweep(orion_hip_ctx* c, int type) {
                      c->d_err, type, c->p, c->stream);
 }
 
+// the four sweeps: one chained launch, or a launch per sweep
+static void op_sweeps(orion_hip_ctx* c) {
+  if (c->schedule == ORION_SCHEDULE_CHAINED) {
+    launch_lusgs_chain(c->sw, c->db, c->d_tiles, c->d_chain, c->n_chain, c->d_flags, c->d_epoch, c->d_err, c->p,
+                       c->stream);
+    return;
+  }
+  for (int t = 0; t < 4; ++t) op_sweep(c, t);
+}
+
 // state_packed: the step already launched the state part of the pack (on
 // the side stream, beside the right-hand side)
 static void op_update(orion_hip_ctx* c, bool state_packed = false) {
@@ -1335,7 +1477,7 @@ static void op_update(orion_hip_ctx* c, bool state_packed = false) {
   }
   if (!state_packed) launch_lu_pack(LA(c, c->ptile), c->ptile.total(), c->sw, 1);
   launch_lu_pack(LA(c, c->ptile), c->ptile.total(), c->sw, 2);
-  for (int t = 0; t < 4; ++t) op_sweep(c, t);
+  op_sweeps(c);
   launch_update(LA(c, c->ptile), c->ptile.total(), c->sw, c->p.gama, c->intr.d, c->d_upart,
                 c->d_upart + c->ptile.nwg, c->d_ukey, c->d_epoch);
 }
@@ -1605,7 +1747,13 @@ int orion_hip_profile_run(orion_hip_ctx* c, int nsteps, double* phase_ms, long l
     launch_inv_diff(LA(c, c->dtile), c->dtile.total(), c->p.nvis != 1); mark(1);
     op_avg(c); mark(2);
     launch_lu_pack(LA(c, c->ptile), c->ptile.total(), c->sw, 2); mark(1);
-    for (int t = 0; t < 4; ++t) { op_sweep(c, t); mark(1); }
+    if (c->schedule == ORION_SCHEDULE_CHAINED) {
+      mark(0); mark(0); mark(0); mark(0);
+      op_sweeps(c); mark(1);
+    } else {
+      for (int t = 0; t < 4; ++t) { op_sweep(c, t); mark(1); }
+      mark(0);
+    }
     launch_update(LA(c, c->ptile), c->ptile.total(), c->sw, c->p.gama, c->intr.d, c->d_upart,
                   c->d_upart + c->ptile.nwg, c->d_ukey, c->d_epoch);
     mark(1);
@@ -1746,13 +1894,20 @@ int orion_hip_set_stream(orion_hip_ctx* c, void* s) {
 int orion_hip_set_schedule(orion_hip_ctx* c, int which) {
   if (!c || which < 0 || which >= ORION_SCHEDULES) return fail("set_schedule: no schedule %d", which);
   if (which == c->schedule) return 0;
-  // the captured step reads the order arrays in place: only their contents change
-  for (int k = 0; k < 2; ++k) {
-    const std::vector<int>& v = c->sched_order[which][k];
-    if (!v.empty())
-      HIPCHK(hipMemcpyAsync(c->d_order[k], v.data(), sizeof(int) * v.size(), hipMemcpyHostToDevice, c->stream));
+  if (which == ORION_SCHEDULE_CHAINED) {
+    if (c->n_chain == 0)
+      return fail("set_schedule: the chained schedule needs every block's LU-SGS arrays below 4 GiB");
+  } else {
+    // the captured step reads the order arrays in place: only their contents change
+    for (int k = 0; k < 2; ++k) {
+      const std::vector<int>& v = c->sched_order[which][k];
+      if (!v.empty())
+        HIPCHK(hipMemcpyAsync(c->d_order[k], v.data(), sizeof(int) * v.size(), hipMemcpyHostToDevice, c->stream));
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
   }
-  HIPCHK(hipStreamSynchronize(c->stream));
+  // the chained schedule is another launch: capture the step again
+  if (which == ORION_SCHEDULE_CHAINED || c->schedule == ORION_SCHEDULE_CHAINED) c->graph_stream = nullptr;
   c->schedule = which;
   return 0;
 }
@@ -1809,6 +1964,10 @@ int orion_hip_tune_schedule(orion_hip_ctx* c, int steps, int* chosen, double* ms
   double best_ms = 0.0;
   for (int k = 0; rc == 0 && k < ORION_SCHEDULES; ++k) {
     double ms = 0.0;
+    if (k == ORION_SCHEDULE_CHAINED && c->n_chain == 0) {
+      if (ms_per_step) ms_per_step[k] = 0.0;
+      continue;
+    }
     rc = orion_hip_set_schedule(c, k);
     if (rc == 0) rc = T.trial(steps, &ms);
     if (rc == 0) {
@@ -1854,6 +2013,7 @@ int orion_hip_tune(orion_hip_ctx* c, int steps, int* schedule, int* segment, int
   int bs = c->schedule;
   for (int k = 0; rc == 0 && k < ORION_SCHEDULES; ++k) {
     double ms = 0.0;
+    if (k == ORION_SCHEDULE_CHAINED && c->n_chain == 0) continue;
     rc = orion_hip_set_schedule(c, k);
     if (rc == 0) rc = measure(&ms);
     if (rc == 0 && (best < 0.0 || ms < best)) {

@@ -474,21 +474,33 @@ __device__ __forceinline__ void dma_pair(const double2* g, long long rb, int c, 
                                    (__attribute__((address_space(3))) void*)wbase, 16, 0, 0);
 }
 
-// A block's dq pairs as a buffer resource, for 16-byte stores with the cache
-// policy of an agent-scope atomic store (sc1: written through to where
-// every XCD reads it).  Base: the block's first cell; the host checks that
-// a block's sweep arrays stay below 4 GiB, so byte offsets fit 32 bits.
+// A block's share of a sweep array (dq, and in the chained launch ga and
+// pre) as a buffer resource, for 16-byte stores and loads with the cache
+// policy of agent-scope atomics (sc1: written through to, and read from,
+// where every XCD sees it).  Base: the block's first cell; the host checks
+// that a block's share stays below 4 GiB, so byte offsets fit 32 bits.
 // Word 3 is the gfx9 raw-buffer format (32-bit data, no swizzle).
 typedef unsigned v4u32 __attribute__((ext_vector_type(4)));
 typedef double v2f64 __attribute__((ext_vector_type(2)));
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t dq_resource(const SweepArrays& S, const BlockDev& b) {
-  const long long bytes = (long long)(b.sn[0] + b.sn[1] + b.sn[2] - 2) * b.sn[2] * b.sncp * SW_DQ * 16;
-  return __builtin_amdgcn_make_buffer_rsrc(S.dq + b.sbase * SW_DQ, 0, (int)(unsigned)bytes, 0x00020000);
+// a block's share of a sweep array of K pairs per cell
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t pair_resource(const double2* arr, const BlockDev& b, int K) {
+  const long long bytes = (long long)(b.sn[0] + b.sn[1] + b.sn[2] - 2) * b.sn[2] * b.sncp * K * 16;
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<double2*>(arr) + b.sbase * K, 0, (int)(unsigned)bytes,
+                                           0x00020000);
+}
+// byte offset of pair k of cell (q, c) in the block's share of an array of K pairs
+__device__ __forceinline__ unsigned pair_off(const BlockDev& b, long long q, int c, int ncp, int K, int k) {
+  return (unsigned)(((q - c - b.sbase) * K + (long long)k * ncp + c) * 16);
 }
 // one cell's pair (x, y) at byte offset off of the resource, coherent for the device
 __device__ __forceinline__ void store_pair_coherent(__amdgpu_buffer_rsrc_t r, unsigned off, double x, double y) {
   const v2f64 v = {x, y};
   __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(v4u32, v), r, off, 0, CPOL_SC1);
+}
+// and the coherent load of one
+__device__ __forceinline__ double2 load_pair_coherent(__amdgpu_buffer_rsrc_t r, unsigned off) {
+  const v2f64 v = __builtin_bit_cast(v2f64, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, CPOL_SC1));
+  return make_double2(v.x, v.y);
 }
 
 // the same with the cache policy of an agent-scope atomic load (sc1: the
@@ -542,12 +554,27 @@ __device__ __forceinline__ void dma_pair_coherent(const double2* g, long long rb
 // by step parity, D by step parity (a step's update reads the D of the step
 // before while the loaders fill the step's own), dq by parity, edge values
 // by parity; own inputs and D are kept as the global pairs.
-template <int TYPE, bool VIS>
-__global__ void __launch_bounds__(2 * TL, 1) lusgs_tile_kernel(SweepArrays S, const BlockDev* __restrict__ blocks,
-                                                            const SweepTile* __restrict__ tiles,
-                                                            const int* __restrict__ order, unsigned long long* flags,
-                                                            const unsigned long long* __restrict__ epoch, int* err,
-                                                            double gm1) {
+// LDS of one tile by sweep type: own inputs (three step buffers, two in the
+// Gauss-Seidel forward sweep), D by step parity, new dq (and the Gauss-Seidel
+// forward partial sums) by parity, edge D and edge dq by step mod 3.
+__host__ __device__ constexpr int sweep_nob(int type) { return type != 2 ? 3 : 2; }
+__host__ __device__ constexpr int sweep_nq(int type) { return type == 2 ? 10 : 5; }
+__host__ __device__ constexpr unsigned sweep_lds_bytes(int type) {
+  return (unsigned)(16 * sweep_nob(type) * SW_OWN * TL + 16 * 2 * SW_D * TL + 8 * 2 * sweep_nq(type) * TL +
+                    16 * 2 * 3 * 128);
+}
+constexpr unsigned SWEEP_LDS_MAX = sweep_lds_bytes(0) > sweep_lds_bytes(2) ? sweep_lds_bytes(0) : sweep_lds_bytes(2);
+
+// One tile's sweep of type TYPE.  e4: four times the step epoch (the sweep
+// serial base).  CHAIN: the tile runs in the chained launch of all four
+// sweeps, where its sweep TYPE waits for its own sweep TYPE - 1 to have
+// published its last plane, and the values one sweep leaves for the next
+// (the forward dq, ga, pre) are stored and loaded with the device-coherent
+// policy, as the two sweeps of a tile may run on different XCDs.
+template <int TYPE, bool VIS, bool CHAIN>
+__device__ __forceinline__ void sweep_tile(const SweepArrays& S, const BlockDev* __restrict__ blocks,
+                                           const SweepTile* __restrict__ tiles, int tix, unsigned long long* flags,
+                                           unsigned long long e4, int* err, double gm1, unsigned char* lds) {
   constexpr bool FWD = (TYPE == 0 || TYPE == 2);
   constexpr int dir = FWD ? 1 : -1;
   constexpr int SL0 = FWD ? 0 : 3;
@@ -555,16 +582,19 @@ __global__ void __launch_bounds__(2 * TL, 1) lusgs_tile_kernel(SweepArrays S, co
   // own inputs loaded two steps ahead (three buffers), except in the
   // Gauss-Seidel forward sweep, whose LDS holds one buffer less
   constexpr bool OWN2 = TYPE != 2;
-  constexpr int NOB = OWN2 ? 3 : 2;
-  __shared__ double2 ownP[NOB][SW_OWN][TL];
-  __shared__ double2 Dp[2][SW_D][TL];
-  constexpr int NQ = TYPE == 2 ? 10 : 5;  // new dq, then (Gauss-Seidel forward) the partial sums
-  __shared__ double dqB[2][NQ][TL];
-  __shared__ double2 shXD[3][128];           // edge D by step mod 3: [role][index][pair] (96 used)
-  __shared__ double2 shXQ[3][128];           // edge dq by step mod 3: [role][index][pair] (96 used)
+  constexpr int NOB = sweep_nob(TYPE);
+  constexpr int NQ = sweep_nq(TYPE);  // new dq, then (Gauss-Seidel forward) the partial sums
+  constexpr unsigned OFF_D = 16u * NOB * SW_OWN * TL;
+  constexpr unsigned OFF_Q = OFF_D + 16u * 2 * SW_D * TL;
+  constexpr unsigned OFF_XD = OFF_Q + 8u * 2 * NQ * TL;
+  constexpr unsigned OFF_XQ = OFF_XD + 16u * 3 * 128;
+  double2(*ownP)[SW_OWN][TL] = reinterpret_cast<double2(*)[SW_OWN][TL]>(lds);
+  double2(*Dp)[SW_D][TL] = reinterpret_cast<double2(*)[SW_D][TL]>(lds + OFF_D);
+  double(*dqB)[NQ][TL] = reinterpret_cast<double(*)[NQ][TL]>(lds + OFF_Q);
+  double2(*shXD)[128] = reinterpret_cast<double2(*)[128]>(lds + OFF_XD);  // edge D: [step mod 3][item] (96 used)
+  double2(*shXQ)[128] = reinterpret_cast<double2(*)[128]>(lds + OFF_XQ);  // edge dq: [step mod 3][item] (96 used)
 
   auto ob = [](int u) { return OWN2 ? (u + 3) % 3 : u & 1; };  // own buffer of step u >= -1
-  const int tix = order[blockIdx.x];
   const SweepTile T = tiles[tix];
   const BlockDev& b = blocks[T.block];
   const int nm = b.sn[0], nc = b.sn[1], nr = b.sn[2];
@@ -580,14 +610,17 @@ __global__ void __launch_bounds__(2 * TL, 1) lusgs_tile_kernel(SweepArrays S, co
   // the sweep's serial: four sweeps per step epoch (the update kernel after
   // them advances the epoch), so serials rise from launch to launch and a
   // flag left by an earlier sweep is always short of this one's keys
-  const unsigned long long ep = ((*epoch) * 4 + TYPE + 1) << KEY_BITS;
+  const unsigned long long ep = (e4 + TYPE + 1) << KEY_BITS;
   SweepTile ut[2];
   ut[0] = up_c >= 0 ? tiles[up_c] : T;
   ut[1] = up_r >= 0 ? tiles[up_r] : T;
   const int nsteps = T.s_last - T.s_first + 1;
   const int s0 = FWD ? T.s_first : T.s_last;
   const int ncp = b.sncp;
-  const __amdgpu_buffer_rsrc_t dq_rsrc = dq_resource(S, b);
+  const __amdgpu_buffer_rsrc_t dq_rsrc = pair_resource(S.dq, b, SW_DQ);
+  // the chained launch's cross-sweep arrays (unused otherwise)
+  const __amdgpu_buffer_rsrc_t ga_rsrc = pair_resource(S.ga, b, SW_GA);
+  const __amdgpu_buffer_rsrc_t pre_rsrc = pair_resource(S.pre, b, SW_PRE);
   const long long qstep = dir * (long long)nr * ncp;
   const int m0 = s0 - r - c;
   const long long rb0 = b.sbase + ((long long)s0 * nr + r) * ncp;
@@ -664,7 +697,12 @@ __global__ void __launch_bounds__(2 * TL, 1) lusgs_tile_kernel(SweepArrays S, co
 #pragma unroll
       for (int k = 0; k < 5; ++k) dma_pair(S.own, rb, cc, ncp, SW_OWN, k, &dst[k][wb]);
 #pragma unroll
-      for (int k = 0; k < 3; ++k) dma_pair(S.dq, rb, cc, ncp, SW_DQ, k, &dst[5 + k][wb]);
+      for (int k = 0; k < 3; ++k) {
+        if constexpr (CHAIN)
+          dma_pair_coherent(S.dq, rb, cc, ncp, SW_DQ, k, &dst[5 + k][wb]);
+        else
+          dma_pair(S.dq, rb, cc, ncp, SW_DQ, k, &dst[5 + k][wb]);
+      }
     } else {
 #pragma unroll
       for (int k = 0; k < SW_OWN; ++k) dma_pair(S.own, rb, cc, ncp, SW_OWN, k, &dst[k][wb]);
@@ -744,24 +782,42 @@ __global__ void __launch_bounds__(2 * TL, 1) lusgs_tile_kernel(SweepArrays S, co
     if constexpr (TYPE == 3) {
       const long long rb = cell_rb(u);
       const int cc = cell_c(u);
+      if constexpr (CHAIN) {
 #pragma unroll
-      for (int k = 0; k < 2; ++k) {
-        const double2 v = ld_pair(S.pre, rb, cc, ncp, SW_PRE, k);
-        g[2 * k] = v.x;
-        g[2 * k + 1] = v.y;
+        for (int k = 0; k < 3; ++k) {
+          const double2 v = load_pair_coherent(pre_rsrc, pair_off(b, rb + cc, cc, ncp, SW_PRE, k));
+          g[2 * k] = v.x;
+          if (k < 2) g[2 * k + 1] = v.y;
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+          const double2 v = ld_pair(S.pre, rb, cc, ncp, SW_PRE, k);
+          g[2 * k] = v.x;
+          g[2 * k + 1] = v.y;
+        }
+        g[4] = ld_half(S.pre, rb, cc, ncp, SW_PRE, 2);
       }
-      g[4] = ld_half(S.pre, rb, cc, ncp, SW_PRE, 2);
     }
     if constexpr (TYPE == 2) {
       const long long rb = cell_rb(u);
       const int cc = cell_c(u);
+      if constexpr (CHAIN) {
 #pragma unroll
-      for (int k = 0; k < 7; ++k) {
-        const double2 v = ld_pair(S.ga, rb, cc, ncp, SW_GA, k);
-        g[2 * k] = v.x;
-        g[2 * k + 1] = v.y;
+        for (int k = 0; k < 8; ++k) {
+          const double2 v = load_pair_coherent(ga_rsrc, pair_off(b, rb + cc, cc, ncp, SW_GA, k));
+          g[2 * k] = v.x;
+          if (k < 7) g[2 * k + 1] = v.y;
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < 7; ++k) {
+          const double2 v = ld_pair(S.ga, rb, cc, ncp, SW_GA, k);
+          g[2 * k] = v.x;
+          g[2 * k + 1] = v.y;
+        }
+        g[14] = ld_half(S.ga, rb, cc, ncp, SW_GA, 7);
       }
-      g[14] = ld_half(S.ga, rb, cc, ncp, SW_GA, 7);
     }
   };
 
@@ -781,28 +837,43 @@ __global__ void __launch_bounds__(2 * TL, 1) lusgs_tile_kernel(SweepArrays S, co
       if (t + 2 < nsteps) ensure_up(s + dir, pf);  // the edge values of step t+2 lie on plane s(t+1)
     }
     if (TYPE == 1 && cell_ok(t - 1)) {  // the terms the update lanes left in the own-input slot of step t-1
-      double2* gp = S.ga + (rb0 + (long long)(t - 1) * qstep) * SW_GA + c;
       const int q = ob(t - 1);
+      if constexpr (CHAIN) {
+        const long long qc = rb0 + (long long)(t - 1) * qstep + c;
 #pragma unroll
-      for (int k = 0; k < SW_GA; ++k) gp[(long long)k * ncp] = ownP[q][k][lane];
+        for (int k = 0; k < SW_GA; ++k) {
+          const double2 v = ownP[q][k][lane];
+          store_pair_coherent(ga_rsrc, pair_off(b, qc, c, ncp, SW_GA, k), v.x, v.y);
+        }
+      } else {
+        double2* gp = S.ga + (rb0 + (long long)(t - 1) * qstep) * SW_GA + c;
+#pragma unroll
+        for (int k = 0; k < SW_GA; ++k) gp[(long long)k * ncp] = ownP[q][k][lane];
+      }
     }
     if (cell_ok(t - 1)) {
       const long long q = rb0 + (long long)(t - 1) * qstep + c;
       // the cell's three dq pairs (the last one's second half is padding)
       // as whole 16-byte stores: half-pair stores written through cost
       // twice the write traffic
-      const unsigned off = (unsigned)(((q - c - b.sbase) * SW_DQ + c) * 16);
+      const unsigned off = pair_off(b, q, c, ncp, SW_DQ, 0);
       const unsigned kst = (unsigned)ncp * 16u;
       const double* w = &dqB[(t - 1) & 1][0][lane];
       store_pair_coherent(dq_rsrc, off, w[0], w[TL]);
       store_pair_coherent(dq_rsrc, off + kst, w[2 * TL], w[3 * TL]);
       store_pair_coherent(dq_rsrc, off + 2u * kst, w[4 * TL], 0.0);
       if constexpr (TYPE == 2) {  // the partial sums the update lanes left beside the new dq
-        double2* pp = S.pre + (q - c) * SW_PRE + c;
         const double* w = &dqB[(t - 1) & 1][5][lane];
-        pp[0] = make_double2(w[0], w[TL]);
-        pp[ncp] = make_double2(w[2 * TL], w[3 * TL]);
-        pp[2 * ncp] = make_double2(w[4 * TL], 0.0);
+        if constexpr (CHAIN) {
+          store_pair_coherent(pre_rsrc, pair_off(b, q, c, ncp, SW_PRE, 0), w[0], w[TL]);
+          store_pair_coherent(pre_rsrc, pair_off(b, q, c, ncp, SW_PRE, 1), w[2 * TL], w[3 * TL]);
+          store_pair_coherent(pre_rsrc, pair_off(b, q, c, ncp, SW_PRE, 2), w[4 * TL], 0.0);
+        } else {
+          double2* pp = S.pre + (q - c) * SW_PRE + c;
+          pp[0] = make_double2(w[0], w[TL]);
+          pp[ncp] = make_double2(w[2 * TL], w[3 * TL]);
+          pp[2 * ncp] = make_double2(w[4 * TL], 0.0);
+        }
       }
     }
     // the stores above took their values from LDS before they issued, so
@@ -921,6 +992,31 @@ __global__ void __launch_bounds__(2 * TL, 1) lusgs_tile_kernel(SweepArrays S, co
     __syncthreads();
   };
 
+  // Chained launch: the tile's previous sweep must have published its last
+  // plane (its last stores had completed before that publication, as for
+  // every plane) before anything of it is read; the first loader wave waits
+  // (bounded spin, as for the upstream tiles) and a barrier releases all.
+  if constexpr (CHAIN && TYPE > 0) {
+    if (lw0) {
+      constexpr bool PFWD = (TYPE - 1) == 0 || (TYPE - 1) == 2;
+      const unsigned long long want =
+          ((e4 + TYPE) << KEY_BITS) + (PFWD ? plane_key(true, T.s_last) : plane_key(false, T.s_first));
+      unsigned long long v =
+          __hip_atomic_load(&flags[(long long)tix * FLAG_STRIDE], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      long long spins = 0;
+      while (v < want) {
+        if (++spins > (1LL << 20) ||
+            ((spins & 255) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)) {
+          if (xl == 0) atomicOr(err, 2);
+          break;
+        }
+        if (spins > 1) __builtin_amdgcn_s_sleep(2);
+        v = __hip_atomic_load(&flags[(long long)tix * FLAG_STRIDE], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
+    __syncthreads();
+  }
+
   // Both kinds of lanes run their own loop with one barrier per step (and
   // one after the set-up), so every barrier is reached by all waves.
   const int TT = (nsteps + 3) & ~1;  // whole pairs of steps, at least nsteps + 2
@@ -956,6 +1052,54 @@ __global__ void __launch_bounds__(2 * TL, 1) lusgs_tile_kernel(SweepArrays S, co
       ustep(t + 1, gaR[1], gaR[0]);
     }
   }
+}
+
+// One launch per sweep: tile order[w] on workgroup w.
+template <int TYPE, bool VIS>
+__global__ void __launch_bounds__(2 * TL, 1) lusgs_tile_kernel(SweepArrays S, const BlockDev* __restrict__ blocks,
+                                                            const SweepTile* __restrict__ tiles,
+                                                            const int* __restrict__ order, unsigned long long* flags,
+                                                            const unsigned long long* __restrict__ epoch, int* err,
+                                                            double gm1) {
+  __shared__ __align__(16) unsigned char lds[sweep_lds_bytes(TYPE)];
+  sweep_tile<TYPE, VIS, false>(S, blocks, tiles, order[blockIdx.x], flags, (*epoch) * 4, err, gm1, lds);
+}
+
+// The four sweeps as one chained launch: workgroup w runs task tasks[w] =
+// tile << 2 | sweep type.  The task order is topological (a tile's sweep
+// after its previous sweep, and after the tiles it waits for in its sweep),
+// so every wait targets a task launched earlier and the grid drains; a tile
+// whose blocks' smaller neighbours finished early starts its next sweep
+// while larger blocks are still in an earlier one.
+template <bool VIS>
+__global__ void __launch_bounds__(2 * TL, 1) lusgs_chain_kernel(SweepArrays S, const BlockDev* __restrict__ blocks,
+                                                             const SweepTile* __restrict__ tiles,
+                                                             const int* __restrict__ tasks, unsigned long long* flags,
+                                                             const unsigned long long* __restrict__ epoch, int* err,
+                                                             double gm1) {
+  __shared__ __align__(16) unsigned char lds[SWEEP_LDS_MAX];
+  const int task = tasks[blockIdx.x];
+  const int tix = task >> 2;
+  const unsigned long long e4 = (*epoch) * 4;
+  switch (task & 3) {
+    case 0: sweep_tile<0, VIS, true>(S, blocks, tiles, tix, flags, e4, err, gm1, lds); break;
+    case 1: sweep_tile<1, VIS, true>(S, blocks, tiles, tix, flags, e4, err, gm1, lds); break;
+    case 2: sweep_tile<2, VIS, true>(S, blocks, tiles, tix, flags, e4, err, gm1, lds); break;
+    default: sweep_tile<3, VIS, true>(S, blocks, tiles, tix, flags, e4, err, gm1, lds); break;
+  }
+}
+
+void launch_lusgs_chain(const SweepArrays& S, const BlockDev* blocks, const SweepTile* tiles, const int* tasks,
+                        int ntasks, unsigned long long* flags, unsigned long long* epoch, int* err, const Params& p,
+                        hipStream_t s) {
+  if (ntasks <= 0) return;
+  const double gm1 = p.gama - 1.0;
+  if (p.nvis == 1)
+    hipLaunchKernelGGL((lusgs_chain_kernel<true>), dim3(ntasks), dim3(2 * TL), 0, s, S, blocks, tiles, tasks, flags,
+                       epoch, err, gm1);
+  else
+    hipLaunchKernelGGL((lusgs_chain_kernel<false>), dim3(ntasks), dim3(2 * TL), 0, s, S, blocks, tiles, tasks, flags,
+                       epoch, err, gm1);
 }
 
 void launch_lu_geom(const Launch& La, long long n, const SweepArrays& S) {

@@ -92,6 +92,11 @@ void launch_lu_pack(const Launch& La, long long n, const SweepArrays& S, int par
 void launch_lusgs_tiles(const SweepArrays& S, const BlockDev* blocks, const SweepTile* tiles, const int* order,
                         int ntiles, unsigned long long* flags, unsigned long long* epoch, int* err, int type,
                         const Params& p, hipStream_t s);
+// the four sweeps as one launch of tasks (tile << 2 | sweep type) in a
+// topological order
+void launch_lusgs_chain(const SweepArrays& S, const BlockDev* blocks, const SweepTile* tiles, const int* tasks,
+                        int ntasks, unsigned long long* flags, unsigned long long* epoch, int* err, const Params& p,
+                        hipStream_t s);
 // ioff: prefix of the interior node counts; psum/pmax/pkey: one residual
 // partial per workgroup of the launch
 void launch_update(const Launch& La, long long n, const SweepArrays& S, double gamma, const long long* ioff,

@@ -199,11 +199,12 @@ class FlowFieldSet:
         return {"schedule": self.SCHEDULES[sch.value], "visc_segment": seg.value, "visc_grid": grid.value,
                 "ms_per_step": ms.value, "trials": n.value, "steps_per_trial": int(steps)}
 
-    SCHEDULES = ("diagonal", "critical_path")
+    SCHEDULES = ("diagonal", "critical_path", "chained")
 
     def set_schedule(self, which: int) -> None:
         """The LU-SGS tile launch order (0 diagonal, 1 critical path first,
-        the default); the results are the same under every order."""
+        the default, both a launch per sweep; 2 the four sweeps chained in
+        one launch); the results are the same under every order."""
         check(_lib.load().orion_hip_set_schedule(self.ctx, int(which)), "set_schedule")
 
     def schedule(self) -> int:

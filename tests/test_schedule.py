@@ -1,5 +1,6 @@
-"""The LU-SGS tile launch orders, their autotuner, and the plain per-block
-launch form.
+"""The LU-SGS tile launch orders (a launch per sweep, or the four sweeps
+chained in one launch), their autotuner, and the plain per-block launch
+form.
 
 Every schedule launches a tile after the tiles it waits for, and each
 cell's arithmetic does not depend on the order, so the results must be the
@@ -24,7 +25,7 @@ def test_schedules_give_identical_results():
     case = _case()
     print("\n[gpu] sweep schedules: %d blocks, %d nodes" % (case.nblocks, case.interior_nodes()), flush=True)
     runs = []
-    for k in (0, 1):
+    for k in (0, 1, 2):
         s = OrionSolver(case)
         s.fs.set_schedule(k)
         assert s.fs.schedule() == k
@@ -32,9 +33,10 @@ def test_schedules_give_identical_results():
         runs.append(([x.rms_residual for x in h], [x.max_loc for x in h],
                      [s.fs.field(b, "prim") for b in range(case.nblocks)]))
         s.fs.close()
-    assert runs[0][0] == runs[1][0] and runs[0][1] == runs[1][1]
-    for a, b in zip(runs[0][2], runs[1][2]):
-        assert np.array_equal(a, b)
+    for k in (1, 2):
+        assert runs[0][0] == runs[k][0] and runs[0][1] == runs[k][1], k
+        for a, b in zip(runs[0][2], runs[k][2]):
+            assert np.array_equal(a, b), k
 
 
 @pytest.mark.gpu
@@ -46,7 +48,7 @@ def test_tuning_restores_the_state():
     a.solve(2)
     before = [a.fs.field(b, "prim") for b in range(case.nblocks)]
     chosen, ms = a.fs.tune_schedule(2)
-    assert chosen in (0, 1) and a.fs.schedule() == chosen and all(m > 0 for m in ms)
+    assert chosen in (0, 1, 2) and a.fs.schedule() == chosen and all(m > 0 for m in ms)
     for b in range(case.nblocks):
         assert np.array_equal(a.fs.field(b, "prim"), before[b]), b
     # and the solve goes on as one that was never tuned
