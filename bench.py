@@ -82,6 +82,13 @@ def use_sweep_kernels(chained):
     DOMINANT = ("lusgs_chain_kernel" if chained else "lusgs_tile_kernel",) + DOMINANT[1:3] + (k.split("+"),)
 
 
+def use_visc_kernels(fused):
+    """Name the viscous kernels of the form in use in the stage table."""
+    faces = "visc_fused_kernel<true>" if fused else "visc_face_queue_kernel+diff_kernel<false,true>"
+    STAGES["rhs"] = STAGES["rhs"][:2] + ("visc_deriv_kernel+" + faces + "+inv_face_kernel+inv_closure_kernel"
+                                         "+diff_kernel<true,false>",)
+
+
 def make_case(name, nsteps):
     from orion_amd import cases
     if name == "uneven64":
@@ -450,6 +457,7 @@ def main():
         print("[bench] plain per-block form: %.3f ms/step (batched %.3f), same results: %s" % (
             1e3 * tp / n, 1e3 * t / a.steps, same), file=sys.stderr, flush=True)
     use_sweep_kernels(s.fs.schedule() == 2)
+    use_visc_kernels(s.fs.visc_form())
     phase = profile_phases(s.fs, a.profile_steps)
     phases = {k: round(v[0], 4) for k, v in phase.items()}
     print("[bench] phase ms/step: " + json.dumps(phases), file=sys.stderr, flush=True)

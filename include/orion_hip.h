@@ -186,15 +186,28 @@ int orion_hip_tune_schedule(orion_hip_ctx* ctx, int steps, int* chosen, double* 
 
 /* No counterpart in the reference: the tiling of the viscous face-flux
  * stage, one persistent launch of `grid` workgroups that take items from a
- * block work queue -- along j and k, 256 lines by `segment` faces, each
- * lane marching along its line; along i, 256 face pairs.  Results do not
- * depend on the tiling.  Defaults: segment 32, grid 2048. */
+ * block work queue.  In the fused form (orion_hip_set_visc_form) an item is
+ * 16 x 16 columns by `segment` planes; in the face-flux queue form, along j
+ * and k, 256 lines by `segment` faces, each lane marching along its line,
+ * and along i, 256 face pairs.  Results do not depend on the tiling.
+ * Defaults: segment 32, grid 2048. */
 int orion_hip_set_visc_tiles(orion_hip_ctx* ctx, int segment, int grid);
 int orion_hip_get_visc_tiles(const orion_hip_ctx* ctx, int* segment, int* grid);
+/* No counterpart in the reference: the form of the viscous right-hand side
+ * after the node derivatives.  1 (where every block has at least 4 nodes
+ * along each axis): fused -- one persistent launch whose
+ * items are tiles of 16 x 16 (i, j) columns by `segment` planes along k;
+ * each forms its i, j and k face fluxes on the chip and differences them
+ * into dq, so no face flux goes through device memory.  0 (the default):
+ * the face-flux queue above, then a difference kernel.  Results are the same bit for
+ * bit.  get returns the form in use; orion_hip_tune tries both. */
+int orion_hip_set_visc_form(orion_hip_ctx* ctx, int fused);
+int orion_hip_get_visc_form(const orion_hip_ctx* ctx);
 /* The autotuner of the step's launch shapes: from the current state, times
  * `steps` pseudo-steps (after one untimed step) per candidate, restoring the
  * state after each, and keeps the fastest by coordinate search -- the LU-SGS
- * tile schedule (ORION_SCHEDULES), then the viscous segment (8, 16, 32, 64),
+ * tile schedule (ORION_SCHEDULES), then the viscous form (fused or face
+ * queue, where both apply), then the viscous segment (8, 16, 32, 64),
  * then its grid (1024, 2048, 4096 workgroups).  Outputs (each may be NULL):
  * the chosen schedule, segment and grid, the best ms per step, the trials
  * run.  The solution state afterwards is the one before the call, bit for

@@ -89,6 +89,8 @@ def _open():
         "orion_hip_set_per_block": ([P, C.c_int], C.c_int),
         "orion_hip_set_visc_tiles": ([P, C.c_int, C.c_int], C.c_int),
         "orion_hip_get_visc_tiles": ([P, C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
+        "orion_hip_set_visc_form": ([P, C.c_int], C.c_int),
+        "orion_hip_get_visc_form": ([P], C.c_int),
         "orion_hip_tune": ([P, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
                             C.POINTER(C.c_double), C.POINTER(C.c_int)], C.c_int),
         "orion_hip_residual_noalloc": ([C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),

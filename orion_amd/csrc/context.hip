@@ -152,6 +152,14 @@ struct orion_hip_ctx {
   int* d_vhead = nullptr;
   int visc_seg = 32;
   int visc_grid = 2048;
+  // the fused viscous form (visc_fused_kernel: face fluxes and their
+  // difference in one pass, items of 16 x 16 columns by visc_seg planes),
+  // the default where every block line has at least 4 nodes
+  bool visc_fused = false;
+  bool visc_fused_ok = false;
+  ViscTile* d_vtiles = nullptr;
+  long long vtiles_cap = 0;
+  int n_vtiles = 0;
 };
 
 // Device allocations of a context, freed with it.  Tables up to 64 KB share
@@ -618,8 +626,42 @@ static std::vector<ViscItem> visc_items(const orion_hip_ctx* c, int seg) {
   return v;
 }
 
+// The fused viscous form's items: per block, tiles of 16 x 16 (i, j)
+// columns by `seg` planes along k, the items with the most nodes first.
+static std::vector<ViscTile> column_tiles(const orion_hip_ctx* c, int seg) {
+  std::vector<ViscTile> v;
+  for (int b = 0; b < c->nb; ++b) {
+    const BlockDev& B = c->hb[b];
+    for (int k0 = 0; k0 < B.kdim; k0 += seg)
+      for (int j0 = 0; j0 < B.jdim; j0 += 16)
+        for (int i0 = 0; i0 < B.idim; i0 += 16) v.push_back(ViscTile{b, i0, j0, k0, std::min(B.kdim, k0 + seg)});
+  }
+  auto nodes = [&](const ViscTile& t) {
+    const BlockDev& B = c->hb[t.block];
+    return (long long)std::min(16, B.idim - t.i0) * std::min(16, B.jdim - t.j0) * (t.k1 - t.k0);
+  };
+  std::stable_sort(v.begin(), v.end(), [&](const ViscTile& a, const ViscTile& b) { return nodes(a) > nodes(b); });
+  return v;
+}
+
 static int set_visc_queue(orion_hip_ctx* c, int seg, int grid) {
   if (seg < 1 || grid < 1) return fail("viscous queue: segment and grid must be positive");
+  if (c->visc_fused) {
+    std::vector<ViscTile> v = column_tiles(c, seg);
+    if (v.size() >= (size_t)INT32_MAX) return fail("viscous queue: too many items");
+    if ((long long)v.size() > c->vtiles_cap) {
+      const long long cap = std::max<long long>((long long)v.size(), (long long)column_tiles(c, 8).size());
+      if (dalloc(c, (void**)&c->d_vtiles, sizeof(ViscTile) * cap)) return -1;
+      c->vtiles_cap = cap;
+    }
+    HIPCHK(hipMemcpyAsync(c->d_vtiles, v.data(), sizeof(ViscTile) * v.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->n_vtiles = (int)v.size();
+    c->visc_seg = seg;
+    c->visc_grid = grid;
+    c->graph_stream = nullptr;  // the captured step holds the old count: capture again
+    return 0;
+  }
   std::vector<ViscItem> v = visc_items(c, seg);
   if (v.size() >= (size_t)INT32_MAX) return fail("viscous queue: too many items");
   if ((long long)v.size() > c->vitems_cap) {
@@ -1216,6 +1258,9 @@ static int create_ctx(int nreq, const int* req_nblocks, int nblocks, const int* 
   }
   if (upload_vec(c, &c->db, c->hb)) return bail(-1);
   if (c->p.nvis == 1) {
+    c->visc_fused_ok = true;
+    for (const BlockDev& B : c->hb) c->visc_fused_ok = c->visc_fused_ok && B.idim >= 4 && B.jdim >= 4 && B.kdim >= 4;
+    c->visc_fused = false;  // the face-flux queue measured faster; the tuner tries both
     if (dalloc(c, (void**)&c->d_vhead, sizeof(int)) || set_visc_queue(c, c->visc_seg, c->visc_grid)) return bail(-1);
   }
   if (build_topology(c, R)) return bail(-1);
@@ -1407,6 +1452,10 @@ static void op_visc(orion_hip_ctx* c, bool first = false) {
     return;
   }
   launch_visc_deriv(LA(c, c->intr), c->intr.total(), c->d_vhead);
+  if (c->visc_fused) {
+    launch_visc_fused(c->db, c->d_vtiles, c->n_vtiles, c->d_vhead, c->visc_grid, first, c->stream);
+    return;
+  }
   launch_visc_face_queue(c->db, c->d_vitems, c->n_vitems, c->d_vhead, c->visc_grid, c->stream);
   launch_visc_diff(LA(c, c->dtile), c->dtile.total(), first);
 }
@@ -1732,9 +1781,15 @@ int orion_hip_profile_run(orion_hip_ctx* c, int nsteps, double* phase_ms, long l
     launch_state(LA(c, c->pad), c->pad.total(), 3 | 4, c->d_dtmin + (long long)s * c->nreq, nullptr, c->nreq); mark(1);
     if (c->p.nvis == 1) {
       launch_visc_deriv(LA(c, c->intr), c->intr.total(), c->d_vhead); mark(1);
-      launch_visc_face_queue(c->db, c->d_vitems, c->n_vitems, c->d_vhead, c->visc_grid, c->stream);
-      mark(1);
-      launch_visc_diff(LA(c, c->dtile), c->dtile.total(), true); mark(1);
+      if (c->visc_fused) {  // faces and difference in one launch, timed as the face phase
+        launch_visc_fused(c->db, c->d_vtiles, c->n_vtiles, c->d_vhead, c->visc_grid, true, c->stream);
+        mark(1);
+        mark(0);
+      } else {
+        launch_visc_face_queue(c->db, c->d_vitems, c->n_vitems, c->d_vhead, c->visc_grid, c->stream);
+        mark(1);
+        launch_visc_diff(LA(c, c->dtile), c->dtile.total(), true); mark(1);
+      }
       op_avg(c); mark(2);
     } else {
       mark(0); mark(0); mark(0); mark(0);
@@ -1990,6 +2045,17 @@ int orion_hip_set_visc_tiles(orion_hip_ctx* c, int segment, int grid) {
   return set_visc_queue(c, segment, grid);
 }
 
+int orion_hip_set_visc_form(orion_hip_ctx* c, int fused) {
+  if (!c) return fail("set_visc_form: null context");
+  if (c->p.nvis != 1) return 0;  // no viscous flux
+  if (fused && !c->visc_fused_ok) return fail("set_visc_form: the fused form needs at least 4 nodes on every block line");
+  if ((fused != 0) == c->visc_fused) return 0;
+  c->visc_fused = fused != 0;
+  return set_visc_queue(c, c->visc_seg, c->visc_grid);
+}
+
+int orion_hip_get_visc_form(const orion_hip_ctx* c) { return c && c->visc_fused ? 1 : 0; }
+
 int orion_hip_get_visc_tiles(const orion_hip_ctx* c, int* segment, int* grid) {
   if (!c) return fail("get_visc_tiles: null context");
   if (segment) *segment = c->visc_seg;
@@ -2022,8 +2088,17 @@ int orion_hip_tune(orion_hip_ctx* c, int steps, int* schedule, int* segment, int
     }
   }
   if (rc == 0) rc = orion_hip_set_schedule(c, bs);
-  // the viscous faces per marching item, then the persistent grid
+  // the viscous form, then its planes or faces per item, then the
+  // persistent grid
   if (c->p.nvis == 1) {
+    if (c->visc_fused_ok && rc == 0) {
+      const bool was = c->visc_fused;
+      double ms = 0.0;
+      rc = orion_hip_set_visc_form(c, was ? 0 : 1);
+      if (rc == 0) rc = measure(&ms);
+      if (rc == 0 && !(ms < best)) rc = orion_hip_set_visc_form(c, was ? 1 : 0);
+      else if (rc == 0) best = ms;
+    }
     int bseg = c->visc_seg, bgrid = c->visc_grid;
     static const int SEGS[] = {8, 16, 32, 64};
     static const int GRIDS[] = {1024, 2048, 4096};

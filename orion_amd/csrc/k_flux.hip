@@ -567,8 +567,9 @@ __device__ __forceinline__ constexpr unsigned uvwt_field(int v) { return v < 3 ?
 // by constants and stay in registers (a run-time index puts them in
 // scratch memory).
 template <int D, class WF, class MF, class XF>
-__device__ __forceinline__ void visc_face_at(const BlockDev& b, unsigned base, unsigned s, int n, int f,
-                                             bool fl, bool fr, const WF& Wv, const MF& Mv, const XF& Xv) {
+__device__ __forceinline__ void visc_face_val(const BlockDev& b, unsigned base, unsigned s, int n, int f,
+                                              bool fl, bool fr, const WF& Wv, const MF& Mv, const XF& Xv,
+                                              double* out) {
   constexpr int d = D;
   const unsigned P = (unsigned)b.P;
   constexpr int o0 = (d == 0) ? 1 : 0, o1 = (d == 2) ? 1 : 2;
@@ -621,13 +622,26 @@ __device__ __forceinline__ void visc_face_at(const BlockDev& b, unsigned base, u
   const double f3 = txz * nx + tyz * ny + tzz * nz;
   const double work = uvwt[0] * f1 + uvwt[1] * f2 + uvwt[2] * f3;
   const double heat = vs2 * (gx[9] * nx + gx[10] * ny + gx[11] * nz);
-  // this direction's face fluxes; the mass component is zero, is never
-  // stored, and the viscous difference skips it (dq0 - Re^-1 0 == dq0)
-  double* F = b.fx + (long long)d * 5 * b.P;
-  sto(F, boff(1, P, g), f1);
-  sto(F, boff(2, P, g), f2);
-  sto(F, boff(3, P, g), f3);
-  sto(F, boff(4, P, g), work + heat);
+  // components 1..4; the mass component is zero, and the viscous
+  // difference skips it (dq0 - Re^-1 0 == dq0)
+  out[0] = f1;
+  out[1] = f2;
+  out[2] = f3;
+  out[3] = work + heat;
+}
+
+// visc_face_val stored at the face's slot of this direction's flux arrays
+// (components 1..4; the zero mass component is never stored)
+template <int D, class WF, class MF, class XF>
+__device__ __forceinline__ void visc_face_at(const BlockDev& b, unsigned base, unsigned s, int n, int f,
+                                             bool fl, bool fr, const WF& Wv, const MF& Mv, const XF& Xv) {
+  double o[4];
+  visc_face_val<D>(b, base, s, n, f, fl, fr, Wv, Mv, Xv, o);
+  const unsigned P = (unsigned)b.P;
+  const unsigned g = base + (unsigned)(f + 2) * s;
+  double* F = b.fx + (long long)D * 5 * b.P;
+#pragma unroll
+  for (int m = 0; m < 4; ++m) sto(F, boff(1 + m, P, g), o[m]);
 }
 
 __device__ __forceinline__ int clamp_interior(int p, int n) { return p < NG ? NG : (p > n + NG - 1 ? n + NG - 1 : p); }
@@ -715,19 +729,25 @@ __device__ __forceinline__ void visc_face_march(const BlockDev& b, long long lin
 #pragma unroll
   for (int j = 0; j < 4; ++j) visc_window_load<D>(b, base, s, n, f0 + 1 + j, W[j], M[j], X[j]);
   for (int f = f0; f < f1; ++f) {
+    // the next face's new stencil values are loaded before this face's
+    // flux is formed (and stored), so that their latency overlaps it
+    double Wn[4] = {0.0, 0.0, 0.0, 0.0}, Mn = 0.0, Xn[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (f + 1 < f1) visc_window_load<D>(b, base, s, n, f + 5, Wn, Mn, Xn);
     visc_face_at<D>(b, base, s, n, f, fl, fr, [&](int v, int j) { return W[j][v]; }, [&](int j) { return M[j]; },
                     [&](int c, int j) { return X[j][c]; });
-    if (f + 1 < f1) {
 #pragma unroll
-      for (int j = 0; j < 3; ++j) {
+    for (int j = 0; j < 3; ++j) {
 #pragma unroll
-        for (int v = 0; v < 4; ++v) W[j][v] = W[j + 1][v];
-        M[j] = M[j + 1];
+      for (int v = 0; v < 4; ++v) W[j][v] = W[j + 1][v];
+      M[j] = M[j + 1];
 #pragma unroll
-        for (int c = 0; c < 8; ++c) X[j][c] = X[j + 1][c];
-      }
-      visc_window_load<D>(b, base, s, n, f + 5, W[3], M[3], X[3]);
+      for (int c = 0; c < 8; ++c) X[j][c] = X[j + 1][c];
     }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) W[3][v] = Wn[v];
+    M[3] = Mn;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) X[3][c] = Xn[c];
   }
 }
 
@@ -772,6 +792,150 @@ __device__ __forceinline__ double diff_node_f(const FF& F, int n, int t) {
   if (t == n - 2) return -(F(n) - 27.0 * F(n - 1) + 27.0 * F(n - 2) - F(n - 3)) / 24.0;
   if (t == 0) return (-23.0 * F(0) + 21.0 * F(1) + 3.0 * F(2) - F(3)) / 24.0;
   return -(-23.0 * F(n) + 21.0 * F(n - 1) + 3.0 * F(n - 2) - F(n - 3)) / 24.0;
+}
+
+// The difference of diff_node_f at node t over a window W(w) of the six
+// faces t-2 .. t+3 (w = 0..5), with the same branches in the same order and
+// the same expressions; each branch reads its faces at constant window
+// slots (face f at slot f - t + 2), so a window held in registers stays
+// there.  Slots of faces outside 0..n are never read.
+template <class WF>
+__device__ __forceinline__ double diff_node_w(const WF& W, int n, int t) {
+  if (t >= 2 && t < n - 2)
+    return (2250.0 * (W(3) - W(2)) + -125.0 * (W(4) - W(1)) + 9.0 * (W(5) - W(0))) / 1920.0;
+  if (t == 1) return (W(1) - 27.0 * W(2) + 27.0 * W(3) - W(4)) / 24.0;
+  if (t == n - 2) return -(W(4) - 27.0 * W(3) + 27.0 * W(2) - W(1)) / 24.0;
+  if (t == 0) return (-23.0 * W(2) + 21.0 * W(3) + 3.0 * W(4) - W(5)) / 24.0;
+  return -(-23.0 * W(3) + 21.0 * W(2) + 3.0 * W(1) - W(0)) / 24.0;
+}
+
+// first and last face that diff_node_f reads for node t of a line of n >= 4
+// nodes; both are non-decreasing in t, so the faces of nodes ta..tb lie in
+// [face_lo(ta), face_hi(tb)], at most tb - ta + 6 of them
+__device__ __forceinline__ int face_lo(int t, int n) { return t < 2 ? 0 : (t >= n - 2 ? n - 3 : t - 2); }
+__device__ __forceinline__ int face_hi(int t, int n) { return t < 2 ? 3 : (t >= n - 2 ? n : t + 3); }
+
+// One viscous face flux (visc_face_val) of the line along D whose position
+// 0 is at padded index base, every stencil value loaded from the arrays.
+template <int D>
+__device__ __forceinline__ void visc_face_direct(const BlockDev& b, unsigned base, unsigned s, int n, int f,
+                                                 double* o) {
+  constexpr int o0 = (D == 0) ? 1 : 0, o1 = (D == 2) ? 1 : 2;
+  const unsigned P = (unsigned)b.P;
+  const bool fl = ldo(b.prim, base << 3) < 0.0;
+  const bool fr = ldo(b.prim, (base + (unsigned)(n + 5) * s) << 3) < 0.0;
+  visc_face_val<D>(
+      b, base, s, n, f, fl, fr,
+      [&](int v, int j) { return ldo(b.prim, boff(uvwt_field(v), P, base + (unsigned)(f + 1 + j) * s)); },
+      [&](int j) { return ldo(b.mu, (base + (unsigned)(f + 1 + j) * s) << 3); },
+      [&](int c, int j) {
+        const unsigned e = base + (unsigned)clamp_interior(f + 1 + j, n) * s;
+        return ldo(b.duvwt, boff(c < 4 ? 4 * o0 + c : 4 * o1 + c - 4, P, e));
+      },
+      o);
+}
+
+// The viscous right-hand side after the node derivatives, as one
+// persistent launch fed from a block work queue (the derivative kernel
+// zeroes *head).  An item is a tile of VT x VT (i, j) interior nodes of one
+// block over the planes [k0, k1); its workgroup marches along k, and at
+// each plane
+//  * forms the i faces and j faces that the tile's nodes difference (at most
+//    VT + 5 per row or column) into LDS, one face per thread in turn, with
+//    loads along i (i faces) or across columns (j faces) coalesced;
+//  * forms, per column, the k face three planes ahead, into a window of the
+//    six k faces the node differences, held in registers (the item starts
+//    by forming the five faces before its first plane's lead face);
+//  * differences i, j and k in that order into dq: dq -= Re^-1 dF per
+//    direction (compute_flux_diff_1d / compute_flux_diff_high_order).
+// Each face flux is the value the face kernels store for it and each
+// difference the expression diff_kernel forms over the same values, so dq
+// is the same bit for bit; no face flux goes through HBM.  Face arrays in
+// LDS alternate by plane, so one barrier per plane separates a plane's
+// faces from its difference and the difference from the faces of the plane
+// after next.  FIRST: the step's first right-hand-side kernel (dq starts
+// from zero; component 0 is written as +0).  Lines of fewer than 4 nodes
+// are not handled (the host keeps such contexts on the face-flux queue).
+constexpr int VT = 16;
+template <bool FIRST>
+__global__ void __launch_bounds__(256) visc_fused_kernel(const BlockDev* __restrict__ blocks,
+                                                         const ViscTile* __restrict__ items, int nitems,
+                                                         int* head) {
+  __shared__ double Fi[2][4][VT][VT + 5];  // [plane parity][component][row][face - first i face]
+  __shared__ double Fj[2][4][VT + 5][VT];  // [plane parity][component][face - first j face][column]
+  __shared__ int s_item;
+  const int lane = threadIdx.x, li = lane % VT, lj = lane / VT;
+  for (;;) {
+    if (lane == 0) s_item = atomicAdd(head, 1);
+    __syncthreads();
+    const int it = __builtin_amdgcn_readfirstlane(s_item);  // the same for the whole workgroup
+    __syncthreads();
+    if (it >= nitems) return;
+    const ViscTile Q = items[it];
+    const BlockDev& b = blocks[Q.block];
+    const unsigned P = (unsigned)b.P;
+    const int ni = b.idim, nj = b.jdim, nk = b.kdim;
+    const int ti1 = min(Q.i0 + VT, ni), tj1 = min(Q.j0 + VT, nj);
+    const int ncol = ti1 - Q.i0, nrow = tj1 - Q.j0;
+    const int fi0 = face_lo(Q.i0, ni), nfi = face_hi(ti1 - 1, ni) - fi0 + 1;
+    const int fj0 = face_lo(Q.j0, nj), nfj = face_hi(tj1 - 1, nj) - fj0 + 1;
+    const int nface_i = nfi * nrow, nface = nface_i + nfj * ncol;
+    const bool mine = li < ncol && lj < nrow;
+    const int ti = Q.i0 + li, tj = Q.j0 + lj;  // this lane's node (0-based interior)
+    const unsigned sk = (unsigned)b.nx * (unsigned)b.ny;
+    const unsigned kbase = mine ? (unsigned)pidx(b, ti + NG, tj + NG, 0) : 0u;  // its column's k line
+    const double re_inv = b.re_inv;
+    double KW[6][4];
+#pragma unroll
+    for (int w = 0; w < 6; ++w)
+#pragma unroll
+      for (int m = 0; m < 4; ++m) KW[w][m] = 0.0;
+#pragma unroll
+    for (int w = 0; w < 5; ++w) {
+      const int f = Q.k0 - 2 + w;
+      if (mine && f >= 0 && f <= nk) visc_face_direct<2>(b, kbase, sk, nk, f, KW[w]);
+    }
+    int par = 0;
+    for (int tk = Q.k0; tk < Q.k1; ++tk) {
+      if (mine && tk + 3 <= nk) visc_face_direct<2>(b, kbase, sk, nk, tk + 3, KW[5]);
+      const int k = tk + NG;
+      for (int e = lane; e < nface; e += 256) {
+        double o[4];
+        if (e < nface_i) {
+          unsigned row, sl;
+          udivmod((unsigned)e, (unsigned)nfi, &row, &sl);
+          visc_face_direct<0>(b, (unsigned)pidx(b, 0, Q.j0 + (int)row + NG, k), 1u, ni, fi0 + (int)sl, o);
+#pragma unroll
+          for (int m = 0; m < 4; ++m) Fi[par][m][row][sl] = o[m];
+        } else {
+          unsigned sl, col;
+          udivmod((unsigned)(e - nface_i), (unsigned)ncol, &sl, &col);
+          visc_face_direct<1>(b, (unsigned)pidx(b, Q.i0 + (int)col + NG, 0, k), (unsigned)b.nx, nj, fj0 + (int)sl,
+                              o);
+#pragma unroll
+          for (int m = 0; m < 4; ++m) Fj[par][m][sl][col] = o[m];
+        }
+      }
+      __syncthreads();
+      if (mine) {
+        const unsigned id = kbase + (unsigned)k * sk;
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+          double acc = FIRST ? 0.0 : ldo(b.dq, boff(1 + m, P, id));
+          acc -= re_inv * diff_node_f([&](int f) { return Fi[par][m][lj][f - fi0]; }, ni, ti);
+          acc -= re_inv * diff_node_f([&](int f) { return Fj[par][m][f - fj0][li]; }, nj, tj);
+          acc -= re_inv * diff_node_w([&](int w) { return KW[w][m]; }, nk, tk);
+          sto(b.dq, boff(1 + m, P, id), acc);
+        }
+        if (FIRST) sto(b.dq, boff(0, P, id), 0.0);
+      }
+#pragma unroll
+      for (int w = 0; w < 5; ++w)
+#pragma unroll
+        for (int m = 0; m < 4; ++m) KW[w][m] = KW[w + 1][m];
+      par ^= 1;
+    }
+  }
 }
 
 // Flux differences of the inviscid (INV, closure overrides included) or the
@@ -902,6 +1066,15 @@ void launch_visc_face_queue(const BlockDev* blocks, const ViscItem* items, int n
   if (nitems <= 0) return;
   const int g = grid < nitems ? grid : nitems;
   hipLaunchKernelGGL(visc_face_queue_kernel, dim3((unsigned)g), dim3(256), 0, s, blocks, items, nitems, head);
+}
+void launch_visc_fused(const BlockDev* blocks, const ViscTile* items, int nitems, int* head, int grid, bool first,
+                       hipStream_t s) {
+  if (nitems <= 0) return;
+  const int g = grid < nitems ? grid : nitems;
+  if (first)
+    hipLaunchKernelGGL(visc_fused_kernel<true>, dim3((unsigned)g), dim3(256), 0, s, blocks, items, nitems, head);
+  else
+    hipLaunchKernelGGL(visc_fused_kernel<false>, dim3((unsigned)g), dim3(256), 0, s, blocks, items, nitems, head);
 }
 void launch_face_geom(const Launch& L, int d, long long n) {
   if (n <= 0) return;

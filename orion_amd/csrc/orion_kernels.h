@@ -57,6 +57,17 @@ struct ViscItem {
 };
 void launch_visc_face_queue(const BlockDev* blocks, const ViscItem* items, int nitems, int* head, int grid,
                             hipStream_t s);
+// one item of the fused viscous right-hand side (visc_fused_kernel): the
+// tile of 16 x 16 (i, j) interior nodes from (i0, j0) (0-based) of block
+// over the planes [k0, k1) along k
+struct ViscTile {
+  int block, i0, j0, k0, k1;
+};
+// the face fluxes and their difference in one persistent launch over the
+// item queue (*head zeroed by the derivative kernel before it); first: the
+// step's first right-hand-side kernel (dq from zero)
+void launch_visc_fused(const BlockDev* blocks, const ViscTile* items, int nitems, int* head, int grid, bool first,
+                       hipStream_t s);
 void launch_face_geom(const Launch& L, int d, long long n);
 void launch_visc_face(const Launch& L, int d, long long n);
 void launch_visc_diff(const Launch& L, long long n, bool first);

@@ -179,9 +179,20 @@ class FlowFieldSet:
         check(_lib.load().orion_hip_set_per_block(self.ctx, int(bool(on))), "set_per_block")
 
     def set_visc_tiles(self, segment: int, grid: int) -> None:
-        """The viscous face-flux stage's tiling: faces per marching item and
-        workgroups of its persistent launch (results do not depend on it)."""
+        """The viscous face-flux stage's tiling: planes per item (fused
+        form) or faces per marching item (queue form), and the workgroups of
+        its persistent launch (results do not depend on it)."""
         check(_lib.load().orion_hip_set_visc_tiles(self.ctx, int(segment), int(grid)), "set_visc_tiles")
+
+    def set_visc_form(self, fused: bool) -> None:
+        """The viscous right-hand side after the node derivatives: fused
+        (face fluxes and their difference in one pass, the default where
+        every block line has at least 4 nodes) or the face-flux queue and a
+        difference kernel; the results are the same bit for bit."""
+        check(_lib.load().orion_hip_set_visc_form(self.ctx, int(bool(fused))), "set_visc_form")
+
+    def visc_form(self) -> bool:
+        return bool(_lib.load().orion_hip_get_visc_form(self.ctx))
 
     def visc_tiles(self):
         seg, grid = C.c_int(), C.c_int()
@@ -196,7 +207,8 @@ class FlowFieldSet:
         ms = C.c_double()
         check(_lib.load().orion_hip_tune(self.ctx, int(steps), C.byref(sch), C.byref(seg), C.byref(grid),
                                          C.byref(ms), C.byref(n)), "tune")
-        return {"schedule": self.SCHEDULES[sch.value], "visc_segment": seg.value, "visc_grid": grid.value,
+        return {"schedule": self.SCHEDULES[sch.value], "visc_form": "fused" if self.visc_form() else "face_queue",
+                "visc_segment": seg.value, "visc_grid": grid.value,
                 "ms_per_step": ms.value, "trials": n.value, "steps_per_trial": int(steps)}
 
     SCHEDULES = ("diagonal", "critical_path", "chained")

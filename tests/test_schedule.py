@@ -96,22 +96,25 @@ def test_per_block_form_equals_batched(name):
 
 @pytest.mark.gpu
 def test_viscous_tiling_gives_identical_results():
-    """The viscous face-flux work queue under other tilings (faces per
-    marching item, persistent grid) gives the same histories and fields bit
-    for bit, and the launch-shape tuner leaves the state as it found it."""
+    """The viscous face-flux work queue and the fused viscous form under
+    other tilings (faces or planes per item, persistent grid) give the same
+    histories and fields bit for bit, and the launch-shape tuner leaves the
+    state as it found it."""
     from orion_amd.solver import OrionSolver
     case = _cases.columns_small(nsteps=4)
     print("\n[gpu] viscous tilings", flush=True)
     ref = OrionSolver(case)
     hr = ref.solve(4)
-    for seg, grid in ((8, 64), (64, 4096), (1, 1)):
+    for fused, seg, grid in ((False, 8, 64), (False, 64, 4096), (False, 1, 1), (True, 1, 1), (True, 5, 7),
+                             (True, 64, 4096)):
         s = OrionSolver(case)
+        s.fs.set_visc_form(fused)
         s.fs.set_visc_tiles(seg, grid)
         assert s.fs.visc_tiles() == (seg, grid)
         hs = s.solve(4)
-        assert [h.rms_residual for h in hs] == [h.rms_residual for h in hr], (seg, grid)
+        assert [h.rms_residual for h in hs] == [h.rms_residual for h in hr], (fused, seg, grid)
         for b in range(case.nblocks):
-            assert np.array_equal(s.fs.field(b, "prim"), ref.fs.field(b, "prim")), (seg, grid, b)
+            assert np.array_equal(s.fs.field(b, "prim"), ref.fs.field(b, "prim")), (fused, seg, grid, b)
         s.fs.close()
     t = OrionSolver(case)
     t.solve(2)
@@ -122,3 +125,48 @@ def test_viscous_tiling_gives_identical_results():
         assert np.array_equal(t.fs.field(b, "prim"), before[b])
     t.fs.close()
     ref.fs.close()
+
+
+VISCOUS = [n for n in sorted(_cases.ALL) if n not in ("bc_variety_inviscid", "lattice_supersonic")]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", VISCOUS)
+def test_fused_viscous_form_equals_face_queue(name):
+    """The fused viscous form (face fluxes of the three directions and their
+    difference in one pass) and the face-flux queue with its difference
+    kernel give the same residual histories and fields bit for bit: in the
+    captured step, through a switch mid-run, and operator by operator (the
+    accumulating, not-first form of the kernel)."""
+    from orion_amd.solver import OrionSolver
+    case = _cases.ALL[name](nsteps=6)
+    if int(case.inputs["nvis"]) != 1:
+        pytest.skip("inviscid case")
+    print("\n[gpu] fused viscous vs face queue:", name, flush=True)
+    a, b = OrionSolver(case), OrionSolver(case)
+    assert not a.fs.visc_form()
+    a.fs.set_visc_form(True)
+    assert a.fs.visc_form()
+    b.fs.set_visc_form(False)
+    assert not b.fs.visc_form()
+    ha, hb = a.solve(3), b.solve(3)
+    b.fs.set_visc_form(True)
+    a.fs.set_visc_form(False)
+    ha += a.solve(3)
+    hb += b.solve(3)
+    assert [h.rms_residual for h in ha] == [h.rms_residual for h in hb]
+    assert [h.max_loc for h in ha] == [h.max_loc for h in hb]
+    for blk in range(case.nblocks):
+        for f in ("prim", "dq", "dt"):
+            assert np.array_equal(a.fs.field(blk, f), b.fs.field(blk, f)), (blk, f)
+    # one viscous operator call on top of a non-zero dq (the inviscid one
+    # first), in both forms, on the same state
+    from orion_amd.solver import FluxComputer, InviscidFluxComputer
+    for s, fused in ((a, True), (b, False)):
+        s.fs.set_visc_form(fused)
+        InviscidFluxComputer.compute_inviscid_rhs(s.fs, s.params)
+        FluxComputer.compute_viscous_rhs(s.fs, s.params)
+    for blk in range(case.nblocks):
+        assert np.array_equal(a.fs.field(blk, "dq"), b.fs.field(blk, "dq")), blk
+    a.fs.close()
+    b.fs.close()
