@@ -172,12 +172,14 @@ __device__ inline void weno5(double qm2, double qm1, double q0, double qp1, doub
   }
 }
 
+// The prim values of one line: field m at position p (0 .. n+5).  Offsets
+// are 32-bit (a block's prim holds 6 fields of P < 2^25 points).
 struct LineRef {
   const double* prim;
-  long long P;
-  long long base;
-  long long s;
-  __device__ inline double at(int m, int p) const { return prim[(long long)m * P + base + (long long)p * s]; }
+  unsigned P;
+  unsigned base;  // padded index of position 0
+  unsigned s;     // stride along the line
+  __device__ inline double at(int m, int p) const { return ldo(prim, boff((unsigned)m, P, base + (unsigned)p * s)); }
 };
 
 // Initial face states of face f (after the WCNS pass and the fixed boundary
@@ -270,22 +272,24 @@ __global__ void __launch_bounds__(256) inv_face_kernel(const BlockDev* __restric
   int f;
   decode_face(b, d, gi, &base, &f);
   const int n = dcount(b, d);
-  const long long s = dstride(b, d);
-  const LineRef L{b.prim, b.P, base, s};
+  const unsigned s = (unsigned)dstride(b, d);
+  const unsigned P = (unsigned)b.P;
+  const LineRef L{b.prim, P, (unsigned)base, s};
   const int ist = (L.at(0, 0) < 1.0e-20) ? 2 : 0;
   const int ied = (L.at(0, n + 5) < 1.0e-20) ? n - 2 : n;
   double qwl[5], qwr[5], k3[3], fo[5];
   face_states<true>(L, n, ist, ied, f, qwl, qwr);
+  const unsigned g = (unsigned)base + (unsigned)(f + 2) * s;  // the face's slot
   {
-    const double* G = b.fgeo + (long long)(d * FG_N + 10) * b.P + base + (long long)(f + 2) * s;
-    k3[0] = G[0];
-    k3[1] = G[b.P];
-    k3[2] = G[2 * b.P];
+    const double* G = b.fgeo + (long long)(d * FG_N + 10) * b.P;  // this direction's inviscid metrics
+    k3[0] = ldo(G, boff(0, P, g));
+    k3[1] = ldo(G, boff(1, P, g));
+    k3[2] = ldo(G, boff(2, P, g));
   }
   face_flux(L, n, f, qwl, qwr, k3, gamma, fo);
-  double* F = b.fxi + (long long)d * 5 * b.P + base + (long long)(f + 2) * s;
+  double* F = b.fxi + (long long)d * 5 * b.P;
 #pragma unroll
-  for (int m = 0; m < 5; ++m) F[(long long)m * b.P] = fo[m];
+  for (int m = 0; m < 5; ++m) sto(F, boff((unsigned)m, P, g), fo[m]);
 }
 
 // Boundary recalculation (InviscidFluxComputer.cpp:377-521) at a line end
@@ -315,7 +319,7 @@ __global__ void __launch_bounds__(256) inv_closure_kernel(const BlockDev* __rest
   const long long base = decode_line(b, d, ln);
   const int n = dcount(b, d);
   const long long s = dstride(b, d);
-  const LineRef L{b.prim, b.P, base, s};
+  const LineRef L{b.prim, (unsigned)b.P, (unsigned)base, (unsigned)s};
   const int ist = (L.at(0, 0) < 1.0e-20) ? 2 : 0;
   const int ied = (L.at(0, n + 5) < 1.0e-20) ? n - 2 : n;
   const bool right = task >= 3;
@@ -729,25 +733,19 @@ __device__ __forceinline__ void visc_face_march(const BlockDev& b, long long lin
 #pragma unroll
   for (int j = 0; j < 4; ++j) visc_window_load<D>(b, base, s, n, f0 + 1 + j, W[j], M[j], X[j]);
   for (int f = f0; f < f1; ++f) {
-    // the next face's new stencil values are loaded before this face's
-    // flux is formed (and stored), so that their latency overlaps it
-    double Wn[4] = {0.0, 0.0, 0.0, 0.0}, Mn = 0.0, Xn[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    if (f + 1 < f1) visc_window_load<D>(b, base, s, n, f + 5, Wn, Mn, Xn);
     visc_face_at<D>(b, base, s, n, f, fl, fr, [&](int v, int j) { return W[j][v]; }, [&](int j) { return M[j]; },
                     [&](int c, int j) { return X[j][c]; });
+    if (f + 1 < f1) {
 #pragma unroll
-    for (int j = 0; j < 3; ++j) {
+      for (int j = 0; j < 3; ++j) {
 #pragma unroll
-      for (int v = 0; v < 4; ++v) W[j][v] = W[j + 1][v];
-      M[j] = M[j + 1];
+        for (int v = 0; v < 4; ++v) W[j][v] = W[j + 1][v];
+        M[j] = M[j + 1];
 #pragma unroll
-      for (int c = 0; c < 8; ++c) X[j][c] = X[j + 1][c];
+        for (int c = 0; c < 8; ++c) X[j][c] = X[j + 1][c];
+      }
+      visc_window_load<D>(b, base, s, n, f + 5, W[3], M[3], X[3]);
     }
-#pragma unroll
-    for (int v = 0; v < 4; ++v) W[3][v] = Wn[v];
-    M[3] = Mn;
-#pragma unroll
-    for (int c = 0; c < 8; ++c) X[3][c] = Xn[c];
   }
 }
 
@@ -759,7 +757,11 @@ __device__ __forceinline__ void visc_face_march(const BlockDev& b, long long lin
 //   d = 1, 2: 256 consecutive lines along j or k (lane = line, i fastest)
 //          over faces f0 .. f1-1, each lane marching along its line.
 // *head is zeroed by the derivative kernel that runs before this one.
-__global__ void __launch_bounds__(256) visc_face_queue_kernel(const BlockDev* __restrict__ blocks,
+// Held to two waves per SIMD (256 VGPRs): beside it, on the side stream,
+// the inviscid face kernel then gets more waves (measured faster per step
+// than the unbounded allocation, which leaves one wave of this kernel and
+// one of the other per SIMD).
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) visc_face_queue_kernel(const BlockDev* __restrict__ blocks,
                                                               const ViscItem* __restrict__ items, int nitems,
                                                               int* head) {
   __shared__ int s_item;
@@ -963,7 +965,8 @@ __global__ void __launch_bounds__(256) diff_kernel(const BlockDev* __restrict__ 
   if (!span_item(SP, &bi, &gi)) return;  // whole tiles: every lane of a workgroup has an item
   const BlockDev& b = blocks[bi];
   const double re_inv = b.re_inv;
-  const long long P = b.P;
+  // 32-bit offsets: the flux arrays hold 15 fields of P < 2^25 points
+  const unsigned P = (unsigned)b.P, nx = (unsigned)b.nx, nxy = (unsigned)b.nx * (unsigned)b.ny;
   const unsigned tci = (unsigned)(b.idim + DT - 1) / DT, tcj = (unsigned)(b.jdim + DT - 1) / DT;
   unsigned q, ti, q2, tj;
   udivmod((unsigned)(gi / 256), tci, &q, &ti);
@@ -972,39 +975,49 @@ __global__ void __launch_bounds__(256) diff_kernel(const BlockDev* __restrict__ 
   const int i0 = NG + DT * (int)ti, j0 = NG + DT * (int)tj;
   const int lane = threadIdx.x, li = lane % DT, lj = lane / DT;
   const int i = i0 + li, j = j0 + lj;
-  const double* __restrict__ FX = INV ? b.fxi : b.fx;
-  for (int e = lane; e < NM * DT * DW; e += 256) {
-    const int m = e / (DT * DW), r = e % (DT * DW);
-    {
-      const int row = r / DW, col = r % DW;
-      const int jj = j0 + row, ii = i0 - 3 + col;
-      const bool ok = jj < NG + b.jdim && ii < b.nx;
-      X[m][row][col] = ok ? FX[(long long)(M0 + m) * P + pidx(b, ii, jj, k)] : 0.0;
-    }
-    {
-      const int row = r / DT, col = r % DT;
-      const int ii = i0 + col, jj = j0 - 3 + row;
-      const bool ok = ii < NG + b.idim && jj < b.ny;
-      Y[m][row][col] = ok ? FX[(long long)(5 + M0 + m) * P + pidx(b, ii, jj, k)] : 0.0;
+  const double* FX = INV ? b.fxi : b.fx;
+  const unsigned kplane = (unsigned)k * nxy;
+  // each lane stages up to two (row, slot) positions of each direction, all
+  // components of a position from one offset
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int p = lane + 256 * h;
+    if (p < DT * DW) {
+      {
+        const int row = p / DW, col = p % DW;
+        const int jj = j0 + row, ii = i0 - 3 + col;
+        const bool ok = jj < NG + b.jdim && ii < b.nx;
+        const unsigned e = (unsigned)ii + (unsigned)jj * nx + kplane;
+#pragma unroll
+        for (int m = 0; m < NM; ++m) X[m][row][col] = ok ? ldo(FX, boff((unsigned)(M0 + m), P, e)) : 0.0;
+      }
+      {
+        const int row = p / DT, col = p % DT;
+        const int ii = i0 + col, jj = j0 - 3 + row;
+        const bool ok = ii < NG + b.idim && jj < b.ny;
+        const unsigned e = (unsigned)ii + (unsigned)jj * nx + kplane;
+#pragma unroll
+        for (int m = 0; m < NM; ++m) Y[m][row][col] = ok ? ldo(FX, boff((unsigned)(5 + M0 + m), P, e)) : 0.0;
+      }
     }
   }
   __syncthreads();
   if (!(i < NG + b.idim && j < NG + b.jdim)) return;
-  const long long id = pidx(b, i, j, k);
+  const unsigned id = (unsigned)i + (unsigned)j * nx + kplane;
   double acc[5];
 #pragma unroll
-  for (int m = 0; m < 5; ++m) acc[m] = FIRST ? 0.0 : (m < M0 ? 0.0 : b.dq[(long long)m * P + id]);
+  for (int m = 0; m < 5; ++m) acc[m] = FIRST ? 0.0 : (m < M0 ? 0.0 : ldo(b.dq, boff((unsigned)m, P, id)));
   const int pos[3] = {i, j, k};
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     const int n = dcount(b, d);
-    const long long s = dstride(b, d);
+    const unsigned s = d == 0 ? 1u : (d == 1 ? nx : nxy);
     const int t = pos[d] - NG;
-    const long long lbase = id - (long long)pos[d] * s;
+    const unsigned lbase = id - (unsigned)pos[d] * s;
     const double* cov = nullptr;
     if (INV) {
-      const bool lflag = b.prim[lbase] < 1.0e-20;
-      const bool rflag = b.prim[lbase + (long long)(n + 5) * s] < 1.0e-20;
+      const bool lflag = ldo(b.prim, lbase << 3) < 1.0e-20;
+      const bool rflag = ldo(b.prim, (lbase + (unsigned)(n + 5) * s) << 3) < 1.0e-20;
       if (lflag && t <= 2) cov = b.fcov + b.fcov_off[d] + line_id(b, d, i, j, k) * 30 + t * 5;
       else if (rflag && t >= n - 3) cov = b.fcov + b.fcov_off[d] + line_id(b, d, i, j, k) * 30 + (3 + t - (n - 3)) * 5;
     }
@@ -1016,8 +1029,8 @@ __global__ void __launch_bounds__(256) diff_kernel(const BlockDev* __restrict__ 
       } else if (d == 1) {
         fc = diff_node_f([&](int f) { return Y[m - M0][f + 2 - (j0 - 3)][li]; }, n, t);
       } else {
-        const double* F0 = FX + (long long)(10 + m) * P + lbase;
-        fc = diff_node_f([&](int f) { return F0[(long long)(f + 2) * s]; }, n, t);
+        fc = diff_node_f([&](int f) { return ldo(FX, boff((unsigned)(10 + m), P, lbase + (unsigned)(f + 2) * s)); },
+                         n, t);
       }
       if (INV) {
         // the difference is formed for every node and replaced by the
@@ -1033,7 +1046,7 @@ __global__ void __launch_bounds__(256) diff_kernel(const BlockDev* __restrict__ 
   // step's first kernel, otherwise left as it is
 #pragma unroll
   for (int m = 0; m < 5; ++m)
-    if (m >= M0 || FIRST) b.dq[(long long)m * P + id] = acc[m];
+    if (m >= M0 || FIRST) sto(b.dq, boff((unsigned)m, P, id), acc[m]);
 }
 
 // ---------------------------------------------------------------------------
