@@ -445,12 +445,12 @@ __device__ inline double uvwt_at(const BlockDev& b, long long id, int v) {
 __device__ inline double node_deriv(const double* q, long long s, int n, int i1, bool fl, bool fr) {
   // q points at line position 0; Q(p) = value at position p
   auto Q = [&](int p) { return q[(long long)p * s]; };
-  if (fl && i1 == 1) return (-11.0 * Q(3) + 18.0 * Q(4) - 9.0 * Q(5) + 2.0 * Q(6)) / 6.0;
-  if (fl && i1 == 2) return (-3.0 * Q(3) + -10.0 * Q(4) + 18.0 * Q(5) + -6.0 * Q(6) + 1.0 * Q(7)) / 12.0;
-  if (fr && i1 == n) return -(-11.0 * Q(n + 2) + 18.0 * Q(n + 1) - 9.0 * Q(n) + 2.0 * Q(n - 1)) / 6.0;
+  if (fl && i1 == 1) return div_c<6>((-11.0 * Q(3) + 18.0 * Q(4) - 9.0 * Q(5) + 2.0 * Q(6)));
+  if (fl && i1 == 2) return div_c<12>((-3.0 * Q(3) + -10.0 * Q(4) + 18.0 * Q(5) + -6.0 * Q(6) + 1.0 * Q(7)));
+  if (fr && i1 == n) return div_c<6>(-(-11.0 * Q(n + 2) + 18.0 * Q(n + 1) - 9.0 * Q(n) + 2.0 * Q(n - 1)));
   if (fr && i1 == n - 1)
-    return -(-3.0 * Q(n + 2) + -10.0 * Q(n + 1) + 18.0 * Q(n) + -6.0 * Q(n - 1) + 1.0 * Q(n - 2)) / 12.0;
-  return (8.0 * (Q(i1 + 3) - Q(i1 + 1)) + -1.0 * (Q(i1 + 4) - Q(i1))) / 12.0;
+    return div_c<12>(-(-3.0 * Q(n + 2) + -10.0 * Q(n + 1) + 18.0 * Q(n) + -6.0 * Q(n - 1) + 1.0 * Q(n - 2)));
+  return div_c<12>((8.0 * (Q(i1 + 3) - Q(i1 + 1)) + -1.0 * (Q(i1 + 4) - Q(i1))));
 }
 
 __global__ void __launch_bounds__(256) visc_deriv_kernel(const BlockDev* __restrict__ blocks,
@@ -501,12 +501,12 @@ __device__ inline double half_flagged(const double* q, long long s, int n, int f
 __device__ inline double half_deriv(const double* q, long long s, int n, int f, bool fl, bool fr) {
   auto Q = [&](int p) { return q[(long long)p * s]; };
   const double c1 = Q(f + 1), c2 = Q(f + 2), c3 = Q(f + 3), c4 = Q(f + 4);
-  if (fl && f == 0) return (-71.0 * Q(3) + 141.0 * Q(4) + -93.0 * Q(5) + 23.0 * Q(6)) / 24.0;
-  if (fl && f == 1) return (-22.0 * Q(3) + 17.0 * Q(4) + 9.0 * Q(5) + -5.0 * Q(6) + 1.0 * Q(7)) / 24.0;
-  if (fr && f == n) return -(-71.0 * Q(n + 2) + 141.0 * Q(n + 1) + -93.0 * Q(n) + 23.0 * Q(n - 1)) / 24.0;
+  if (fl && f == 0) return div_c<24>((-71.0 * Q(3) + 141.0 * Q(4) + -93.0 * Q(5) + 23.0 * Q(6)));
+  if (fl && f == 1) return div_c<24>((-22.0 * Q(3) + 17.0 * Q(4) + 9.0 * Q(5) + -5.0 * Q(6) + 1.0 * Q(7)));
+  if (fr && f == n) return div_c<24>(-(-71.0 * Q(n + 2) + 141.0 * Q(n + 1) + -93.0 * Q(n) + 23.0 * Q(n - 1)));
   if (fr && f == n - 1)
-    return -(-22.0 * Q(n + 2) + 17.0 * Q(n + 1) + 9.0 * Q(n) + -5.0 * Q(n - 1) + 1.0 * Q(n - 2)) / 24.0;
-  return (27.0 * (c3 - c2) + -1.0 * (c4 - c1)) / 24.0;
+    return div_c<24>(-(-22.0 * Q(n + 2) + 17.0 * Q(n + 1) + 9.0 * Q(n) + -5.0 * Q(n - 1) + 1.0 * Q(n - 2)));
+  return div_c<24>((27.0 * (c3 - c2) + -1.0 * (c4 - c1)));
 }
 
 // Face geometry (once, after the metrics): the central interpolation of
@@ -548,12 +548,12 @@ __device__ __forceinline__ double half_flagged_c(const QF& Q, int n, int f, bool
 template <class QF>
 __device__ __forceinline__ double half_deriv_c(const QF& Q, int n, int f, bool fl, bool fr, double c1, double c2,
                                                double c3, double c4) {
-  if (fl && f == 0) return (-71.0 * Q(3) + 141.0 * Q(4) + -93.0 * Q(5) + 23.0 * Q(6)) / 24.0;
-  if (fl && f == 1) return (-22.0 * Q(3) + 17.0 * Q(4) + 9.0 * Q(5) + -5.0 * Q(6) + 1.0 * Q(7)) / 24.0;
-  if (fr && f == n) return -(-71.0 * Q(n + 2) + 141.0 * Q(n + 1) + -93.0 * Q(n) + 23.0 * Q(n - 1)) / 24.0;
+  if (fl && f == 0) return div_c<24>((-71.0 * Q(3) + 141.0 * Q(4) + -93.0 * Q(5) + 23.0 * Q(6)));
+  if (fl && f == 1) return div_c<24>((-22.0 * Q(3) + 17.0 * Q(4) + 9.0 * Q(5) + -5.0 * Q(6) + 1.0 * Q(7)));
+  if (fr && f == n) return div_c<24>(-(-71.0 * Q(n + 2) + 141.0 * Q(n + 1) + -93.0 * Q(n) + 23.0 * Q(n - 1)));
   if (fr && f == n - 1)
-    return -(-22.0 * Q(n + 2) + 17.0 * Q(n + 1) + 9.0 * Q(n) + -5.0 * Q(n - 1) + 1.0 * Q(n - 2)) / 24.0;
-  return (27.0 * (c3 - c2) + -1.0 * (c4 - c1)) / 24.0;
+    return div_c<24>(-(-22.0 * Q(n + 2) + 17.0 * Q(n + 1) + 9.0 * Q(n) + -5.0 * Q(n - 1) + 1.0 * Q(n - 2)));
+  return div_c<24>((27.0 * (c3 - c2) + -1.0 * (c4 - c1)));
 }
 
 // prim field of viscous variable v (u, v, w, T = prim 1, 2, 3, 5)
@@ -789,11 +789,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) v
 template <class FF>
 __device__ __forceinline__ double diff_node_f(const FF& F, int n, int t) {
   if (t >= 2 && t < n - 2)
-    return (2250.0 * (F(t + 1) - F(t)) + -125.0 * (F(t + 2) - F(t - 1)) + 9.0 * (F(t + 3) - F(t - 2))) / 1920.0;
-  if (t == 1) return (F(0) - 27.0 * F(1) + 27.0 * F(2) - F(3)) / 24.0;
-  if (t == n - 2) return -(F(n) - 27.0 * F(n - 1) + 27.0 * F(n - 2) - F(n - 3)) / 24.0;
-  if (t == 0) return (-23.0 * F(0) + 21.0 * F(1) + 3.0 * F(2) - F(3)) / 24.0;
-  return -(-23.0 * F(n) + 21.0 * F(n - 1) + 3.0 * F(n - 2) - F(n - 3)) / 24.0;
+    return div_c<1920>((2250.0 * (F(t + 1) - F(t)) + -125.0 * (F(t + 2) - F(t - 1)) + 9.0 * (F(t + 3) - F(t - 2))));
+  if (t == 1) return div_c<24>((F(0) - 27.0 * F(1) + 27.0 * F(2) - F(3)));
+  if (t == n - 2) return div_c<24>(-(F(n) - 27.0 * F(n - 1) + 27.0 * F(n - 2) - F(n - 3)));
+  if (t == 0) return div_c<24>((-23.0 * F(0) + 21.0 * F(1) + 3.0 * F(2) - F(3)));
+  return div_c<24>(-(-23.0 * F(n) + 21.0 * F(n - 1) + 3.0 * F(n - 2) - F(n - 3)));
 }
 
 // The difference of diff_node_f at node t over a window W(w) of the six
@@ -804,11 +804,11 @@ __device__ __forceinline__ double diff_node_f(const FF& F, int n, int t) {
 template <class WF>
 __device__ __forceinline__ double diff_node_w(const WF& W, int n, int t) {
   if (t >= 2 && t < n - 2)
-    return (2250.0 * (W(3) - W(2)) + -125.0 * (W(4) - W(1)) + 9.0 * (W(5) - W(0))) / 1920.0;
-  if (t == 1) return (W(1) - 27.0 * W(2) + 27.0 * W(3) - W(4)) / 24.0;
-  if (t == n - 2) return -(W(4) - 27.0 * W(3) + 27.0 * W(2) - W(1)) / 24.0;
-  if (t == 0) return (-23.0 * W(2) + 21.0 * W(3) + 3.0 * W(4) - W(5)) / 24.0;
-  return -(-23.0 * W(3) + 21.0 * W(2) + 3.0 * W(1) - W(0)) / 24.0;
+    return div_c<1920>((2250.0 * (W(3) - W(2)) + -125.0 * (W(4) - W(1)) + 9.0 * (W(5) - W(0))));
+  if (t == 1) return div_c<24>((W(1) - 27.0 * W(2) + 27.0 * W(3) - W(4)));
+  if (t == n - 2) return div_c<24>(-(W(4) - 27.0 * W(3) + 27.0 * W(2) - W(1)));
+  if (t == 0) return div_c<24>((-23.0 * W(2) + 21.0 * W(3) + 3.0 * W(4) - W(5)));
+  return div_c<24>(-(-23.0 * W(3) + 21.0 * W(2) + 3.0 * W(1) - W(0)));
 }
 
 // first and last face that diff_node_f reads for node t of a line of n >= 4
