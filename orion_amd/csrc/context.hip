@@ -1468,7 +1468,7 @@ static void op_avg(orion_hip_ctx* c) {
 // Inviscid face fluxes and boundary closures: they read the flow state
 // and the static face geometry only, and write fxi and fcov.
 static void op_inv_faces(orion_hip_ctx* c, hipStream_t st) {
-  launch_inv_face(LA3(c, c->face3, st), -1, c->face3.total(), c->p.gama);
+  launch_inv_face_pairs(LA3(c, c->fpair3, st), c->fpair3.total(), c->p.gama);
   launch_inv_closure(LA3(c, c->ltask3, st), -1, c->ltask3.total(), c->p.gama);
 }
 
@@ -1794,7 +1794,7 @@ int orion_hip_profile_run(orion_hip_ctx* c, int nsteps, double* phase_ms, long l
     } else {
       mark(0); mark(0); mark(0); mark(0);
     }
-    launch_inv_face(LA3(c, c->face3), -1, c->face3.total(), c->p.gama);
+    launch_inv_face_pairs(LA3(c, c->fpair3), c->fpair3.total(), c->p.gama);
     mark(1);
     launch_inv_closure(LA3(c, c->ltask3), -1, c->ltask3.total(), c->p.gama);
     mark(1);

@@ -188,12 +188,9 @@ struct LineRef {
 // loads then need not wait for the boundary flags -- and the faces outside
 // [ist, ied] (f < 2 or f > n - 2) have both states replaced below, exactly
 // as the reference leaves them to the fixed stencils.
-template <bool SHARED>
-__device__ inline void face_states(const LineRef& L, int n, int ist, int ied, int f, double* qwl, double* qwr) {
-#pragma unroll
-  for (int m = 0; m < 5; ++m)
-    weno5<SHARED>(L.at(m, f), L.at(m, f + 1), L.at(m, f + 2), L.at(m, f + 3), L.at(m, f + 4), L.at(m, f + 5), &qwl[m],
-          &qwr[m]);
+// The fixed boundary stencils' states of face f where they replace the WCNS
+// states (face_states after its WCNS pass).
+__device__ inline void boundary_states(const LineRef& L, int n, int ist, int ied, int f, double* qwl, double* qwr) {
   if (ist > 1 && f <= 2) {
 #pragma unroll
     for (int m = 0; m < 5; ++m) {
@@ -213,6 +210,73 @@ __device__ inline void face_states(const LineRef& L, int n, int ist, int ied, in
       else qwr[m] = (-qn + 9. * qn1 + 9. * qn2 - qn3) / 16.;
       if (f >= n - 1) qwl[m] = qwr[m];
     }
+  }
+}
+
+template <bool SHARED>
+__device__ inline void face_states(const LineRef& L, int n, int ist, int ied, int f, double* qwl, double* qwr) {
+#pragma unroll
+  for (int m = 0; m < 5; ++m)
+    weno5<SHARED>(L.at(m, f), L.at(m, f + 1), L.at(m, f + 2), L.at(m, f + 3), L.at(m, f + 4), L.at(m, f + 5), &qwl[m],
+          &qwr[m]);
+  boundary_states(L, n, ist, ied, f, qwl, qwr);
+}
+
+// weno5 of two neighbouring faces of a line, face 0 from v[0..5] and face 1
+// from v[1..6].  Face 1's left state is built from the three stencils of
+// face 0's right state: its s, g and smoothness indicators are face 0's
+// r1s, r2s, r3s, h1, h2, h3 and js1..3, formed by the same expressions from
+// the same values, so they are computed once; its weight of the middle
+// stencil (CL2 / (EPS + js2)^2) is face 0's right weight v2, and its other
+// two weights divide by face 0's right denominators, whose reciprocals
+// (SharedDiv) serve both quotients.  When either face has an indicator of
+// 1e100 or more, each face is formed by weno5 alone.  Either way every
+// state is the one weno5 forms for its face, bit for bit.
+__device__ inline void weno5_pair(const double* v, double* ql0, double* qr0, double* ql1, double* qr1) {
+  const double CL1 = 1.0 / 16.0, CL2 = 10.0 / 16.0, CL3 = 5.0 / 16.0, EPS = 1.0e-6;
+  const double s1 = v[0] - 2.0 * v[1] + v[2];
+  const double s2 = v[1] - 2.0 * v[2] + v[3];
+  const double s3 = v[2] - 2.0 * v[3] + v[4];
+  const double g1 = 0.5 * (v[0] - 4.0 * v[1] + 3.0 * v[2]);
+  const double g2 = 0.5 * (v[3] - v[1]);
+  const double g3 = 0.5 * (-3.0 * v[2] + 4.0 * v[3] - v[4]);
+  const double is1 = g1 * g1 + s1 * s1, is2 = g2 * g2 + s2 * s2, is3 = g3 * g3 + s3 * s3;
+  const double r3s = v[3] - 2.0 * v[4] + v[5];
+  const double h1 = 0.5 * (v[1] - 4.0 * v[2] + 3.0 * v[3]);
+  const double h2 = 0.5 * (v[4] - v[2]);
+  const double h3 = 0.5 * (-3.0 * v[3] + 4.0 * v[4] - v[5]);
+  const double js1 = h1 * h1 + s2 * s2, js2 = h2 * h2 + s3 * s3, js3 = h3 * h3 + r3s * r3s;
+  // face 1's right state
+  const double r3t = v[4] - 2.0 * v[5] + v[6];
+  const double k1 = 0.5 * (v[2] - 4.0 * v[3] + 3.0 * v[4]);
+  const double k2 = 0.5 * (v[5] - v[3]);
+  const double k3 = 0.5 * (-3.0 * v[4] + 4.0 * v[5] - v[6]);
+  const double ks1 = k1 * k1 + s3 * s3, ks2 = k2 * k2 + r3s * r3s, ks3 = k3 * k3 + r3t * r3t;
+  const double m0 = fmax(fmax(fmax(is1, is2), fmax(is3, js1)), fmax(js2, js3));
+  const double m1 = fmax(fmax(fmax(js1, js2), fmax(js3, ks1)), fmax(ks2, ks3));
+  if (m0 < 1.0e100 && m1 < 1.0e100) {
+    const double w1 = SharedDiv((EPS + is1) * (EPS + is1))(CL1);
+    const double w2 = SharedDiv((EPS + is2) * (EPS + is2))(CL2);
+    const double w3 = SharedDiv((EPS + is3) * (EPS + is3))(CL3);
+    const SharedDiv sw(w1 + w2 + w3);
+    *ql0 = v[2] + 0.125 * (sw(w1) * (s1 + 4.0 * g1) + sw(w2) * (s2 + 4.0 * g2) + sw(w3) * (s3 + 4.0 * g3));
+    const SharedDiv dj1((EPS + js1) * (EPS + js1)), dj3((EPS + js3) * (EPS + js3));
+    const double v1 = dj1(CL3);
+    const double v2 = SharedDiv((EPS + js2) * (EPS + js2))(CL2);
+    const double v3 = dj3(CL1);
+    const SharedDiv sv(v1 + v2 + v3);
+    *qr0 = v[3] + 0.125 * (sv(v1) * (s2 - 4.0 * h1) + sv(v2) * (s3 - 4.0 * h2) + sv(v3) * (r3s - 4.0 * h3));
+    const double u1 = dj1(CL1), u3 = dj3(CL3);
+    const SharedDiv su(u1 + v2 + u3);
+    *ql1 = v[3] + 0.125 * (su(u1) * (s2 + 4.0 * h1) + su(v2) * (s3 + 4.0 * h2) + su(u3) * (r3s + 4.0 * h3));
+    const double x1 = SharedDiv((EPS + ks1) * (EPS + ks1))(CL3);
+    const double x2 = SharedDiv((EPS + ks2) * (EPS + ks2))(CL2);
+    const double x3 = SharedDiv((EPS + ks3) * (EPS + ks3))(CL1);
+    const SharedDiv sx(x1 + x2 + x3);
+    *qr1 = v[4] + 0.125 * (sx(x1) * (s3 - 4.0 * k1) + sx(x2) * (r3s - 4.0 * k2) + sx(x3) * (r3t - 4.0 * k3));
+  } else {
+    weno5<true>(v[0], v[1], v[2], v[3], v[4], v[5], ql0, qr0);
+    weno5<true>(v[1], v[2], v[3], v[4], v[5], v[6], ql1, qr1);
   }
 }
 
@@ -290,6 +354,67 @@ __global__ void __launch_bounds__(256) inv_face_kernel(const BlockDev* __restric
   double* F = b.fxi + (long long)d * 5 * b.P;
 #pragma unroll
   for (int m = 0; m < 5; ++m) sto(F, boff((unsigned)m, P, g), fo[m]);
+}
+
+// The inviscid face fluxes of inv_face_kernel, faces f0 = 2g and f0 + 1 of
+// a line per thread (the face-pair space), the two faces' WCNS states formed
+// together (weno5_pair).  The pair's stencils cover positions f0 .. f0+6;
+// for a last pair with one face (f0 = n) position f0+6 is read at n+5
+// instead, so face 1's states are not those of any face and are neither
+// differenced into a flux nor stored.
+__global__ void __launch_bounds__(256) inv_face_pair_kernel(const BlockDev* __restrict__ blocks, Span SP,
+                                                            double gamma) {
+  int bi;
+  long long gi;
+  if (!span_item(SP, &bi, &gi)) return;
+  int d;
+  {  // the three directions in one launch: virtual block d * nb + b
+    const int nbr = SP.nb / 3;
+    d = bi / nbr;
+    bi -= d * nbr;
+  }
+  const BlockDev& b = blocks[bi];
+  long long base;
+  int f0;
+  decode_face_pair(b, d, gi, &base, &f0);
+  const int n = dcount(b, d);
+  const unsigned s = (unsigned)dstride(b, d);
+  const unsigned P = (unsigned)b.P;
+  const LineRef L{b.prim, P, (unsigned)base, s};
+  const int ist = (L.at(0, 0) < 1.0e-20) ? 2 : 0;
+  const int ied = (L.at(0, n + 5) < 1.0e-20) ? n - 2 : n;
+  const bool two = f0 + 1 <= n;
+  const int p6 = two ? f0 + 6 : n + 5;
+  double ql0[5], qr0[5], ql1[5], qr1[5];
+#pragma unroll
+  for (int m = 0; m < 5; ++m) {
+    double v[7];
+#pragma unroll
+    for (int j = 0; j < 6; ++j) v[j] = L.at(m, f0 + j);
+    v[6] = L.at(m, p6);
+    weno5_pair(v, &ql0[m], &qr0[m], &ql1[m], &qr1[m]);
+  }
+  const double* G = b.fgeo + (long long)(d * FG_N + 10) * b.P;  // this direction's inviscid metrics
+  double* F = b.fxi + (long long)d * 5 * b.P;
+  {
+    boundary_states(L, n, ist, ied, f0, ql0, qr0);
+    const unsigned g = (unsigned)base + (unsigned)(f0 + 2) * s;
+    const double k3[3] = {ldo(G, boff(0, P, g)), ldo(G, boff(1, P, g)), ldo(G, boff(2, P, g))};
+    double fo[5];
+    face_flux(L, n, f0, ql0, qr0, k3, gamma, fo);
+#pragma unroll
+    for (int m = 0; m < 5; ++m) sto(F, boff((unsigned)m, P, g), fo[m]);
+  }
+  if (two) {
+    const int f1 = f0 + 1;
+    boundary_states(L, n, ist, ied, f1, ql1, qr1);
+    const unsigned g = (unsigned)base + (unsigned)(f1 + 2) * s;
+    const double k3[3] = {ldo(G, boff(0, P, g)), ldo(G, boff(1, P, g)), ldo(G, boff(2, P, g))};
+    double fo[5];
+    face_flux(L, n, f1, ql1, qr1, k3, gamma, fo);
+#pragma unroll
+    for (int m = 0; m < 5; ++m) sto(F, boff((unsigned)m, P, g), fo[m]);
+  }
 }
 
 // Boundary recalculation (InviscidFluxComputer.cpp:377-521) at a line end
@@ -1057,6 +1182,11 @@ static inline unsigned grid_of(long long n, int tb) { return (unsigned)((n + tb 
 void launch_inv_face(const Launch& L, int d, long long n, double gamma) {
   if (n <= 0) return;
   hipLaunchKernelGGL(inv_face_kernel, dim3((unsigned)L.nwg), dim3(SPAN_TB), 0, L.stream, L.blocks, L.span, d, gamma);
+}
+void launch_inv_face_pairs(const Launch& L, long long n, double gamma) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(inv_face_pair_kernel, dim3((unsigned)L.nwg), dim3(SPAN_TB), 0, L.stream, L.blocks, L.span,
+                     gamma);
 }
 void launch_inv_closure(const Launch& L, int d, long long n, double gamma) {
   if (n <= 0) return;

@@ -40,6 +40,9 @@ struct AvgDst {
 // face kernels: d = 0, 1, 2 for one direction's space; d = -1 for a space of
 // 3 nb virtual blocks d * nb + b holding the three directions' spaces
 void launch_inv_face(const Launch& L, int d, long long n, double gamma);
+// the same fluxes, two neighbouring faces of a line per thread, over the
+// three directions' face-pair spaces as 3 nb virtual blocks
+void launch_inv_face_pairs(const Launch& L, long long n, double gamma);
 void launch_inv_closure(const Launch& L, int d, long long n, double gamma);
 // first: the step's first right-hand-side kernel, starting from dq = 0
 // without reading it (the step's state kernel then skips the zeroing)
