@@ -51,7 +51,7 @@ HBM_PEAK_GBS = 8000.0
 # kernels that implement it and the profiled phases they run in.
 STAGES = {
     "rhs": (20, ["visc_deriv", "visc_face", "visc_diff", "inv_face", "inv_closure", "inv_diff"],
-            "visc_deriv_kernel+visc_face_queue_kernel+diff_kernel<false,true>+inv_face_kernel+inv_closure_kernel"
+            "visc_deriv_kernel+visc_face_queue_kernel+diff_kernel<false,true>+inv_face_pair_kernel+inv_closure_kernel"
             "+diff_kernel<true,false>"),
     # a layout copy in two parts: no algorithmic words of its own
     "lu_pack": (0, ["lu_pack_state", "lu_pack"], "lu_pack_kernel<1>+lu_pack_kernel<2>"),
@@ -85,7 +85,7 @@ def use_sweep_kernels(chained):
 def use_visc_kernels(fused):
     """Name the viscous kernels of the form in use in the stage table."""
     faces = "visc_fused_kernel<true>" if fused else "visc_face_queue_kernel+diff_kernel<false,true>"
-    STAGES["rhs"] = STAGES["rhs"][:2] + ("visc_deriv_kernel+" + faces + "+inv_face_kernel+inv_closure_kernel"
+    STAGES["rhs"] = STAGES["rhs"][:2] + ("visc_deriv_kernel+" + faces + "+inv_face_pair_kernel+inv_closure_kernel"
                                          "+diff_kernel<true,false>",)
 
 
