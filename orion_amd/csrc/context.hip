@@ -8,6 +8,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
+#include <functional>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -20,6 +22,7 @@
 #include <mutex>
 #include <sstream>
 #include <string>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
@@ -379,7 +382,35 @@ static long long arena_fields(const BlockDev& b, int keep_q, int viscous, long l
   return n;
 }
 
-static int build_topology(orion_hip_ctx* c, const std::vector<std::vector<Reg>>& R) {
+// Runs fn(b, scratch) for b = 0 .. n-1 on up to 16 host threads (blocks
+// taken in turn from a shared counter); scratch is a per-thread vector that
+// fn may size and reuse.  Returns the error message fn gave for the lowest
+// block that gave one, or an empty string.
+template <class F>
+static std::string parallel_blocks(int n, const F& fn) {
+  const int hw = (int)std::thread::hardware_concurrency();
+  const int nt = std::max(1, std::min(std::min(n, 16), hw > 0 ? hw : 1));
+  std::atomic<int> next(0);
+  std::vector<std::string> err((size_t)std::max(n, 0));
+  auto work = [&]() {
+    std::vector<long long> scratch;
+    for (;;) {
+      const int b = next.fetch_add(1);
+      if (b >= n) break;
+      err[(size_t)b] = fn(b, scratch);
+    }
+  };
+  std::vector<std::thread> th;
+  for (int t = 1; t < nt; ++t) th.emplace_back(work);
+  work();
+  for (auto& t : th) t.join();
+  for (auto& e : err)
+    if (!e.empty()) return e;
+  return std::string();
+}
+
+static int build_topology(orion_hip_ctx* c, const std::vector<std::vector<Reg>>& R,
+                          const std::function<void(const char*)>& timing) {
   const int nb = c->nb;
   // ---------------- halo gather list ----------------
   struct Msg {
@@ -403,13 +434,13 @@ static int build_topology(orion_hip_ctx* c, const std::vector<std::vector<Reg>>&
   auto arena_off = [&](int b, int m, long long id) {
     return (long long)((c->hb[b].prim - (double*)c->arena) + (long long)m * c->hb[b].P + id);
   };
-  long long maxP = 1;
-  for (int b = 0; b < nb; ++b) maxP = std::max(maxP, c->hb[b].P);
-  std::vector<long long> where((size_t)(5 * maxP), -1);  // block-local destination -> its entry in pairs
-  std::vector<long long> touched;
-  std::vector<long long> pairs;
-  for (int b = 0; b < nb; ++b) {
-    touched.clear();
+  // the partner message of every receiving window, matched in the order
+  // the reference receives (receiving blocks, then their windows)
+  struct Link {
+    int reg, sblock, sreg;
+  };
+  std::vector<std::vector<Link>> links(nb);
+  for (int b = 0; b < nb; ++b)
     for (int r = 0; r < (int)R[b].size(); ++r) {
       const Reg& rg = R[b][r];
       if (rg.d.nbt <= 0) continue;
@@ -421,10 +452,21 @@ static int build_topology(orion_hip_ctx* c, const std::vector<std::vector<Reg>>&
           if (!msgs[q].used) { M = &msgs[q]; break; }
       if (!M) return fail("halo: no partner message for block %d window %d", b + 1, r + 1);
       M->used = true;
-      const BlockDev& A = c->hb[M->block];
-      const BlockDev& B = c->hb[b];
+      links[b].push_back({r, M->block, M->reg});
+    }
+  // each receiving block's writes, reduced to its last writers, on host
+  // threads (a block's destinations are its own, so blocks are independent)
+  std::vector<std::vector<long long>> bpairs(nb);
+  parallel_blocks(nb, [&](int b, std::vector<long long>& where) {
+    const BlockDev& B = c->hb[b];
+    if ((long long)where.size() < 5 * B.P) where.assign((size_t)(5 * B.P), -1);
+    std::vector<long long>& pairs = bpairs[b];
+    std::vector<long long> touched;
+    for (const Link& L : links[b]) {
+      const Reg& rg = R[b][L.reg];
+      const BlockDev& A = c->hb[L.sblock];
       int sb[3], se[3], rb[3], re[3];
-      memory_window(R[M->block][M->reg], A, 0, sb, se);
+      memory_window(R[L.sblock][L.sreg], A, 0, sb, se);
       memory_window(rg, B, 1, rb, re);
       const long long npts = (long long)(se[0] - sb[0] + 1) * (se[1] - sb[1] + 1) * (se[2] - sb[2] + 1);
       const long long count = 5 * npts;
@@ -439,7 +481,7 @@ static int build_topology(orion_hip_ctx* c, const std::vector<std::vector<Reg>>&
               if (ptr >= count) continue;
               ++ptr;
               const long long local = (long long)m * B.P + pidx(B, i, j, k);
-              const long long src = arena_off(M->block, sm, pidx(A, si, sj, sk));
+              const long long src = arena_off(L.sblock, sm, pidx(A, si, sj, sk));
               if (++si > se[0]) {
                 si = sb[0];
                 if (++sj > se[1]) {
@@ -462,9 +504,23 @@ static int build_topology(orion_hip_ctx* c, const std::vector<std::vector<Reg>>&
             }
     }
     for (long long t : touched) where[(size_t)t] = -1;
+    return std::string();
+  });
+  {  // the blocks' lists one after another, copied from where they were built
+    size_t tot = 0;
+    for (auto& v : bpairs) tot += v.size();
+    c->n_halo = (long long)(tot / 2);
+    if (dalloc(c, (void**)&c->d_halo, sizeof(long long) * std::max<size_t>(tot, 1))) return -1;
+    size_t off = 0;
+    for (auto& v : bpairs) {
+      if (!v.empty())
+        HIPCHK(hipMemcpyAsync(c->d_halo + off, v.data(), sizeof(long long) * v.size(), hipMemcpyHostToDevice,
+                              c->stream));
+      off += v.size();
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
   }
-  c->n_halo = (long long)(pairs.size() / 2);
-  if (upload_vec(c, &c->d_halo, pairs)) return -1;
+  timing("  halo gather list");
 
   // ---------------- physical BC tasks per rank ----------------
   int maxrank = 0;
@@ -503,6 +559,7 @@ static int build_topology(orion_hip_ctx* c, const std::vector<std::vector<Reg>>&
     c->bc_total.push_back(tot);
   }
   if (upload_vec(c, &c->d_bc, tasks)) return -1;
+  timing("  BC tasks");
 
   // ---------------- interface averaging maps ----------------
   // Every face node of every interface window W (block b) takes the dq/vol
@@ -523,14 +580,18 @@ static int build_topology(orion_hip_ctx* c, const std::vector<std::vector<Reg>>&
     int a;
     long long sidx;
   };
-  std::vector<AE> ae;
-  long long seq = 0;
-  for (int b = 0; b < nb; ++b)
+  // each block's entries (in its windows' order, so seq orders them as one
+  // global count would), sorted by (node, region order) on host threads
+  std::vector<std::vector<AE>> bae(nb);
+  const std::string aerr = parallel_blocks(nb, [&](int b, std::vector<long long>&) -> std::string {
+    std::vector<AE>& ae = bae[b];
+    long long seq = 0;
     for (int w = 0; w < (int)R[b].size(); ++w) {
       const Reg& W = R[b][w];
       if (W.d.bctype >= 0) continue;
       auto it = fill_of.find({b, w});
-      if (it == fill_of.end()) return fail("interface buffer never filled for block %d window %d", b + 1, w + 1);
+      if (it == fill_of.end())
+        return "interface buffer never filled for block " + std::to_string(b + 1) + " window " + std::to_string(w + 1);
       const int a = it->second.first;
       const Reg& X = R[a][it->second.second];
       int xlo[3], xn[3], lo[3], n[3], tlo[3], tn[3];
@@ -555,16 +616,30 @@ static int build_topology(orion_hip_ctx* c, const std::vector<std::vector<Reg>>&
             const int ir = W.image[f] - tlo[0], jr = W.jmage[f] - tlo[1], kr = W.kmage[f] - tlo[2];
             const long long rem = (long long)kr * (tn[0] * tn[1]) + (long long)jr * tn[0] + ir;
             if (rem < 0 || rem >= xcount)
-              return fail("interface map of block %d window %d leaves its partner window", b + 1, w + 1);
+              return "interface map of block " + std::to_string(b + 1) + " window " + std::to_string(w + 1) +
+                     " leaves its partner window";
             const int xi = (int)(rem % xn[0]), xj = (int)((rem / xn[0]) % xn[1]), xk = (int)(rem / ((long long)xn[0] * xn[1]));
             const long long sidx = pidx(c->hb[a], xlo[0] - 1 + NG + xi, xlo[1] - 1 + NG + xj, xlo[2] - 1 + NG + xk);
             const long long didx = pidx(c->hb[b], lo[0] - 1 + NG + i, lo[1] - 1 + NG + j, lo[2] - 1 + NG + k);
             ae.push_back({b, didx, seq++, a, sidx});
           }
     }
-  std::sort(ae.begin(), ae.end(), [](const AE& x, const AE& y) {
-    return x.b < y.b || (x.b == y.b && (x.didx < y.didx || (x.didx == y.didx && x.seq < y.seq)));
+    std::sort(ae.begin(), ae.end(), [](const AE& x, const AE& y) {
+      return x.didx < y.didx || (x.didx == y.didx && x.seq < y.seq);
+    });
+    return std::string();
   });
+  if (!aerr.empty()) return fail("%s", aerr.c_str());
+  std::vector<AE> ae;
+  {
+    size_t tot = 0;
+    for (auto& v : bae) tot += v.size();
+    ae.reserve(tot);
+    for (auto& v : bae) {
+      ae.insert(ae.end(), v.begin(), v.end());
+      std::vector<AE>().swap(v);
+    }
+  }
   std::vector<AvgSrc> src;
   std::vector<AvgDst> dst;
   src.reserve(ae.size());
@@ -585,6 +660,7 @@ static int build_topology(orion_hip_ctx* c, const std::vector<std::vector<Reg>>&
   if (upload_vec(c, &c->d_avg_src, src)) return -1;
   if (upload_vec(c, &c->d_avg_dst, dst)) return -1;
   if (dalloc(c, (void**)&c->d_stage, sizeof(double) * 5 * std::max<long long>(1, c->n_avg_src))) return -1;
+  timing("  interface averaging lists");
 
   // ---------------- frozen masks (set_boundary_dq_zero windows) ----------------
   for (int b = 0; b < nb; ++b) {
@@ -1263,7 +1339,8 @@ static int create_ctx(int nreq, const int* req_nblocks, int nblocks, const int* 
     c->visc_fused = false;  // the face-flux queue measured faster; the tuner tries both
     if (dalloc(c, (void**)&c->d_vhead, sizeof(int)) || set_visc_queue(c, c->visc_seg, c->visc_grid)) return bail(-1);
   }
-  if (build_topology(c, R)) return bail(-1);
+  timing("block table, viscous work queue");
+  if (build_topology(c, R, timing)) return bail(-1);
   timing("boundary topology (halo, BC, averaging, masks)");
   c->nparts = residual_parts(c->intr.total());
   if (c->ptile.nwg >= (1LL << 31)) return bail(fail("too many update workgroups"));
