@@ -102,6 +102,13 @@ int orion_hip_ctx_create(int nblocks, const int* dims, const double* xyz, int nr
 int orion_hip_batch_create(int nreq, const int* req_nblocks, int nblocks, const int* dims, const double* xyz,
                            int nregions, const orion_region_desc* regions, const orion_inputs* inputs, int keep_q,
                            orion_hip_ctx** out);
+/* No counterpart in the reference: orion_hip_batch_create with each
+ * request's grid where the caller holds it (req_xyz[r]: request r's blocks'
+ * coordinates one after another), so that a serving layer rebuilding a
+ * batch of resident requests does not gather their grids into one array. */
+int orion_hip_batch_create_grids(int nreq, const int* req_nblocks, int nblocks, const int* dims,
+                                 const double* const* req_xyz, int nregions, const orion_region_desc* regions,
+                                 const orion_inputs* inputs, int keep_q, orion_hip_ctx** out);
 /* Requests in the context (1 unless batched). */
 int orion_hip_requests(const orion_hip_ctx* ctx);
 void orion_hip_ctx_destroy(orion_hip_ctx* ctx);

@@ -80,6 +80,9 @@ def _open():
                                   C.POINTER(RegionDesc), C.POINTER(Inputs), C.c_int, C.POINTER(P)], C.c_int),
         "orion_hip_batch_create": ([C.c_int, C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double),
                                    C.c_int, C.POINTER(RegionDesc), C.POINTER(Inputs), C.c_int, C.POINTER(P)], C.c_int),
+        "orion_hip_batch_create_grids": ([C.c_int, C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int),
+                                          C.POINTER(C.POINTER(C.c_double)), C.c_int, C.POINTER(RegionDesc),
+                                          C.POINTER(Inputs), C.c_int, C.POINTER(P)], C.c_int),
         "orion_hip_requests": ([P], C.c_int),
         "orion_hip_ctx_destroy": ([P], None),
         "orion_hip_field": ([P, C.c_int, C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_longlong)], C.c_int),

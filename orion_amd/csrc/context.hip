@@ -1098,9 +1098,11 @@ static int check_requests(int nreq, const int* req_nblocks, int nblocks, int nre
   return 0;
 }
 
+// xyz: every block's coordinates one after another; or, with req_xyz,
+// request r's blocks' coordinates one after another at req_xyz[r]
 static int create_ctx(int nreq, const int* req_nblocks, int nblocks, const int* dims, const double* xyz,
-                      int nregions, const orion_region_desc* regions, const orion_inputs* in, int keep_q,
-                      orion_hip_ctx** out) {
+                      const double* const* req_xyz, int nregions, const orion_region_desc* regions,
+                      const orion_inputs* in, int keep_q, orion_hip_ctx** out) {
   // ORION_HIP_TIMING=1: the set-up's phases on stderr (each waits for the
   // device work queued before it)
   static const bool timed = std::getenv("ORION_HIP_TIMING") && std::atoi(std::getenv("ORION_HIP_TIMING")) != 0;
@@ -1385,10 +1387,21 @@ static int create_ctx(int nreq, const int* req_nblocks, int nblocks, const int* 
     // each block's coordinates as given (no padding) to a staging part of
     // its face-flux scratch, then into the padded layout on the device
     long long off = 0;
+    int r = 0, rb = 0;  // the block's request and its place in it
     for (int b = 0; b < nblocks; ++b) {
       const BlockDev& B = c->hb[b];
       const long long n = (long long)B.idim * B.jdim * B.kdim;
-      if (hipMemcpyAsync(B.fx + 3 * B.P, xyz + off, sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->stream) !=
+      if (req_xyz) {
+        while (rb == req_nblocks[r]) {
+          ++r;
+          rb = 0;
+          off = 0;
+        }
+        ++rb;
+      }
+      const double* src = req_xyz ? req_xyz[r] + off : xyz + off;
+      if (!src) return bail(fail("null grid for request %d", r));
+      if (hipMemcpyAsync(B.fx + 3 * B.P, src, sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->stream) !=
           hipSuccess)
         return bail(fail("grid upload failed"));
       off += 3 * n;
@@ -1416,14 +1429,21 @@ static int create_ctx(int nreq, const int* req_nblocks, int nblocks, const int* 
 int orion_hip_ctx_create(int nblocks, const int* dims, const double* xyz, int nregions,
                          const orion_region_desc* regions, const orion_inputs* in, int keep_q,
                          orion_hip_ctx** out) {
-  return create_ctx(1, &nblocks, nblocks, dims, xyz, nregions, regions, in, keep_q, out);
+  return create_ctx(1, &nblocks, nblocks, dims, xyz, nullptr, nregions, regions, in, keep_q, out);
 }
 
 int orion_hip_batch_create(int nreq, const int* req_nblocks, int nblocks, const int* dims, const double* xyz,
                            int nregions, const orion_region_desc* regions, const orion_inputs* inputs, int keep_q,
                            orion_hip_ctx** out) {
   if (!out) return fail("batch_create: null output");
-  return create_ctx(nreq, req_nblocks, nblocks, dims, xyz, nregions, regions, inputs, keep_q, out);
+  return create_ctx(nreq, req_nblocks, nblocks, dims, xyz, nullptr, nregions, regions, inputs, keep_q, out);
+}
+
+int orion_hip_batch_create_grids(int nreq, const int* req_nblocks, int nblocks, const int* dims,
+                                 const double* const* req_xyz, int nregions, const orion_region_desc* regions,
+                                 const orion_inputs* inputs, int keep_q, orion_hip_ctx** out) {
+  if (!out || !req_xyz) return fail("batch_create_grids: null output or grids");
+  return create_ctx(nreq, req_nblocks, nblocks, dims, nullptr, req_xyz, nregions, regions, inputs, keep_q, out);
 }
 
 int orion_hip_requests(const orion_hip_ctx* c) { return c ? c->nreq : 0; }

@@ -259,15 +259,17 @@ class Batch:
         self.first = [int(v) for v in np.concatenate([[0], np.cumsum(nbl)[:-1]])]
         dims = np.ascontiguousarray(np.concatenate([np.asarray(c.dims, dtype=np.int32).reshape(-1)
                                                     for c in self.cases]))
-        xyz = np.ascontiguousarray(np.concatenate([np.asarray(c.xyz_flat(), dtype=np.float64) for c in self.cases]))
+        # each request's grid where its case holds it (no gathered copy)
+        grids = [np.ascontiguousarray(c.xyz_flat(), dtype=np.float64) for c in self.cases]
+        gptr = (C.POINTER(C.c_double) * n)(*[g.ctypes.data_as(C.POINTER(C.c_double)) for g in grids])
         regs, nreg = region_descs(self.cases, self.first)
         inps = (_lib.Inputs * n)(*[case_inputs(c) for c in self.cases])
         h = C.c_void_p()
         t_cr = time.perf_counter()
-        _lib.check(L.orion_hip_batch_create(n, nbl.ctypes.data_as(C.POINTER(C.c_int)), int(nbl.sum()),
-                                            dims.ctypes.data_as(C.POINTER(C.c_int)),
-                                            xyz.ctypes.data_as(C.POINTER(C.c_double)), nreg, regs, inps, 0,
-                                            C.byref(h)), "batch_create")
+        _lib.check(L.orion_hip_batch_create_grids(n, nbl.ctypes.data_as(C.POINTER(C.c_int)), int(nbl.sum()),
+                                                  dims.ctypes.data_as(C.POINTER(C.c_int)), gptr, nreg, regs, inps,
+                                                  0, C.byref(h)), "batch_create_grids")
+        del grids
         self.ctx = h
         t_su = time.perf_counter()
         try:
