@@ -77,7 +77,7 @@ struct orion_hip_ctx {
   hipStream_t own_stream = nullptr;
   hipStream_t stream = nullptr;
   hipStream_t side_stream = nullptr;  // inviscid face fluxes beside the viscous chain (captured step)
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_state = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   int nb = 0;
   std::vector<BlockDev> hb;
   BlockDev* db = nullptr;
@@ -1065,7 +1065,6 @@ void orion_hip_ctx_destroy(orion_hip_ctx* c) {
   if (c->graph) hipGraphDestroy(c->graph);
   if (c->ev_fork) hipEventDestroy(c->ev_fork);
   if (c->ev_join) hipEventDestroy(c->ev_join);
-  if (c->ev_state) hipEventDestroy(c->ev_state);
   give_stream(c->device, c->side_stream);
   give_stream(c->device, c->own_stream);
   delete c;
@@ -1132,8 +1131,7 @@ static int create_ctx(int nreq, const int* req_nblocks, int nblocks, const int* 
   if (take_stream(c->device, &c->own_stream) || take_stream(c->device, &c->side_stream))
     return bail(fail("hipStreamCreate failed"));
   if (hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&c->ev_state, hipEventDisableTiming) != hipSuccess)
+      hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess)
     return bail(fail("hipEventCreate failed"));
   c->stream = c->own_stream;
   tstream = c->stream;
@@ -1600,8 +1598,8 @@ static void op_sweeps(orion_hip_ctx* c) {
   for (int t = 0; t < 4; ++t) op_sweep(c, t);
 }
 
-// state_packed: the step already launched the state part of the pack (on
-// the side stream, beside the right-hand side)
+// state_packed: the step already formed the pack's state part (with the
+// interior flow states, state_pack_kernel)
 static void op_update(orion_hip_ctx* c, bool state_packed = false) {
   if (c->per_block) {
     // the reference's update_solution loop: pack, four sweeps and the
@@ -1760,45 +1758,41 @@ static void step_body(orion_hip_ctx* c) {
     op_update_residual(c, c->d_res, c->d_step);
     return;
   }
-  // mode 4 of the state kernel: the first right-hand-side kernel after it
-  // writes dq whole
+  // The step's flow states: the ghost points (state kernel, mode 8) and
+  // the interior with the LU-SGS pack's state part (state_pack_kernel);
+  // mode 4: the first right-hand-side kernel after them writes dq whole.
+  auto states = [&]() {
+    launch_state(LA(c, c->pad), c->pad.total(), 3 | 4 | 8, c->d_dtmin, c->d_step, c->nreq);
+    launch_state_pack(LA(c, c->ptile), c->ptile.total(), c->sw, c->d_dtmin, c->d_step, c->nreq);
+  };
   if (c->p.nvis == 1 && !c->overlap) {
-    launch_state(LA(c, c->pad), c->pad.total(), 3 | 4, c->d_dtmin, c->d_step, c->nreq);
+    states();
     op_visc(c, true);
     op_avg(c);
     op_inv(c, false);
   } else if (c->p.nvis == 1) {
     // The inviscid face fluxes (FP64-bound) run on the side stream beside
-    // the state kernel and the viscous chain (load-bound).  They read prim's
+    // the flow states and the viscous chain (load-bound).  They read prim's
     // first five fields and the static face geometry, which neither writes
-    // (the state kernel writes T, c, mu, q and dt), write their own flux
-    // arrays, and join before the inviscid difference, the first kernel
-    // after them that touches dq.  dq is thus updated in the operators'
-    // order.  After them the side stream packs the LU-SGS state part (it
-    // reads the state kernel's mu and dt, so it waits for that kernel, and
-    // writes only the sweep arrays, which nothing reads before the sweeps).
+    // (the state kernels write T, c, mu, q, dt and the sweep arrays' state
+    // part), write their own flux arrays, and join before the inviscid
+    // difference, the first kernel after them that touches dq.  dq is thus
+    // updated in the operators' order.
     hipEventRecord(c->ev_fork, c->stream);
     hipStreamWaitEvent(c->side_stream, c->ev_fork, 0);
     op_inv_faces(c, c->side_stream);
-    launch_state(LA(c, c->pad), c->pad.total(), 3 | 4, c->d_dtmin, c->d_step, c->nreq);
-    hipEventRecord(c->ev_state, c->stream);
-    hipStreamWaitEvent(c->side_stream, c->ev_state, 0);
-    launch_lu_pack(LA(c, c->ptile, c->side_stream), c->ptile.total(), c->sw, 1);
     hipEventRecord(c->ev_join, c->side_stream);
+    states();
     op_visc(c, true);
     op_avg(c);
     hipStreamWaitEvent(c->stream, c->ev_join, 0);
     launch_inv_diff(LA(c, c->dtile), c->dtile.total(), false);
-    op_avg(c);
-    op_update(c, true);
-    op_update_residual(c, c->d_res, c->d_step);  // and advances *d_step
-    return;
   } else {
-    launch_state(LA(c, c->pad), c->pad.total(), 3 | 4, c->d_dtmin, c->d_step, c->nreq);
+    states();
     op_inv(c, true);
   }
   op_avg(c);
-  op_update(c);
+  op_update(c, true);
   op_update_residual(c, c->d_res, c->d_step);  // and advances *d_step
 }
 
@@ -1875,7 +1869,10 @@ int orion_hip_profile_run(orion_hip_ctx* c, int nsteps, double* phase_ms, long l
     hipEventRecord(ev[0], c->stream);
     op_halo(c); mark(1);
     op_bc(c); mark((long long)c->bc_first.size());
-    launch_state(LA(c, c->pad), c->pad.total(), 3 | 4, c->d_dtmin + (long long)s * c->nreq, nullptr, c->nreq); mark(1);
+    launch_state(LA(c, c->pad), c->pad.total(), 3 | 4 | 8, c->d_dtmin + (long long)s * c->nreq, nullptr, c->nreq);
+    launch_state_pack(LA(c, c->ptile), c->ptile.total(), c->sw, c->d_dtmin + (long long)s * c->nreq, nullptr,
+                      c->nreq);
+    mark(2);
     if (c->p.nvis == 1) {
       launch_visc_deriv(LA(c, c->intr), c->intr.total(), c->d_vhead); mark(1);
       if (c->visc_fused) {  // faces and difference in one launch, timed as the face phase
@@ -1895,7 +1892,7 @@ int orion_hip_profile_run(orion_hip_ctx* c, int nsteps, double* phase_ms, long l
     mark(1);
     launch_inv_closure(LA3(c, c->ltask3), -1, c->ltask3.total(), c->p.gama);
     mark(1);
-    launch_lu_pack(LA(c, c->ptile), c->ptile.total(), c->sw, 1); mark(1);
+    mark(0);  // the pack's state part is in the state phase
     launch_inv_diff(LA(c, c->dtile), c->dtile.total(), c->p.nvis != 1); mark(1);
     op_avg(c); mark(2);
     launch_lu_pack(LA(c, c->ptile), c->ptile.total(), c->sw, 2); mark(1);

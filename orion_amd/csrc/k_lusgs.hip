@@ -244,6 +244,159 @@ __global__ void __launch_bounds__(256) lu_pack_kernel(const BlockDev* __restrict
   }
 }
 
+// The step's flow states and time step at the interior points
+// (state_kernel, modes 1 | 2 | 4, whose interior branch this restates
+// expression for expression: c, T, mu, q, dt and the time step's minimum)
+// fused with the LU-SGS pack's state part (lu_pack_kernel<1>, from the same
+// values in registers instead of reading back mu and dt), over the pack's
+// tiles: a workgroup takes 16 x 16 (march, col) cells of one row, forms the
+// state with lane = (march, col) and stages the pack's words through LDS
+// into the sweep layout.  The ghost points keep their own launch
+// (state_kernel, mode 8).  Each stored value is the one the two kernels
+// store; the minimum of dt vol is order-free.
+__global__ void __launch_bounds__(256) state_pack_kernel(const BlockDev* __restrict__ blocks, Span SP,
+                                                         SweepArrays S, unsigned long long* dtmin_base,
+                                                         const int* step, int nreq) {
+  __shared__ double sh[8][PT * PT];
+  __shared__ double smin[PT * PT];
+  int bi;
+  long long gi;
+  if (!span_item(SP, &bi, &gi)) return;  // whole tiles: every lane of a workgroup has an item
+  const BlockDev& b = blocks[bi];
+  const Params& p = b.par;  // the inputs of the block's request
+  int m0, c0, r;
+  pack_tile(b, gi, &m0, &c0, &r);
+  const int lane = threadIdx.x;
+  double mydt = 1000.0;
+  {
+    const int m = m0 + lane / PT, c = c0 + lane % PT;
+    if (m < b.sn[0] && c < b.sn[1]) {
+      const long long P = b.P;
+      const long long id = natural_id(b, m, r, c);
+      // update_flow_states at an interior point (state_kernel, mode 1)
+      double cs = 0.0, mu = 0.0;
+      {
+        double rho = b.prim[id];
+        const double pp = b.prim[4 * P + id];
+        const double a2 = p.gama * pp / rho;
+        cs = sqrt(fabs(a2));
+        b.c[id] = cs;
+        if (p.nvis == 1) b.prim[5 * P + id] = (p.moo * p.moo) * a2;
+      }
+      if (p.nvis == 1) {
+        double tm = b.prim[5 * P + id];
+        if (tm < 1e-6) tm = 1e-6;
+        mu = tm * sqrt(tm) * (1.0 + p.visc) / (tm + p.visc);
+        b.mu[id] = mu;
+      }
+      if (b.q) {
+        const double rho = b.prim[id], u = b.prim[P + id], v = b.prim[2 * P + id], w = b.prim[3 * P + id];
+        const double pp = b.prim[4 * P + id];
+        const double rc = (rho < 0.0) ? fabs(rho) : rho;
+        b.q[id] = rc;
+        b.q[P + id] = rc * u;
+        b.q[2 * P + id] = rc * v;
+        b.q[3 * P + id] = rc * w;
+        const double ke = 0.5 * rc * (u * u + v * v + w * w);
+        b.q[4 * P + id] = pp * (1.0 / (p.gama - 1.0)) + ke;
+      }
+      // calculate_time_step (state_kernel, mode 2)
+      double pr[5];
+#pragma unroll
+      for (int f = 0; f < 5; ++f) pr[f] = b.prim[f * P + id];
+      double met[9];
+#pragma unroll
+      for (int f = 0; f < 9; ++f) met[f] = b.met[f * P + id];
+      const double vol0 = b.vol[id];
+      double dtv;
+      {
+        const double u = pr[1], v = pr[2], w = pr[3];
+        const double rho = pr[0], vol = vol0;
+        const double csrv = (double)p.csrv;
+        double sum = 0.0;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          const double mx = met[3 * d], my = met[3 * d + 1], mz = met[3 * d + 2];
+          const double U = fabs(u * mx + v * my + w * mz);
+          const double gl = sqrt(mx * mx + my * my + mz * mz);
+          const double spec = U + cs * gl;
+          double sv = 0.0;
+          if (p.nvis == 1) {
+            const double g2 = mx * mx + my * my + mz * mz;
+            const double denom = p.reynolds * rho * vol + 1.0e-30;
+            sv = (2.0 * mu / denom) * g2 * csrv;
+          }
+          sum += spec + sv;
+        }
+        const double dtp = p.cfl / (sum + 1.0e-30);
+        dtv = dtp / vol;
+        b.dt[id] = dtv;
+        const double back = dtv * vol;
+        mydt = back < mydt ? back : mydt;
+      }
+      // the pack's state part (lu_pack_kernel<1>) from the same values;
+      // without viscosity mu is the array's value, as the pack reads it
+      if (p.nvis != 1) mu = b.mu[id];
+      const double gamma = p.gama, re = p.reynolds;
+      sh[0][lane] = met[8];
+      double vol = vol0;
+      if (vol < 1e-30) vol = 1e-30;
+      {  // get_diag_coeff
+        const double ra = spec_rad(pr, gamma, met[0], met[1], met[2]);
+        const double rb = spec_rad(pr, gamma, met[3], met[4], met[5]);
+        const double rcd = spec_rad(pr, gamma, met[6], met[7], met[8]);
+        double rva = 0, rvb = 0, rvc = 0;
+        if (p.nvis == 1) {
+          rva = visc_rad(pr[0], mu, vol, re, met[0], met[1], met[2]);
+          rvb = visc_rad(pr[0], mu, vol, re, met[3], met[4], met[5]);
+          rvc = visc_rad(pr[0], mu, vol, re, met[6], met[7], met[8]);
+        }
+        const double rs = ra + rb + rcd + rva + rvb + rvc;
+        const double term = (1.0 / dtv) + 1.0 * 1.0 * rs;
+        sh[1][lane] = 1.0 / term;
+      }
+      double rm = pr[0], pm = pr[4];
+      const double v2 = pr[1] * pr[1] + pr[2] * pr[2] + pr[3] * pr[3];
+      if (rm < 1e-30) rm = 1e-30;
+      if (pm < 1e-30) pm = 1e-30;
+      const double c2 = gamma * pm / rm;
+      sh[2][lane] = pr[1];
+      sh[3][lane] = pr[2];
+      sh[4][lane] = pr[3];
+      sh[5][lane] = c2;
+      sh[6][lane] = (gamma / (gamma - 1.0)) * (pm / rm) + 0.5 * v2;
+      sh[7][lane] = 2.0 * mu / (fabs(re) * (rm * vol) + 1.0e-30);
+    }
+  }
+  smin[lane] = mydt;
+  __syncthreads();
+  {
+    const int src = sweep_slot(lane);
+    const int m = m0 + src / PT, c = c0 + src % PT;
+    if (m < b.sn[0] && c < b.sn[1]) {
+      const int ncp = b.sncp;
+      const long long rb = sweep_index(b, m, r, c) - c;
+      S.own[rb * SW_OWN + 4LL * ncp + c] = make_double2(sh[0][src], sh[1][src]);
+#pragma unroll
+      for (int k = 0; k < SW_D; ++k)
+        S.D[rb * SW_D + (long long)k * ncp + c] = make_double2(sh[2 + 2 * k][src], sh[3 + 2 * k][src]);
+    }
+  }
+  // the time step's minimum over the workgroup, into the slot of (step,
+  // request) as state_kernel leaves it
+  for (int h = PT * PT / 2; h > 0; h >>= 1) {
+    if (lane < h) {
+      const double o = smin[lane + h];
+      if (o < smin[lane]) smin[lane] = o;
+    }
+    __syncthreads();
+  }
+  if (lane == 0 && smin[0] < 1000.0) {
+    unsigned long long* dtmin_bits = dtmin_base + (long long)(step ? *step : 0) * nreq + b.req;
+    atomicMin(dtmin_bits, (unsigned long long)__double_as_longlong(smin[0]));
+  }
+}
+
 // dq back to the block arrays (zero on physical-BC windows,
 // StateUpdater.cpp:577) and update_limited (:470-542).
 // The update also forms the residual's terms (ResidualMonitor.cpp, as in
@@ -1113,6 +1266,13 @@ void launch_lu_pack(const Launch& La, long long n, const SweepArrays& S, int par
     hipLaunchKernelGGL(lu_pack_kernel<1>, dim3((unsigned)La.nwg), dim3(SPAN_TB), 0, La.stream, La.blocks, La.span, S);
   else
     hipLaunchKernelGGL(lu_pack_kernel<2>, dim3((unsigned)La.nwg), dim3(SPAN_TB), 0, La.stream, La.blocks, La.span, S);
+}
+
+void launch_state_pack(const Launch& La, long long n, const SweepArrays& S, unsigned long long* dtmin_base,
+                       const int* step, int nreq) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(state_pack_kernel, dim3((unsigned)La.nwg), dim3(SPAN_TB), 0, La.stream, La.blocks, La.span, S,
+                     dtmin_base, step, nreq);
 }
 
 void launch_update(const Launch& La, long long n, const SweepArrays& S, double gamma, const long long* ioff,

@@ -31,69 +31,72 @@ __global__ void __launch_bounds__(256) state_kernel(const BlockDev* __restrict__
     const int i = (int)ui, j = (int)uj, k = (int)q2;
     const int oi = (i < NG || i >= NG + b.idim), oj = (j < NG || j >= NG + b.jdim), ok = (k < NG || k >= NG + b.kdim);
     const int nout = oi + oj + ok;
-    const long long P = b.P;
-    const long long id = n;
-    double c = 0.0, mu = 0.0;
-    if (mode & 1) {
-      if (nout <= 1) {
-        double rho = b.prim[id];
-        const double pp = b.prim[4 * P + id];
-        if (nout == 1) rho = fabs(rho);
-        const double a2 = p.gama * pp / rho;
-        c = sqrt(fabs(a2));
-        b.c[id] = c;
-        if (p.nvis == 1) b.prim[5 * P + id] = (p.moo * p.moo) * a2;
-      }
-      if (p.nvis == 1) {
-        double tm = b.prim[5 * P + id];
-        if (tm < 1e-6) tm = 1e-6;
-        mu = tm * sqrt(tm) * (1.0 + p.visc) / (tm + p.visc);
-        b.mu[id] = mu;
-      }
-      if (b.q && nout <= 1) {
-        const double rho = b.prim[id], u = b.prim[P + id], v = b.prim[2 * P + id], w = b.prim[3 * P + id];
-        const double pp = b.prim[4 * P + id];
-        const double rc = (nout == 1 || rho < 0.0) ? fabs(rho) : rho;
-        b.q[id] = rc;
-        b.q[P + id] = rc * u;
-        b.q[2 * P + id] = rc * v;
-        b.q[3 * P + id] = rc * w;
-        const double ke = 0.5 * rc * (u * u + v * v + w * w);
-        b.q[4 * P + id] = pp * (1.0 / (p.gama - 1.0)) + ke;
-      }
-      if (nout == 0 && !(mode & 4)) {  // mode 4: the first right-hand-side kernel writes dq whole
-#pragma unroll
-        for (int m = 0; m < 5; ++m) b.dq[m * P + id] = 0.0;
-      }
-    }
-    if ((mode & 2) && nout == 0) {
-      if (!(mode & 1)) {
-        c = b.c[id];
-        mu = b.mu[id];
-      }
-      const double u = b.prim[P + id], v = b.prim[2 * P + id], w = b.prim[3 * P + id];
-      const double rho = b.prim[id], vol = b.vol[id];
-      const double csrv = (double)p.csrv;
-      double sum = 0.0;
-#pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        const double mx = b.met[(3 * d) * P + id], my = b.met[(3 * d + 1) * P + id], mz = b.met[(3 * d + 2) * P + id];
-        const double U = fabs(u * mx + v * my + w * mz);
-        const double gl = sqrt(mx * mx + my * my + mz * mz);
-        const double spec = U + c * gl;
-        double sv = 0.0;
-        if (p.nvis == 1) {
-          const double g2 = mx * mx + my * my + mz * mz;
-          const double denom = p.reynolds * rho * vol + 1.0e-30;
-          sv = (2.0 * mu / denom) * g2 * csrv;
+    // mode 8: ghost points only (state_pack_kernel forms the interior)
+    if (!((mode & 8) && nout == 0)) {
+      const long long P = b.P;
+      const long long id = n;
+      double c = 0.0, mu = 0.0;
+      if (mode & 1) {
+        if (nout <= 1) {
+          double rho = b.prim[id];
+          const double pp = b.prim[4 * P + id];
+          if (nout == 1) rho = fabs(rho);
+          const double a2 = p.gama * pp / rho;
+          c = sqrt(fabs(a2));
+          b.c[id] = c;
+          if (p.nvis == 1) b.prim[5 * P + id] = (p.moo * p.moo) * a2;
         }
-        sum += spec + sv;
+        if (p.nvis == 1) {
+          double tm = b.prim[5 * P + id];
+          if (tm < 1e-6) tm = 1e-6;
+          mu = tm * sqrt(tm) * (1.0 + p.visc) / (tm + p.visc);
+          b.mu[id] = mu;
+        }
+        if (b.q && nout <= 1) {
+          const double rho = b.prim[id], u = b.prim[P + id], v = b.prim[2 * P + id], w = b.prim[3 * P + id];
+          const double pp = b.prim[4 * P + id];
+          const double rc = (nout == 1 || rho < 0.0) ? fabs(rho) : rho;
+          b.q[id] = rc;
+          b.q[P + id] = rc * u;
+          b.q[2 * P + id] = rc * v;
+          b.q[3 * P + id] = rc * w;
+          const double ke = 0.5 * rc * (u * u + v * v + w * w);
+          b.q[4 * P + id] = pp * (1.0 / (p.gama - 1.0)) + ke;
+        }
+        if (nout == 0 && !(mode & 4)) {  // mode 4: the first right-hand-side kernel writes dq whole
+  #pragma unroll
+          for (int m = 0; m < 5; ++m) b.dq[m * P + id] = 0.0;
+        }
       }
-      const double dtp = p.cfl / (sum + 1.0e-30);
-      const double dtv = dtp / vol;
-      b.dt[id] = dtv;
-      const double back = dtv * vol;
-      mydt = back < mydt ? back : mydt;
+      if ((mode & 2) && nout == 0) {
+        if (!(mode & 1)) {
+          c = b.c[id];
+          mu = b.mu[id];
+        }
+        const double u = b.prim[P + id], v = b.prim[2 * P + id], w = b.prim[3 * P + id];
+        const double rho = b.prim[id], vol = b.vol[id];
+        const double csrv = (double)p.csrv;
+        double sum = 0.0;
+  #pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          const double mx = b.met[(3 * d) * P + id], my = b.met[(3 * d + 1) * P + id], mz = b.met[(3 * d + 2) * P + id];
+          const double U = fabs(u * mx + v * my + w * mz);
+          const double gl = sqrt(mx * mx + my * my + mz * mz);
+          const double spec = U + c * gl;
+          double sv = 0.0;
+          if (p.nvis == 1) {
+            const double g2 = mx * mx + my * my + mz * mz;
+            const double denom = p.reynolds * rho * vol + 1.0e-30;
+            sv = (2.0 * mu / denom) * g2 * csrv;
+          }
+          sum += spec + sv;
+        }
+        const double dtp = p.cfl / (sum + 1.0e-30);
+        const double dtv = dtp / vol;
+        b.dt[id] = dtv;
+        const double back = dtv * vol;
+        mydt = back < mydt ? back : mydt;
+      }
     }
   }
   if (!dtmin_base) return;

@@ -103,6 +103,11 @@ void launch_avg_apply(const BlockDev* blocks, const AvgDst* dst, long long n, co
 void launch_lu_geom(const Launch& La, long long n, const SweepArrays& S);
 // part 1: met8, coed and D (state only); part 2: the rhs copy
 void launch_lu_pack(const Launch& La, long long n, const SweepArrays& S, int part);
+// the step's interior flow states and time step (state modes 1 | 2 | 4)
+// with the pack's state part, over the pack's tile space; the ghost points
+// take launch_state with mode 8
+void launch_state_pack(const Launch& La, long long n, const SweepArrays& S, unsigned long long* dtmin_base,
+                       const int* step, int nreq);
 void launch_lusgs_tiles(const SweepArrays& S, const BlockDev* blocks, const SweepTile* tiles, const int* order,
                         int ntiles, unsigned long long* flags, unsigned long long* epoch, int* err, int type,
                         const Params& p, hipStream_t s);
