@@ -1068,10 +1068,10 @@ __global__ void __launch_bounds__(256) visc_fused_kernel(const BlockDev* __restr
 // Flux differences of the inviscid (INV, closure overrides included) or the
 // viscous face fluxes into dq, per direction i, j, k in that order
 // (compute_flux_diff_1d / compute_flux_diff_high_order; viscous: dq -=
-// diff / Re, inviscid: dq += diff).  A workgroup takes 16 x 16 (i, j) nodes
-// of one k plane; the i and j stencils of the tile (21 face slots per row
-// or column) are staged through LDS once, the k stencils are loaded per
-// node.  Node t of a line reads faces t-2..t+3, which sit at the padded
+// diff / Re, inviscid: dq += diff).  A workgroup takes DIFF_TI x DIFF_TJ
+// (i, j) nodes of one k plane; the i and j stencils of the tile (5 face
+// slots more than nodes per row or column) are staged through LDS once, the
+// k stencils are loaded per node.  Node t of a line reads faces t-2..t+3, which sit at the padded
 // slots t..t+5.  FIRST: the step's first right-hand-side kernel, which starts
 // from dq = 0 (what update_flow_states leaves) instead of reading it.
 // The viscous mass flux is identically zero (compute_viscous_flux_1d sets
@@ -1081,10 +1081,10 @@ __global__ void __launch_bounds__(256) visc_fused_kernel(const BlockDev* __restr
 // it.
 template <bool INV, bool FIRST>
 __global__ void __launch_bounds__(256) diff_kernel(const BlockDev* __restrict__ blocks, Span SP) {
-  constexpr int DT = 16, DW = DT + 5;
+  constexpr int TI = DIFF_TI, TJ = DIFF_TJ;     // the tile's nodes along i and j
   constexpr int M0 = INV ? 0 : 1, NM = 5 - M0;  // flux components differenced
-  __shared__ double X[NM][DT][DW];  // i slots i0-3 .. i0+17 of each row of the tile
-  __shared__ double Y[NM][DW][DT];  // j slots j0-3 .. j0+17 of each column
+  __shared__ double X[NM][TJ][TI + 5];  // i slots i0-3 .. i0+TI+1 of each row of the tile
+  __shared__ double Y[NM][TJ + 5][TI];  // j slots j0-3 .. j0+TJ+1 of each column
   int bi;
   long long gi;
   if (!span_item(SP, &bi, &gi)) return;  // whole tiles: every lane of a workgroup has an item
@@ -1092,38 +1092,37 @@ __global__ void __launch_bounds__(256) diff_kernel(const BlockDev* __restrict__ 
   const double re_inv = b.re_inv;
   // 32-bit offsets: the flux arrays hold 15 fields of P < 2^25 points
   const unsigned P = (unsigned)b.P, nx = (unsigned)b.nx, nxy = (unsigned)b.nx * (unsigned)b.ny;
-  const unsigned tci = (unsigned)(b.idim + DT - 1) / DT, tcj = (unsigned)(b.jdim + DT - 1) / DT;
+  const unsigned tci = (unsigned)(b.idim + TI - 1) / TI, tcj = (unsigned)(b.jdim + TJ - 1) / TJ;
   unsigned q, ti, q2, tj;
   udivmod((unsigned)(gi / 256), tci, &q, &ti);
   udivmod(q, tcj, &q2, &tj);
   const int k = (int)q2 + NG;
-  const int i0 = NG + DT * (int)ti, j0 = NG + DT * (int)tj;
-  const int lane = threadIdx.x, li = lane % DT, lj = lane / DT;
+  const int i0 = NG + TI * (int)ti, j0 = NG + TJ * (int)tj;
+  const int lane = threadIdx.x, li = lane % TI, lj = lane / TI;
   const int i = i0 + li, j = j0 + lj;
   const double* FX = INV ? b.fxi : b.fx;
   const unsigned kplane = (unsigned)k * nxy;
   // each lane stages up to two (row, slot) positions of each direction, all
   // components of a position from one offset
+  static_assert(TI * TJ == 256 && TJ * (TI + 5) <= 512 && (TJ + 5) * TI <= 512, "tile of 256 nodes");
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
     const int p = lane + 256 * h;
-    if (p < DT * DW) {
-      {
-        const int row = p / DW, col = p % DW;
-        const int jj = j0 + row, ii = i0 - 3 + col;
-        const bool ok = jj < NG + b.jdim && ii < b.nx;
-        const unsigned e = (unsigned)ii + (unsigned)jj * nx + kplane;
+    if (p < TJ * (TI + 5)) {
+      const int row = p / (TI + 5), col = p % (TI + 5);
+      const int jj = j0 + row, ii = i0 - 3 + col;
+      const bool ok = jj < NG + b.jdim && ii < b.nx;
+      const unsigned e = (unsigned)ii + (unsigned)jj * nx + kplane;
 #pragma unroll
-        for (int m = 0; m < NM; ++m) X[m][row][col] = ok ? ldo(FX, boff((unsigned)(M0 + m), P, e)) : 0.0;
-      }
-      {
-        const int row = p / DT, col = p % DT;
-        const int ii = i0 + col, jj = j0 - 3 + row;
-        const bool ok = ii < NG + b.idim && jj < b.ny;
-        const unsigned e = (unsigned)ii + (unsigned)jj * nx + kplane;
+      for (int m = 0; m < NM; ++m) X[m][row][col] = ok ? ldo(FX, boff((unsigned)(M0 + m), P, e)) : 0.0;
+    }
+    if (p < (TJ + 5) * TI) {
+      const int row = p / TI, col = p % TI;
+      const int ii = i0 + col, jj = j0 - 3 + row;
+      const bool ok = ii < NG + b.idim && jj < b.ny;
+      const unsigned e = (unsigned)ii + (unsigned)jj * nx + kplane;
 #pragma unroll
-        for (int m = 0; m < NM; ++m) Y[m][row][col] = ok ? ldo(FX, boff((unsigned)(5 + M0 + m), P, e)) : 0.0;
-      }
+      for (int m = 0; m < NM; ++m) Y[m][row][col] = ok ? ldo(FX, boff((unsigned)(5 + M0 + m), P, e)) : 0.0;
     }
   }
   __syncthreads();

@@ -157,6 +157,11 @@ __host__ __device__ inline long long sweep_field(long long q, int c, int ncp, in
 // unrelated tiles in one line would make those accesses contend.
 constexpr int FLAG_STRIDE = 32;
 
+// The flux differences' tiles: DIFF_TI x DIFF_TJ (i, j) nodes of a k plane
+// per workgroup (256 nodes).
+constexpr int DIFF_TI = 16;
+constexpr int DIFF_TJ = 16;
+
 // LU-SGS tiles: SWEEP_TW columns by SWEEP_TR rows of pencils (the block
 // layout pads columns to whole multiples of SWEEP_TW).  16 x 8 tiles were
 // measured slower than 16 x 16 on the 64-block mesh: a step's time is set
