@@ -111,8 +111,10 @@ struct orion_hip_ctx {
   unsigned long long* d_flags = nullptr; // per-tile progress
   unsigned long long* d_epoch = nullptr; // sweep counter
   int* d_err = nullptr;                  // set by a sweep whose wait ran out
-  long long* d_halo = nullptr;
+  long long* d_halo = nullptr;          // halo: (destination, source) arena offsets, one pair per field value
   long long n_halo = 0;
+  unsigned long long* d_halo_pt = nullptr;  // halo: whole points, packed (halo_point_kernel)
+  long long n_halo_pt = 0;
   BCTask* d_bc = nullptr;
   std::vector<int> bc_first, bc_count;  // per rank: task offset and count
   std::vector<long long> bc_total;      // per rank: face nodes
@@ -455,13 +457,68 @@ static int build_topology(orion_hip_ctx* c, const std::vector<std::vector<Reg>>&
       links[b].push_back({r, M->block, M->reg});
     }
   // each receiving block's writes, reduced to its last writers, on host
-  // threads (a block's destinations are its own, so blocks are independent)
+  // threads (a block's destinations are its own, so blocks are independent).
+  // A block whose windows each unpack as many points as their senders pack
+  // receives whole points: field m of a ghost comes from field m of one
+  // source point, so its writes reduce per point to one entry of five
+  // fields (halo_point).  A window of another length shifts the fields
+  // across points; its block keeps one pair per field value.
   std::vector<std::vector<long long>> bpairs(nb);
+  std::vector<std::vector<unsigned long long>> bpts(nb);
   parallel_blocks(nb, [&](int b, std::vector<long long>& where) {
     const BlockDev& B = c->hb[b];
     if ((long long)where.size() < 5 * B.P) where.assign((size_t)(5 * B.P), -1);
-    std::vector<long long>& pairs = bpairs[b];
     std::vector<long long> touched;
+    bool whole = B.P < (1ll << (64 - HALO_OFF_BITS));
+    for (const Link& L : links[b]) {
+      const BlockDev& A = c->hb[L.sblock];
+      int sb[3], se[3], rb[3], re[3];
+      memory_window(R[L.sblock][L.sreg], A, 0, sb, se);
+      memory_window(R[b][L.reg], B, 1, rb, re);
+      const long long ns = (long long)(se[0] - sb[0] + 1) * (se[1] - sb[1] + 1) * (se[2] - sb[2] + 1);
+      const long long nr = (long long)(re[0] - rb[0] + 1) * (re[1] - rb[1] + 1) * (re[2] - rb[2] + 1);
+      const long long end = arena_off(L.sblock, 4, A.P);
+      if (ns != nr || A.P >= (1ll << (64 - HALO_OFF_BITS)) || end > (long long)HALO_OFF_MASK ||
+          arena_off(b, 4, B.P) > (long long)HALO_OFF_MASK)
+        whole = false;
+    }
+    if (whole) {
+      std::vector<unsigned long long>& pts = bpts[b];
+      for (const Link& L : links[b]) {
+        const BlockDev& A = c->hb[L.sblock];
+        int sb[3], se[3], rb[3], re[3];
+        memory_window(R[L.sblock][L.sreg], A, 0, sb, se);
+        memory_window(R[b][L.reg], B, 1, rb, re);
+        // the sender's packing order (k, j, i over its window) walked
+        // alongside the receiver's unpacking order, the same for each field
+        int si = sb[0], sj = sb[1], sk = sb[2];
+        for (int k = rb[2]; k <= re[2]; ++k)
+          for (int j = rb[1]; j <= re[1]; ++j)
+            for (int i = rb[0]; i <= re[0]; ++i) {
+              const long long local = pidx(B, i, j, k);
+              const unsigned long long src = halo_point(arena_off(L.sblock, 0, pidx(A, si, sj, sk)), A.P);
+              if (++si > se[0]) {
+                si = sb[0];
+                if (++sj > se[1]) {
+                  sj = sb[1];
+                  ++sk;
+                }
+              }
+              long long& w = where[(size_t)local];
+              if (w >= 0) {
+                pts[2 * w + 1] = src;
+              } else {
+                w = (long long)(pts.size() / 2);
+                pts.push_back(halo_point(arena_off(b, 0, local), B.P));
+                pts.push_back(src);
+                touched.push_back(local);
+              }
+            }
+      }
+      for (long long t : touched) where[(size_t)t] = -1;
+      return std::string();
+    }
+    std::vector<long long>& pairs = bpairs[b];
     for (const Link& L : links[b]) {
       const Reg& rg = R[b][L.reg];
       const BlockDev& A = c->hb[L.sblock];
@@ -516,6 +573,17 @@ static int build_topology(orion_hip_ctx* c, const std::vector<std::vector<Reg>>&
       if (!v.empty())
         HIPCHK(hipMemcpyAsync(c->d_halo + off, v.data(), sizeof(long long) * v.size(), hipMemcpyHostToDevice,
                               c->stream));
+      off += v.size();
+    }
+    size_t ptot = 0;
+    for (auto& v : bpts) ptot += v.size();
+    c->n_halo_pt = (long long)(ptot / 2);
+    if (dalloc(c, (void**)&c->d_halo_pt, sizeof(unsigned long long) * std::max<size_t>(ptot, 1))) return -1;
+    off = 0;
+    for (auto& v : bpts) {
+      if (!v.empty())
+        HIPCHK(hipMemcpyAsync(c->d_halo_pt + off, v.data(), sizeof(unsigned long long) * v.size(),
+                              hipMemcpyHostToDevice, c->stream));
       off += v.size();
     }
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -1526,7 +1594,12 @@ static void op_state(orion_hip_ctx* c, int mode, unsigned long long* dtmin, cons
   launch_state(LA(c, c->pad), c->pad.total(), mode, dtmin, step, c->nreq);
 }
 
-static void op_halo(orion_hip_ctx* c) { launch_halo(c->d_halo, c->n_halo, (double*)c->arena, c->stream); }
+// the blocks that receive whole points, then those that receive field
+// values (their destinations are disjoint)
+static void op_halo(orion_hip_ctx* c) {
+  launch_halo_points(c->d_halo_pt, c->n_halo_pt, (double*)c->arena, c->stream);
+  launch_halo(c->d_halo, c->n_halo, (double*)c->arena, c->stream);
+}
 
 static void op_bc(orion_hip_ctx* c) {
   for (size_t rk = 0; rk < c->bc_first.size(); ++rk)

@@ -7,6 +7,8 @@
 //               in the receiver's own order, last writer kept, built once
 //               from the BC topology.  Sources are interior points and
 //               destinations ghosts, so the gather has no ordering hazards.
+//               Blocks whose windows pair equal point counts move whole
+//               points (five fields per entry, halo_point_kernel).
 //   bc          apply_physical_bc (src/bc/PhysicalBC.cpp:12-613): one launch
 //               per region rank (regions keep file order within a block),
 //               one thread per face node.
@@ -25,6 +27,23 @@ __global__ void __launch_bounds__(256) halo_kernel(const long long* __restrict__
   const long long g = xcd_thread();
   if (g >= n) return;
   arena[pairs[2 * g]] = arena[pairs[2 * g + 1]];
+}
+
+// The same gather by whole points: entry g moves the five halo fields of
+// one source point to one ghost point (halo_point packs each side's field 0
+// offset and field stride), all five loaded before any is stored.
+__global__ void __launch_bounds__(256) halo_point_kernel(const unsigned long long* __restrict__ pts, long long n,
+                                                         double* __restrict__ arena) {
+  const long long g = xcd_thread();
+  if (g >= n) return;
+  const unsigned long long d = pts[2 * g], s = pts[2 * g + 1];
+  const unsigned long long d0 = d & HALO_OFF_MASK, dP = d >> HALO_OFF_BITS;
+  const unsigned long long s0 = s & HALO_OFF_MASK, sP = s >> HALO_OFF_BITS;
+  double v[5];
+#pragma unroll
+  for (int m = 0; m < 5; ++m) v[m] = arena[s0 + (unsigned long long)m * sP];
+#pragma unroll
+  for (int m = 0; m < 5; ++m) arena[d0 + (unsigned long long)m * dP] = v[m];
 }
 
 __device__ inline double& PR(const BlockDev& b, int i, int j, int k, int m) {
@@ -329,6 +348,10 @@ static inline unsigned grid_of(long long n, int tb) { return (unsigned)((n + tb 
 void launch_halo(const long long* pairs, long long n, double* arena, hipStream_t s) {
   if (n <= 0) return;
   hipLaunchKernelGGL(halo_kernel, dim3(grid_of(n, 256)), dim3(256), 0, s, pairs, n, arena);
+}
+void launch_halo_points(const unsigned long long* pts, long long n, double* arena, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(halo_point_kernel, dim3(grid_of(n, 256)), dim3(256), 0, s, pts, n, arena);
 }
 void launch_bc(const BlockDev* blocks, const BCTask* tasks, int ntasks, long long n, hipStream_t s) {
   if (n <= 0 || ntasks <= 0) return;

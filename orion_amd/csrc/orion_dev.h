@@ -152,6 +152,15 @@ __host__ __device__ inline long long sweep_field(long long q, int c, int ncp, in
   return 2 * ((q - c) * K + (long long)(f >> 1) * ncp + c) + (f & 1);
 }
 
+// A halo point entry: the arena offset of field 0 of a point (low
+// HALO_OFF_BITS bits) and the field stride P of its block (the bits above),
+// one entry for the destination and one for the source.
+constexpr int HALO_OFF_BITS = 39;
+constexpr unsigned long long HALO_OFF_MASK = (1ull << HALO_OFF_BITS) - 1;
+__host__ __device__ inline unsigned long long halo_point(long long off, long long P) {
+  return (unsigned long long)off | ((unsigned long long)P << HALO_OFF_BITS);
+}
+
 // Progress flags of the LU-SGS tiles, one per 256 bytes: a flag is polled by
 // the tile's downstream neighbours while the tile stores it, and flags of
 // unrelated tiles in one line would make those accesses contend.

@@ -95,6 +95,7 @@ void launch_temperature(const Launch& L, long long n);
 
 // k_bc.hip
 void launch_halo(const long long* pairs, long long n, double* arena, hipStream_t s);
+void launch_halo_points(const unsigned long long* pts, long long n, double* arena, hipStream_t s);
 void launch_bc(const BlockDev* blocks, const BCTask* tasks, int ntasks, long long n, hipStream_t s);
 void launch_avg_stage(const BlockDev* blocks, const AvgSrc* src, long long n, double* stage, hipStream_t s);
 void launch_avg_apply(const BlockDev* blocks, const AvgDst* dst, long long n, const double* stage, hipStream_t s);
