@@ -53,8 +53,10 @@ STAGES = {
     "rhs": (20, ["visc_deriv", "visc_face", "visc_diff", "inv_face", "inv_closure", "inv_diff"],
             "visc_deriv_kernel+visc_face_queue_kernel+diff_kernel<false,true>+inv_face_pair_kernel+inv_closure_kernel"
             "+diff_kernel<true,false>"),
-    # a layout copy in two parts: no algorithmic words of its own
-    "lu_pack": (0, ["lu_pack_state", "lu_pack"], "lu_pack_kernel<1>+lu_pack_kernel<2>"),
+    # a layout copy: no algorithmic words of its own.  Its state part runs
+    # inside state_pack_kernel with the flow states (the state phase); the
+    # rhs part is lu_pack_kernel<2>
+    "lu_pack": (0, ["lu_pack_state", "lu_pack"], "lu_pack_kernel<2>"),
     # the four sweeps (LU forward 25, LU backward 25, GS forward 30, GS
     # backward 30 words): one chained launch, or one launch per sweep
     "lu_sweeps": (110, ["lu_fwd", "lu_bwd", "gs_fwd", "gs_bwd", "lu_chain"], "lusgs_chain_kernel<true>"),
