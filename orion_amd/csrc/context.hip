@@ -98,7 +98,7 @@ struct orion_hip_ctx {
   // the same three spaces each joined over the directions (virtual blocks
   // d * nb + b): one launch per step for all directions
   Space face3, fpair3, ltask3;
-  Space dtile;                         // flux differences: DIFF_TI x DIFF_TJ (i, j) nodes of a k plane
+  Space dtile;                         // flux differences: DIFF_TI x DIFF_TJ (i, j) nodes of DIFF_TK k planes
   Space ptile;                         // LU-SGS pack tiles: 16 x 16 (march, col) cells of one row
   SweepArrays sw{};                    // LU-SGS working set in sweep layout
   std::vector<SweepTile> tiles;
@@ -1371,7 +1371,8 @@ static int create_ctx(int nreq, const int* req_nblocks, int nblocks, const int* 
     std::vector<long long> ct(nblocks);
     for (int b = 0; b < nblocks; ++b) {
       const BlockDev& B = c->hb[b];
-      ct[b] = (long long)((B.idim + DIFF_TI - 1) / DIFF_TI) * ((B.jdim + DIFF_TJ - 1) / DIFF_TJ) * B.kdim * 256;
+      ct[b] = (long long)((B.idim + DIFF_TI - 1) / DIFF_TI) * ((B.jdim + DIFF_TJ - 1) / DIFF_TJ) *
+              ((B.kdim + DIFF_TK - 1) / DIFF_TK) * 256;
     }
     if (make_space(c, c->dtile, ct)) return bail(-1);
   }

@@ -1069,11 +1069,15 @@ __global__ void __launch_bounds__(256) visc_fused_kernel(const BlockDev* __restr
 // viscous face fluxes into dq, per direction i, j, k in that order
 // (compute_flux_diff_1d / compute_flux_diff_high_order; viscous: dq -=
 // diff / Re, inviscid: dq += diff).  A workgroup takes DIFF_TI x DIFF_TJ
-// (i, j) nodes of one k plane; the i and j stencils of the tile (5 face
-// slots more than nodes per row or column) are staged through LDS once, the
-// k stencils are loaded per node.  Node t of a line reads faces t-2..t+3, which sit at the padded
-// slots t..t+5.  FIRST: the step's first right-hand-side kernel, which starts
-// from dq = 0 (what update_flow_states leaves) instead of reading it.
+// (i, j) nodes of DIFF_TK consecutive k planes.  Each lane first forms the
+// k differences of its column's DIFF_TK nodes from one register window of
+// the k faces they read (faces t0-2 .. t0+DIFF_TK+2 of the line, loaded
+// once for the DIFF_TK nodes); then, plane by plane, the i and j stencils
+// of the tile (5 face slots more than nodes per row or column) are staged
+// through LDS once.  Node t of a line reads faces t-2..t+3, which sit at the
+// padded slots t..t+5.  FIRST: the step's first right-hand-side kernel,
+// which starts from dq = 0 (what update_flow_states leaves) instead of
+// reading it.
 // The viscous mass flux is identically zero (compute_viscous_flux_1d sets
 // it to 0): its difference is +-0, and dq0 - Re^-1 (+-0) is dq0 bit for bit
 // for every dq0 the step produces (dq starts from +0), so the viscous
@@ -1081,8 +1085,8 @@ __global__ void __launch_bounds__(256) visc_fused_kernel(const BlockDev* __restr
 // it.
 template <bool INV, bool FIRST>
 __global__ void __launch_bounds__(256) diff_kernel(const BlockDev* __restrict__ blocks, Span SP) {
-  constexpr int TI = DIFF_TI, TJ = DIFF_TJ;     // the tile's nodes along i and j
-  constexpr int M0 = INV ? 0 : 1, NM = 5 - M0;  // flux components differenced
+  constexpr int TI = DIFF_TI, TJ = DIFF_TJ, TK = DIFF_TK;  // the tile's nodes along i, j and k
+  constexpr int M0 = INV ? 0 : 1, NM = 5 - M0;               // flux components differenced
   __shared__ double X[NM][TJ][TI + 5];  // i slots i0-3 .. i0+TI+1 of each row of the tile
   __shared__ double Y[NM][TJ + 5][TI];  // j slots j0-3 .. j0+TJ+1 of each column
   int bi;
@@ -1096,81 +1100,124 @@ __global__ void __launch_bounds__(256) diff_kernel(const BlockDev* __restrict__ 
   unsigned q, ti, q2, tj;
   udivmod((unsigned)(gi / 256), tci, &q, &ti);
   udivmod(q, tcj, &q2, &tj);
-  const int k = (int)q2 + NG;
+  const int k0 = TK * (int)q2 + NG;
   const int i0 = NG + TI * (int)ti, j0 = NG + TJ * (int)tj;
   const int lane = threadIdx.x, li = lane % TI, lj = lane / TI;
   const int i = i0 + li, j = j0 + lj;
+  const bool in_ij = i < NG + b.idim && j < NG + b.jdim;
   const double* FX = INV ? b.fxi : b.fx;
-  const unsigned kplane = (unsigned)k * nxy;
+  // the k differences of nodes (i, j, k0 .. k0+TK-1), closure values
+  // applied.  Window faces outside 0..n are never read; their slots are
+  // clamped to the padded line (slot n+5 at most: a last group of fewer
+  // than TK planes would otherwise reach past it)
+  const int nk = b.kdim, t0 = k0 - NG;
+  const unsigned lbk = (unsigned)i + (unsigned)j * nx;  // the k line's position 0
+  double fk[TK][5];
+  if (in_ij) {
+    const double* cov[TK];
+#pragma unroll
+    for (int kk = 0; kk < TK; ++kk) cov[kk] = nullptr;
+    if (INV) {
+      const bool lflag = ldo(b.prim, lbk << 3) < 1.0e-20;
+      const bool rflag = ldo(b.prim, (lbk + (unsigned)(nk + 5) * nxy) << 3) < 1.0e-20;
+#pragma unroll
+      for (int kk = 0; kk < TK; ++kk) {
+        const int t = t0 + kk;
+        if (t >= nk) continue;  // no node (a last group of fewer than TK planes)
+        if (lflag && t <= 2) cov[kk] = b.fcov + b.fcov_off[2] + line_id(b, 2, i, j, k0 + kk) * 30 + t * 5;
+        else if (rflag && t >= nk - 3)
+          cov[kk] = b.fcov + b.fcov_off[2] + line_id(b, 2, i, j, k0 + kk) * 30 + (3 + t - (nk - 3)) * 5;
+      }
+    }
+#pragma unroll
+    for (int m = M0; m < 5; ++m) {
+      double W[TK + 5];
+#pragma unroll
+      for (int w = 0; w < TK + 5; ++w)
+        W[w] = ldo(FX, boff((unsigned)(10 + m), P, lbk + (unsigned)min(t0 + w, nk + 5) * nxy));
+#pragma unroll
+      for (int kk = 0; kk < TK; ++kk) {
+        fk[kk][m] = diff_node_w([&](int w) { return W[kk + w]; }, nk, t0 + kk);
+        if (INV && cov[kk]) fk[kk][m] = cov[kk][m];
+      }
+    }
+  }
   // each lane stages up to two (row, slot) positions of each direction, all
   // components of a position from one offset
   static_assert(TI * TJ == 256 && TJ * (TI + 5) <= 512 && (TJ + 5) * TI <= 512, "tile of 256 nodes");
 #pragma unroll
-  for (int h = 0; h < 2; ++h) {
-    const int p = lane + 256 * h;
-    if (p < TJ * (TI + 5)) {
-      const int row = p / (TI + 5), col = p % (TI + 5);
-      const int jj = j0 + row, ii = i0 - 3 + col;
-      const bool ok = jj < NG + b.jdim && ii < b.nx;
-      const unsigned e = (unsigned)ii + (unsigned)jj * nx + kplane;
+  for (int kk = 0; kk < TK; ++kk) {
+    const int k = k0 + kk;
+    if (k >= NG + nk) break;  // the same for the whole workgroup
+    if (kk > 0) __syncthreads();  // every lane has read the plane before
+    const unsigned kplane = (unsigned)k * nxy;
 #pragma unroll
-      for (int m = 0; m < NM; ++m) X[m][row][col] = ok ? ldo(FX, boff((unsigned)(M0 + m), P, e)) : 0.0;
-    }
-    if (p < (TJ + 5) * TI) {
-      const int row = p / TI, col = p % TI;
-      const int ii = i0 + col, jj = j0 - 3 + row;
-      const bool ok = ii < NG + b.idim && jj < b.ny;
-      const unsigned e = (unsigned)ii + (unsigned)jj * nx + kplane;
+    for (int h = 0; h < 2; ++h) {
+      const int p = lane + 256 * h;
+      if (p < TJ * (TI + 5)) {
+        const int row = p / (TI + 5), col = p % (TI + 5);
+        const int jj = j0 + row, ii = i0 - 3 + col;
+        const bool ok = jj < NG + b.jdim && ii < b.nx;
+        const unsigned e = (unsigned)ii + (unsigned)jj * nx + kplane;
 #pragma unroll
-      for (int m = 0; m < NM; ++m) Y[m][row][col] = ok ? ldo(FX, boff((unsigned)(5 + M0 + m), P, e)) : 0.0;
-    }
-  }
-  __syncthreads();
-  if (!(i < NG + b.idim && j < NG + b.jdim)) return;
-  const unsigned id = (unsigned)i + (unsigned)j * nx + kplane;
-  double acc[5];
-#pragma unroll
-  for (int m = 0; m < 5; ++m) acc[m] = FIRST ? 0.0 : (m < M0 ? 0.0 : ldo(b.dq, boff((unsigned)m, P, id)));
-  const int pos[3] = {i, j, k};
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    const int n = dcount(b, d);
-    const unsigned s = d == 0 ? 1u : (d == 1 ? nx : nxy);
-    const int t = pos[d] - NG;
-    const unsigned lbase = id - (unsigned)pos[d] * s;
-    const double* cov = nullptr;
-    if (INV) {
-      const bool lflag = ldo(b.prim, lbase << 3) < 1.0e-20;
-      const bool rflag = ldo(b.prim, (lbase + (unsigned)(n + 5) * s) << 3) < 1.0e-20;
-      if (lflag && t <= 2) cov = b.fcov + b.fcov_off[d] + line_id(b, d, i, j, k) * 30 + t * 5;
-      else if (rflag && t >= n - 3) cov = b.fcov + b.fcov_off[d] + line_id(b, d, i, j, k) * 30 + (3 + t - (n - 3)) * 5;
-    }
-#pragma unroll
-    for (int m = M0; m < 5; ++m) {
-      double fc;
-      if (d == 0) {
-        fc = diff_node_f([&](int f) { return X[m - M0][lj][f + 2 - (i0 - 3)]; }, n, t);
-      } else if (d == 1) {
-        fc = diff_node_f([&](int f) { return Y[m - M0][f + 2 - (j0 - 3)][li]; }, n, t);
-      } else {
-        fc = diff_node_f([&](int f) { return ldo(FX, boff((unsigned)(10 + m), P, lbase + (unsigned)(f + 2) * s)); },
-                         n, t);
+        for (int m = 0; m < NM; ++m) X[m][row][col] = ok ? ldo(FX, boff((unsigned)(M0 + m), P, e)) : 0.0;
       }
-      if (INV) {
-        // the difference is formed for every node and replaced by the
-        // closure value where the line has one
-        if (cov) fc = cov[m];
-        acc[m] += fc;
-      } else {
-        acc[m] -= re_inv * fc;
+      if (p < (TJ + 5) * TI) {
+        const int row = p / TI, col = p % TI;
+        const int ii = i0 + col, jj = j0 - 3 + row;
+        const bool ok = ii < NG + b.idim && jj < b.ny;
+        const unsigned e = (unsigned)ii + (unsigned)jj * nx + kplane;
+#pragma unroll
+        for (int m = 0; m < NM; ++m) Y[m][row][col] = ok ? ldo(FX, boff((unsigned)(5 + M0 + m), P, e)) : 0.0;
       }
     }
-  }
-  // component 0 of the viscous difference: written (as +0) only by the
-  // step's first kernel, otherwise left as it is
+    __syncthreads();
+    if (in_ij) {
+      const unsigned id = lbk + kplane;
+      double acc[5];
 #pragma unroll
-  for (int m = 0; m < 5; ++m)
-    if (m >= M0 || FIRST) sto(b.dq, boff((unsigned)m, P, id), acc[m]);
+      for (int m = 0; m < 5; ++m) acc[m] = FIRST ? 0.0 : (m < M0 ? 0.0 : ldo(b.dq, boff((unsigned)m, P, id)));
+      const int pos[2] = {i, j};
+#pragma unroll
+      for (int d = 0; d < 2; ++d) {
+        const int n = dcount(b, d);
+        const unsigned s = d == 0 ? 1u : nx;
+        const int t = pos[d] - NG;
+        const unsigned lbase = id - (unsigned)pos[d] * s;
+        const double* cov = nullptr;
+        if (INV) {
+          const bool lflag = ldo(b.prim, lbase << 3) < 1.0e-20;
+          const bool rflag = ldo(b.prim, (lbase + (unsigned)(n + 5) * s) << 3) < 1.0e-20;
+          if (lflag && t <= 2) cov = b.fcov + b.fcov_off[d] + line_id(b, d, i, j, k) * 30 + t * 5;
+          else if (rflag && t >= n - 3) cov = b.fcov + b.fcov_off[d] + line_id(b, d, i, j, k) * 30 + (3 + t - (n - 3)) * 5;
+        }
+#pragma unroll
+        for (int m = M0; m < 5; ++m) {
+          double fc;
+          if (d == 0) fc = diff_node_f([&](int f) { return X[m - M0][lj][f + 2 - (i0 - 3)]; }, n, t);
+          else fc = diff_node_f([&](int f) { return Y[m - M0][f + 2 - (j0 - 3)][li]; }, n, t);
+          if (INV) {
+            // the difference is formed for every node and replaced by the
+            // closure value where the line has one
+            if (cov) fc = cov[m];
+            acc[m] += fc;
+          } else {
+            acc[m] -= re_inv * fc;
+          }
+        }
+      }
+#pragma unroll
+      for (int m = M0; m < 5; ++m) {
+        if (INV) acc[m] += fk[kk][m];
+        else acc[m] -= re_inv * fk[kk][m];
+      }
+      // component 0 of the viscous difference: written (as +0) only by the
+      // step's first kernel, otherwise left as it is
+#pragma unroll
+      for (int m = 0; m < 5; ++m)
+        if (m >= M0 || FIRST) sto(b.dq, boff((unsigned)m, P, id), acc[m]);
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------

@@ -166,10 +166,12 @@ __host__ __device__ inline unsigned long long halo_point(long long off, long lon
 // unrelated tiles in one line would make those accesses contend.
 constexpr int FLAG_STRIDE = 32;
 
-// The flux differences' tiles: DIFF_TI x DIFF_TJ (i, j) nodes of a k plane
-// per workgroup (256 nodes).
+// The flux differences' tiles: DIFF_TI x DIFF_TJ (i, j) nodes of DIFF_TK
+// consecutive k planes per workgroup (256 lanes, a column of DIFF_TK nodes
+// each).
 constexpr int DIFF_TI = 16;
 constexpr int DIFF_TJ = 16;
+constexpr int DIFF_TK = 2;
 
 // LU-SGS tiles: SWEEP_TW columns by SWEEP_TR rows of pencils (the block
 // layout pads columns to whole multiples of SWEEP_TW).  16 x 8 tiles were
