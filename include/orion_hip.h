@@ -183,8 +183,9 @@ int orion_hip_profile_run(orion_hip_ctx* ctx, int nsteps, double* phase_ms, long
 int orion_hip_set_schedule(orion_hip_ctx* ctx, int schedule);
 int orion_hip_get_schedule(const orion_hip_ctx* ctx);
 /* The autotuner of the sweep schedule: from the current state, times
- * `steps` pseudo-steps (after one untimed step) under each schedule,
- * restoring the state after each trial, and keeps the fastest.  chosen
+ * `steps` pseudo-steps (after one untimed step; the faster of two runs from
+ * the same state) under each schedule, restoring the state after each
+ * trial, and keeps the fastest.  chosen
  * (may be NULL) gets its index, ms_per_step (ORION_SCHEDULES entries, may
  * be NULL) each schedule's time.  The solution state afterwards is the one
  * before the call, bit for bit (prim alone fixes the next step).
@@ -211,7 +212,8 @@ int orion_hip_get_visc_tiles(const orion_hip_ctx* ctx, int* segment, int* grid);
 int orion_hip_set_visc_form(orion_hip_ctx* ctx, int fused);
 int orion_hip_get_visc_form(const orion_hip_ctx* ctx);
 /* The autotuner of the step's launch shapes: from the current state, times
- * `steps` pseudo-steps (after one untimed step) per candidate, restoring the
+ * `steps` pseudo-steps (after one untimed step; the faster of two runs from
+ * the same state) per candidate, restoring the
  * state after each, and keeps the fastest by coordinate search -- the LU-SGS
  * tile schedule (ORION_SCHEDULES), then the viscous form (fused or face
  * queue, where both apply), then the viscous segment (8, 16, 32, 64),
