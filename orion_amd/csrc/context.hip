@@ -386,7 +386,7 @@ static long long arena_fields(const BlockDev& b, int keep_q, int viscous, long l
 
 // Runs fn(b, scratch) for b = 0 .. n-1 on up to 16 host threads (blocks
 // taken in turn from a shared counter); scratch is a per-thread vector that
-// fn may size and reuse.  Returns the error message fn gave for the lowest
+// fn may size and reuse (32-bit: it indexes at most 5 P < 2^28 values).  Returns the error message fn gave for the lowest
 // block that gave one, or an empty string.
 template <class F>
 static std::string parallel_blocks(int n, const F& fn) {
@@ -395,7 +395,7 @@ static std::string parallel_blocks(int n, const F& fn) {
   std::atomic<int> next(0);
   std::vector<std::string> err((size_t)std::max(n, 0));
   auto work = [&]() {
-    std::vector<long long> scratch;
+    std::vector<int> scratch;
     for (;;) {
       const int b = next.fetch_add(1);
       if (b >= n) break;
@@ -465,9 +465,8 @@ static int build_topology(orion_hip_ctx* c, const std::vector<std::vector<Reg>>&
   // across points; its block keeps one pair per field value.
   std::vector<std::vector<long long>> bpairs(nb);
   std::vector<std::vector<unsigned long long>> bpts(nb);
-  parallel_blocks(nb, [&](int b, std::vector<long long>& where) {
+  parallel_blocks(nb, [&](int b, std::vector<int>& where) {
     const BlockDev& B = c->hb[b];
-    if ((long long)where.size() < 5 * B.P) where.assign((size_t)(5 * B.P), -1);
     std::vector<long long> touched;
     bool whole = B.P < (1ll << (64 - HALO_OFF_BITS));
     for (const Link& L : links[b]) {
@@ -482,6 +481,11 @@ static int build_topology(orion_hip_ctx* c, const std::vector<std::vector<Reg>>&
           arena_off(b, 4, B.P) > (long long)HALO_OFF_MASK)
         whole = false;
     }
+    // the index of each destination's entry (-1: none yet), over the block's
+    // points (whole points) or its five fields' values; every slot is -1
+    // again when the block is done
+    const long long nslot = whole ? B.P : 5 * B.P;
+    if ((long long)where.size() < nslot) where.assign((size_t)nslot, -1);
     if (whole) {
       std::vector<unsigned long long>& pts = bpts[b];
       for (const Link& L : links[b]) {
@@ -504,11 +508,11 @@ static int build_topology(orion_hip_ctx* c, const std::vector<std::vector<Reg>>&
                   ++sk;
                 }
               }
-              long long& w = where[(size_t)local];
+              int& w = where[(size_t)local];
               if (w >= 0) {
-                pts[2 * w + 1] = src;
+                pts[2 * (size_t)w + 1] = src;
               } else {
-                w = (long long)(pts.size() / 2);
+                w = (int)(pts.size() / 2);
                 pts.push_back(halo_point(arena_off(b, 0, local), B.P));
                 pts.push_back(src);
                 touched.push_back(local);
@@ -549,11 +553,11 @@ static int build_topology(orion_hip_ctx* c, const std::vector<std::vector<Reg>>&
                   }
                 }
               }
-              long long& w = where[(size_t)local];
+              int& w = where[(size_t)local];
               if (w >= 0) {
-                pairs[2 * w + 1] = src;
+                pairs[2 * (size_t)w + 1] = src;
               } else {
-                w = (long long)(pairs.size() / 2);
+                w = (int)(pairs.size() / 2);
                 pairs.push_back(arena_off(b, m, pidx(B, i, j, k)));
                 pairs.push_back(src);
                 touched.push_back(local);
@@ -651,7 +655,8 @@ static int build_topology(orion_hip_ctx* c, const std::vector<std::vector<Reg>>&
   // each block's entries (in its windows' order, so seq orders them as one
   // global count would), sorted by (node, region order) on host threads
   std::vector<std::vector<AE>> bae(nb);
-  const std::string aerr = parallel_blocks(nb, [&](int b, std::vector<long long>&) -> std::string {
+  std::vector<long long> bruns(nb, 0);  // each block's runs of equal nodes
+  const std::string aerr = parallel_blocks(nb, [&](int b, std::vector<int>&) -> std::string {
     std::vector<AE>& ae = bae[b];
     long long seq = 0;
     for (int w = 0; w < (int)R[b].size(); ++w) {
@@ -695,34 +700,38 @@ static int build_topology(orion_hip_ctx* c, const std::vector<std::vector<Reg>>&
     std::sort(ae.begin(), ae.end(), [](const AE& x, const AE& y) {
       return x.didx < y.didx || (x.didx == y.didx && x.seq < y.seq);
     });
+    for (size_t q = 0; q < ae.size(); ++q) bruns[b] += (q == 0 || ae[q].didx != ae[q - 1].didx) ? 1 : 0;
     return std::string();
   });
   if (!aerr.empty()) return fail("%s", aerr.c_str());
-  std::vector<AE> ae;
-  {
-    size_t tot = 0;
-    for (auto& v : bae) tot += v.size();
-    ae.reserve(tot);
-    for (auto& v : bae) {
-      ae.insert(ae.end(), v.begin(), v.end());
-      std::vector<AE>().swap(v);
-    }
+  // the staging entries (blocks in turn, each in its sorted order) and one
+  // destination per run of equal nodes, each block's written at its offsets
+  // on host threads
+  std::vector<long long> s_off(nb + 1, 0), d_off(nb + 1, 0);
+  for (int b = 0; b < nb; ++b) {
+    s_off[b + 1] = s_off[b] + (long long)bae[b].size();
+    d_off[b + 1] = d_off[b] + bruns[b];
   }
-  std::vector<AvgSrc> src;
-  std::vector<AvgDst> dst;
-  src.reserve(ae.size());
-  for (size_t q = 0; q < ae.size(); ++q) {
-    if (q == 0 || ae[q].b != ae[q - 1].b || ae[q].didx != ae[q - 1].didx) {
-      AvgDst D{};
-      D.block = ae[q].b;
-      D.idx = ae[q].didx;
-      D.start = (long long)src.size();
-      D.count = 0;
-      dst.push_back(D);
+  std::vector<AvgSrc> src((size_t)s_off[nb]);
+  std::vector<AvgDst> dst((size_t)d_off[nb]);
+  parallel_blocks(nb, [&](int b, std::vector<int>&) -> std::string {
+    const std::vector<AE>& v = bae[b];
+    long long dn = d_off[b];
+    for (size_t q = 0; q < v.size(); ++q) {
+      const long long sq = s_off[b] + (long long)q;
+      if (q == 0 || v[q].didx != v[q - 1].didx) {
+        AvgDst& D = dst[(size_t)dn++];
+        D.block = b;
+        D.idx = v[q].didx;
+        D.start = sq;
+        D.count = 0;
+      }
+      dst[(size_t)(dn - 1)].count++;
+      src[(size_t)sq] = AvgSrc{v[q].a, v[q].sidx};
     }
-    dst.back().count++;
-    src.push_back(AvgSrc{ae[q].a, ae[q].sidx});
-  }
+    std::vector<AE>().swap(bae[b]);
+    return std::string();
+  });
   c->n_avg_src = (long long)src.size();
   c->n_avg_dst = (long long)dst.size();
   if (upload_vec(c, &c->d_avg_src, src)) return -1;
