@@ -216,7 +216,7 @@ int orion_hip_get_visc_form(const orion_hip_ctx* ctx);
  * the same state) per candidate, restoring the
  * state after each, and keeps the fastest by coordinate search -- the LU-SGS
  * tile schedule (ORION_SCHEDULES), then the viscous form (fused or face
- * queue, where both apply), then the viscous segment (8, 16, 32, 64),
+ * queue, where both apply), then the viscous segment (8, 16, 32, 64, 128),
  * then its grid (256 to 4096 workgroups).  Outputs (each may be NULL):
  * the chosen schedule, segment and grid, the best ms per step, the trials
  * run.  The solution state afterwards is the one before the call, bit for

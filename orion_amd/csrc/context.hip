@@ -2283,7 +2283,7 @@ int orion_hip_tune(orion_hip_ctx* c, int steps, int* schedule, int* segment, int
       else if (rc == 0) best = ms;
     }
     int bseg = c->visc_seg, bgrid = c->visc_grid;
-    static const int SEGS[] = {8, 16, 32, 64};
+    static const int SEGS[] = {8, 16, 32, 64, 128};
     static const int GRIDS[] = {256, 512, 1024, 2048, 4096};
     for (int seg : SEGS) {
       if (rc != 0 || seg == bseg) continue;

@@ -186,9 +186,9 @@ class FlowFieldSet:
 
     def set_visc_form(self, fused: bool) -> None:
         """The viscous right-hand side after the node derivatives: fused
-        (face fluxes and their difference in one pass, the default where
-        every block line has at least 4 nodes) or the face-flux queue and a
-        difference kernel; the results are the same bit for bit."""
+        (face fluxes and their difference in one pass; every block line
+        needs at least 4 nodes) or the face-flux queue and a difference
+        kernel (the default); the results are the same bit for bit."""
         check(_lib.load().orion_hip_set_visc_form(self.ctx, int(bool(fused))), "set_visc_form")
 
     def visc_form(self) -> bool:
