@@ -463,12 +463,16 @@ static int build_topology(orion_hip_ctx* c, const std::vector<std::vector<Reg>>&
   // source point, so its writes reduce per point to one entry of five
   // fields (halo_point).  A window of another length shifts the fields
   // across points; its block keeps one pair per field value.
+  // ORION_HIP_HALO_FIELDS=1 keeps every block on the per-field pairs (the
+  // two forms' results are compared in the tests)
+  const char* hf = std::getenv("ORION_HIP_HALO_FIELDS");
+  const bool fields_only = hf && std::atoi(hf) != 0;
   std::vector<std::vector<long long>> bpairs(nb);
   std::vector<std::vector<unsigned long long>> bpts(nb);
   parallel_blocks(nb, [&](int b, std::vector<int>& where) {
     const BlockDev& B = c->hb[b];
     std::vector<long long> touched;
-    bool whole = B.P < (1ll << (64 - HALO_OFF_BITS));
+    bool whole = !fields_only && B.P < (1ll << (64 - HALO_OFF_BITS));
     for (const Link& L : links[b]) {
       const BlockDev& A = c->hb[L.sblock];
       int sb[3], se[3], rb[3], re[3];

@@ -170,3 +170,34 @@ def test_fused_viscous_form_equals_face_queue(name):
         assert np.array_equal(a.fs.field(blk, "dq"), b.fs.field(blk, "dq")), blk
     a.fs.close()
     b.fs.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["rotated_pair", "columns_small", "bc_variety_visc"])
+def test_halo_field_pairs_equal_whole_points(name, monkeypatch):
+    """The halo moves whole points (five fields per entry) for blocks whose
+    interface windows pair equal point counts, and one field value per
+    entry otherwise; a context built with ORION_HIP_HALO_FIELDS=1 keeps
+    every block on the per-field pairs.  Both give the same residual
+    histories and fields bit for bit."""
+    from orion_amd.solver import OrionSolver
+    case = _cases.ALL[name](nsteps=4)
+    print("\n[gpu] halo per-field pairs vs whole points:", name, flush=True)
+    a = OrionSolver(case)
+    monkeypatch.setenv("ORION_HIP_HALO_FIELDS", "1")
+    b = OrionSolver(case)
+    monkeypatch.delenv("ORION_HIP_HALO_FIELDS")
+    # the per-field list holds five entries per ghost point: on the 12-block
+    # mesh both lists exceed the small-allocation pool, so the context that
+    # keeps them is larger (the switch took effect)
+    assert b.fs.device_bytes() >= a.fs.device_bytes()
+    if name == "columns_small":
+        assert b.fs.device_bytes() > a.fs.device_bytes()
+    ha, hb = a.solve(4), b.solve(4)
+    assert [h.rms_residual for h in ha] == [h.rms_residual for h in hb]
+    assert [h.max_loc for h in ha] == [h.max_loc for h in hb]
+    for blk in range(case.nblocks):
+        for f in ("prim", "dq"):
+            assert np.array_equal(a.fs.field(blk, f), b.fs.field(blk, f)), (blk, f)
+    a.fs.close()
+    b.fs.close()
